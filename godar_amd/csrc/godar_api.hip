@@ -1,0 +1,812 @@
+// Host side of libgodar_amd.so: context, member management, the step
+// launcher and the extern "C" entry points declared in include/godar_gpu.h.
+#include "godar_gpu.h"
+#include "godar_engine.h"
+#include "godar_physics.h"
+#include "radix_sort.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+namespace godar {
+__global__ void k_bin(DS);
+__global__ void k_clear_cells(DS);
+__global__ void k_ranges(DS);
+__global__ void k_cand(DS);
+template <bool M> __global__ void k_spec(DS);
+template <bool M> __global__ void k_repair(DS, int);
+template <bool M> __global__ void k_fixup(DS);
+template <bool M> __global__ void k_final(DS);
+
+__global__ void k_fill_u8(uint8_t* p, int64_t n, uint8_t v) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+__global__ void k_fill_i32(int32_t* p, int64_t n, int32_t v) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+}  // namespace godar
+
+using namespace godar;
+
+struct Range { int64_t first, n; };
+
+struct godar_ctx {
+    hipStream_t stream = nullptr;
+    int precision = GODAR_FP64;
+    int64_t S = 0;
+    int Mcap = 0;
+    int CO = 32, CI = 32, HC = 16, BC = 16;
+    int SEEDK = 2, HEAPK = 16, TABK = 4;
+    int block = 256;
+    int record = 0;
+    int parity = 0;
+    int32_t N_used = 0;
+    int32_t ncells_cap = 0;
+    std::vector<Member> members;
+    std::vector<DevParams> dparams;
+    std::vector<int32_t> cell_n;        // cells owned by each member
+    std::vector<WindGridDesc> wgrids;
+    std::vector<double> wpool;
+    int64_t wpool_dev_cap = 0;
+    int wgrids_dev_cap = 0;
+    DS d{};
+    // double-buffered pair state
+    HistRec* hist[2] = {nullptr, nullptr};
+    int32_t* hcnt[2] = {nullptr, nullptr};
+    BondRec* bond[2] = {nullptr, nullptr};
+    int32_t* bcnt[2] = {nullptr, nullptr};
+    RadixScratch rs;
+    std::string err;
+    // state of the last complete step, for rolling back an aborted step
+    int pend_parity = 0;
+    std::vector<int32_t> pend_t0;
+    bool pending = false;
+};
+
+namespace {
+
+int fail(godar_ctx* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+#define HIPCHK(c, expr)                                                                    \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return fail((c), GODAR_ERR_HIP, std::string(#expr ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+template <typename T>
+hipError_t dalloc(T** p, int64_t count) {
+    *p = nullptr;
+    if (count <= 0) count = 1;
+    return hipMalloc((void**)p, sizeof(T) * count);
+}
+
+template <typename T>
+void dfree(T*& p) {
+    if (p) (void)hipFree((void*)p);
+    p = nullptr;
+}
+
+void derive_params(const godar_params& p, DevParams& d) {
+    std::memset(&d, 0, sizeof(d));
+    d.dt = p.dt; d.rtree = p.rtree; d.nx = p.nx; d.ny = p.ny;
+    d.rhoair = p.rhoair; d.rhoice = p.rhoice; d.rhowater = p.rhowater;
+    d.Cdair = p.Cdair; d.Cdwater = p.Cdwater; d.Csair = p.Csair; d.Cswater = p.Cswater;
+    d.z0w = p.z0w;
+    // par_get.f90:35,64-68,263-266
+    d.pi = 4.0 * std::atan(1.0);
+    d.ec = p.e_modul / (2.0 * (1.0 - p.poiss_ratio * p.poiss_ratio));
+    d.gc = p.e_modul / (4.0 * (1.0 - p.poiss_ratio) * (2.0 + p.poiss_ratio));
+    double lr = std::log(p.rest_coeff);
+    d.beta = 10.0 * lr / std::sqrt(lr * lr + d.pi * d.pi);
+    d.pi_ec = d.pi * d.ec;
+    d.k6 = 6.0 * d.gc / d.ec;
+    d.two_beta = 2.0 * d.beta;
+    d.sinlat = std::sin(p.lat);
+    d.log10z0 = std::log(10.0 / p.z0w);
+    d.friction_coeff = p.friction_coeff; d.sigmanc_crit = p.sigmanc_crit;
+    d.eb = p.eb; d.lambda_rb = p.lambda_rb; d.lambda_lb = p.lambda_lb;
+    d.sigmatb_crit = p.sigmatb_crit; d.sigmacb_crit = p.sigmacb_crit; d.tau_crit = p.tau_crit;
+    d.bond_lim = p.bond_lim; d.dmax = p.dmax; d.dtd = p.dtd; d.dth = p.dth;
+    d.ua = p.ua; d.va = p.va; d.uw = p.uw; d.vw = p.vw;
+    d.slipping = p.slipping; d.cohesion = p.cohesion; d.ridging = p.ridging;
+    d.shelter = p.shelter; d.restart = p.restart;
+    double fr = p.bond_frac;
+    d.bond_thr24 = fr >= 1.0 ? (1u << 24) : (fr <= 0.0 ? 0u : (uint32_t)std::floor(fr * 16777216.0));
+    d.bond_seed = p.bond_seed;
+    d.wind_grid = -1;
+}
+
+void free_lists(godar_ctx* c) {
+    DS& d = c->d;
+    dfree(d.oj); dfree(d.od); dfree(d.ik); dfree(d.evf); dfree(d.ov);
+    for (int k = 0; k < 2; ++k) { dfree(c->hist[k]); dfree(c->bond[k]); }
+}
+
+hipError_t alloc_lists(godar_ctx* c) {
+    DS& d = c->d;
+    int64_t S = c->S;
+    hipError_t e;
+    if ((e = dalloc(&d.oj, S * c->CO)) != hipSuccess) return e;
+    if ((e = dalloc(&d.od, S * c->CO)) != hipSuccess) return e;
+    if ((e = dalloc(&d.ik, S * c->CI)) != hipSuccess) return e;
+    if ((e = dalloc(&d.evf, S * c->CO)) != hipSuccess) return e;
+    if ((e = dalloc(&d.ov, S * c->CO)) != hipSuccess) return e;
+    for (int k = 0; k < 2; ++k) {
+        if ((e = dalloc(&c->hist[k], S * c->HC)) != hipSuccess) return e;
+        if ((e = dalloc(&c->bond[k], S * c->BC)) != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+void bind_parity(godar_ctx* c) {
+    int o = c->parity, n = 1 - c->parity;
+    c->d.hist_old = c->hist[o]; c->d.hist_new = c->hist[n];
+    c->d.hcnt_old = c->hcnt[o]; c->d.hcnt_new = c->hcnt[n];
+    c->d.bond_old = c->bond[o]; c->d.bond_new = c->bond[n];
+    c->d.bcnt_old = c->bcnt[o]; c->d.bcnt_new = c->bcnt[n];
+}
+
+void sync_ds(godar_ctx* c) {
+    DS& d = c->d;
+    d.S = c->S; d.N = c->N_used;
+    d.CO = c->CO; d.CI = c->CI; d.HC = c->HC; d.BC = c->BC;
+    d.SEEDK = c->SEEDK; d.HEAPK = c->HEAPK; d.TABK = c->TABK;
+    d.record_forces = c->record;
+    d.ncells = c->ncells_cap;
+    bind_parity(c);
+}
+
+int ensure_cells(godar_ctx* c, int32_t need) {
+    if (need <= c->ncells_cap) return GODAR_OK;
+    int32_t cap = std::max(need, c->ncells_cap * 2);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    dfree(c->d.cstart); dfree(c->d.cend);
+    HIPCHK(c, dalloc(&c->d.cstart, (int64_t)cap + 1));
+    HIPCHK(c, dalloc(&c->d.cend, (int64_t)cap + 1));
+    c->ncells_cap = cap;
+    return GODAR_OK;
+}
+
+int upload_tables(godar_ctx* c) {
+    HIPCHK(c, hipMemcpyAsync(c->d.members, c->members.data(), sizeof(Member) * c->Mcap,
+                             hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync((void*)c->d.params, c->dparams.data(), sizeof(DevParams) * c->Mcap,
+                             hipMemcpyHostToDevice, c->stream));
+    return GODAR_OK;
+}
+
+int upload_members_only(godar_ctx* c) {
+    HIPCHK(c, hipMemcpyAsync(c->d.members, c->members.data(), sizeof(Member) * c->Mcap,
+                             hipMemcpyHostToDevice, c->stream));
+    return GODAR_OK;
+}
+
+int download_members(godar_ctx* c) {
+    HIPCHK(c, hipMemcpyAsync(c->members.data(), c->d.members, sizeof(Member) * c->Mcap,
+                             hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return GODAR_OK;
+}
+
+// smallest free range of length n (first fit over the gaps between members)
+bool find_range(const godar_ctx* c, int64_t n, int64_t cap, const std::vector<Range>& used,
+                int64_t& first) {
+    std::vector<Range> u = used;
+    std::sort(u.begin(), u.end(), [](const Range& a, const Range& b) { return a.first < b.first; });
+    int64_t pos = 0;
+    for (const Range& r : u) {
+        if (r.first - pos >= n) { first = pos; return true; }
+        pos = std::max(pos, r.first + r.n);
+    }
+    if (cap - pos >= n) { first = pos; return true; }
+    (void)c;
+    return false;
+}
+
+bool valid_member(const godar_ctx* c, int32_t m) {
+    return c && m >= 0 && m < c->Mcap && c->members[m].active;
+}
+
+int launch_step(godar_ctx* c) {
+    DS& d = c->d;
+    sync_ds(c);
+    const int B = c->block;
+    int N = c->N_used;
+    if (N == 0) return GODAR_OK;
+    dim3 g((N + B - 1) / B), b(B);
+    hipStream_t s = c->stream;
+    hipLaunchKernelGGL(k_bin, g, b, 0, s, d);
+    int bits = 1;
+    while ((1u << bits) <= (uint32_t)c->ncells_cap && bits < 32) ++bits;
+    HIPCHK(c, radix_sort_pairs(c->rs, d.ckey, d.cval, N, bits, d.err, s));
+    hipLaunchKernelGGL(k_clear_cells, dim3((c->ncells_cap + 1 + 255) / 256), dim3(256), 0, s, d);
+    hipLaunchKernelGGL(k_ranges, g, b, 0, s, d);
+    hipLaunchKernelGGL(k_cand, g, b, 0, s, d);
+    int nm = c->Mcap;
+    if (c->precision == GODAR_MIXED) {
+        hipLaunchKernelGGL(k_spec<true>, g, b, 0, s, d);
+        hipLaunchKernelGGL(k_repair<true>, dim3(nm), dim3(64), 0, s, d, nm);
+        hipLaunchKernelGGL(k_fixup<true>, g, b, 0, s, d);
+        hipLaunchKernelGGL(k_final<true>, g, b, 0, s, d);
+    } else {
+        hipLaunchKernelGGL(k_spec<false>, g, b, 0, s, d);
+        hipLaunchKernelGGL(k_repair<false>, dim3(nm), dim3(64), 0, s, d, nm);
+        hipLaunchKernelGGL(k_fixup<false>, g, b, 0, s, d);
+        hipLaunchKernelGGL(k_final<false>, g, b, 0, s, d);
+    }
+    HIPCHK(c, hipGetLastError());
+    c->parity ^= 1;
+    return GODAR_OK;
+}
+
+std::string err_text(uint32_t e) {
+    std::string s;
+    if (e & ERR_OUTER_CAP) s += " outer-candidate list full;";
+    if (e & ERR_INNER_CAP) s += " inner-candidate list full;";
+    if (e & ERR_HIST_CAP) s += " contact-history list full;";
+    if (e & ERR_BOND_CAP) s += " bond list full;";
+    if (e & ERR_REPAIR_CAP) s += " ridging repair table full;";
+    if (e & ERR_SEED_CAP) s += " ridging seed list full;";
+    return s;
+}
+
+}  // namespace
+
+// ===================================================================== ABI
+extern "C" {
+
+int godar_params_default(godar_params* p) {
+    if (!p) return GODAR_ERR_ARG;
+    std::memset(p, 0, sizeof(*p));
+    // par_get.f90:1-111
+    p->dt = 1e-3; p->rtree = 1e3; p->nx = 5e3; p->ny = 5e3;
+    p->rhoair = 1.3; p->rhoice = 9e2; p->rhowater = 1026.0;
+    p->Cdair = 3e-1; p->Cdwater = 3e-1; p->Csair = 5e-4; p->Cswater = 2e-3;
+    p->z0w = 3.0e-4;
+    p->lat = (double)1.4f;   // default-real literal assigned to a double (par_get.f90:47)
+    p->e_modul = 6e9; p->poiss_ratio = 33e-2; p->friction_coeff = 7e-1;
+    p->rest_coeff = 88e-2; p->sigmanc_crit = 1e6;
+    p->eb = 6e9; p->lambda_rb = 8e-1; p->lambda_lb = 1.0;
+    p->sigmatb_crit = 1e5; p->sigmacb_crit = 1e6; p->tau_crit = 1e6;
+    p->bond_lim = 1e-2; p->dmax = 9e-1; p->dtd = 1.0; p->dth = 1e5;
+    p->bond_frac = 1.0;
+    p->ntree = 1; p->slipping = 1; p->cohesion = 1; p->ridging = 1; p->shelter = 1;
+    p->restart = 0; p->bond_seed = 0;
+    return GODAR_OK;
+}
+
+int godar_create(int64_t max_floes, int32_t max_members, int32_t precision, godar_ctx** out) {
+    if (!out || max_floes <= 0 || max_floes > (int64_t)1 << 30 || max_members <= 0 ||
+        (precision != GODAR_FP64 && precision != GODAR_MIXED))
+        return GODAR_ERR_ARG;
+    *out = nullptr;
+    godar_ctx* c = new godar_ctx();
+    c->precision = precision;
+    c->S = max_floes;
+    c->Mcap = max_members;
+    c->members.assign(max_members, Member{0, 0, 1, 0});
+    c->dparams.assign(max_members, DevParams{});
+    c->cell_n.assign(max_members, 0);
+    for (auto& m : c->members) m.active = 0;
+    DS& d = c->d;
+    int64_t S = c->S;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+#define ACHK(x)                                        \
+    do {                                               \
+        e = (x);                                       \
+        if (e != hipSuccess) {                         \
+            godar_destroy(c);                          \
+            return GODAR_ERR_NOMEM;                    \
+        }                                              \
+    } while (0)
+    if (e != hipSuccess) { delete c; return GODAR_ERR_HIP; }
+    double** dbl[] = {&d.x, &d.y, &d.u, &d.v, &d.theta, &d.omega, &d.r, &d.h, &d.mass, &d.inertia,
+                      &d.hfa, &d.hfw, &d.uai, &d.vai, &d.sx, &d.sy, &d.sr};
+    for (double** p : dbl) ACHK(dalloc(p, S));
+    ACHK(dalloc(&d.wbc, 4 * S));
+    ACHK(dalloc(&d.member_of, S));
+    ACHK(dalloc(&d.members, max_members));
+    DevParams* pp;
+    ACHK(dalloc(&pp, max_members));
+    d.params = pp;
+    ACHK(dalloc(&d.ckey, S));
+    ACHK(dalloc(&d.cval, S));
+    ACHK(dalloc(&d.sidx, S));
+    ACHK(dalloc(&d.ocnt, S));
+    ACHK(dalloc(&d.icnt, S));
+    for (int k = 0; k < 2; ++k) {
+        ACHK(dalloc(&c->hcnt[k], S));
+        ACHK(dalloc(&c->bcnt[k], S));
+    }
+    ACHK(alloc_lists(c));
+    ACHK(dalloc(&d.acc, (int64_t)NACC * S));
+    ACHK(dalloc(&d.wout, S));
+    ACHK(dalloc(&d.seeds, (int64_t)c->SEEDK * S));
+    ACHK(dalloc(&d.nseeds, max_members));
+    ACHK(dalloc(&d.heap, (int64_t)c->HEAPK * S));
+    ACHK(dalloc(&d.etab, (int64_t)c->TABK * S));
+    ACHK(dalloc(&d.netab, max_members));
+    ACHK(dalloc(&d.wtab, S));
+    ACHK(dalloc(&d.nwtab, max_members));
+    ACHK(dalloc(&d.ovw, S));
+    ACHK(dalloc(&d.cur, 6 * S));
+    ACHK(dalloc(&d.dirty, S));
+    ACHK(dalloc(&d.affected, S));
+    ACHK(dalloc(&d.stats, 4 * (int64_t)max_members));
+    ACHK(dalloc(&d.fout, 33 * S));
+    ACHK(dalloc(&d.err, 1));
+    ACHK(radix_alloc(c->rs, S));
+#undef ACHK
+    (void)hipMemsetAsync(d.err, 0, sizeof(uint32_t), c->stream);
+    (void)hipMemsetAsync(d.dirty, 0, S, c->stream);
+    (void)hipMemsetAsync(d.affected, 0, S, c->stream);
+    (void)hipMemsetAsync(d.stats, 0, sizeof(int64_t) * 4 * max_members, c->stream);
+    hipLaunchKernelGGL(k_fill_i32, dim3((S + 255) / 256), dim3(256), 0, c->stream, d.member_of, S, -1);
+    for (int k = 0; k < 2; ++k) {
+        (void)hipMemsetAsync(c->hcnt[k], 0, sizeof(int32_t) * S, c->stream);
+        (void)hipMemsetAsync(c->bcnt[k], 0, sizeof(int32_t) * S, c->stream);
+    }
+    if (ensure_cells(c, 1024) != GODAR_OK) { godar_destroy(c); return GODAR_ERR_NOMEM; }
+    if (upload_tables(c) != GODAR_OK || hipStreamSynchronize(c->stream) != hipSuccess) {
+        godar_destroy(c);
+        return GODAR_ERR_HIP;
+    }
+    *out = c;
+    return GODAR_OK;
+}
+
+int godar_destroy(godar_ctx* c) {
+    if (!c) return GODAR_ERR_ARG;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    DS& d = c->d;
+    double* dbl[] = {d.x, d.y, d.u, d.v, d.theta, d.omega, d.r, d.h, d.mass, d.inertia,
+                     d.hfa, d.hfw, d.uai, d.vai, d.sx, d.sy, d.sr, d.wbc, d.acc, d.cur, d.fout};
+    for (double* p : dbl) dfree(p);
+    dfree(d.member_of); dfree(d.members);
+    DevParams* pp = const_cast<DevParams*>(d.params); dfree(pp);
+    WindGridDesc* wg = const_cast<WindGridDesc*>(d.wgrids); dfree(wg);
+    double* wp = const_cast<double*>(d.wpool); dfree(wp);
+    dfree(d.ckey); dfree(d.cval); dfree(d.sidx); dfree(d.ocnt); dfree(d.icnt);
+    dfree(d.cstart); dfree(d.cend);
+    for (int k = 0; k < 2; ++k) { dfree(c->hcnt[k]); dfree(c->bcnt[k]); }
+    free_lists(c);
+    dfree(d.wout); dfree(d.seeds); dfree(d.nseeds); dfree(d.heap); dfree(d.etab);
+    dfree(d.netab); dfree(d.wtab); dfree(d.nwtab); dfree(d.ovw); dfree(d.dirty);
+    dfree(d.affected); dfree(d.stats); dfree(d.err);
+    radix_free(c->rs);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return GODAR_OK;
+}
+
+const char* godar_last_error(const godar_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int godar_member_footprint(const godar_ctx* c, int32_t n, int64_t* bytes) {
+    if (!c || !bytes || n < 0) return GODAR_ERR_ARG;
+    int64_t per = 0;
+    per += 17 * 8 + 4 * 8 + 4 * 7;                                   // floe arrays, walls, ints
+    per += (int64_t)c->CO * (4 + 8 + 4 + 4) + (int64_t)c->CI * 4;    // lists
+    per += 2 * ((int64_t)c->HC * sizeof(HistRec) + (int64_t)c->BC * sizeof(BondRec) + 8);
+    per += NACC * 8 + sizeof(WallOut) * 2 + 6 * 8 + 2 + 33 * 8;
+    per += (int64_t)c->SEEDK * 8 + (int64_t)c->HEAPK * 8 + (int64_t)c->TABK * sizeof(EventOut);
+    *bytes = per * n;
+    return GODAR_OK;
+}
+
+int godar_capacity_left(const godar_ctx* c, int64_t* floes, int32_t* members) {
+    if (!c) return GODAR_ERR_ARG;
+    int64_t used = 0;
+    int32_t nm = 0;
+    for (const Member& m : c->members)
+        if (m.active) { used += m.n; ++nm; }
+    if (floes) *floes = c->S - used;
+    if (members) *members = c->Mcap - nm;
+    return GODAR_OK;
+}
+
+int godar_member_add(godar_ctx* c, const godar_params* p, int32_t n, const double* x,
+                     const double* y, const double* u, const double* v, const double* theta,
+                     const double* omega, const double* r, const double* h, int32_t tstep0,
+                     int32_t* member) {
+    if (!c || !p || n <= 0 || !x || !y || !u || !v || !theta || !omega || !r || !h || !member)
+        return fail(c, GODAR_ERR_ARG, "member_add: bad argument");
+    if (n > 0xFFFF * 64) return fail(c, GODAR_ERR_ARG, "member_add: member too large");
+    if (!(p->dt > 0) || !(p->rtree >= 0) || !(p->nx > 0) || !(p->ny > 0))
+        return fail(c, GODAR_ERR_ARG, "member_add: dt, nx, ny must be positive, rtree >= 0");
+    int m = -1;
+    for (int k = 0; k < c->Mcap; ++k)
+        if (!c->members[k].active) { m = k; break; }
+    if (m < 0) return fail(c, GODAR_ERR_NOMEM, "member_add: no free member slot");
+    std::vector<Range> used;
+    for (const Member& mm : c->members)
+        if (mm.active) used.push_back({mm.first, mm.n});
+    int64_t first;
+    if (!find_range(c, n, c->S, used, first))
+        return fail(c, GODAR_ERR_NOMEM, "member_add: no room for the floes");
+    // cell grid: cell >= max query radius (r only shrinks, by ridging)
+    double rmax = 0;
+    for (int i = 0; i < n; ++i) {
+        if (!(r[i] > 0) || !(h[i] > 0) || !std::isfinite(x[i]) || !std::isfinite(y[i]))
+            return fail(c, GODAR_ERR_ARG, "member_add: radii/thicknesses must be positive and positions finite");
+        rmax = std::max(rmax, r[i]);
+    }
+    DevParams dp;
+    derive_params(*p, dp);
+    dp.cell = (rmax + p->rtree) * 1.0000001 + 1e-9;
+    int64_t gx = std::max<int64_t>(1, (int64_t)std::floor(p->nx / dp.cell));
+    int64_t gy = std::max<int64_t>(1, (int64_t)std::floor(p->ny / dp.cell));
+    while (gx * gy > 4 * (int64_t)n + 64) { gx = (gx + 1) / 2; gy = (gy + 1) / 2; dp.cell *= 2; }
+    dp.gx = (int32_t)gx;
+    dp.gy = (int32_t)gy;
+    int32_t ncell = (int32_t)(gx * gy);
+    std::vector<Range> cused;
+    for (int k = 0; k < c->Mcap; ++k)
+        if (c->members[k].active) cused.push_back({c->dparams[k].cell_base, c->cell_n[k]});
+    int64_t cbase;
+    find_range(c, ncell, (int64_t)1 << 30, cused, cbase);
+    int rc = ensure_cells(c, (int32_t)(cbase + ncell));
+    if (rc != GODAR_OK) return rc;
+    dp.cell_base = (int32_t)cbase;
+    c->cell_n[m] = ncell;
+    // floe state and floe_properties (physics.f90:179-197), as ini_get does
+    std::vector<double> mass(n), inertia(n), hfa(n), hfw(n);
+    for (int i = 0; i < n; ++i) {
+        mass[i] = dp.rhoice * dp.pi * h[i] * (r[i] * r[i]);
+        inertia[i] = 0.5 * mass[i] * (r[i] * r[i]);
+        hfa[i] = h[i] * (dp.rhowater - dp.rhoice) / dp.rhowater;
+        hfw[i] = h[i] * dp.rhoice / dp.rhowater;
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    DS& d = c->d;
+    const double* src[] = {x, y, u, v, theta, omega, r, h, mass.data(), inertia.data(), hfa.data(), hfw.data()};
+    double* dst[] = {d.x, d.y, d.u, d.v, d.theta, d.omega, d.r, d.h, d.mass, d.inertia, d.hfa, d.hfw};
+    for (int k = 0; k < 12; ++k)
+        HIPCHK(c, hipMemcpy(dst[k] + first, src[k], sizeof(double) * n, hipMemcpyHostToDevice));
+    for (int k = 0; k < 4; ++k) HIPCHK(c, hipMemset(d.wbc + k * c->S + first, 0, sizeof(double) * n));
+    for (int k = 0; k < 2; ++k) {
+        HIPCHK(c, hipMemset(c->hcnt[k] + first, 0, sizeof(int32_t) * n));
+        HIPCHK(c, hipMemset(c->bcnt[k] + first, 0, sizeof(int32_t) * n));
+    }
+    HIPCHK(c, hipMemset(d.dirty + first, 0, n));
+    HIPCHK(c, hipMemset(d.affected + first, 0, n));
+    std::vector<int32_t> mo(n, m);
+    HIPCHK(c, hipMemcpy(d.member_of + first, mo.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
+    c->dparams[m] = dp;
+    c->members[m].first = (int32_t)first;
+    c->members[m].n = n;
+    c->members[m].tstep = tstep0;
+    c->members[m].active = 1;
+    c->N_used = std::max<int32_t>(c->N_used, (int32_t)(first + n));
+    rc = upload_tables(c);
+    if (rc != GODAR_OK) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    *member = m;
+    return GODAR_OK;
+}
+
+int godar_member_remove(godar_ctx* c, int32_t m) {
+    if (!valid_member(c, m)) return fail(c, GODAR_ERR_ARG, "member_remove: bad member");
+    int rc = download_members(c);   // keep device-side step counters
+    if (rc != GODAR_OK) return rc;
+    Member& mm = c->members[m];
+    hipLaunchKernelGGL(k_fill_i32, dim3((mm.n + 255) / 256), dim3(256), 0, c->stream,
+                       c->d.member_of + mm.first, (int64_t)mm.n, -1);
+    mm.active = 0;
+    c->cell_n[m] = 0;
+    int32_t hw = 0;
+    for (const Member& q : c->members)
+        if (q.active) hw = std::max(hw, q.first + q.n);
+    c->N_used = hw;
+    rc = upload_members_only(c);
+    if (rc != GODAR_OK) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return GODAR_OK;
+}
+
+int godar_set_forcing(godar_ctx* c, int32_t m, double ua, double va, double uw, double vw) {
+    if (!valid_member(c, m)) return fail(c, GODAR_ERR_ARG, "set_forcing: bad member");
+    DevParams& d = c->dparams[m];
+    d.ua = ua; d.va = va; d.uw = uw; d.vw = vw;
+    HIPCHK(c, hipMemcpyAsync((void*)(c->d.params + m), &d, sizeof(DevParams), hipMemcpyHostToDevice,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return GODAR_OK;
+}
+
+int godar_set_wind_grid(godar_ctx* c, int32_t m, int32_t nxn, int32_t nyn, double x0, double y0,
+                        double dx, double dy, const double* ua, const double* va) {
+    if (!valid_member(c, m) || nxn < 2 || nyn < 2 || !(dx > 0) || !(dy > 0) || !ua || !va)
+        return fail(c, GODAR_ERR_ARG, "set_wind_grid: bad argument");
+    WindGridDesc g;
+    g.nxn = nxn; g.nyn = nyn; g.x0 = x0; g.y0 = y0; g.dx = dx; g.dy = dy;
+    g.offset = (int64_t)c->wpool.size();
+    int64_t cnt = (int64_t)nxn * nyn;
+    c->wpool.insert(c->wpool.end(), ua, ua + cnt);
+    c->wpool.insert(c->wpool.end(), va, va + cnt);
+    c->wgrids.push_back(g);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if ((int64_t)c->wpool.size() > c->wpool_dev_cap) {
+        double* wp = const_cast<double*>(c->d.wpool);
+        dfree(wp);
+        c->wpool_dev_cap = std::max<int64_t>((int64_t)c->wpool.size(), 2 * c->wpool_dev_cap);
+        HIPCHK(c, dalloc(&wp, c->wpool_dev_cap));
+        c->d.wpool = wp;
+    }
+    if ((int)c->wgrids.size() > c->wgrids_dev_cap) {
+        WindGridDesc* wg = const_cast<WindGridDesc*>(c->d.wgrids);
+        dfree(wg);
+        c->wgrids_dev_cap = std::max<int>((int)c->wgrids.size(), 2 * c->wgrids_dev_cap);
+        HIPCHK(c, dalloc(&wg, c->wgrids_dev_cap));
+        c->d.wgrids = wg;
+    }
+    HIPCHK(c, hipMemcpy((void*)c->d.wpool, c->wpool.data(), sizeof(double) * c->wpool.size(),
+                        hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy((void*)c->d.wgrids, c->wgrids.data(), sizeof(WindGridDesc) * c->wgrids.size(),
+                        hipMemcpyHostToDevice));
+    c->dparams[m].wind_grid = (int32_t)c->wgrids.size() - 1;
+    HIPCHK(c, hipMemcpy((void*)(c->d.params + m), &c->dparams[m], sizeof(DevParams), hipMemcpyHostToDevice));
+    return GODAR_OK;
+}
+
+int godar_set_bonds(godar_ctx* c, int32_t m, int32_t nb, const int32_t* bi, const int32_t* bj,
+                    const double* st) {
+    if (!valid_member(c, m) || nb < 0 || (nb > 0 && (!bi || !bj || !st)))
+        return fail(c, GODAR_ERR_ARG, "set_bonds: bad argument");
+    const Member& mm = c->members[m];
+    int64_t S = c->S;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    int o = c->parity;
+    std::vector<int32_t> cnt(mm.n, 0);
+    std::vector<BondRec> recs((int64_t)mm.n * c->BC);
+    for (int k = 0; k < nb; ++k) {
+        int i = bi[k], j = bj[k];
+        if (i < 0 || j <= i || j >= mm.n) return fail(c, GODAR_ERR_ARG, "set_bonds: need 0 <= i < j < n");
+        if (cnt[i] >= c->BC) return fail(c, GODAR_ERR_CAPACITY, "set_bonds: bond list full, raise it with godar_reserve");
+        BondRec& b = recs[(int64_t)cnt[i] * mm.n + i];
+        const double* s = st + 11 * k;
+        b.j = mm.first + j; b.bond = (int32_t)s[0]; b.damage = s[1]; b.deltanb = s[2]; b.deltatb = s[3];
+        b.trb_ji = s[4]; b.trb_ij = s[5]; b.rb = s[6]; b.hb = s[7]; b.lb = s[8]; b.sb = s[9]; b.ib = s[10];
+        cnt[i]++;
+    }
+    for (int q = 0; q < c->BC; ++q)
+        HIPCHK(c, hipMemcpy(c->bond[o] + q * S + mm.first, recs.data() + (int64_t)q * mm.n,
+                            sizeof(BondRec) * mm.n, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->bcnt[o] + mm.first, cnt.data(), sizeof(int32_t) * mm.n, hipMemcpyHostToDevice));
+    return GODAR_OK;
+}
+
+int godar_reserve(godar_ctx* c, int32_t outer, int32_t inner, int32_t histc, int32_t bonds) {
+    if (!c || outer < 0 || inner < 0 || histc < 0 || bonds < 0 || outer > 0xFFFE)
+        return fail(c, GODAR_ERR_ARG, "reserve: bad argument");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    int nCO = outer ? outer : c->CO, nCI = inner ? inner : c->CI;
+    int nHC = histc ? histc : c->HC, nBC = bonds ? bonds : c->BC;
+    if (nCO == c->CO && nCI == c->CI && nHC == c->HC && nBC == c->BC) return GODAR_OK;
+    // keep the pair state of the current parity
+    int64_t S = c->S;
+    int o = c->parity;
+    HistRec* oh = c->hist[o];
+    BondRec* ob = c->bond[o];
+    c->hist[o] = nullptr;
+    c->bond[o] = nullptr;
+    int oHC = c->HC, oBC = c->BC;
+    free_lists(c);
+    c->CO = nCO; c->CI = nCI; c->HC = nHC; c->BC = nBC;
+    hipError_t e = alloc_lists(c);
+    if (e != hipSuccess) { dfree(oh); dfree(ob); return fail(c, GODAR_ERR_NOMEM, "reserve: out of device memory"); }
+    HIPCHK(c, hipMemcpy(c->hist[o], oh, sizeof(HistRec) * S * std::min(oHC, nHC), hipMemcpyDeviceToDevice));
+    HIPCHK(c, hipMemcpy(c->bond[o], ob, sizeof(BondRec) * S * std::min(oBC, nBC), hipMemcpyDeviceToDevice));
+    dfree(oh);
+    dfree(ob);
+    return GODAR_OK;
+}
+
+int godar_set_block(godar_ctx* c, int32_t threads) {
+    if (!c || threads < 64 || threads > 1024 || (threads & 63)) return GODAR_ERR_ARG;
+    c->block = threads;
+    return GODAR_OK;
+}
+
+int godar_record_forces(godar_ctx* c, int32_t on) {
+    if (!c) return GODAR_ERR_ARG;
+    c->record = on ? 1 : 0;
+    return GODAR_OK;
+}
+
+int godar_step_async(godar_ctx* c, int32_t nsteps) {
+    if (!c || nsteps < 0) return GODAR_ERR_ARG;
+    if (!c->pending) {
+        int rc = download_members(c);
+        if (rc != GODAR_OK) return rc;
+        c->pend_t0.resize(c->Mcap);
+        for (int k = 0; k < c->Mcap; ++k) c->pend_t0[k] = c->members[k].tstep;
+        c->pend_parity = c->parity;
+        c->pending = true;
+    }
+    for (int k = 0; k < nsteps; ++k) {
+        int rc = launch_step(c);
+        if (rc != GODAR_OK) return rc;
+    }
+    return GODAR_OK;
+}
+
+int godar_sync(godar_ctx* c) {
+    if (!c) return GODAR_ERR_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    uint32_t e = 0;
+    HIPCHK(c, hipMemcpy(&e, c->d.err, sizeof(uint32_t), hipMemcpyDeviceToHost));
+    int rc = download_members(c);
+    if (rc != GODAR_OK) return rc;
+    bool was_pending = c->pending;
+    c->pending = false;
+    if (e) {
+        // Roll back to the last complete step.  Steps after the failing one
+        // were no-ops; the device step counters tell how many completed, which
+        // fixes the pair-state parity.  Repair flags and the error word are
+        // cleared so the caller can reserve more room and step again.
+        if (was_pending) {
+            int done = 0;
+            for (int k = 0; k < c->Mcap; ++k)
+                if (c->members[k].active) { done = c->members[k].tstep - c->pend_t0[k]; break; }
+            c->parity = c->pend_parity ^ (done & 1);
+            bind_parity(c);
+        }
+        HIPCHK(c, hipMemset(c->d.dirty, 0, c->S));
+        HIPCHK(c, hipMemset(c->d.affected, 0, c->S));
+        HIPCHK(c, hipMemset(c->d.err, 0, sizeof(uint32_t)));
+        return fail(c, GODAR_ERR_CAPACITY, "step aborted:" + err_text(e));
+    }
+    return GODAR_OK;
+}
+
+int godar_step(godar_ctx* c, int32_t nsteps) {
+    if (!c || nsteps < 0) return GODAR_ERR_ARG;
+    int rc = godar_step_async(c, nsteps);
+    if (rc != GODAR_OK) return rc;
+    return godar_sync(c);
+}
+
+int godar_member_tstep(const godar_ctx* c, int32_t m, int32_t* t) {
+    if (!valid_member(c, m) || !t) return GODAR_ERR_ARG;
+    godar_ctx* cc = const_cast<godar_ctx*>(c);
+    int rc = download_members(cc);
+    if (rc != GODAR_OK) return rc;
+    *t = c->members[m].tstep;
+    return GODAR_OK;
+}
+
+int godar_get_floes(godar_ctx* c, int32_t m, double* x, double* y, double* u, double* v,
+                    double* theta, double* omega, double* r, double* h, double* mass,
+                    double* inertia, double* hfa, double* hfw) {
+    if (!valid_member(c, m)) return fail(c, GODAR_ERR_ARG, "get_floes: bad member");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const Member& mm = c->members[m];
+    DS& d = c->d;
+    double* dst[] = {x, y, u, v, theta, omega, r, h, mass, inertia, hfa, hfw};
+    double* src[] = {d.x, d.y, d.u, d.v, d.theta, d.omega, d.r, d.h, d.mass, d.inertia, d.hfa, d.hfw};
+    for (int k = 0; k < 12; ++k)
+        if (dst[k]) HIPCHK(c, hipMemcpy(dst[k], src[k] + mm.first, sizeof(double) * mm.n, hipMemcpyDeviceToHost));
+    return GODAR_OK;
+}
+
+int godar_get_forces(godar_ctx* c, int32_t m, double* f) {
+    if (!valid_member(c, m) || !f) return fail(c, GODAR_ERR_ARG, "get_forces: bad argument");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const Member& mm = c->members[m];
+    for (int k = 0; k < 33; ++k)
+        HIPCHK(c, hipMemcpy(f + (int64_t)k * mm.n, c->d.fout + k * c->S + mm.first, sizeof(double) * mm.n,
+                            hipMemcpyDeviceToHost));
+    return GODAR_OK;
+}
+
+int godar_get_walls(godar_ctx* c, int32_t m, double* w) {
+    if (!valid_member(c, m) || !w) return fail(c, GODAR_ERR_ARG, "get_walls: bad argument");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const Member& mm = c->members[m];
+    for (int k = 0; k < 4; ++k)
+        HIPCHK(c, hipMemcpy(w + (int64_t)k * mm.n, c->d.wbc + k * c->S + mm.first, sizeof(double) * mm.n,
+                            hipMemcpyDeviceToHost));
+    return GODAR_OK;
+}
+
+int godar_get_pairs(godar_ctx* c, int32_t m, int32_t kind, int32_t cap, int32_t* pi, int32_t* pj,
+                    int32_t* count) {
+    if (!valid_member(c, m) || !count || cap < 0 || (cap > 0 && (!pi || !pj)))
+        return fail(c, GODAR_ERR_ARG, "get_pairs: bad argument");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const Member& mm = c->members[m];
+    int64_t S = c->S;
+    std::vector<std::pair<int32_t, int32_t>> out;
+    if (kind == GODAR_PAIRS_CANDIDATE || kind == GODAR_PAIRS_CONTACT) {
+        std::vector<int32_t> cnt(mm.n);
+        HIPCHK(c, hipMemcpy(cnt.data(), c->d.ocnt + mm.first, sizeof(int32_t) * mm.n, hipMemcpyDeviceToHost));
+        std::vector<int32_t> oj((int64_t)c->CO * mm.n);
+        std::vector<uint32_t> ef((int64_t)c->CO * mm.n);
+        for (int s = 0; s < c->CO; ++s) {
+            HIPCHK(c, hipMemcpy(oj.data() + (int64_t)s * mm.n, c->d.oj + s * S + mm.first,
+                                sizeof(int32_t) * mm.n, hipMemcpyDeviceToHost));
+            HIPCHK(c, hipMemcpy(ef.data() + (int64_t)s * mm.n, c->d.evf + s * S + mm.first,
+                                sizeof(uint32_t) * mm.n, hipMemcpyDeviceToHost));
+        }
+        for (int i = 0; i < mm.n; ++i)
+            for (int s = 0; s < cnt[i] && s < c->CO; ++s) {
+                uint32_t fl = ef[(int64_t)s * mm.n + i];
+                if (fl & EV_REMOVED) continue;
+                if (kind == GODAR_PAIRS_CONTACT && !(fl & EV_CONTACT)) continue;
+                out.push_back({i, oj[(int64_t)s * mm.n + i] - mm.first});
+            }
+    } else if (kind == GODAR_PAIRS_BOND) {
+        int o = c->parity;
+        std::vector<int32_t> cnt(mm.n);
+        HIPCHK(c, hipMemcpy(cnt.data(), c->bcnt[o] + mm.first, sizeof(int32_t) * mm.n, hipMemcpyDeviceToHost));
+        std::vector<BondRec> rec(mm.n);
+        for (int q = 0; q < c->BC; ++q) {
+            HIPCHK(c, hipMemcpy(rec.data(), c->bond[o] + q * S + mm.first, sizeof(BondRec) * mm.n,
+                                hipMemcpyDeviceToHost));
+            for (int i = 0; i < mm.n; ++i)
+                if (q < cnt[i] && rec[i].j >= 0 && rec[i].bond == 1) out.push_back({i, rec[i].j - mm.first});
+        }
+    } else {
+        return fail(c, GODAR_ERR_ARG, "get_pairs: bad kind");
+    }
+    std::sort(out.begin(), out.end());
+    *count = (int32_t)out.size();
+    for (int32_t k = 0; k < cap && k < (int32_t)out.size(); ++k) { pi[k] = out[k].first; pj[k] = out[k].second; }
+    return GODAR_OK;
+}
+
+int godar_get_bond_state(godar_ctx* c, int32_t m, int32_t cap, double* st, int32_t* count) {
+    if (!valid_member(c, m) || !count || cap < 0 || (cap > 0 && !st))
+        return fail(c, GODAR_ERR_ARG, "get_bond_state: bad argument");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const Member& mm = c->members[m];
+    int64_t S = c->S;
+    int o = c->parity;
+    std::vector<int32_t> cnt(mm.n);
+    HIPCHK(c, hipMemcpy(cnt.data(), c->bcnt[o] + mm.first, sizeof(int32_t) * mm.n, hipMemcpyDeviceToHost));
+    std::vector<std::pair<std::pair<int32_t, int32_t>, BondRec>> out;
+    std::vector<BondRec> rec(mm.n);
+    for (int q = 0; q < c->BC; ++q) {
+        HIPCHK(c, hipMemcpy(rec.data(), c->bond[o] + q * S + mm.first, sizeof(BondRec) * mm.n,
+                            hipMemcpyDeviceToHost));
+        for (int i = 0; i < mm.n; ++i)
+            if (q < cnt[i] && rec[i].j >= 0 && rec[i].bond == 1)
+                out.push_back({{i, rec[i].j - mm.first}, rec[i]});
+    }
+    std::sort(out.begin(), out.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+    *count = (int32_t)out.size();
+    for (int32_t k = 0; k < cap && k < (int32_t)out.size(); ++k) {
+        const BondRec& b = out[k].second;
+        double v[11] = {(double)b.bond, b.damage, b.deltanb, b.deltatb, b.trb_ji, b.trb_ij, b.rb, b.hb, b.lb, b.sb, b.ib};
+        std::memcpy(st + 11 * (int64_t)k, v, sizeof(v));
+    }
+    return GODAR_OK;
+}
+
+int godar_get_stats(godar_ctx* c, int32_t m, int64_t* stats) {
+    if (!valid_member(c, m) || !stats) return fail(c, GODAR_ERR_ARG, "get_stats: bad argument");
+    int32_t cnt = 0;
+    int rc = godar_get_pairs(c, m, GODAR_PAIRS_CANDIDATE, 0, nullptr, nullptr, &cnt);
+    if (rc != GODAR_OK) return rc;
+    stats[0] = cnt;
+    rc = godar_get_pairs(c, m, GODAR_PAIRS_CONTACT, 0, nullptr, nullptr, &cnt);
+    if (rc != GODAR_OK) return rc;
+    stats[1] = cnt;
+    int64_t dev[4];
+    HIPCHK(c, hipMemcpy(dev, c->d.stats + 4 * m, sizeof(dev), hipMemcpyDeviceToHost));
+    stats[2] = dev[2];
+    stats[3] = dev[3];
+    return GODAR_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,698 @@
+// Kernels of the GODAR step pipeline (one launch sequence per step):
+//
+//   k_bin -> radix sort -> k_clear_cells -> k_ranges -> k_cand -> k_spec
+//        -> k_repair -> k_fixup -> k_final
+//
+// Sequential semantics of the reference (stepper.f90:71-280 visits floes i
+// in index order and, for each, its candidates j > i in ascending distance;
+// ridging reshapes floes in place, so later pairs see the new radius and
+// thickness) are reproduced as follows:
+//   * k_spec evaluates every pair event with the start-of-step floe
+//     properties.  One thread per floe gathers its own contributions in the
+//     reference's per-floe order: first the pairs where it is the inner
+//     index (ascending outer index), then its own candidates in distance
+//     order.  Each pair is evaluated by both of its floes, which removes
+//     atomics and keeps FP64 sums identical to the reference.
+//   * An event's speculative result is exact unless one of its floes was
+//     reshaped earlier in the sequence.  Events that ridge are seeds; the
+//     per-member k_repair walks, in sequence order, the seeds and every later
+//     event of each reshaped floe, recomputes them with the current floe
+//     properties and records the results.
+//   * k_fixup re-gathers the floes touched by the repair, k_final applies
+//     the final floe properties, forcing, Coriolis, totals and integration.
+#include "godar_engine.h"
+#include "godar_physics.h"
+
+namespace godar {
+
+__device__ __forceinline__ void set_err(const DS& S, uint32_t bit) { atomicOr(S.err, bit); }
+
+__device__ __forceinline__ Floe load_floe(const DS& S, int f) {
+    Floe F;
+    F.x = S.x[f]; F.y = S.y[f]; F.u = S.u[f]; F.v = S.v[f]; F.omega = S.omega[f];
+    F.r = S.r[f]; F.h = S.h[f]; F.mass = S.mass[f]; F.inertia = S.inertia[f];
+    F.hfa = S.hfa[f]; F.hfw = S.hfw[f];
+    return F;
+}
+
+// floe with the properties current during the ordered repair
+__device__ __forceinline__ Floe load_floe_cur(const DS& S, int f) {
+    Floe F = load_floe(S, f);
+    if (S.dirty[f]) {
+        F.r = S.cur[0 * S.S + f]; F.h = S.cur[1 * S.S + f]; F.mass = S.cur[2 * S.S + f];
+        F.inertia = S.cur[3 * S.S + f]; F.hfa = S.cur[4 * S.S + f]; F.hfw = S.cur[5 * S.S + f];
+    }
+    return F;
+}
+
+__device__ __forceinline__ int cell_coord(double p, double cell, int g) {
+    double q = floor(p / cell);
+    if (!(q >= 0.0)) return 0;          // also catches NaN
+    if (q >= (double)(g - 1)) return g - 1;
+    return (int)q;
+}
+
+// Old pair state of (i, j), owner i.
+__device__ __forceinline__ void load_pair_state(const DS& S, int i, int j, PairState& ps) {
+    ps.deltat = 0.0; ps.thetarelc = 0.0; ps.bond = 0; ps.has_bond_rec = false;
+    ps.damage = ps.deltanb = ps.deltatb = ps.trb_ji = ps.trb_ij = 0.0;
+    ps.rb = ps.hb = ps.lb = ps.sb = ps.ib = 0.0;
+    int hc = S.hcnt_old[i];
+    for (int q = 0; q < hc; ++q) {
+        const HistRec& hr = S.hist_old[(int64_t)q * S.S + i];
+        if (hr.j == j) { ps.deltat = hr.deltat; ps.thetarelc = hr.thetarelc; break; }
+    }
+    int bc = S.bcnt_old[i];
+    for (int q = 0; q < bc; ++q) {
+        const BondRec& b = S.bond_old[(int64_t)q * S.S + i];
+        if (b.j == j) {
+            ps.has_bond_rec = true; ps.bond = b.bond; ps.damage = b.damage;
+            ps.deltanb = b.deltanb; ps.deltatb = b.deltatb; ps.trb_ji = b.trb_ji; ps.trb_ij = b.trb_ij;
+            ps.rb = b.rb; ps.hb = b.hb; ps.lb = b.lb; ps.sb = b.sb; ps.ib = b.ib;
+            break;
+        }
+    }
+}
+
+__device__ __forceinline__ BondRec bond_from(const PairState& ps, int j) {
+    BondRec b;
+    b.j = j; b.bond = ps.bond; b.damage = ps.damage; b.deltanb = ps.deltanb; b.deltatb = ps.deltatb;
+    b.trb_ji = ps.trb_ji; b.trb_ij = ps.trb_ij; b.rb = ps.rb; b.hb = ps.hb; b.lb = ps.lb;
+    b.sb = ps.sb; b.ib = ps.ib;
+    return b;
+}
+
+// Gather of one event into floe i's accumulators (stepper.f90:113-146,
+// 165-198, 215-223), operand order kept.
+__device__ __forceinline__ void apply_i(double* a, const EventOut& o) {
+    if (o.flags & EV_CONTACT) {
+        a[A_FCX] = a[A_FCX] - o.fcn_c + o.fcr_s;
+        a[A_FCY] = a[A_FCY] - o.fcn_s - o.fcr_c;
+        a[A_MC] = a[A_MC] - o.ri_fct - o.mcc;
+        a[A_TAC] = a[A_TAC] + o.ac;
+        a[A_PC] = a[A_PC] - o.fcn_ac;
+    }
+    if (o.flags & EV_BONDED) {
+        a[A_FBX] = a[A_FBX] + o.fbn_c - o.fbt_s;
+        a[A_FBY] = a[A_FBY] + o.fbn_s + o.fbt_c;
+        a[A_MB] = a[A_MB] - o.lri_fbt - o.mbb_ji;
+        a[A_TAB] = a[A_TAB] + o.sb;
+        a[A_PB] = a[A_PB] - o.fbn_sb;
+    }
+    if (o.flags & EV_SHELTER) {
+        a[A_SHA] = fmin(a[A_SHA], o.hsfa_ji);
+        a[A_SHW] = fmin(a[A_SHW], o.hsfw_ji);
+    }
+}
+
+// ... and into floe j's.
+__device__ __forceinline__ void apply_j(double* a, const EventOut& o) {
+    if (o.flags & EV_CONTACT) {
+        a[A_FCX] = a[A_FCX] + o.fcn_c - o.fcr_s;
+        a[A_FCY] = a[A_FCY] + o.fcn_s + o.fcr_c;
+        a[A_MC] = a[A_MC] - o.rj_fct + o.mcc;
+        a[A_TAC] = a[A_TAC] + o.ac;
+        a[A_PC] = a[A_PC] - o.fcn_ac;
+    }
+    if (o.flags & EV_BONDED) {
+        a[A_FBX] = a[A_FBX] - o.fbn_c + o.fbt_s;
+        a[A_FBY] = a[A_FBY] - o.fbn_s - o.fbt_c;
+        a[A_MB] = a[A_MB] - o.lrj_fbt - o.mbb_ij;
+        a[A_TAB] = a[A_TAB] + o.sb;
+        a[A_PB] = a[A_PB] - o.fbn_sb;
+    }
+    if (o.flags & EV_SHELTER) {
+        a[A_SHA] = fmin(a[A_SHA], o.hsfa_ij);
+        a[A_SHW] = fmin(a[A_SHW], o.hsfw_ij);
+    }
+}
+
+__device__ __forceinline__ void acc_init(double* a) {
+#pragma unroll
+    for (int k = 0; k < NACC; ++k) a[k] = 0.0;
+    a[A_SHA] = 1.0;
+    a[A_SHW] = 1.0;
+}
+
+// ---------------------------------------------------------------------
+// 1. cell keys and per-floe wind (winds_currents, forcing.f90:1-29)
+// ---------------------------------------------------------------------
+__global__ void k_bin(DS S) {
+    int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= S.N || *S.err) return;
+    int m = S.member_of[f];
+    if (m < 0) { S.ckey[f] = (uint32_t)S.ncells; S.cval[f] = (uint32_t)f; return; }
+    const DevParams& P = S.params[m];
+    double x = S.x[f], y = S.y[f];
+    if (P.wind_grid >= 0) {
+        double ua, va;
+        sample_wind(S.wgrids[P.wind_grid], S.wpool, x, y, ua, va);
+        S.uai[f] = ua; S.vai[f] = va;
+    } else {
+        S.uai[f] = P.ua; S.vai[f] = P.va;
+    }
+    int cx = cell_coord(x, P.cell, P.gx), cy = cell_coord(y, P.cell, P.gy);
+    S.ckey[f] = (uint32_t)(P.cell_base + cy * P.gx + cx);
+    S.cval[f] = (uint32_t)f;
+    if (f == S.members[m].first) {
+        S.nseeds[m] = 0; S.netab[m] = 0; S.nwtab[m] = 0;
+        S.stats[4 * m + 2] = 0; S.stats[4 * m + 3] = 0;
+    }
+}
+
+__global__ void k_clear_cells(DS S) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c > S.ncells || *S.err) return;
+    S.cstart[c] = 0;
+    S.cend[c] = 0;
+}
+
+// cell ranges over the sorted keys, plus sorted copies of x, y, r
+__global__ void k_ranges(DS S) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= S.N || *S.err) return;
+    uint32_t key = S.ckey[k];
+    int f = (int)S.cval[k];
+    S.sidx[k] = f;
+    S.sx[k] = S.x[f]; S.sy[k] = S.y[f]; S.sr[k] = S.r[f];
+    if (key >= (uint32_t)S.ncells) return;
+    if (k == 0 || S.ckey[k - 1] != key) S.cstart[key] = k;
+    if (k == S.N - 1 || S.ckey[k + 1] != key) S.cend[key] = k + 1;
+}
+
+// ---------------------------------------------------------------------
+// 2. candidate lists (replaces coretran kNearest, stepper.f90:77-87)
+//    outer(f) = { j > f : dist <= r(f) + rtree } by (dist, j)
+//    inner(f) = { k < f : dist <= r(k) + rtree } by k
+// ---------------------------------------------------------------------
+__global__ void k_cand(DS S) {
+    int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= S.N || *S.err) return;
+    int m = S.member_of[f];
+    if (m < 0) { S.ocnt[f] = 0; S.icnt[f] = 0; return; }
+    const DevParams& P = S.params[m];
+    double xf = S.x[f], yf = S.y[f];
+    double rad = S.r[f] + P.rtree;
+    int cx = cell_coord(xf, P.cell, P.gx), cy = cell_coord(yf, P.cell, P.gy);
+    int no = 0, ni = 0;
+    uint32_t over = 0;
+    const int64_t SS = S.S;
+    for (int b = cy - 1; b <= cy + 1; ++b) {
+        if (b < 0 || b >= P.gy) continue;
+        for (int a = cx - 1; a <= cx + 1; ++a) {
+            if (a < 0 || a >= P.gx) continue;
+            int c = P.cell_base + b * P.gx + a;
+            int k1 = S.cend[c];
+            for (int k = S.cstart[c]; k < k1; ++k) {
+                int j = S.sidx[k];
+                if (j == f) continue;
+                double dx = xf - S.sx[k], dy = yf - S.sy[k];
+                double d = sqrt(dx * dx + dy * dy);
+                if (j > f) {
+                    if (d <= rad) {
+                        if (no >= S.CO) { over |= ERR_OUTER_CAP; continue; }
+                        int p = no;
+                        while (p > 0) {
+                            double dp = S.od[(p - 1) * SS + f];
+                            int jp = S.oj[(p - 1) * SS + f];
+                            if (dp < d || (dp == d && jp < j)) break;
+                            S.od[p * SS + f] = dp;
+                            S.oj[p * SS + f] = jp;
+                            --p;
+                        }
+                        S.od[p * SS + f] = d;
+                        S.oj[p * SS + f] = j;
+                        ++no;
+                    }
+                } else {
+                    if (d <= S.sr[k] + P.rtree) {
+                        if (ni >= S.CI) { over |= ERR_INNER_CAP; continue; }
+                        int p = ni;
+                        while (p > 0) {
+                            int jp = S.ik[(p - 1) * SS + f];
+                            if (jp < j) break;
+                            S.ik[p * SS + f] = jp;
+                            --p;
+                        }
+                        S.ik[p * SS + f] = j;
+                        ++ni;
+                    }
+                }
+            }
+        }
+    }
+    if (over) set_err(S, over);
+    S.ocnt[f] = no;
+    S.icnt[f] = ni;
+}
+
+// ---------------------------------------------------------------------
+// 3. speculative events + per-floe gather + owner state writes
+// ---------------------------------------------------------------------
+template <bool MIXED>
+__global__ void k_spec(DS S) {
+    int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= S.N || *S.err) return;
+    int m = S.member_of[f];
+    if (m < 0) return;
+    const DevParams& P = S.params[m];
+    const int tstep = S.members[m].tstep;
+    const int first = S.members[m].first;
+    const int64_t SS = S.S;
+    const Floe F0 = load_floe(S, f);
+    double acc[NACC];
+    acc_init(acc);
+    uint32_t over = 0;
+
+    // pairs (k, f), f inner, ascending k
+    const int ni = S.icnt[f];
+    for (int q = 0; q < ni; ++q) {
+        int k = S.ik[q * SS + f];
+        Floe Fk = load_floe(S, k);
+        Floe Ff = F0;
+        PairState ps;
+        load_pair_state(S, k, f, ps);
+        EventOut o;
+        pair_event<MIXED>(P, tstep, k - first, f - first, Fk, Ff, S.uai[k], S.vai[k], P.uw, P.vw,
+                          ps, o, 2);
+        apply_j(acc, o);
+    }
+
+    // pairs (f, j), f outer (owner), distance order
+    const int no = S.ocnt[f];
+    int nb = 0;
+    {
+        const int bco = S.bcnt_old[f];
+        for (int q = 0; q < bco; ++q) {
+            BondRec b = S.bond_old[q * SS + f];
+            if (b.j >= 0) S.bond_new[(nb++) * SS + f] = b;   // drops cleared entries
+        }
+    }
+    int nh = 0;
+    const double uaf = S.uai[f], vaf = S.vai[f];
+    for (int s = 0; s < no; ++s) {
+        int j = S.oj[s * SS + f];
+        Floe Fi = F0;
+        Floe Fj = load_floe(S, j);
+        PairState ps;
+        load_pair_state(S, f, j, ps);
+        EventOut o;
+        pair_event<MIXED>(P, tstep, f - first, j - first, Fi, Fj, uaf, vaf, P.uw, P.vw, ps, o, 1);
+        apply_i(acc, o);
+        S.evf[s * SS + f] = o.flags;
+        S.ov[s * SS + f] = -1;
+        if (ps.deltat != 0.0 || ps.thetarelc != 0.0) {
+            if (nh < S.HC) {
+                HistRec hr; hr.j = j; hr.pad_ = 0; hr.deltat = ps.deltat; hr.thetarelc = ps.thetarelc;
+                S.hist_new[nh * SS + f] = hr;
+                ++nh;
+            } else {
+                over |= ERR_HIST_CAP;
+            }
+        }
+        if (ps.has_bond_rec) {
+            int qb = -1;
+            for (int q = 0; q < nb; ++q)
+                if (S.bond_new[q * SS + f].j == j) { qb = q; break; }
+            if (qb < 0) {
+                if (nb < S.BC) qb = nb++;
+                else over |= ERR_BOND_CAP;
+            }
+            if (qb >= 0) S.bond_new[qb * SS + f] = bond_from(ps, j);
+        }
+        if (o.flags & EV_RIDGED) {
+            int p = atomicAdd(&S.nseeds[m], 1);
+            if (p < S.SEEDK * S.members[m].n)
+                S.seeds[(int64_t)S.SEEDK * first + p] = ((uint64_t)(f - first) << 16) | (uint64_t)s;
+            else
+                over |= ERR_SEED_CAP;
+        }
+    }
+    // contact histories of pairs that are not candidates this step persist
+    // (the reference only resets a candidate pair, reset_contact)
+    const int hco = S.hcnt_old[f];
+    for (int q = 0; q < hco; ++q) {
+        HistRec hr = S.hist_old[q * SS + f];
+        if (hr.j < 0) continue;
+        bool cand = false;
+        for (int s = 0; s < no; ++s)
+            if (S.oj[s * SS + f] == hr.j) { cand = true; break; }
+        if (!cand) {
+            if (nh < S.HC) S.hist_new[nh * SS + f] = hr, ++nh;
+            else over |= ERR_HIST_CAP;
+        }
+    }
+    S.hcnt_new[f] = nh;
+    S.bcnt_new[f] = nb;
+
+    // walls of f (verify_bc, boundaries.f90:1-97)
+    {
+        Floe Fw = F0;
+        double hist[4] = {S.wbc[0 * SS + f], S.wbc[1 * SS + f], S.wbc[2 * SS + f], S.wbc[3 * SS + f]};
+        WallOut W;
+        walls<MIXED>(P, Fw, hist, W);
+        S.wout[f] = W;
+        if (W.flags & 4u) {
+            int p = atomicAdd(&S.nseeds[m], 1);
+            if (p < S.SEEDK * S.members[m].n)
+                S.seeds[(int64_t)S.SEEDK * first + p] = ((uint64_t)(f - first) << 16) | WALL_SLOT;
+            else
+                over |= ERR_SEED_CAP;
+        }
+    }
+    S.ovw[f] = -1;
+#pragma unroll
+    for (int a = 0; a < NACC; ++a) S.acc[a * SS + f] = acc[a];
+    if (over) set_err(S, over);
+}
+
+// ---------------------------------------------------------------------
+// 4. ordered repair, one member per block, serial in lane 0
+// ---------------------------------------------------------------------
+struct Heap {
+    uint64_t* a;
+    int n, cap;
+    __device__ bool push(uint64_t k) {
+        if (n >= cap) return false;
+        int i = n++;
+        while (i > 0) {
+            int p = (i - 1) >> 1;
+            if (a[p] <= k) break;
+            a[i] = a[p];
+            i = p;
+        }
+        a[i] = k;
+        return true;
+    }
+    __device__ uint64_t pop() {
+        uint64_t top = a[0];
+        uint64_t last = a[--n];
+        int i = 0;
+        while (true) {
+            int l = 2 * i + 1;
+            if (l >= n) break;
+            int c = (l + 1 < n && a[l + 1] < a[l]) ? l + 1 : l;
+            if (a[c] >= last) break;
+            a[i] = a[c];
+            i = c;
+        }
+        if (n > 0) a[i] = last;
+        return top;
+    }
+};
+
+__device__ __forceinline__ void set_cur(const DS& S, int g, double r, double h, double mass,
+                                        double inertia, double hfa, double hfw) {
+    S.cur[0 * S.S + g] = r; S.cur[1 * S.S + g] = h; S.cur[2 * S.S + g] = mass;
+    S.cur[3 * S.S + g] = inertia; S.cur[4 * S.S + g] = hfa; S.cur[5 * S.S + g] = hfw;
+}
+
+__device__ __forceinline__ int slot_of(const DS& S, int k, int g) {
+    int no = S.ocnt[k];
+    for (int s = 0; s < no; ++s)
+        if (S.oj[(int64_t)s * S.S + k] == g) return s;
+    return -1;
+}
+
+// write (or clear) owner i's new contact history entry for partner j
+__device__ __forceinline__ bool put_hist(const DS& S, int i, int j, bool want, double dt_, double th_) {
+    int n = S.hcnt_new[i];
+    int q = -1;
+    for (int p = 0; p < n; ++p)
+        if (S.hist_new[(int64_t)p * S.S + i].j == j) { q = p; break; }
+    if (want) {
+        if (q < 0) {
+            if (n >= S.HC) return false;
+            q = n;
+            S.hcnt_new[i] = n + 1;
+        }
+        HistRec hr; hr.j = j; hr.pad_ = 0; hr.deltat = dt_; hr.thetarelc = th_;
+        S.hist_new[(int64_t)q * S.S + i] = hr;
+    } else if (q >= 0) {
+        S.hist_new[(int64_t)q * S.S + i].j = -1;
+    }
+    return true;
+}
+
+__device__ __forceinline__ bool put_bond(const DS& S, int i, int j, bool want, const BondRec& b) {
+    int n = S.bcnt_new[i];
+    int q = -1;
+    for (int p = 0; p < n; ++p)
+        if (S.bond_new[(int64_t)p * S.S + i].j == j) { q = p; break; }
+    if (want) {
+        if (q < 0) {
+            if (n >= S.BC) return false;
+            q = n;
+            S.bcnt_new[i] = n + 1;
+        }
+        S.bond_new[(int64_t)q * S.S + i] = b;
+    } else if (q >= 0) {
+        S.bond_new[(int64_t)q * S.S + i].j = -1;
+    }
+    return true;
+}
+
+template <bool MIXED>
+__global__ void k_repair(DS S, int nmembers) {
+    int m = blockIdx.x;
+    if (m >= nmembers || threadIdx.x != 0 || *S.err) return;
+    if (!S.members[m].active) return;
+    int ns = S.nseeds[m];
+    if (ns == 0) return;
+    const DevParams& P = S.params[m];
+    const int first = S.members[m].first;
+    const int nmem = S.members[m].n;
+    const int tstep = S.members[m].tstep;
+    const int64_t SS = S.S;
+    if (ns > S.SEEDK * nmem) ns = S.SEEDK * nmem;
+    Heap H;
+    H.a = S.heap + (int64_t)S.HEAPK * first;
+    H.n = 0;
+    H.cap = S.HEAPK * nmem;
+    uint32_t over = 0;
+    for (int p = 0; p < ns; ++p)
+        if (!H.push(S.seeds[(int64_t)S.SEEDK * first + p])) over |= ERR_REPAIR_CAP;
+    EventOut* etab = S.etab + (int64_t)S.TABK * first;
+    const int tcap = S.TABK * nmem;
+    WallOut* wtab = S.wtab + first;
+    int nt = 0, nwt = 0;
+    uint64_t last = ~0ull;
+    int cur_block = -1;
+    double rq = 0.0;
+    int64_t repaired = 0;
+    while (H.n > 0 && !over) {
+        uint64_t key = H.pop();
+        if (key == last) continue;
+        last = key;
+        int b = (int)(key >> 16), s = (int)(key & 0xFFFF);
+        int f = first + b;
+        ++repaired;
+        if (s == WALL_SLOT) {
+            Floe F = load_floe_cur(S, f);
+            double hist[4] = {S.wbc[0 * SS + f], S.wbc[1 * SS + f], S.wbc[2 * SS + f], S.wbc[3 * SS + f]};
+            WallOut W;
+            walls<MIXED>(P, F, hist, W);
+            if (nwt >= nmem) { over |= ERR_REPAIR_CAP; break; }
+            wtab[nwt] = W;
+            S.ovw[f] = nwt++;
+            S.affected[f] = 1;
+            if (W.flags & 4u) {
+                set_cur(S, f, W.nr, W.nh, W.nmass, W.ninertia, W.nhfa, W.nhfw);
+                S.dirty[f] = 1;
+            }
+            continue;
+        }
+        if (b != cur_block) {
+            cur_block = b;
+            rq = S.dirty[f] ? S.cur[f] : S.r[f];   // query radius at the top of iteration f
+        }
+        int j = S.oj[(int64_t)s * SS + f];
+        Floe Fi = load_floe_cur(S, f);
+        Floe Fj = load_floe_cur(S, j);
+        S.affected[f] = 1;
+        S.affected[j] = 1;
+        EventOut o;
+        double dx = Fi.x - Fj.x, dy = Fi.y - Fj.y;
+        double d = sqrt(dx * dx + dy * dy);
+        if (!(d <= rq + P.rtree)) {
+            // no longer a candidate: the pair keeps its old state
+            o.flags = EV_REMOVED;
+            o.ridged = -1;
+            PairState ps;
+            load_pair_state(S, f, j, ps);
+            bool okh = put_hist(S, f, j, ps.deltat != 0.0 || ps.thetarelc != 0.0, ps.deltat, ps.thetarelc);
+            bool okb = put_bond(S, f, j, ps.has_bond_rec, bond_from(ps, j));
+            if (!okh) over |= ERR_HIST_CAP;
+            if (!okb) over |= ERR_BOND_CAP;
+        } else {
+            PairState ps;
+            load_pair_state(S, f, j, ps);
+            pair_event<MIXED>(P, tstep, b, j - first, Fi, Fj, S.uai[f], S.vai[f], P.uw, P.vw, ps, o, 0);
+            bool okh = put_hist(S, f, j, ps.deltat != 0.0 || ps.thetarelc != 0.0, ps.deltat, ps.thetarelc);
+            bool okb = put_bond(S, f, j, ps.has_bond_rec, bond_from(ps, j));
+            if (!okh) over |= ERR_HIST_CAP;
+            if (!okb) over |= ERR_BOND_CAP;
+            if (o.flags & EV_RIDGED) {
+                int g = o.ridged == 0 ? f : j;
+                set_cur(S, g, o.nr, o.nh, o.nmass, o.ninertia, o.nhfa, o.nhfw);
+                if (!S.dirty[g]) {
+                    S.dirty[g] = 1;
+                    int gb = g - first;
+                    // every later event of g is recomputed
+                    if (g != f) {
+                        int gi = S.icnt[g];
+                        for (int q = 0; q < gi; ++q) {
+                            int k = S.ik[(int64_t)q * SS + g];
+                            if (k <= f) continue;
+                            int sk = slot_of(S, k, g);
+                            if (sk >= 0 && !H.push(((uint64_t)(k - first) << 16) | (uint64_t)sk))
+                                over |= ERR_REPAIR_CAP;
+                        }
+                    }
+                    int s0 = (g == f) ? s + 1 : 0;
+                    int gno = S.ocnt[g];
+                    for (int s2 = s0; s2 < gno; ++s2)
+                        if (!H.push(((uint64_t)gb << 16) | (uint64_t)s2)) over |= ERR_REPAIR_CAP;
+                    if (!H.push(((uint64_t)gb << 16) | WALL_SLOT)) over |= ERR_REPAIR_CAP;
+                }
+            }
+        }
+        if (nt >= tcap) { over |= ERR_REPAIR_CAP; break; }
+        etab[nt] = o;
+        S.ov[(int64_t)s * SS + f] = nt++;
+        S.evf[(int64_t)s * SS + f] = o.flags;
+    }
+    S.netab[m] = nt;
+    S.nwtab[m] = nwt;
+    S.stats[4 * m + 2] = ns;
+    S.stats[4 * m + 3] = repaired;
+    if (over) set_err(S, over);
+}
+
+// ---------------------------------------------------------------------
+// 5. re-gather floes touched by the repair
+// ---------------------------------------------------------------------
+template <bool MIXED>
+__global__ void k_fixup(DS S) {
+    int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= S.N || *S.err) return;
+    if (!S.affected[f]) return;
+    int m = S.member_of[f];
+    if (m < 0) return;
+    const DevParams& P = S.params[m];
+    const int tstep = S.members[m].tstep;
+    const int first = S.members[m].first;
+    const int64_t SS = S.S;
+    const EventOut* etab = S.etab + (int64_t)S.TABK * first;
+    const Floe F0 = load_floe(S, f);
+    double acc[NACC];
+    acc_init(acc);
+    const int ni = S.icnt[f];
+    for (int q = 0; q < ni; ++q) {
+        int k = S.ik[q * SS + f];
+        int sk = slot_of(S, k, f);
+        int idx = sk >= 0 ? S.ov[(int64_t)sk * SS + k] : -1;
+        if (idx >= 0) {
+            EventOut o = etab[idx];
+            if (!(o.flags & EV_REMOVED)) apply_j(acc, o);
+        } else {
+            Floe Fk = load_floe(S, k);
+            Floe Ff = F0;
+            PairState ps;
+            load_pair_state(S, k, f, ps);
+            EventOut o;
+            pair_event<MIXED>(P, tstep, k - first, f - first, Fk, Ff, S.uai[k], S.vai[k], P.uw, P.vw,
+                              ps, o, 2);
+            apply_j(acc, o);
+        }
+    }
+    const int no = S.ocnt[f];
+    for (int s = 0; s < no; ++s) {
+        int idx = S.ov[s * SS + f];
+        if (idx >= 0) {
+            EventOut o = etab[idx];
+            if (!(o.flags & EV_REMOVED)) apply_i(acc, o);
+        } else {
+            int j = S.oj[s * SS + f];
+            Floe Fi = F0;
+            Floe Fj = load_floe(S, j);
+            PairState ps;
+            load_pair_state(S, f, j, ps);
+            EventOut o;
+            pair_event<MIXED>(P, tstep, f - first, j - first, Fi, Fj, S.uai[f], S.vai[f], P.uw, P.vw,
+                              ps, o, 1);
+            apply_i(acc, o);
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < NACC; ++a) S.acc[a * SS + f] = acc[a];
+}
+
+// ---------------------------------------------------------------------
+// 6. final properties, forcing + coriolis, totals, integration
+//    (stepper.f90:295-343, integration.f90:1-38)
+// ---------------------------------------------------------------------
+template <bool MIXED>
+__global__ void k_final(DS S) {
+    int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= S.N || *S.err) return;
+    int m = S.member_of[f];
+    if (m < 0) return;
+    const DevParams& P = S.params[m];
+    const int64_t SS = S.S;
+    Floe F = load_floe(S, f);
+    if (S.dirty[f]) {
+        F.r = S.cur[0 * SS + f]; F.h = S.cur[1 * SS + f]; F.mass = S.cur[2 * SS + f];
+        F.inertia = S.cur[3 * SS + f]; F.hfa = S.cur[4 * SS + f]; F.hfw = S.cur[5 * SS + f];
+        S.r[f] = F.r; S.h[f] = F.h; S.mass[f] = F.mass; S.inertia[f] = F.inertia;
+        S.hfa[f] = F.hfa; S.hfw[f] = F.hfw;
+    }
+    int wi = S.ovw[f];
+    const WallOut W = wi >= 0 ? S.wtab[S.members[m].first + wi] : S.wout[f];
+    S.wbc[0 * SS + f] = W.deltat_bc1; S.wbc[1 * SS + f] = W.deltat_bc2;
+    S.wbc[2 * SS + f] = W.theta_bc1; S.wbc[3 * SS + f] = W.theta_bc2;
+    double a[NACC];
+#pragma unroll
+    for (int k = 0; k < NACC; ++k) a[k] = S.acc[k * SS + f];
+    // shelter minima (reset_shelter + thread reduction); with shelter off
+    // the reference's minima stay at their zero initial value
+    double sha = P.shelter ? a[A_SHA] : 0.0;
+    double shw = P.shelter ? a[A_SHW] : 0.0;
+    Forcing Fo;
+    forcing<MIXED>(P, F, S.uai[f], S.vai[f], P.uw, P.vw, sha, shw, Fo);
+    double tfx = a[A_FCX] + a[A_FBX] + Fo.fax + Fo.fwx + Fo.fcorx + W.fx_bc;
+    double tfy = a[A_FCY] + a[A_FBY] + Fo.fay + Fo.fwy + Fo.fcory + W.fy_bc;
+    double mt = a[A_MC] + a[A_MB] + Fo.ma + Fo.mw + W.m_bc;
+    double u = F.u + (tfx / F.mass) * P.dt;
+    double v = F.v + (tfy / F.mass) * P.dt;
+    double om = F.omega + (mt / F.inertia) * P.dt;
+    S.u[f] = u; S.v[f] = v; S.omega[f] = om;
+    S.x[f] = F.x + u * P.dt;
+    S.y[f] = F.y + v * P.dt;
+    S.theta[f] = S.theta[f] + om * P.dt;
+    if (S.record_forces) {
+        double tp = (a[A_TAC] != 0.0 ? a[A_PC] / a[A_TAC] : 0.0)
+                  + (a[A_TAB] != 0.0 ? a[A_PB] / a[A_TAB] : 0.0)
+                  + (W.ta_bc != 0.0 ? W.p_bc / W.ta_bc : 0.0);
+        double cols[33] = {tfx, tfy, mt, a[A_FCX], a[A_FCY], a[A_MC], a[A_FBX], a[A_FBY], a[A_MB],
+                           Fo.fax, Fo.fay, Fo.ma, Fo.fwx, Fo.fwy, Fo.mw, Fo.fcorx, Fo.fcory,
+                           W.fx_bc, W.fy_bc, W.m_bc, sha, shw, tp, 0.0, 0.0, 0.0, 0.0,
+                           a[A_TAC], a[A_PC], a[A_TAB], a[A_PB], W.ta_bc, W.p_bc};
+        for (int c = 0; c < 33; ++c) S.fout[c * SS + f] = cols[c];
+    }
+    S.dirty[f] = 0;
+    S.affected[f] = 0;
+    if (f == S.members[m].first) S.members[m].tstep += 1;
+}
+
+// explicit instantiations used by the host launcher
+template __global__ void k_spec<false>(DS);
+template __global__ void k_spec<true>(DS);
+template __global__ void k_repair<false>(DS, int);
+template __global__ void k_repair<true>(DS, int);
+template __global__ void k_fixup<false>(DS);
+template __global__ void k_fixup<true>(DS);
+template __global__ void k_final<false>(DS);
+template __global__ void k_final<true>(DS);
+
+}  // namespace godar

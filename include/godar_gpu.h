@@ -1,0 +1,157 @@
+/*
+ * godar_gpu.h -- C ABI of the MI355X-native GODAR step library
+ * (libgodar_amd.so).
+ *
+ * The reference has no FFI: its driver calls one external subroutine,
+ *     call stepper (tstep, restart)              model/src/godar.f90:193
+ * which reads and writes the COMMON blocks of model/inc/CB_*.h.  This header
+ * is what a Fortran iso_c_binding module binds instead (see
+ * godar_amd/fortran/godar_gpu_mod.f90): state moves across the boundary as
+ * plain arrays, device state persists between calls, and every call returns
+ * an int status (0 = ok).
+ *
+ * Indices are 0-based; the Fortran module converts.  All arrays are host
+ * pointers to n contiguous doubles unless stated otherwise.
+ */
+#ifndef GODAR_GPU_H
+#define GODAR_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. */
+#define GODAR_OK             0
+#define GODAR_ERR_ARG        1   /* bad argument */
+#define GODAR_ERR_HIP        2   /* HIP runtime error */
+#define GODAR_ERR_CAPACITY   3   /* a per-floe list or repair table overflowed; the step was not applied */
+#define GODAR_ERR_NOMEM      4   /* device allocation failed / no room for a member */
+#define GODAR_ERR_STATE      5   /* call out of order (e.g. step before set_floes) */
+
+/* Arithmetic of the force laws. */
+#define GODAR_FP64           0   /* everything FP64 (the reference's precision) */
+#define GODAR_MIXED          1   /* FP32 contact/bond/shelter/drag, FP64 geometry, histories,
+                                    ridging bookkeeping and integration */
+
+/* Pair kinds for godar_get_pairs. */
+#define GODAR_PAIRS_CANDIDATE 0  /* pairs (i<j) processed in the last step */
+#define GODAR_PAIRS_CONTACT   1  /* candidates with deltan > 0 in the last step */
+#define GODAR_PAIRS_BOND      2  /* currently bonded pairs */
+
+/*
+ * Run parameters.  Mirrors the reference's parameter COMMON blocks:
+ * CB_const.h (dt, rtree, ntree, densities, drag coefficients, z0w, lat,
+ * disk moduli), CB_bond.h (bond constants), CB_options.h (switches),
+ * CB_forcings.h (ua, va, uw, vw) and parameter.h (nx, ny).  Defaults are
+ * get_default, model/src/par_get.f90:1-111.  Two workload extensions:
+ * bond_frac/bond_seed (step-1 bonding of a seeded fraction of the touching
+ * pairs; 1.0 = the reference rule, stepper.f90:93-101) and the wind grid.
+ */
+typedef struct godar_params {
+    double dt, rtree, nx, ny;
+    double rhoair, rhoice, rhowater;
+    double Cdair, Cdwater, Csair, Cswater;
+    double z0w, lat;
+    double e_modul, poiss_ratio, friction_coeff, rest_coeff, sigmanc_crit;
+    double eb, lambda_rb, lambda_lb, sigmatb_crit, sigmacb_crit, tau_crit;
+    double bond_lim, dmax, dtd, dth;
+    double ua, va, uw, vw;
+    double bond_frac;
+    int32_t ntree, slipping, cohesion, ridging, shelter, restart;
+    uint64_t bond_seed;
+} godar_params;
+
+typedef struct godar_ctx godar_ctx;
+
+/* Fill *p with the reference defaults (par_get.f90:1-111). */
+int godar_params_default(godar_params *p);
+
+/*
+ * A context owns one HIP stream, device memory for up to max_floes floes in
+ * up to max_members independent simulations ("members"), and the step
+ * pipeline.  precision is GODAR_FP64 or GODAR_MIXED.  One host thread per
+ * context; contexts are independent.
+ */
+int godar_create(int64_t max_floes, int32_t max_members, int32_t precision, godar_ctx **out);
+int godar_destroy(godar_ctx *ctx);
+const char *godar_last_error(const godar_ctx *ctx);
+
+/*
+ * Add a member (one simulation): copies the floe state to the device and
+ * derives mass, inertia and freeboards (floe_properties, physics.f90:179-197)
+ * as ini_get does (ini_get.f90:170-247).  tstep0 is the reference step
+ * number of the member's next step (1 for a fresh run; bonds form in step 1
+ * unless p->restart == 1).  *member receives the member id.
+ */
+int godar_member_add(godar_ctx *ctx, const godar_params *p, int32_t n,
+                     const double *x, const double *y, const double *u, const double *v,
+                     const double *theta, const double *omega, const double *r, const double *h,
+                     int32_t tstep0, int32_t *member);
+int godar_member_remove(godar_ctx *ctx, int32_t member);
+/* Device bytes one member of n floes needs in this context (admission). */
+int godar_member_footprint(const godar_ctx *ctx, int32_t n, int64_t *bytes);
+/* Free floe slots and member slots left. */
+int godar_capacity_left(const godar_ctx *ctx, int64_t *floes, int32_t *members);
+
+/* Replace the uniform forcing (winds_currents, forcing.f90:1-29). */
+int godar_set_forcing(godar_ctx *ctx, int32_t member, double ua, double va, double uw, double vw);
+/* Gridded wind (extension): node (a,b) at (x0 + a*dx, y0 + b*dy), row-major
+ * (nyn, nxn); sampled bilinearly at floe centres every step. */
+int godar_set_wind_grid(godar_ctx *ctx, int32_t member, int32_t nxn, int32_t nyn, double x0,
+                        double y0, double dx, double dy, const double *ua, const double *va);
+/* Load bonds with their state, e.g. from a restart (i<j, 0-based).
+ * state[11*k ..] = bond, damage, deltanb, deltatb, thetarelb(j,i),
+ * thetarelb(i,j), rb, hb, lb, sb, ib (CB_bond.h). */
+int godar_set_bonds(godar_ctx *ctx, int32_t member, int32_t nb, const int32_t *bi,
+                    const int32_t *bj, const double *state);
+
+/*
+ * Advance every active member by nsteps steps: the replacement of
+ * nsteps calls of stepper(tstep, restart) (stepper.f90:1-345).  Returns
+ * GODAR_ERR_CAPACITY if a per-floe list overflowed; the state is then that
+ * of the last complete step (see godar_member_tstep) and the context's
+ * capacities can be raised with godar_reserve before stepping on.
+ */
+int godar_step(godar_ctx *ctx, int32_t nsteps);
+/* Enqueue without waiting; godar_sync waits and reports errors. */
+int godar_step_async(godar_ctx *ctx, int32_t nsteps);
+int godar_sync(godar_ctx *ctx);
+int godar_member_tstep(const godar_ctx *ctx, int32_t member, int32_t *tstep);
+/* Per-floe list capacities (outer candidates, inner candidates, contact
+ * histories, bonds per floe); 0 keeps the current value. */
+int godar_reserve(godar_ctx *ctx, int32_t outer, int32_t inner, int32_t hist, int32_t bonds);
+
+/* Copy a member's floe state back: any pointer may be NULL. */
+int godar_get_floes(godar_ctx *ctx, int32_t member, double *x, double *y, double *u, double *v,
+                    double *theta, double *omega, double *r, double *h, double *mass,
+                    double *inertia, double *hfa, double *hfw);
+/* Per-floe forces of the last step, 33 columns row-major (33, n):
+ * tfx tfy m fcx fcy mc fbx fby mb fax fay ma fwx fwy mw fcorx fcory
+ * fx_bc fy_bc m_bc hsfa_min hsfw_min tp tsigxx tsigyy tsigxy tsigyx
+ * tac pc tab pb ta_bc p_bc  (tsig* are not computed and read 0).
+ * Requires godar_record_forces(ctx, 1) before the step. */
+int godar_record_forces(godar_ctx *ctx, int32_t on);
+int godar_get_forces(godar_ctx *ctx, int32_t member, double *f);
+/* Wall histories (4, n): deltat_bc1 deltat_bc2 theta_bc1 theta_bc2. */
+int godar_get_walls(godar_ctx *ctx, int32_t member, double *w);
+/* Pair lists of a member, canonically sorted by (i, j), 0-based member-local
+ * indices.  Writes up to cap pairs, *count gets the total. */
+int godar_get_pairs(godar_ctx *ctx, int32_t member, int32_t kind, int32_t cap, int32_t *pi,
+                    int32_t *pj, int32_t *count);
+/* Bond state of the member's bonds (same order as GODAR_PAIRS_BOND),
+ * 11 doubles per bond as in godar_set_bonds. */
+int godar_get_bond_state(godar_ctx *ctx, int32_t member, int32_t cap, double *state,
+                         int32_t *count);
+/* Step statistics of the last step: [0] candidate pairs, [1] contacts,
+ * [2] ridging seeds, [3] events re-evaluated by the ordered repair. */
+int godar_get_stats(godar_ctx *ctx, int32_t member, int64_t *stats);
+
+/* Pipeline tuning (autotuner): threads per block of the per-floe kernels. */
+int godar_set_block(godar_ctx *ctx, int32_t threads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

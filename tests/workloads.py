@@ -1,0 +1,39 @@
+"""Small seeded workloads shared by the tests (sizes the oracle finishes in seconds)."""
+from __future__ import annotations
+
+import numpy as np
+
+from godar_amd import generators as g
+from godar_amd.params import GodarParams, workload_params
+
+# Stiffness values used by the workload configurations (stable at dt = 1 s).
+SOFT = dict(e_modul=1e8, eb=1e8, bond_lim=10.0)
+
+
+def config0_params(field, **kw):
+    d = dict(SOFT, bond_seed=12345, bond_frac=0.5)
+    d.update(kw)
+    return workload_params(field, dt=1.0, **d)
+
+
+def packed_box(n=64, seed=3, box=2.0e4, r_lo=500.0, r_hi=1500.0):
+    """A dense walled box: contacts, walls and ridging from the first steps."""
+    return g.make_field(n, seed, radius="uniform", r_lo=r_lo, r_hi=r_hi, h_median=1.0,
+                        h_sigma=0.3, packing=0.6, box=box, wind_speed=15.0, current_speed=0.2,
+                        gap=0.5)
+
+
+def squeeze(field, factor=0.997):
+    """Shrink positions toward the box centre so neighbours overlap slightly."""
+    c = field.box / 2
+    field.x = c + (field.x - c) * factor
+    field.y = c + (field.y - c) * factor
+    return field
+
+
+def kick(field, seed=0, speed=0.3, spin=2e-4):
+    rng = np.random.default_rng(seed)
+    field.u = rng.normal(0, speed, field.n)
+    field.v = rng.normal(0, speed, field.n)
+    field.omega = rng.normal(0, spin, field.n)
+    return field
