@@ -9,6 +9,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libgodar_amd.so")
+PLACE_LIB = os.path.join(HERE, "libgodar_place.so")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("GODAR_ARCH", "gfx950")
@@ -23,7 +24,20 @@ FLAGS = [
 ]
 
 
+def build_place(verbose: bool = False) -> str:
+    """Host-only placement helper of the workload generators (gcc)."""
+    tmp = PLACE_LIB + ".tmp"
+    cmd = ["gcc", "-O2", "-fPIC", "-shared", "-std=c11", "-o", tmp,
+           os.path.join(CSRC, "place.c"), "-lm"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, PLACE_LIB)
+    return PLACE_LIB
+
+
 def build(verbose: bool = False) -> str:
+    build_place(verbose)
     src = os.path.join(CSRC, "godar_lib.hip")
     tmp = LIB + ".tmp"
     cmd = [HIPCC, *FLAGS, "-I", os.path.join(ROOT, "include"), "-I", CSRC, "-o", tmp, src]

@@ -63,6 +63,12 @@ struct godar_ctx {
     int32_t* bcnt[2] = {nullptr, nullptr};
     RadixScratch rs;
     std::string err;
+    // stage timing with HIP events on the context stream (godar_profile)
+    bool prof = false;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_pending;
+    double stage_ms[GODAR_NSTAGES] = {0};
+    int64_t stage_n[GODAR_NSTAGES] = {0};
     // state of the last complete step, for rolling back an aborted step
     int pend_parity = 0;
     std::vector<int32_t> pend_t0;
@@ -217,6 +223,47 @@ bool valid_member(const godar_ctx* c, int32_t m) {
     return c && m >= 0 && m < c->Mcap && c->members[m].active;
 }
 
+hipEvent_t take_event(godar_ctx* c) {
+    if (!c->ev_pool.empty()) {
+        hipEvent_t e = c->ev_pool.back();
+        c->ev_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+struct StageTimer {
+    godar_ctx* c;
+    int stage;
+    hipEvent_t a = nullptr, b = nullptr;
+    StageTimer(godar_ctx* c_, int s_) : c(c_), stage(s_) {
+        if (!c->prof) return;
+        a = take_event(c);
+        b = take_event(c);
+        if (a) (void)hipEventRecord(a, c->stream);
+    }
+    ~StageTimer() {
+        if (!c->prof || !a || !b) return;
+        (void)hipEventRecord(b, c->stream);
+        c->ev_pending.push_back({stage, {a, b}});
+    }
+};
+
+void collect_events(godar_ctx* c) {
+    for (auto& p : c->ev_pending) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, p.second.first, p.second.second) == hipSuccess) {
+            c->stage_ms[p.first] += ms;
+            c->stage_n[p.first] += 1;
+        }
+        c->ev_pool.push_back(p.second.first);
+        c->ev_pool.push_back(p.second.second);
+    }
+    c->ev_pending.clear();
+}
+
 int launch_step(godar_ctx* c) {
     DS& d = c->d;
     sync_ds(c);
@@ -225,25 +272,28 @@ int launch_step(godar_ctx* c) {
     if (N == 0) return GODAR_OK;
     dim3 g((N + B - 1) / B), b(B);
     hipStream_t s = c->stream;
-    hipLaunchKernelGGL(k_bin, g, b, 0, s, d);
+    { StageTimer t(c, 0); hipLaunchKernelGGL(k_bin, g, b, 0, s, d); }
     int bits = 1;
     while ((1u << bits) <= (uint32_t)c->ncells_cap && bits < 32) ++bits;
-    HIPCHK(c, radix_sort_pairs(c->rs, d.ckey, d.cval, N, bits, d.err, s));
-    hipLaunchKernelGGL(k_clear_cells, dim3((c->ncells_cap + 1 + 255) / 256), dim3(256), 0, s, d);
-    hipLaunchKernelGGL(k_ranges, g, b, 0, s, d);
-    hipLaunchKernelGGL(k_cand, g, b, 0, s, d);
-    int nm = c->Mcap;
-    if (c->precision == GODAR_MIXED) {
-        hipLaunchKernelGGL(k_spec<true>, g, b, 0, s, d);
-        hipLaunchKernelGGL(k_repair<true>, dim3(nm), dim3(64), 0, s, d, nm);
-        hipLaunchKernelGGL(k_fixup<true>, g, b, 0, s, d);
-        hipLaunchKernelGGL(k_final<true>, g, b, 0, s, d);
-    } else {
-        hipLaunchKernelGGL(k_spec<false>, g, b, 0, s, d);
-        hipLaunchKernelGGL(k_repair<false>, dim3(nm), dim3(64), 0, s, d, nm);
-        hipLaunchKernelGGL(k_fixup<false>, g, b, 0, s, d);
-        hipLaunchKernelGGL(k_final<false>, g, b, 0, s, d);
+    {
+        StageTimer t(c, 1);
+        HIPCHK(c, radix_sort_pairs(c->rs, d.ckey, d.cval, N, bits, d.err, s));
     }
+    {
+        StageTimer t(c, 2);
+        hipLaunchKernelGGL(k_clear_cells, dim3((c->ncells_cap + 1 + 255) / 256), dim3(256), 0, s, d);
+        hipLaunchKernelGGL(k_ranges, g, b, 0, s, d);
+    }
+    { StageTimer t(c, 3); hipLaunchKernelGGL(k_cand, g, b, 0, s, d); }
+    int nm = c->Mcap;
+    const bool mixed = c->precision == GODAR_MIXED;
+    { StageTimer t(c, 4); hipLaunchKernelGGL(mixed ? k_spec<true> : k_spec<false>, g, b, 0, s, d); }
+    {
+        StageTimer t(c, 5);
+        hipLaunchKernelGGL(mixed ? k_repair<true> : k_repair<false>, dim3(nm), dim3(64), 0, s, d, nm);
+    }
+    { StageTimer t(c, 6); hipLaunchKernelGGL(mixed ? k_fixup<true> : k_fixup<false>, g, b, 0, s, d); }
+    { StageTimer t(c, 7); hipLaunchKernelGGL(mixed ? k_final<true> : k_final<false>, g, b, 0, s, d); }
     HIPCHK(c, hipGetLastError());
     c->parity ^= 1;
     return GODAR_OK;
@@ -384,6 +434,8 @@ int godar_destroy(godar_ctx* c) {
     dfree(d.netab); dfree(d.wtab); dfree(d.nwtab); dfree(d.ovw); dfree(d.dirty);
     dfree(d.affected); dfree(d.stats); dfree(d.err);
     radix_free(c->rs);
+    collect_events(c);
+    for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return GODAR_OK;
@@ -643,6 +695,7 @@ int godar_step_async(godar_ctx* c, int32_t nsteps) {
 int godar_sync(godar_ctx* c) {
     if (!c) return GODAR_ERR_ARG;
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    collect_events(c);
     uint32_t e = 0;
     HIPCHK(c, hipMemcpy(&e, c->d.err, sizeof(uint32_t), hipMemcpyDeviceToHost));
     int rc = download_members(c);
@@ -791,6 +844,30 @@ int godar_get_bond_state(godar_ctx* c, int32_t m, int32_t cap, double* st, int32
         std::memcpy(st + 11 * (int64_t)k, v, sizeof(v));
     }
     return GODAR_OK;
+}
+
+static const char* kStageNames[GODAR_NSTAGES] = {
+    "bin", "sort", "cells", "candidates", "events", "repair", "fixup", "integrate"};
+
+int godar_profile(godar_ctx* c, int32_t on) {
+    if (!c) return GODAR_ERR_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    collect_events(c);
+    c->prof = on != 0;
+    for (int k = 0; k < GODAR_NSTAGES; ++k) { c->stage_ms[k] = 0; c->stage_n[k] = 0; }
+    return GODAR_OK;
+}
+
+int godar_get_profile(godar_ctx* c, double* ms, int64_t* launches) {
+    if (!c || !ms || !launches) return GODAR_ERR_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    collect_events(c);
+    for (int k = 0; k < GODAR_NSTAGES; ++k) { ms[k] = c->stage_ms[k]; launches[k] = c->stage_n[k]; }
+    return GODAR_OK;
+}
+
+const char* godar_stage_name(int32_t k) {
+    return (k >= 0 && k < GODAR_NSTAGES) ? kStageNames[k] : "";
 }
 
 int godar_get_stats(godar_ctx* c, int32_t m, int64_t* stats) {

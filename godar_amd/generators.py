@@ -249,12 +249,50 @@ def _relax_overlaps(x, y, r, box, gap, max_iter=20000):
     return pos[:, 0].copy(), pos[:, 1].copy(), it
 
 
+def _place_lib():
+    import ctypes
+    here = os.path.dirname(os.path.abspath(__file__))
+    path = os.path.join(here, "libgodar_place.so")
+    if not os.path.exists(path):
+        from .build import build_place
+        build_place()
+    L = ctypes.CDLL(path)
+    dp = ctypes.POINTER(ctypes.c_double)
+    L.godar_place_disks.argtypes = [ctypes.c_int, dp, ctypes.c_double, ctypes.c_double,
+                                    ctypes.c_uint64, ctypes.c_int, ctypes.c_int, dp, dp,
+                                    ctypes.POINTER(ctypes.c_int)]
+    L.godar_place_disks.restype = ctypes.c_int
+    return L
+
+
+def place_disks(r, box, seed, gap=1.0, max_tries=400, max_relax=20000):
+    """Non-overlapping centres for radii r in a [0, box]^2 walled box (compiled
+    largest-first random sequential addition + relaxation, csrc/place.c)."""
+    import ctypes
+    L = _place_lib()
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    x = np.zeros(r.size)
+    y = np.zeros(r.size)
+    st = (ctypes.c_int * 2)()
+    dp = ctypes.POINTER(ctypes.c_double)
+    rc = L.godar_place_disks(r.size, r.ctypes.data_as(dp), float(box), float(gap),
+                             int(seed) & MASK64, int(max_tries), int(max_relax),
+                             x.ctypes.data_as(dp), y.ctypes.data_as(dp), st)
+    if rc != 0:
+        raise RuntimeError(f"placement failed (code {rc}); lower the packing")
+    return x, y, dict(rsa_failed=int(st[0]), relax_iters=int(st[1]))
+
+
 def make_field(n: int, seed: int, *, radius: str, r_lo: float, r_hi: float,
                h_median: float, h_sigma: float, packing: float,
                box: Optional[float] = None, power: float = -2.5,
                wind_speed: float = 10.0, current_speed: float = 0.1,
-               gap: float = 1.0) -> FloeField:
-    """Generic seeded field: radii, lognormal thickness, jittered grid, relaxation."""
+               gap: float = 1.0, placement: str = "relax") -> FloeField:
+    """Generic seeded field: radii, lognormal thickness, non-overlapping centres.
+
+    placement="relax": jittered grid + numpy relaxation; "rsa": compiled
+    largest-first random sequential addition (needed for large, dense fields).
+    """
     rng = np.random.default_rng(seed)
     if radius == "uniform":
         r = rng.uniform(r_lo, r_hi, n)
@@ -269,14 +307,20 @@ def make_field(n: int, seed: int, *, radius: str, r_lo: float, r_hi: float,
     area = float(np.sum(np.pi * r ** 2))
     if box is None:
         box = math.sqrt(area / packing)
-    m = int(math.ceil(math.sqrt(n)))
-    spacing = box / m
-    cells = rng.permutation(m * m)[:n]
-    cx = (cells % m + 0.5) * spacing + rng.uniform(-0.25, 0.25, n) * spacing
-    cy = (cells // m + 0.5) * spacing + rng.uniform(-0.25, 0.25, n) * spacing
-    cx = np.clip(cx, r + gap, box - r - gap)
-    cy = np.clip(cy, r + gap, box - r - gap)
-    x, y, iters = _relax_overlaps(cx, cy, r, box, gap)
+    if placement == "relax":
+        # jittered grid + push-apart relaxation (numpy; fine up to ~10^4 floes)
+        m = int(math.ceil(math.sqrt(n)))
+        spacing = box / m
+        cells = rng.permutation(m * m)[:n]
+        cx = (cells % m + 0.5) * spacing + rng.uniform(-0.25, 0.25, n) * spacing
+        cy = (cells // m + 0.5) * spacing + rng.uniform(-0.25, 0.25, n) * spacing
+        cx = np.clip(cx, r + gap, box - r - gap)
+        cy = np.clip(cy, r + gap, box - r - gap)
+        x, y, iters = _relax_overlaps(cx, cy, r, box, gap)
+        pstats = dict(relax_iters=int(iters))
+    else:
+        x, y, pstats = place_disks(r, box, int(rng.integers(0, 2 ** 63)), gap)
+        iters = pstats["relax_iters"]
     ang = rng.uniform(0.0, 2 * np.pi)
     cang = rng.uniform(0.0, 2 * np.pi)
     wind = (wind_speed * math.cos(ang), wind_speed * math.sin(ang))
@@ -285,6 +329,7 @@ def make_field(n: int, seed: int, *, radius: str, r_lo: float, r_hi: float,
     meta = dict(seed=seed, n=n, radius=radius, r_lo=r_lo, r_hi=r_hi,
                 h_median=h_median, h_sigma=h_sigma, packing_target=packing,
                 packing=float(area / box ** 2), box=box, relax_iters=int(iters),
+                placement=placement, placement_stats=pstats,
                 wind_angle=ang, current_angle=cang)
     return FloeField(x, y, r, h, z.copy(), z.copy(), z.copy(), z.copy(), box, seed,
                      wind, current, meta)
@@ -304,7 +349,7 @@ def config0_member(seed: int, n: int) -> FloeField:
     packing of the 1,000-floe / 100 km field."""
     return make_field(n, seed, radius="uniform", r_lo=500.0, r_hi=2000.0,
                       h_median=1.0, h_sigma=0.3, packing=0.55, box=None,
-                      wind_speed=10.0, current_speed=0.1)
+                      wind_speed=10.0, current_speed=0.1, placement="rsa")
 
 
 def config1(seed: int = 1, n: int = 262144, packing: float = 0.7,
@@ -314,7 +359,7 @@ def config1(seed: int = 1, n: int = 262144, packing: float = 0.7,
     (mean 8 m/s, sigma 3 m/s, 50 km correlation)."""
     f = make_field(n, seed, radius="powerlaw", r_lo=200.0, r_hi=5000.0,
                    h_median=1.5, h_sigma=0.3, packing=packing, box=None,
-                   wind_speed=8.0, current_speed=0.1)
+                   wind_speed=8.0, current_speed=0.1, placement="rsa")
     if with_wind_grid:
         f.wind_grid = gaussian_wind_grid(f.box, 8.0, 3.0, 5e4, seed)
         f.meta["wind_grid"] = dict(mean=8.0, sigma=3.0, corr=5e4,

@@ -95,24 +95,14 @@ class GodarParams(ctypes.Structure):
 
 
 def workload_params(field, *, dt: float = 1.0, rtree=None, **kw) -> GodarParams:
-    """Parameters for a generated floe field (box, uniform forcing, rtree >= max r).
-
-    Stiffness values are chosen for stability of the explicit integrator at
-    the workload's dt; see ``STABLE_OVERRIDES``.
-    """
+    """Parameters for a generated floe field: its box, its uniform forcing and
+    rtree = ceil(max r) (the reference's generator advice,
+    tools/init/gridded_placement.py:273-275); other values from ``kw``."""
     if rtree is None:
         rtree = float(math.ceil(float(np.max(field.r))))
     p = GodarParams(dt=dt, rtree=rtree, nx=field.box, ny=field.box,
                     ua=field.wind[0], va=field.wind[1],
                     uw=field.current[0], vw=field.current[1])
-    for k, v in STABLE_OVERRIDES.items():
-        setattr(p, k, v)
     for k, v in kw.items():
         setattr(p, k, v)
     return p
-
-
-# Namelist overrides used by every workload (the reference defaults give an
-# explicitly unstable contact/bond oscillator at dt = 1 s).  Filled in from
-# the stability study of the oracle.
-STABLE_OVERRIDES: dict = {}

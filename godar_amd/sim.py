@@ -139,6 +139,18 @@ class Context:
     def set_block(self, threads: int):
         self._chk(self.L.godar_set_block(self.h, int(threads)))
 
+    def profile(self, on: bool = True):
+        self._chk(self.L.godar_profile(self.h, 1 if on else 0))
+
+    def stage_times(self) -> dict:
+        """{stage: (total ms, timed launches)} since the last profile(True)."""
+        ms = np.zeros(_lib.NSTAGES)
+        n = np.zeros(_lib.NSTAGES, dtype=np.int64)
+        self._chk(self.L.godar_get_profile(self.h, _ptr(ms),
+                                           n.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))))
+        return {self.L.godar_stage_name(k).decode(): (float(ms[k]), int(n[k]))
+                for k in range(_lib.NSTAGES)}
+
     # ------------------------------------------------------------ stepping
     def step(self, nsteps: int = 1):
         self._chk(self.L.godar_step(self.h, int(nsteps)))

@@ -1,0 +1,69 @@
+"""The benchmark workloads (configs 0-4) as (field(s), parameters).
+
+Namelist values shared by every workload: the reference's defaults
+(par_get.f90:1-111) except e_modul = eb = 1e8 Pa (the defaults, 6e9 Pa, make
+the explicit contact/bond oscillator unstable at dt = 1 s: bonds break within
+ten steps) and bond_lim = 10 m (the generators leave >= 1 m clearance between
+floes, so "touching" means a gap of at most 10 m).
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from . import generators as g
+from .params import GodarParams, workload_params
+
+SOFT = dict(e_modul=1e8, eb=1e8, bond_lim=10.0)
+
+
+def params_for(field, **kw) -> GodarParams:
+    d = dict(SOFT, bond_seed=12345, bond_frac=0.5, dt=1.0)
+    d.update(kw)
+    return workload_params(field, **d)
+
+
+def config0(seed: int = 0):
+    """1,000 floes, 100 km box, r ~ U[500,2000], 50% of touching pairs bonded,
+    wind 10 m/s, current 0.1 m/s, dt = 1 s (1,000 steps in the reference run)."""
+    f = g.config0(seed)
+    return f, params_for(f)
+
+
+def config1(seed: int = 1, n: int = 262144):
+    """262,144 floes, power-law radii on [200, 5000] m, packing 0.7 (754 km box),
+    30% bonded, Gaussian wind field, dt = 0.5 s (2,000 steps)."""
+    f = g.config1(seed, n=n)
+    return f, params_for(f, dt=0.5, bond_frac=0.3, bond_seed=seed * 7919 + 1)
+
+
+def config3(seed: int = 3, members: int = 512, lo: int = 500, hi: int = 8000):
+    """512 independent fields, floe counts log-uniform in [500, 8000], each the
+    config-0 generator with its own seed and wind direction (box scaled to keep
+    the packing); mixed precision, 5,000 steps."""
+    sizes = g.member_sizes(members, lo, hi, seed)
+    out = []
+    for k, n in enumerate(sizes):
+        f = g.config0_member(seed * 100003 + k, int(n))
+        out.append((f, params_for(f, bond_seed=seed * 100003 + k)))
+    return out
+
+
+def config4_request(rng: np.random.Generator, seed: int):
+    """One served request: 1,000-50,000 floes (log-uniform) from the config-1
+    generator (power-law radii, packing 0.7), 500-5,000 steps, dt = 1 s."""
+    n = int(math.floor(math.exp(rng.uniform(math.log(1e3), math.log(5e4)))))
+    steps = int(rng.integers(500, 5001))
+    f = g.make_field(n, seed, radius="powerlaw", r_lo=200.0, r_hi=5000.0, h_median=1.5,
+                     h_sigma=0.3, packing=0.7, box=None, wind_speed=8.0, current_speed=0.1,
+                     placement="rsa")
+    return f, params_for(f, bond_frac=0.3, bond_seed=seed), steps
+
+
+def algorithmic_bytes(n_floes: int, ncand: int, ncont: int, nbond: int) -> float:
+    """Bytes per floe-step a step must move at minimum (FP64):
+    144 per floe (read 12 state words, write 6) + 8 per candidate pair
+    + 32 per contact (history read + write) + 120 per bond (5 evolving words
+    read + write, 5 constants read)."""
+    return 144.0 * n_floes + 8.0 * ncand + 32.0 * ncont + 120.0 * nbond

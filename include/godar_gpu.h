@@ -151,6 +151,18 @@ int godar_get_stats(godar_ctx *ctx, int32_t member, int64_t *stats);
 /* Pipeline tuning (autotuner): threads per block of the per-floe kernels. */
 int godar_set_block(godar_ctx *ctx, int32_t threads);
 
+/*
+ * Stage timing (tooling; no counterpart in the reference, which only times
+ * the whole loop with omp_get_wtime, godar.f90:185-219).  When on, every
+ * step records HIP events around its stages on the context's stream;
+ * godar_get_profile returns the summed milliseconds and the number of timed
+ * launches per stage.  godar_profile resets the totals.
+ */
+#define GODAR_NSTAGES 8
+int godar_profile(godar_ctx *ctx, int32_t on);
+int godar_get_profile(godar_ctx *ctx, double *ms, int64_t *launches);
+const char *godar_stage_name(int32_t stage);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,112 @@
+"""Write the golden fixtures under tests/golden/ (run on a CPU, committed).
+
+Fixtures are produced by GODAR's own routines compiled from the reference
+(oracle/_ref, built by oracle/Makefile) where that library exists, otherwise
+by the C oracle (which the CPU tests pin bit-for-bit to it).  Each fixture
+holds the inputs (initial floe field, parameters) and the expected outputs:
+
+* config0_seed0.npz -- the 1,000-floe field (100 km box) with the workload
+  parameters; state after 1, 100 and 1,000 steps and the candidate/contact
+  pair sets of step 1, 100 and 1,000.
+* dense64.npz -- 64 floes squeezed in a 20 km box with heavy ridging; state
+  after 1, 10 and 50 steps and the pair sets of every step.
+* config0_1e4.npz -- the same 1,000-floe field after 10,000 steps (the
+  mixed-precision error budget is stated against this).
+
+Usage:  python tests/golden/make_golden.py [--long]
+"""
+from __future__ import annotations
+
+import argparse
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+from godar_amd import generators as g  # noqa: E402
+from workloads import config0_params, kick, packed_box, squeeze  # noqa: E402
+
+STATE = ["x", "y", "u", "v", "omega", "theta", "r", "h"]
+
+
+def params_vector(p):
+    return np.array([getattr(p, k) for k, _ in p._fields_], dtype=object)
+
+
+def field_arrays(f):
+    return dict(x0=f.x, y0=f.y, r0=f.r, h0=f.h, u0=f.u, v0=f.v, omega0=f.omega, theta0=f.theta,
+                box=np.float64(f.box))
+
+
+def run(field, params, checkpoints, engine="ref", cfg=None, pair_steps=None):
+    out = {}
+    pair_steps = set(checkpoints) if pair_steps is None else set(pair_steps)
+    if engine == "ref":
+        import ref_harness as rh
+        H = rh.RefHarness(cfg)
+        H.setup(params, field, bondmask=rh.dense_bondmask(field.n, params.bond_seed, params.bond_frac))
+        step = lambda t: H.step(t)
+        state = H.state
+        log = H.log
+    else:
+        from oracle_py import OracleSim
+        O = OracleSim(params, field)
+        step = lambda t: O.step()
+        state = O.state
+        def log():
+            li, lj, lf = O.log()
+            return li, lj, (lf & 1).astype(bool)
+    for t in range(1, max(checkpoints | pair_steps) + 1):
+        step(t)
+        if t in checkpoints:
+            st = state()
+            for k in STATE:
+                out[f"{k}_{t}"] = st[k].copy()
+        if t in pair_steps:
+            li, lj, lc = log()
+            out[f"cand_{t}"] = np.stack([li, lj]).astype(np.int32)
+            out[f"cont_{t}"] = np.stack([li[lc], lj[lc]]).astype(np.int32)
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--long", action="store_true", help="also the 10,000-step fixture")
+    a = ap.parse_args()
+    import ref_harness as rh
+    eng = "ref" if rh.available("n1000") and rh.available("n64") else "oracle"
+    print("engine:", eng)
+    f0 = g.config0(0)
+    p0 = config0_params(f0)
+    t = time.time()
+    out = run(f0, p0, {1, 100, 1000}, eng, "n1000")
+    np.savez_compressed(os.path.join(HERE, "config0_seed0.npz"), engine=eng,
+                        pnames=np.array([k for k, _ in p0._fields_]),
+                        pvalues=np.array([float(getattr(p0, k)) for k, _ in p0._fields_]),
+                        **field_arrays(f0), **out)
+    print("config0_seed0.npz", time.time() - t)
+    fb = kick(squeeze(packed_box(64, 3)), 1)
+    pb = config0_params(fb, sigmanc_crit=1e5)
+    out = run(fb, pb, {1, 10, 50}, eng, "n64", pair_steps=range(1, 51))
+    np.savez_compressed(os.path.join(HERE, "dense64.npz"), engine=eng,
+                        pnames=np.array([k for k, _ in pb._fields_]),
+                        pvalues=np.array([float(getattr(pb, k)) for k, _ in pb._fields_]),
+                        **field_arrays(fb), **out)
+    print("dense64.npz")
+    if a.long:
+        t = time.time()
+        out = run(f0, p0, {10000}, eng, "n1000")
+        keep = {k: v for k, v in out.items() if k.startswith(("x_", "y_", "u_", "v_", "r_", "h_"))}
+        np.savez_compressed(os.path.join(HERE, "config0_1e4.npz"), engine=eng, **keep)
+        print("config0_1e4.npz", time.time() - t)
+
+
+if __name__ == "__main__":
+    main()

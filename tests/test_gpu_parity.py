@@ -1,0 +1,301 @@
+"""HIP path vs the oracle, through the C ABI (libgodar_amd.so).
+
+Bars:
+  * candidate and contact pair sets: identical to the oracle's, every step;
+  * FP64 state: identical inputs give identical results except where device
+    exp/log (sheltering, log profile, ridging acos) round differently from
+    glibc by one ulp; the written tolerance below bounds the drift
+    (relative 1e-8 after hundreds of steps, observed ~1e-10);
+  * batch invariance: a member's trajectory does not depend on its batch
+    mates (bitwise), in FP64 and in mixed precision.
+"""
+import numpy as np
+import pytest
+
+from fixtures import load, load_long, pairs_at
+from godar_amd import generators as g
+from workloads import config0_params, kick, packed_box, squeeze
+
+pytestmark = pytest.mark.gpu
+
+STATE = ["x", "y", "u", "v", "omega", "r", "h"]
+FP64_REL_TOL = 1e-8
+
+
+def _ctx(n, members=1, precision="fp64", **kw):
+    from godar_amd.sim import Context
+    return Context(n, members, precision, record_forces=True, **kw)
+
+
+def _rel(a, b):
+    scale = np.maximum(np.abs(b), 1e-300)
+    return float(np.max(np.abs(a - b) / scale))
+
+
+def _pairs(ctx, m, kind):
+    i, j = ctx.pairs(m, kind)
+    return list(zip(i.tolist(), j.tolist()))
+
+
+def _oracle_pairs(O):
+    li, lj, lf = O.log()
+    cand = sorted(zip(li.tolist(), lj.tolist()))
+    c = (lf & 1).astype(bool)
+    cont = sorted(zip(li[c].tolist(), lj[c].tolist()))
+    return cand, cont
+
+
+def _lockstep(field, params, nsteps, precision="fp64", check_every=1):
+    from oracle_py import OracleSim
+    ctx = _ctx(field.n, 1, precision)
+    m = ctx.add_field(params, field)
+    O = OracleSim(params, field)
+    worst = 0.0
+    seeds = repaired = 0
+    for t in range(1, nsteps + 1):
+        ctx.step(1)
+        O.step()
+        st = ctx.stats(m)
+        seeds += st["seeds"]
+        repaired += st["repaired"]
+        if t % check_every and t != nsteps:
+            continue
+        cand, cont = _oracle_pairs(O)
+        assert _pairs(ctx, m, "candidate") == cand, f"candidate set differs at step {t}"
+        assert _pairs(ctx, m, "contact") == cont, f"contact set differs at step {t}"
+        a, b = ctx.floes(m), O.state()
+        for k in STATE:
+            worst = max(worst, _rel(a[k], b[k]) if k in ("x", "y", "r", "h") else
+                        float(np.max(np.abs(a[k] - b[k]))) / (float(np.max(np.abs(b[k]))) + 1e-300))
+    ctx.close()
+    return worst, seeds, repaired
+
+
+def test_one_step_identical_inputs():
+    """Step 1 from identical inputs: pair sets exact, state equal to the last bits."""
+    from oracle_py import OracleSim
+    for field, params in ((g.config0(0), None), (kick(squeeze(packed_box(64, 3)), 1), "dense")):
+        p = config0_params(field) if params is None else config0_params(field, sigmanc_crit=1e5)
+        ctx = _ctx(field.n)
+        m = ctx.add_field(p, field)
+        O = OracleSim(p, field)
+        ctx.step(1)
+        O.step()
+        cand, cont = _oracle_pairs(O)
+        assert _pairs(ctx, m, "candidate") == cand
+        assert _pairs(ctx, m, "contact") == cont
+        a, b = ctx.floes(m), O.state()
+        fa, fb = ctx.forces(m), O.forces()
+        for k in STATE:
+            assert _rel(a[k], b[k]) <= 1e-13, k
+        for k in ("fcx", "fcy", "mc", "fbx", "fby", "mb", "fx_bc", "fy_bc", "m_bc", "tac", "pc"):
+            scale = float(np.max(np.abs(fb[k]))) + 1e-300
+            assert float(np.max(np.abs(fa[k] - fb[k]))) / scale <= 1e-13, k
+        ctx.close()
+
+
+def test_config0_trajectory_fp64():
+    f, p, d = load("config0_seed0.npz")
+    worst, _, _ = _lockstep(f, p, 300, check_every=20)
+    assert worst < FP64_REL_TOL
+
+
+def test_config0_golden_fp64():
+    """1,000 steps against the reference-made fixture."""
+    f, p, d = load("config0_seed0.npz")
+    ctx = _ctx(f.n)
+    m = ctx.add_field(p, f)
+    for t in (1, 100, 1000):
+        ctx.step(t - ctx.tstep(m) + 1)
+        if t in (1, 100, 1000):
+            assert _pairs(ctx, m, "candidate") == pairs_at(d, "cand", t)
+            assert _pairs(ctx, m, "contact") == pairs_at(d, "cont", t)
+            a = ctx.floes(m)
+            for k in ("x", "y", "r", "h"):
+                assert _rel(a[k], d[f"{k}_{t}"]) < FP64_REL_TOL, (k, t)
+    ctx.close()
+
+
+def test_dense_ridging_repair_fp64():
+    """Heavy in-loop ridging: the ordered repair must reproduce the sequential semantics."""
+    f, p, d = load("dense64.npz")
+    ctx = _ctx(f.n)
+    m = ctx.add_field(p, f)
+    seeds = 0
+    for t in range(1, 51):
+        ctx.step(1)
+        seeds += ctx.stats(m)["seeds"]
+        assert _pairs(ctx, m, "candidate") == pairs_at(d, "cand", t), t
+        assert _pairs(ctx, m, "contact") == pairs_at(d, "cont", t), t
+        if t in (1, 10, 50):
+            a = ctx.floes(m)
+            for k in ("x", "y", "r", "h"):
+                assert _rel(a[k], d[f"{k}_{t}"]) < FP64_REL_TOL, (k, t)
+    assert seeds > 300
+    ctx.close()
+
+
+def test_stiff_bond_breaking_fp64():
+    f = kick(squeeze(packed_box(64, 5), 0.995), 2, speed=0.5)
+    p = config0_params(f, sigmanc_crit=1e5, e_modul=6e9, eb=6e9, bond_lim=50.0, bond_frac=1.0)
+    worst, seeds, _ = _lockstep(f, p, 150, check_every=5)
+    assert worst < FP64_REL_TOL and seeds > 0
+
+
+def test_switches_off_fp64():
+    f = kick(squeeze(packed_box(64, 4)), 3)
+    p = config0_params(f, shelter=0, ridging=0, slipping=0, cohesion=0)
+    worst, seeds, _ = _lockstep(f, p, 120, check_every=10)
+    assert worst < FP64_REL_TOL and seeds == 0
+
+
+def test_wind_grid_fp64():
+    f = kick(squeeze(packed_box(64, 6)), 4)
+    f.wind_grid = g.gaussian_wind_grid(f.box, 8.0, 3.0, 5e3, seed=9, spacing=1e3)
+    p = config0_params(f, sigmanc_crit=1e5)
+    worst, _, _ = _lockstep(f, p, 120, check_every=10)
+    assert worst < FP64_REL_TOL
+
+
+def test_bonds_and_bond_state_match_oracle():
+    from oracle_py import OracleSim
+    f, p, _ = load("config0_seed0.npz")
+    ctx = _ctx(f.n)
+    m = ctx.add_field(p, f)
+    O = OracleSim(p, f)
+    ctx.step(50)
+    O.step(50)
+    bi, bj = ctx.pairs(m, "bond")
+    oi, oj = O.bonds()
+    assert np.array_equal(bi, oi) and np.array_equal(bj, oj) and bi.size > 200
+    st = ctx.bond_state(m)
+    ost = O.pairs(oi, oj)
+    for k, ko in (("damage", "damage"), ("deltanb", "deltanb"), ("deltatb", "deltatb"),
+                  ("rb", "rb"), ("hb", "hb"), ("lb", "lb"), ("sb", "sb"), ("ib", "ib")):
+        scale = float(np.max(np.abs(ost[ko]))) + 1e-300
+        assert float(np.max(np.abs(st[k] - ost[ko]))) / scale < 1e-9, k
+    ctx.close()
+
+
+def test_batch_invariance_bitwise():
+    """Three fields of different sizes in one context == each alone, bitwise."""
+    fields = [g.config0_member(11, 600), kick(squeeze(packed_box(64, 3)), 1), g.config0_member(12, 900)]
+    params = [config0_params(fields[0]), config0_params(fields[1], sigmanc_crit=1e5),
+              config0_params(fields[2])]
+    for prec in ("fp64", "mixed"):
+        tot = sum(f.n for f in fields)
+        ctx = _ctx(tot, 3, prec)
+        ms = [ctx.add_field(p, f) for f, p in zip(fields, params)]
+        ctx.step(60)
+        for f, p, m in zip(fields, params, ms):
+            alone = _ctx(f.n, 1, prec)
+            ma = alone.add_field(p, f)
+            alone.step(60)
+            a, b = ctx.floes(m), alone.floes(ma)
+            for k in STATE:
+                assert np.array_equal(a[k], b[k]), (prec, k)
+            assert _pairs(ctx, m, "contact") == _pairs(alone, ma, "contact")
+            alone.close()
+        ctx.close()
+
+
+def test_capacity_overflow_rolls_back():
+    """A too-small candidate list aborts the step without advancing the state;
+    after reserving more room the run continues exactly as an unhindered one."""
+    from godar_amd._lib import GodarError
+    f = kick(squeeze(packed_box(64, 3)), 1)
+    p = config0_params(f, sigmanc_crit=1e5, rtree=6000.0)
+    ref = _ctx(f.n)
+    mr = ref.add_field(p, f)
+    ref.step(20)
+    ctx = _ctx(f.n, outer=4, inner=4)
+    m = ctx.add_field(p, f)
+    with pytest.raises(GodarError):
+        ctx.step(20)
+    assert ctx.tstep(m) == 1
+    x0 = ctx.floes(m)["x"]
+    assert np.array_equal(x0, f.x)
+    ctx.reserve(outer=64, inner=64)
+    ctx.step(20)
+    a, b = ctx.floes(m), ref.floes(mr)
+    for k in STATE:
+        assert np.array_equal(a[k], b[k]), k
+    ctx.close()
+    ref.close()
+
+
+def test_restart_with_loaded_bonds():
+    """Bonds handed over through godar_set_bonds (restart) continue exactly."""
+    f, p, _ = load("config0_seed0.npz")
+    ctx = _ctx(f.n)
+    m = ctx.add_field(p, f)
+    ctx.step(30)
+    st = ctx.floes(m)
+    bi, bj = ctx.pairs(m, "bond")
+    bs = ctx.bond_state(m)
+    cols = np.stack([bs[k] for k in ("bond", "damage", "deltanb", "deltatb", "thetarelb_ji",
+                                      "thetarelb_ij", "rb", "hb", "lb", "sb", "ib")], axis=1)
+    ctx.step(1)
+    a = ctx.floes(m)
+    # fresh member at step 31 with the same floes and bonds (contact histories
+    # are not part of the reference's restart files; pick a contact-free state)
+    f2 = g.FloeField(st["x"], st["y"], st["r"], st["h"], st["theta"], st["omega"], st["u"], st["v"],
+                     f.box, 0, f.wind, f.current)
+    p2 = p.copy(restart=1)
+    if _pairs(ctx, m, "contact"):
+        pytest.skip("state has contacts; restart does not carry contact history")
+    c2 = _ctx(f.n)
+    m2 = c2.add_field(p2, f2, tstep0=31)
+    c2.set_bonds(m2, bi, bj, cols)
+    c2.step(1)
+    b = c2.floes(m2)
+    for k in STATE:
+        assert np.array_equal(a[k], b[k]), k
+    ctx.close()
+    c2.close()
+
+
+def test_config1_scale_pair_sets_exact():
+    """262,144 floes (config 1 generator): one step from identical inputs gives the
+    oracle's candidate and contact sets, and ridging conserves every r^2 h."""
+    from oracle_py import OracleSim
+    f = g.config1(1, n=262144)
+    p = config0_params(f, dt=0.5, bond_frac=0.3, bond_seed=77)
+    ctx = _ctx(f.n, outer=128, inner=128)
+    m = ctx.add_field(p, f)
+    O = OracleSim(p, f)
+    ctx.step(1)
+    O.step()
+    cand, cont = _oracle_pairs(O)
+    assert _pairs(ctx, m, "candidate") == cand
+    assert _pairs(ctx, m, "contact") == cont
+    a, b = ctx.floes(m), O.state()
+    for k in ("x", "y", "r", "h"):
+        assert _rel(a[k], b[k]) < 1e-12, k
+    v0 = f.r ** 2 * f.h
+    ctx.step(20)
+    a = ctx.floes(m)
+    assert np.max(np.abs(a["r"] ** 2 * a["h"] / v0 - 1)) < 1e-12
+    ctx.close()
+
+
+def test_long_run_error_budget():
+    """10,000 steps of the 1,000-floe field against the reference-made fixture:
+    FP64 and mixed-precision relative position error (stated bar 1e-6)."""
+    f, p, _ = load("config0_seed0.npz")
+    L = load_long()
+    errs = {}
+    for prec in ("fp64", "mixed"):
+        ctx = _ctx(f.n, 1, prec)
+        m = ctx.add_field(p, f)
+        ctx.step(10000)
+        a = ctx.floes(m)
+        xy = np.concatenate([a["x"], a["y"]])
+        ref = np.concatenate([L["x_10000"], L["y_10000"]])
+        errs[prec] = float(np.max(np.abs(xy - ref) / np.abs(ref)))
+        vol = np.sum(a["r"] ** 2 * a["h"])
+        assert abs(vol / np.sum(L["r_10000"] ** 2 * L["h_10000"]) - 1) < 1e-9
+        ctx.close()
+    print("relative position error after 1e4 steps:", errs)
+    assert errs["fp64"] < 1e-6
+    assert errs["mixed"] < 1e-6

@@ -4,16 +4,9 @@ from __future__ import annotations
 import numpy as np
 
 from godar_amd import generators as g
-from godar_amd.params import GodarParams, workload_params
+from godar_amd.workloads import SOFT, params_for  # noqa: F401
 
-# Stiffness values used by the workload configurations (stable at dt = 1 s).
-SOFT = dict(e_modul=1e8, eb=1e8, bond_lim=10.0)
-
-
-def config0_params(field, **kw):
-    d = dict(SOFT, bond_seed=12345, bond_frac=0.5)
-    d.update(kw)
-    return workload_params(field, dt=1.0, **d)
+config0_params = params_for
 
 
 def packed_box(n=64, seed=3, box=2.0e4, r_lo=500.0, r_hi=1500.0):
