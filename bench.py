@@ -99,6 +99,10 @@ def run_plain(which: str, steps: int, warmup: int):
 
 
 def pos_error_1e4(precision="mixed"):
+    """Relative position error after 10^4 steps of the 1,000-floe field against
+    the fixture made by GODAR's own routines: (max, median, envelope max,
+    envelope median), the envelope being the reference's own deviation when
+    every initial centre moves by one ulp."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from fixtures import load, load_long
     from godar_amd.sim import Context
@@ -111,7 +115,11 @@ def pos_error_1e4(precision="mixed"):
     ctx.close()
     xy = np.concatenate([a["x"], a["y"]])
     ref = np.concatenate([L["x_10000"], L["y_10000"]])
-    return float(np.max(np.abs(xy - ref) / np.abs(ref)))
+    rel = np.abs(xy - ref) / np.abs(ref)
+    env_max = max(float(L["ulp_env_max_10000_0"]), float(L["ulp_env_max_10000_1"]))
+    env_med = max(float(L["ulp_env_median_10000_0"]), float(L["ulp_env_median_10000_1"]))
+    return dict(max=float(rel.max()), median=float(np.median(rel)),
+                ulp_envelope_max=env_max, ulp_envelope_median=env_med)
 
 
 def cpu_baseline(target_seconds: float):
@@ -221,7 +229,7 @@ def main():
                              "config1_floe_steps_s": p1, "config1_ms_per_step": ms1}
         out["batched_vs_plain_config0"] = value / p0
         out["batched_vs_plain_config1"] = value / p1
-        out["pos_err_1e4_mixed"] = pos_error_1e4("mixed")
+        out["pos_err_1e4"] = {"mixed": pos_error_1e4("mixed"), "fp64": pos_error_1e4("fp64")}
         cb = cpu_baseline(args.cpu_seconds)
         out["cpu_baseline"] = cb
         out["vs_cpu_baseline"] = value / cb["value"]

@@ -73,6 +73,8 @@ struct godar_ctx {
     int pend_parity = 0;
     std::vector<int32_t> pend_t0;
     bool pending = false;
+    uint32_t last_err_bits = 0;
+    bool auto_grow = true;
 };
 
 namespace {
@@ -717,16 +719,57 @@ int godar_sync(godar_ctx* c) {
         HIPCHK(c, hipMemset(c->d.dirty, 0, c->S));
         HIPCHK(c, hipMemset(c->d.affected, 0, c->S));
         HIPCHK(c, hipMemset(c->d.err, 0, sizeof(uint32_t)));
+        c->last_err_bits = e;
         return fail(c, GODAR_ERR_CAPACITY, "step aborted:" + err_text(e));
     }
     return GODAR_OK;
 }
 
+// Raise the capacities an aborted step ran out of (lists double, repair
+// pools grow by half).  Returns false when a limit is reached.
+static bool grow_for(godar_ctx* c, uint32_t bits) {
+    int nCO = c->CO, nCI = c->CI, nHC = c->HC, nBC = c->BC;
+    if (bits & ERR_OUTER_CAP) nCO = std::min(2 * c->CO, 0xFFFE);
+    if (bits & ERR_INNER_CAP) nCI = std::min(2 * c->CI, 1 << 15);
+    if (bits & ERR_HIST_CAP) nHC = std::min(2 * c->HC, 1 << 12);
+    if (bits & ERR_BOND_CAP) nBC = std::min(2 * c->BC, 1 << 12);
+    bool grew = nCO != c->CO || nCI != c->CI || nHC != c->HC || nBC != c->BC;
+    if (grew && godar_reserve(c, nCO, nCI, nHC, nBC) != GODAR_OK) return false;
+    if (bits & (ERR_REPAIR_CAP | ERR_SEED_CAP)) {
+        if (c->HEAPK >= 1024) return false;
+        if (hipStreamSynchronize(c->stream) != hipSuccess) return false;
+        c->SEEDK *= 2; c->HEAPK *= 2; c->TABK *= 2;
+        dfree(c->d.seeds); dfree(c->d.heap); dfree(c->d.etab);
+        if (dalloc(&c->d.seeds, (int64_t)c->SEEDK * c->S) != hipSuccess) return false;
+        if (dalloc(&c->d.heap, (int64_t)c->HEAPK * c->S) != hipSuccess) return false;
+        if (dalloc(&c->d.etab, (int64_t)c->TABK * c->S) != hipSuccess) return false;
+        grew = true;
+    }
+    return grew;
+}
+
 int godar_step(godar_ctx* c, int32_t nsteps) {
     if (!c || nsteps < 0) return GODAR_ERR_ARG;
-    int rc = godar_step_async(c, nsteps);
+    int rc = download_members(c);
     if (rc != GODAR_OK) return rc;
-    return godar_sync(c);
+    int32_t ref = -1;
+    for (int k = 0; k < c->Mcap; ++k)
+        if (c->members[k].active) { ref = k; break; }
+    int32_t t_start = ref >= 0 ? c->members[ref].tstep : 0;
+    int32_t left = nsteps;
+    for (int attempt = 0; attempt < 64; ++attempt) {
+        rc = godar_step_async(c, left);
+        if (rc != GODAR_OK) return rc;
+        rc = godar_sync(c);
+        if (rc != GODAR_ERR_CAPACITY || !c->auto_grow || ref < 0) return rc;
+        // some steps completed before the overflow: only the rest is retried
+        left = nsteps - (c->members[ref].tstep - t_start);
+        std::string why = c->err;
+        if (!grow_for(c, c->last_err_bits))
+            return fail(c, GODAR_ERR_CAPACITY, why + " (capacity limit reached)");
+        if (left <= 0) return GODAR_OK;
+    }
+    return fail(c, GODAR_ERR_CAPACITY, "capacity growth did not converge");
 }
 
 int godar_member_tstep(const godar_ctx* c, int32_t m, int32_t* t) {

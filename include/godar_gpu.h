@@ -109,13 +109,18 @@ int godar_set_bonds(godar_ctx *ctx, int32_t member, int32_t nb, const int32_t *b
 
 /*
  * Advance every active member by nsteps steps: the replacement of
- * nsteps calls of stepper(tstep, restart) (stepper.f90:1-345).  Returns
- * GODAR_ERR_CAPACITY if a per-floe list overflowed; the state is then that
- * of the last complete step (see godar_member_tstep) and the context's
- * capacities can be raised with godar_reserve before stepping on.
+ * nsteps calls of stepper(tstep, restart) (stepper.f90:1-345).  A step that
+ * overflows a per-floe list or the ridging-repair pools is rolled back; the
+ * capacities that ran out are raised and the remaining steps run again, so
+ * the result equals an uninterrupted run.  GODAR_ERR_CAPACITY is returned
+ * only when a hard limit is reached (the state is then that of the last
+ * complete step, see godar_member_tstep).
+ *
+ * godar_step_async only enqueues (no capacity growth); godar_sync waits and
+ * reports GODAR_ERR_CAPACITY after rolling back to the last complete step,
+ * after which godar_reserve can raise the capacities.
  */
 int godar_step(godar_ctx *ctx, int32_t nsteps);
-/* Enqueue without waiting; godar_sync waits and reports errors. */
 int godar_step_async(godar_ctx *ctx, int32_t nsteps);
 int godar_sync(godar_ctx *ctx);
 int godar_member_tstep(const godar_ctx *ctx, int32_t member, int32_t *tstep);

@@ -10,8 +10,11 @@ holds the inputs (initial floe field, parameters) and the expected outputs:
   pair sets of step 1, 100 and 1,000.
 * dense64.npz -- 64 floes squeezed in a 20 km box with heavy ridging; state
   after 1, 10 and 50 steps and the pair sets of every step.
-* config0_1e4.npz -- the same 1,000-floe field after 10,000 steps (the
-  mixed-precision error budget is stated against this).
+* config0_1e4.npz -- the same 1,000-floe field after 2,000 and 10,000 steps,
+  plus the reference's sensitivity envelope: the max and median relative
+  position deviation of runs whose initial centres were moved by one ulp
+  (the dynamics are chaotic; any result that is not bit-identical is judged
+  against this envelope).
 
 Usage:  python tests/golden/make_golden.py [--long]
 """
@@ -102,10 +105,25 @@ def main():
     print("dense64.npz")
     if a.long:
         t = time.time()
-        out = run(f0, p0, {10000}, eng, "n1000")
+        out = run(f0, p0, {2000, 10000}, eng, "n1000")
         keep = {k: v for k, v in out.items() if k.startswith(("x_", "y_", "u_", "v_", "r_", "h_"))}
-        np.savez_compressed(os.path.join(HERE, "config0_1e4.npz"), engine=eng, **keep)
-        print("config0_1e4.npz", time.time() - t)
+        # sensitivity envelope: the same run with every floe centre moved by
+        # one ulp (two seeds); max and median relative position deviation
+        env = {}
+        for k, seed in enumerate((1, 2)):
+            fp = g.config0(0)
+            rng = np.random.default_rng(seed)
+            fp.x = np.where(rng.random(fp.n) < 0.5, np.nextafter(fp.x, np.inf), np.nextafter(fp.x, -np.inf))
+            fp.y = np.where(rng.random(fp.n) < 0.5, np.nextafter(fp.y, np.inf), np.nextafter(fp.y, -np.inf))
+            o = run(fp, p0, {2000, 10000}, "oracle")
+            for T in (2000, 10000):
+                xy = np.concatenate([o[f"x_{T}"], o[f"y_{T}"]])
+                ref = np.concatenate([out[f"x_{T}"], out[f"y_{T}"]])
+                rel = np.abs(xy - ref) / np.abs(ref)
+                env[f"ulp_env_max_{T}_{k}"] = np.float64(rel.max())
+                env[f"ulp_env_median_{T}_{k}"] = np.float64(np.median(rel))
+        np.savez_compressed(os.path.join(HERE, "config0_1e4.npz"), engine=eng, **keep, **env)
+        print("config0_1e4.npz", time.time() - t, {k: float(v) for k, v in env.items()})
 
 
 if __name__ == "__main__":
