@@ -165,7 +165,19 @@ def main():
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--quick", action="store_true", help="batched run only (for profilers)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 1 GiB device copy (calibrates rocprof byte counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        a = torch.ones(1 << 27, dtype=torch.float64, device="cuda")   # 1 GiB
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": a.numel() * 8, "copy_bytes_written": a.numel() * 8,
+                          "launches": 3}))
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU design is documented, not built)", file=sys.stderr)
