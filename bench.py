@@ -46,12 +46,8 @@ def build_batch(members: int, seed: int):
 
 
 def pair_totals(ctx, mids):
-    ncand = ncont = nbond = 0
-    for m in mids:
-        ncand += ctx.pairs(m, "candidate")[0].size
-        ncont += ctx.pairs(m, "contact")[0].size
-        nbond += ctx.pairs(m, "bond")[0].size
-    return ncand, ncont, nbond
+    t = ctx.totals()
+    return t["candidates"], t["contacts"], t["bonds"]
 
 
 def run_batched(args):

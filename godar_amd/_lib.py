@@ -32,7 +32,7 @@ EXPORTS = [
     "godar_step", "godar_step_async", "godar_sync", "godar_member_tstep", "godar_reserve",
     "godar_get_floes", "godar_record_forces", "godar_get_forces", "godar_get_walls",
     "godar_get_pairs", "godar_get_bond_state", "godar_get_stats", "godar_set_block",
-    "godar_profile", "godar_get_profile", "godar_stage_name",
+    "godar_profile", "godar_get_profile", "godar_stage_name", "godar_get_totals",
 ]
 NSTAGES = 8
 
@@ -91,6 +91,7 @@ def load(path: str = LIB_PATH):
         "godar_profile": (_i32, [_vp, _i32]),
         "godar_get_profile": (_i32, [_vp, _dp, _lp]),
         "godar_stage_name": (ctypes.c_char_p, [_i32]),
+        "godar_get_totals": (_i32, [_vp, _lp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

@@ -17,10 +17,12 @@ __global__ void k_bin(DS);
 __global__ void k_clear_cells(DS);
 __global__ void k_ranges(DS);
 __global__ void k_cand(DS);
-template <bool M> __global__ void k_spec(DS);
+template <bool M> __global__ void k_fast(DS);
+template <bool M> __global__ void k_heavy(DS);
 template <bool M> __global__ void k_repair(DS, int);
 template <bool M> __global__ void k_fixup(DS);
 template <bool M> __global__ void k_final(DS);
+__global__ void k_totals(DS, int64_t*);
 
 __global__ void k_fill_u8(uint8_t* p, int64_t n, uint8_t v) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -178,9 +180,8 @@ int ensure_cells(godar_ctx* c, int32_t need) {
     if (need <= c->ncells_cap) return GODAR_OK;
     int32_t cap = std::max(need, c->ncells_cap * 2);
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    dfree(c->d.cstart); dfree(c->d.cend);
-    HIPCHK(c, dalloc(&c->d.cstart, (int64_t)cap + 1));
-    HIPCHK(c, dalloc(&c->d.cend, (int64_t)cap + 1));
+    dfree(c->d.crange);
+    HIPCHK(c, dalloc(&c->d.crange, (int64_t)cap + 1));
     c->ncells_cap = cap;
     return GODAR_OK;
 }
@@ -289,7 +290,11 @@ int launch_step(godar_ctx* c) {
     { StageTimer t(c, 3); hipLaunchKernelGGL(k_cand, g, b, 0, s, d); }
     int nm = c->Mcap;
     const bool mixed = c->precision == GODAR_MIXED;
-    { StageTimer t(c, 4); hipLaunchKernelGGL(mixed ? k_spec<true> : k_spec<false>, g, b, 0, s, d); }
+    {
+        StageTimer t(c, 4);
+        hipLaunchKernelGGL(mixed ? k_fast<true> : k_fast<false>, g, b, 0, s, d);
+        hipLaunchKernelGGL(mixed ? k_heavy<true> : k_heavy<false>, g, b, 0, s, d);
+    }
     {
         StageTimer t(c, 5);
         hipLaunchKernelGGL(mixed ? k_repair<true> : k_repair<false>, dim3(nm), dim3(64), 0, s, d, nm);
@@ -362,9 +367,10 @@ int godar_create(int64_t max_floes, int32_t max_members, int32_t precision, goda
         }                                              \
     } while (0)
     if (e != hipSuccess) { delete c; return GODAR_ERR_HIP; }
-    double** dbl[] = {&d.x, &d.y, &d.u, &d.v, &d.theta, &d.omega, &d.r, &d.h, &d.mass, &d.inertia,
-                      &d.hfa, &d.hfw, &d.uai, &d.vai, &d.sx, &d.sy, &d.sr};
-    for (double** p : dbl) ACHK(dalloc(p, S));
+    ACHK(dalloc(&d.rec, S));
+    ACHK(dalloc(&d.srec, S));
+    ACHK(dalloc(&d.pos_of, S));
+    ACHK(dalloc(&d.wind, S));
     ACHK(dalloc(&d.wbc, 4 * S));
     ACHK(dalloc(&d.member_of, S));
     ACHK(dalloc(&d.members, max_members));
@@ -373,7 +379,6 @@ int godar_create(int64_t max_floes, int32_t max_members, int32_t precision, goda
     d.params = pp;
     ACHK(dalloc(&d.ckey, S));
     ACHK(dalloc(&d.cval, S));
-    ACHK(dalloc(&d.sidx, S));
     ACHK(dalloc(&d.ocnt, S));
     ACHK(dalloc(&d.icnt, S));
     for (int k = 0; k < 2; ++k) {
@@ -383,6 +388,10 @@ int godar_create(int64_t max_floes, int32_t max_members, int32_t precision, goda
     ACHK(alloc_lists(c));
     ACHK(dalloc(&d.acc, (int64_t)NACC * S));
     ACHK(dalloc(&d.wout, S));
+    ACHK(dalloc(&d.wflag, S));
+    ACHK(dalloc(&d.cls, S));
+    ACHK(dalloc(&d.heavy, S));
+    ACHK(dalloc(&d.nheavy, 1));
     ACHK(dalloc(&d.seeds, (int64_t)c->SEEDK * S));
     ACHK(dalloc(&d.nseeds, max_members));
     ACHK(dalloc(&d.heap, (int64_t)c->HEAPK * S));
@@ -421,18 +430,18 @@ int godar_destroy(godar_ctx* c) {
     if (!c) return GODAR_ERR_ARG;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     DS& d = c->d;
-    double* dbl[] = {d.x, d.y, d.u, d.v, d.theta, d.omega, d.r, d.h, d.mass, d.inertia,
-                     d.hfa, d.hfw, d.uai, d.vai, d.sx, d.sy, d.sr, d.wbc, d.acc, d.cur, d.fout};
+    double* dbl[] = {d.wbc, d.acc, d.cur, d.fout};
     for (double* p : dbl) dfree(p);
+    dfree(d.rec); dfree(d.srec); dfree(d.pos_of); dfree(d.wind);
     dfree(d.member_of); dfree(d.members);
     DevParams* pp = const_cast<DevParams*>(d.params); dfree(pp);
     WindGridDesc* wg = const_cast<WindGridDesc*>(d.wgrids); dfree(wg);
     double* wp = const_cast<double*>(d.wpool); dfree(wp);
-    dfree(d.ckey); dfree(d.cval); dfree(d.sidx); dfree(d.ocnt); dfree(d.icnt);
-    dfree(d.cstart); dfree(d.cend);
+    dfree(d.ckey); dfree(d.cval); dfree(d.ocnt); dfree(d.icnt);
+    dfree(d.crange);
     for (int k = 0; k < 2; ++k) { dfree(c->hcnt[k]); dfree(c->bcnt[k]); }
     free_lists(c);
-    dfree(d.wout); dfree(d.seeds); dfree(d.nseeds); dfree(d.heap); dfree(d.etab);
+    dfree(d.wout); dfree(d.wflag); dfree(d.cls); dfree(d.heavy); dfree(d.nheavy); dfree(d.seeds); dfree(d.nseeds); dfree(d.heap); dfree(d.etab);
     dfree(d.netab); dfree(d.wtab); dfree(d.nwtab); dfree(d.ovw); dfree(d.dirty);
     dfree(d.affected); dfree(d.stats); dfree(d.err);
     radix_free(c->rs);
@@ -513,19 +522,19 @@ int godar_member_add(godar_ctx* c, const godar_params* p, int32_t n, const doubl
     dp.cell_base = (int32_t)cbase;
     c->cell_n[m] = ncell;
     // floe state and floe_properties (physics.f90:179-197), as ini_get does
-    std::vector<double> mass(n), inertia(n), hfa(n), hfw(n);
+    std::vector<FloeRec> recs(n);
     for (int i = 0; i < n; ++i) {
-        mass[i] = dp.rhoice * dp.pi * h[i] * (r[i] * r[i]);
-        inertia[i] = 0.5 * mass[i] * (r[i] * r[i]);
-        hfa[i] = h[i] * (dp.rhowater - dp.rhoice) / dp.rhowater;
-        hfw[i] = h[i] * dp.rhoice / dp.rhowater;
+        FloeRec& q = recs[i];
+        q.x = x[i]; q.y = y[i]; q.u = u[i]; q.v = v[i]; q.omega = omega[i]; q.theta = theta[i];
+        q.r = r[i]; q.h = h[i];
+        q.mass = dp.rhoice * dp.pi * h[i] * (r[i] * r[i]);
+        q.inertia = 0.5 * q.mass * (r[i] * r[i]);
+        q.hfa = h[i] * (dp.rhowater - dp.rhoice) / dp.rhowater;
+        q.hfw = h[i] * dp.rhoice / dp.rhowater;
     }
     HIPCHK(c, hipStreamSynchronize(c->stream));
     DS& d = c->d;
-    const double* src[] = {x, y, u, v, theta, omega, r, h, mass.data(), inertia.data(), hfa.data(), hfw.data()};
-    double* dst[] = {d.x, d.y, d.u, d.v, d.theta, d.omega, d.r, d.h, d.mass, d.inertia, d.hfa, d.hfw};
-    for (int k = 0; k < 12; ++k)
-        HIPCHK(c, hipMemcpy(dst[k] + first, src[k], sizeof(double) * n, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(d.rec + first, recs.data(), sizeof(FloeRec) * n, hipMemcpyHostToDevice));
     for (int k = 0; k < 4; ++k) HIPCHK(c, hipMemset(d.wbc + k * c->S + first, 0, sizeof(double) * n));
     for (int k = 0; k < 2; ++k) {
         HIPCHK(c, hipMemset(c->hcnt[k] + first, 0, sizeof(int32_t) * n));
@@ -787,11 +796,15 @@ int godar_get_floes(godar_ctx* c, int32_t m, double* x, double* y, double* u, do
     if (!valid_member(c, m)) return fail(c, GODAR_ERR_ARG, "get_floes: bad member");
     HIPCHK(c, hipStreamSynchronize(c->stream));
     const Member& mm = c->members[m];
-    DS& d = c->d;
+    std::vector<FloeRec> recs(mm.n);
+    HIPCHK(c, hipMemcpy(recs.data(), c->d.rec + mm.first, sizeof(FloeRec) * mm.n, hipMemcpyDeviceToHost));
     double* dst[] = {x, y, u, v, theta, omega, r, h, mass, inertia, hfa, hfw};
-    double* src[] = {d.x, d.y, d.u, d.v, d.theta, d.omega, d.r, d.h, d.mass, d.inertia, d.hfa, d.hfw};
-    for (int k = 0; k < 12; ++k)
-        if (dst[k]) HIPCHK(c, hipMemcpy(dst[k], src[k] + mm.first, sizeof(double) * mm.n, hipMemcpyDeviceToHost));
+    for (int i = 0; i < mm.n; ++i) {
+        const FloeRec& q = recs[i];
+        double val[12] = {q.x, q.y, q.u, q.v, q.theta, q.omega, q.r, q.h, q.mass, q.inertia, q.hfa, q.hfw};
+        for (int k = 0; k < 12; ++k)
+            if (dst[k]) dst[k][i] = val[k];
+    }
     return GODAR_OK;
 }
 
@@ -824,23 +837,33 @@ int godar_get_pairs(godar_ctx* c, int32_t m, int32_t kind, int32_t cap, int32_t*
     int64_t S = c->S;
     std::vector<std::pair<int32_t, int32_t>> out;
     if (kind == GODAR_PAIRS_CANDIDATE || kind == GODAR_PAIRS_CONTACT) {
-        std::vector<int32_t> cnt(mm.n);
-        HIPCHK(c, hipMemcpy(cnt.data(), c->d.ocnt + mm.first, sizeof(int32_t) * mm.n, hipMemcpyDeviceToHost));
-        std::vector<int32_t> oj((int64_t)c->CO * mm.n);
-        std::vector<uint32_t> ef((int64_t)c->CO * mm.n);
-        for (int s = 0; s < c->CO; ++s) {
-            HIPCHK(c, hipMemcpy(oj.data() + (int64_t)s * mm.n, c->d.oj + s * S + mm.first,
-                                sizeof(int32_t) * mm.n, hipMemcpyDeviceToHost));
-            HIPCHK(c, hipMemcpy(ef.data() + (int64_t)s * mm.n, c->d.evf + s * S + mm.first,
-                                sizeof(uint32_t) * mm.n, hipMemcpyDeviceToHost));
+        // the member's floes occupy a contiguous range of this step's cell order
+        std::vector<int32_t> pos(mm.n);
+        HIPCHK(c, hipMemcpy(pos.data(), c->d.pos_of + mm.first, sizeof(int32_t) * mm.n, hipMemcpyDeviceToHost));
+        int32_t p0 = *std::min_element(pos.begin(), pos.end());
+        int32_t len = *std::max_element(pos.begin(), pos.end()) - p0 + 1;
+        std::vector<int32_t> cnt(len);
+        HIPCHK(c, hipMemcpy(cnt.data(), c->d.ocnt + p0, sizeof(int32_t) * len, hipMemcpyDeviceToHost));
+        int32_t cmax = 0;
+        for (int32_t v : cnt) cmax = std::max(cmax, v);
+        cmax = std::min(cmax, c->CO);
+        std::vector<int64_t> oj((int64_t)cmax * len);
+        std::vector<uint32_t> ef((int64_t)cmax * len);
+        for (int s = 0; s < cmax; ++s) {
+            HIPCHK(c, hipMemcpy(oj.data() + (int64_t)s * len, c->d.oj + s * S + p0,
+                                sizeof(int64_t) * len, hipMemcpyDeviceToHost));
+            HIPCHK(c, hipMemcpy(ef.data() + (int64_t)s * len, c->d.evf + s * S + p0,
+                                sizeof(uint32_t) * len, hipMemcpyDeviceToHost));
         }
-        for (int i = 0; i < mm.n; ++i)
-            for (int s = 0; s < cnt[i] && s < c->CO; ++s) {
-                uint32_t fl = ef[(int64_t)s * mm.n + i];
+        for (int i = 0; i < mm.n; ++i) {
+            int32_t r0 = pos[i] - p0;
+            for (int s = 0; s < cnt[r0] && s < cmax; ++s) {
+                uint32_t fl = ef[(int64_t)s * len + r0];
                 if (fl & EV_REMOVED) continue;
                 if (kind == GODAR_PAIRS_CONTACT && !(fl & EV_CONTACT)) continue;
-                out.push_back({i, oj[(int64_t)s * mm.n + i] - mm.first});
+                out.push_back({i, (int32_t)(oj[(int64_t)s * len + r0] >> 32) - mm.first});
             }
+        }
     } else if (kind == GODAR_PAIRS_BOND) {
         int o = c->parity;
         std::vector<int32_t> cnt(mm.n);
@@ -886,6 +909,22 @@ int godar_get_bond_state(godar_ctx* c, int32_t m, int32_t cap, double* st, int32
         double v[11] = {(double)b.bond, b.damage, b.deltanb, b.deltatb, b.trb_ji, b.trb_ij, b.rb, b.hb, b.lb, b.sb, b.ib};
         std::memcpy(st + 11 * (int64_t)k, v, sizeof(v));
     }
+    return GODAR_OK;
+}
+
+int godar_get_totals(godar_ctx* c, int64_t* out) {
+    if (!c || !out) return GODAR_ERR_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    int64_t* dout = nullptr;
+    HIPCHK(c, hipMalloc(&dout, 3 * sizeof(int64_t)));
+    HIPCHK(c, hipMemset(dout, 0, 3 * sizeof(int64_t)));
+    sync_ds(c);
+    if (c->N_used > 0)
+        hipLaunchKernelGGL(k_totals, dim3((c->N_used + 255) / 256), dim3(256), 0, c->stream, c->d, dout);
+    hipError_t e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) e = hipMemcpy(out, dout, 3 * sizeof(int64_t), hipMemcpyDeviceToHost);
+    (void)hipFree(dout);
+    HIPCHK(c, e);
     return GODAR_OK;
 }
 

@@ -12,10 +12,26 @@ enum : int {
 };
 
 constexpr int WALL_SLOT = 0xFFFF;
+constexpr int FLOE_BLOCK = 256;   // upper bound of the per-floe kernels' block size
 
-// Everything the kernels touch.  Per-floe arrays have length S (the floe
-// capacity); per-floe lists are slot-major, entry s of floe f at [s*S + f],
-// so consecutive floes read consecutive addresses.
+// Hot per-floe record (96 bytes): everything a pair event reads of a floe.
+// The persistent copy is indexed by floe id; each step a copy sorted by
+// cell (SRec) is built, so that neighbours are close in memory.
+struct FloeRec {
+    double x, y, u, v, omega;
+    double r, h, mass, inertia, hfa, hfw;
+    double theta;
+};
+struct SRec {
+    double x, y, u, v, omega;
+    double r, h, mass, inertia, hfa, hfw;
+    int32_t id;      // floe id (global index of the persistent arrays)
+    int32_t member;
+};
+
+// Everything the kernels touch.  "id" arrays are indexed by floe id; "pos"
+// arrays by the floe's position in this step's cell order.  Per-floe lists
+// are slot-major, entry s of row r at [s*S + r].
 struct DS {
     int64_t S;                 // floe capacity = stride of slot-major arrays
     int32_t N;                 // floe slots in use [0, N)
@@ -23,47 +39,53 @@ struct DS {
     int32_t SEEDK, HEAPK, TABK;  // repair pool entries per floe slot
     int32_t record_forces;
     int32_t ncells;            // cell slots (sentinel key = ncells)
-    // floe state
-    double *x, *y, *u, *v, *theta, *omega, *r, *h, *mass, *inertia, *hfa, *hfw;
+    // persistent floe state (id)
+    FloeRec* rec;
     double* wbc;               // [4][S] deltat_bc1, deltat_bc2, theta_bc1, theta_bc2
-    double *uai, *vai;         // wind at each floe this step
     int32_t* member_of;        // [S], -1 = free slot
     Member* members;
     const DevParams* params;
     const WindGridDesc* wgrids;
     const double* wpool;
     // cell list
-    uint32_t *ckey, *cval;
-    int32_t *cstart, *cend;
-    double *sx, *sy, *sr;
-    int32_t* sidx;
-    // candidate lists
-    int32_t *ocnt, *oj;
+    uint32_t *ckey, *cval;     // sort keys / floe ids
+    int2* crange;              // [ncells+1] (start, end) in cell order
+    SRec* srec;                // [S] pos
+    int32_t* pos_of;           // [S] id -> pos
+    double2* wind;             // [S] pos: wind at the floe this step
+    // candidate lists (pos rows; entries are partner positions)
+    int32_t* ocnt;
+    int64_t* oj;
     double* od;
-    int32_t *icnt, *ik;
+    int32_t* icnt;
+    int64_t* ik;
     uint32_t* evf;
     int32_t* ov;
-    // pair state, double-buffered
+    // pair state, double-buffered (id rows; partner stored as id)
     HistRec *hist_old, *hist_new;
     int32_t *hcnt_old, *hcnt_new;
     BondRec *bond_old, *bond_new;
     int32_t *bcnt_old, *bcnt_new;
-    // speculative gather
+    // speculative gather (pos)
     double* acc;               // [NACC][S]
-    WallOut* wout;             // [S]
-    // ordered repair
-    uint64_t* seeds;           // SEEDK per floe slot, region of member m starts at SEEDK*first
+    WallOut* wout;             // [S], valid where wflag is set
+    uint8_t* wflag;            // [S] 1: the floe touched a wall this step
+    int32_t* ovw;              // [S]
+    uint8_t* cls;              // [S] 0: sheltering-only floe, 1: full gather
+    int32_t* heavy;            // [S] positions of the full-gather floes
+    int32_t* nheavy;           // count of `heavy`
+    // ordered repair (id)
+    uint64_t* seeds;           // SEEDK per floe slot, member region starts at SEEDK*first
     int32_t* nseeds;           // [members]
     uint64_t* heap;            // HEAPK per floe slot
     EventOut* etab;            // TABK per floe slot
     int32_t* netab;
     WallOut* wtab;             // one per floe slot
     int32_t* nwtab;
-    int32_t* ovw;              // [S]
     double* cur;               // [6][S] r h mass inertia hfa hfw during repair
     uint8_t *dirty, *affected; // [S]
     int64_t* stats;            // [members][4]
-    double* fout;              // [33][S]
+    double* fout;              // [33][S] (id)
     uint32_t* err;
 };
 

@@ -1,14 +1,16 @@
 // Kernels of the GODAR step pipeline (one launch sequence per step):
 //
-//   k_bin -> radix sort -> k_clear_cells -> k_ranges -> k_cand -> k_spec
-//        -> k_repair -> k_fixup -> k_final
+//   k_bin -> radix sort -> k_clear_cells -> k_ranges -> k_cand -> k_fast
+//        -> k_heavy -> k_repair -> k_fixup -> k_final
 //
 // Sequential semantics of the reference (stepper.f90:71-280 visits floes i
 // in index order and, for each, its candidates j > i in ascending distance;
 // ridging reshapes floes in place, so later pairs see the new radius and
 // thickness) are reproduced as follows:
-//   * k_spec evaluates every pair event with the start-of-step floe
-//     properties.  One thread per floe gathers its own contributions in the
+//   * k_fast/k_heavy evaluate every pair event with the start-of-step floe
+//     properties.  k_fast handles floes whose events are all sheltering-only
+//     (no contact, bond, contact history or wall) and queues the others for
+//     k_heavy's full gather.  One thread per floe gathers its own contributions in the
 //     reference's per-floe order: first the pairs where it is the inner
 //     index (ascending outer index), then its own candidates in distance
 //     order.  Each pair is evaluated by both of its floes, which removes
@@ -20,6 +22,11 @@
 //     properties and records the results.
 //   * k_fixup re-gathers the floes touched by the repair, k_final applies
 //     the final floe properties, forcing, Coriolis, totals and integration.
+//
+// Layout: the per-step kernels run over floes in cell order ("pos"), reading
+// a sorted copy of the 96-byte floe records, so that a floe's neighbours sit
+// close in memory.  Pair state persists per floe id.  List entries pack the
+// partner as (id << 32 | pos).
 #include "godar_engine.h"
 #include "godar_physics.h"
 
@@ -27,17 +34,29 @@ namespace godar {
 
 __device__ __forceinline__ void set_err(const DS& S, uint32_t bit) { atomicOr(S.err, bit); }
 
-__device__ __forceinline__ Floe load_floe(const DS& S, int f) {
+__device__ __forceinline__ int ent_id(int64_t e) { return (int)(e >> 32); }
+__device__ __forceinline__ int ent_pos(int64_t e) { return (int)(uint32_t)(e & 0xffffffffll); }
+__device__ __forceinline__ int64_t ent(int id, int pos) {
+    return ((int64_t)id << 32) | (int64_t)(uint32_t)pos;
+}
+
+__device__ __forceinline__ Floe floe_of(const SRec& s) {
     Floe F;
-    F.x = S.x[f]; F.y = S.y[f]; F.u = S.u[f]; F.v = S.v[f]; F.omega = S.omega[f];
-    F.r = S.r[f]; F.h = S.h[f]; F.mass = S.mass[f]; F.inertia = S.inertia[f];
-    F.hfa = S.hfa[f]; F.hfw = S.hfw[f];
+    F.x = s.x; F.y = s.y; F.u = s.u; F.v = s.v; F.omega = s.omega;
+    F.r = s.r; F.h = s.h; F.mass = s.mass; F.inertia = s.inertia; F.hfa = s.hfa; F.hfw = s.hfw;
     return F;
 }
 
-// floe with the properties current during the ordered repair
+__device__ __forceinline__ Floe floe_of(const FloeRec& s) {
+    Floe F;
+    F.x = s.x; F.y = s.y; F.u = s.u; F.v = s.v; F.omega = s.omega;
+    F.r = s.r; F.h = s.h; F.mass = s.mass; F.inertia = s.inertia; F.hfa = s.hfa; F.hfw = s.hfw;
+    return F;
+}
+
+// floe by id with the properties current during the ordered repair
 __device__ __forceinline__ Floe load_floe_cur(const DS& S, int f) {
-    Floe F = load_floe(S, f);
+    Floe F = floe_of(S.rec[f]);
     if (S.dirty[f]) {
         F.r = S.cur[0 * S.S + f]; F.h = S.cur[1 * S.S + f]; F.mass = S.cur[2 * S.S + f];
         F.inertia = S.cur[3 * S.S + f]; F.hfa = S.cur[4 * S.S + f]; F.hfw = S.cur[5 * S.S + f];
@@ -52,7 +71,7 @@ __device__ __forceinline__ int cell_coord(double p, double cell, int g) {
     return (int)q;
 }
 
-// Old pair state of (i, j), owner i.
+// Old pair state of (i, j), owner i (ids).
 __device__ __forceinline__ void load_pair_state(const DS& S, int i, int j, PairState& ps) {
     ps.deltat = 0.0; ps.thetarelc = 0.0; ps.bond = 0; ps.has_bond_rec = false;
     ps.damage = ps.deltanb = ps.deltatb = ps.trb_ji = ps.trb_ij = 0.0;
@@ -135,25 +154,18 @@ __device__ __forceinline__ void acc_init(double* a) {
 }
 
 // ---------------------------------------------------------------------
-// 1. cell keys and per-floe wind (winds_currents, forcing.f90:1-29)
+// 1. cell keys (one per floe id)
 // ---------------------------------------------------------------------
 __global__ void k_bin(DS S) {
     int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= S.N || *S.err) return;
     int m = S.member_of[f];
-    if (m < 0) { S.ckey[f] = (uint32_t)S.ncells; S.cval[f] = (uint32_t)f; return; }
+    S.cval[f] = (uint32_t)f;
+    if (m < 0) { S.ckey[f] = (uint32_t)S.ncells; return; }
     const DevParams& P = S.params[m];
-    double x = S.x[f], y = S.y[f];
-    if (P.wind_grid >= 0) {
-        double ua, va;
-        sample_wind(S.wgrids[P.wind_grid], S.wpool, x, y, ua, va);
-        S.uai[f] = ua; S.vai[f] = va;
-    } else {
-        S.uai[f] = P.ua; S.vai[f] = P.va;
-    }
+    double x = S.rec[f].x, y = S.rec[f].y;
     int cx = cell_coord(x, P.cell, P.gx), cy = cell_coord(y, P.cell, P.gy);
     S.ckey[f] = (uint32_t)(P.cell_base + cy * P.gx + cx);
-    S.cval[f] = (uint32_t)f;
     if (f == S.members[m].first) {
         S.nseeds[m] = 0; S.netab[m] = 0; S.nwtab[m] = 0;
         S.stats[4 * m + 2] = 0; S.stats[4 * m + 3] = 0;
@@ -163,21 +175,33 @@ __global__ void k_bin(DS S) {
 __global__ void k_clear_cells(DS S) {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c > S.ncells || *S.err) return;
-    S.cstart[c] = 0;
-    S.cend[c] = 0;
+    S.crange[c] = make_int2(0, 0);
+    if (c == 0) *S.nheavy = 0;
 }
 
-// cell ranges over the sorted keys, plus sorted copies of x, y, r
+// cell ranges over the sorted keys, the sorted floe records and the wind at
+// each floe (winds_currents, forcing.f90:1-29)
 __global__ void k_ranges(DS S) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= S.N || *S.err) return;
     uint32_t key = S.ckey[k];
     int f = (int)S.cval[k];
-    S.sidx[k] = f;
-    S.sx[k] = S.x[f]; S.sy[k] = S.y[f]; S.sr[k] = S.r[f];
-    if (key >= (uint32_t)S.ncells) return;
-    if (k == 0 || S.ckey[k - 1] != key) S.cstart[key] = k;
-    if (k == S.N - 1 || S.ckey[k + 1] != key) S.cend[key] = k + 1;
+    int m = key < (uint32_t)S.ncells ? S.member_of[f] : -1;
+    const FloeRec fr = S.rec[f];
+    SRec s;
+    s.x = fr.x; s.y = fr.y; s.u = fr.u; s.v = fr.v; s.omega = fr.omega;
+    s.r = fr.r; s.h = fr.h; s.mass = fr.mass; s.inertia = fr.inertia; s.hfa = fr.hfa; s.hfw = fr.hfw;
+    s.id = f;
+    s.member = m;
+    S.srec[k] = s;
+    S.pos_of[f] = k;
+    if (m < 0) return;
+    if (k == 0 || S.ckey[k - 1] != key) S.crange[key].x = k;
+    if (k == S.N - 1 || S.ckey[k + 1] != key) S.crange[key].y = k + 1;
+    const DevParams& P = S.params[m];
+    double ua = P.ua, va = P.va;
+    if (P.wind_grid >= 0) sample_wind(S.wgrids[P.wind_grid], S.wpool, fr.x, fr.y, ua, va);
+    S.wind[k] = make_double2(ua, va);
 }
 
 // ---------------------------------------------------------------------
@@ -185,56 +209,59 @@ __global__ void k_ranges(DS S) {
 //    outer(f) = { j > f : dist <= r(f) + rtree } by (dist, j)
 //    inner(f) = { k < f : dist <= r(k) + rtree } by k
 // ---------------------------------------------------------------------
-__global__ void k_cand(DS S) {
-    int f = blockIdx.x * blockDim.x + threadIdx.x;
-    if (f >= S.N || *S.err) return;
-    int m = S.member_of[f];
-    if (m < 0) { S.ocnt[f] = 0; S.icnt[f] = 0; return; }
-    const DevParams& P = S.params[m];
-    double xf = S.x[f], yf = S.y[f];
-    double rad = S.r[f] + P.rtree;
-    int cx = cell_coord(xf, P.cell, P.gx), cy = cell_coord(yf, P.cell, P.gy);
+__global__ __launch_bounds__(FLOE_BLOCK) void k_cand(DS S) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= S.N || *S.err) return;
+    const SRec me = S.srec[k];
+    if (me.member < 0) { S.ocnt[k] = 0; S.icnt[k] = 0; return; }
+    const DevParams& P = S.params[me.member];
+    const double rad = me.r + P.rtree;
+    const int myid = me.id;
+    int cx = cell_coord(me.x, P.cell, P.gx), cy = cell_coord(me.y, P.cell, P.gy);
     int no = 0, ni = 0;
     uint32_t over = 0;
     const int64_t SS = S.S;
+    int64_t* oj = S.oj;
+    int64_t* ik = S.ik;
     for (int b = cy - 1; b <= cy + 1; ++b) {
         if (b < 0 || b >= P.gy) continue;
         for (int a = cx - 1; a <= cx + 1; ++a) {
             if (a < 0 || a >= P.gx) continue;
-            int c = P.cell_base + b * P.gx + a;
-            int k1 = S.cend[c];
-            for (int k = S.cstart[c]; k < k1; ++k) {
-                int j = S.sidx[k];
-                if (j == f) continue;
-                double dx = xf - S.sx[k], dy = yf - S.sy[k];
+            const int2 rg = S.crange[P.cell_base + b * P.gx + a];
+            for (int q = rg.x; q < rg.y; ++q) {
+                if (q == k) continue;
+                const SRec* o = &S.srec[q];
+                double dx = me.x - o->x, dy = me.y - o->y;
                 double d = sqrt(dx * dx + dy * dy);
-                if (j > f) {
+                int j = o->id;
+                if (j > myid) {
                     if (d <= rad) {
                         if (no >= S.CO) { over |= ERR_OUTER_CAP; continue; }
                         int p = no;
                         while (p > 0) {
-                            double dp = S.od[(p - 1) * SS + f];
-                            int jp = S.oj[(p - 1) * SS + f];
-                            if (dp < d || (dp == d && jp < j)) break;
-                            S.od[p * SS + f] = dp;
-                            S.oj[p * SS + f] = jp;
+                            double dp = S.od[(p - 1) * SS + k];
+                            int64_t ep = oj[(p - 1) * SS + k];
+                            if (dp < d || (dp == d && ent_id(ep) < j)) break;
+                            S.od[p * SS + k] = dp;
+                            oj[p * SS + k] = ep;
                             --p;
                         }
-                        S.od[p * SS + f] = d;
-                        S.oj[p * SS + f] = j;
+                        S.od[p * SS + k] = d;
+                        oj[p * SS + k] = ent(j, q);
                         ++no;
                     }
                 } else {
-                    if (d <= S.sr[k] + P.rtree) {
+                    if (d <= o->r + P.rtree) {
                         if (ni >= S.CI) { over |= ERR_INNER_CAP; continue; }
                         int p = ni;
+                        int64_t e = ent(j, q);
                         while (p > 0) {
-                            int jp = S.ik[(p - 1) * SS + f];
-                            if (jp < j) break;
-                            S.ik[p * SS + f] = jp;
+                            int64_t ep = ik[(p - 1) * SS + k];
+                            if (ep < e) break;
+                            ik[p * SS + k] = ep;
                             --p;
                         }
-                        S.ik[p * SS + f] = j;
+                        ik[p * SS + k] = e;
                         ++ni;
                     }
                 }
@@ -242,44 +269,151 @@ __global__ void k_cand(DS S) {
         }
     }
     if (over) set_err(S, over);
-    S.ocnt[f] = no;
-    S.icnt[f] = ni;
+    S.ocnt[k] = no;
+    S.icnt[k] = ni;
 }
 
 // ---------------------------------------------------------------------
 // 3. speculative events + per-floe gather + owner state writes
 // ---------------------------------------------------------------------
+// Sheltering-only floes: geometry and the floe's own shelter coefficients
+// (forcing.f90:162-186); any other event sends the floe to k_heavy.
 template <bool MIXED>
-__global__ void k_spec(DS S) {
-    int f = blockIdx.x * blockDim.x + threadIdx.x;
-    if (f >= S.N || *S.err) return;
-    int m = S.member_of[f];
+__global__ __launch_bounds__(FLOE_BLOCK) void k_fast(DS S) {
+    using T = typename Prec<MIXED>::T;
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    bool heavy = false;
+    bool live = k < S.N && !*S.err;
+    SRec me;
+    if (live) {
+        me = S.srec[k];
+        live = me.member >= 0;
+    }
+    if (live) {
+        const int m = me.member;
+        const DevParams& P = S.params[m];
+        const int f = me.id;
+        const int64_t SS = S.S;
+        const int first = S.members[m].first;
+        const bool bond_step = S.members[m].tstep == 1 && P.restart != 1 && P.cohesion;
+        heavy = S.hcnt_old[f] > 0 || S.bcnt_old[f] > 0 ||
+                me.y - me.r < 0 || me.y + me.r > P.ny || me.x - me.r < 0 || me.x + me.r > P.nx;
+        T sha = T(1), shw = T(1);
+        const T uwi = T(P.uw), vwi = T(P.vw);
+        const int ni = S.icnt[k];
+        for (int q = 0; q < ni && !heavy; ++q) {
+            const int64_t e = S.ik[q * SS + k];
+            const int i = ent_id(e), qi = ent_pos(e);
+            const SRec* o = &S.srec[qi];
+            double dx = o->x - me.x, dy = o->y - me.y;
+            double dist = sqrt(dx * dx + dy * dy);
+            double deltan = o->r + me.r - dist;
+            if (deltan > 0) { heavy = true; break; }
+            if (bond_step && deltan >= -P.bond_lim &&
+                bond_keep(i - first, f - first, P.bond_seed, P.bond_thr24)) { heavy = true; break; }
+            int bc = S.bcnt_old[i];
+            for (int b = 0; b < bc; ++b)
+                if (S.bond_old[(int64_t)b * SS + i].j == f) { heavy = true; break; }
+            if (heavy) break;
+            if (P.shelter) {
+                // pair (i, f): cosa/sina point from i to f; floe f takes hsf*(i,f)
+                double cosa = (me.x - o->x) / dist, sina = (me.y - o->y) / dist;
+                const double2 wi = S.wind[qi];
+                T dn = T(deltan), c = T(cosa), sn = T(sina);
+                sha = dmin(sha, S_shelter(T(o->hfa), T(me.hfa), dn, c, sn, T(wi.x), T(wi.y)));
+                shw = dmin(shw, S_shelter(T(o->hfw), T(me.hfw), dn, c, sn, uwi, vwi));
+            }
+        }
+        const int no = S.ocnt[k];
+        const double2 wf = S.wind[k];
+        for (int s = 0; s < no && !heavy; ++s) {
+            const int64_t e = S.oj[s * SS + k];
+            const int j = ent_id(e), qj = ent_pos(e);
+            const SRec* o = &S.srec[qj];
+            double dx = me.x - o->x, dy = me.y - o->y;
+            double dist = sqrt(dx * dx + dy * dy);
+            double deltan = me.r + o->r - dist;
+            if (deltan > 0) { heavy = true; break; }
+            if (bond_step && deltan >= -P.bond_lim &&
+                bond_keep(f - first, j - first, P.bond_seed, P.bond_thr24)) { heavy = true; break; }
+            if (P.shelter) {
+                // pair (f, j): floe f takes hsf*(j,f), evaluated with -cosa, -sina
+                double cosa = (o->x - me.x) / dist, sina = (o->y - me.y) / dist;
+                T dn = T(deltan), c = T(cosa), sn = T(sina);
+                sha = dmin(sha, S_shelter(T(o->hfa), T(me.hfa), dn, -c, -sn, T(wf.x), T(wf.y)));
+                shw = dmin(shw, S_shelter(T(o->hfw), T(me.hfw), dn, -c, -sn, uwi, vwi));
+            }
+        }
+        if (!heavy) {
+            const uint32_t fl = P.shelter ? EV_SHELTER : 0u;
+            for (int s = 0; s < no; ++s) {
+                S.evf[s * SS + k] = fl;
+                S.ov[s * SS + k] = -1;
+            }
+            S.acc[A_SHA * SS + k] = double(sha);
+            S.acc[A_SHW * SS + k] = double(shw);
+            S.hcnt_new[f] = 0;
+            S.bcnt_new[f] = 0;
+            S.ovw[k] = -1;
+            S.wflag[k] = 0;
+            S.cls[k] = 0;
+        } else {
+            S.cls[k] = 1;
+        }
+    }
+    // queue the heavy floes, one atomic per wavefront
+    uint64_t mask = __ballot(heavy);
+    if (mask) {
+        const int lane = threadIdx.x & 63;
+        const int leader = __ffsll((long long)mask) - 1;
+        int base = 0;
+        if (lane == leader) base = atomicAdd(S.nheavy, __popcll(mask));
+        base = __shfl(base, leader);
+        if (heavy) {
+            const uint64_t below = lane ? mask & ((~0ull) >> (64 - lane)) : 0ull;
+            S.heavy[base + __popcll(below)] = k;
+        }
+    }
+}
+
+// Full speculative gather of the queued floes + owner state writes.
+template <bool MIXED>
+__global__ __launch_bounds__(FLOE_BLOCK) void k_heavy(DS S) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (*S.err || t >= *S.nheavy) return;
+    const int k = S.heavy[t];
+    const SRec me = S.srec[k];
+    const int m = me.member;
     if (m < 0) return;
     const DevParams& P = S.params[m];
     const int tstep = S.members[m].tstep;
     const int first = S.members[m].first;
+    const int f = me.id;
     const int64_t SS = S.S;
-    const Floe F0 = load_floe(S, f);
+    const int64_t* oj = S.oj;
+    const int64_t* ik = S.ik;
+    const Floe F0 = floe_of(me);
     double acc[NACC];
     acc_init(acc);
     uint32_t over = 0;
 
-    // pairs (k, f), f inner, ascending k
-    const int ni = S.icnt[f];
+    // pairs (i, f), f inner, ascending i
+    const int ni = S.icnt[k];
     for (int q = 0; q < ni; ++q) {
-        int k = S.ik[q * SS + f];
-        Floe Fk = load_floe(S, k);
+        const int64_t e = ik[q * SS + k];
+        const int i = ent_id(e), qi = ent_pos(e);
+        Floe Fi = floe_of(S.srec[qi]);
         Floe Ff = F0;
         PairState ps;
-        load_pair_state(S, k, f, ps);
+        load_pair_state(S, i, f, ps);
+        const double2 wi = S.wind[qi];
         EventOut o;
-        pair_event<MIXED>(P, tstep, k - first, f - first, Fk, Ff, S.uai[k], S.vai[k], P.uw, P.vw,
-                          ps, o, 2);
+        pair_event<MIXED>(P, tstep, i - first, f - first, Fi, Ff, wi.x, wi.y, P.uw, P.vw, ps, o, 2);
         apply_j(acc, o);
     }
 
     // pairs (f, j), f outer (owner), distance order
-    const int no = S.ocnt[f];
+    const int no = S.ocnt[k];
     int nb = 0;
     {
         const int bco = S.bcnt_old[f];
@@ -289,18 +423,19 @@ __global__ void k_spec(DS S) {
         }
     }
     int nh = 0;
-    const double uaf = S.uai[f], vaf = S.vai[f];
+    const double2 wf = S.wind[k];
     for (int s = 0; s < no; ++s) {
-        int j = S.oj[s * SS + f];
+        const int64_t e = oj[s * SS + k];
+        const int j = ent_id(e), qj = ent_pos(e);
         Floe Fi = F0;
-        Floe Fj = load_floe(S, j);
+        Floe Fj = floe_of(S.srec[qj]);
         PairState ps;
         load_pair_state(S, f, j, ps);
         EventOut o;
-        pair_event<MIXED>(P, tstep, f - first, j - first, Fi, Fj, uaf, vaf, P.uw, P.vw, ps, o, 1);
+        pair_event<MIXED>(P, tstep, f - first, j - first, Fi, Fj, wf.x, wf.y, P.uw, P.vw, ps, o, 1);
         apply_i(acc, o);
-        S.evf[s * SS + f] = o.flags;
-        S.ov[s * SS + f] = -1;
+        S.evf[s * SS + k] = o.flags;
+        S.ov[s * SS + k] = -1;
         if (ps.deltat != 0.0 || ps.thetarelc != 0.0) {
             if (nh < S.HC) {
                 HistRec hr; hr.j = j; hr.pad_ = 0; hr.deltat = ps.deltat; hr.thetarelc = ps.thetarelc;
@@ -336,7 +471,7 @@ __global__ void k_spec(DS S) {
         if (hr.j < 0) continue;
         bool cand = false;
         for (int s = 0; s < no; ++s)
-            if (S.oj[s * SS + f] == hr.j) { cand = true; break; }
+            if (ent_id(oj[s * SS + k]) == hr.j) { cand = true; break; }
         if (!cand) {
             if (nh < S.HC) S.hist_new[nh * SS + f] = hr, ++nh;
             else over |= ERR_HIST_CAP;
@@ -351,7 +486,8 @@ __global__ void k_spec(DS S) {
         double hist[4] = {S.wbc[0 * SS + f], S.wbc[1 * SS + f], S.wbc[2 * SS + f], S.wbc[3 * SS + f]};
         WallOut W;
         walls<MIXED>(P, Fw, hist, W);
-        S.wout[f] = W;
+        S.wflag[k] = (uint8_t)(W.flags & 1u);
+        if (W.flags & 1u) S.wout[k] = W;
         if (W.flags & 4u) {
             int p = atomicAdd(&S.nseeds[m], 1);
             if (p < S.SEEDK * S.members[m].n)
@@ -360,9 +496,9 @@ __global__ void k_spec(DS S) {
                 over |= ERR_SEED_CAP;
         }
     }
-    S.ovw[f] = -1;
+    S.ovw[k] = -1;
 #pragma unroll
-    for (int a = 0; a < NACC; ++a) S.acc[a * SS + f] = acc[a];
+    for (int a = 0; a < NACC; ++a) S.acc[a * SS + k] = acc[a];
     if (over) set_err(S, over);
 }
 
@@ -407,10 +543,12 @@ __device__ __forceinline__ void set_cur(const DS& S, int g, double r, double h, 
     S.cur[3 * S.S + g] = inertia; S.cur[4 * S.S + g] = hfa; S.cur[5 * S.S + g] = hfw;
 }
 
-__device__ __forceinline__ int slot_of(const DS& S, int k, int g) {
-    int no = S.ocnt[k];
+// slot of partner id g in the outer list of the floe at position kp
+__device__ __forceinline__ int slot_of(const DS& S, int kp, int g) {
+    const int64_t* oj = S.oj;
+    int no = S.ocnt[kp];
     for (int s = 0; s < no; ++s)
-        if (S.oj[(int64_t)s * S.S + k] == g) return s;
+        if (ent_id(oj[(int64_t)s * S.S + kp]) == g) return s;
     return -1;
 }
 
@@ -464,6 +602,8 @@ __global__ void k_repair(DS S, int nmembers) {
     const int nmem = S.members[m].n;
     const int tstep = S.members[m].tstep;
     const int64_t SS = S.S;
+    const int64_t* oj = S.oj;
+    const int64_t* ik = S.ik;
     if (ns > S.SEEDK * nmem) ns = S.SEEDK * nmem;
     Heap H;
     H.a = S.heap + (int64_t)S.HEAPK * first;
@@ -486,6 +626,7 @@ __global__ void k_repair(DS S, int nmembers) {
         last = key;
         int b = (int)(key >> 16), s = (int)(key & 0xFFFF);
         int f = first + b;
+        int kf = S.pos_of[f];
         ++repaired;
         if (s == WALL_SLOT) {
             Floe F = load_floe_cur(S, f);
@@ -494,7 +635,7 @@ __global__ void k_repair(DS S, int nmembers) {
             walls<MIXED>(P, F, hist, W);
             if (nwt >= nmem) { over |= ERR_REPAIR_CAP; break; }
             wtab[nwt] = W;
-            S.ovw[f] = nwt++;
+            S.ovw[kf] = nwt++;
             S.affected[f] = 1;
             if (W.flags & 4u) {
                 set_cur(S, f, W.nr, W.nh, W.nmass, W.ninertia, W.nhfa, W.nhfw);
@@ -504,9 +645,10 @@ __global__ void k_repair(DS S, int nmembers) {
         }
         if (b != cur_block) {
             cur_block = b;
-            rq = S.dirty[f] ? S.cur[f] : S.r[f];   // query radius at the top of iteration f
+            rq = S.dirty[f] ? S.cur[f] : S.rec[f].r;   // query radius at the top of iteration f
         }
-        int j = S.oj[(int64_t)s * SS + f];
+        const int64_t e = oj[(int64_t)s * SS + kf];
+        const int j = ent_id(e);
         Floe Fi = load_floe_cur(S, f);
         Floe Fj = load_floe_cur(S, j);
         S.affected[f] = 1;
@@ -514,53 +656,50 @@ __global__ void k_repair(DS S, int nmembers) {
         EventOut o;
         double dx = Fi.x - Fj.x, dy = Fi.y - Fj.y;
         double d = sqrt(dx * dx + dy * dy);
+        PairState ps;
+        load_pair_state(S, f, j, ps);
         if (!(d <= rq + P.rtree)) {
             // no longer a candidate: the pair keeps its old state
             o.flags = EV_REMOVED;
             o.ridged = -1;
-            PairState ps;
-            load_pair_state(S, f, j, ps);
-            bool okh = put_hist(S, f, j, ps.deltat != 0.0 || ps.thetarelc != 0.0, ps.deltat, ps.thetarelc);
-            bool okb = put_bond(S, f, j, ps.has_bond_rec, bond_from(ps, j));
-            if (!okh) over |= ERR_HIST_CAP;
-            if (!okb) over |= ERR_BOND_CAP;
         } else {
-            PairState ps;
-            load_pair_state(S, f, j, ps);
-            pair_event<MIXED>(P, tstep, b, j - first, Fi, Fj, S.uai[f], S.vai[f], P.uw, P.vw, ps, o, 0);
-            bool okh = put_hist(S, f, j, ps.deltat != 0.0 || ps.thetarelc != 0.0, ps.deltat, ps.thetarelc);
-            bool okb = put_bond(S, f, j, ps.has_bond_rec, bond_from(ps, j));
-            if (!okh) over |= ERR_HIST_CAP;
-            if (!okb) over |= ERR_BOND_CAP;
-            if (o.flags & EV_RIDGED) {
-                int g = o.ridged == 0 ? f : j;
-                set_cur(S, g, o.nr, o.nh, o.nmass, o.ninertia, o.nhfa, o.nhfw);
-                if (!S.dirty[g]) {
-                    S.dirty[g] = 1;
-                    int gb = g - first;
-                    // every later event of g is recomputed
-                    if (g != f) {
-                        int gi = S.icnt[g];
-                        for (int q = 0; q < gi; ++q) {
-                            int k = S.ik[(int64_t)q * SS + g];
-                            if (k <= f) continue;
-                            int sk = slot_of(S, k, g);
-                            if (sk >= 0 && !H.push(((uint64_t)(k - first) << 16) | (uint64_t)sk))
-                                over |= ERR_REPAIR_CAP;
-                        }
+            const double2 wf = S.wind[kf];
+            pair_event<MIXED>(P, tstep, b, j - first, Fi, Fj, wf.x, wf.y, P.uw, P.vw, ps, o, 0);
+        }
+        bool okh = put_hist(S, f, j, ps.deltat != 0.0 || ps.thetarelc != 0.0, ps.deltat, ps.thetarelc);
+        bool okb = put_bond(S, f, j, ps.has_bond_rec, bond_from(ps, j));
+        if (!okh) over |= ERR_HIST_CAP;
+        if (!okb) over |= ERR_BOND_CAP;
+        if (o.flags & EV_RIDGED) {
+            int g = o.ridged == 0 ? f : j;
+            set_cur(S, g, o.nr, o.nh, o.nmass, o.ninertia, o.nhfa, o.nhfw);
+            if (!S.dirty[g]) {
+                S.dirty[g] = 1;
+                int gb = g - first;
+                int kg = S.pos_of[g];
+                // every later event of g is recomputed
+                if (g != f) {
+                    int gi = S.icnt[kg];
+                    for (int q = 0; q < gi; ++q) {
+                        const int64_t ei = ik[(int64_t)q * SS + kg];
+                        int i2 = ent_id(ei);
+                        if (i2 <= f) continue;
+                        int sk = slot_of(S, ent_pos(ei), g);
+                        if (sk >= 0 && !H.push(((uint64_t)(i2 - first) << 16) | (uint64_t)sk))
+                            over |= ERR_REPAIR_CAP;
                     }
-                    int s0 = (g == f) ? s + 1 : 0;
-                    int gno = S.ocnt[g];
-                    for (int s2 = s0; s2 < gno; ++s2)
-                        if (!H.push(((uint64_t)gb << 16) | (uint64_t)s2)) over |= ERR_REPAIR_CAP;
-                    if (!H.push(((uint64_t)gb << 16) | WALL_SLOT)) over |= ERR_REPAIR_CAP;
                 }
+                int s0 = (g == f) ? s + 1 : 0;
+                int gno = S.ocnt[kg];
+                for (int s2 = s0; s2 < gno; ++s2)
+                    if (!H.push(((uint64_t)gb << 16) | (uint64_t)s2)) over |= ERR_REPAIR_CAP;
+                if (!H.push(((uint64_t)gb << 16) | WALL_SLOT)) over |= ERR_REPAIR_CAP;
             }
         }
         if (nt >= tcap) { over |= ERR_REPAIR_CAP; break; }
         etab[nt] = o;
-        S.ov[(int64_t)s * SS + f] = nt++;
-        S.evf[(int64_t)s * SS + f] = o.flags;
+        S.ov[(int64_t)s * SS + kf] = nt++;
+        S.evf[(int64_t)s * SS + kf] = o.flags;
     }
     S.netab[m] = nt;
     S.nwtab[m] = nwt;
@@ -573,59 +712,66 @@ __global__ void k_repair(DS S, int nmembers) {
 // 5. re-gather floes touched by the repair
 // ---------------------------------------------------------------------
 template <bool MIXED>
-__global__ void k_fixup(DS S) {
-    int f = blockIdx.x * blockDim.x + threadIdx.x;
-    if (f >= S.N || *S.err) return;
-    if (!S.affected[f]) return;
-    int m = S.member_of[f];
+__global__ __launch_bounds__(FLOE_BLOCK) void k_fixup(DS S) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= S.N || *S.err) return;
+    const SRec me = S.srec[k];
+    const int m = me.member;
     if (m < 0) return;
+    const int f = me.id;
+    if (!S.affected[f]) return;
     const DevParams& P = S.params[m];
     const int tstep = S.members[m].tstep;
     const int first = S.members[m].first;
     const int64_t SS = S.S;
+    const int64_t* oj = S.oj;
+    const int64_t* ik = S.ik;
     const EventOut* etab = S.etab + (int64_t)S.TABK * first;
-    const Floe F0 = load_floe(S, f);
+    const Floe F0 = floe_of(me);
     double acc[NACC];
     acc_init(acc);
-    const int ni = S.icnt[f];
+    const int ni = S.icnt[k];
     for (int q = 0; q < ni; ++q) {
-        int k = S.ik[q * SS + f];
-        int sk = slot_of(S, k, f);
-        int idx = sk >= 0 ? S.ov[(int64_t)sk * SS + k] : -1;
+        const int64_t e = ik[q * SS + k];
+        const int i = ent_id(e), qi = ent_pos(e);
+        int sk = slot_of(S, qi, f);
+        int idx = sk >= 0 ? S.ov[(int64_t)sk * SS + qi] : -1;
         if (idx >= 0) {
             EventOut o = etab[idx];
             if (!(o.flags & EV_REMOVED)) apply_j(acc, o);
         } else {
-            Floe Fk = load_floe(S, k);
+            Floe Fi = floe_of(S.srec[qi]);
             Floe Ff = F0;
             PairState ps;
-            load_pair_state(S, k, f, ps);
+            load_pair_state(S, i, f, ps);
+            const double2 wi = S.wind[qi];
             EventOut o;
-            pair_event<MIXED>(P, tstep, k - first, f - first, Fk, Ff, S.uai[k], S.vai[k], P.uw, P.vw,
-                              ps, o, 2);
+            pair_event<MIXED>(P, tstep, i - first, f - first, Fi, Ff, wi.x, wi.y, P.uw, P.vw, ps, o, 2);
             apply_j(acc, o);
         }
     }
-    const int no = S.ocnt[f];
+    const int no = S.ocnt[k];
+    const double2 wf = S.wind[k];
     for (int s = 0; s < no; ++s) {
-        int idx = S.ov[s * SS + f];
+        int idx = S.ov[s * SS + k];
         if (idx >= 0) {
             EventOut o = etab[idx];
             if (!(o.flags & EV_REMOVED)) apply_i(acc, o);
         } else {
-            int j = S.oj[s * SS + f];
+            const int64_t e = oj[s * SS + k];
+            const int j = ent_id(e), qj = ent_pos(e);
             Floe Fi = F0;
-            Floe Fj = load_floe(S, j);
+            Floe Fj = floe_of(S.srec[qj]);
             PairState ps;
             load_pair_state(S, f, j, ps);
             EventOut o;
-            pair_event<MIXED>(P, tstep, f - first, j - first, Fi, Fj, S.uai[f], S.vai[f], P.uw, P.vw,
-                              ps, o, 1);
+            pair_event<MIXED>(P, tstep, f - first, j - first, Fi, Fj, wf.x, wf.y, P.uw, P.vw, ps, o, 1);
             apply_i(acc, o);
         }
     }
 #pragma unroll
-    for (int a = 0; a < NACC; ++a) S.acc[a * SS + f] = acc[a];
+    for (int a = 0; a < NACC; ++a) S.acc[a * SS + k] = acc[a];
+    S.cls[k] = 1;
 }
 
 // ---------------------------------------------------------------------
@@ -633,43 +779,63 @@ __global__ void k_fixup(DS S) {
 //    (stepper.f90:295-343, integration.f90:1-38)
 // ---------------------------------------------------------------------
 template <bool MIXED>
-__global__ void k_final(DS S) {
-    int f = blockIdx.x * blockDim.x + threadIdx.x;
-    if (f >= S.N || *S.err) return;
-    int m = S.member_of[f];
+__global__ __launch_bounds__(FLOE_BLOCK) void k_final(DS S) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= S.N || *S.err) return;
+    const SRec me = S.srec[k];
+    const int m = me.member;
     if (m < 0) return;
+    const int f = me.id;
     const DevParams& P = S.params[m];
     const int64_t SS = S.S;
-    Floe F = load_floe(S, f);
+    Floe F = floe_of(me);
     if (S.dirty[f]) {
         F.r = S.cur[0 * SS + f]; F.h = S.cur[1 * SS + f]; F.mass = S.cur[2 * SS + f];
         F.inertia = S.cur[3 * SS + f]; F.hfa = S.cur[4 * SS + f]; F.hfw = S.cur[5 * SS + f];
-        S.r[f] = F.r; S.h[f] = F.h; S.mass[f] = F.mass; S.inertia[f] = F.inertia;
-        S.hfa[f] = F.hfa; S.hfw[f] = F.hfw;
     }
-    int wi = S.ovw[f];
-    const WallOut W = wi >= 0 ? S.wtab[S.members[m].first + wi] : S.wout[f];
+    int wi = S.ovw[k];
+    WallOut W;
+    if (wi >= 0) {
+        W = S.wtab[S.members[m].first + wi];
+    } else if (S.wflag[k]) {
+        W = S.wout[k];
+    } else {
+        // no wall touched: reset_boundary(i, 0) cleared every wall history
+        W.fx_bc = W.fy_bc = W.m_bc = W.ta_bc = W.p_bc = 0.0;
+        W.deltat_bc1 = W.deltat_bc2 = W.theta_bc1 = W.theta_bc2 = 0.0;
+    }
     S.wbc[0 * SS + f] = W.deltat_bc1; S.wbc[1 * SS + f] = W.deltat_bc2;
     S.wbc[2 * SS + f] = W.theta_bc1; S.wbc[3 * SS + f] = W.theta_bc2;
     double a[NACC];
+    if (S.cls[k]) {
 #pragma unroll
-    for (int k = 0; k < NACC; ++k) a[k] = S.acc[k * SS + f];
+        for (int c = 0; c < NACC; ++c) a[c] = S.acc[c * SS + k];
+    } else {
+#pragma unroll
+        for (int c = 0; c < NACC; ++c) a[c] = 0.0;
+        a[A_SHA] = S.acc[A_SHA * SS + k];
+        a[A_SHW] = S.acc[A_SHW * SS + k];
+    }
     // shelter minima (reset_shelter + thread reduction); with shelter off
     // the reference's minima stay at their zero initial value
     double sha = P.shelter ? a[A_SHA] : 0.0;
     double shw = P.shelter ? a[A_SHW] : 0.0;
+    const double2 w = S.wind[k];
     Forcing Fo;
-    forcing<MIXED>(P, F, S.uai[f], S.vai[f], P.uw, P.vw, sha, shw, Fo);
+    forcing<MIXED>(P, F, w.x, w.y, P.uw, P.vw, sha, shw, Fo);
     double tfx = a[A_FCX] + a[A_FBX] + Fo.fax + Fo.fwx + Fo.fcorx + W.fx_bc;
     double tfy = a[A_FCY] + a[A_FBY] + Fo.fay + Fo.fwy + Fo.fcory + W.fy_bc;
     double mt = a[A_MC] + a[A_MB] + Fo.ma + Fo.mw + W.m_bc;
     double u = F.u + (tfx / F.mass) * P.dt;
     double v = F.v + (tfy / F.mass) * P.dt;
     double om = F.omega + (mt / F.inertia) * P.dt;
-    S.u[f] = u; S.v[f] = v; S.omega[f] = om;
-    S.x[f] = F.x + u * P.dt;
-    S.y[f] = F.y + v * P.dt;
-    S.theta[f] = S.theta[f] + om * P.dt;
+    FloeRec out;
+    out.x = F.x + u * P.dt;
+    out.y = F.y + v * P.dt;
+    out.u = u; out.v = v; out.omega = om;
+    out.r = F.r; out.h = F.h; out.mass = F.mass; out.inertia = F.inertia; out.hfa = F.hfa; out.hfw = F.hfw;
+    out.theta = S.rec[f].theta + om * P.dt;
+    S.rec[f] = out;
     if (S.record_forces) {
         double tp = (a[A_TAC] != 0.0 ? a[A_PC] / a[A_TAC] : 0.0)
                   + (a[A_TAB] != 0.0 ? a[A_PB] / a[A_TAB] : 0.0)
@@ -685,9 +851,44 @@ __global__ void k_final(DS S) {
     if (f == S.members[m].first) S.members[m].tstep += 1;
 }
 
+// per-member totals of the last step: candidates, contacts (evf flags) and
+// current bonds; diagnostics only
+__global__ void k_totals(DS S, int64_t* out) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t c = 0, ct = 0, bd = 0;
+    if (k < S.N && S.srec[k].member >= 0) {
+        int no = S.ocnt[k];
+        for (int s = 0; s < no; ++s) {
+            uint32_t fl = S.evf[(int64_t)s * S.S + k];
+            if (fl & EV_REMOVED) continue;
+            ++c;
+            if (fl & EV_CONTACT) ++ct;
+        }
+        int f = S.srec[k].id;
+        int nb = S.bcnt_old[f];
+        for (int q = 0; q < nb; ++q) {
+            const BondRec& b = S.bond_old[(int64_t)q * S.S + f];
+            if (b.j >= 0 && b.bond == 1) ++bd;
+        }
+    }
+    // wavefront reduction, one atomic per wave
+    for (int off = 32; off > 0; off >>= 1) {
+        c += __shfl_down(c, off);
+        ct += __shfl_down(ct, off);
+        bd += __shfl_down(bd, off);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd((unsigned long long*)&out[0], (unsigned long long)c);
+        atomicAdd((unsigned long long*)&out[1], (unsigned long long)ct);
+        atomicAdd((unsigned long long*)&out[2], (unsigned long long)bd);
+    }
+}
+
 // explicit instantiations used by the host launcher
-template __global__ void k_spec<false>(DS);
-template __global__ void k_spec<true>(DS);
+template __global__ void k_fast<false>(DS);
+template __global__ void k_fast<true>(DS);
+template __global__ void k_heavy<false>(DS);
+template __global__ void k_heavy<true>(DS);
 template __global__ void k_repair<false>(DS, int);
 template __global__ void k_repair<true>(DS, int);
 template __global__ void k_fixup<false>(DS);

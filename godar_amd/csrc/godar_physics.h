@@ -95,144 +95,147 @@ __device__ __forceinline__ double delt_ridge_of(double ri, double rj, double dis
     return 2.0 * sqrt(ri * ri - t * t);
 }
 
-// One pair event (i < j), the body of stepper.f90:89-271 for one (j,i).
-// fi/fj are updated in place if ridging reshapes one of them.  ps is updated
-// to the pair's new state.  Returns the gather terms in `o`.
-template <bool MIXED>
-__device__ __forceinline__ void pair_event(const DevParams& P, int tstep, int gi, int gj,
-                                           Floe& fi, Floe& fj, double uai, double vai,
-                                           double uwi, double vwi, PairState& ps,
-                                           EventOut& o, int side = 0) {
-    using T = typename Prec<MIXED>::T;
-    o.flags = 0;
-    o.ridged = -1;
-    // rel_pos_vel, physics.f90:73-114
-    double dx = fi.x - fj.x, dy = fi.y - fj.y;
-    double dist = sqrt(dx * dx + dy * dy);
-    double cosa = (fj.x - fi.x) / dist;
-    double sina = (fj.y - fi.y) / dist;
-    double omegarel = fj.omega - fi.omega;
-    double veln = (fj.u - fi.u) * cosa + (fj.v - fi.v) * sina;
-    double veltb = -(fj.u - fi.u) * sina + (fj.v - fi.v) * cosa;
-    double velt = veltb - (fi.omega * fi.r + fj.omega * fj.r);
-    double deltan = fi.r + fj.r - dist;
+// Pair kinematics, rel_pos_vel (physics.f90:73-114); alpha is never read.
+struct Kin {
+    double dist, cosa, sina, omegarel, veln, veltb, velt, deltan;
+};
 
-    // step-1 bonds, stepper.f90:93-101, bond_properties bonds.f90:237-268
+__device__ __forceinline__ void kinematics(const Floe& fi, const Floe& fj, Kin& k) {
+    double dx = fi.x - fj.x, dy = fi.y - fj.y;
+    k.dist = sqrt(dx * dx + dy * dy);
+    k.cosa = (fj.x - fi.x) / k.dist;
+    k.sina = (fj.y - fi.y) / k.dist;
+    k.omegarel = fj.omega - fi.omega;
+    k.veln = (fj.u - fi.u) * k.cosa + (fj.v - fi.v) * k.sina;
+    k.veltb = -(fj.u - fi.u) * k.sina + (fj.v - fi.v) * k.cosa;
+    k.velt = k.veltb - (fi.omega * fi.r + fj.omega * fj.r);
+    k.deltan = fi.r + fj.r - k.dist;
+}
+
+// Step-1 bonds, stepper.f90:93-101 (+ the bond mask of the workloads) and
+// bond_properties, bonds.f90:237-268.
+__device__ __forceinline__ void bond_create(const DevParams& P, int tstep, int gi, int gj,
+                                            const Floe& fi, const Floe& fj, const Kin& k,
+                                            PairState& ps) {
     if (tstep == 1 && P.restart != 1 && P.cohesion) {
-        if (deltan >= -P.bond_lim && bond_keep(gi, gj, P.bond_seed, P.bond_thr24)) {
+        if (k.deltan >= -P.bond_lim && bond_keep(gi, gj, P.bond_seed, P.bond_thr24)) {
             ps.bond = 1;
             ps.has_bond_rec = true;
             ps.damage = 0.0;
             ps.rb = P.lambda_rb * fmin(fi.r, fj.r);
             ps.hb = (fi.h + fj.h) / 2.0;
-            ps.lb = dist;
+            ps.lb = k.dist;
             ps.sb = 2.0 * ps.rb * ps.hb;
             ps.ib = 2.0 / 3.0 * ps.hb * (ps.rb * ps.rb * ps.rb);
         }
     }
+}
 
-    T fcn_t = T(0), fct_t = T(0);
-    if (deltan > 0) {
-        o.flags |= EV_CONTACT;
-        // contact_forces, contact_forces.f90:1-87
-        ps.deltat = -velt * P.dt + ps.deltat;
-        double delt_ridge = delt_ridge_of(fi.r, fj.r, dist);
-        ps.thetarelc = -omegarel * P.dt + ps.thetarelc;
-        T m_redu = T(fi.mass) * T(fj.mass) / (T(fi.mass) + T(fj.mass));
-        T r_redu = T(fi.r) * T(fj.r) / (T(fi.r) + T(fj.r));
-        T hmin = dmin(T(fi.h), T(fj.h));
-        T knc = T(P.pi_ec) * hmin * fit(T(deltan) * r_redu / (T(2) * (hmin * hmin)));
-        T ktc = T(P.k6) * knc;
-        T dr = T(delt_ridge);
-        T dr2 = dr * dr;
-        T krc = knc * dr2 / T(12);
-        T gamn = -(T(P.beta) * gsqrt(T(4) * knc * m_redu));
-        T gamt = -(T(P.two_beta) * gsqrt(T(2.0 / 3.0) * ktc * m_redu));
-        T gamr = gamn * dr2 / T(12);
-        T fcn = dmax(knc * T(deltan) - gamn * T(veln), T(0));
-        T fct = ktc * T(ps.deltat) - gamt * T(velt);
-        if (P.ridging) {
-            if (T(P.sigmanc_crit) * hmin <= fcn / dr / hmin) {
-                // plastic_contact, ridging.f90:1-33
-                T h2 = hmin * hmin;
-                T kp = T(P.sigmanc_crit) * h2 * dr / T(deltan);
-                ktc = T(P.k6) * kp;
-                krc = kp * dr2 / T(12);
-                gamn = -(T(P.beta) * gsqrt(T(4) * kp * m_redu));
-                gamt = -(T(P.two_beta) * gsqrt(T(2.0 / 3.0) * ktc * m_redu));
-                gamr = gamn * dr2 / T(12);
-                fcn = dmin(fcn, T(P.sigmanc_crit) * h2 * dr);
-                fct = ktc * T(ps.deltat) - gamt * T(velt);
-                // update_shape, ridging.f90:36-99 (FP64 volume bookkeeping)
-                {
-                    double hmin64 = fmin(fi.h, fj.h);
-                    double ri = fi.r, rj = fj.r;
-                    double dij = (dist * dist - rj * rj + ri * ri) / (2.0 * dist);
-                    double dji = (dist * dist - ri * ri + rj * rj) / (2.0 * dist);
-                    double area = ri * ri * acos(dij / ri) - dij * delt_ridge / 2.0
-                                + rj * rj * acos(dji / rj) - dji * delt_ridge / 2.0;
-                    double vol = fmax(area * hmin64 / 2.0, 0.0);
-                    Floe* w = nullptr;
-                    int who = -1;
-                    if (hmin64 == fi.h) { w = &fi; who = 0; }
-                    else if (hmin64 == fj.h) { w = &fj; who = 1; }
-                    if (w) {
-                        double rw = w->r, hw = w->h;
-                        double dh = vol / (P.pi * (rw * rw));
-                        w->r = rw * sqrt(hw / (hw + dh));
-                        w->h = hw + dh;
-                        floe_properties(P, *w);
-                        o.flags |= EV_RIDGED;
-                        o.ridged = who;
-                        o.nr = w->r; o.nh = w->h; o.nmass = w->mass;
-                        o.ninertia = w->inertia; o.nhfa = w->hfa; o.nhfw = w->hfw;
-                    }
-                    delt_ridge = delt_ridge_of(fi.r, fj.r, dist);
-                    dr = T(delt_ridge);
-                }
+// Contact of a pair with deltan > 0: contact_forces (contact_forces.f90:1-87)
+// with plastic_contact/update_shape (ridging.f90:1-99) and coulomb
+// (physics.f90:1-33).  fi/fj are reshaped in place when ridging occurs.
+template <bool MIXED>
+__device__ __forceinline__ void contact_part(const DevParams& P, Floe& fi, Floe& fj, const Kin& k,
+                                             PairState& ps, EventOut& o) {
+    using T = typename Prec<MIXED>::T;
+    o.flags |= EV_CONTACT;
+    const double dist = k.dist;
+    ps.deltat = -k.velt * P.dt + ps.deltat;
+    double delt_ridge = delt_ridge_of(fi.r, fj.r, dist);
+    ps.thetarelc = -k.omegarel * P.dt + ps.thetarelc;
+    T m_redu = T(fi.mass) * T(fj.mass) / (T(fi.mass) + T(fj.mass));
+    T r_redu = T(fi.r) * T(fj.r) / (T(fi.r) + T(fj.r));
+    T hmin = dmin(T(fi.h), T(fj.h));
+    T knc = T(P.pi_ec) * hmin * fit(T(k.deltan) * r_redu / (T(2) * (hmin * hmin)));
+    T ktc = T(P.k6) * knc;
+    T dr = T(delt_ridge);
+    T dr2 = dr * dr;
+    T krc = knc * dr2 / T(12);
+    T gamn = -(T(P.beta) * gsqrt(T(4) * knc * m_redu));
+    T gamt = -(T(P.two_beta) * gsqrt(T(2.0 / 3.0) * ktc * m_redu));
+    T gamr = gamn * dr2 / T(12);
+    T fcn = dmax(knc * T(k.deltan) - gamn * T(k.veln), T(0));
+    T fct = ktc * T(ps.deltat) - gamt * T(k.velt);
+    if (P.ridging) {
+        if (T(P.sigmanc_crit) * hmin <= fcn / dr / hmin) {
+            // plastic_contact, ridging.f90:1-33
+            T h2 = hmin * hmin;
+            T kp = T(P.sigmanc_crit) * h2 * dr / T(k.deltan);
+            ktc = T(P.k6) * kp;
+            krc = kp * dr2 / T(12);
+            gamn = -(T(P.beta) * gsqrt(T(4) * kp * m_redu));
+            gamt = -(T(P.two_beta) * gsqrt(T(2.0 / 3.0) * ktc * m_redu));
+            gamr = gamn * dr2 / T(12);
+            fcn = dmin(fcn, T(P.sigmanc_crit) * h2 * dr);
+            fct = ktc * T(ps.deltat) - gamt * T(k.velt);
+            // update_shape, ridging.f90:36-99 (FP64 volume bookkeeping)
+            double hmin64 = fmin(fi.h, fj.h);
+            double ri = fi.r, rj = fj.r;
+            double dij = (dist * dist - rj * rj + ri * ri) / (2.0 * dist);
+            double dji = (dist * dist - ri * ri + rj * rj) / (2.0 * dist);
+            double area = ri * ri * acos(dij / ri) - dij * delt_ridge / 2.0
+                        + rj * rj * acos(dji / rj) - dji * delt_ridge / 2.0;
+            double vol = fmax(area * hmin64 / 2.0, 0.0);
+            Floe* w = nullptr;
+            int who = -1;
+            if (hmin64 == fi.h) { w = &fi; who = 0; }
+            else if (hmin64 == fj.h) { w = &fj; who = 1; }
+            if (w) {
+                double rw = w->r, hw = w->h;
+                double dh = vol / (P.pi * (rw * rw));
+                w->r = rw * sqrt(hw / (hw + dh));
+                w->h = hw + dh;
+                floe_properties(P, *w);
+                o.flags |= EV_RIDGED;
+                o.ridged = who;
+                o.nr = w->r; o.nh = w->h; o.nmass = w->mass;
+                o.ninertia = w->inertia; o.nhfa = w->hfa; o.nhfw = w->hfw;
             }
+            delt_ridge = delt_ridge_of(fi.r, fj.r, dist);
+            dr = T(delt_ridge);
         }
-        // coulomb, physics.f90:1-33
-        T fcr;
-        if (gabs(fct) > T(P.friction_coeff) * gabs(fcn)) {
-            fct = -(T(P.friction_coeff) * gabs(fcn) * gcopysign(T(1), T(velt)));
-            fcr = fct;
-            if (P.slipping) ps.deltat = double((fct + gamt * T(velt)) / ktc);
-        } else {
-            fcr = fct;
-        }
-        // rolling moment (bonded pairs keep mcc = 0, see header note)
-        T mcc = T(0);
-        if (ps.bond == 0) {
-            T mrolling = krc * T(ps.thetarelc) - gamr * T(omegarel);
-            if (gabs(T(ps.thetarelc)) > T(2) * gabs(fcn) / knc / dr) mrolling = T(0);
-            mcc = mrolling;
-        }
-        // stepper.f90:113-146 terms
-        T c = T(cosa), s = T(sina);
-        o.fcn_c = double(fcn * c);
-        o.fcr_s = double(fcr * s);
-        o.fcn_s = double(fcn * s);
-        o.fcr_c = double(fcr * c);
-        o.ri_fct = double(T(fi.r) * fct);
-        o.rj_fct = double(T(fj.r) * fct);
-        o.mcc = double(mcc);
-        double ac = delt_ridge * fmin(fi.h, fj.h);
-        o.ac = ac;
-        o.fcn_ac = double(fcn) * ac;
-        fcn_t = fcn;
-        fct_t = fct;
-    } else {
-        // reset_contact, reset.f90:1-15
-        ps.thetarelc = 0.0;
-        ps.deltat = 0.0;
     }
-    (void)fcn_t; (void)fct_t;
+    // coulomb, physics.f90:1-33
+    T fcr;
+    if (gabs(fct) > T(P.friction_coeff) * gabs(fcn)) {
+        fct = -(T(P.friction_coeff) * gabs(fcn) * gcopysign(T(1), T(k.velt)));
+        fcr = fct;
+        if (P.slipping) ps.deltat = double((fct + gamt * T(k.velt)) / ktc);
+    } else {
+        fcr = fct;
+    }
+    // rolling moment: bonds only form in step 1, before any contact of the
+    // pair, so a bonded pair's mcc is still its initial zero
+    T mcc = T(0);
+    if (ps.bond == 0) {
+        T mrolling = krc * T(ps.thetarelc) - gamr * T(k.omegarel);
+        if (gabs(T(ps.thetarelc)) > T(2) * gabs(fcn) / knc / dr) mrolling = T(0);
+        mcc = mrolling;
+    }
+    // stepper.f90:113-146 terms
+    T c = T(k.cosa), s = T(k.sina);
+    o.fcn_c = double(fcn * c);
+    o.fcr_s = double(fcr * s);
+    o.fcn_s = double(fcn * s);
+    o.fcr_c = double(fcr * c);
+    o.ri_fct = double(T(fi.r) * fct);
+    o.rj_fct = double(T(fj.r) * fct);
+    o.mcc = double(mcc);
+    double ac = delt_ridge * fmin(fi.h, fj.h);
+    o.ac = ac;
+    o.fcn_ac = double(fcn) * ac;
+}
 
+// Bond of a pair: bond_forces_euler (bonds.f90:74-161) and bond_breaking
+// (:164-212) when bonded; reset_bond (reset.f90:18-34) when a record exists
+// but the bond is broken.  Strain accumulators and damage stay FP64.
+template <bool MIXED>
+__device__ __forceinline__ void bond_part(const DevParams& P, const Floe& fi, const Floe& fj,
+                                          const Kin& k, PairState& ps, EventOut& o) {
+    using T = typename Prec<MIXED>::T;
     if (ps.bond == 1) {
-        // bond_forces_euler, bonds.f90:74-161 (strain accumulators FP64)
-        ps.deltanb = veln * P.dt + ps.deltanb;
-        ps.deltatb = veltb * P.dt + ps.deltatb;
+        ps.deltanb = k.veln * P.dt + ps.deltanb;
+        ps.deltatb = k.veltb * P.dt + ps.deltatb;
         ps.trb_ji = fi.omega * P.dt + ps.trb_ji;
         ps.trb_ij = fj.omega * P.dt + ps.trb_ij;
         T m_redu = T(fi.mass) * T(fj.mass) / (T(fi.mass) + T(fj.mass));
@@ -254,8 +257,8 @@ __device__ __forceinline__ void pair_event(const DevParams& P, int tstep, int gi
         T gam_rot1 = T(4) * eta_i / L;
         T gam_rot2 = T(2) * eta_i / L;
         T oi = T(fi.omega), oj = T(fj.omega);
-        T tji = T(ps.trb_ji), tij = T(ps.trb_ij), dtb = T(ps.deltatb), vtb = T(veltb);
-        T fbn = k_axial * T(ps.deltanb) + gam_axial * T(veln);
+        T tji = T(ps.trb_ji), tij = T(ps.trb_ij), dtb = T(ps.deltatb), vtb = T(k.veltb);
+        T fbn = k_axial * T(ps.deltanb) + gam_axial * T(k.veln);
         T fbt = k_shear1 * dtb + k_shear2 * (tji + tij) + gam_shear1 * vtb + gam_shear2 * (oi + oj);
         T mbb_ji = k_rot1 * tji + k_rot2 * tij - k_shear2 * dtb + gam_rot1 * oi + gam_rot2 * oj
                  - gam_shear2 * vtb;
@@ -281,7 +284,7 @@ __device__ __forceinline__ void pair_event(const DevParams& P, int tstep, int gi
         }
         if (ps.bond == 1) {
             o.flags |= EV_BONDED;
-            T c = T(cosa), s = T(sina);
+            T c = T(k.cosa), s = T(k.sina);
             o.fbn_c = double(fbn * c);
             o.fbt_s = double(fbt * s);
             o.fbn_s = double(fbn * s);
@@ -294,24 +297,52 @@ __device__ __forceinline__ void pair_event(const DevParams& P, int tstep, int gi
             o.fbn_sb = double(fbn) * ps.sb;
         }
     } else if (ps.has_bond_rec) {
-        // reset_bond, reset.f90:18-34
         ps.trb_ji = ps.trb_ij = ps.deltanb = ps.deltatb = 0.0;
     }
+}
 
-    if (P.shelter) {
-        // sheltering, forcing.f90:162-186 (wind of the outer floe i)
-        o.flags |= EV_SHELTER;
-        T dn = T(deltan), c = T(cosa), s = T(sina);
-        // side 1: only floe i's coefficients are needed; side 2: only floe j's
-        if (side != 1) {
-            o.hsfa_ij = double(S_shelter(T(fi.hfa), T(fj.hfa), dn, c, s, T(uai), T(vai)));
-            o.hsfw_ij = double(S_shelter(T(fi.hfw), T(fj.hfw), dn, c, s, T(uwi), T(vwi)));
-        }
-        if (side != 2) {
-            o.hsfa_ji = double(S_shelter(T(fj.hfa), T(fi.hfa), dn, -c, -s, T(uai), T(vai)));
-            o.hsfw_ji = double(S_shelter(T(fj.hfw), T(fi.hfw), dn, -c, -s, T(uwi), T(vwi)));
-        }
+// Sheltering, forcing.f90:162-186 (wind of the outer floe i).  side 1: only
+// floe i's coefficients hsf*(j,i); side 2: only floe j's hsf*(i,j); 0: both.
+template <bool MIXED>
+__device__ __forceinline__ void shelter_part(const DevParams& P, const Floe& fi, const Floe& fj,
+                                             const Kin& k, double uai, double vai, double uwi,
+                                             double vwi, EventOut& o, int side) {
+    using T = typename Prec<MIXED>::T;
+    if (!P.shelter) return;
+    o.flags |= EV_SHELTER;
+    T dn = T(k.deltan), c = T(k.cosa), s = T(k.sina);
+    if (side != 1) {
+        o.hsfa_ij = double(S_shelter(T(fi.hfa), T(fj.hfa), dn, c, s, T(uai), T(vai)));
+        o.hsfw_ij = double(S_shelter(T(fi.hfw), T(fj.hfw), dn, c, s, T(uwi), T(vwi)));
     }
+    if (side != 2) {
+        o.hsfa_ji = double(S_shelter(T(fj.hfa), T(fi.hfa), dn, -c, -s, T(uai), T(vai)));
+        o.hsfw_ji = double(S_shelter(T(fj.hfw), T(fi.hfw), dn, -c, -s, T(uwi), T(vwi)));
+    }
+}
+
+// One pair event (i < j), the body of stepper.f90:89-271 for one (j,i).
+// fi/fj are updated in place if ridging reshapes one of them.  ps is updated
+// to the pair's new state.  Returns the gather terms in `o`.
+template <bool MIXED>
+__device__ __forceinline__ void pair_event(const DevParams& P, int tstep, int gi, int gj,
+                                           Floe& fi, Floe& fj, double uai, double vai,
+                                           double uwi, double vwi, PairState& ps,
+                                           EventOut& o, int side = 0) {
+    o.flags = 0;
+    o.ridged = -1;
+    Kin k;
+    kinematics(fi, fj, k);
+    bond_create(P, tstep, gi, gj, fi, fj, k, ps);
+    if (k.deltan > 0) {
+        contact_part<MIXED>(P, fi, fj, k, ps, o);
+    } else {
+        // reset_contact, reset.f90:1-15
+        ps.thetarelc = 0.0;
+        ps.deltat = 0.0;
+    }
+    bond_part<MIXED>(P, fi, fj, k, ps, o);
+    shelter_part<MIXED>(P, fi, fj, k, uai, vai, uwi, vwi, o, side);
 }
 
 // Walls of one floe: verify_bc (boundaries.f90:1-97) -> check_wall (:100-146)

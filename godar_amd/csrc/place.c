@@ -13,6 +13,7 @@
  *    violation plus a quarter of the gap, positions clamped to the box) then
  *    removes every overlap.
  * Deterministic for a given seed (splitmix64 stream).
+ * godar_relax_disks runs step 2 alone from caller-given centres.
  */
 #include <math.h>
 #include <stdint.h>
@@ -86,6 +87,9 @@ static int clear_of(level_t *lev, int nlev, const double *px, const double *py, 
     return 1;
 }
 
+static int relax(int n, const double *r, double box, double gap, int max_relax, level_t *lev,
+                 int nlev, const int *lvl, double *x, double *y);
+
 int godar_place_disks(int n, const double *r, double box, double gap, uint64_t seed,
                       int max_tries, int max_relax, double *x, double *y, int *stats) {
     if (n <= 0 || box <= 0) return 1;
@@ -134,8 +138,19 @@ int godar_place_disks(int n, const double *r, double box, double gap, uint64_t s
         level_insert(&lev[lvl[i]], i, px, py);
     }
     int it = 0;
-    if (failed) {
-        /* push-apart relaxation over every overlapping pair */
+    if (failed) it = relax(n, r, box, gap, max_relax, lev, nlev, lvl, x, y);
+    if (stats) { stats[0] = failed; stats[1] = it; }
+    for (int l = 0; l < nlev; ++l) { free(lev[l].head); free(lev[l].next); }
+    free(order); free(lvl); free(pr);
+    return (failed && it >= max_relax) ? 3 : 0;
+}
+
+/* push-apart relaxation over every overlapping pair; returns the sweeps used
+ * (max_relax if overlaps remain) */
+static int relax(int n, const double *r, double box, double gap, int max_relax, level_t *lev,
+                 int nlev, const int *lvl, double *x, double *y) {
+    int it = 0;
+    {
         double *dx = calloc(n, sizeof(double)), *dy = calloc(n, sizeof(double));
         for (it = 0; it < max_relax; ++it) {
             for (int l = 0; l < nlev; ++l) {
@@ -182,8 +197,52 @@ int godar_place_disks(int n, const double *r, double box, double gap, uint64_t s
         free(dx);
         free(dy);
     }
-    if (stats) { stats[0] = failed; stats[1] = it; }
+    return it;
+}
+
+static int build_levels(int n, const double *r, double box, double gap, level_t *lev, int **lvl_out) {
+    double rmax = 0, rmin = INFINITY;
+    for (int i = 0; i < n; ++i) {
+        if (r[i] > rmax) rmax = r[i];
+        if (r[i] < rmin) rmin = r[i];
+    }
+    int nlev = 1;
+    while (nlev < 12 && rmax / (1 << nlev) >= rmin) ++nlev;
+    for (int l = 0; l < nlev; ++l) {
+        lev[l].rcap = rmax / (double)(1 << l);
+        lev[l].cs = 2.0 * lev[l].rcap + gap;
+        long g = (long)ceil(box / lev[l].cs);
+        if (g < 1) g = 1;
+        while (g * g > 8L * n + 64 && g > 1) { g = (g + 1) / 2; lev[l].cs *= 2; }
+        lev[l].g = (int)g;
+        lev[l].head = malloc(sizeof(int) * g * g);
+        lev[l].next = malloc(sizeof(int) * n);
+        for (long c = 0; c < g * g; ++c) lev[l].head[c] = -1;
+    }
+    int *lvl = malloc(sizeof(int) * n);
+    for (int i = 0; i < n; ++i) lvl[i] = level_index(r[i], rmax, nlev);
+    *lvl_out = lvl;
+    return nlev;
+}
+
+/* Overlap removal only, from the caller's initial centres (e.g. a jittered
+ * grid); centres are clamped into [r+gap, box-r-gap] first. */
+int godar_relax_disks(int n, const double *r, double box, double gap, int max_relax, double *x,
+                      double *y, int *sweeps) {
+    if (n <= 0 || box <= 0) return 1;
+    for (int i = 0; i < n; ++i) {
+        if (!(r[i] > 0)) return 1;
+        if (2 * (r[i] + gap) >= box) return 2;
+        double lo = r[i] + gap, hi = box - r[i] - gap;
+        x[i] = fmin(fmax(x[i], lo), hi);
+        y[i] = fmin(fmax(y[i], lo), hi);
+    }
+    level_t lev[12];
+    int *lvl;
+    int nlev = build_levels(n, r, box, gap, lev, &lvl);
+    int it = relax(n, r, box, gap, max_relax, lev, nlev, lvl, x, y);
+    if (sweeps) *sweeps = it;
     for (int l = 0; l < nlev; ++l) { free(lev[l].head); free(lev[l].next); }
-    free(order); free(lvl); free(pr);
-    return (failed && it >= max_relax) ? 3 : 0;
+    free(lvl);
+    return it >= max_relax ? 3 : 0;
 }

@@ -12,6 +12,7 @@ struct RadixScratch {
     uint32_t* keys_alt = nullptr;
     uint32_t* vals_alt = nullptr;
     uint32_t* hist = nullptr;       // 256 * max_tiles
+    uint32_t* dtot = nullptr;       // 256 digit totals
     int64_t capacity = 0;
     int max_tiles = 0;
 };

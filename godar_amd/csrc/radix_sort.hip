@@ -23,30 +23,33 @@ __global__ __launch_bounds__(RS_THREADS) void rs_hist_kernel(const uint32_t* __r
     hist[(int64_t)t * ntiles + blockIdx.x] = h[t];
 }
 
-// Exclusive scan of m = 256 * ntiles counters by one workgroup.
-__global__ __launch_bounds__(1024) void rs_scan_kernel(uint32_t* __restrict__ hist, int64_t m,
-                                                       const uint32_t* err) {
+// Exclusive scan of each digit's row of per-tile counts (one workgroup per
+// digit); the row total goes to dtot[digit].
+__global__ __launch_bounds__(256) void rs_scan_kernel(uint32_t* __restrict__ hist, int ntiles,
+                                                      uint32_t* __restrict__ dtot, const uint32_t* err) {
     if (err && *err) return;
-    __shared__ uint32_t part[1024];
-    int t = threadIdx.x;
-    int64_t per = (m + 1023) / 1024;
-    int64_t lo = t * per, hi = lo + per < m ? lo + per : m;
+    __shared__ uint32_t part[256];
+    const int t = threadIdx.x;
+    uint32_t* row = hist + (int64_t)blockIdx.x * ntiles;
+    const int per = (ntiles + 255) / 256;
+    const int lo = t * per, hi = lo + per < ntiles ? lo + per : ntiles;
     uint32_t sum = 0;
-    for (int64_t k = lo; k < hi; ++k) sum += hist[k];
+    for (int k = lo; k < hi; ++k) sum += row[k];
     part[t] = sum;
     __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
+    for (int off = 1; off < 256; off <<= 1) {
         uint32_t v = t >= off ? part[t - off] : 0u;
         __syncthreads();
         part[t] += v;
         __syncthreads();
     }
     uint32_t run = t ? part[t - 1] : 0u;
-    for (int64_t k = lo; k < hi; ++k) {
-        uint32_t v = hist[k];
-        hist[k] = run;
+    for (int k = lo; k < hi; ++k) {
+        uint32_t v = row[k];
+        row[k] = run;
         run += v;
     }
+    if (t == 255) dtot[blockIdx.x] = part[255];
 }
 
 // Stable scatter.  Items of a tile are visited in rounds of 256 (round r
@@ -55,14 +58,25 @@ __global__ __launch_bounds__(1024) void rs_scan_kernel(uint32_t* __restrict__ hi
 __global__ __launch_bounds__(RS_THREADS) void rs_scatter_kernel(
     const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin, uint32_t* __restrict__ kout,
     uint32_t* __restrict__ vout, int64_t n, int shift, int ntiles, const uint32_t* __restrict__ hist,
-    const uint32_t* err) {
+    const uint32_t* __restrict__ dtot, const uint32_t* err) {
     if (err && *err) return;
     constexpr int NW = RS_THREADS / 64;
     __shared__ uint32_t base_off[256];
     __shared__ uint32_t wcount[NW][256];
     int t = threadIdx.x;
     int w = t >> 6, lane = t & 63;
-    base_off[t] = hist[(int64_t)t * ntiles + blockIdx.x];
+    // digit bases: exclusive scan of the digit totals
+    base_off[t] = dtot[t];
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        uint32_t v = t >= off ? base_off[t - off] : 0u;
+        __syncthreads();
+        base_off[t] += v;
+        __syncthreads();
+    }
+    uint32_t excl = t ? base_off[t - 1] : 0u;
+    __syncthreads();
+    base_off[t] = excl + hist[(int64_t)t * ntiles + blockIdx.x];
     const uint64_t lt_mask = lane ? ((~0ull) >> (64 - lane)) : 0ull;
     int64_t base = (int64_t)blockIdx.x * RS_TILE;
     for (int r = 0; r < RS_IPT; ++r) {
@@ -111,6 +125,7 @@ hipError_t radix_alloc(RadixScratch& s, int64_t n) {
     if ((e = hipMalloc(&s.keys_alt, cap * sizeof(uint32_t))) != hipSuccess) return e;
     if ((e = hipMalloc(&s.vals_alt, cap * sizeof(uint32_t))) != hipSuccess) return e;
     if ((e = hipMalloc(&s.hist, (int64_t)256 * tiles * sizeof(uint32_t))) != hipSuccess) return e;
+    if ((e = hipMalloc(&s.dtot, 256 * sizeof(uint32_t))) != hipSuccess) return e;
     s.capacity = cap;
     s.max_tiles = tiles;
     return hipSuccess;
@@ -120,6 +135,7 @@ void radix_free(RadixScratch& s) {
     if (s.keys_alt) (void)hipFree(s.keys_alt);
     if (s.vals_alt) (void)hipFree(s.vals_alt);
     if (s.hist) (void)hipFree(s.hist);
+    if (s.dtot) (void)hipFree(s.dtot);
     s = RadixScratch{};
 }
 
@@ -135,10 +151,10 @@ hipError_t radix_sort_pairs(RadixScratch& s, uint32_t* keys, uint32_t* vals, int
         int shift = ps * 8;
         hipLaunchKernelGGL(rs_hist_kernel, dim3(ntiles), dim3(RS_THREADS), 0, stream, ka, n, shift,
                            ntiles, s.hist, err);
-        hipLaunchKernelGGL(rs_scan_kernel, dim3(1), dim3(1024), 0, stream, s.hist,
-                           (int64_t)256 * ntiles, err);
+        hipLaunchKernelGGL(rs_scan_kernel, dim3(256), dim3(256), 0, stream, s.hist, ntiles, s.dtot,
+                           err);
         hipLaunchKernelGGL(rs_scatter_kernel, dim3(ntiles), dim3(RS_THREADS), 0, stream, ka, va, kb,
-                           vb, n, shift, ntiles, s.hist, err);
+                           vb, n, shift, ntiles, s.hist, s.dtot, err);
         uint32_t* tk = ka; ka = kb; kb = tk;
         uint32_t* tv = va; va = vb; vb = tv;
     }

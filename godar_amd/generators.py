@@ -283,6 +283,25 @@ def place_disks(r, box, seed, gap=1.0, max_tries=400, max_relax=20000):
     return x, y, dict(rsa_failed=int(st[0]), relax_iters=int(st[1]))
 
 
+def relax_disks(x, y, r, box, gap=1.0, max_relax=20000):
+    """Compiled overlap removal from given centres (csrc/place.c)."""
+    import ctypes
+    L = _place_lib()
+    dp = ctypes.POINTER(ctypes.c_double)
+    L.godar_relax_disks.argtypes = [ctypes.c_int, dp, ctypes.c_double, ctypes.c_double,
+                                    ctypes.c_int, dp, dp, ctypes.POINTER(ctypes.c_int)]
+    L.godar_relax_disks.restype = ctypes.c_int
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    x = np.array(x, dtype=np.float64)
+    y = np.array(y, dtype=np.float64)
+    sw = ctypes.c_int()
+    rc = L.godar_relax_disks(r.size, r.ctypes.data_as(dp), float(box), float(gap), int(max_relax),
+                             x.ctypes.data_as(dp), y.ctypes.data_as(dp), ctypes.byref(sw))
+    if rc != 0:
+        raise RuntimeError(f"overlap relaxation failed (code {rc}); lower the packing")
+    return x, y, sw.value
+
+
 def make_field(n: int, seed: int, *, radius: str, r_lo: float, r_hi: float,
                h_median: float, h_sigma: float, packing: float,
                box: Optional[float] = None, power: float = -2.5,
@@ -290,8 +309,9 @@ def make_field(n: int, seed: int, *, radius: str, r_lo: float, r_hi: float,
                gap: float = 1.0, placement: str = "relax") -> FloeField:
     """Generic seeded field: radii, lognormal thickness, non-overlapping centres.
 
-    placement="relax": jittered grid + numpy relaxation; "rsa": compiled
-    largest-first random sequential addition (needed for large, dense fields).
+    placement="relax": jittered grid + numpy relaxation; "grid": the same with
+    the compiled relaxation; "rsa": compiled largest-first random sequential
+    addition (needed for large, dense fields).
     """
     rng = np.random.default_rng(seed)
     if radius == "uniform":
@@ -307,8 +327,8 @@ def make_field(n: int, seed: int, *, radius: str, r_lo: float, r_hi: float,
     area = float(np.sum(np.pi * r ** 2))
     if box is None:
         box = math.sqrt(area / packing)
-    if placement == "relax":
-        # jittered grid + push-apart relaxation (numpy; fine up to ~10^4 floes)
+    if placement in ("relax", "grid"):
+        # jittered grid + push-apart relaxation ("relax": numpy, "grid": compiled)
         m = int(math.ceil(math.sqrt(n)))
         spacing = box / m
         cells = rng.permutation(m * m)[:n]
@@ -316,7 +336,10 @@ def make_field(n: int, seed: int, *, radius: str, r_lo: float, r_hi: float,
         cy = (cells // m + 0.5) * spacing + rng.uniform(-0.25, 0.25, n) * spacing
         cx = np.clip(cx, r + gap, box - r - gap)
         cy = np.clip(cy, r + gap, box - r - gap)
-        x, y, iters = _relax_overlaps(cx, cy, r, box, gap)
+        if placement == "relax":
+            x, y, iters = _relax_overlaps(cx, cy, r, box, gap)
+        else:
+            x, y, iters = relax_disks(cx, cy, r, box, gap)
         pstats = dict(relax_iters=int(iters))
     else:
         x, y, pstats = place_disks(r, box, int(rng.integers(0, 2 ** 63)), gap)
@@ -349,7 +372,7 @@ def config0_member(seed: int, n: int) -> FloeField:
     packing of the 1,000-floe / 100 km field."""
     return make_field(n, seed, radius="uniform", r_lo=500.0, r_hi=2000.0,
                       h_median=1.0, h_sigma=0.3, packing=0.55, box=None,
-                      wind_speed=10.0, current_speed=0.1, placement="rsa")
+                      wind_speed=10.0, current_speed=0.1, placement="grid")
 
 
 def config1(seed: int = 1, n: int = 262144, packing: float = 0.7,

@@ -204,6 +204,12 @@ class Context:
         st = st[:cnt.value]
         return dict(zip(BOND_KEYS, st.T))
 
+    def totals(self) -> dict:
+        """Candidate pairs, contacts (last step) and bonded pairs over all members."""
+        t = np.zeros(3, dtype=np.int64)
+        self._chk(self.L.godar_get_totals(self.h, t.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))))
+        return dict(candidates=int(t[0]), contacts=int(t[1]), bonds=int(t[2]))
+
     def stats(self, m: int) -> dict:
         s = np.zeros(4, dtype=np.int64)
         self._chk(self.L.godar_get_stats(self.h, int(m), s.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))))

@@ -9,6 +9,7 @@ floes, so "touching" means a gap of at most 10 m).
 from __future__ import annotations
 
 import math
+import os
 
 import numpy as np
 
@@ -38,16 +39,26 @@ def config1(seed: int = 1, n: int = 262144):
     return f, params_for(f, dt=0.5, bond_frac=0.3, bond_seed=seed * 7919 + 1)
 
 
-def config3(seed: int = 3, members: int = 512, lo: int = 500, hi: int = 8000):
+def _member(args):
+    s, n = args
+    f = g.config0_member(s, n)
+    return f, params_for(f, bond_seed=s)
+
+
+def config3(seed: int = 3, members: int = 512, lo: int = 500, hi: int = 8000, workers: int = 0):
     """512 independent fields, floe counts log-uniform in [500, 8000], each the
     config-0 generator with its own seed and wind direction (box scaled to keep
-    the packing); mixed precision, 5,000 steps."""
+    the packing); mixed precision, 5,000 steps.  Members are generated in
+    parallel processes (deterministic: each depends on its own seed only)."""
     sizes = g.member_sizes(members, lo, hi, seed)
-    out = []
-    for k, n in enumerate(sizes):
-        f = g.config0_member(seed * 100003 + k, int(n))
-        out.append((f, params_for(f, bond_seed=seed * 100003 + k)))
-    return out
+    jobs = [(seed * 100003 + k, int(n)) for k, n in enumerate(sizes)]
+    if workers <= 0:
+        workers = min(16, os.cpu_count() or 1)
+    if workers > 1 and len(jobs) > 8:
+        import multiprocessing as mp
+        with mp.get_context("fork").Pool(workers) as pool:
+            return pool.map(_member, jobs, chunksize=4)
+    return [_member(j) for j in jobs]
 
 
 def config4_request(rng: np.random.Generator, seed: int):

@@ -152,6 +152,9 @@ int godar_get_bond_state(godar_ctx *ctx, int32_t member, int32_t cap, double *st
 /* Step statistics of the last step: [0] candidate pairs, [1] contacts,
  * [2] ridging seeds, [3] events re-evaluated by the ordered repair. */
 int godar_get_stats(godar_ctx *ctx, int32_t member, int64_t *stats);
+/* Totals over all members of the last step: [0] candidate pairs,
+ * [1] contacts, [2] bonded pairs. */
+int godar_get_totals(godar_ctx *ctx, int64_t *totals);
 
 /* Pipeline tuning (autotuner): threads per block of the per-floe kernels. */
 int godar_set_block(godar_ctx *ctx, int32_t threads);
