@@ -47,7 +47,7 @@ def build_batch(members: int, seed: int):
 
 def pair_totals(ctx, mids):
     t = ctx.totals()
-    return t["candidates"], t["contacts"], t["bonds"]
+    return t["candidates"], t["contacts"], t["bonds"], t
 
 
 def run_batched(args):
@@ -70,10 +70,10 @@ def run_batched(args):
     stages = ctx.stage_times()
     ctx.profile(False)
     dt = t1 - t0
-    ncand, ncont, nbond = pair_totals(ctx, mids)
+    ncand, ncont, nbond, tot = pair_totals(ctx, mids)
     ctx.close()
     return dict(nfloes=nfloes, members=len(batch), seconds=dt, stages=stages,
-                ncand=ncand, ncont=ncont, nbond=nbond,
+                ncand=ncand, ncont=ncont, nbond=nbond, totals=tot,
                 alg_bytes_step=algorithmic_bytes(nfloes, ncand, ncont, nbond))
 
 
@@ -190,11 +190,20 @@ def main():
     dom_ms = stages[dom][0] / stages[dom][1]
     nf, nc, nk, nb = B["nfloes"], B["ncand"], B["ncont"], B["nbond"]
     kbar, kc, kb = nc / nf, nk / nf, nb / nf
-    # per-floe algorithmic bytes of the pair-event gather kernel: own state
-    # (11 words) + wind (2) read, 12 accumulators written, two list entries per
-    # candidate pair side, contact history and bond records read and written
+    qc, qb = B["totals"]["queued_contact"], B["totals"]["queued_bond"]
+    # algorithmic bytes per launch of each stage (partner records are re-reads
+    # of other floes' rows and count once, in the floe's own term):
+    #  shelter: x, y, r, hfa, hfw + wind read, 2 minima written, history and
+    #    bond counts read, class written; per candidate pair one list entry
+    #    read on each side, flag and override words written
+    #  contacts: queued floe's 11 words read, 5 accumulators + count written;
+    #    per contact two list entries read, history (16 B) read and written
+    #  bonds: same per queued floe; per bond two list entries read, the
+    #    88-byte record read and written
     per_stage_bytes = {
-        "events": nf * (13 * 8 + 12 * 8) + 8.0 * nc + 32.0 * nk + 120.0 * nb,
+        "shelter": nf * (7 * 8 + 2 * 8 + 2 * 4 + 1) + nc * (2 * 8 + 4 + 4),
+        "contacts": qc * (11 * 8 + 5 * 8 + 4) + nk * (2 * 8 + 32),
+        "bonds": qb * (11 * 8 + 5 * 8 + 4) + nb * (2 * 8 + 2 * 88),
         "candidates": nf * (3 * 8 + 8) + 12.0 * 2 * nc,
         "integrate": nf * (12 * 8 + 12 * 8 + 6 * 8 + 4 * 8),
         "sort": nf * 8.0 * 2 * 3,

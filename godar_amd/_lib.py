@@ -34,7 +34,7 @@ EXPORTS = [
     "godar_get_pairs", "godar_get_bond_state", "godar_get_stats", "godar_set_block",
     "godar_profile", "godar_get_profile", "godar_stage_name", "godar_get_totals",
 ]
-NSTAGES = 8
+NSTAGES = 11
 
 
 class GodarError(RuntimeError):

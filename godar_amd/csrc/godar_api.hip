@@ -17,8 +17,10 @@ __global__ void k_bin(DS);
 __global__ void k_clear_cells(DS);
 __global__ void k_ranges(DS);
 __global__ void k_cand(DS);
-template <bool M> __global__ void k_fast(DS);
-template <bool M> __global__ void k_heavy(DS);
+template <bool M> __global__ void k_shelter(DS);
+template <bool M> __global__ void k_contact(DS);
+template <bool M> __global__ void k_bond(DS);
+template <bool M> __global__ void k_walls(DS);
 template <bool M> __global__ void k_repair(DS, int);
 template <bool M> __global__ void k_fixup(DS);
 template <bool M> __global__ void k_final(DS);
@@ -290,17 +292,16 @@ int launch_step(godar_ctx* c) {
     { StageTimer t(c, 3); hipLaunchKernelGGL(k_cand, g, b, 0, s, d); }
     int nm = c->Mcap;
     const bool mixed = c->precision == GODAR_MIXED;
+    { StageTimer t(c, 4); hipLaunchKernelGGL(mixed ? k_shelter<true> : k_shelter<false>, g, b, 0, s, d); }
+    { StageTimer t(c, 5); hipLaunchKernelGGL(mixed ? k_contact<true> : k_contact<false>, g, b, 0, s, d); }
+    { StageTimer t(c, 6); hipLaunchKernelGGL(mixed ? k_bond<true> : k_bond<false>, g, b, 0, s, d); }
+    { StageTimer t(c, 7); hipLaunchKernelGGL(mixed ? k_walls<true> : k_walls<false>, g, b, 0, s, d); }
     {
-        StageTimer t(c, 4);
-        hipLaunchKernelGGL(mixed ? k_fast<true> : k_fast<false>, g, b, 0, s, d);
-        hipLaunchKernelGGL(mixed ? k_heavy<true> : k_heavy<false>, g, b, 0, s, d);
-    }
-    {
-        StageTimer t(c, 5);
+        StageTimer t(c, 8);
         hipLaunchKernelGGL(mixed ? k_repair<true> : k_repair<false>, dim3(nm), dim3(64), 0, s, d, nm);
     }
-    { StageTimer t(c, 6); hipLaunchKernelGGL(mixed ? k_fixup<true> : k_fixup<false>, g, b, 0, s, d); }
-    { StageTimer t(c, 7); hipLaunchKernelGGL(mixed ? k_final<true> : k_final<false>, g, b, 0, s, d); }
+    { StageTimer t(c, 9); hipLaunchKernelGGL(mixed ? k_fixup<true> : k_fixup<false>, g, b, 0, s, d); }
+    { StageTimer t(c, 10); hipLaunchKernelGGL(mixed ? k_final<true> : k_final<false>, g, b, 0, s, d); }
     HIPCHK(c, hipGetLastError());
     c->parity ^= 1;
     return GODAR_OK;
@@ -390,8 +391,8 @@ int godar_create(int64_t max_floes, int32_t max_members, int32_t precision, goda
     ACHK(dalloc(&d.wout, S));
     ACHK(dalloc(&d.wflag, S));
     ACHK(dalloc(&d.cls, S));
-    ACHK(dalloc(&d.heavy, S));
-    ACHK(dalloc(&d.nheavy, 1));
+    ACHK(dalloc(&d.qlist, 3 * S));
+    ACHK(dalloc(&d.qn, 3));
     ACHK(dalloc(&d.seeds, (int64_t)c->SEEDK * S));
     ACHK(dalloc(&d.nseeds, max_members));
     ACHK(dalloc(&d.heap, (int64_t)c->HEAPK * S));
@@ -441,7 +442,7 @@ int godar_destroy(godar_ctx* c) {
     dfree(d.crange);
     for (int k = 0; k < 2; ++k) { dfree(c->hcnt[k]); dfree(c->bcnt[k]); }
     free_lists(c);
-    dfree(d.wout); dfree(d.wflag); dfree(d.cls); dfree(d.heavy); dfree(d.nheavy); dfree(d.seeds); dfree(d.nseeds); dfree(d.heap); dfree(d.etab);
+    dfree(d.wout); dfree(d.wflag); dfree(d.cls); dfree(d.qlist); dfree(d.qn); dfree(d.seeds); dfree(d.nseeds); dfree(d.heap); dfree(d.etab);
     dfree(d.netab); dfree(d.wtab); dfree(d.nwtab); dfree(d.ovw); dfree(d.dirty);
     dfree(d.affected); dfree(d.stats); dfree(d.err);
     radix_free(c->rs);
@@ -923,13 +924,17 @@ int godar_get_totals(godar_ctx* c, int64_t* out) {
         hipLaunchKernelGGL(k_totals, dim3((c->N_used + 255) / 256), dim3(256), 0, c->stream, c->d, dout);
     hipError_t e = hipStreamSynchronize(c->stream);
     if (e == hipSuccess) e = hipMemcpy(out, dout, 3 * sizeof(int64_t), hipMemcpyDeviceToHost);
+    int32_t q[3] = {0, 0, 0};
+    if (e == hipSuccess) e = hipMemcpy(q, c->d.qn, sizeof(q), hipMemcpyDeviceToHost);
+    for (int k = 0; k < 3; ++k) out[3 + k] = q[k];
     (void)hipFree(dout);
     HIPCHK(c, e);
     return GODAR_OK;
 }
 
 static const char* kStageNames[GODAR_NSTAGES] = {
-    "bin", "sort", "cells", "candidates", "events", "repair", "fixup", "integrate"};
+    "bin", "sort", "cells", "candidates", "shelter", "contacts", "bonds", "walls", "repair",
+    "fixup", "integrate"};
 
 int godar_profile(godar_ctx* c, int32_t on) {
     if (!c) return GODAR_ERR_ARG;

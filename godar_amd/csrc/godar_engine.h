@@ -12,6 +12,8 @@ enum : int {
 };
 
 constexpr int WALL_SLOT = 0xFFFF;
+// cls bits: which accumulator rows of a floe hold gathered values
+constexpr uint8_t CLS_CONTACT = 1, CLS_BOND = 2;
 constexpr int FLOE_BLOCK = 256;   // upper bound of the per-floe kernels' block size
 
 // Hot per-floe record (96 bytes): everything a pair event reads of a floe.
@@ -71,9 +73,9 @@ struct DS {
     WallOut* wout;             // [S], valid where wflag is set
     uint8_t* wflag;            // [S] 1: the floe touched a wall this step
     int32_t* ovw;              // [S]
-    uint8_t* cls;              // [S] 0: sheltering-only floe, 1: full gather
-    int32_t* heavy;            // [S] positions of the full-gather floes
-    int32_t* nheavy;           // count of `heavy`
+    uint8_t* cls;              // [S] CLS_* bits
+    int32_t* qlist;            // [3][S] positions queued for contact / bond / wall work
+    int32_t* qn;               // [3] queue lengths
     // ordered repair (id)
     uint64_t* seeds;           // SEEDK per floe slot, member region starts at SEEDK*first
     int32_t* nseeds;           // [members]

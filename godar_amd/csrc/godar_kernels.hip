@@ -1,20 +1,22 @@
 // Kernels of the GODAR step pipeline (one launch sequence per step):
 //
-//   k_bin -> radix sort -> k_clear_cells -> k_ranges -> k_cand -> k_fast
-//        -> k_heavy -> k_repair -> k_fixup -> k_final
+//   k_bin -> radix sort -> k_clear_cells -> k_ranges -> k_cand -> k_shelter
+//        -> k_contact -> k_bond -> k_walls -> k_repair -> k_fixup -> k_final
 //
 // Sequential semantics of the reference (stepper.f90:71-280 visits floes i
 // in index order and, for each, its candidates j > i in ascending distance;
 // ridging reshapes floes in place, so later pairs see the new radius and
 // thickness) are reproduced as follows:
-//   * k_fast/k_heavy evaluate every pair event with the start-of-step floe
-//     properties.  k_fast handles floes whose events are all sheltering-only
-//     (no contact, bond, contact history or wall) and queues the others for
-//     k_heavy's full gather.  One thread per floe gathers its own contributions in the
-//     reference's per-floe order: first the pairs where it is the inner
-//     index (ascending outer index), then its own candidates in distance
-//     order.  Each pair is evaluated by both of its floes, which removes
-//     atomics and keeps FP64 sums identical to the reference.
+//   * The event kernels evaluate every pair event with the start-of-step
+//     floe properties.  Contact, bond and sheltering terms feed disjoint
+//     accumulators, so each kind has a kernel of its own: k_shelter computes
+//     every floe's sheltering minima and queues the floes that have contact,
+//     bond or wall work for k_contact, k_bond and k_walls.  One thread per
+//     floe gathers its own contributions in the reference's per-floe order:
+//     first the pairs where it is the inner index (ascending outer index),
+//     then its own candidates in distance order.  Each pair is evaluated by
+//     both of its floes, which removes atomics and keeps FP64 sums identical
+//     to the reference.
 //   * An event's speculative result is exact unless one of its floes was
 //     reshaped earlier in the sequence.  Events that ridge are seeds; the
 //     per-member k_repair walks, in sequence order, the seeds and every later
@@ -93,6 +95,15 @@ __device__ __forceinline__ void load_pair_state(const DS& S, int i, int j, PairS
     }
 }
 
+// Record copy word by word (a struct copy is lowered through scratch memory).
+static_assert(sizeof(BondRec) % 8 == 0, "BondRec is copied in 8-byte words");
+__device__ __forceinline__ void copy_bond(BondRec* dst, const BondRec* src) {
+    const uint64_t* s = reinterpret_cast<const uint64_t*>(src);
+    uint64_t* d = reinterpret_cast<uint64_t*>(dst);
+#pragma unroll
+    for (int w = 0; w < (int)(sizeof(BondRec) / 8); ++w) d[w] = s[w];
+}
+
 __device__ __forceinline__ BondRec bond_from(const PairState& ps, int j) {
     BondRec b;
     b.j = j; b.bond = ps.bond; b.damage = ps.damage; b.deltanb = ps.deltanb; b.deltatb = ps.deltatb;
@@ -102,8 +113,8 @@ __device__ __forceinline__ BondRec bond_from(const PairState& ps, int j) {
 }
 
 // Gather of one event into floe i's accumulators (stepper.f90:113-146,
-// 165-198, 215-223), operand order kept.
-__device__ __forceinline__ void apply_i(double* a, const EventOut& o) {
+// 165-198, 215-223), operand order kept; one helper per accumulator kind.
+__device__ __forceinline__ void apply_contact_i(double* a, const EventOut& o) {
     if (o.flags & EV_CONTACT) {
         a[A_FCX] = a[A_FCX] - o.fcn_c + o.fcr_s;
         a[A_FCY] = a[A_FCY] - o.fcn_s - o.fcr_c;
@@ -111,6 +122,8 @@ __device__ __forceinline__ void apply_i(double* a, const EventOut& o) {
         a[A_TAC] = a[A_TAC] + o.ac;
         a[A_PC] = a[A_PC] - o.fcn_ac;
     }
+}
+__device__ __forceinline__ void apply_bond_i(double* a, const EventOut& o) {
     if (o.flags & EV_BONDED) {
         a[A_FBX] = a[A_FBX] + o.fbn_c - o.fbt_s;
         a[A_FBY] = a[A_FBY] + o.fbn_s + o.fbt_c;
@@ -118,6 +131,10 @@ __device__ __forceinline__ void apply_i(double* a, const EventOut& o) {
         a[A_TAB] = a[A_TAB] + o.sb;
         a[A_PB] = a[A_PB] - o.fbn_sb;
     }
+}
+__device__ __forceinline__ void apply_i(double* a, const EventOut& o) {
+    apply_contact_i(a, o);
+    apply_bond_i(a, o);
     if (o.flags & EV_SHELTER) {
         a[A_SHA] = fmin(a[A_SHA], o.hsfa_ji);
         a[A_SHW] = fmin(a[A_SHW], o.hsfw_ji);
@@ -125,7 +142,7 @@ __device__ __forceinline__ void apply_i(double* a, const EventOut& o) {
 }
 
 // ... and into floe j's.
-__device__ __forceinline__ void apply_j(double* a, const EventOut& o) {
+__device__ __forceinline__ void apply_contact_j(double* a, const EventOut& o) {
     if (o.flags & EV_CONTACT) {
         a[A_FCX] = a[A_FCX] + o.fcn_c - o.fcr_s;
         a[A_FCY] = a[A_FCY] + o.fcn_s + o.fcr_c;
@@ -133,6 +150,8 @@ __device__ __forceinline__ void apply_j(double* a, const EventOut& o) {
         a[A_TAC] = a[A_TAC] + o.ac;
         a[A_PC] = a[A_PC] - o.fcn_ac;
     }
+}
+__device__ __forceinline__ void apply_bond_j(double* a, const EventOut& o) {
     if (o.flags & EV_BONDED) {
         a[A_FBX] = a[A_FBX] - o.fbn_c + o.fbt_s;
         a[A_FBY] = a[A_FBY] - o.fbn_s - o.fbt_c;
@@ -140,6 +159,10 @@ __device__ __forceinline__ void apply_j(double* a, const EventOut& o) {
         a[A_TAB] = a[A_TAB] + o.sb;
         a[A_PB] = a[A_PB] - o.fbn_sb;
     }
+}
+__device__ __forceinline__ void apply_j(double* a, const EventOut& o) {
+    apply_contact_j(a, o);
+    apply_bond_j(a, o);
     if (o.flags & EV_SHELTER) {
         a[A_SHA] = fmin(a[A_SHA], o.hsfa_ij);
         a[A_SHW] = fmin(a[A_SHW], o.hsfw_ij);
@@ -176,7 +199,7 @@ __global__ void k_clear_cells(DS S) {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c > S.ncells || *S.err) return;
     S.crange[c] = make_int2(0, 0);
-    if (c == 0) *S.nheavy = 0;
+    if (c < 3) S.qn[c] = 0;
 }
 
 // cell ranges over the sorted keys, the sorted floe records and the wind at
@@ -276,13 +299,32 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_cand(DS S) {
 // ---------------------------------------------------------------------
 // 3. speculative events + per-floe gather + owner state writes
 // ---------------------------------------------------------------------
-// Sheltering-only floes: geometry and the floe's own shelter coefficients
-// (forcing.f90:162-186); any other event sends the floe to k_heavy.
+// Appends position k to a work queue, one atomic per wavefront.  Every lane
+// of the wavefront must call it.
+__device__ __forceinline__ void enqueue(bool want, int32_t* q, int32_t* cnt, int k) {
+    const uint64_t mask = __ballot(want);
+    if (!mask) return;
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll((long long)mask) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(cnt, __popcll(mask));
+    base = __shfl(base, leader);
+    if (want) {
+        const uint64_t below = lane ? mask & ((~0ull) >> (64 - lane)) : 0ull;
+        q[base + __popcll(below)] = k;
+    }
+}
+
+// Sheltering minima of every floe (forcing.f90:162-186, start-of-step
+// properties) and the work classification: a floe joins the contact queue if
+// it overlaps a candidate partner or holds contact histories, the bond queue
+// if it or an inner partner holds a bond record for it (or a bond forms this
+// step), the wall queue if it crosses the box.
 template <bool MIXED>
-__global__ __launch_bounds__(FLOE_BLOCK) void k_fast(DS S) {
+__global__ __launch_bounds__(FLOE_BLOCK) void k_shelter(DS S) {
     using T = typename Prec<MIXED>::T;
     int k = blockIdx.x * blockDim.x + threadIdx.x;
-    bool heavy = false;
+    bool qc = false, qb = false, qw = false;
     bool live = k < S.N && !*S.err;
     SRec me;
     if (live) {
@@ -296,25 +338,32 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_fast(DS S) {
         const int64_t SS = S.S;
         const int first = S.members[m].first;
         const bool bond_step = S.members[m].tstep == 1 && P.restart != 1 && P.cohesion;
-        heavy = S.hcnt_old[f] > 0 || S.bcnt_old[f] > 0 ||
-                me.y - me.r < 0 || me.y + me.r > P.ny || me.x - me.r < 0 || me.x + me.r > P.nx;
+        qc = S.hcnt_old[f] > 0;
+        qb = S.bcnt_old[f] > 0;
+        qw = me.y - me.r < 0 || me.y + me.r > P.ny || me.x - me.r < 0 || me.x + me.r > P.nx;
         T sha = T(1), shw = T(1);
         const T uwi = T(P.uw), vwi = T(P.vw);
         const int ni = S.icnt[k];
-        for (int q = 0; q < ni && !heavy; ++q) {
+        for (int q = 0; q < ni; ++q) {
             const int64_t e = S.ik[q * SS + k];
             const int i = ent_id(e), qi = ent_pos(e);
             const SRec* o = &S.srec[qi];
             double dx = o->x - me.x, dy = o->y - me.y;
             double dist = sqrt(dx * dx + dy * dy);
             double deltan = o->r + me.r - dist;
-            if (deltan > 0) { heavy = true; break; }
-            if (bond_step && deltan >= -P.bond_lim &&
-                bond_keep(i - first, f - first, P.bond_seed, P.bond_thr24)) { heavy = true; break; }
-            int bc = S.bcnt_old[i];
-            for (int b = 0; b < bc; ++b)
-                if (S.bond_old[(int64_t)b * SS + i].j == f) { heavy = true; break; }
-            if (heavy) break;
+            if (deltan > 0) qc = true;
+            if (!qb) {
+                if (bond_step && deltan >= -P.bond_lim &&
+                    bond_keep(i - first, f - first, P.bond_seed, P.bond_thr24)) {
+                    qb = true;
+                } else {
+                    const int bc = S.bcnt_old[i];
+                    for (int b = 0; b < bc; ++b) {
+                        const BondRec& br = S.bond_old[(int64_t)b * SS + i];
+                        if (br.j == f && br.bond == 1) { qb = true; break; }
+                    }
+                }
+            }
             if (P.shelter) {
                 // pair (i, f): cosa/sina point from i to f; floe f takes hsf*(i,f)
                 double cosa = (me.x - o->x) / dist, sina = (me.y - o->y) / dist;
@@ -326,16 +375,18 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_fast(DS S) {
         }
         const int no = S.ocnt[k];
         const double2 wf = S.wind[k];
-        for (int s = 0; s < no && !heavy; ++s) {
+        const uint32_t fl = P.shelter ? EV_SHELTER : 0u;
+        for (int s = 0; s < no; ++s) {
             const int64_t e = S.oj[s * SS + k];
             const int j = ent_id(e), qj = ent_pos(e);
             const SRec* o = &S.srec[qj];
             double dx = me.x - o->x, dy = me.y - o->y;
             double dist = sqrt(dx * dx + dy * dy);
             double deltan = me.r + o->r - dist;
-            if (deltan > 0) { heavy = true; break; }
-            if (bond_step && deltan >= -P.bond_lim &&
-                bond_keep(f - first, j - first, P.bond_seed, P.bond_thr24)) { heavy = true; break; }
+            if (deltan > 0) qc = true;
+            if (!qb && bond_step && deltan >= -P.bond_lim &&
+                bond_keep(f - first, j - first, P.bond_seed, P.bond_thr24))
+                qb = true;
             if (P.shelter) {
                 // pair (f, j): floe f takes hsf*(j,f), evaluated with -cosa, -sina
                 double cosa = (o->x - me.x) / dist, sina = (o->y - me.y) / dist;
@@ -343,48 +394,48 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_fast(DS S) {
                 sha = dmin(sha, S_shelter(T(o->hfa), T(me.hfa), dn, -c, -sn, T(wf.x), T(wf.y)));
                 shw = dmin(shw, S_shelter(T(o->hfw), T(me.hfw), dn, -c, -sn, uwi, vwi));
             }
+            S.evf[s * SS + k] = fl;
+            S.ov[s * SS + k] = -1;
         }
-        if (!heavy) {
-            const uint32_t fl = P.shelter ? EV_SHELTER : 0u;
-            for (int s = 0; s < no; ++s) {
-                S.evf[s * SS + k] = fl;
-                S.ov[s * SS + k] = -1;
-            }
-            S.acc[A_SHA * SS + k] = double(sha);
-            S.acc[A_SHW * SS + k] = double(shw);
-            S.hcnt_new[f] = 0;
-            S.bcnt_new[f] = 0;
-            S.ovw[k] = -1;
-            S.wflag[k] = 0;
-            S.cls[k] = 0;
-        } else {
-            S.cls[k] = 1;
-        }
+        S.acc[A_SHA * SS + k] = double(sha);
+        S.acc[A_SHW * SS + k] = double(shw);
+        if (!qc) S.hcnt_new[f] = 0;
+        if (!qb) S.bcnt_new[f] = 0;
+        S.ovw[k] = -1;
+        S.wflag[k] = 0;
+        S.cls[k] = (qc ? CLS_CONTACT : 0) | (qb ? CLS_BOND : 0);
     }
-    // queue the heavy floes, one atomic per wavefront
-    uint64_t mask = __ballot(heavy);
-    if (mask) {
-        const int lane = threadIdx.x & 63;
-        const int leader = __ffsll((long long)mask) - 1;
-        int base = 0;
-        if (lane == leader) base = atomicAdd(S.nheavy, __popcll(mask));
-        base = __shfl(base, leader);
-        if (heavy) {
-            const uint64_t below = lane ? mask & ((~0ull) >> (64 - lane)) : 0ull;
-            S.heavy[base + __popcll(below)] = k;
-        }
+    enqueue(qc, S.qlist, &S.qn[0], k);
+    enqueue(qb, S.qlist + S.S, &S.qn[1], k);
+    enqueue(qw, S.qlist + 2 * S.S, &S.qn[2], k);
+}
+
+// Contact state of pair (i, j), owner i: history and whether it is bonded.
+__device__ __forceinline__ void load_contact_state(const DS& S, int i, int j, PairState& ps) {
+    ps.deltat = 0.0; ps.thetarelc = 0.0; ps.bond = 0; ps.has_bond_rec = false;
+    ps.damage = ps.deltanb = ps.deltatb = ps.trb_ji = ps.trb_ij = 0.0;
+    ps.rb = ps.hb = ps.lb = ps.sb = ps.ib = 0.0;
+    const int hc = S.hcnt_old[i];
+    for (int q = 0; q < hc; ++q) {
+        const HistRec& hr = S.hist_old[(int64_t)q * S.S + i];
+        if (hr.j == j) { ps.deltat = hr.deltat; ps.thetarelc = hr.thetarelc; break; }
+    }
+    const int bc = S.bcnt_old[i];
+    for (int q = 0; q < bc; ++q) {
+        const BondRec& b = S.bond_old[(int64_t)q * S.S + i];
+        if (b.j == j) { ps.has_bond_rec = true; ps.bond = b.bond; break; }
     }
 }
 
-// Full speculative gather of the queued floes + owner state writes.
+// Contact terms of the queued floes (contact_forces, plastic_contact,
+// update_shape, coulomb; stepper.f90:103-146) + the owner's new histories.
 template <bool MIXED>
-__global__ __launch_bounds__(FLOE_BLOCK) void k_heavy(DS S) {
+__global__ __launch_bounds__(FLOE_BLOCK) void k_contact(DS S) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (*S.err || t >= *S.nheavy) return;
-    const int k = S.heavy[t];
+    if (*S.err || t >= S.qn[0]) return;
+    const int k = S.qlist[t];
     const SRec me = S.srec[k];
     const int m = me.member;
-    if (m < 0) return;
     const DevParams& P = S.params[m];
     const int tstep = S.members[m].tstep;
     const int first = S.members[m].first;
@@ -397,45 +448,47 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_heavy(DS S) {
     acc_init(acc);
     uint32_t over = 0;
 
-    // pairs (i, f), f inner, ascending i
+    // pairs (i, f), f inner, ascending i; only overlapping pairs contribute
     const int ni = S.icnt[k];
     for (int q = 0; q < ni; ++q) {
         const int64_t e = ik[q * SS + k];
         const int i = ent_id(e), qi = ent_pos(e);
         Floe Fi = floe_of(S.srec[qi]);
         Floe Ff = F0;
+        Kin kn;
+        kinematics(Fi, Ff, kn);
+        if (!(kn.deltan > 0)) continue;
         PairState ps;
-        load_pair_state(S, i, f, ps);
-        const double2 wi = S.wind[qi];
+        load_contact_state(S, i, f, ps);
+        bond_create(P, tstep, i - first, f - first, Fi, Ff, kn, ps);
         EventOut o;
-        pair_event<MIXED>(P, tstep, i - first, f - first, Fi, Ff, wi.x, wi.y, P.uw, P.vw, ps, o, 2);
-        apply_j(acc, o);
+        o.flags = 0;
+        o.ridged = -1;
+        contact_part<MIXED>(P, Fi, Ff, kn, ps, o);
+        apply_contact_j(acc, o);
     }
 
     // pairs (f, j), f outer (owner), distance order
     const int no = S.ocnt[k];
-    int nb = 0;
-    {
-        const int bco = S.bcnt_old[f];
-        for (int q = 0; q < bco; ++q) {
-            BondRec b = S.bond_old[q * SS + f];
-            if (b.j >= 0) S.bond_new[(nb++) * SS + f] = b;   // drops cleared entries
-        }
-    }
     int nh = 0;
-    const double2 wf = S.wind[k];
     for (int s = 0; s < no; ++s) {
         const int64_t e = oj[s * SS + k];
         const int j = ent_id(e), qj = ent_pos(e);
         Floe Fi = F0;
         Floe Fj = floe_of(S.srec[qj]);
+        Kin kn;
+        kinematics(Fi, Fj, kn);
+        // no overlap: reset_contact (reset.f90:1-15) drops the history
+        if (!(kn.deltan > 0)) continue;
         PairState ps;
-        load_pair_state(S, f, j, ps);
+        load_contact_state(S, f, j, ps);
+        bond_create(P, tstep, f - first, j - first, Fi, Fj, kn, ps);
         EventOut o;
-        pair_event<MIXED>(P, tstep, f - first, j - first, Fi, Fj, wf.x, wf.y, P.uw, P.vw, ps, o, 1);
-        apply_i(acc, o);
-        S.evf[s * SS + k] = o.flags;
-        S.ov[s * SS + k] = -1;
+        o.flags = 0;
+        o.ridged = -1;
+        contact_part<MIXED>(P, Fi, Fj, kn, ps, o);
+        apply_contact_i(acc, o);
+        S.evf[s * SS + k] |= o.flags;
         if (ps.deltat != 0.0 || ps.thetarelc != 0.0) {
             if (nh < S.HC) {
                 HistRec hr; hr.j = j; hr.pad_ = 0; hr.deltat = ps.deltat; hr.thetarelc = ps.thetarelc;
@@ -444,16 +497,6 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_heavy(DS S) {
             } else {
                 over |= ERR_HIST_CAP;
             }
-        }
-        if (ps.has_bond_rec) {
-            int qb = -1;
-            for (int q = 0; q < nb; ++q)
-                if (S.bond_new[q * SS + f].j == j) { qb = q; break; }
-            if (qb < 0) {
-                if (nb < S.BC) qb = nb++;
-                else over |= ERR_BOND_CAP;
-            }
-            if (qb >= 0) S.bond_new[qb * SS + f] = bond_from(ps, j);
         }
         if (o.flags & EV_RIDGED) {
             int p = atomicAdd(&S.nseeds[m], 1);
@@ -478,28 +521,149 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_heavy(DS S) {
         }
     }
     S.hcnt_new[f] = nh;
-    S.bcnt_new[f] = nb;
+    S.acc[A_FCX * SS + k] = acc[A_FCX];
+    S.acc[A_FCY * SS + k] = acc[A_FCY];
+    S.acc[A_MC * SS + k] = acc[A_MC];
+    S.acc[A_TAC * SS + k] = acc[A_TAC];
+    S.acc[A_PC * SS + k] = acc[A_PC];
+    if (over) set_err(S, over);
+}
 
-    // walls of f (verify_bc, boundaries.f90:1-97)
+// Bond state of pair (i, j), owner i, from record q of i's old bond list
+// (q < 0: no record).
+__device__ __forceinline__ void load_bond_state(const DS& S, int i, int q, PairState& ps) {
+    ps.deltat = 0.0; ps.thetarelc = 0.0; ps.bond = 0; ps.has_bond_rec = false;
+    ps.damage = ps.deltanb = ps.deltatb = ps.trb_ji = ps.trb_ij = 0.0;
+    ps.rb = ps.hb = ps.lb = ps.sb = ps.ib = 0.0;
+    if (q < 0) return;
+    const BondRec& b = S.bond_old[(int64_t)q * S.S + i];
+    ps.has_bond_rec = true; ps.bond = b.bond; ps.damage = b.damage;
+    ps.deltanb = b.deltanb; ps.deltatb = b.deltatb; ps.trb_ji = b.trb_ji; ps.trb_ij = b.trb_ij;
+    ps.rb = b.rb; ps.hb = b.hb; ps.lb = b.lb; ps.sb = b.sb; ps.ib = b.ib;
+}
+
+// Bond terms of the queued floes (bond_forces_euler, bond_breaking;
+// stepper.f90:148-198) + the owner's new bond records.
+template <bool MIXED>
+__global__ __launch_bounds__(FLOE_BLOCK) void k_bond(DS S) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (*S.err || t >= S.qn[1]) return;
+    const int k = S.qlist[S.S + t];
+    const SRec me = S.srec[k];
+    const int m = me.member;
+    const DevParams& P = S.params[m];
+    const int tstep = S.members[m].tstep;
+    const int first = S.members[m].first;
+    const bool bond_step = tstep == 1 && P.restart != 1 && P.cohesion;
+    const int f = me.id;
+    const int64_t SS = S.S;
+    const int64_t* oj = S.oj;
+    const int64_t* ik = S.ik;
+    const Floe F0 = floe_of(me);
+    double acc[NACC];
+    acc_init(acc);
+    uint32_t over = 0;
+
+    // pairs (i, f), f inner: bonds that i holds for f, or that form now
+    const int ni = S.icnt[k];
+    for (int q = 0; q < ni; ++q) {
+        const int64_t e = ik[q * SS + k];
+        const int i = ent_id(e), qi = ent_pos(e);
+        int rq = -1;
+        const int bc = S.bcnt_old[i];
+        for (int b = 0; b < bc; ++b)
+            if (S.bond_old[(int64_t)b * SS + i].j == f) { rq = b; break; }
+        if (rq < 0 && !bond_step) continue;
+        Floe Fi = floe_of(S.srec[qi]);
+        Floe Ff = F0;
+        Kin kn;
+        kinematics(Fi, Ff, kn);
+        PairState ps;
+        load_bond_state(S, i, rq, ps);
+        bond_create(P, tstep, i - first, f - first, Fi, Ff, kn, ps);
+        if (ps.bond != 1) continue;
+        EventOut o;
+        o.flags = 0;
+        o.ridged = -1;
+        bond_part<MIXED>(P, Fi, Ff, kn, ps, o);
+        apply_bond_j(acc, o);
+    }
+
+    // pairs (f, j), f outer (owner), distance order
+    int nb = 0;
     {
-        Floe Fw = F0;
-        double hist[4] = {S.wbc[0 * SS + f], S.wbc[1 * SS + f], S.wbc[2 * SS + f], S.wbc[3 * SS + f]};
-        WallOut W;
-        walls<MIXED>(P, Fw, hist, W);
-        S.wflag[k] = (uint8_t)(W.flags & 1u);
-        if (W.flags & 1u) S.wout[k] = W;
-        if (W.flags & 4u) {
-            int p = atomicAdd(&S.nseeds[m], 1);
-            if (p < S.SEEDK * S.members[m].n)
-                S.seeds[(int64_t)S.SEEDK * first + p] = ((uint64_t)(f - first) << 16) | WALL_SLOT;
-            else
-                over |= ERR_SEED_CAP;
+        const int bco = S.bcnt_old[f];
+        for (int q = 0; q < bco; ++q) {
+            const BondRec* b = &S.bond_old[q * SS + f];
+            if (b->j >= 0) copy_bond(&S.bond_new[(nb++) * SS + f], b);   // drops cleared entries
         }
     }
-    S.ovw[k] = -1;
-#pragma unroll
-    for (int a = 0; a < NACC; ++a) S.acc[a * SS + k] = acc[a];
+    const int bco = S.bcnt_old[f];
+    const int no = S.ocnt[k];
+    for (int s = 0; s < no; ++s) {
+        const int64_t e = oj[s * SS + k];
+        const int j = ent_id(e), qj = ent_pos(e);
+        int rq = -1;
+        for (int b = 0; b < bco; ++b)
+            if (S.bond_old[(int64_t)b * SS + f].j == j) { rq = b; break; }
+        if (rq < 0 && !bond_step) continue;
+        Floe Fi = F0;
+        Floe Fj = floe_of(S.srec[qj]);
+        Kin kn;
+        kinematics(Fi, Fj, kn);
+        PairState ps;
+        load_bond_state(S, f, rq, ps);
+        bond_create(P, tstep, f - first, j - first, Fi, Fj, kn, ps);
+        if (!ps.has_bond_rec) continue;
+        EventOut o;
+        o.flags = 0;
+        o.ridged = -1;
+        bond_part<MIXED>(P, Fi, Fj, kn, ps, o);
+        apply_bond_i(acc, o);
+        if (o.flags & EV_BONDED) S.evf[s * SS + k] |= EV_BONDED;
+        int qb = -1;
+        for (int q = 0; q < nb; ++q)
+            if (S.bond_new[q * SS + f].j == j) { qb = q; break; }
+        if (qb < 0) {
+            if (nb < S.BC) qb = nb++;
+            else over |= ERR_BOND_CAP;
+        }
+        if (qb >= 0) S.bond_new[qb * SS + f] = bond_from(ps, j);
+    }
+    S.bcnt_new[f] = nb;
+    S.acc[A_FBX * SS + k] = acc[A_FBX];
+    S.acc[A_FBY * SS + k] = acc[A_FBY];
+    S.acc[A_MB * SS + k] = acc[A_MB];
+    S.acc[A_TAB * SS + k] = acc[A_TAB];
+    S.acc[A_PB * SS + k] = acc[A_PB];
     if (over) set_err(S, over);
+}
+
+// Walls of the queued floes (verify_bc, boundaries.f90:1-97).
+template <bool MIXED>
+__global__ __launch_bounds__(FLOE_BLOCK) void k_walls(DS S) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (*S.err || t >= S.qn[2]) return;
+    const int k = S.qlist[2 * S.S + t];
+    const SRec me = S.srec[k];
+    const int m = me.member;
+    const DevParams& P = S.params[m];
+    const int first = S.members[m].first;
+    const int f = me.id;
+    const int64_t SS = S.S;
+    Floe Fw = floe_of(me);
+    double hist[4] = {S.wbc[0 * SS + f], S.wbc[1 * SS + f], S.wbc[2 * SS + f], S.wbc[3 * SS + f]};
+    WallOut W;
+    walls<MIXED>(P, Fw, hist, W);
+    S.wflag[k] = (uint8_t)(W.flags & 1u);
+    if (W.flags & 1u) S.wout[k] = W;
+    if (W.flags & 4u) {
+        int p = atomicAdd(&S.nseeds[m], 1);
+        if (p < S.SEEDK * S.members[m].n)
+            S.seeds[(int64_t)S.SEEDK * first + p] = ((uint64_t)(f - first) << 16) | WALL_SLOT;
+        else
+            set_err(S, ERR_SEED_CAP);
+    }
 }
 
 // ---------------------------------------------------------------------
@@ -771,7 +935,7 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_fixup(DS S) {
     }
 #pragma unroll
     for (int a = 0; a < NACC; ++a) S.acc[a * SS + k] = acc[a];
-    S.cls[k] = 1;
+    S.cls[k] = CLS_CONTACT | CLS_BOND;
 }
 
 // ---------------------------------------------------------------------
@@ -807,14 +971,13 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_final(DS S) {
     S.wbc[0 * SS + f] = W.deltat_bc1; S.wbc[1 * SS + f] = W.deltat_bc2;
     S.wbc[2 * SS + f] = W.theta_bc1; S.wbc[3 * SS + f] = W.theta_bc2;
     double a[NACC];
-    if (S.cls[k]) {
+    const uint8_t cl = S.cls[k];
 #pragma unroll
-        for (int c = 0; c < NACC; ++c) a[c] = S.acc[c * SS + k];
-    } else {
-#pragma unroll
-        for (int c = 0; c < NACC; ++c) a[c] = 0.0;
-        a[A_SHA] = S.acc[A_SHA * SS + k];
-        a[A_SHW] = S.acc[A_SHW * SS + k];
+    for (int c = 0; c < NACC; ++c) {
+        const bool bond_row = c == A_FBX || c == A_FBY || c == A_MB || c == A_TAB || c == A_PB;
+        const bool shel_row = c == A_SHA || c == A_SHW;
+        const bool valid = shel_row || (bond_row ? (cl & CLS_BOND) : (cl & CLS_CONTACT));
+        a[c] = valid ? S.acc[c * SS + k] : 0.0;
     }
     // shelter minima (reset_shelter + thread reduction); with shelter off
     // the reference's minima stay at their zero initial value
@@ -885,10 +1048,14 @@ __global__ void k_totals(DS S, int64_t* out) {
 }
 
 // explicit instantiations used by the host launcher
-template __global__ void k_fast<false>(DS);
-template __global__ void k_fast<true>(DS);
-template __global__ void k_heavy<false>(DS);
-template __global__ void k_heavy<true>(DS);
+template __global__ void k_shelter<false>(DS);
+template __global__ void k_shelter<true>(DS);
+template __global__ void k_contact<false>(DS);
+template __global__ void k_contact<true>(DS);
+template __global__ void k_bond<false>(DS);
+template __global__ void k_bond<true>(DS);
+template __global__ void k_walls<false>(DS);
+template __global__ void k_walls<true>(DS);
 template __global__ void k_repair<false>(DS, int);
 template __global__ void k_repair<true>(DS, int);
 template __global__ void k_fixup<false>(DS);

@@ -176,20 +176,32 @@ __device__ __forceinline__ void contact_part(const DevParams& P, Floe& fi, Floe&
             double area = ri * ri * acos(dij / ri) - dij * delt_ridge / 2.0
                         + rj * rj * acos(dji / rj) - dji * delt_ridge / 2.0;
             double vol = fmax(area * hmin64 / 2.0, 0.0);
-            Floe* w = nullptr;
+            // the thinner floe is reshaped; its six properties are selected
+            // field by field (a pointer to either floe, or a select of whole
+            // records, would force both floes out of registers)
             int who = -1;
-            if (hmin64 == fi.h) { w = &fi; who = 0; }
-            else if (hmin64 == fj.h) { w = &fj; who = 1; }
-            if (w) {
-                double rw = w->r, hw = w->h;
+            if (hmin64 == fi.h) who = 0;
+            else if (hmin64 == fj.h) who = 1;
+            if (who >= 0) {
+                Floe w;
+                const double rw = who == 0 ? fi.r : fj.r;
+                const double hw = who == 0 ? fi.h : fj.h;
                 double dh = vol / (P.pi * (rw * rw));
-                w->r = rw * sqrt(hw / (hw + dh));
-                w->h = hw + dh;
-                floe_properties(P, *w);
+                w.r = rw * sqrt(hw / (hw + dh));
+                w.h = hw + dh;
+                floe_properties(P, w);
+                const bool wi = who == 0;
+                fi.r = wi ? w.r : fi.r;             fj.r = wi ? fj.r : w.r;
+                fi.h = wi ? w.h : fi.h;             fj.h = wi ? fj.h : w.h;
+                fi.mass = wi ? w.mass : fi.mass;    fj.mass = wi ? fj.mass : w.mass;
+                fi.inertia = wi ? w.inertia : fi.inertia;
+                fj.inertia = wi ? fj.inertia : w.inertia;
+                fi.hfa = wi ? w.hfa : fi.hfa;       fj.hfa = wi ? fj.hfa : w.hfa;
+                fi.hfw = wi ? w.hfw : fi.hfw;       fj.hfw = wi ? fj.hfw : w.hfw;
                 o.flags |= EV_RIDGED;
                 o.ridged = who;
-                o.nr = w->r; o.nh = w->h; o.nmass = w->mass;
-                o.ninertia = w->inertia; o.nhfa = w->hfa; o.nhfw = w->hfw;
+                o.nr = w.r; o.nh = w.h; o.nmass = w.mass;
+                o.ninertia = w.inertia; o.nhfa = w.hfa; o.nhfw = w.hfw;
             }
             delt_ridge = delt_ridge_of(fi.r, fj.r, dist);
             dr = T(delt_ridge);

@@ -205,10 +205,13 @@ class Context:
         return dict(zip(BOND_KEYS, st.T))
 
     def totals(self) -> dict:
-        """Candidate pairs, contacts (last step) and bonded pairs over all members."""
-        t = np.zeros(3, dtype=np.int64)
+        """Candidate pairs, contacts (last step) and bonded pairs over all
+        members, and the floes the last step queued for contact, bond and wall
+        work."""
+        t = np.zeros(6, dtype=np.int64)
         self._chk(self.L.godar_get_totals(self.h, t.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))))
-        return dict(candidates=int(t[0]), contacts=int(t[1]), bonds=int(t[2]))
+        return dict(candidates=int(t[0]), contacts=int(t[1]), bonds=int(t[2]),
+                    queued_contact=int(t[3]), queued_bond=int(t[4]), queued_wall=int(t[5]))
 
     def stats(self, m: int) -> dict:
         s = np.zeros(4, dtype=np.int64)

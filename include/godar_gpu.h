@@ -153,7 +153,9 @@ int godar_get_bond_state(godar_ctx *ctx, int32_t member, int32_t cap, double *st
  * [2] ridging seeds, [3] events re-evaluated by the ordered repair. */
 int godar_get_stats(godar_ctx *ctx, int32_t member, int64_t *stats);
 /* Totals over all members of the last step: [0] candidate pairs,
- * [1] contacts, [2] bonded pairs. */
+ * [1] contacts, [2] bonded pairs, [3..5] floes queued for contact, bond and
+ * wall work.  `totals` holds GODAR_NTOTALS entries. */
+#define GODAR_NTOTALS 6
 int godar_get_totals(godar_ctx *ctx, int64_t *totals);
 
 /* Pipeline tuning (autotuner): threads per block of the per-floe kernels. */
@@ -166,7 +168,7 @@ int godar_set_block(godar_ctx *ctx, int32_t threads);
  * godar_get_profile returns the summed milliseconds and the number of timed
  * launches per stage.  godar_profile resets the totals.
  */
-#define GODAR_NSTAGES 8
+#define GODAR_NSTAGES 11
 int godar_profile(godar_ctx *ctx, int32_t on);
 int godar_get_profile(godar_ctx *ctx, double *ms, int64_t *launches);
 const char *godar_stage_name(int32_t stage);
