@@ -31,7 +31,7 @@ EXPORTS = [
     "godar_capacity_left", "godar_set_forcing", "godar_set_wind_grid", "godar_set_bonds",
     "godar_step", "godar_step_async", "godar_sync", "godar_member_tstep", "godar_reserve",
     "godar_get_floes", "godar_record_forces", "godar_get_forces", "godar_get_walls",
-    "godar_get_pairs", "godar_get_bond_state", "godar_get_stats", "godar_set_block",
+    "godar_get_pairs", "godar_get_bond_state", "godar_get_stats", "godar_set_block", "godar_use_graphs",
     "godar_profile", "godar_get_profile", "godar_stage_name", "godar_get_totals",
 ]
 NSTAGES = 11
@@ -88,6 +88,7 @@ def load(path: str = LIB_PATH):
         "godar_get_bond_state": (_i32, [_vp, _i32, _i32, _dp, _ip]),
         "godar_get_stats": (_i32, [_vp, _i32, _lp]),
         "godar_set_block": (_i32, [_vp, _i32]),
+        "godar_use_graphs": (_i32, [_vp, _i32]),
         "godar_profile": (_i32, [_vp, _i32]),
         "godar_get_profile": (_i32, [_vp, _dp, _lp]),
         "godar_stage_name": (ctypes.c_char_p, [_i32]),

@@ -79,6 +79,12 @@ struct godar_ctx {
     bool pending = false;
     uint32_t last_err_bits = 0;
     bool auto_grow = true;
+    // captured step graphs, one per pair-state parity (the kernels take the
+    // device-state pointers by value), each with the launch signature it was
+    // captured for
+    bool use_graphs = true;
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    std::vector<unsigned char> gsig[2];
 };
 
 namespace {
@@ -276,6 +282,8 @@ int launch_step(godar_ctx* c) {
     int N = c->N_used;
     if (N == 0) return GODAR_OK;
     dim3 g((N + B - 1) / B), b(B);
+    // grid of the kernels with XCD-aware block order: a multiple of XCD_COUNT
+    dim3 gx(((N + B - 1) / B + XCD_COUNT - 1) / XCD_COUNT * XCD_COUNT);
     hipStream_t s = c->stream;
     { StageTimer t(c, 0); hipLaunchKernelGGL(k_bin, g, b, 0, s, d); }
     int bits = 1;
@@ -289,20 +297,74 @@ int launch_step(godar_ctx* c) {
         hipLaunchKernelGGL(k_clear_cells, dim3((c->ncells_cap + 1 + 255) / 256), dim3(256), 0, s, d);
         hipLaunchKernelGGL(k_ranges, g, b, 0, s, d);
     }
-    { StageTimer t(c, 3); hipLaunchKernelGGL(k_cand, g, b, 0, s, d); }
+    { StageTimer t(c, 3); hipLaunchKernelGGL(k_cand, gx, b, 0, s, d); }
     int nm = c->Mcap;
     const bool mixed = c->precision == GODAR_MIXED;
-    { StageTimer t(c, 4); hipLaunchKernelGGL(mixed ? k_shelter<true> : k_shelter<false>, g, b, 0, s, d); }
-    { StageTimer t(c, 5); hipLaunchKernelGGL(mixed ? k_contact<true> : k_contact<false>, g, b, 0, s, d); }
-    { StageTimer t(c, 6); hipLaunchKernelGGL(mixed ? k_bond<true> : k_bond<false>, g, b, 0, s, d); }
-    { StageTimer t(c, 7); hipLaunchKernelGGL(mixed ? k_walls<true> : k_walls<false>, g, b, 0, s, d); }
+    { StageTimer t(c, 4); hipLaunchKernelGGL(mixed ? k_shelter<true> : k_shelter<false>, gx, b, 0, s, d); }
+    { StageTimer t(c, 5); hipLaunchKernelGGL(mixed ? k_contact<true> : k_contact<false>, gx, b, 0, s, d); }
+    { StageTimer t(c, 6); hipLaunchKernelGGL(mixed ? k_bond<true> : k_bond<false>, gx, b, 0, s, d); }
+    { StageTimer t(c, 7); hipLaunchKernelGGL(mixed ? k_walls<true> : k_walls<false>, gx, b, 0, s, d); }
     {
         StageTimer t(c, 8);
         hipLaunchKernelGGL(mixed ? k_repair<true> : k_repair<false>, dim3(nm), dim3(64), 0, s, d, nm);
     }
-    { StageTimer t(c, 9); hipLaunchKernelGGL(mixed ? k_fixup<true> : k_fixup<false>, g, b, 0, s, d); }
+    { StageTimer t(c, 9); hipLaunchKernelGGL(mixed ? k_fixup<true> : k_fixup<false>, gx, b, 0, s, d); }
     { StageTimer t(c, 10); hipLaunchKernelGGL(mixed ? k_final<true> : k_final<false>, g, b, 0, s, d); }
     HIPCHK(c, hipGetLastError());
+    c->parity ^= 1;
+    return GODAR_OK;
+}
+
+// Everything a captured step depends on: the kernels' by-value state and
+// the launch shapes (radix scratch pointers and sizes included).
+std::vector<unsigned char> launch_signature(godar_ctx* c) {
+    sync_ds(c);
+    std::vector<unsigned char> sig(sizeof(DS) + sizeof(RadixScratch) + 4 * sizeof(int));
+    unsigned char* p = sig.data();
+    std::memcpy(p, &c->d, sizeof(DS)); p += sizeof(DS);
+    std::memcpy(p, &c->rs, sizeof(RadixScratch)); p += sizeof(RadixScratch);
+    int extra[4] = {c->block, c->precision, c->Mcap, c->N_used};
+    std::memcpy(p, extra, sizeof(extra));
+    return sig;
+}
+
+void drop_graphs(godar_ctx* c) {
+    for (int k = 0; k < 2; ++k) {
+        if (c->gexec[k]) (void)hipGraphExecDestroy(c->gexec[k]);
+        c->gexec[k] = nullptr;
+        c->gsig[k].clear();
+    }
+}
+
+// One step as a replay of the captured graph of the current parity; the
+// graph is (re)captured when the launch signature changed.
+int graph_step(godar_ctx* c) {
+    if (c->N_used == 0) return GODAR_OK;
+    const int par = c->parity;
+    std::vector<unsigned char> sig = launch_signature(c);
+    if (!c->gexec[par] || sig != c->gsig[par]) {
+        if (c->gexec[par]) (void)hipGraphExecDestroy(c->gexec[par]);
+        c->gexec[par] = nullptr;
+        c->gsig[par].clear();
+        HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        int rc = launch_step(c);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(c->stream, &g);
+        c->parity = par;   // the captured step has not run yet
+        bind_parity(c);
+        if (rc != GODAR_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        HIPCHK(c, e);
+        hipGraphExec_t x = nullptr;
+        e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        HIPCHK(c, e);
+        c->gexec[par] = x;
+        c->gsig[par] = std::move(sig);
+    }
+    HIPCHK(c, hipGraphLaunch(c->gexec[par], c->stream));
     c->parity ^= 1;
     return GODAR_OK;
 }
@@ -369,9 +431,10 @@ int godar_create(int64_t max_floes, int32_t max_members, int32_t precision, goda
     } while (0)
     if (e != hipSuccess) { delete c; return GODAR_ERR_HIP; }
     ACHK(dalloc(&d.rec, S));
-    ACHK(dalloc(&d.srec, S));
+    ACHK(dalloc(&d.geo, S));
+    ACHK(dalloc(&d.shel, S));
+    ACHK(dalloc(&d.kin, S));
     ACHK(dalloc(&d.pos_of, S));
-    ACHK(dalloc(&d.wind, S));
     ACHK(dalloc(&d.wbc, 4 * S));
     ACHK(dalloc(&d.member_of, S));
     ACHK(dalloc(&d.members, max_members));
@@ -430,10 +493,11 @@ int godar_create(int64_t max_floes, int32_t max_members, int32_t precision, goda
 int godar_destroy(godar_ctx* c) {
     if (!c) return GODAR_ERR_ARG;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    drop_graphs(c);
     DS& d = c->d;
     double* dbl[] = {d.wbc, d.acc, d.cur, d.fout};
     for (double* p : dbl) dfree(p);
-    dfree(d.rec); dfree(d.srec); dfree(d.pos_of); dfree(d.wind);
+    dfree(d.rec); dfree(d.geo); dfree(d.shel); dfree(d.kin); dfree(d.pos_of);
     dfree(d.member_of); dfree(d.members);
     DevParams* pp = const_cast<DevParams*>(d.params); dfree(pp);
     WindGridDesc* wg = const_cast<WindGridDesc*>(d.wgrids); dfree(wg);
@@ -455,15 +519,26 @@ int godar_destroy(godar_ctx* c) {
 
 const char* godar_last_error(const godar_ctx* c) { return c ? c->err.c_str() : "null context"; }
 
+// Device bytes per floe slot: every array godar_create and alloc_lists size
+// by the floe capacity (radix scratch included), at the context's current
+// list capacities.
+static int64_t bytes_per_floe(const godar_ctx* c) {
+    int64_t per = 0;
+    per += sizeof(FloeRec) + sizeof(GeoRec) + sizeof(ShelRec) + sizeof(KinRec);
+    per += 4 /*pos_of*/ + 4 * 8 /*wbc*/ + 4 /*member_of*/ + 4 + 4 /*ckey, cval*/;
+    per += 4 + 4 /*ocnt, icnt*/ + 2 * (4 + 4) /*hcnt, bcnt*/;
+    per += (int64_t)c->CO * (8 + 8 + 4 + 4) + (int64_t)c->CI * 8;   // oj, od, evf, ov; ik
+    per += 2 * ((int64_t)c->HC * sizeof(HistRec) + (int64_t)c->BC * sizeof(BondRec));
+    per += NACC * 8 + sizeof(WallOut) + 1 /*wflag*/ + 1 /*cls*/ + 3 * 4 /*qlist*/;
+    per += (int64_t)c->SEEDK * 8 + (int64_t)c->HEAPK * 8 + (int64_t)c->TABK * sizeof(EventOut);
+    per += sizeof(WallOut) /*wtab*/ + 4 /*ovw*/ + 6 * 8 /*cur*/ + 1 + 1 /*dirty, affected*/;
+    per += 33 * 8 /*fout*/ + 4 + 4 /*radix keys, values*/;
+    return per;
+}
+
 int godar_member_footprint(const godar_ctx* c, int32_t n, int64_t* bytes) {
     if (!c || !bytes || n < 0) return GODAR_ERR_ARG;
-    int64_t per = 0;
-    per += 17 * 8 + 4 * 8 + 4 * 7;                                   // floe arrays, walls, ints
-    per += (int64_t)c->CO * (4 + 8 + 4 + 4) + (int64_t)c->CI * 4;    // lists
-    per += 2 * ((int64_t)c->HC * sizeof(HistRec) + (int64_t)c->BC * sizeof(BondRec) + 8);
-    per += NACC * 8 + sizeof(WallOut) * 2 + 6 * 8 + 2 + 33 * 8;
-    per += (int64_t)c->SEEDK * 8 + (int64_t)c->HEAPK * 8 + (int64_t)c->TABK * sizeof(EventOut);
-    *bytes = per * n;
+    *bytes = bytes_per_floe(c) * n;
     return GODAR_OK;
 }
 
@@ -697,10 +772,19 @@ int godar_step_async(godar_ctx* c, int32_t nsteps) {
         c->pend_parity = c->parity;
         c->pending = true;
     }
+    const bool graphs = c->use_graphs && !c->prof;
     for (int k = 0; k < nsteps; ++k) {
-        int rc = launch_step(c);
+        int rc = graphs ? graph_step(c) : launch_step(c);
         if (rc != GODAR_OK) return rc;
     }
+    return GODAR_OK;
+}
+
+int godar_use_graphs(godar_ctx* c, int32_t on) {
+    if (!c) return GODAR_ERR_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->use_graphs = on != 0;
+    if (!c->use_graphs) drop_graphs(c);
     return GODAR_OK;
 }
 

@@ -15,21 +15,26 @@ constexpr int WALL_SLOT = 0xFFFF;
 // cls bits: which accumulator rows of a floe hold gathered values
 constexpr uint8_t CLS_CONTACT = 1, CLS_BOND = 2;
 constexpr int FLOE_BLOCK = 256;   // upper bound of the per-floe kernels' block size
+constexpr int XCD_COUNT = 8;      // accelerator complex dies of an MI355X
 
-// Hot per-floe record (96 bytes): everything a pair event reads of a floe.
-// The persistent copy is indexed by floe id; each step a copy sorted by
-// cell (SRec) is built, so that neighbours are close in memory.
+// Persistent per-floe record (96 bytes), indexed by floe id.
 struct FloeRec {
     double x, y, u, v, omega;
     double r, h, mass, inertia, hfa, hfw;
     double theta;
 };
-struct SRec {
-    double x, y, u, v, omega;
-    double r, h, mass, inertia, hfa, hfw;
+// Each step the floes are copied in cell order ("pos"), so that a floe's
+// neighbours are close in memory, and split by what the kernels read:
+//   GeoRec  - neighbour search and every pair test (32 B)
+//   ShelRec - sheltering: draft coefficients and the wind at the floe (32 B)
+//   KinRec  - contact, bond, wall and forcing terms (48 B)
+struct GeoRec {
+    double x, y, r;
     int32_t id;      // floe id (global index of the persistent arrays)
-    int32_t member;
+    int32_t member;  // -1: free slot
 };
+struct ShelRec { double hfa, hfw, wx, wy; };
+struct KinRec { double u, v, omega, h, mass, inertia; };
 
 // Everything the kernels touch.  "id" arrays are indexed by floe id; "pos"
 // arrays by the floe's position in this step's cell order.  Per-floe lists
@@ -52,9 +57,10 @@ struct DS {
     // cell list
     uint32_t *ckey, *cval;     // sort keys / floe ids
     int2* crange;              // [ncells+1] (start, end) in cell order
-    SRec* srec;                // [S] pos
+    GeoRec* geo;               // [S] pos
+    ShelRec* shel;             // [S] pos (wind at the floe this step)
+    KinRec* kin;               // [S] pos
     int32_t* pos_of;           // [S] id -> pos
-    double2* wind;             // [S] pos: wind at the floe this step
     // candidate lists (pos rows; entries are partner positions)
     int32_t* ocnt;
     int64_t* oj;

@@ -36,17 +36,36 @@ namespace godar {
 
 __device__ __forceinline__ void set_err(const DS& S, uint32_t bit) { atomicOr(S.err, bit); }
 
+// Workgroups are dispatched to the 8 XCDs round-robin, and each XCD has its
+// own L2.  Kernels that read their neighbours' rows take this logical block
+// index instead of blockIdx.x, so that each XCD works on one contiguous eighth
+// of the floes (whose neighbours then sit in the same L2).  The grid must be a
+// multiple of XCD_COUNT blocks (launch_grid on the host).
+__device__ __forceinline__ int xcd_block() {
+    const int per = (int)(gridDim.x / XCD_COUNT);
+    return (int)(blockIdx.x % XCD_COUNT) * per + (int)(blockIdx.x / XCD_COUNT);
+}
+
 __device__ __forceinline__ int ent_id(int64_t e) { return (int)(e >> 32); }
 __device__ __forceinline__ int ent_pos(int64_t e) { return (int)(uint32_t)(e & 0xffffffffll); }
 __device__ __forceinline__ int64_t ent(int id, int pos) {
     return ((int64_t)id << 32) | (int64_t)(uint32_t)pos;
 }
 
-__device__ __forceinline__ Floe floe_of(const SRec& s) {
+// floe at position k of this step's cell order
+__device__ __forceinline__ Floe floe_at(const DS& S, int k) {
+    const GeoRec g = S.geo[k];
+    const ShelRec h = S.shel[k];
+    const KinRec q = S.kin[k];
     Floe F;
-    F.x = s.x; F.y = s.y; F.u = s.u; F.v = s.v; F.omega = s.omega;
-    F.r = s.r; F.h = s.h; F.mass = s.mass; F.inertia = s.inertia; F.hfa = s.hfa; F.hfw = s.hfw;
+    F.x = g.x; F.y = g.y; F.u = q.u; F.v = q.v; F.omega = q.omega;
+    F.r = g.r; F.h = q.h; F.mass = q.mass; F.inertia = q.inertia; F.hfa = h.hfa; F.hfw = h.hfw;
     return F;
+}
+
+__device__ __forceinline__ double2 wind_at(const DS& S, int k) {
+    const ShelRec h = S.shel[k];
+    return make_double2(h.wx, h.wy);
 }
 
 __device__ __forceinline__ Floe floe_of(const FloeRec& s) {
@@ -211,20 +230,24 @@ __global__ void k_ranges(DS S) {
     int f = (int)S.cval[k];
     int m = key < (uint32_t)S.ncells ? S.member_of[f] : -1;
     const FloeRec fr = S.rec[f];
-    SRec s;
-    s.x = fr.x; s.y = fr.y; s.u = fr.u; s.v = fr.v; s.omega = fr.omega;
-    s.r = fr.r; s.h = fr.h; s.mass = fr.mass; s.inertia = fr.inertia; s.hfa = fr.hfa; s.hfw = fr.hfw;
-    s.id = f;
-    s.member = m;
-    S.srec[k] = s;
+    GeoRec g;
+    g.x = fr.x; g.y = fr.y; g.r = fr.r; g.id = f; g.member = m;
+    S.geo[k] = g;
+    KinRec q;
+    q.u = fr.u; q.v = fr.v; q.omega = fr.omega; q.h = fr.h; q.mass = fr.mass; q.inertia = fr.inertia;
+    S.kin[k] = q;
     S.pos_of[f] = k;
-    if (m < 0) return;
-    if (k == 0 || S.ckey[k - 1] != key) S.crange[key].x = k;
-    if (k == S.N - 1 || S.ckey[k + 1] != key) S.crange[key].y = k + 1;
-    const DevParams& P = S.params[m];
-    double ua = P.ua, va = P.va;
-    if (P.wind_grid >= 0) sample_wind(S.wgrids[P.wind_grid], S.wpool, fr.x, fr.y, ua, va);
-    S.wind[k] = make_double2(ua, va);
+    double ua = 0.0, va = 0.0;
+    if (m >= 0) {
+        if (k == 0 || S.ckey[k - 1] != key) S.crange[key].x = k;
+        if (k == S.N - 1 || S.ckey[k + 1] != key) S.crange[key].y = k + 1;
+        const DevParams& P = S.params[m];
+        ua = P.ua; va = P.va;
+        if (P.wind_grid >= 0) sample_wind(S.wgrids[P.wind_grid], S.wpool, fr.x, fr.y, ua, va);
+    }
+    ShelRec h;
+    h.hfa = fr.hfa; h.hfw = fr.hfw; h.wx = ua; h.wy = va;
+    S.shel[k] = h;
 }
 
 // ---------------------------------------------------------------------
@@ -233,9 +256,9 @@ __global__ void k_ranges(DS S) {
 //    inner(f) = { k < f : dist <= r(k) + rtree } by k
 // ---------------------------------------------------------------------
 __global__ __launch_bounds__(FLOE_BLOCK) void k_cand(DS S) {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    int k = xcd_block() * blockDim.x + threadIdx.x;
     if (k >= S.N || *S.err) return;
-    const SRec me = S.srec[k];
+    const GeoRec me = S.geo[k];
     if (me.member < 0) { S.ocnt[k] = 0; S.icnt[k] = 0; return; }
     const DevParams& P = S.params[me.member];
     const double rad = me.r + P.rtree;
@@ -253,7 +276,7 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_cand(DS S) {
             const int2 rg = S.crange[P.cell_base + b * P.gx + a];
             for (int q = rg.x; q < rg.y; ++q) {
                 if (q == k) continue;
-                const SRec* o = &S.srec[q];
+                const GeoRec* o = &S.geo[q];
                 double dx = me.x - o->x, dy = me.y - o->y;
                 double d = sqrt(dx * dx + dy * dy);
                 int j = o->id;
@@ -323,12 +346,12 @@ __device__ __forceinline__ void enqueue(bool want, int32_t* q, int32_t* cnt, int
 template <bool MIXED>
 __global__ __launch_bounds__(FLOE_BLOCK) void k_shelter(DS S) {
     using T = typename Prec<MIXED>::T;
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    int k = xcd_block() * blockDim.x + threadIdx.x;
     bool qc = false, qb = false, qw = false;
     bool live = k < S.N && !*S.err;
-    SRec me;
+    GeoRec me;
     if (live) {
-        me = S.srec[k];
+        me = S.geo[k];
         live = me.member >= 0;
     }
     if (live) {
@@ -343,11 +366,12 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_shelter(DS S) {
         qw = me.y - me.r < 0 || me.y + me.r > P.ny || me.x - me.r < 0 || me.x + me.r > P.nx;
         T sha = T(1), shw = T(1);
         const T uwi = T(P.uw), vwi = T(P.vw);
+        const ShelRec msh = S.shel[k];
         const int ni = S.icnt[k];
         for (int q = 0; q < ni; ++q) {
             const int64_t e = S.ik[q * SS + k];
             const int i = ent_id(e), qi = ent_pos(e);
-            const SRec* o = &S.srec[qi];
+            const GeoRec* o = &S.geo[qi];
             double dx = o->x - me.x, dy = o->y - me.y;
             double dist = sqrt(dx * dx + dy * dy);
             double deltan = o->r + me.r - dist;
@@ -367,19 +391,18 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_shelter(DS S) {
             if (P.shelter) {
                 // pair (i, f): cosa/sina point from i to f; floe f takes hsf*(i,f)
                 double cosa = (me.x - o->x) / dist, sina = (me.y - o->y) / dist;
-                const double2 wi = S.wind[qi];
+                const ShelRec oh = S.shel[qi];
                 T dn = T(deltan), c = T(cosa), sn = T(sina);
-                sha = dmin(sha, S_shelter(T(o->hfa), T(me.hfa), dn, c, sn, T(wi.x), T(wi.y)));
-                shw = dmin(shw, S_shelter(T(o->hfw), T(me.hfw), dn, c, sn, uwi, vwi));
+                sha = dmin(sha, S_shelter(T(oh.hfa), T(msh.hfa), dn, c, sn, T(oh.wx), T(oh.wy)));
+                shw = dmin(shw, S_shelter(T(oh.hfw), T(msh.hfw), dn, c, sn, uwi, vwi));
             }
         }
         const int no = S.ocnt[k];
-        const double2 wf = S.wind[k];
         const uint32_t fl = P.shelter ? EV_SHELTER : 0u;
         for (int s = 0; s < no; ++s) {
             const int64_t e = S.oj[s * SS + k];
             const int j = ent_id(e), qj = ent_pos(e);
-            const SRec* o = &S.srec[qj];
+            const GeoRec* o = &S.geo[qj];
             double dx = me.x - o->x, dy = me.y - o->y;
             double dist = sqrt(dx * dx + dy * dy);
             double deltan = me.r + o->r - dist;
@@ -390,9 +413,10 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_shelter(DS S) {
             if (P.shelter) {
                 // pair (f, j): floe f takes hsf*(j,f), evaluated with -cosa, -sina
                 double cosa = (o->x - me.x) / dist, sina = (o->y - me.y) / dist;
+                const ShelRec oh = S.shel[qj];
                 T dn = T(deltan), c = T(cosa), sn = T(sina);
-                sha = dmin(sha, S_shelter(T(o->hfa), T(me.hfa), dn, -c, -sn, T(wf.x), T(wf.y)));
-                shw = dmin(shw, S_shelter(T(o->hfw), T(me.hfw), dn, -c, -sn, uwi, vwi));
+                sha = dmin(sha, S_shelter(T(oh.hfa), T(msh.hfa), dn, -c, -sn, T(msh.wx), T(msh.wy)));
+                shw = dmin(shw, S_shelter(T(oh.hfw), T(msh.hfw), dn, -c, -sn, uwi, vwi));
             }
             S.evf[s * SS + k] = fl;
             S.ov[s * SS + k] = -1;
@@ -431,10 +455,10 @@ __device__ __forceinline__ void load_contact_state(const DS& S, int i, int j, Pa
 // update_shape, coulomb; stepper.f90:103-146) + the owner's new histories.
 template <bool MIXED>
 __global__ __launch_bounds__(FLOE_BLOCK) void k_contact(DS S) {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int t = xcd_block() * blockDim.x + threadIdx.x;
     if (*S.err || t >= S.qn[0]) return;
     const int k = S.qlist[t];
-    const SRec me = S.srec[k];
+    const GeoRec me = S.geo[k];
     const int m = me.member;
     const DevParams& P = S.params[m];
     const int tstep = S.members[m].tstep;
@@ -443,7 +467,7 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_contact(DS S) {
     const int64_t SS = S.S;
     const int64_t* oj = S.oj;
     const int64_t* ik = S.ik;
-    const Floe F0 = floe_of(me);
+    const Floe F0 = floe_at(S, k);
     double acc[NACC];
     acc_init(acc);
     uint32_t over = 0;
@@ -453,7 +477,7 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_contact(DS S) {
     for (int q = 0; q < ni; ++q) {
         const int64_t e = ik[q * SS + k];
         const int i = ent_id(e), qi = ent_pos(e);
-        Floe Fi = floe_of(S.srec[qi]);
+        Floe Fi = floe_at(S, qi);
         Floe Ff = F0;
         Kin kn;
         kinematics(Fi, Ff, kn);
@@ -475,7 +499,7 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_contact(DS S) {
         const int64_t e = oj[s * SS + k];
         const int j = ent_id(e), qj = ent_pos(e);
         Floe Fi = F0;
-        Floe Fj = floe_of(S.srec[qj]);
+        Floe Fj = floe_at(S, qj);
         Kin kn;
         kinematics(Fi, Fj, kn);
         // no overlap: reset_contact (reset.f90:1-15) drops the history
@@ -546,10 +570,10 @@ __device__ __forceinline__ void load_bond_state(const DS& S, int i, int q, PairS
 // stepper.f90:148-198) + the owner's new bond records.
 template <bool MIXED>
 __global__ __launch_bounds__(FLOE_BLOCK) void k_bond(DS S) {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int t = xcd_block() * blockDim.x + threadIdx.x;
     if (*S.err || t >= S.qn[1]) return;
     const int k = S.qlist[S.S + t];
-    const SRec me = S.srec[k];
+    const GeoRec me = S.geo[k];
     const int m = me.member;
     const DevParams& P = S.params[m];
     const int tstep = S.members[m].tstep;
@@ -559,7 +583,7 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_bond(DS S) {
     const int64_t SS = S.S;
     const int64_t* oj = S.oj;
     const int64_t* ik = S.ik;
-    const Floe F0 = floe_of(me);
+    const Floe F0 = floe_at(S, k);
     double acc[NACC];
     acc_init(acc);
     uint32_t over = 0;
@@ -574,7 +598,7 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_bond(DS S) {
         for (int b = 0; b < bc; ++b)
             if (S.bond_old[(int64_t)b * SS + i].j == f) { rq = b; break; }
         if (rq < 0 && !bond_step) continue;
-        Floe Fi = floe_of(S.srec[qi]);
+        Floe Fi = floe_at(S, qi);
         Floe Ff = F0;
         Kin kn;
         kinematics(Fi, Ff, kn);
@@ -608,7 +632,7 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_bond(DS S) {
             if (S.bond_old[(int64_t)b * SS + f].j == j) { rq = b; break; }
         if (rq < 0 && !bond_step) continue;
         Floe Fi = F0;
-        Floe Fj = floe_of(S.srec[qj]);
+        Floe Fj = floe_at(S, qj);
         Kin kn;
         kinematics(Fi, Fj, kn);
         PairState ps;
@@ -642,16 +666,16 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_bond(DS S) {
 // Walls of the queued floes (verify_bc, boundaries.f90:1-97).
 template <bool MIXED>
 __global__ __launch_bounds__(FLOE_BLOCK) void k_walls(DS S) {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int t = xcd_block() * blockDim.x + threadIdx.x;
     if (*S.err || t >= S.qn[2]) return;
     const int k = S.qlist[2 * S.S + t];
-    const SRec me = S.srec[k];
+    const GeoRec me = S.geo[k];
     const int m = me.member;
     const DevParams& P = S.params[m];
     const int first = S.members[m].first;
     const int f = me.id;
     const int64_t SS = S.S;
-    Floe Fw = floe_of(me);
+    Floe Fw = floe_at(S, k);
     double hist[4] = {S.wbc[0 * SS + f], S.wbc[1 * SS + f], S.wbc[2 * SS + f], S.wbc[3 * SS + f]};
     WallOut W;
     walls<MIXED>(P, Fw, hist, W);
@@ -827,7 +851,7 @@ __global__ void k_repair(DS S, int nmembers) {
             o.flags = EV_REMOVED;
             o.ridged = -1;
         } else {
-            const double2 wf = S.wind[kf];
+            const double2 wf = wind_at(S, kf);
             pair_event<MIXED>(P, tstep, b, j - first, Fi, Fj, wf.x, wf.y, P.uw, P.vw, ps, o, 0);
         }
         bool okh = put_hist(S, f, j, ps.deltat != 0.0 || ps.thetarelc != 0.0, ps.deltat, ps.thetarelc);
@@ -877,9 +901,9 @@ __global__ void k_repair(DS S, int nmembers) {
 // ---------------------------------------------------------------------
 template <bool MIXED>
 __global__ __launch_bounds__(FLOE_BLOCK) void k_fixup(DS S) {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    int k = xcd_block() * blockDim.x + threadIdx.x;
     if (k >= S.N || *S.err) return;
-    const SRec me = S.srec[k];
+    const GeoRec me = S.geo[k];
     const int m = me.member;
     if (m < 0) return;
     const int f = me.id;
@@ -891,7 +915,7 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_fixup(DS S) {
     const int64_t* oj = S.oj;
     const int64_t* ik = S.ik;
     const EventOut* etab = S.etab + (int64_t)S.TABK * first;
-    const Floe F0 = floe_of(me);
+    const Floe F0 = floe_at(S, k);
     double acc[NACC];
     acc_init(acc);
     const int ni = S.icnt[k];
@@ -904,18 +928,18 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_fixup(DS S) {
             EventOut o = etab[idx];
             if (!(o.flags & EV_REMOVED)) apply_j(acc, o);
         } else {
-            Floe Fi = floe_of(S.srec[qi]);
+            Floe Fi = floe_at(S, qi);
             Floe Ff = F0;
             PairState ps;
             load_pair_state(S, i, f, ps);
-            const double2 wi = S.wind[qi];
+            const double2 wi = wind_at(S, qi);
             EventOut o;
             pair_event<MIXED>(P, tstep, i - first, f - first, Fi, Ff, wi.x, wi.y, P.uw, P.vw, ps, o, 2);
             apply_j(acc, o);
         }
     }
     const int no = S.ocnt[k];
-    const double2 wf = S.wind[k];
+    const double2 wf = wind_at(S, k);
     for (int s = 0; s < no; ++s) {
         int idx = S.ov[s * SS + k];
         if (idx >= 0) {
@@ -925,7 +949,7 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_fixup(DS S) {
             const int64_t e = oj[s * SS + k];
             const int j = ent_id(e), qj = ent_pos(e);
             Floe Fi = F0;
-            Floe Fj = floe_of(S.srec[qj]);
+            Floe Fj = floe_at(S, qj);
             PairState ps;
             load_pair_state(S, f, j, ps);
             EventOut o;
@@ -946,13 +970,13 @@ template <bool MIXED>
 __global__ __launch_bounds__(FLOE_BLOCK) void k_final(DS S) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= S.N || *S.err) return;
-    const SRec me = S.srec[k];
+    const GeoRec me = S.geo[k];
     const int m = me.member;
     if (m < 0) return;
     const int f = me.id;
     const DevParams& P = S.params[m];
     const int64_t SS = S.S;
-    Floe F = floe_of(me);
+    Floe F = floe_at(S, k);
     if (S.dirty[f]) {
         F.r = S.cur[0 * SS + f]; F.h = S.cur[1 * SS + f]; F.mass = S.cur[2 * SS + f];
         F.inertia = S.cur[3 * SS + f]; F.hfa = S.cur[4 * SS + f]; F.hfw = S.cur[5 * SS + f];
@@ -983,7 +1007,7 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_final(DS S) {
     // the reference's minima stay at their zero initial value
     double sha = P.shelter ? a[A_SHA] : 0.0;
     double shw = P.shelter ? a[A_SHW] : 0.0;
-    const double2 w = S.wind[k];
+    const double2 w = wind_at(S, k);
     Forcing Fo;
     forcing<MIXED>(P, F, w.x, w.y, P.uw, P.vw, sha, shw, Fo);
     double tfx = a[A_FCX] + a[A_FBX] + Fo.fax + Fo.fwx + Fo.fcorx + W.fx_bc;
@@ -1019,7 +1043,7 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_final(DS S) {
 __global__ void k_totals(DS S, int64_t* out) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     int64_t c = 0, ct = 0, bd = 0;
-    if (k < S.N && S.srec[k].member >= 0) {
+    if (k < S.N && S.geo[k].member >= 0) {
         int no = S.ocnt[k];
         for (int s = 0; s < no; ++s) {
             uint32_t fl = S.evf[(int64_t)s * S.S + k];
@@ -1027,7 +1051,7 @@ __global__ void k_totals(DS S, int64_t* out) {
             ++c;
             if (fl & EV_CONTACT) ++ct;
         }
-        int f = S.srec[k].id;
+        int f = S.geo[k].id;
         int nb = S.bcnt_old[f];
         for (int q = 0; q < nb; ++q) {
             const BondRec& b = S.bond_old[(int64_t)q * S.S + f];

@@ -139,6 +139,10 @@ class Context:
     def set_block(self, threads: int):
         self._chk(self.L.godar_set_block(self.h, int(threads)))
 
+    def use_graphs(self, on: bool = True):
+        """Replay a captured HIP graph per step (default) or launch kernels one by one."""
+        self._chk(self.L.godar_use_graphs(self.h, int(bool(on))))
+
     def profile(self, on: bool = True):
         self._chk(self.L.godar_profile(self.h, 1 if on else 0))
 

@@ -160,6 +160,12 @@ int godar_get_totals(godar_ctx *ctx, int64_t *totals);
 
 /* Pipeline tuning (autotuner): threads per block of the per-floe kernels. */
 int godar_set_block(godar_ctx *ctx, int32_t threads);
+/* Step launch mode (no counterpart in the reference): on (the default), each
+ * step replays a captured HIP graph of the whole kernel sequence, recaptured
+ * whenever members, capacities or the block size change; off, the kernels are
+ * launched one by one.  Results are identical.  Steps timed by godar_profile
+ * are always launched one by one. */
+int godar_use_graphs(godar_ctx *ctx, int32_t on);
 
 /*
  * Stage timing (tooling; no counterpart in the reference, which only times

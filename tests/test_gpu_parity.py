@@ -199,6 +199,34 @@ def test_batch_invariance_bitwise():
         ctx.close()
 
 
+def test_graph_replay_matches_eager():
+    """Graph-replayed steps == kernel-by-kernel steps, bitwise, on every
+    replay: replays alternate with eager work (pair and force read-backs) and
+    synchronisations, and a member added mid-run forces a recapture."""
+    fa = g.config0_member(21, 700)
+    fb = kick(squeeze(packed_box(64, 3)), 1)
+    pa, pb = config0_params(fa), config0_params(fb, sigmanc_crit=1e5)
+    for prec in ("fp64", "mixed"):
+        eager, graph = _ctx(fa.n + fb.n, 2, prec), _ctx(fa.n + fb.n, 2, prec)
+        eager.use_graphs(False)
+        ms_e, ms_g = [eager.add_field(pa, fa)], [graph.add_field(pa, fa)]
+        for rnd in range(8):
+            if rnd == 3:
+                ms_e.append(eager.add_field(pb, fb))
+                ms_g.append(graph.add_field(pb, fb))
+            k = 1 + rnd % 3
+            eager.step(k)
+            graph.step(k)
+            for me, mg in zip(ms_e, ms_g):
+                a, b = graph.floes(mg), eager.floes(me)
+                for key in STATE:
+                    assert np.array_equal(a[key], b[key]), (prec, rnd, key)
+                assert _pairs(graph, mg, "contact") == _pairs(eager, me, "contact")
+                assert np.array_equal(graph.forces(mg)["tfx"], eager.forces(me)["tfx"])
+        eager.close()
+        graph.close()
+
+
 def test_capacity_overflow_rolls_back():
     """A too-small candidate list aborts the step without advancing the state;
     after reserving more room the run continues exactly as an unhindered one.
