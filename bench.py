@@ -56,10 +56,13 @@ def run_batched(args):
     batch = build_batch(args.members, args.seed)
     nfloes = sum(f.n for f, _ in batch)
     ctx = Context(nfloes, len(batch), args.precision)
-    if args.block:
-        ctx.set_block(args.block)
     mids = [ctx.add_field(p, f) for f, p in batch]
     ctx.step(args.warmup)
+    if args.block:
+        ctx.set_block(args.block)
+        blocks = ctx.stage_blocks()
+    else:
+        blocks = ctx.autotune(2)      # real steps: part of the warm-up
     ctx.sync()
     # timed region: graph-replayed steps, nothing else on the stream
     t0 = time.perf_counter()
@@ -76,9 +79,32 @@ def run_batched(args):
     ctx.profile(False)
     ncand, ncont, nbond, tot = pair_totals(ctx, mids)
     ctx.close()
-    return dict(nfloes=nfloes, members=len(batch), seconds=dt, stages=stages,
+    return dict(nfloes=nfloes, members=len(batch), seconds=dt, stages=stages, blocks=blocks,
                 ncand=ncand, ncont=ncont, nbond=nbond, totals=tot,
                 alg_bytes_step=algorithmic_bytes(nfloes, ncand, ncont, nbond))
+
+
+def run_serving(nreq: int, seed: int = 4, hbm_gib: float = 16.0, chunk: int = 25):
+    """Served requests (Poisson arrivals at 20/s, 1,000-50,000 floes,
+    500-5,000 steps, mixed precision) through the continuous-batching loop."""
+    from godar_amd import workloads as W
+    from godar_amd.serve import Request, Server
+    t = time.time()
+    reqs = [Request(k, f, p, s, a) for k, (f, p, s, a) in enumerate(W.config4_requests(nreq, seed))]
+    log(f"[bench] generated {nreq} served requests in {time.time() - t:.1f}s")
+    srv = Server(hbm_bytes=int(hbm_gib * (1 << 30)), max_members=256, precision="mixed",
+                 chunk=chunk)
+    # warm the context (graph capture, first launches) on a throwaway request
+    warm = Request(-1, reqs[0].field, reqs[0].params, 3, 0.0)
+    srv.run([warm])
+    out = srv.run(reqs)
+    srv.close()
+    return dict(requests=out["requests"], arrival_rate_per_s=20.0, precision="mixed",
+                hbm_budget_gib=hbm_gib, capacity_floes=out["capacity_floes"],
+                floe_steps=out["floe_steps"], makespan_s=out["makespan_s"],
+                floe_steps_per_s=out["floe_steps_per_s"],
+                latency_p50_s=out["latency_p50_s"], latency_p99_s=out["latency_p99_s"],
+                peak_members=out["peak_members"], peak_floes=out["peak_floes"], chunk=chunk)
 
 
 def run_plain(which: str, steps: int, warmup: int):
@@ -165,6 +191,8 @@ def main():
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--quick", action="store_true", help="batched run only (for profilers)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--serve-requests", type=int, default=100,
+                    help="served requests of the continuous-batching leg (0: skip)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (calibrates rocprof byte counters)")
     args = ap.parse_args()
@@ -243,6 +271,7 @@ def main():
         },
         "stages_ms_per_step": {k: v[0] / v[1] for k, v in stages.items()},
         "pair_totals_last_step": B["totals"],
+        "block_sizes": B["blocks"],
     }
     if not args.quick:
         p0, ms0 = run_plain("config0", 500, 20)
@@ -252,6 +281,8 @@ def main():
         out["batched_vs_plain_config0"] = value / p0
         out["batched_vs_plain_config1"] = value / p1
         out["pos_err_1e4"] = {"mixed": pos_error_1e4("mixed"), "fp64": pos_error_1e4("fp64")}
+        if args.serve_requests > 0:
+            out["serving"] = run_serving(args.serve_requests)
         cb = cpu_baseline(args.cpu_seconds)
         out["cpu_baseline"] = cb
         out["vs_cpu_baseline"] = value / cb["value"]

@@ -30,11 +30,13 @@ EXPORTS = [
     "godar_member_add", "godar_member_remove", "godar_member_footprint",
     "godar_capacity_left", "godar_set_forcing", "godar_set_wind_grid", "godar_set_bonds",
     "godar_step", "godar_step_async", "godar_sync", "godar_member_tstep", "godar_reserve",
+    "godar_member_run_until", "godar_member_tsteps",
     "godar_get_floes", "godar_record_forces", "godar_get_forces", "godar_get_walls",
     "godar_get_pairs", "godar_get_bond_state", "godar_get_stats", "godar_set_block", "godar_use_graphs",
+    "godar_set_stage_block", "godar_get_stage_blocks", "godar_autotune",
     "godar_profile", "godar_get_profile", "godar_stage_name", "godar_get_totals",
 ]
-NSTAGES = 11
+NSTAGES = 12
 
 
 class GodarError(RuntimeError):
@@ -79,6 +81,8 @@ def load(path: str = LIB_PATH):
         "godar_step_async": (_i32, [_vp, _i32]),
         "godar_sync": (_i32, [_vp]),
         "godar_member_tstep": (_i32, [_vp, _i32, _ip]),
+        "godar_member_run_until": (_i32, [_vp, _i32, _i32]),
+        "godar_member_tsteps": (_i32, [_vp, _ip]),
         "godar_reserve": (_i32, [_vp, _i32, _i32, _i32, _i32]),
         "godar_get_floes": (_i32, [_vp, _i32] + [_dp] * 12),
         "godar_record_forces": (_i32, [_vp, _i32]),
@@ -89,6 +93,9 @@ def load(path: str = LIB_PATH):
         "godar_get_stats": (_i32, [_vp, _i32, _lp]),
         "godar_set_block": (_i32, [_vp, _i32]),
         "godar_use_graphs": (_i32, [_vp, _i32]),
+        "godar_set_stage_block": (_i32, [_vp, _i32, _i32]),
+        "godar_get_stage_blocks": (_i32, [_vp, _ip]),
+        "godar_autotune": (_i32, [_vp, _i32]),
         "godar_profile": (_i32, [_vp, _i32]),
         "godar_get_profile": (_i32, [_vp, _dp, _lp]),
         "godar_stage_name": (ctypes.c_char_p, [_i32]),

@@ -16,10 +16,11 @@ namespace godar {
 __global__ void k_bin(DS);
 __global__ void k_clear_cells(DS);
 __global__ void k_ranges(DS);
-__global__ void k_cand(DS);
-template <bool M> __global__ void k_shelter(DS);
+template <bool M> __global__ void k_cand(DS);
 template <bool M> __global__ void k_contact(DS);
 template <bool M> __global__ void k_bond(DS);
+__global__ void k_bond_gather(DS);
+__global__ void k_stress(DS);
 template <bool M> __global__ void k_walls(DS);
 template <bool M> __global__ void k_repair(DS, int);
 template <bool M> __global__ void k_fixup(DS);
@@ -48,6 +49,7 @@ struct godar_ctx {
     int CO = 32, CI = 32, HC = 16, BC = 16;
     int SEEDK = 2, HEAPK = 16, TABK = 4;
     int block = 256;
+    int sblock[GODAR_NSTAGES];          // threads per block of each stage's per-floe kernels
     int record = 0;
     int parity = 0;
     int32_t N_used = 0;
@@ -75,7 +77,7 @@ struct godar_ctx {
     int64_t stage_n[GODAR_NSTAGES] = {0};
     // state of the last complete step, for rolling back an aborted step
     int pend_parity = 0;
-    std::vector<int32_t> pend_t0;
+    int64_t pend_done0 = 0;
     bool pending = false;
     uint32_t last_err_bits = 0;
     bool auto_grow = true;
@@ -146,7 +148,8 @@ void derive_params(const godar_params& p, DevParams& d) {
 
 void free_lists(godar_ctx* c) {
     DS& d = c->d;
-    dfree(d.oj); dfree(d.od); dfree(d.ik); dfree(d.evf); dfree(d.ov);
+    dfree(d.oj); dfree(d.od); dfree(d.ik); dfree(d.evf); dfree(d.ov); dfree(d.bterm);
+    dfree(d.ssc); dfree(d.ssb);
     for (int k = 0; k < 2; ++k) { dfree(c->hist[k]); dfree(c->bond[k]); }
 }
 
@@ -159,6 +162,9 @@ hipError_t alloc_lists(godar_ctx* c) {
     if ((e = dalloc(&d.ik, S * c->CI)) != hipSuccess) return e;
     if ((e = dalloc(&d.evf, S * c->CO)) != hipSuccess) return e;
     if ((e = dalloc(&d.ov, S * c->CO)) != hipSuccess) return e;
+    if ((e = dalloc(&d.bterm, S * c->BC)) != hipSuccess) return e;
+    if ((e = dalloc(&d.ssc, S * c->CO)) != hipSuccess) return e;
+    if ((e = dalloc(&d.ssb, S * c->CO)) != hipSuccess) return e;
     for (int k = 0; k < 2; ++k) {
         if ((e = dalloc(&c->hist[k], S * c->HC)) != hipSuccess) return e;
         if ((e = dalloc(&c->bond[k], S * c->BC)) != hipSuccess) return e;
@@ -205,6 +211,13 @@ int upload_tables(godar_ctx* c) {
 int upload_members_only(godar_ctx* c) {
     HIPCHK(c, hipMemcpyAsync(c->d.members, c->members.data(), sizeof(Member) * c->Mcap,
                              hipMemcpyHostToDevice, c->stream));
+    return GODAR_OK;
+}
+
+// steps completed without error since the context was created
+int steps_done(godar_ctx* c, int64_t* out) {
+    HIPCHK(c, hipMemcpyAsync(out, c->d.steps_done, sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return GODAR_OK;
 }
 
@@ -275,17 +288,22 @@ void collect_events(godar_ctx* c) {
     c->ev_pending.clear();
 }
 
+// grid of a per-floe kernel of stage st over N items; kernels with the
+// XCD-aware block order need a multiple of XCD_COUNT blocks
+static dim3 grid_for(int N, int B, bool xcd) {
+    int g = (N + B - 1) / B;
+    if (xcd) g = (g + XCD_COUNT - 1) / XCD_COUNT * XCD_COUNT;
+    return dim3(g);
+}
+
 int launch_step(godar_ctx* c) {
     DS& d = c->d;
     sync_ds(c);
-    const int B = c->block;
     int N = c->N_used;
     if (N == 0) return GODAR_OK;
-    dim3 g((N + B - 1) / B), b(B);
-    // grid of the kernels with XCD-aware block order: a multiple of XCD_COUNT
-    dim3 gx(((N + B - 1) / B + XCD_COUNT - 1) / XCD_COUNT * XCD_COUNT);
+    const int* B = c->sblock;
     hipStream_t s = c->stream;
-    { StageTimer t(c, 0); hipLaunchKernelGGL(k_bin, g, b, 0, s, d); }
+    { StageTimer t(c, 0); hipLaunchKernelGGL(k_bin, grid_for(N, B[0], false), dim3(B[0]), 0, s, d); }
     int bits = 1;
     while ((1u << bits) <= (uint32_t)c->ncells_cap && bits < 32) ++bits;
     {
@@ -295,21 +313,52 @@ int launch_step(godar_ctx* c) {
     {
         StageTimer t(c, 2);
         hipLaunchKernelGGL(k_clear_cells, dim3((c->ncells_cap + 1 + 255) / 256), dim3(256), 0, s, d);
-        hipLaunchKernelGGL(k_ranges, g, b, 0, s, d);
+        hipLaunchKernelGGL(k_ranges, grid_for(N, B[2], false), dim3(B[2]), 0, s, d);
     }
-    { StageTimer t(c, 3); hipLaunchKernelGGL(k_cand, gx, b, 0, s, d); }
     int nm = c->Mcap;
     const bool mixed = c->precision == GODAR_MIXED;
-    { StageTimer t(c, 4); hipLaunchKernelGGL(mixed ? k_shelter<true> : k_shelter<false>, gx, b, 0, s, d); }
-    { StageTimer t(c, 5); hipLaunchKernelGGL(mixed ? k_contact<true> : k_contact<false>, gx, b, 0, s, d); }
-    { StageTimer t(c, 6); hipLaunchKernelGGL(mixed ? k_bond<true> : k_bond<false>, gx, b, 0, s, d); }
-    { StageTimer t(c, 7); hipLaunchKernelGGL(mixed ? k_walls<true> : k_walls<false>, gx, b, 0, s, d); }
+    {
+        StageTimer t(c, 3);
+        hipLaunchKernelGGL(mixed ? k_cand<true> : k_cand<false>, grid_for(N, B[3], true),
+                           dim3(B[3]), 0, s, d);
+    }
+    {
+        StageTimer t(c, 4);
+        hipLaunchKernelGGL(mixed ? k_contact<true> : k_contact<false>, grid_for(N, B[4], true),
+                           dim3(B[4]), 0, s, d);
+    }
+    {
+        StageTimer t(c, 5);
+        hipLaunchKernelGGL(mixed ? k_bond<true> : k_bond<false>, grid_for(N, B[5], true),
+                           dim3(B[5]), 0, s, d);
+    }
+    {
+        StageTimer t(c, 6);
+        hipLaunchKernelGGL(k_bond_gather, grid_for(N, B[6], true), dim3(B[6]), 0, s, d);
+    }
+    {
+        StageTimer t(c, 7);
+        hipLaunchKernelGGL(mixed ? k_walls<true> : k_walls<false>, grid_for(N, B[7], true),
+                           dim3(B[7]), 0, s, d);
+    }
     {
         StageTimer t(c, 8);
         hipLaunchKernelGGL(mixed ? k_repair<true> : k_repair<false>, dim3(nm), dim3(64), 0, s, d, nm);
     }
-    { StageTimer t(c, 9); hipLaunchKernelGGL(mixed ? k_fixup<true> : k_fixup<false>, gx, b, 0, s, d); }
-    { StageTimer t(c, 10); hipLaunchKernelGGL(mixed ? k_final<true> : k_final<false>, g, b, 0, s, d); }
+    {
+        StageTimer t(c, 9);
+        hipLaunchKernelGGL(mixed ? k_fixup<true> : k_fixup<false>, grid_for(N, B[9], true),
+                           dim3(B[9]), 0, s, d);
+    }
+    if (c->record) {
+        StageTimer t(c, 10);
+        hipLaunchKernelGGL(k_stress, grid_for(N, B[10], true), dim3(B[10]), 0, s, d);
+    }
+    {
+        StageTimer t(c, 11);
+        hipLaunchKernelGGL(mixed ? k_final<true> : k_final<false>, grid_for(N, B[11], false),
+                           dim3(B[11]), 0, s, d);
+    }
     HIPCHK(c, hipGetLastError());
     c->parity ^= 1;
     return GODAR_OK;
@@ -319,11 +368,13 @@ int launch_step(godar_ctx* c) {
 // the launch shapes (radix scratch pointers and sizes included).
 std::vector<unsigned char> launch_signature(godar_ctx* c) {
     sync_ds(c);
-    std::vector<unsigned char> sig(sizeof(DS) + sizeof(RadixScratch) + 4 * sizeof(int));
+    std::vector<unsigned char> sig(sizeof(DS) + sizeof(RadixScratch) + sizeof(c->sblock) +
+                                   3 * sizeof(int));
     unsigned char* p = sig.data();
     std::memcpy(p, &c->d, sizeof(DS)); p += sizeof(DS);
     std::memcpy(p, &c->rs, sizeof(RadixScratch)); p += sizeof(RadixScratch);
-    int extra[4] = {c->block, c->precision, c->Mcap, c->N_used};
+    std::memcpy(p, c->sblock, sizeof(c->sblock)); p += sizeof(c->sblock);
+    int extra[3] = {c->precision, c->Mcap, c->N_used};
     std::memcpy(p, extra, sizeof(extra));
     return sig;
 }
@@ -412,9 +463,10 @@ int godar_create(int64_t max_floes, int32_t max_members, int32_t precision, goda
     *out = nullptr;
     godar_ctx* c = new godar_ctx();
     c->precision = precision;
+    for (int k = 0; k < GODAR_NSTAGES; ++k) c->sblock[k] = FLOE_BLOCK;
     c->S = max_floes;
     c->Mcap = max_members;
-    c->members.assign(max_members, Member{0, 0, 1, 0});
+    c->members.assign(max_members, Member{0, 0, 1, 0, 0});
     c->dparams.assign(max_members, DevParams{});
     c->cell_n.assign(max_members, 0);
     for (auto& m : c->members) m.active = 0;
@@ -455,6 +507,7 @@ int godar_create(int64_t max_floes, int32_t max_members, int32_t precision, goda
     ACHK(dalloc(&d.wflag, S));
     ACHK(dalloc(&d.cls, S));
     ACHK(dalloc(&d.qlist, 3 * S));
+    ACHK(dalloc(&d.bgat, S));
     ACHK(dalloc(&d.qn, 3));
     ACHK(dalloc(&d.seeds, (int64_t)c->SEEDK * S));
     ACHK(dalloc(&d.nseeds, max_members));
@@ -470,9 +523,11 @@ int godar_create(int64_t max_floes, int32_t max_members, int32_t precision, goda
     ACHK(dalloc(&d.stats, 4 * (int64_t)max_members));
     ACHK(dalloc(&d.fout, 33 * S));
     ACHK(dalloc(&d.err, 1));
+    ACHK(dalloc(&d.steps_done, 1));
     ACHK(radix_alloc(c->rs, S));
 #undef ACHK
     (void)hipMemsetAsync(d.err, 0, sizeof(uint32_t), c->stream);
+    (void)hipMemsetAsync(d.steps_done, 0, sizeof(int64_t), c->stream);
     (void)hipMemsetAsync(d.dirty, 0, S, c->stream);
     (void)hipMemsetAsync(d.affected, 0, S, c->stream);
     (void)hipMemsetAsync(d.stats, 0, sizeof(int64_t) * 4 * max_members, c->stream);
@@ -506,9 +561,9 @@ int godar_destroy(godar_ctx* c) {
     dfree(d.crange);
     for (int k = 0; k < 2; ++k) { dfree(c->hcnt[k]); dfree(c->bcnt[k]); }
     free_lists(c);
-    dfree(d.wout); dfree(d.wflag); dfree(d.cls); dfree(d.qlist); dfree(d.qn); dfree(d.seeds); dfree(d.nseeds); dfree(d.heap); dfree(d.etab);
+    dfree(d.wout); dfree(d.wflag); dfree(d.cls); dfree(d.qlist); dfree(d.qn); dfree(d.bgat); dfree(d.seeds); dfree(d.nseeds); dfree(d.heap); dfree(d.etab);
     dfree(d.netab); dfree(d.wtab); dfree(d.nwtab); dfree(d.ovw); dfree(d.dirty);
-    dfree(d.affected); dfree(d.stats); dfree(d.err);
+    dfree(d.affected); dfree(d.stats); dfree(d.err); dfree(d.steps_done);
     radix_free(c->rs);
     collect_events(c);
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
@@ -527,8 +582,9 @@ static int64_t bytes_per_floe(const godar_ctx* c) {
     per += sizeof(FloeRec) + sizeof(GeoRec) + sizeof(ShelRec) + sizeof(KinRec);
     per += 4 /*pos_of*/ + 4 * 8 /*wbc*/ + 4 /*member_of*/ + 4 + 4 /*ckey, cval*/;
     per += 4 + 4 /*ocnt, icnt*/ + 2 * (4 + 4) /*hcnt, bcnt*/;
-    per += (int64_t)c->CO * (8 + 8 + 4 + 4) + (int64_t)c->CI * 8;   // oj, od, evf, ov; ik
+    per += (int64_t)c->CO * (8 + 8 + 4 + 4 + 8 + 8) + (int64_t)c->CI * 8;   // oj od evf ov ssc ssb; ik
     per += 2 * ((int64_t)c->HC * sizeof(HistRec) + (int64_t)c->BC * sizeof(BondRec));
+    per += (int64_t)c->BC * sizeof(BondTerms) + 1 /*bgat*/;
     per += NACC * 8 + sizeof(WallOut) + 1 /*wflag*/ + 1 /*cls*/ + 3 * 4 /*qlist*/;
     per += (int64_t)c->SEEDK * 8 + (int64_t)c->HEAPK * 8 + (int64_t)c->TABK * sizeof(EventOut);
     per += sizeof(WallOut) /*wtab*/ + 4 /*ovw*/ + 6 * 8 /*cur*/ + 1 + 1 /*dirty, affected*/;
@@ -625,6 +681,7 @@ int godar_member_add(godar_ctx* c, const godar_params* p, int32_t n, const doubl
     c->members[m].n = n;
     c->members[m].tstep = tstep0;
     c->members[m].active = 1;
+    c->members[m].tstop = 0;
     c->N_used = std::max<int32_t>(c->N_used, (int32_t)(first + n));
     rc = upload_tables(c);
     if (rc != GODAR_OK) return rc;
@@ -750,10 +807,59 @@ int godar_reserve(godar_ctx* c, int32_t outer, int32_t inner, int32_t histc, int
     return GODAR_OK;
 }
 
+// stages whose per-floe kernels take a tunable block size
+static bool tunable_stage(int st) { return st == 0 || (st >= 2 && st <= 7) || st >= 9; }
+
 int godar_set_block(godar_ctx* c, int32_t threads) {
-    if (!c || threads < 64 || threads > 1024 || (threads & 63)) return GODAR_ERR_ARG;
+    if (!c || threads < 64 || threads > FLOE_BLOCK || (threads & 63)) return GODAR_ERR_ARG;
     c->block = threads;
+    for (int k = 0; k < GODAR_NSTAGES; ++k)
+        if (tunable_stage(k)) c->sblock[k] = threads;
     return GODAR_OK;
+}
+
+int godar_set_stage_block(godar_ctx* c, int32_t stage, int32_t threads) {
+    if (!c || stage < 0 || stage >= GODAR_NSTAGES || !tunable_stage(stage) || threads < 64 ||
+        threads > FLOE_BLOCK || (threads & 63))
+        return GODAR_ERR_ARG;
+    c->sblock[stage] = threads;
+    return GODAR_OK;
+}
+
+int godar_get_stage_blocks(const godar_ctx* c, int32_t* threads) {
+    if (!c || !threads) return GODAR_ERR_ARG;
+    for (int k = 0; k < GODAR_NSTAGES; ++k) threads[k] = c->sblock[k];
+    return GODAR_OK;
+}
+
+// Coordinate search over the block size of each tunable stage: every
+// candidate runs `trial` real steps of the current members (they advance;
+// results do not depend on block sizes) with stage timing, and the fastest
+// is kept.  Capacity growth is handled by godar_step.
+int godar_autotune(godar_ctx* c, int32_t trial) {
+    if (!c || trial < 1) return GODAR_ERR_ARG;
+    static const int cand[] = {64, 128, 256};
+    const bool prof0 = c->prof;
+    int rc = godar_step(c, 1);   // settle capacities first
+    if (rc != GODAR_OK) return rc;
+    for (int st = 0; st < GODAR_NSTAGES; ++st) {
+        if (!tunable_stage(st)) continue;
+        int best = c->sblock[st];
+        double best_ms = 1e300;
+        for (int b : cand) {
+            c->sblock[st] = b;
+            rc = godar_profile(c, 1);
+            if (rc == GODAR_OK) rc = godar_step(c, trial);
+            if (rc == GODAR_OK) rc = godar_sync(c);
+            if (rc != GODAR_OK) { c->prof = prof0; return rc; }
+            collect_events(c);
+            double ms = c->stage_n[st] > 0 ? c->stage_ms[st] / (double)c->stage_n[st] : 1e300;
+            if (ms < best_ms) { best_ms = ms; best = b; }
+        }
+        c->sblock[st] = best;
+    }
+    int r2 = godar_profile(c, prof0 ? 1 : 0);
+    return r2;
 }
 
 int godar_record_forces(godar_ctx* c, int32_t on) {
@@ -765,10 +871,8 @@ int godar_record_forces(godar_ctx* c, int32_t on) {
 int godar_step_async(godar_ctx* c, int32_t nsteps) {
     if (!c || nsteps < 0) return GODAR_ERR_ARG;
     if (!c->pending) {
-        int rc = download_members(c);
+        int rc = steps_done(c, &c->pend_done0);
         if (rc != GODAR_OK) return rc;
-        c->pend_t0.resize(c->Mcap);
-        for (int k = 0; k < c->Mcap; ++k) c->pend_t0[k] = c->members[k].tstep;
         c->pend_parity = c->parity;
         c->pending = true;
     }
@@ -804,10 +908,10 @@ int godar_sync(godar_ctx* c) {
         // fixes the pair-state parity.  Repair flags and the error word are
         // cleared so the caller can reserve more room and step again.
         if (was_pending) {
-            int done = 0;
-            for (int k = 0; k < c->Mcap; ++k)
-                if (c->members[k].active) { done = c->members[k].tstep - c->pend_t0[k]; break; }
-            c->parity = c->pend_parity ^ (done & 1);
+            int64_t now = 0;
+            rc = steps_done(c, &now);
+            if (rc != GODAR_OK) return rc;
+            c->parity = c->pend_parity ^ (int)((now - c->pend_done0) & 1);
             bind_parity(c);
         }
         HIPCHK(c, hipMemset(c->d.dirty, 0, c->S));
@@ -844,26 +948,44 @@ static bool grow_for(godar_ctx* c, uint32_t bits) {
 
 int godar_step(godar_ctx* c, int32_t nsteps) {
     if (!c || nsteps < 0) return GODAR_ERR_ARG;
-    int rc = download_members(c);
+    int64_t start = 0, now = 0;
+    int rc = steps_done(c, &start);
     if (rc != GODAR_OK) return rc;
-    int32_t ref = -1;
-    for (int k = 0; k < c->Mcap; ++k)
-        if (c->members[k].active) { ref = k; break; }
-    int32_t t_start = ref >= 0 ? c->members[ref].tstep : 0;
     int32_t left = nsteps;
     for (int attempt = 0; attempt < 64; ++attempt) {
         rc = godar_step_async(c, left);
         if (rc != GODAR_OK) return rc;
         rc = godar_sync(c);
-        if (rc != GODAR_ERR_CAPACITY || !c->auto_grow || ref < 0) return rc;
+        if (rc != GODAR_ERR_CAPACITY || !c->auto_grow) return rc;
         // some steps completed before the overflow: only the rest is retried
-        left = nsteps - (c->members[ref].tstep - t_start);
+        rc = steps_done(c, &now);
+        if (rc != GODAR_OK) return rc;
+        left = nsteps - (int32_t)(now - start);
         std::string why = c->err;
         if (!grow_for(c, c->last_err_bits))
             return fail(c, GODAR_ERR_CAPACITY, why + " (capacity limit reached)");
         if (left <= 0) return GODAR_OK;
     }
     return fail(c, GODAR_ERR_CAPACITY, "capacity growth did not converge");
+}
+
+int godar_member_run_until(godar_ctx* c, int32_t m, int32_t tlast) {
+    if (!valid_member(c, m) || tlast < 0) return fail(c, GODAR_ERR_ARG, "member_run_until: bad argument");
+    int rc = download_members(c);   // keep device-side step counters
+    if (rc != GODAR_OK) return rc;
+    c->members[m].tstop = tlast > 0 ? tlast + 1 : 0;
+    rc = upload_members_only(c);
+    if (rc != GODAR_OK) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return GODAR_OK;
+}
+
+int godar_member_tsteps(godar_ctx* c, int32_t* tsteps) {
+    if (!c || !tsteps) return GODAR_ERR_ARG;
+    int rc = download_members(c);
+    if (rc != GODAR_OK) return rc;
+    for (int k = 0; k < c->Mcap; ++k) tsteps[k] = c->members[k].active ? c->members[k].tstep : -1;
+    return GODAR_OK;
 }
 
 int godar_member_tstep(const godar_ctx* c, int32_t m, int32_t* t) {
@@ -1017,8 +1139,8 @@ int godar_get_totals(godar_ctx* c, int64_t* out) {
 }
 
 static const char* kStageNames[GODAR_NSTAGES] = {
-    "bin", "sort", "cells", "candidates", "shelter", "contacts", "bonds", "walls", "repair",
-    "fixup", "integrate"};
+    "bin", "sort", "cells", "candidates", "contacts", "bonds", "bond_gather", "walls", "repair",
+    "fixup", "stress", "integrate"};
 
 int godar_profile(godar_ctx* c, int32_t on) {
     if (!c) return GODAR_ERR_ARG;

@@ -8,11 +8,15 @@ namespace godar {
 
 // Accumulator rows of the per-floe gather (stepper.f90 accumulators).
 enum : int {
-    A_FCX = 0, A_FCY, A_MC, A_FBX, A_FBY, A_MB, A_SHA, A_SHW, A_TAC, A_PC, A_TAB, A_PB, NACC
+    A_FCX = 0, A_FCY, A_MC, A_FBX, A_FBY, A_MB, A_SHA, A_SHW, A_TAC, A_PC, A_TAB, A_PB,
+    A_SXX, A_SYY, A_SXY, A_SYX,   // stress diagnostics (stepper.f90:238-271)
+    NACC
 };
 
 constexpr int WALL_SLOT = 0xFFFF;
 // cls bits: which accumulator rows of a floe hold gathered values
+// (evf words keep EV_* flags in the low 16 bits and, for bonded events,
+// 1 + the owner's bond record slot in the high 16 bits)
 constexpr uint8_t CLS_CONTACT = 1, CLS_BOND = 2;
 constexpr int FLOE_BLOCK = 256;   // upper bound of the per-floe kernels' block size
 constexpr int XCD_COUNT = 8;      // accelerator complex dies of an MI355X
@@ -27,14 +31,14 @@ struct FloeRec {
 // neighbours are close in memory, and split by what the kernels read:
 //   GeoRec  - neighbour search and every pair test (32 B)
 //   ShelRec - sheltering: draft coefficients and the wind at the floe (32 B)
-//   KinRec  - contact, bond, wall and forcing terms (48 B)
+//   KinRec  - contact, bond, wall and forcing terms, integration (56 B)
 struct GeoRec {
     double x, y, r;
     int32_t id;      // floe id (global index of the persistent arrays)
     int32_t member;  // -1: free slot
 };
 struct ShelRec { double hfa, hfw, wx, wy; };
-struct KinRec { double u, v, omega, h, mass, inertia; };
+struct KinRec { double u, v, omega, h, mass, inertia, theta; };
 
 // Everything the kernels touch.  "id" arrays are indexed by floe id; "pos"
 // arrays by the floe's position in this step's cell order.  Per-floe lists
@@ -69,11 +73,15 @@ struct DS {
     int64_t* ik;
     uint32_t* evf;
     int32_t* ov;
+    double *ssc, *ssb;         // [CO][S] pos: the pair's |(fcn,fct)| and |(fbn,fbt)|
+                               // (written only when forces are recorded)
     // pair state, double-buffered (id rows; partner stored as id)
     HistRec *hist_old, *hist_new;
     int32_t *hcnt_old, *hcnt_new;
     BondRec *bond_old, *bond_new;
     int32_t *bcnt_old, *bcnt_new;
+    BondTerms* bterm;          // [BC][S] (id): this step's terms of owner record q
+    uint8_t* bgat;             // [S] pos: 1 if the floe has a bonded event this step
     // speculative gather (pos)
     double* acc;               // [NACC][S]
     WallOut* wout;             // [S], valid where wflag is set
@@ -95,6 +103,7 @@ struct DS {
     int64_t* stats;            // [members][4]
     double* fout;              // [33][S] (id)
     uint32_t* err;
+    int64_t* steps_done;       // steps completed without error (whole context)
 };
 
 }  // namespace godar

@@ -1,7 +1,7 @@
 // Kernels of the GODAR step pipeline (one launch sequence per step):
 //
-//   k_bin -> radix sort -> k_clear_cells -> k_ranges -> k_cand -> k_shelter
-//        -> k_contact -> k_bond -> k_walls -> k_repair -> k_fixup -> k_final
+//   k_bin -> radix sort -> k_clear_cells -> k_ranges -> k_cand -> k_contact
+//        -> k_bond -> k_bond_gather -> k_walls -> k_repair -> k_fixup -> k_final
 //
 // Sequential semantics of the reference (stepper.f90:71-280 visits floes i
 // in index order and, for each, its candidates j > i in ascending distance;
@@ -9,14 +9,16 @@
 // thickness) are reproduced as follows:
 //   * The event kernels evaluate every pair event with the start-of-step
 //     floe properties.  Contact, bond and sheltering terms feed disjoint
-//     accumulators, so each kind has a kernel of its own: k_shelter computes
-//     every floe's sheltering minima and queues the floes that have contact,
-//     bond or wall work for k_contact, k_bond and k_walls.  One thread per
+//     accumulators, so each kind has a kernel of its own: the candidate
+//     search k_cand also computes every floe's sheltering minima and queues
+//     the floes that have contact, bond or wall work for k_contact, k_bond
+//     and k_walls.  One thread per
 //     floe gathers its own contributions in the reference's per-floe order:
 //     first the pairs where it is the inner index (ascending outer index),
-//     then its own candidates in distance order.  Each pair is evaluated by
-//     both of its floes, which removes atomics and keeps FP64 sums identical
-//     to the reference.
+//     then its own candidates in distance order.  Contact and sheltering
+//     terms are evaluated by both floes of a pair; a bond is evaluated once,
+//     by its owner, which stores the terms for k_bond_gather.  No atomics,
+//     and FP64 sums stay identical to the reference.
 //   * An event's speculative result is exact unless one of its floes was
 //     reshaped earlier in the sequence.  Events that ridge are seeds; the
 //     per-member k_repair walks, in sequence order, the seeds and every later
@@ -94,13 +96,17 @@ __device__ __forceinline__ int cell_coord(double p, double cell, int g) {
 
 // Old pair state of (i, j), owner i (ids).
 __device__ __forceinline__ void load_pair_state(const DS& S, int i, int j, PairState& ps) {
-    ps.deltat = 0.0; ps.thetarelc = 0.0; ps.bond = 0; ps.has_bond_rec = false;
+    ps.deltat = 0.0; ps.thetarelc = 0.0; ps.fcn = 0.0; ps.fct = 0.0;
+    ps.bond = 0; ps.has_bond_rec = false;
     ps.damage = ps.deltanb = ps.deltatb = ps.trb_ji = ps.trb_ij = 0.0;
     ps.rb = ps.hb = ps.lb = ps.sb = ps.ib = 0.0;
     int hc = S.hcnt_old[i];
     for (int q = 0; q < hc; ++q) {
         const HistRec& hr = S.hist_old[(int64_t)q * S.S + i];
-        if (hr.j == j) { ps.deltat = hr.deltat; ps.thetarelc = hr.thetarelc; break; }
+        if (hr.j == j) {
+            ps.deltat = hr.deltat; ps.thetarelc = hr.thetarelc; ps.fcn = hr.fcn; ps.fct = hr.fct;
+            break;
+        }
     }
     int bc = S.bcnt_old[i];
     for (int q = 0; q < bc; ++q) {
@@ -142,14 +148,16 @@ __device__ __forceinline__ void apply_contact_i(double* a, const EventOut& o) {
         a[A_PC] = a[A_PC] - o.fcn_ac;
     }
 }
+template <typename E>   // EventOut or BondTerms
+__device__ __forceinline__ void add_bond_i(double* a, const E& o) {
+    a[A_FBX] = a[A_FBX] + o.fbn_c - o.fbt_s;
+    a[A_FBY] = a[A_FBY] + o.fbn_s + o.fbt_c;
+    a[A_MB] = a[A_MB] - o.lri_fbt - o.mbb_ji;
+    a[A_TAB] = a[A_TAB] + o.sb;
+    a[A_PB] = a[A_PB] - o.fbn_sb;
+}
 __device__ __forceinline__ void apply_bond_i(double* a, const EventOut& o) {
-    if (o.flags & EV_BONDED) {
-        a[A_FBX] = a[A_FBX] + o.fbn_c - o.fbt_s;
-        a[A_FBY] = a[A_FBY] + o.fbn_s + o.fbt_c;
-        a[A_MB] = a[A_MB] - o.lri_fbt - o.mbb_ji;
-        a[A_TAB] = a[A_TAB] + o.sb;
-        a[A_PB] = a[A_PB] - o.fbn_sb;
-    }
+    if (o.flags & EV_BONDED) add_bond_i(a, o);
 }
 __device__ __forceinline__ void apply_i(double* a, const EventOut& o) {
     apply_contact_i(a, o);
@@ -170,14 +178,16 @@ __device__ __forceinline__ void apply_contact_j(double* a, const EventOut& o) {
         a[A_PC] = a[A_PC] - o.fcn_ac;
     }
 }
+template <typename E>
+__device__ __forceinline__ void add_bond_j(double* a, const E& o) {
+    a[A_FBX] = a[A_FBX] - o.fbn_c + o.fbt_s;
+    a[A_FBY] = a[A_FBY] - o.fbn_s - o.fbt_c;
+    a[A_MB] = a[A_MB] - o.lrj_fbt - o.mbb_ij;
+    a[A_TAB] = a[A_TAB] + o.sb;
+    a[A_PB] = a[A_PB] - o.fbn_sb;
+}
 __device__ __forceinline__ void apply_bond_j(double* a, const EventOut& o) {
-    if (o.flags & EV_BONDED) {
-        a[A_FBX] = a[A_FBX] - o.fbn_c + o.fbt_s;
-        a[A_FBY] = a[A_FBY] - o.fbn_s - o.fbt_c;
-        a[A_MB] = a[A_MB] - o.lrj_fbt - o.mbb_ij;
-        a[A_TAB] = a[A_TAB] + o.sb;
-        a[A_PB] = a[A_PB] - o.fbn_sb;
-    }
+    if (o.flags & EV_BONDED) add_bond_j(a, o);
 }
 __device__ __forceinline__ void apply_j(double* a, const EventOut& o) {
     apply_contact_j(a, o);
@@ -203,7 +213,9 @@ __global__ void k_bin(DS S) {
     if (f >= S.N || *S.err) return;
     int m = S.member_of[f];
     S.cval[f] = (uint32_t)f;
-    if (m < 0) { S.ckey[f] = (uint32_t)S.ncells; return; }
+    // floes of free slots and of members past their stop step sort last and
+    // take no part in the step
+    if (m < 0 || !member_live(S.members[m])) { S.ckey[f] = (uint32_t)S.ncells; return; }
     const DevParams& P = S.params[m];
     double x = S.rec[f].x, y = S.rec[f].y;
     int cx = cell_coord(x, P.cell, P.gx), cy = cell_coord(y, P.cell, P.gy);
@@ -235,6 +247,7 @@ __global__ void k_ranges(DS S) {
     S.geo[k] = g;
     KinRec q;
     q.u = fr.u; q.v = fr.v; q.omega = fr.omega; q.h = fr.h; q.mass = fr.mass; q.inertia = fr.inertia;
+    q.theta = fr.theta;
     S.kin[k] = q;
     S.pos_of[f] = k;
     double ua = 0.0, va = 0.0;
@@ -250,78 +263,6 @@ __global__ void k_ranges(DS S) {
     S.shel[k] = h;
 }
 
-// ---------------------------------------------------------------------
-// 2. candidate lists (replaces coretran kNearest, stepper.f90:77-87)
-//    outer(f) = { j > f : dist <= r(f) + rtree } by (dist, j)
-//    inner(f) = { k < f : dist <= r(k) + rtree } by k
-// ---------------------------------------------------------------------
-__global__ __launch_bounds__(FLOE_BLOCK) void k_cand(DS S) {
-    int k = xcd_block() * blockDim.x + threadIdx.x;
-    if (k >= S.N || *S.err) return;
-    const GeoRec me = S.geo[k];
-    if (me.member < 0) { S.ocnt[k] = 0; S.icnt[k] = 0; return; }
-    const DevParams& P = S.params[me.member];
-    const double rad = me.r + P.rtree;
-    const int myid = me.id;
-    int cx = cell_coord(me.x, P.cell, P.gx), cy = cell_coord(me.y, P.cell, P.gy);
-    int no = 0, ni = 0;
-    uint32_t over = 0;
-    const int64_t SS = S.S;
-    int64_t* oj = S.oj;
-    int64_t* ik = S.ik;
-    for (int b = cy - 1; b <= cy + 1; ++b) {
-        if (b < 0 || b >= P.gy) continue;
-        for (int a = cx - 1; a <= cx + 1; ++a) {
-            if (a < 0 || a >= P.gx) continue;
-            const int2 rg = S.crange[P.cell_base + b * P.gx + a];
-            for (int q = rg.x; q < rg.y; ++q) {
-                if (q == k) continue;
-                const GeoRec* o = &S.geo[q];
-                double dx = me.x - o->x, dy = me.y - o->y;
-                double d = sqrt(dx * dx + dy * dy);
-                int j = o->id;
-                if (j > myid) {
-                    if (d <= rad) {
-                        if (no >= S.CO) { over |= ERR_OUTER_CAP; continue; }
-                        int p = no;
-                        while (p > 0) {
-                            double dp = S.od[(p - 1) * SS + k];
-                            int64_t ep = oj[(p - 1) * SS + k];
-                            if (dp < d || (dp == d && ent_id(ep) < j)) break;
-                            S.od[p * SS + k] = dp;
-                            oj[p * SS + k] = ep;
-                            --p;
-                        }
-                        S.od[p * SS + k] = d;
-                        oj[p * SS + k] = ent(j, q);
-                        ++no;
-                    }
-                } else {
-                    if (d <= o->r + P.rtree) {
-                        if (ni >= S.CI) { over |= ERR_INNER_CAP; continue; }
-                        int p = ni;
-                        int64_t e = ent(j, q);
-                        while (p > 0) {
-                            int64_t ep = ik[(p - 1) * SS + k];
-                            if (ep < e) break;
-                            ik[p * SS + k] = ep;
-                            --p;
-                        }
-                        ik[p * SS + k] = e;
-                        ++ni;
-                    }
-                }
-            }
-        }
-    }
-    if (over) set_err(S, over);
-    S.ocnt[k] = no;
-    S.icnt[k] = ni;
-}
-
-// ---------------------------------------------------------------------
-// 3. speculative events + per-floe gather + owner state writes
-// ---------------------------------------------------------------------
 // Appends position k to a work queue, one atomic per wavefront.  Every lane
 // of the wavefront must call it.
 __device__ __forceinline__ void enqueue(bool want, int32_t* q, int32_t* cnt, int k) {
@@ -338,88 +279,133 @@ __device__ __forceinline__ void enqueue(bool want, int32_t* q, int32_t* cnt, int
     }
 }
 
-// Sheltering minima of every floe (forcing.f90:162-186, start-of-step
-// properties) and the work classification: a floe joins the contact queue if
-// it overlaps a candidate partner or holds contact histories, the bond queue
-// if it or an inner partner holds a bond record for it (or a bond forms this
-// step), the wall queue if it crosses the box.
+// ---------------------------------------------------------------------
+// 2. candidate lists (replaces coretran kNearest, stepper.f90:77-87)
+//    outer(f) = { j > f : dist <= r(f) + rtree } by (dist, j)
+//    inner(f) = { k < f : dist <= r(k) + rtree } by k
+//    fused with each floe's sheltering minima (forcing.f90:162-186; a
+//    minimum does not depend on the order its terms come in, and every term
+//    is computed from exactly the operands the pair event would use) and the
+//    work classification: a floe joins the contact queue if it overlaps a
+//    candidate partner or holds contact histories, the bond queue if it owns
+//    bond records (or a bond of its own forms this step), the wall queue if
+//    it crosses the box.
+// ---------------------------------------------------------------------
 template <bool MIXED>
-__global__ __launch_bounds__(FLOE_BLOCK) void k_shelter(DS S) {
+__global__ __launch_bounds__(FLOE_BLOCK) void k_cand(DS S) {
     using T = typename Prec<MIXED>::T;
-    int k = xcd_block() * blockDim.x + threadIdx.x;
+    const int k = xcd_block() * blockDim.x + threadIdx.x;
     bool qc = false, qb = false, qw = false;
     bool live = k < S.N && !*S.err;
     GeoRec me;
     if (live) {
         me = S.geo[k];
-        live = me.member >= 0;
+        if (me.member < 0) {
+            S.ocnt[k] = 0;
+            S.icnt[k] = 0;
+            live = false;
+        }
     }
     if (live) {
         const int m = me.member;
         const DevParams& P = S.params[m];
+        const double rad = me.r + P.rtree;
         const int f = me.id;
-        const int64_t SS = S.S;
         const int first = S.members[m].first;
         const bool bond_step = S.members[m].tstep == 1 && P.restart != 1 && P.cohesion;
+        const int cx = cell_coord(me.x, P.cell, P.gx), cy = cell_coord(me.y, P.cell, P.gy);
+        const ShelRec msh = S.shel[k];
+        const T uwi = T(P.uw), vwi = T(P.vw);
+        T sha = T(1), shw = T(1);
         qc = S.hcnt_old[f] > 0;
         qb = S.bcnt_old[f] > 0;
         qw = me.y - me.r < 0 || me.y + me.r > P.ny || me.x - me.r < 0 || me.x + me.r > P.nx;
-        T sha = T(1), shw = T(1);
-        const T uwi = T(P.uw), vwi = T(P.vw);
-        const ShelRec msh = S.shel[k];
-        const int ni = S.icnt[k];
-        for (int q = 0; q < ni; ++q) {
-            const int64_t e = S.ik[q * SS + k];
-            const int i = ent_id(e), qi = ent_pos(e);
-            const GeoRec* o = &S.geo[qi];
-            double dx = o->x - me.x, dy = o->y - me.y;
-            double dist = sqrt(dx * dx + dy * dy);
-            double deltan = o->r + me.r - dist;
-            if (deltan > 0) qc = true;
-            if (!qb) {
-                if (bond_step && deltan >= -P.bond_lim &&
-                    bond_keep(i - first, f - first, P.bond_seed, P.bond_thr24)) {
-                    qb = true;
-                } else {
-                    const int bc = S.bcnt_old[i];
-                    for (int b = 0; b < bc; ++b) {
-                        const BondRec& br = S.bond_old[(int64_t)b * SS + i];
-                        if (br.j == f && br.bond == 1) { qb = true; break; }
+        int no = 0, ni = 0;
+        uint32_t over = 0;
+        const int64_t SS = S.S;
+        int64_t* oj = S.oj;
+        int64_t* ik = S.ik;
+        for (int b = cy - 1; b <= cy + 1; ++b) {
+            if (b < 0 || b >= P.gy) continue;
+            for (int a = cx - 1; a <= cx + 1; ++a) {
+                if (a < 0 || a >= P.gx) continue;
+                const int2 rg = S.crange[P.cell_base + b * P.gx + a];
+                for (int q = rg.x; q < rg.y; ++q) {
+                    if (q == k) continue;
+                    const GeoRec* o = &S.geo[q];
+                    double dx = me.x - o->x, dy = me.y - o->y;
+                    double d = sqrt(dx * dx + dy * dy);
+                    int j = o->id;
+                    if (j > f) {
+                        if (!(d <= rad)) continue;
+                        if (no >= S.CO) { over |= ERR_OUTER_CAP; continue; }
+                        int p = no;
+                        while (p > 0) {
+                            double dp = S.od[(p - 1) * SS + k];
+                            int64_t ep = oj[(p - 1) * SS + k];
+                            if (dp < d || (dp == d && ent_id(ep) < j)) break;
+                            S.od[p * SS + k] = dp;
+                            oj[p * SS + k] = ep;
+                            --p;
+                        }
+                        S.od[p * SS + k] = d;
+                        oj[p * SS + k] = ent(j, q);
+                        ++no;
+                        // pair (f, j): f is the outer floe i of the event
+                        const double deltan = me.r + o->r - d;
+                        if (deltan > 0) qc = true;
+                        if (!qb && bond_step && deltan >= -P.bond_lim &&
+                            bond_keep(f - first, j - first, P.bond_seed, P.bond_thr24))
+                            qb = true;
+                        if (P.shelter) {
+                            // floe f takes hsf*(j,f), evaluated with -cosa, -sina
+                            const double cosa = (o->x - me.x) / d, sina = (o->y - me.y) / d;
+                            const ShelRec oh = S.shel[q];
+                            const T dn = T(deltan), c = T(cosa), sn = T(sina);
+                            sha = dmin(sha, S_shelter(T(oh.hfa), T(msh.hfa), dn, -c, -sn, T(msh.wx), T(msh.wy)));
+                            shw = dmin(shw, S_shelter(T(oh.hfw), T(msh.hfw), dn, -c, -sn, uwi, vwi));
+                        }
+                    } else {
+                        if (!(d <= o->r + P.rtree)) continue;
+                        if (ni >= S.CI) { over |= ERR_INNER_CAP; continue; }
+                        int p = ni;
+                        const int64_t e = ent(j, q);
+                        while (p > 0) {
+                            int64_t ep = ik[(p - 1) * SS + k];
+                            if (ep < e) break;
+                            ik[p * SS + k] = ep;
+                            --p;
+                        }
+                        ik[p * SS + k] = e;
+                        ++ni;
+                        // pair (j, f): f is the inner floe j of the event
+                        const double deltan = o->r + me.r - d;
+                        if (deltan > 0) qc = true;
+                        if (P.shelter) {
+                            // cosa/sina point from j to f; floe f takes hsf*(j,f) with j's wind
+                            const double cosa = (me.x - o->x) / d, sina = (me.y - o->y) / d;
+                            const ShelRec oh = S.shel[q];
+                            const T dn = T(deltan), c = T(cosa), sn = T(sina);
+                            sha = dmin(sha, S_shelter(T(oh.hfa), T(msh.hfa), dn, c, sn, T(oh.wx), T(oh.wy)));
+                            shw = dmin(shw, S_shelter(T(oh.hfw), T(msh.hfw), dn, c, sn, uwi, vwi));
+                        }
                     }
                 }
             }
-            if (P.shelter) {
-                // pair (i, f): cosa/sina point from i to f; floe f takes hsf*(i,f)
-                double cosa = (me.x - o->x) / dist, sina = (me.y - o->y) / dist;
-                const ShelRec oh = S.shel[qi];
-                T dn = T(deltan), c = T(cosa), sn = T(sina);
-                sha = dmin(sha, S_shelter(T(oh.hfa), T(msh.hfa), dn, c, sn, T(oh.wx), T(oh.wy)));
-                shw = dmin(shw, S_shelter(T(oh.hfw), T(msh.hfw), dn, c, sn, uwi, vwi));
-            }
         }
-        const int no = S.ocnt[k];
+        if (over) set_err(S, over);
+        S.ocnt[k] = no;
+        S.icnt[k] = ni;
         const uint32_t fl = P.shelter ? EV_SHELTER : 0u;
-        for (int s = 0; s < no; ++s) {
-            const int64_t e = S.oj[s * SS + k];
-            const int j = ent_id(e), qj = ent_pos(e);
-            const GeoRec* o = &S.geo[qj];
-            double dx = me.x - o->x, dy = me.y - o->y;
-            double dist = sqrt(dx * dx + dy * dy);
-            double deltan = me.r + o->r - dist;
-            if (deltan > 0) qc = true;
-            if (!qb && bond_step && deltan >= -P.bond_lim &&
-                bond_keep(f - first, j - first, P.bond_seed, P.bond_thr24))
-                qb = true;
-            if (P.shelter) {
-                // pair (f, j): floe f takes hsf*(j,f), evaluated with -cosa, -sina
-                double cosa = (o->x - me.x) / dist, sina = (o->y - me.y) / dist;
-                const ShelRec oh = S.shel[qj];
-                T dn = T(deltan), c = T(cosa), sn = T(sina);
-                sha = dmin(sha, S_shelter(T(oh.hfa), T(msh.hfa), dn, -c, -sn, T(msh.wx), T(msh.wy)));
-                shw = dmin(shw, S_shelter(T(oh.hfw), T(msh.hfw), dn, -c, -sn, uwi, vwi));
+        for (int s2 = 0; s2 < no; ++s2) {
+            S.evf[s2 * SS + k] = fl;
+            S.ov[s2 * SS + k] = -1;
+        }
+        if (S.record_forces) {
+            for (int s2 = 0; s2 < no; ++s2) {
+                S.ssc[s2 * SS + k] = 0.0;
+                S.ssb[s2 * SS + k] = 0.0;
             }
-            S.evf[s * SS + k] = fl;
-            S.ov[s * SS + k] = -1;
         }
         S.acc[A_SHA * SS + k] = double(sha);
         S.acc[A_SHW * SS + k] = double(shw);
@@ -427,22 +413,31 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_shelter(DS S) {
         if (!qb) S.bcnt_new[f] = 0;
         S.ovw[k] = -1;
         S.wflag[k] = 0;
-        S.cls[k] = (qc ? CLS_CONTACT : 0) | (qb ? CLS_BOND : 0);
+        S.bgat[k] = 0;
+        S.cls[k] = qc ? CLS_CONTACT : 0;   // k_bond_gather adds CLS_BOND
     }
     enqueue(qc, S.qlist, &S.qn[0], k);
     enqueue(qb, S.qlist + S.S, &S.qn[1], k);
     enqueue(qw, S.qlist + 2 * S.S, &S.qn[2], k);
 }
 
-// Contact state of pair (i, j), owner i: history and whether it is bonded.
+// ---------------------------------------------------------------------
+// 3. speculative events + per-floe gather + owner state writes
+// ---------------------------------------------------------------------
+// Contact state of pair (i, j), owner i: history, last contact forces and
+// whether it is bonded.
 __device__ __forceinline__ void load_contact_state(const DS& S, int i, int j, PairState& ps) {
-    ps.deltat = 0.0; ps.thetarelc = 0.0; ps.bond = 0; ps.has_bond_rec = false;
+    ps.deltat = 0.0; ps.thetarelc = 0.0; ps.fcn = 0.0; ps.fct = 0.0;
+    ps.bond = 0; ps.has_bond_rec = false;
     ps.damage = ps.deltanb = ps.deltatb = ps.trb_ji = ps.trb_ij = 0.0;
     ps.rb = ps.hb = ps.lb = ps.sb = ps.ib = 0.0;
     const int hc = S.hcnt_old[i];
     for (int q = 0; q < hc; ++q) {
         const HistRec& hr = S.hist_old[(int64_t)q * S.S + i];
-        if (hr.j == j) { ps.deltat = hr.deltat; ps.thetarelc = hr.thetarelc; break; }
+        if (hr.j == j) {
+            ps.deltat = hr.deltat; ps.thetarelc = hr.thetarelc; ps.fcn = hr.fcn; ps.fct = hr.fct;
+            break;
+        }
     }
     const int bc = S.bcnt_old[i];
     for (int q = 0; q < bc; ++q) {
@@ -495,6 +490,7 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_contact(DS S) {
     // pairs (f, j), f outer (owner), distance order
     const int no = S.ocnt[k];
     int nh = 0;
+    const int hco = S.hcnt_old[f];
     for (int s = 0; s < no; ++s) {
         const int64_t e = oj[s * SS + k];
         const int j = ent_id(e), qj = ent_pos(e);
@@ -502,8 +498,31 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_contact(DS S) {
         Floe Fj = floe_at(S, qj);
         Kin kn;
         kinematics(Fi, Fj, kn);
-        // no overlap: reset_contact (reset.f90:1-15) drops the history
-        if (!(kn.deltan > 0)) continue;
+        if (!(kn.deltan > 0)) {
+            // reset_contact (reset.f90:1-15): the history goes, the last
+            // contact forces stay
+            for (int q = 0; q < hco; ++q) {
+                const HistRec& h0 = S.hist_old[(int64_t)q * SS + f];
+                if (h0.j != j) continue;
+                if (h0.fcn != 0.0 || h0.fct != 0.0) {
+                    if (nh < S.HC) {
+                        HistRec hr; hr.j = j; hr.pad_ = 0; hr.deltat = 0.0; hr.thetarelc = 0.0;
+                        hr.fcn = h0.fcn; hr.fct = h0.fct;
+                        S.hist_new[nh * SS + f] = hr;
+                        ++nh;
+                    } else {
+                        over |= ERR_HIST_CAP;
+                    }
+                    if (S.record_forces) {
+                        PairState pz;
+                        pz.fcn = h0.fcn; pz.fct = h0.fct;
+                        S.ssc[s * SS + k] = stale_contact_magnitude<MIXED>(pz);
+                    }
+                }
+                break;
+            }
+            continue;
+        }
         PairState ps;
         load_contact_state(S, f, j, ps);
         bond_create(P, tstep, f - first, j - first, Fi, Fj, kn, ps);
@@ -513,9 +532,11 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_contact(DS S) {
         contact_part<MIXED>(P, Fi, Fj, kn, ps, o);
         apply_contact_i(acc, o);
         S.evf[s * SS + k] |= o.flags;
-        if (ps.deltat != 0.0 || ps.thetarelc != 0.0) {
+        if (S.record_forces) S.ssc[s * SS + k] = o.sc;
+        if (ps.deltat != 0.0 || ps.thetarelc != 0.0 || ps.fcn != 0.0 || ps.fct != 0.0) {
             if (nh < S.HC) {
                 HistRec hr; hr.j = j; hr.pad_ = 0; hr.deltat = ps.deltat; hr.thetarelc = ps.thetarelc;
+                hr.fcn = ps.fcn; hr.fct = ps.fct;
                 S.hist_new[nh * SS + f] = hr;
                 ++nh;
             } else {
@@ -532,7 +553,6 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_contact(DS S) {
     }
     // contact histories of pairs that are not candidates this step persist
     // (the reference only resets a candidate pair, reset_contact)
-    const int hco = S.hcnt_old[f];
     for (int q = 0; q < hco; ++q) {
         HistRec hr = S.hist_old[q * SS + f];
         if (hr.j < 0) continue;
@@ -553,21 +573,10 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_contact(DS S) {
     if (over) set_err(S, over);
 }
 
-// Bond state of pair (i, j), owner i, from record q of i's old bond list
-// (q < 0: no record).
-__device__ __forceinline__ void load_bond_state(const DS& S, int i, int q, PairState& ps) {
-    ps.deltat = 0.0; ps.thetarelc = 0.0; ps.bond = 0; ps.has_bond_rec = false;
-    ps.damage = ps.deltanb = ps.deltatb = ps.trb_ji = ps.trb_ij = 0.0;
-    ps.rb = ps.hb = ps.lb = ps.sb = ps.ib = 0.0;
-    if (q < 0) return;
-    const BondRec& b = S.bond_old[(int64_t)q * S.S + i];
-    ps.has_bond_rec = true; ps.bond = b.bond; ps.damage = b.damage;
-    ps.deltanb = b.deltanb; ps.deltatb = b.deltatb; ps.trb_ji = b.trb_ji; ps.trb_ij = b.trb_ij;
-    ps.rb = b.rb; ps.hb = b.hb; ps.lb = b.lb; ps.sb = b.sb; ps.ib = b.ib;
-}
-
-// Bond terms of the queued floes (bond_forces_euler, bond_breaking;
-// stepper.f90:148-198) + the owner's new bond records.
+// Bond events of the queued owners (bond_forces_euler, bond_breaking;
+// stepper.f90:148-198): each bond is evaluated once, by its owner (the lower
+// index), which writes the new record and the event's terms for both floes;
+// k_bond_gather then sums every floe's terms in the reference's order.
 template <bool MIXED>
 __global__ __launch_bounds__(FLOE_BLOCK) void k_bond(DS S) {
     int t = xcd_block() * blockDim.x + threadIdx.x;
@@ -582,85 +591,105 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_bond(DS S) {
     const int f = me.id;
     const int64_t SS = S.S;
     const int64_t* oj = S.oj;
-    const int64_t* ik = S.ik;
     const Floe F0 = floe_at(S, k);
-    double acc[NACC];
-    acc_init(acc);
     uint32_t over = 0;
 
-    // pairs (i, f), f inner: bonds that i holds for f, or that form now
-    const int ni = S.icnt[k];
-    for (int q = 0; q < ni; ++q) {
-        const int64_t e = ik[q * SS + k];
-        const int i = ent_id(e), qi = ent_pos(e);
-        int rq = -1;
-        const int bc = S.bcnt_old[i];
-        for (int b = 0; b < bc; ++b)
-            if (S.bond_old[(int64_t)b * SS + i].j == f) { rq = b; break; }
-        if (rq < 0 && !bond_step) continue;
-        Floe Fi = floe_at(S, qi);
-        Floe Ff = F0;
-        Kin kn;
-        kinematics(Fi, Ff, kn);
-        PairState ps;
-        load_bond_state(S, i, rq, ps);
-        bond_create(P, tstep, i - first, f - first, Fi, Ff, kn, ps);
-        if (ps.bond != 1) continue;
-        EventOut o;
-        o.flags = 0;
-        o.ridged = -1;
-        bond_part<MIXED>(P, Fi, Ff, kn, ps, o);
-        apply_bond_j(acc, o);
-    }
-
-    // pairs (f, j), f outer (owner), distance order
+    // the owner's records carry over (cleared entries dropped)
     int nb = 0;
     {
         const int bco = S.bcnt_old[f];
         for (int q = 0; q < bco; ++q) {
             const BondRec* b = &S.bond_old[q * SS + f];
-            if (b->j >= 0) copy_bond(&S.bond_new[(nb++) * SS + f], b);   // drops cleared entries
+            if (b->j >= 0) copy_bond(&S.bond_new[(nb++) * SS + f], b);
         }
     }
-    const int bco = S.bcnt_old[f];
+    bool any = false;
     const int no = S.ocnt[k];
     for (int s = 0; s < no; ++s) {
         const int64_t e = oj[s * SS + k];
         const int j = ent_id(e), qj = ent_pos(e);
         int rq = -1;
-        for (int b = 0; b < bco; ++b)
-            if (S.bond_old[(int64_t)b * SS + f].j == j) { rq = b; break; }
+        for (int q = 0; q < nb; ++q)
+            if (S.bond_new[(int64_t)q * SS + f].j == j) { rq = q; break; }
         if (rq < 0 && !bond_step) continue;
         Floe Fi = F0;
         Floe Fj = floe_at(S, qj);
         Kin kn;
         kinematics(Fi, Fj, kn);
         PairState ps;
-        load_bond_state(S, f, rq, ps);
+        ps.deltat = 0.0; ps.thetarelc = 0.0; ps.fcn = 0.0; ps.fct = 0.0;
+        ps.bond = 0; ps.has_bond_rec = false;
+        ps.damage = ps.deltanb = ps.deltatb = ps.trb_ji = ps.trb_ij = 0.0;
+        ps.rb = ps.hb = ps.lb = ps.sb = ps.ib = 0.0;
+        if (rq >= 0) {
+            const BondRec& b = S.bond_new[(int64_t)rq * SS + f];
+            ps.has_bond_rec = true; ps.bond = b.bond; ps.damage = b.damage;
+            ps.deltanb = b.deltanb; ps.deltatb = b.deltatb; ps.trb_ji = b.trb_ji; ps.trb_ij = b.trb_ij;
+            ps.rb = b.rb; ps.hb = b.hb; ps.lb = b.lb; ps.sb = b.sb; ps.ib = b.ib;
+        }
         bond_create(P, tstep, f - first, j - first, Fi, Fj, kn, ps);
         if (!ps.has_bond_rec) continue;
         EventOut o;
         o.flags = 0;
         o.ridged = -1;
+        o.sbd = 0.0;
         bond_part<MIXED>(P, Fi, Fj, kn, ps, o);
-        apply_bond_i(acc, o);
-        if (o.flags & EV_BONDED) S.evf[s * SS + k] |= EV_BONDED;
-        int qb = -1;
-        for (int q = 0; q < nb; ++q)
-            if (S.bond_new[q * SS + f].j == j) { qb = q; break; }
-        if (qb < 0) {
-            if (nb < S.BC) qb = nb++;
-            else over |= ERR_BOND_CAP;
+        if (S.record_forces) S.ssb[s * SS + k] = o.sbd;
+        if (rq < 0) {
+            if (nb < S.BC) rq = nb++;
+            else { over |= ERR_BOND_CAP; continue; }
         }
-        if (qb >= 0) S.bond_new[qb * SS + f] = bond_from(ps, j);
+        S.bond_new[(int64_t)rq * SS + f] = bond_from(ps, j);
+        if (o.flags & EV_BONDED) {
+            BondTerms bt;
+            bt.fbn_c = o.fbn_c; bt.fbt_s = o.fbt_s; bt.fbn_s = o.fbn_s; bt.fbt_c = o.fbt_c;
+            bt.lri_fbt = o.lri_fbt; bt.lrj_fbt = o.lrj_fbt; bt.mbb_ji = o.mbb_ji; bt.mbb_ij = o.mbb_ij;
+            bt.sb = o.sb; bt.fbn_sb = o.fbn_sb;
+            S.bterm[(int64_t)rq * SS + f] = bt;
+            S.evf[s * SS + k] |= EV_BONDED | ((uint32_t)(rq + 1) << 16);
+            S.bgat[qj] = 1;
+            any = true;
+        }
     }
+    if (any) S.bgat[k] = 1;
     S.bcnt_new[f] = nb;
-    S.acc[A_FBX * SS + k] = acc[A_FBX];
-    S.acc[A_FBY * SS + k] = acc[A_FBY];
-    S.acc[A_MB * SS + k] = acc[A_MB];
-    S.acc[A_TAB * SS + k] = acc[A_TAB];
-    S.acc[A_PB * SS + k] = acc[A_PB];
     if (over) set_err(S, over);
+}
+
+// Bond rows of every floe with a bonded event: the terms its owners wrote,
+// first the pairs where it is the inner index (ascending outer index), then
+// its own, in distance order (stepper.f90:165-198).
+__global__ __launch_bounds__(FLOE_BLOCK) void k_bond_gather(DS S) {
+    int k = xcd_block() * blockDim.x + threadIdx.x;
+    if (k >= S.N || *S.err || !S.bgat[k]) return;
+    const int f = S.geo[k].id;
+    const int64_t SS = S.S;
+    const int64_t* oj = S.oj;
+    double a[NACC];
+    acc_init(a);
+    const int ni = S.icnt[k];
+    for (int q = 0; q < ni; ++q) {
+        const int64_t e = S.ik[q * SS + k];
+        const int i = ent_id(e), qi = ent_pos(e);
+        const int noi = S.ocnt[qi];
+        for (int s = 0; s < noi; ++s) {
+            if (ent_id(oj[s * SS + qi]) != f) continue;
+            const uint32_t fl = S.evf[s * SS + qi];
+            if (fl & EV_BONDED) add_bond_j(a, S.bterm[(int64_t)((fl >> 16) - 1) * SS + i]);
+            break;
+        }
+    }
+    const int no = S.ocnt[k];
+    for (int s = 0; s < no; ++s) {
+        const uint32_t fl = S.evf[s * SS + k];
+        if (fl & EV_BONDED) add_bond_i(a, S.bterm[(int64_t)((fl >> 16) - 1) * SS + f]);
+    }
+    S.acc[A_FBX * SS + k] = a[A_FBX];
+    S.acc[A_FBY * SS + k] = a[A_FBY];
+    S.acc[A_MB * SS + k] = a[A_MB];
+    S.acc[A_TAB * SS + k] = a[A_TAB];
+    S.acc[A_PB * SS + k] = a[A_PB];
+    S.cls[k] |= CLS_BOND;
 }
 
 // Walls of the queued floes (verify_bc, boundaries.f90:1-97).
@@ -740,8 +769,10 @@ __device__ __forceinline__ int slot_of(const DS& S, int kp, int g) {
     return -1;
 }
 
-// write (or clear) owner i's new contact history entry for partner j
-__device__ __forceinline__ bool put_hist(const DS& S, int i, int j, bool want, double dt_, double th_) {
+// write (or clear) owner i's new contact history entry for partner j; the
+// entry lives while any of its four values is non-zero
+__device__ __forceinline__ bool put_hist(const DS& S, int i, int j, const PairState& ps) {
+    const bool want = ps.deltat != 0.0 || ps.thetarelc != 0.0 || ps.fcn != 0.0 || ps.fct != 0.0;
     int n = S.hcnt_new[i];
     int q = -1;
     for (int p = 0; p < n; ++p)
@@ -752,7 +783,8 @@ __device__ __forceinline__ bool put_hist(const DS& S, int i, int j, bool want, d
             q = n;
             S.hcnt_new[i] = n + 1;
         }
-        HistRec hr; hr.j = j; hr.pad_ = 0; hr.deltat = dt_; hr.thetarelc = th_;
+        HistRec hr; hr.j = j; hr.pad_ = 0; hr.deltat = ps.deltat; hr.thetarelc = ps.thetarelc;
+        hr.fcn = ps.fcn; hr.fct = ps.fct;
         S.hist_new[(int64_t)q * S.S + i] = hr;
     } else if (q >= 0) {
         S.hist_new[(int64_t)q * S.S + i].j = -1;
@@ -782,7 +814,7 @@ template <bool MIXED>
 __global__ void k_repair(DS S, int nmembers) {
     int m = blockIdx.x;
     if (m >= nmembers || threadIdx.x != 0 || *S.err) return;
-    if (!S.members[m].active) return;
+    if (!member_live(S.members[m])) return;
     int ns = S.nseeds[m];
     if (ns == 0) return;
     const DevParams& P = S.params[m];
@@ -854,7 +886,7 @@ __global__ void k_repair(DS S, int nmembers) {
             const double2 wf = wind_at(S, kf);
             pair_event<MIXED>(P, tstep, b, j - first, Fi, Fj, wf.x, wf.y, P.uw, P.vw, ps, o, 0);
         }
-        bool okh = put_hist(S, f, j, ps.deltat != 0.0 || ps.thetarelc != 0.0, ps.deltat, ps.thetarelc);
+        bool okh = put_hist(S, f, j, ps);
         bool okb = put_bond(S, f, j, ps.has_bond_rec, bond_from(ps, j));
         if (!okh) over |= ERR_HIST_CAP;
         if (!okb) over |= ERR_BOND_CAP;
@@ -963,6 +995,79 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_fixup(DS S) {
 }
 
 // ---------------------------------------------------------------------
+// 5b. stress diagnostics (stepper.f90:238-271), only when forces are
+//     recorded: for every candidate pair (j,i), both floes take
+//     sig -= S * (direction terms) * r(i) with S = |(fcn,fct)| + |(fbn,fbt)|,
+//     the pair's current or last contact forces and current bond forces, in
+//     the reference's per-floe order.
+// ---------------------------------------------------------------------
+__device__ __forceinline__ void stress_terms(double Sv, double c, double sn, double ri, double* a) {
+    a[A_SXX] = a[A_SXX] - Sv * c * ri * c;
+    a[A_SYY] = a[A_SYY] - Sv * sn * ri * sn;
+    a[A_SXY] = a[A_SXY] - Sv * sn * ri * c;
+    a[A_SYX] = a[A_SYX] - Sv * c * ri * sn;
+}
+
+__global__ __launch_bounds__(FLOE_BLOCK) void k_stress(DS S) {
+    const int k = xcd_block() * blockDim.x + threadIdx.x;
+    if (k >= S.N || *S.err) return;
+    const GeoRec me = S.geo[k];
+    const int m = me.member;
+    if (m < 0) return;
+    const int f = me.id;
+    const int64_t SS = S.S;
+    const EventOut* etab = S.etab + (int64_t)S.TABK * S.members[m].first;
+    double a[NACC];
+    a[A_SXX] = a[A_SYY] = a[A_SXY] = a[A_SYX] = 0.0;
+    // pairs (i, f), f inner, ascending i
+    const int ni = S.icnt[k];
+    for (int q = 0; q < ni; ++q) {
+        const int64_t e = S.ik[q * SS + k];
+        const int qi = ent_pos(e);
+        const int s = slot_of(S, qi, f);
+        if (s < 0) continue;
+        const int idx = S.ov[(int64_t)s * SS + qi];
+        const GeoRec gi = S.geo[qi];
+        double sc, sb, ri;
+        if (idx >= 0) {
+            const EventOut& o = etab[idx];
+            if (o.flags & EV_REMOVED) continue;
+            sc = o.sc; sb = o.sbd; ri = o.sri;
+        } else {
+            sc = S.ssc[(int64_t)s * SS + qi]; sb = S.ssb[(int64_t)s * SS + qi]; ri = gi.r;
+        }
+        const double dx = gi.x - me.x, dy = gi.y - me.y;
+        const double dist = sqrt(dx * dx + dy * dy);
+        const double c = (me.x - gi.x) / dist, sn = (me.y - gi.y) / dist;
+        stress_terms(sc + sb, c, sn, ri, a);
+    }
+    // pairs (f, j), f outer, distance order
+    const int no = S.ocnt[k];
+    for (int s = 0; s < no; ++s) {
+        const int64_t e = S.oj[s * SS + k];
+        const int qj = ent_pos(e);
+        const int idx = S.ov[s * SS + k];
+        double sc, sb, ri;
+        if (idx >= 0) {
+            const EventOut& o = etab[idx];
+            if (o.flags & EV_REMOVED) continue;
+            sc = o.sc; sb = o.sbd; ri = o.sri;
+        } else {
+            sc = S.ssc[s * SS + k]; sb = S.ssb[s * SS + k]; ri = me.r;
+        }
+        const GeoRec gj = S.geo[qj];
+        const double dx = me.x - gj.x, dy = me.y - gj.y;
+        const double dist = sqrt(dx * dx + dy * dy);
+        const double c = (gj.x - me.x) / dist, sn = (gj.y - me.y) / dist;
+        stress_terms(sc + sb, c, sn, ri, a);
+    }
+    S.acc[A_SXX * SS + k] = a[A_SXX];
+    S.acc[A_SYY * SS + k] = a[A_SYY];
+    S.acc[A_SXY * SS + k] = a[A_SXY];
+    S.acc[A_SYX * SS + k] = a[A_SYX];
+}
+
+// ---------------------------------------------------------------------
 // 6. final properties, forcing + coriolis, totals, integration
 //    (stepper.f90:295-343, integration.f90:1-38)
 // ---------------------------------------------------------------------
@@ -970,6 +1075,8 @@ template <bool MIXED>
 __global__ __launch_bounds__(FLOE_BLOCK) void k_final(DS S) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= S.N || *S.err) return;
+    // every earlier kernel of the step has finished: the step is complete
+    if (k == 0) *S.steps_done += 1;
     const GeoRec me = S.geo[k];
     const int m = me.member;
     if (m < 0) return;
@@ -977,7 +1084,8 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_final(DS S) {
     const DevParams& P = S.params[m];
     const int64_t SS = S.S;
     Floe F = floe_at(S, k);
-    if (S.dirty[f]) {
+    const bool dirty = S.dirty[f] != 0;
+    if (dirty) {
         F.r = S.cur[0 * SS + f]; F.h = S.cur[1 * SS + f]; F.mass = S.cur[2 * SS + f];
         F.inertia = S.cur[3 * SS + f]; F.hfa = S.cur[4 * SS + f]; F.hfw = S.cur[5 * SS + f];
     }
@@ -991,6 +1099,7 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_final(DS S) {
         // no wall touched: reset_boundary(i, 0) cleared every wall history
         W.fx_bc = W.fy_bc = W.m_bc = W.ta_bc = W.p_bc = 0.0;
         W.deltat_bc1 = W.deltat_bc2 = W.theta_bc1 = W.theta_bc2 = 0.0;
+        W.sxx_bc = W.syy_bc = W.sxy_bc = W.syx_bc = 0.0;
     }
     S.wbc[0 * SS + f] = W.deltat_bc1; S.wbc[1 * SS + f] = W.deltat_bc2;
     S.wbc[2 * SS + f] = W.theta_bc1; S.wbc[3 * SS + f] = W.theta_bc2;
@@ -1000,7 +1109,9 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_final(DS S) {
     for (int c = 0; c < NACC; ++c) {
         const bool bond_row = c == A_FBX || c == A_FBY || c == A_MB || c == A_TAB || c == A_PB;
         const bool shel_row = c == A_SHA || c == A_SHW;
-        const bool valid = shel_row || (bond_row ? (cl & CLS_BOND) : (cl & CLS_CONTACT));
+        const bool stress_row = c >= A_SXX;
+        const bool valid = stress_row ? S.record_forces != 0
+                         : shel_row || (bond_row ? (cl & CLS_BOND) : (cl & CLS_CONTACT));
         a[c] = valid ? S.acc[c * SS + k] : 0.0;
     }
     // shelter minima (reset_shelter + thread reduction); with shelter off
@@ -1021,20 +1132,27 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_final(DS S) {
     out.y = F.y + v * P.dt;
     out.u = u; out.v = v; out.omega = om;
     out.r = F.r; out.h = F.h; out.mass = F.mass; out.inertia = F.inertia; out.hfa = F.hfa; out.hfw = F.hfw;
-    out.theta = S.rec[f].theta + om * P.dt;
+    out.theta = S.kin[k].theta + om * P.dt;
     S.rec[f] = out;
     if (S.record_forces) {
         double tp = (a[A_TAC] != 0.0 ? a[A_PC] / a[A_TAC] : 0.0)
                   + (a[A_TAB] != 0.0 ? a[A_PB] / a[A_TAB] : 0.0)
                   + (W.ta_bc != 0.0 ? W.p_bc / W.ta_bc : 0.0);
+        // total stress: pairs + last wall + drag (forcing.f90 sig*_aw)
+        const double sxx_aw = (Fo.fax + Fo.fwx) * F.r, syy_aw = (Fo.fay + Fo.fwy) * F.r;
+        const double tsxx = a[A_SXX] + W.sxx_bc + sxx_aw;
+        const double tsyy = a[A_SYY] + W.syy_bc + syy_aw;
+        const double tsxy = a[A_SXY] + W.sxy_bc + 0.0;
+        const double tsyx = a[A_SYX] + W.syx_bc + 0.0;
         double cols[33] = {tfx, tfy, mt, a[A_FCX], a[A_FCY], a[A_MC], a[A_FBX], a[A_FBY], a[A_MB],
                            Fo.fax, Fo.fay, Fo.ma, Fo.fwx, Fo.fwy, Fo.mw, Fo.fcorx, Fo.fcory,
-                           W.fx_bc, W.fy_bc, W.m_bc, sha, shw, tp, 0.0, 0.0, 0.0, 0.0,
+                           W.fx_bc, W.fy_bc, W.m_bc, sha, shw, tp, tsxx, tsyy, tsxy, tsyx,
                            a[A_TAC], a[A_PC], a[A_TAB], a[A_PB], W.ta_bc, W.p_bc};
         for (int c = 0; c < 33; ++c) S.fout[c * SS + f] = cols[c];
     }
-    S.dirty[f] = 0;
-    S.affected[f] = 0;
+    // flags set by the repair are cleared (most floes have none: no write)
+    if (dirty) S.dirty[f] = 0;
+    if (S.affected[f]) S.affected[f] = 0;
     if (f == S.members[m].first) S.members[m].tstep += 1;
 }
 
@@ -1072,8 +1190,8 @@ __global__ void k_totals(DS S, int64_t* out) {
 }
 
 // explicit instantiations used by the host launcher
-template __global__ void k_shelter<false>(DS);
-template __global__ void k_shelter<true>(DS);
+template __global__ void k_cand<false>(DS);
+template __global__ void k_cand<true>(DS);
 template __global__ void k_contact<false>(DS);
 template __global__ void k_contact<true>(DS);
 template __global__ void k_bond<false>(DS);

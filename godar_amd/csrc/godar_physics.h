@@ -85,6 +85,7 @@ __device__ __forceinline__ T S_shelter(T hfi, T hfj, T deltan, T cosa, T sina, T
 // Old pair state of one event (contact history + bond record).
 struct PairState {
     double deltat, thetarelc;
+    double fcn, fct;        // last contact forces (kept after separation)
     int bond;               // 0 none/broken, 1 bonded
     bool has_bond_rec;
     double damage, deltanb, deltatb, trb_ji, trb_ij, rb, hb, lb, sb, ib;
@@ -224,6 +225,9 @@ __device__ __forceinline__ void contact_part(const DevParams& P, Floe& fi, Floe&
         if (gabs(T(ps.thetarelc)) > T(2) * gabs(fcn) / knc / dr) mrolling = T(0);
         mcc = mrolling;
     }
+    ps.fcn = double(fcn);
+    ps.fct = double(fct);
+    o.sc = double(gsqrt(fcn * fcn + fct * fct));
     // stepper.f90:113-146 terms
     T c = T(k.cosa), s = T(k.sina);
     o.fcn_c = double(fcn * c);
@@ -294,6 +298,7 @@ __device__ __forceinline__ void bond_part(const DevParams& P, const Floe& fi, co
             fbn = T(0); fbt = T(0); mbb_ji = T(0);
             ps.damage = 1.0;
         }
+        o.sbd = double(gsqrt(fbn * fbn + fbt * fbt));
         if (ps.bond == 1) {
             o.flags |= EV_BONDED;
             T c = T(k.cosa), s = T(k.sina);
@@ -333,6 +338,14 @@ __device__ __forceinline__ void shelter_part(const DevParams& P, const Floe& fi,
     }
 }
 
+// |(fcn, fct)| of a pair not in contact: its last contact forces.
+template <bool MIXED>
+__device__ __forceinline__ double stale_contact_magnitude(const PairState& ps) {
+    using T = typename Prec<MIXED>::T;
+    const T a = T(ps.fcn), b = T(ps.fct);
+    return double(gsqrt(a * a + b * b));
+}
+
 // One pair event (i < j), the body of stepper.f90:89-271 for one (j,i).
 // fi/fj are updated in place if ridging reshapes one of them.  ps is updated
 // to the pair's new state.  Returns the gather terms in `o`.
@@ -343,18 +356,21 @@ __device__ __forceinline__ void pair_event(const DevParams& P, int tstep, int gi
                                            EventOut& o, int side = 0) {
     o.flags = 0;
     o.ridged = -1;
+    o.sbd = 0.0;
     Kin k;
     kinematics(fi, fj, k);
     bond_create(P, tstep, gi, gj, fi, fj, k, ps);
     if (k.deltan > 0) {
         contact_part<MIXED>(P, fi, fj, k, ps, o);
     } else {
-        // reset_contact, reset.f90:1-15
+        // reset_contact, reset.f90:1-15; the last contact forces stay
         ps.thetarelc = 0.0;
         ps.deltat = 0.0;
+        o.sc = stale_contact_magnitude<MIXED>(ps);
     }
     bond_part<MIXED>(P, fi, fj, k, ps, o);
     shelter_part<MIXED>(P, fi, fj, k, uai, vai, uwi, vwi, o, side);
+    o.sri = fi.r;
 }
 
 // Walls of one floe: verify_bc (boundaries.f90:1-97) -> check_wall (:100-146)
@@ -446,6 +462,7 @@ template <bool MIXED>
 __device__ __forceinline__ void walls(const DevParams& P, Floe& f, double hist[4], WallOut& w) {
     w.flags = 0;
     w.fx_bc = 0.0; w.fy_bc = 0.0; w.m_bc = 0.0; w.ta_bc = 0.0; w.p_bc = 0.0;
+    w.sxx_bc = 0.0; w.syy_bc = 0.0; w.sxy_bc = 0.0; w.syx_bc = 0.0;
     int t1 = 0, t2 = 0;
     const int dirs[4][3] = {{0, 0, 1}, {0, 1, 1}, {1, 0, 2}, {1, 1, 2}};
 #pragma unroll
@@ -469,6 +486,12 @@ __device__ __forceinline__ void walls(const DevParams& P, Floe& f, double hist[4
         w.fx_bc = w.fx_bc - fn * c + fr * sn;
         w.fy_bc = w.fy_bc - fn * sn - fr * c;
         w.m_bc = w.m_bc - mcb - f.r * ft;
+        // wall stress, boundaries.f90:100-146 (each wall overwrites it)
+        const double fnt = sqrt(fn * fn + ft * ft);
+        w.sxx_bc = -fnt * f.r * (c * c);
+        w.syy_bc = -fnt * f.r * (sn * sn);
+        w.sxy_bc = -fnt * f.r * c * sn;
+        w.syx_bc = -fnt * f.r * sn * c;
         w.ta_bc = w.ta_bc + drb * f.h;
         w.p_bc = w.p_bc - fn * drb * f.h;
     }

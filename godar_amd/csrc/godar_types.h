@@ -45,7 +45,13 @@ struct Member {
     int32_t n;             // floe count
     int32_t tstep;         // reference step number of the next step (1-based)
     int32_t active;
+    int32_t tstop;         // the member stops before step tstop (0: no limit)
 };
+
+// A member takes part in a step while active and short of its stop step.
+__host__ __device__ inline bool member_live(const Member& m) {
+    return m.active && (m.tstop <= 0 || m.tstep < m.tstop);
+}
 
 // Error/status bits (sticky, device side).
 enum : uint32_t {
@@ -65,11 +71,21 @@ struct BondRec {
     double damage, deltanb, deltatb, trb_ji, trb_ij, rb, hb, lb, sb, ib;
 };
 
-// Contact history record (owner = lower floe index): deltat, thetarelc.
+// Bond terms of one bonded event, written by the owner (lower index) for
+// both floes' gathers.
+struct BondTerms {
+    double fbn_c, fbt_s, fbn_s, fbt_c, lri_fbt, lrj_fbt, mbb_ji, mbb_ij, sb, fbn_sb;
+};
+
+// Contact history record (owner = lower floe index): deltat, thetarelc,
+// and the pair's last contact forces fcn, fct, which the reference keeps in
+// its (n,n) arrays after the floes separate and which the stress
+// diagnostics read (stepper.f90:238-271).  A record lives while any of the
+// four is non-zero.
 struct HistRec {
     int32_t j;             // partner, -1 = empty
     int32_t pad_;
-    double deltat, thetarelc;
+    double deltat, thetarelc, fcn, fct;
 };
 
 // Per-event result the gather needs, kept in the repair table for events
@@ -86,6 +102,10 @@ struct EventOut {
     double hsfa_ji, hsfw_ji, hsfa_ij, hsfw_ij;
     // new props of the reshaped floe
     double nr, nh, nmass, ninertia, nhfa, nhfw;
+    // stress diagnostics (stepper.f90:238-271): |(fcn, fct)| (current or
+    // last contact) and |(fbn, fbt)| (current bond) of the pair, and r(i)
+    // after the event
+    double sc, sbd, sri;
 };
 
 enum : uint32_t {
@@ -103,6 +123,7 @@ struct WallOut {
     double fx_bc, fy_bc, m_bc, ta_bc, p_bc;
     double deltat_bc1, deltat_bc2, theta_bc1, theta_bc2;
     double nr, nh, nmass, ninertia, nhfa, nhfw;
+    double sxx_bc, syy_bc, sxy_bc, syx_bc;   // wall stress of the last wall touched
 };
 
 }  // namespace godar

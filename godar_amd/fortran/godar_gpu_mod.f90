@@ -162,6 +162,19 @@ module godar_gpu
             integer(c_int32_t), intent(out) :: tstep
         end function
 
+        integer(c_int) function godar_member_run_until(ctx, member, tlast) &
+                bind(C, name="godar_member_run_until")
+            import :: c_int, c_int32_t, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), value :: member, tlast
+        end function
+
+        integer(c_int) function godar_member_tsteps(ctx, tsteps) bind(C, name="godar_member_tsteps")
+            import :: c_int, c_int32_t, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), intent(out) :: tsteps(*)
+        end function
+
         integer(c_int) function godar_reserve(ctx, outer, inner, hist, bonds) &
                 bind(C, name="godar_reserve")
             import :: c_int, c_int32_t, c_ptr

@@ -53,6 +53,7 @@ class Context:
             raise _lib.GodarError(code, "godar_create failed")
         self.h = h
         self.max_floes = int(max_floes)
+        self.max_members = int(max_members)
         self.sizes: dict[int, int] = {}
         if outer or inner or hist or bonds:
             self.reserve(outer, inner, hist, bonds)
@@ -139,6 +140,19 @@ class Context:
     def set_block(self, threads: int):
         self._chk(self.L.godar_set_block(self.h, int(threads)))
 
+    def set_stage_block(self, stage: int, threads: int):
+        self._chk(self.L.godar_set_stage_block(self.h, int(stage), int(threads)))
+
+    def stage_blocks(self) -> dict:
+        t = np.zeros(_lib.NSTAGES, dtype=np.int32)
+        self._chk(self.L.godar_get_stage_blocks(self.h, t.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
+        return {self.L.godar_stage_name(k).decode(): int(t[k]) for k in range(_lib.NSTAGES)}
+
+    def autotune(self, trial: int = 3) -> dict:
+        """Pick each stage's block size by timing real steps (members advance)."""
+        self._chk(self.L.godar_autotune(self.h, int(trial)))
+        return self.stage_blocks()
+
     def use_graphs(self, on: bool = True):
         """Replay a captured HIP graph per step (default) or launch kernels one by one."""
         self._chk(self.L.godar_use_graphs(self.h, int(bool(on))))
@@ -164,6 +178,16 @@ class Context:
 
     def sync(self):
         self._chk(self.L.godar_sync(self.h))
+
+    def run_until(self, m: int, tlast: int):
+        """Member m stops advancing after step tlast (0: no limit)."""
+        self._chk(self.L.godar_member_run_until(self.h, int(m), int(tlast)))
+
+    def tsteps(self) -> np.ndarray:
+        """Next step number of every member slot (-1: free slot)."""
+        t = np.zeros(self.max_members, dtype=np.int32)
+        self._chk(self.L.godar_member_tsteps(self.h, t.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
+        return t
 
     def tstep(self, m: int) -> int:
         t = ctypes.c_int32()

@@ -66,10 +66,41 @@ def config4_request(rng: np.random.Generator, seed: int):
     generator (power-law radii, packing 0.7), 500-5,000 steps, dt = 1 s."""
     n = int(math.floor(math.exp(rng.uniform(math.log(1e3), math.log(5e4)))))
     steps = int(rng.integers(500, 5001))
+    f, p = config4_field((n, seed))
+    return f, p, steps
+
+
+def config4_field(args):
+    """Field and parameters of a served request of n floes (seeded)."""
+    n, seed = args
     f = g.make_field(n, seed, radius="powerlaw", r_lo=200.0, r_hi=5000.0, h_median=1.5,
                      h_sigma=0.3, packing=0.7, box=None, wind_speed=8.0, current_speed=0.1,
                      placement="rsa")
-    return f, params_for(f, bond_frac=0.3, bond_seed=seed), steps
+    return f, params_for(f, bond_frac=0.3, bond_seed=seed)
+
+
+def config4_requests(count: int, seed: int = 4, rate: float = 20.0, workers: int = 0):
+    """`count` served requests: Poisson arrivals at `rate` per second, sizes
+    and step counts as in config4_request, fields generated in parallel
+    processes (each depends on its own seed only).  Returns
+    [(field, params, steps, arrival)]."""
+    rng = np.random.default_rng(seed)
+    specs = []
+    for k in range(count):
+        n = int(math.floor(math.exp(rng.uniform(math.log(1e3), math.log(5e4)))))
+        steps = int(rng.integers(500, 5001))
+        specs.append((n, steps, seed * 1000003 + k))
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=count))
+    jobs = [(n, s) for n, _, s in specs]
+    if workers <= 0:
+        workers = min(16, os.cpu_count() or 1)
+    if workers > 1 and count > 4:
+        import multiprocessing as mp
+        with mp.get_context("fork").Pool(workers) as pool:
+            fps = pool.map(config4_field, jobs, chunksize=1)
+    else:
+        fps = [config4_field(j) for j in jobs]
+    return [(f, p, st, float(a)) for (f, p), (_, st, _), a in zip(fps, specs, arrivals)]
 
 
 def algorithmic_bytes(n_floes: int, ncand: int, ncont: int, nbond: int) -> float:

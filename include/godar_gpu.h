@@ -124,6 +124,12 @@ int godar_step(godar_ctx *ctx, int32_t nsteps);
 int godar_step_async(godar_ctx *ctx, int32_t nsteps);
 int godar_sync(godar_ctx *ctx);
 int godar_member_tstep(const godar_ctx *ctx, int32_t member, int32_t *tstep);
+/* Serving (no counterpart in the reference): the member stops advancing
+ * after step tlast (0: no limit); later godar_step calls leave it untouched
+ * while the other members go on. */
+int godar_member_run_until(godar_ctx *ctx, int32_t member, int32_t tlast);
+/* Next step number of every member slot (max_members entries; -1 = free). */
+int godar_member_tsteps(godar_ctx *ctx, int32_t *tsteps);
 /* Per-floe list capacities (outer candidates, inner candidates, contact
  * histories, bonds per floe); 0 keeps the current value. */
 int godar_reserve(godar_ctx *ctx, int32_t outer, int32_t inner, int32_t hist, int32_t bonds);
@@ -135,8 +141,10 @@ int godar_get_floes(godar_ctx *ctx, int32_t member, double *x, double *y, double
 /* Per-floe forces of the last step, 33 columns row-major (33, n):
  * tfx tfy m fcx fcy mc fbx fby mb fax fay ma fwx fwy mw fcorx fcory
  * fx_bc fy_bc m_bc hsfa_min hsfw_min tp tsigxx tsigyy tsigxy tsigyx
- * tac pc tab pb ta_bc p_bc  (tsig* are not computed and read 0).
- * Requires godar_record_forces(ctx, 1) before the step. */
+ * tac pc tab pb ta_bc p_bc.
+ * Requires godar_record_forces(ctx, 1) before the step; the stress
+ * diagnostics tsig* (stepper.f90:238-271) are computed only in recorded
+ * steps (the last contact forces they read are kept in every step). */
 int godar_record_forces(godar_ctx *ctx, int32_t on);
 int godar_get_forces(godar_ctx *ctx, int32_t member, double *f);
 /* Wall histories (4, n): deltat_bc1 deltat_bc2 theta_bc1 theta_bc2. */
@@ -158,8 +166,17 @@ int godar_get_stats(godar_ctx *ctx, int32_t member, int64_t *stats);
 #define GODAR_NTOTALS 6
 int godar_get_totals(godar_ctx *ctx, int64_t *totals);
 
-/* Pipeline tuning (autotuner): threads per block of the per-floe kernels. */
+/* Pipeline tuning (no counterpart in the reference).  Threads per block
+ * (64, 128 or 256) of the per-floe kernels of every stage, or of one stage
+ * (stage numbers as in godar_stage_name; the sort and repair stages have
+ * fixed shapes).  Results do not depend on block sizes. */
 int godar_set_block(godar_ctx *ctx, int32_t threads);
+int godar_set_stage_block(godar_ctx *ctx, int32_t stage, int32_t threads);
+int godar_get_stage_blocks(const godar_ctx *ctx, int32_t *threads);   /* GODAR_NSTAGES entries */
+/* Autotuner: for each tunable stage in turn, time `trial` real steps of the
+ * current members with each block size and keep the fastest.  The members
+ * advance by 1 + 3 * trial * (tunable stages) steps. */
+int godar_autotune(godar_ctx *ctx, int32_t trial);
 /* Step launch mode (no counterpart in the reference): on (the default), each
  * step replays a captured HIP graph of the whole kernel sequence, recaptured
  * whenever members, capacities or the block size change; off, the kernels are
@@ -174,7 +191,7 @@ int godar_use_graphs(godar_ctx *ctx, int32_t on);
  * godar_get_profile returns the summed milliseconds and the number of timed
  * launches per stage.  godar_profile resets the totals.
  */
-#define GODAR_NSTAGES 11
+#define GODAR_NSTAGES 12
 int godar_profile(godar_ctx *ctx, int32_t on);
 int godar_get_profile(godar_ctx *ctx, double *ms, int64_t *launches);
 const char *godar_stage_name(int32_t stage);

@@ -19,6 +19,7 @@ from workloads import config0_params, kick, packed_box, squeeze
 pytestmark = pytest.mark.gpu
 
 STATE = ["x", "y", "u", "v", "omega", "r", "h"]
+STRESS = ["tsigxx", "tsigyy", "tsigxy", "tsigyx"]
 FP64_REL_TOL = 1e-8
 
 
@@ -88,10 +89,37 @@ def test_one_step_identical_inputs():
         fa, fb = ctx.forces(m), O.forces()
         for k in STATE:
             assert _rel(a[k], b[k]) <= 1e-13, k
-        for k in ("fcx", "fcy", "mc", "fbx", "fby", "mb", "fx_bc", "fy_bc", "m_bc", "tac", "pc"):
+        for k in ("fcx", "fcy", "mc", "fbx", "fby", "mb", "fx_bc", "fy_bc", "m_bc", "tac", "pc",
+                  *STRESS):
             scale = float(np.max(np.abs(fb[k]))) + 1e-300
             assert float(np.max(np.abs(fa[k] - fb[k]))) / scale <= 1e-13, k
         ctx.close()
+
+
+def test_stress_diagnostics_match_oracle():
+    """Stress diagnostics (stepper.f90:238-271) in a colliding, ridging box:
+    candidate pairs that separated keep contributing their last contact
+    forces; FP64 vs the oracle every step, within the FP64 tolerance."""
+    from oracle_py import OracleSim
+    f = kick(squeeze(packed_box(64, 3)), 1)
+    p = config0_params(f, sigmanc_crit=1e5)
+    ctx = _ctx(f.n)
+    m = ctx.add_field(p, f)
+    O = OracleSim(p, f)
+    ever, stale_seen, worst = set(), 0, 0.0
+    for t in range(1, 81):
+        ctx.step(1)
+        O.step()
+        cand, cont = _oracle_pairs(O)
+        ever |= set(cont)
+        stale_seen += len((set(cand) - set(cont)) & ever)
+        fa, fb = ctx.forces(m), O.forces()
+        for k in STRESS:
+            scale = float(np.max(np.abs(fb[k]))) + 1e-300
+            worst = max(worst, float(np.max(np.abs(fa[k] - fb[k]))) / scale)
+    ctx.close()
+    assert stale_seen > 0, "no separated pair exercised the last-contact forces"
+    assert worst < FP64_REL_TOL, worst
 
 
 def test_config0_trajectory_fp64():
@@ -225,6 +253,35 @@ def test_graph_replay_matches_eager():
                 assert np.array_equal(graph.forces(mg)["tfx"], eager.forces(me)["tfx"])
         eager.close()
         graph.close()
+
+
+def test_block_sizes_and_autotune_bitwise():
+    """Results do not depend on the kernels' block sizes: 64-thread blocks,
+    the autotuner's choice (which itself steps the members) and the default
+    all give the same trajectory, bitwise."""
+    f = g.config0_member(31, 800)
+    fb = kick(squeeze(packed_box(64, 3)), 1)
+    pa, pb = config0_params(f), config0_params(fb, sigmanc_crit=1e5)
+    runs = []
+    for mode in ("default", "b64", "auto"):
+        ctx = _ctx(f.n + fb.n, 2, "mixed")
+        ms = [ctx.add_field(pa, f), ctx.add_field(pb, fb)]
+        if mode == "b64":
+            ctx.set_block(64)
+        if mode == "auto":
+            blocks = ctx.autotune(1)
+            assert set(blocks.values()) <= {64, 128, 256}
+            done = ctx.tstep(ms[0]) - 1
+        else:
+            done = 0
+        ctx.step(40 - done)
+        assert ctx.tstep(ms[0]) == 41
+        runs.append([ctx.floes(m) for m in ms])
+        ctx.close()
+    for other in runs[1:]:
+        for a, b in zip(runs[0], other):
+            for k in STATE:
+                assert np.array_equal(a[k], b[k]), k
 
 
 def test_capacity_overflow_rolls_back():
