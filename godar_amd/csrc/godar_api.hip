@@ -17,6 +17,7 @@ __global__ void k_bin(DS);
 __global__ void k_clear_cells(DS);
 __global__ void k_ranges(DS);
 template <bool M> __global__ void k_cand(DS);
+template <bool M> __global__ void k_cand_dense(DS);
 template <bool M> __global__ void k_contact(DS);
 template <bool M> __global__ void k_bond(DS);
 __global__ void k_bond_gather(DS);
@@ -85,6 +86,7 @@ struct godar_ctx {
     // device-state pointers by value), each with the launch signature it was
     // captured for
     bool use_graphs = true;
+    int dense_scan = 96;                // floes per 3x3 search above which a member is dense
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
     std::vector<unsigned char> gsig[2];
 };
@@ -149,7 +151,7 @@ void derive_params(const godar_params& p, DevParams& d) {
 void free_lists(godar_ctx* c) {
     DS& d = c->d;
     dfree(d.oj); dfree(d.od); dfree(d.ik); dfree(d.evf); dfree(d.ov); dfree(d.bterm);
-    dfree(d.ssc); dfree(d.ssb);
+    dfree(d.ssc); dfree(d.ssb); dfree(d.ibond);
     for (int k = 0; k < 2; ++k) { dfree(c->hist[k]); dfree(c->bond[k]); }
 }
 
@@ -164,6 +166,7 @@ hipError_t alloc_lists(godar_ctx* c) {
     if ((e = dalloc(&d.ov, S * c->CO)) != hipSuccess) return e;
     if ((e = dalloc(&d.bterm, S * c->BC)) != hipSuccess) return e;
     if ((e = dalloc(&d.ssc, S * c->CO)) != hipSuccess) return e;
+    if ((e = dalloc(&d.ibond, S * c->CI)) != hipSuccess) return e;
     if ((e = dalloc(&d.ssb, S * c->CO)) != hipSuccess) return e;
     for (int k = 0; k < 2; ++k) {
         if ((e = dalloc(&c->hist[k], S * c->HC)) != hipSuccess) return e;
@@ -321,6 +324,14 @@ int launch_step(godar_ctx* c) {
         StageTimer t(c, 3);
         hipLaunchKernelGGL(mixed ? k_cand<true> : k_cand<false>, grid_for(N, B[3], true),
                            dim3(B[3]), 0, s, d);
+        bool any_dense = false;
+        for (int k = 0; k < c->Mcap; ++k)
+            if (c->members[k].active && c->dparams[k].dense) any_dense = true;
+        if (any_dense) {
+            // one wavefront per dense floe, a contiguous share of the queue per XCD
+            hipLaunchKernelGGL(mixed ? k_cand_dense<true> : k_cand_dense<false>,
+                               dim3(XCD_COUNT * 1024), dim3(64), 0, s, d);
+        }
     }
     {
         StageTimer t(c, 4);
@@ -506,9 +517,9 @@ int godar_create(int64_t max_floes, int32_t max_members, int32_t precision, goda
     ACHK(dalloc(&d.wout, S));
     ACHK(dalloc(&d.wflag, S));
     ACHK(dalloc(&d.cls, S));
-    ACHK(dalloc(&d.qlist, 3 * S));
+    ACHK(dalloc(&d.qlist, 4 * S));
     ACHK(dalloc(&d.bgat, S));
-    ACHK(dalloc(&d.qn, 3));
+    ACHK(dalloc(&d.qn, 4));
     ACHK(dalloc(&d.seeds, (int64_t)c->SEEDK * S));
     ACHK(dalloc(&d.nseeds, max_members));
     ACHK(dalloc(&d.heap, (int64_t)c->HEAPK * S));
@@ -582,10 +593,10 @@ static int64_t bytes_per_floe(const godar_ctx* c) {
     per += sizeof(FloeRec) + sizeof(GeoRec) + sizeof(ShelRec) + sizeof(KinRec);
     per += 4 /*pos_of*/ + 4 * 8 /*wbc*/ + 4 /*member_of*/ + 4 + 4 /*ckey, cval*/;
     per += 4 + 4 /*ocnt, icnt*/ + 2 * (4 + 4) /*hcnt, bcnt*/;
-    per += (int64_t)c->CO * (8 + 8 + 4 + 4 + 8 + 8) + (int64_t)c->CI * 8;   // oj od evf ov ssc ssb; ik
+    per += (int64_t)c->CO * (8 + 8 + 4 + 4 + 8 + 8) + (int64_t)c->CI * (8 + 4);   // oj od evf ov ssc ssb; ik ibond
     per += 2 * ((int64_t)c->HC * sizeof(HistRec) + (int64_t)c->BC * sizeof(BondRec));
     per += (int64_t)c->BC * sizeof(BondTerms) + 1 /*bgat*/;
-    per += NACC * 8 + sizeof(WallOut) + 1 /*wflag*/ + 1 /*cls*/ + 3 * 4 /*qlist*/;
+    per += NACC * 8 + sizeof(WallOut) + 1 /*wflag*/ + 1 /*cls*/ + 4 * 4 /*qlist*/;
     per += (int64_t)c->SEEDK * 8 + (int64_t)c->HEAPK * 8 + (int64_t)c->TABK * sizeof(EventOut);
     per += sizeof(WallOut) /*wtab*/ + 4 /*ovw*/ + 6 * 8 /*cur*/ + 1 + 1 /*dirty, affected*/;
     per += 33 * 8 /*fout*/ + 4 + 4 /*radix keys, values*/;
@@ -637,10 +648,25 @@ int godar_member_add(godar_ctx* c, const godar_params* p, int32_t n, const doubl
     }
     DevParams dp;
     derive_params(*p, dp);
-    dp.cell = (rmax + p->rtree) * 1.0000001 + 1e-9;
-    int64_t gx = std::max<int64_t>(1, (int64_t)std::floor(p->nx / dp.cell));
-    int64_t gy = std::max<int64_t>(1, (int64_t)std::floor(p->ny / dp.cell));
-    while (gx * gy > 4 * (int64_t)n + 64) { gx = (gx + 1) / 2; gy = (gy + 1) / 2; dp.cell *= 2; }
+    // cell >= (max query reach) / span, so that (2 span + 1)^2 cells around a
+    // floe hold every candidate.  Grids are coarsened to at most 4n+64 cells.
+    auto grid = [&](int span, double& cell, int64_t& gx, int64_t& gy) {
+        cell = (rmax + p->rtree) * 1.0000001 / span + 1e-9;
+        gx = std::max<int64_t>(1, (int64_t)std::floor(p->nx / cell));
+        gy = std::max<int64_t>(1, (int64_t)std::floor(p->ny / cell));
+        while (gx * gy > 4 * (int64_t)n + 64) { gx = (gx + 1) / 2; gy = (gy + 1) / 2; cell *= 2; }
+    };
+    int64_t gx, gy;
+    grid(1, dp.cell, gx, gy);
+    dp.span = 1;
+    // floes examined per neighbour search (3x3 cells): from about two
+    // wavefronts' worth, a wavefront per floe beats a thread per floe, and
+    // a finer grid (5x5 cells of half the size) examines 30% fewer floes
+    dp.dense = 9.0 * (double)n / (double)(gx * gy) >= (double)c->dense_scan ? 1 : 0;
+    if (dp.dense) {
+        grid(2, dp.cell, gx, gy);
+        dp.span = 2;
+    }
     dp.gx = (int32_t)gx;
     dp.gy = (int32_t)gy;
     int32_t ncell = (int32_t)(gx * gy);
@@ -815,6 +841,12 @@ int godar_set_block(godar_ctx* c, int32_t threads) {
     c->block = threads;
     for (int k = 0; k < GODAR_NSTAGES; ++k)
         if (tunable_stage(k)) c->sblock[k] = threads;
+    return GODAR_OK;
+}
+
+int godar_set_dense_search(godar_ctx* c, int32_t floes) {
+    if (!c || floes < 0) return GODAR_ERR_ARG;
+    c->dense_scan = floes;
     return GODAR_OK;
 }
 

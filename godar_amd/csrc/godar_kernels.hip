@@ -230,7 +230,7 @@ __global__ void k_clear_cells(DS S) {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c > S.ncells || *S.err) return;
     S.crange[c] = make_int2(0, 0);
-    if (c < 3) S.qn[c] = 0;
+    if (c < 4) S.qn[c] = 0;
 }
 
 // cell ranges over the sorted keys, the sorted floe records and the wind at
@@ -291,11 +291,145 @@ __device__ __forceinline__ void enqueue(bool want, int32_t* q, int32_t* cnt, int
 //    bond records (or a bond of its own forms this step), the wall queue if
 //    it crosses the box.
 // ---------------------------------------------------------------------
+// Finishing writes of one floe's search (any search shape): list counts,
+// per-slot resets, sheltering minima and the per-step resets of its rows.
+template <bool MIXED>
+__device__ __forceinline__ void cand_finish(const DS& S, int k, int f, const DevParams& P, int no,
+                                            int ni, double sha, double shw, bool qc, bool qb,
+                                            int lane0, int lanes) {
+    const int64_t SS = S.S;
+    const uint32_t fl = P.shelter ? EV_SHELTER : 0u;
+    for (int s2 = lane0; s2 < no; s2 += lanes) {
+        S.evf[s2 * SS + k] = fl;
+        S.ov[s2 * SS + k] = -1;
+        if (S.record_forces) {
+            S.ssc[s2 * SS + k] = 0.0;
+            S.ssb[s2 * SS + k] = 0.0;
+        }
+    }
+    for (int s2 = lane0; s2 < ni; s2 += lanes) S.ibond[s2 * SS + k] = 0;
+    if (lane0 == 0) {
+        S.ocnt[k] = no;
+        S.icnt[k] = ni;
+        S.acc[A_SHA * SS + k] = sha;
+        S.acc[A_SHW * SS + k] = shw;
+        if (!qc) S.hcnt_new[f] = 0;
+        if (!qb) S.bcnt_new[f] = 0;
+        S.ovw[k] = -1;
+        S.wflag[k] = 0;
+        S.bgat[k] = 0;
+        S.cls[k] = qc ? CLS_CONTACT : 0;   // k_bond_gather adds CLS_BOND
+    }
+}
+
+// Neighbour search of one floe by one thread: candidate lists with
+// insertion into the slot-major rows, sheltering minima, classification and
+// the per-step resets of the floe's rows.
+template <bool MIXED>
+__device__ __forceinline__ void cand_thread(const DS& S, int k, const GeoRec& me, bool& qc,
+                                            bool& qb, bool& qw) {
+    using T = typename Prec<MIXED>::T;
+    const int m = me.member;
+    const DevParams& P = S.params[m];
+    const double rad = me.r + P.rtree;
+    const int f = me.id;
+    const int first = S.members[m].first;
+    const bool bond_step = S.members[m].tstep == 1 && P.restart != 1 && P.cohesion;
+    const int cx = cell_coord(me.x, P.cell, P.gx), cy = cell_coord(me.y, P.cell, P.gy);
+    const ShelRec msh = S.shel[k];
+    const T uwi = T(P.uw), vwi = T(P.vw);
+    T sha = T(1), shw = T(1);
+    qc = S.hcnt_old[f] > 0;
+    qb = S.bcnt_old[f] > 0;
+    qw = me.y - me.r < 0 || me.y + me.r > P.ny || me.x - me.r < 0 || me.x + me.r > P.nx;
+    int no = 0, ni = 0;
+    uint32_t over = 0;
+    const int64_t SS = S.S;
+    int64_t* oj = S.oj;
+    int64_t* ik = S.ik;
+    const int sp = P.span;
+    const int a0 = max(cx - sp, 0), a1 = min(cx + sp, P.gx - 1);
+    for (int b = cy - sp; b <= cy + sp; ++b) {
+        if (b < 0 || b >= P.gy) continue;
+        // the cells of one row of the block are consecutive in the cell
+        // order: one contiguous range of positions
+        int lo = INT_MAX, hi = 0;
+        for (int a = a0; a <= a1; ++a) {
+            const int2 rg = S.crange[P.cell_base + b * P.gx + a];
+            if (rg.y > rg.x) { lo = min(lo, rg.x); hi = max(hi, rg.y); }
+        }
+        {
+            for (int q = lo; q < hi; ++q) {
+                if (q == k) continue;
+                const GeoRec* o = &S.geo[q];
+                double dx = me.x - o->x, dy = me.y - o->y;
+                double d = sqrt(dx * dx + dy * dy);
+                int j = o->id;
+                if (j > f) {
+                    if (!(d <= rad)) continue;
+                    if (no >= S.CO) { over |= ERR_OUTER_CAP; continue; }
+                    int p = no;
+                    while (p > 0) {
+                        double dp = S.od[(p - 1) * SS + k];
+                        int64_t ep = oj[(p - 1) * SS + k];
+                        if (dp < d || (dp == d && ent_id(ep) < j)) break;
+                        S.od[p * SS + k] = dp;
+                        oj[p * SS + k] = ep;
+                        --p;
+                    }
+                    S.od[p * SS + k] = d;
+                    oj[p * SS + k] = ent(j, q);
+                    ++no;
+                    // pair (f, j): f is the outer floe i of the event
+                    const double deltan = me.r + o->r - d;
+                    if (deltan > 0) qc = true;
+                    if (!qb && bond_step && deltan >= -P.bond_lim &&
+                        bond_keep(f - first, j - first, P.bond_seed, P.bond_thr24))
+                        qb = true;
+                    if (P.shelter) {
+                        // floe f takes hsf*(j,f), evaluated with -cosa, -sina
+                        const double cosa = (o->x - me.x) / d, sina = (o->y - me.y) / d;
+                        const ShelRec oh = S.shel[q];
+                        const T dn = T(deltan), c = T(cosa), sn = T(sina);
+                        sha = dmin(sha, S_shelter(T(oh.hfa), T(msh.hfa), dn, -c, -sn, T(msh.wx), T(msh.wy)));
+                        shw = dmin(shw, S_shelter(T(oh.hfw), T(msh.hfw), dn, -c, -sn, uwi, vwi));
+                    }
+                } else {
+                    if (!(d <= o->r + P.rtree)) continue;
+                    if (ni >= S.CI) { over |= ERR_INNER_CAP; continue; }
+                    int p = ni;
+                    const int64_t e = ent(j, q);
+                    while (p > 0) {
+                        int64_t ep = ik[(p - 1) * SS + k];
+                        if (ep < e) break;
+                        ik[p * SS + k] = ep;
+                        --p;
+                    }
+                    ik[p * SS + k] = e;
+                    ++ni;
+                    // pair (j, f): f is the inner floe j of the event
+                    const double deltan = o->r + me.r - d;
+                    if (deltan > 0) qc = true;
+                    if (P.shelter) {
+                        // cosa/sina point from j to f; floe f takes hsf*(j,f) with j's wind
+                        const double cosa = (me.x - o->x) / d, sina = (me.y - o->y) / d;
+                        const ShelRec oh = S.shel[q];
+                        const T dn = T(deltan), c = T(cosa), sn = T(sina);
+                        sha = dmin(sha, S_shelter(T(oh.hfa), T(msh.hfa), dn, c, sn, T(oh.wx), T(oh.wy)));
+                        shw = dmin(shw, S_shelter(T(oh.hfw), T(msh.hfw), dn, c, sn, uwi, vwi));
+                    }
+                }
+            }
+        }
+    }
+    if (over) set_err(S, over);
+    cand_finish<MIXED>(S, k, f, P, no, ni, double(sha), double(shw), qc, qb, 0, 1);
+}
+
 template <bool MIXED>
 __global__ __launch_bounds__(FLOE_BLOCK) void k_cand(DS S) {
-    using T = typename Prec<MIXED>::T;
     const int k = xcd_block() * blockDim.x + threadIdx.x;
-    bool qc = false, qb = false, qw = false;
+    bool qc = false, qb = false, qw = false, qd = false;
     bool live = k < S.N && !*S.err;
     GeoRec me;
     if (live) {
@@ -307,118 +441,219 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_cand(DS S) {
         }
     }
     if (live) {
-        const int m = me.member;
-        const DevParams& P = S.params[m];
-        const double rad = me.r + P.rtree;
-        const int f = me.id;
-        const int first = S.members[m].first;
-        const bool bond_step = S.members[m].tstep == 1 && P.restart != 1 && P.cohesion;
-        const int cx = cell_coord(me.x, P.cell, P.gx), cy = cell_coord(me.y, P.cell, P.gy);
-        const ShelRec msh = S.shel[k];
-        const T uwi = T(P.uw), vwi = T(P.vw);
-        T sha = T(1), shw = T(1);
-        qc = S.hcnt_old[f] > 0;
-        qb = S.bcnt_old[f] > 0;
-        qw = me.y - me.r < 0 || me.y + me.r > P.ny || me.x - me.r < 0 || me.x + me.r > P.nx;
-        int no = 0, ni = 0;
-        uint32_t over = 0;
-        const int64_t SS = S.S;
-        int64_t* oj = S.oj;
-        int64_t* ik = S.ik;
-        for (int b = cy - 1; b <= cy + 1; ++b) {
-            if (b < 0 || b >= P.gy) continue;
-            for (int a = cx - 1; a <= cx + 1; ++a) {
-                if (a < 0 || a >= P.gx) continue;
-                const int2 rg = S.crange[P.cell_base + b * P.gx + a];
-                for (int q = rg.x; q < rg.y; ++q) {
-                    if (q == k) continue;
-                    const GeoRec* o = &S.geo[q];
-                    double dx = me.x - o->x, dy = me.y - o->y;
-                    double d = sqrt(dx * dx + dy * dy);
-                    int j = o->id;
-                    if (j > f) {
-                        if (!(d <= rad)) continue;
-                        if (no >= S.CO) { over |= ERR_OUTER_CAP; continue; }
-                        int p = no;
-                        while (p > 0) {
-                            double dp = S.od[(p - 1) * SS + k];
-                            int64_t ep = oj[(p - 1) * SS + k];
-                            if (dp < d || (dp == d && ent_id(ep) < j)) break;
-                            S.od[p * SS + k] = dp;
-                            oj[p * SS + k] = ep;
-                            --p;
-                        }
-                        S.od[p * SS + k] = d;
-                        oj[p * SS + k] = ent(j, q);
-                        ++no;
-                        // pair (f, j): f is the outer floe i of the event
-                        const double deltan = me.r + o->r - d;
-                        if (deltan > 0) qc = true;
-                        if (!qb && bond_step && deltan >= -P.bond_lim &&
-                            bond_keep(f - first, j - first, P.bond_seed, P.bond_thr24))
-                            qb = true;
-                        if (P.shelter) {
-                            // floe f takes hsf*(j,f), evaluated with -cosa, -sina
-                            const double cosa = (o->x - me.x) / d, sina = (o->y - me.y) / d;
-                            const ShelRec oh = S.shel[q];
-                            const T dn = T(deltan), c = T(cosa), sn = T(sina);
-                            sha = dmin(sha, S_shelter(T(oh.hfa), T(msh.hfa), dn, -c, -sn, T(msh.wx), T(msh.wy)));
-                            shw = dmin(shw, S_shelter(T(oh.hfw), T(msh.hfw), dn, -c, -sn, uwi, vwi));
-                        }
-                    } else {
-                        if (!(d <= o->r + P.rtree)) continue;
-                        if (ni >= S.CI) { over |= ERR_INNER_CAP; continue; }
-                        int p = ni;
-                        const int64_t e = ent(j, q);
-                        while (p > 0) {
-                            int64_t ep = ik[(p - 1) * SS + k];
-                            if (ep < e) break;
-                            ik[p * SS + k] = ep;
-                            --p;
-                        }
-                        ik[p * SS + k] = e;
-                        ++ni;
-                        // pair (j, f): f is the inner floe j of the event
-                        const double deltan = o->r + me.r - d;
-                        if (deltan > 0) qc = true;
-                        if (P.shelter) {
-                            // cosa/sina point from j to f; floe f takes hsf*(j,f) with j's wind
-                            const double cosa = (me.x - o->x) / d, sina = (me.y - o->y) / d;
-                            const ShelRec oh = S.shel[q];
-                            const T dn = T(deltan), c = T(cosa), sn = T(sina);
-                            sha = dmin(sha, S_shelter(T(oh.hfa), T(msh.hfa), dn, c, sn, T(oh.wx), T(oh.wy)));
-                            shw = dmin(shw, S_shelter(T(oh.hfw), T(msh.hfw), dn, c, sn, uwi, vwi));
-                        }
-                    }
-                }
-            }
-        }
-        if (over) set_err(S, over);
-        S.ocnt[k] = no;
-        S.icnt[k] = ni;
-        const uint32_t fl = P.shelter ? EV_SHELTER : 0u;
-        for (int s2 = 0; s2 < no; ++s2) {
-            S.evf[s2 * SS + k] = fl;
-            S.ov[s2 * SS + k] = -1;
-        }
-        if (S.record_forces) {
-            for (int s2 = 0; s2 < no; ++s2) {
-                S.ssc[s2 * SS + k] = 0.0;
-                S.ssb[s2 * SS + k] = 0.0;
-            }
-        }
-        S.acc[A_SHA * SS + k] = double(sha);
-        S.acc[A_SHW * SS + k] = double(shw);
-        if (!qc) S.hcnt_new[f] = 0;
-        if (!qb) S.bcnt_new[f] = 0;
-        S.ovw[k] = -1;
-        S.wflag[k] = 0;
-        S.bgat[k] = 0;
-        S.cls[k] = qc ? CLS_CONTACT : 0;   // k_bond_gather adds CLS_BOND
+        if (S.params[me.member].dense) qd = true;   // k_cand_dense searches it
+        else cand_thread<MIXED>(S, k, me, qc, qb, qw);
     }
     enqueue(qc, S.qlist, &S.qn[0], k);
     enqueue(qb, S.qlist + S.S, &S.qn[1], k);
     enqueue(qw, S.qlist + 2 * S.S, &S.qn[2], k);
+    enqueue(qd, S.qlist + 3 * S.S, &S.qn[3], k);
+}
+
+// Dense members (many floes per cell): one wavefront per floe.  The lanes
+// test 64 cell entries at a time, survivors are compacted into LDS with
+// ballots, then sorted there (bitonic network over the padded power of two):
+// outer candidates by (distance, id), inner by id -- the orders of the thread
+// search.  Sheltering minima are reduced over the lanes (a minimum is exact in
+// any order).  A floe with more than DENSE_CAP candidates on a side falls
+// back to the thread search (lane 0).  Blocks are one wavefront; the queue of
+// dense floes is split into one contiguous share per XCD.
+constexpr int DENSE_CAP = 256;
+
+__device__ __forceinline__ void bitonic_outer(uint64_t* kd, int64_t* ke, int n, int lane) {
+    for (int size = 2; size <= n; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int i = lane; i < n; i += 64) {
+                const int pj = i ^ stride;
+                if (pj > i) {
+                    const bool up = (i & size) == 0;
+                    const uint64_t d1 = kd[i], d2 = kd[pj];
+                    const int64_t e1 = ke[i], e2 = ke[pj];
+                    const bool gt = d1 > d2 || (d1 == d2 && e1 > e2);
+                    if (gt == up) { kd[i] = d2; kd[pj] = d1; ke[i] = e2; ke[pj] = e1; }
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+__device__ __forceinline__ void bitonic_inner(int64_t* ke, int n, int lane) {
+    for (int size = 2; size <= n; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int i = lane; i < n; i += 64) {
+                const int pj = i ^ stride;
+                if (pj > i) {
+                    const bool up = (i & size) == 0;
+                    const int64_t e1 = ke[i], e2 = ke[pj];
+                    if ((e1 > e2) == up) { ke[i] = e2; ke[pj] = e1; }
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_min(T v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = dmin(v, (T)__shfl_xor(v, off));
+    return v;
+}
+
+__device__ __forceinline__ int pow2_at_least(int n) {
+    int p = 1;
+    while (p < n) p <<= 1;
+    return p;
+}
+
+template <bool MIXED>
+__global__ __launch_bounds__(64) void k_cand_dense(DS S) {
+    using T = typename Prec<MIXED>::T;
+    __shared__ uint64_t sd[DENSE_CAP];
+    __shared__ int64_t se[DENSE_CAP];
+    __shared__ int64_t si[DENSE_CAP];
+    if (*S.err) return;
+    const int lane = threadIdx.x;
+    const int nq = S.qn[3];
+    const int nb = (int)(gridDim.x / XCD_COUNT);
+    const int xcd = (int)(blockIdx.x % XCD_COUNT), bi = (int)(blockIdx.x / XCD_COUNT);
+    const int per = (nq + XCD_COUNT - 1) / XCD_COUNT;
+    const int t1 = min(nq, xcd * per + per);
+    const uint64_t lt = lane ? ((~0ull) >> (64 - lane)) : 0ull;
+    const int64_t SS = S.S;
+    for (int t = xcd * per + bi; t < t1; t += nb) {
+        const int k = S.qlist[3 * SS + t];
+        const GeoRec me = S.geo[k];
+        const int m = me.member;
+        const DevParams& P = S.params[m];
+        const double rad = me.r + P.rtree;
+        const int f = me.id;
+        const int sp = P.span;
+        const int cx = cell_coord(me.x, P.cell, P.gx), cy = cell_coord(me.y, P.cell, P.gy);
+        // 1. scan: the cells of one row of the block are consecutive in the
+        //    cell order, so each row is one contiguous range of positions
+        int no = 0, ni = 0;
+        const int a0 = max(cx - sp, 0), a1 = min(cx + sp, P.gx - 1);
+        for (int b = cy - sp; b <= cy + sp; ++b) {
+            if (b < 0 || b >= P.gy) continue;
+            int lo = INT_MAX, hi = 0;
+            for (int a = a0; a <= a1; ++a) {
+                const int2 rg = S.crange[P.cell_base + b * P.gx + a];
+                if (rg.y > rg.x) { lo = min(lo, rg.x); hi = max(hi, rg.y); }
+            }
+            for (int base = lo; base < hi; base += 64) {
+                const int q = base + lane;
+                bool vout = false, vin = false;
+                int oid = 0;
+                double d = 0.0;
+                if (q < hi && q != k) {
+                    const GeoRec o = S.geo[q];
+                    const double dx = me.x - o.x, dy = me.y - o.y;
+                    d = sqrt(dx * dx + dy * dy);
+                    oid = o.id;
+                    if (o.id > f) vout = d <= rad;
+                    else vin = d <= o.r + P.rtree;
+                }
+                const uint64_t mo = __ballot(vout), mi = __ballot(vin);
+                if (vout) {
+                    const int slot = no + (int)__popcll(mo & lt);
+                    if (slot < DENSE_CAP) {
+                        sd[slot] = (uint64_t)__double_as_longlong(d);
+                        se[slot] = ent(oid, q);
+                    }
+                }
+                if (vin) {
+                    const int slot = ni + (int)__popcll(mi & lt);
+                    if (slot < DENSE_CAP) si[slot] = ent(oid, q);
+                }
+                no += (int)__popcll(mo);
+                ni += (int)__popcll(mi);
+            }
+        }
+        __syncthreads();
+        if (no > DENSE_CAP || ni > DENSE_CAP) {
+            // rare: more candidates than LDS holds, the thread search does it
+            if (lane == 0) {
+                bool c2, b2, w2;
+                cand_thread<MIXED>(S, k, me, c2, b2, w2);
+                if (c2) S.qlist[atomicAdd(&S.qn[0], 1)] = k;
+                if (b2) S.qlist[SS + atomicAdd(&S.qn[1], 1)] = k;
+                if (w2) S.qlist[2 * SS + atomicAdd(&S.qn[2], 1)] = k;
+            }
+            __syncthreads();
+            continue;
+        }
+        uint32_t over = 0;
+        if (no > S.CO) { over |= ERR_OUTER_CAP; no = S.CO; }
+        if (ni > S.CI) { over |= ERR_INNER_CAP; ni = S.CI; }
+        if (over && lane == 0) set_err(S, over);
+        // 2. sort: outer by (distance, id), inner by id
+        const int po = pow2_at_least(no), pi = pow2_at_least(ni);
+        for (int i = no + lane; i < po; i += 64) { sd[i] = ~0ull; se[i] = INT64_MAX; }
+        for (int i = ni + lane; i < pi; i += 64) si[i] = INT64_MAX;
+        __syncthreads();
+        bitonic_outer(sd, se, po, lane);
+        bitonic_inner(si, pi, lane);
+        // 3. sheltering and classification over the compacted candidates
+        const int first = S.members[m].first;
+        const bool bond_step = S.members[m].tstep == 1 && P.restart != 1 && P.cohesion;
+        const ShelRec msh = S.shel[k];
+        const T uwi = T(P.uw), vwi = T(P.vw);
+        T sha = T(1), shw = T(1);
+        bool lqc = false, lqb = false;
+        for (int s2 = lane; s2 < no; s2 += 64) {
+            const int64_t e = se[s2];
+            const int j = ent_id(e), q = ent_pos(e);
+            const double d = __longlong_as_double((long long)sd[s2]);
+            S.od[s2 * SS + k] = d;
+            S.oj[s2 * SS + k] = e;
+            const GeoRec o = S.geo[q];
+            // pair (f, j): f is the outer floe i of the event
+            const double deltan = me.r + o.r - d;
+            if (deltan > 0) lqc = true;
+            if (bond_step && deltan >= -P.bond_lim &&
+                bond_keep(f - first, j - first, P.bond_seed, P.bond_thr24))
+                lqb = true;
+            if (P.shelter) {
+                const double cosa = (o.x - me.x) / d, sina = (o.y - me.y) / d;
+                const ShelRec oh = S.shel[q];
+                const T dn = T(deltan), c = T(cosa), sn = T(sina);
+                sha = dmin(sha, S_shelter(T(oh.hfa), T(msh.hfa), dn, -c, -sn, T(msh.wx), T(msh.wy)));
+                shw = dmin(shw, S_shelter(T(oh.hfw), T(msh.hfw), dn, -c, -sn, uwi, vwi));
+            }
+        }
+        for (int s2 = lane; s2 < ni; s2 += 64) {
+            const int64_t e = si[s2];
+            const int q = ent_pos(e);
+            S.ik[s2 * SS + k] = e;
+            const GeoRec o = S.geo[q];
+            const double dx = me.x - o.x, dy = me.y - o.y;
+            const double d = sqrt(dx * dx + dy * dy);
+            // pair (j, f): f is the inner floe j of the event
+            const double deltan = o.r + me.r - d;
+            if (deltan > 0) lqc = true;
+            if (P.shelter) {
+                const double cosa = (me.x - o.x) / d, sina = (me.y - o.y) / d;
+                const ShelRec oh = S.shel[q];
+                const T dn = T(deltan), c = T(cosa), sn = T(sina);
+                sha = dmin(sha, S_shelter(T(oh.hfa), T(msh.hfa), dn, c, sn, T(oh.wx), T(oh.wy)));
+                shw = dmin(shw, S_shelter(T(oh.hfw), T(msh.hfw), dn, c, sn, uwi, vwi));
+            }
+        }
+        sha = wave_min(sha);
+        shw = wave_min(shw);
+        const bool qc = __ballot(lqc) != 0 || S.hcnt_old[f] > 0;
+        const bool qb = __ballot(lqb) != 0 || S.bcnt_old[f] > 0;
+        const bool qw = me.y - me.r < 0 || me.y + me.r > P.ny || me.x - me.r < 0 || me.x + me.r > P.nx;
+        cand_finish<MIXED>(S, k, f, P, no, ni, double(sha), double(shw), qc, qb, lane, 64);
+        if (lane == 0) {
+            if (qc) S.qlist[atomicAdd(&S.qn[0], 1)] = k;
+            if (qb) S.qlist[SS + atomicAdd(&S.qn[1], 1)] = k;
+            if (qw) S.qlist[2 * SS + atomicAdd(&S.qn[2], 1)] = k;
+        }
+        __syncthreads();
+    }
 }
 
 // ---------------------------------------------------------------------
@@ -596,21 +831,31 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_bond(DS S) {
 
     // the owner's records carry over (cleared entries dropped)
     int nb = 0;
-    {
-        const int bco = S.bcnt_old[f];
-        for (int q = 0; q < bco; ++q) {
-            const BondRec* b = &S.bond_old[q * SS + f];
-            if (b->j >= 0) copy_bond(&S.bond_new[(nb++) * SS + f], b);
-        }
+    const int bco = S.bcnt_old[f];
+    for (int q = 0; q < bco; ++q) {
+        const BondRec* b = &S.bond_old[q * SS + f];
+        if (b->j >= 0) copy_bond(&S.bond_new[(nb++) * SS + f], b);
     }
+    const int nb_old = nb;
     bool any = false;
     const int no = S.ocnt[k];
     for (int s = 0; s < no; ++s) {
         const int64_t e = oj[s * SS + k];
         const int j = ent_id(e), qj = ent_pos(e);
-        int rq = -1;
-        for (int q = 0; q < nb; ++q)
-            if (S.bond_new[(int64_t)q * SS + f].j == j) { rq = q; break; }
+        // record of j: its index among the carried-over records, found in
+        // the old list (the copy above holds the same values)
+        int rq = -1, ro = -1;
+        for (int q = 0, c2 = 0; q < bco; ++q) {
+            const int jj = S.bond_old[(int64_t)q * SS + f].j;
+            if (jj < 0) continue;
+            if (jj == j) { rq = c2; ro = q; break; }
+            ++c2;
+        }
+        if (rq < 0) {
+            // a record created earlier in this loop (step-1 bonds)
+            for (int q = nb_old; q < nb; ++q)
+                if (S.bond_new[(int64_t)q * SS + f].j == j) { rq = q; break; }
+        }
         if (rq < 0 && !bond_step) continue;
         Floe Fi = F0;
         Floe Fj = floe_at(S, qj);
@@ -622,7 +867,7 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_bond(DS S) {
         ps.damage = ps.deltanb = ps.deltatb = ps.trb_ji = ps.trb_ij = 0.0;
         ps.rb = ps.hb = ps.lb = ps.sb = ps.ib = 0.0;
         if (rq >= 0) {
-            const BondRec& b = S.bond_new[(int64_t)rq * SS + f];
+            const BondRec& b = ro >= 0 ? S.bond_old[(int64_t)ro * SS + f] : S.bond_new[(int64_t)rq * SS + f];
             ps.has_bond_rec = true; ps.bond = b.bond; ps.damage = b.damage;
             ps.deltanb = b.deltanb; ps.deltatb = b.deltatb; ps.trb_ji = b.trb_ji; ps.trb_ij = b.trb_ij;
             ps.rb = b.rb; ps.hb = b.hb; ps.lb = b.lb; ps.sb = b.sb; ps.ib = b.ib;
@@ -647,6 +892,10 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_bond(DS S) {
             bt.sb = o.sb; bt.fbn_sb = o.fbn_sb;
             S.bterm[(int64_t)rq * SS + f] = bt;
             S.evf[s * SS + k] |= EV_BONDED | ((uint32_t)(rq + 1) << 16);
+            // tell the partner where its terms are: its inner slot of f
+            const int nij = S.icnt[qj];
+            for (int q = 0; q < nij; ++q)
+                if (ent_id(S.ik[(int64_t)q * SS + qj]) == f) { S.ibond[(int64_t)q * SS + qj] = rq + 1; break; }
             S.bgat[qj] = 1;
             any = true;
         }
@@ -669,15 +918,8 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_bond_gather(DS S) {
     acc_init(a);
     const int ni = S.icnt[k];
     for (int q = 0; q < ni; ++q) {
-        const int64_t e = S.ik[q * SS + k];
-        const int i = ent_id(e), qi = ent_pos(e);
-        const int noi = S.ocnt[qi];
-        for (int s = 0; s < noi; ++s) {
-            if (ent_id(oj[s * SS + qi]) != f) continue;
-            const uint32_t fl = S.evf[s * SS + qi];
-            if (fl & EV_BONDED) add_bond_j(a, S.bterm[(int64_t)((fl >> 16) - 1) * SS + i]);
-            break;
-        }
+        const int rq = S.ibond[q * SS + k];
+        if (rq) add_bond_j(a, S.bterm[(int64_t)(rq - 1) * SS + ent_id(S.ik[q * SS + k])]);
     }
     const int no = S.ocnt[k];
     for (int s = 0; s < no; ++s) {
@@ -1192,6 +1434,8 @@ __global__ void k_totals(DS S, int64_t* out) {
 // explicit instantiations used by the host launcher
 template __global__ void k_cand<false>(DS);
 template __global__ void k_cand<true>(DS);
+template __global__ void k_cand_dense<false>(DS);
+template __global__ void k_cand_dense<true>(DS);
 template __global__ void k_contact<false>(DS);
 template __global__ void k_contact<true>(DS);
 template __global__ void k_bond<false>(DS);

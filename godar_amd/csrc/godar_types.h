@@ -30,7 +30,9 @@ struct DevParams {
     uint32_t bond_thr24;
     uint64_t bond_seed;
     int32_t wind_grid;     // -1: uniform ua/va, else index into the wind-grid table
-    int32_t pad_;
+    int32_t dense;         // 1: many floes per cell; one wavefront searches each floe
+    int32_t span;          // cells searched on each side (cell >= (max r + rtree) / span)
+    int32_t pad2_;
 };
 
 struct WindGridDesc {

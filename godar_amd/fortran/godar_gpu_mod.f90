@@ -30,7 +30,8 @@ module godar_gpu
                           GF_FBX = 7, GF_FBY = 8, GF_MB = 9, GF_FAX = 10, GF_FAY = 11, GF_MA = 12, &
                           GF_FWX = 13, GF_FWY = 14, GF_MW = 15, GF_FCORX = 16, GF_FCORY = 17, &
                           GF_FX_BC = 18, GF_FY_BC = 19, GF_M_BC = 20, GF_HSFA_MIN = 21, &
-                          GF_HSFW_MIN = 22, GF_TP = 23, GF_TAC = 28, GF_PC = 29, GF_TAB = 30, &
+                          GF_HSFW_MIN = 22, GF_TP = 23, GF_TSIGXX = 24, GF_TSIGYY = 25, &
+                          GF_TSIGXY = 26, GF_TSIGYX = 27, GF_TAC = 28, GF_PC = 29, GF_TAB = 30, &
                           GF_PB = 31, GF_TA_BC = 32, GF_P_BC = 33
     integer, parameter :: GODAR_NFORCES = 33, GODAR_NBONDSTATE = 11
 

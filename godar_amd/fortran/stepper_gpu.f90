@@ -16,12 +16,12 @@
 ! godar_sync_every = .true. to refresh after every step instead.
 !
 ! What is refreshed: x y u v theta omega r h mass inertia hfa hfw; tfx tfy m
-! fcx fcy mc fbx fby mb fax fay ma fwx fwy mw fcorx fcory fx_bc fy_bc m_bc tp;
-! bond(j,i) and damageb(j,i) of the bonded pairs (others 0); deltan(j,i) > 0
-! marks the contact pairs of the step (value r(i)+r(j)-dist at the refreshed
-! positions, at least tiny(1d0)), which is all the output routine reads of it
-! (diagnostics_outputs.f90:164).  The stress diagnostics tsig* are not
-! computed and stay 0.
+! fcx fcy mc fbx fby mb fax fay ma fwx fwy mw fcorx fcory fx_bc fy_bc m_bc tp
+! tsigxx tsigyy tsigxy tsigyx; bond(j,i) and damageb(j,i) of the bonded pairs
+! (others 0); deltan(j,i) > 0 marks the contact pairs of the step (value
+! r(i)+r(j)-dist at the refreshed positions, at least tiny(1d0)), which is all
+! the output routine reads of it (diagnostics_outputs.f90:164).  Forces and
+! stresses are those of the last step (the library records them every step).
 !=======================================================================
 subroutine stepper_gpu (tstep, restart)
 
@@ -125,6 +125,8 @@ subroutine stepper_gpu (tstep, restart)
     fcorx = fo(:, GF_FCORX); fcory = fo(:, GF_FCORY)
     fx_bc = fo(:, GF_FX_BC); fy_bc = fo(:, GF_FY_BC); m_bc = fo(:, GF_M_BC)
     tp = fo(:, GF_TP)
+    tsigxx = fo(:, GF_TSIGXX); tsigyy = fo(:, GF_TSIGYY)
+    tsigxy = fo(:, GF_TSIGXY); tsigyx = fo(:, GF_TSIGYX)
     deallocate(fo)
 
     ! bonded pairs and damage

@@ -140,6 +140,10 @@ class Context:
     def set_block(self, threads: int):
         self._chk(self.L.godar_set_block(self.h, int(threads)))
 
+    def set_dense_search(self, floes: int):
+        """Members added afterwards examining >= floes per search use one wavefront per floe."""
+        self._chk(self.L.godar_set_dense_search(self.h, int(floes)))
+
     def set_stage_block(self, stage: int, threads: int):
         self._chk(self.L.godar_set_stage_block(self.h, int(stage), int(threads)))
 

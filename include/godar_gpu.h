@@ -172,6 +172,11 @@ int godar_get_totals(godar_ctx *ctx, int64_t *totals);
  * fixed shapes).  Results do not depend on block sizes. */
 int godar_set_block(godar_ctx *ctx, int32_t threads);
 int godar_set_stage_block(godar_ctx *ctx, int32_t stage, int32_t threads);
+/* Members added afterwards whose neighbour search examines at least `floes`
+ * floes per floe (3x3 cells at their grid's mean density; default 96) are
+ * searched by one wavefront per floe instead of one thread (0: every member,
+ * a huge value: none).  Results do not depend on it. */
+int godar_set_dense_search(godar_ctx *ctx, int32_t floes);
 int godar_get_stage_blocks(const godar_ctx *ctx, int32_t *threads);   /* GODAR_NSTAGES entries */
 /* Autotuner: for each tunable stage in turn, time `trial` real steps of the
  * current members with each block size and keep the fastest.  The members

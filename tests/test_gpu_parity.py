@@ -284,6 +284,34 @@ def test_block_sizes_and_autotune_bitwise():
                 assert np.array_equal(a[k], b[k]), k
 
 
+def test_dense_search_matches_thread_search():
+    """The wavefront-per-floe neighbour search (dense members) gives the same
+    lists, sheltering and trajectories as the thread-per-floe search,
+    bitwise: a power-law field (many small floes per cell) and a sparse one,
+    each forced through both searches."""
+    from godar_amd import workloads as W
+    fa, pa = W.config4_field((3000, 77))
+    fb = g.config0_member(41, 900)
+    pb = config0_params(fb)
+    for prec in ("fp64", "mixed"):
+        runs = []
+        for thr in (0, 1 << 30):
+            ctx = _ctx(fa.n + fb.n, 2, prec)
+            ctx.set_dense_search(thr)
+            ms = [ctx.add_field(pa, fa), ctx.add_field(pb, fb)]
+            ctx.step(25)
+            runs.append(([ctx.floes(m) for m in ms], [_pairs(ctx, m, "candidate") for m in ms],
+                         [ctx.forces(m)["hsfa_min"] for m in ms]))
+            ctx.close()
+        (sa, ca, ha), (sb, cb, hb) = runs
+        for a, b in zip(sa, sb):
+            for k in STATE:
+                assert np.array_equal(a[k], b[k]), (prec, k)
+        assert ca == cb
+        for a, b in zip(ha, hb):
+            assert np.array_equal(a, b)
+
+
 def test_capacity_overflow_rolls_back():
     """A too-small candidate list aborts the step without advancing the state;
     after reserving more room the run continues exactly as an unhindered one.
