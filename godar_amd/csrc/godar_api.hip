@@ -58,6 +58,8 @@ struct godar_ctx {
     std::vector<Member> members;
     std::vector<DevParams> dparams;
     std::vector<int32_t> cell_n;        // cells owned by each member
+    // per member: local index of the floe held by storage slot first + t
+    std::vector<std::vector<int32_t>> perm;
     std::vector<WindGridDesc> wgrids;
     std::vector<double> wpool;
     int64_t wpool_dev_cap = 0;
@@ -480,6 +482,7 @@ int godar_create(int64_t max_floes, int32_t max_members, int32_t precision, goda
     c->members.assign(max_members, Member{0, 0, 1, 0, 0});
     c->dparams.assign(max_members, DevParams{});
     c->cell_n.assign(max_members, 0);
+    c->perm.assign(max_members, {});
     for (auto& m : c->members) m.active = 0;
     DS& d = c->d;
     int64_t S = c->S;
@@ -500,6 +503,8 @@ int godar_create(int64_t max_floes, int32_t max_members, int32_t precision, goda
     ACHK(dalloc(&d.pos_of, S));
     ACHK(dalloc(&d.wbc, 4 * S));
     ACHK(dalloc(&d.member_of, S));
+    ACHK(dalloc(&d.gid_of, S));
+    ACHK(dalloc(&d.sid_of, S));
     ACHK(dalloc(&d.members, max_members));
     DevParams* pp;
     ACHK(dalloc(&pp, max_members));
@@ -564,7 +569,7 @@ int godar_destroy(godar_ctx* c) {
     double* dbl[] = {d.wbc, d.acc, d.cur, d.fout};
     for (double* p : dbl) dfree(p);
     dfree(d.rec); dfree(d.geo); dfree(d.shel); dfree(d.kin); dfree(d.pos_of);
-    dfree(d.member_of); dfree(d.members);
+    dfree(d.member_of); dfree(d.gid_of); dfree(d.sid_of); dfree(d.members);
     DevParams* pp = const_cast<DevParams*>(d.params); dfree(pp);
     WindGridDesc* wg = const_cast<WindGridDesc*>(d.wgrids); dfree(wg);
     double* wp = const_cast<double*>(d.wpool); dfree(wp);
@@ -679,20 +684,44 @@ int godar_member_add(godar_ctx* c, const godar_params* p, int32_t n, const doubl
     if (rc != GODAR_OK) return rc;
     dp.cell_base = (int32_t)cbase;
     c->cell_n[m] = ncell;
+    // storage order: the member's floes sorted (stably) by their first cell,
+    // as the step's cell-ordered sweep will visit them
+    std::vector<int32_t> order(n);
+    {
+        std::vector<uint32_t> key(n);
+        auto coord = [](double v, double cell, int g) {
+            double q = std::floor(v / cell);
+            if (!(q >= 0.0)) return 0;
+            if (q >= (double)(g - 1)) return g - 1;
+            return (int)q;
+        };
+        for (int i = 0; i < n; ++i)
+            key[i] = (uint32_t)(coord(y[i], dp.cell, dp.gy) * dp.gx + coord(x[i], dp.cell, dp.gx));
+        for (int i = 0; i < n; ++i) order[i] = i;
+        std::stable_sort(order.begin(), order.end(),
+                         [&](int32_t a, int32_t b) { return key[a] < key[b]; });
+    }
     // floe state and floe_properties (physics.f90:179-197), as ini_get does
     std::vector<FloeRec> recs(n);
-    for (int i = 0; i < n; ++i) {
-        FloeRec& q = recs[i];
+    std::vector<int32_t> gid(n), sid(n);
+    for (int t = 0; t < n; ++t) {
+        const int i = order[t];
+        FloeRec& q = recs[t];
         q.x = x[i]; q.y = y[i]; q.u = u[i]; q.v = v[i]; q.omega = omega[i]; q.theta = theta[i];
         q.r = r[i]; q.h = h[i];
         q.mass = dp.rhoice * dp.pi * h[i] * (r[i] * r[i]);
         q.inertia = 0.5 * q.mass * (r[i] * r[i]);
         q.hfa = h[i] * (dp.rhowater - dp.rhoice) / dp.rhowater;
         q.hfw = h[i] * dp.rhoice / dp.rhowater;
+        gid[t] = (int32_t)(first + i);
+        sid[i] = (int32_t)(first + t);
     }
     HIPCHK(c, hipStreamSynchronize(c->stream));
     DS& d = c->d;
     HIPCHK(c, hipMemcpy(d.rec + first, recs.data(), sizeof(FloeRec) * n, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(d.gid_of + first, gid.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(d.sid_of + first, sid.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
+    c->perm[m] = order;
     for (int k = 0; k < 4; ++k) HIPCHK(c, hipMemset(d.wbc + k * c->S + first, 0, sizeof(double) * n));
     for (int k = 0; k < 2; ++k) {
         HIPCHK(c, hipMemset(c->hcnt[k] + first, 0, sizeof(int32_t) * n));
@@ -712,6 +741,252 @@ int godar_member_add(godar_ctx* c, const godar_params* p, int32_t n, const doubl
     rc = upload_tables(c);
     if (rc != GODAR_OK) return rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    *member = m;
+    return GODAR_OK;
+}
+
+// ---------------------------------------------------------------- checkpoint
+// Block layout (native byte order), everything in the caller's floe order:
+//   CkHeader | godar_params | n x 12 doubles floe state (x y u v omega r h
+//   mass inertia hfa hfw theta) | n x 4 doubles wall histories |
+//   n int32 history counts | histories (int32 partner, int32 0, 4 doubles) |
+//   n int32 bond counts | bonds (int32 partner, int32 bond, 10 doubles)
+// Partners are member-local indices.
+namespace {
+struct CkHeader {
+    char magic[8];
+    int32_t version, n, tstep, precision;
+    int64_t nhist, nbond;
+};
+constexpr int32_t CK_VERSION = 1;
+constexpr size_t CK_HIST = 8 + 4 * 8, CK_BOND = 8 + 10 * 8;
+
+struct MemberState {
+    std::vector<FloeRec> rec;        // caller's order
+    std::vector<double> wbc;         // 4 per floe
+    std::vector<int32_t> hcnt, bcnt;
+    std::vector<HistRec> hist;       // j as member-local index
+    std::vector<BondRec> bond;
+};
+
+int read_member_state(godar_ctx* c, int32_t m, MemberState& out) {
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const Member& mm = c->members[m];
+    const int n = mm.n;
+    const int64_t S = c->S;
+    const int o = c->parity;
+    const std::vector<int32_t>& pm = c->perm[m];
+    std::vector<FloeRec> rs(n);
+    HIPCHK(c, hipMemcpy(rs.data(), c->d.rec + mm.first, sizeof(FloeRec) * n, hipMemcpyDeviceToHost));
+    out.rec.assign(n, FloeRec{});
+    for (int t = 0; t < n; ++t) out.rec[pm[t]] = rs[t];
+    out.wbc.assign((size_t)4 * n, 0.0);
+    std::vector<double> row(n);
+    for (int k = 0; k < 4; ++k) {
+        HIPCHK(c, hipMemcpy(row.data(), c->d.wbc + k * S + mm.first, sizeof(double) * n, hipMemcpyDeviceToHost));
+        for (int t = 0; t < n; ++t) out.wbc[(size_t)4 * pm[t] + k] = row[t];
+    }
+    std::vector<int32_t> hc(n), bc(n);
+    HIPCHK(c, hipMemcpy(hc.data(), c->hcnt[o] + mm.first, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(bc.data(), c->bcnt[o] + mm.first, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+    std::vector<std::vector<HistRec>> hl(n);
+    std::vector<std::vector<BondRec>> bl(n);
+    std::vector<HistRec> hrow(n);
+    for (int q = 0; q < c->HC; ++q) {
+        bool any = false;
+        for (int t = 0; t < n; ++t) any |= q < hc[t];
+        if (!any) break;
+        HIPCHK(c, hipMemcpy(hrow.data(), c->hist[o] + q * S + mm.first, sizeof(HistRec) * n, hipMemcpyDeviceToHost));
+        for (int t = 0; t < n; ++t)
+            if (q < hc[t] && hrow[t].j >= 0) {
+                HistRec h = hrow[t];
+                h.j -= mm.first;
+                hl[pm[t]].push_back(h);
+            }
+    }
+    std::vector<BondRec> brow(n);
+    for (int q = 0; q < c->BC; ++q) {
+        bool any = false;
+        for (int t = 0; t < n; ++t) any |= q < bc[t];
+        if (!any) break;
+        HIPCHK(c, hipMemcpy(brow.data(), c->bond[o] + q * S + mm.first, sizeof(BondRec) * n, hipMemcpyDeviceToHost));
+        for (int t = 0; t < n; ++t)
+            if (q < bc[t] && brow[t].j >= 0) {
+                BondRec b = brow[t];
+                b.j -= mm.first;
+                bl[pm[t]].push_back(b);
+            }
+    }
+    out.hcnt.assign(n, 0);
+    out.bcnt.assign(n, 0);
+    out.hist.clear();
+    out.bond.clear();
+    for (int i = 0; i < n; ++i) {
+        out.hcnt[i] = (int32_t)hl[i].size();
+        out.bcnt[i] = (int32_t)bl[i].size();
+        out.hist.insert(out.hist.end(), hl[i].begin(), hl[i].end());
+        out.bond.insert(out.bond.end(), bl[i].begin(), bl[i].end());
+    }
+    return GODAR_OK;
+}
+}  // namespace
+
+int godar_checkpoint_size(godar_ctx* c, int32_t m, int64_t* bytes) {
+    if (!valid_member(c, m) || !bytes) return fail(c, GODAR_ERR_ARG, "checkpoint_size: bad argument");
+    MemberState st;
+    int rc = read_member_state(c, m, st);
+    if (rc != GODAR_OK) return rc;
+    const int64_t n = c->members[m].n;
+    *bytes = (int64_t)sizeof(CkHeader) + (int64_t)sizeof(godar_params) + n * 12 * 8 + n * 4 * 8 +
+             n * 4 + (int64_t)st.hist.size() * (int64_t)CK_HIST + n * 4 +
+             (int64_t)st.bond.size() * (int64_t)CK_BOND;
+    return GODAR_OK;
+}
+
+int godar_checkpoint_save(godar_ctx* c, int32_t m, const godar_params* p, void* buf, int64_t cap) {
+    if (!valid_member(c, m) || !p || !buf) return fail(c, GODAR_ERR_ARG, "checkpoint_save: bad argument");
+    int rc = download_members(c);
+    if (rc != GODAR_OK) return rc;
+    MemberState st;
+    rc = read_member_state(c, m, st);
+    if (rc != GODAR_OK) return rc;
+    const int n = c->members[m].n;
+    int64_t need = 0;
+    rc = godar_checkpoint_size(c, m, &need);
+    if (rc != GODAR_OK) return rc;
+    if (cap < need) return fail(c, GODAR_ERR_ARG, "checkpoint_save: buffer too small");
+    char* w = static_cast<char*>(buf);
+    CkHeader h{};
+    std::memcpy(h.magic, "GODARCK1", 8);
+    h.version = CK_VERSION; h.n = n; h.tstep = c->members[m].tstep; h.precision = c->precision;
+    h.nhist = (int64_t)st.hist.size(); h.nbond = (int64_t)st.bond.size();
+    std::memcpy(w, &h, sizeof(h)); w += sizeof(h);
+    std::memcpy(w, p, sizeof(godar_params)); w += sizeof(godar_params);
+    for (int i = 0; i < n; ++i) {
+        const FloeRec& q = st.rec[i];
+        double v[12] = {q.x, q.y, q.u, q.v, q.omega, q.r, q.h, q.mass, q.inertia, q.hfa, q.hfw, q.theta};
+        std::memcpy(w, v, sizeof(v)); w += sizeof(v);
+    }
+    std::memcpy(w, st.wbc.data(), sizeof(double) * 4 * n); w += sizeof(double) * 4 * n;
+    std::memcpy(w, st.hcnt.data(), sizeof(int32_t) * n); w += sizeof(int32_t) * n;
+    for (const HistRec& r : st.hist) {
+        int32_t ids[2] = {r.j, 0};
+        double v[4] = {r.deltat, r.thetarelc, r.fcn, r.fct};
+        std::memcpy(w, ids, 8); w += 8;
+        std::memcpy(w, v, sizeof(v)); w += sizeof(v);
+    }
+    std::memcpy(w, st.bcnt.data(), sizeof(int32_t) * n); w += sizeof(int32_t) * n;
+    for (const BondRec& r : st.bond) {
+        int32_t ids[2] = {r.j, r.bond};
+        double v[10] = {r.damage, r.deltanb, r.deltatb, r.trb_ji, r.trb_ij, r.rb, r.hb, r.lb, r.sb, r.ib};
+        std::memcpy(w, ids, 8); w += 8;
+        std::memcpy(w, v, sizeof(v)); w += sizeof(v);
+    }
+    return GODAR_OK;
+}
+
+int godar_member_restore(godar_ctx* c, const void* buf, int64_t size, int32_t* member) {
+    if (!c || !buf || !member || size < (int64_t)(sizeof(CkHeader) + sizeof(godar_params)))
+        return fail(c, GODAR_ERR_ARG, "member_restore: bad argument");
+    const char* r = static_cast<const char*>(buf);
+    const char* end = r + size;
+    CkHeader h;
+    std::memcpy(&h, r, sizeof(h)); r += sizeof(h);
+    if (std::memcmp(h.magic, "GODARCK1", 8) != 0 || h.version != CK_VERSION || h.n <= 0 ||
+        h.nhist < 0 || h.nbond < 0)
+        return fail(c, GODAR_ERR_ARG, "member_restore: not a checkpoint block");
+    const int n = h.n;
+    const int64_t need = (int64_t)sizeof(CkHeader) + (int64_t)sizeof(godar_params) + (int64_t)n * 12 * 8 +
+                         (int64_t)n * 4 * 8 + (int64_t)n * 4 + h.nhist * (int64_t)CK_HIST +
+                         (int64_t)n * 4 + h.nbond * (int64_t)CK_BOND;
+    if (size < need) return fail(c, GODAR_ERR_ARG, "member_restore: block truncated");
+    godar_params p;
+    std::memcpy(&p, r, sizeof(p)); r += sizeof(p);
+    std::vector<double> x(n), y(n), u(n), v(n), om(n), rr(n), hh(n), th(n);
+    for (int i = 0; i < n; ++i) {
+        double q[12];
+        std::memcpy(q, r, sizeof(q)); r += sizeof(q);
+        x[i] = q[0]; y[i] = q[1]; u[i] = q[2]; v[i] = q[3]; om[i] = q[4]; rr[i] = q[5]; hh[i] = q[6];
+        th[i] = q[11];
+    }
+    std::vector<double> wbc((size_t)4 * n);
+    std::memcpy(wbc.data(), r, sizeof(double) * 4 * n); r += sizeof(double) * 4 * n;
+    std::vector<int32_t> hc(n), bc(n);
+    std::memcpy(hc.data(), r, sizeof(int32_t) * n); r += sizeof(int32_t) * n;
+    const char* hist_at = r;
+    r += h.nhist * (int64_t)CK_HIST;
+    std::memcpy(bc.data(), r, sizeof(int32_t) * n); r += sizeof(int32_t) * n;
+    const char* bond_at = r;
+    (void)end;
+    int maxh = 0, maxb = 0;
+    int64_t sh = 0, sb = 0;
+    for (int i = 0; i < n; ++i) {
+        if (hc[i] < 0 || bc[i] < 0) return fail(c, GODAR_ERR_ARG, "member_restore: bad counts");
+        maxh = std::max(maxh, hc[i]); maxb = std::max(maxb, bc[i]);
+        sh += hc[i]; sb += bc[i];
+    }
+    if (sh != h.nhist || sb != h.nbond) return fail(c, GODAR_ERR_ARG, "member_restore: bad counts");
+    if (maxh > c->HC || maxb > c->BC) {
+        int rc = godar_reserve(c, 0, 0, std::max(maxh, c->HC), std::max(maxb, c->BC));
+        if (rc != GODAR_OK) return rc;
+    }
+    int32_t m = -1;
+    int rc = godar_member_add(c, &p, n, x.data(), y.data(), u.data(), v.data(), th.data(), om.data(),
+                              rr.data(), hh.data(), h.tstep, &m);
+    if (rc != GODAR_OK) return rc;
+    // the floe properties that ridging changed are the recomputed ones
+    // (floe_properties of r and h); the pair and wall state goes to storage
+    const Member& mm = c->members[m];
+    const std::vector<int32_t>& pm = c->perm[m];
+    const int64_t S = c->S;
+    const int o = c->parity;
+    std::vector<int32_t> slot(n);
+    for (int t = 0; t < n; ++t) slot[pm[t]] = t;
+    std::vector<double> row(n);
+    for (int k = 0; k < 4; ++k) {
+        for (int i = 0; i < n; ++i) row[slot[i]] = wbc[(size_t)4 * i + k];
+        HIPCHK(c, hipMemcpy(c->d.wbc + k * S + mm.first, row.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    }
+    std::vector<int32_t> cnt(n);
+    for (int i = 0; i < n; ++i) cnt[slot[i]] = hc[i];
+    HIPCHK(c, hipMemcpy(c->hcnt[o] + mm.first, cnt.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
+    for (int i = 0; i < n; ++i) cnt[slot[i]] = bc[i];
+    HIPCHK(c, hipMemcpy(c->bcnt[o] + mm.first, cnt.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
+    {
+        std::vector<HistRec> rows((size_t)std::max(maxh, 1) * n);
+        const char* q = hist_at;
+        for (int i = 0; i < n; ++i)
+            for (int k = 0; k < hc[i]; ++k) {
+                int32_t ids[2];
+                double vv[4];
+                std::memcpy(ids, q, 8); q += 8;
+                std::memcpy(vv, q, sizeof(vv)); q += sizeof(vv);
+                HistRec& hr = rows[(size_t)k * n + slot[i]];
+                hr.j = mm.first + ids[0]; hr.pad_ = 0;
+                hr.deltat = vv[0]; hr.thetarelc = vv[1]; hr.fcn = vv[2]; hr.fct = vv[3];
+            }
+        for (int k = 0; k < maxh; ++k)
+            HIPCHK(c, hipMemcpy(c->hist[o] + k * S + mm.first, rows.data() + (size_t)k * n,
+                                sizeof(HistRec) * n, hipMemcpyHostToDevice));
+    }
+    {
+        std::vector<BondRec> rows((size_t)std::max(maxb, 1) * n);
+        const char* q = bond_at;
+        for (int i = 0; i < n; ++i)
+            for (int k = 0; k < bc[i]; ++k) {
+                int32_t ids[2];
+                double vv[10];
+                std::memcpy(ids, q, 8); q += 8;
+                std::memcpy(vv, q, sizeof(vv)); q += sizeof(vv);
+                BondRec& b = rows[(size_t)k * n + slot[i]];
+                b.j = mm.first + ids[0]; b.bond = ids[1];
+                b.damage = vv[0]; b.deltanb = vv[1]; b.deltatb = vv[2]; b.trb_ji = vv[3]; b.trb_ij = vv[4];
+                b.rb = vv[5]; b.hb = vv[6]; b.lb = vv[7]; b.sb = vv[8]; b.ib = vv[9];
+            }
+        for (int k = 0; k < maxb; ++k)
+            HIPCHK(c, hipMemcpy(c->bond[o] + k * S + mm.first, rows.data() + (size_t)k * n,
+                                sizeof(BondRec) * n, hipMemcpyHostToDevice));
+    }
     *member = m;
     return GODAR_OK;
 }
@@ -788,17 +1063,19 @@ int godar_set_bonds(godar_ctx* c, int32_t m, int32_t nb, const int32_t* bi, cons
     int64_t S = c->S;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     int o = c->parity;
-    std::vector<int32_t> cnt(mm.n, 0);
+    std::vector<int32_t> cnt(mm.n, 0), slot_of(mm.n);
+    for (int t = 0; t < mm.n; ++t) slot_of[c->perm[m][t]] = t;   // caller's index -> storage offset
     std::vector<BondRec> recs((int64_t)mm.n * c->BC);
     for (int k = 0; k < nb; ++k) {
         int i = bi[k], j = bj[k];
         if (i < 0 || j <= i || j >= mm.n) return fail(c, GODAR_ERR_ARG, "set_bonds: need 0 <= i < j < n");
-        if (cnt[i] >= c->BC) return fail(c, GODAR_ERR_CAPACITY, "set_bonds: bond list full, raise it with godar_reserve");
-        BondRec& b = recs[(int64_t)cnt[i] * mm.n + i];
+        const int t = slot_of[i];
+        if (cnt[t] >= c->BC) return fail(c, GODAR_ERR_CAPACITY, "set_bonds: bond list full, raise it with godar_reserve");
+        BondRec& b = recs[(int64_t)cnt[t] * mm.n + t];
         const double* s = st + 11 * k;
         b.j = mm.first + j; b.bond = (int32_t)s[0]; b.damage = s[1]; b.deltanb = s[2]; b.deltatb = s[3];
         b.trb_ji = s[4]; b.trb_ij = s[5]; b.rb = s[6]; b.hb = s[7]; b.lb = s[8]; b.sb = s[9]; b.ib = s[10];
-        cnt[i]++;
+        cnt[t]++;
     }
     for (int q = 0; q < c->BC; ++q)
         HIPCHK(c, hipMemcpy(c->bond[o] + q * S + mm.first, recs.data() + (int64_t)q * mm.n,
@@ -1038,11 +1315,12 @@ int godar_get_floes(godar_ctx* c, int32_t m, double* x, double* y, double* u, do
     std::vector<FloeRec> recs(mm.n);
     HIPCHK(c, hipMemcpy(recs.data(), c->d.rec + mm.first, sizeof(FloeRec) * mm.n, hipMemcpyDeviceToHost));
     double* dst[] = {x, y, u, v, theta, omega, r, h, mass, inertia, hfa, hfw};
-    for (int i = 0; i < mm.n; ++i) {
-        const FloeRec& q = recs[i];
+    const std::vector<int32_t>& pm = c->perm[m];   // storage offset -> caller's index
+    for (int t = 0; t < mm.n; ++t) {
+        const FloeRec& q = recs[t];
         double val[12] = {q.x, q.y, q.u, q.v, q.theta, q.omega, q.r, q.h, q.mass, q.inertia, q.hfa, q.hfw};
         for (int k = 0; k < 12; ++k)
-            if (dst[k]) dst[k][i] = val[k];
+            if (dst[k]) dst[k][pm[t]] = val[k];
     }
     return GODAR_OK;
 }
@@ -1051,9 +1329,13 @@ int godar_get_forces(godar_ctx* c, int32_t m, double* f) {
     if (!valid_member(c, m) || !f) return fail(c, GODAR_ERR_ARG, "get_forces: bad argument");
     HIPCHK(c, hipStreamSynchronize(c->stream));
     const Member& mm = c->members[m];
-    for (int k = 0; k < 33; ++k)
-        HIPCHK(c, hipMemcpy(f + (int64_t)k * mm.n, c->d.fout + k * c->S + mm.first, sizeof(double) * mm.n,
+    const std::vector<int32_t>& pm = c->perm[m];
+    std::vector<double> row(mm.n);
+    for (int k = 0; k < 33; ++k) {
+        HIPCHK(c, hipMemcpy(row.data(), c->d.fout + k * c->S + mm.first, sizeof(double) * mm.n,
                             hipMemcpyDeviceToHost));
+        for (int t = 0; t < mm.n; ++t) f[(int64_t)k * mm.n + pm[t]] = row[t];
+    }
     return GODAR_OK;
 }
 
@@ -1061,9 +1343,13 @@ int godar_get_walls(godar_ctx* c, int32_t m, double* w) {
     if (!valid_member(c, m) || !w) return fail(c, GODAR_ERR_ARG, "get_walls: bad argument");
     HIPCHK(c, hipStreamSynchronize(c->stream));
     const Member& mm = c->members[m];
-    for (int k = 0; k < 4; ++k)
-        HIPCHK(c, hipMemcpy(w + (int64_t)k * mm.n, c->d.wbc + k * c->S + mm.first, sizeof(double) * mm.n,
+    const std::vector<int32_t>& pm = c->perm[m];
+    std::vector<double> row(mm.n);
+    for (int k = 0; k < 4; ++k) {
+        HIPCHK(c, hipMemcpy(row.data(), c->d.wbc + k * c->S + mm.first, sizeof(double) * mm.n,
                             hipMemcpyDeviceToHost));
+        for (int t = 0; t < mm.n; ++t) w[(int64_t)k * mm.n + pm[t]] = row[t];
+    }
     return GODAR_OK;
 }
 
@@ -1100,7 +1386,7 @@ int godar_get_pairs(godar_ctx* c, int32_t m, int32_t kind, int32_t cap, int32_t*
                 uint32_t fl = ef[(int64_t)s * len + r0];
                 if (fl & EV_REMOVED) continue;
                 if (kind == GODAR_PAIRS_CONTACT && !(fl & EV_CONTACT)) continue;
-                out.push_back({i, (int32_t)(oj[(int64_t)s * len + r0] >> 32) - mm.first});
+                out.push_back({c->perm[m][i], (int32_t)(oj[(int64_t)s * len + r0] >> 32) - mm.first});
             }
         }
     } else if (kind == GODAR_PAIRS_BOND) {
@@ -1112,7 +1398,8 @@ int godar_get_pairs(godar_ctx* c, int32_t m, int32_t kind, int32_t cap, int32_t*
             HIPCHK(c, hipMemcpy(rec.data(), c->bond[o] + q * S + mm.first, sizeof(BondRec) * mm.n,
                                 hipMemcpyDeviceToHost));
             for (int i = 0; i < mm.n; ++i)
-                if (q < cnt[i] && rec[i].j >= 0 && rec[i].bond == 1) out.push_back({i, rec[i].j - mm.first});
+                if (q < cnt[i] && rec[i].j >= 0 && rec[i].bond == 1)
+                    out.push_back({c->perm[m][i], rec[i].j - mm.first});
         }
     } else {
         return fail(c, GODAR_ERR_ARG, "get_pairs: bad kind");
@@ -1139,7 +1426,7 @@ int godar_get_bond_state(godar_ctx* c, int32_t m, int32_t cap, double* st, int32
                             hipMemcpyDeviceToHost));
         for (int i = 0; i < mm.n; ++i)
             if (q < cnt[i] && rec[i].j >= 0 && rec[i].bond == 1)
-                out.push_back({{i, rec[i].j - mm.first}, rec[i]});
+                out.push_back({{c->perm[m][i], rec[i].j - mm.first}, rec[i]});
     }
     std::sort(out.begin(), out.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
     *count = (int32_t)out.size();

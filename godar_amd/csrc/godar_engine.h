@@ -34,14 +34,14 @@ struct FloeRec {
 //   KinRec  - contact, bond, wall and forcing terms, integration (56 B)
 struct GeoRec {
     double x, y, r;
-    int32_t id;      // floe id (global index of the persistent arrays)
+    int32_t id;      // floe identity (member first + the caller's index)
     int32_t member;  // -1: free slot
 };
 struct ShelRec { double hfa, hfw, wx, wy; };
 struct KinRec { double u, v, omega, h, mass, inertia, theta; };
 
-// Everything the kernels touch.  "id" arrays are indexed by floe id; "pos"
-// arrays by the floe's position in this step's cell order.  Per-floe lists
+// Everything the kernels touch.  "id" arrays are indexed by a floe's storage
+// slot; "pos" arrays by the floe's position in this step's cell order.  Per-floe lists
 // are slot-major, entry s of row r at [s*S + r].
 struct DS {
     int64_t S;                 // floe capacity = stride of slot-major arrays
@@ -54,6 +54,13 @@ struct DS {
     FloeRec* rec;
     double* wbc;               // [4][S] deltat_bc1, deltat_bc2, theta_bc1, theta_bc2
     int32_t* member_of;        // [S], -1 = free slot
+    // A floe's identity (member first + its index in the caller's arrays)
+    // orders the reference's sequential loop and names it in pair records;
+    // its storage slot indexes the persistent arrays.  Members store their
+    // floes in the cell order of their first step, so that a step's
+    // cell-ordered sweep reads and writes storage nearly sequentially.
+    int32_t* gid_of;           // [S] storage slot -> identity
+    int32_t* sid_of;           // [S] identity -> storage slot
     Member* members;
     const DevParams* params;
     const WindGridDesc* wgrids;
@@ -64,7 +71,7 @@ struct DS {
     GeoRec* geo;               // [S] pos
     ShelRec* shel;             // [S] pos (wind at the floe this step)
     KinRec* kin;               // [S] pos
-    int32_t* pos_of;           // [S] id -> pos
+    int32_t* pos_of;           // [S] storage slot -> pos
     // candidate lists (pos rows; entries are partner positions)
     int32_t* ocnt;
     int64_t* oj;

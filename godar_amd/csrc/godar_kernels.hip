@@ -77,7 +77,7 @@ __device__ __forceinline__ Floe floe_of(const FloeRec& s) {
     return F;
 }
 
-// floe by id with the properties current during the ordered repair
+// floe by storage slot with the properties current during the ordered repair
 __device__ __forceinline__ Floe load_floe_cur(const DS& S, int f) {
     Floe F = floe_of(S.rec[f]);
     if (S.dirty[f]) {
@@ -94,7 +94,7 @@ __device__ __forceinline__ int cell_coord(double p, double cell, int g) {
     return (int)q;
 }
 
-// Old pair state of (i, j), owner i (ids).
+// Old pair state of (i, j): owner i by storage slot, partner j by identity.
 __device__ __forceinline__ void load_pair_state(const DS& S, int i, int j, PairState& ps) {
     ps.deltat = 0.0; ps.thetarelc = 0.0; ps.fcn = 0.0; ps.fct = 0.0;
     ps.bond = 0; ps.has_bond_rec = false;
@@ -243,7 +243,7 @@ __global__ void k_ranges(DS S) {
     int m = key < (uint32_t)S.ncells ? S.member_of[f] : -1;
     const FloeRec fr = S.rec[f];
     GeoRec g;
-    g.x = fr.x; g.y = fr.y; g.r = fr.r; g.id = f; g.member = m;
+    g.x = fr.x; g.y = fr.y; g.r = fr.r; g.id = S.gid_of[f]; g.member = m;
     S.geo[k] = g;
     KinRec q;
     q.u = fr.u; q.v = fr.v; q.omega = fr.omega; q.h = fr.h; q.mass = fr.mass; q.inertia = fr.inertia;
@@ -292,9 +292,10 @@ __device__ __forceinline__ void enqueue(bool want, int32_t* q, int32_t* cnt, int
 //    it crosses the box.
 // ---------------------------------------------------------------------
 // Finishing writes of one floe's search (any search shape): list counts,
-// per-slot resets, sheltering minima and the per-step resets of its rows.
+// per-slot resets, sheltering minima and the per-step resets of its rows
+// (fs: the floe's storage slot).
 template <bool MIXED>
-__device__ __forceinline__ void cand_finish(const DS& S, int k, int f, const DevParams& P, int no,
+__device__ __forceinline__ void cand_finish(const DS& S, int k, int fs, const DevParams& P, int no,
                                             int ni, double sha, double shw, bool qc, bool qb,
                                             int lane0, int lanes) {
     const int64_t SS = S.S;
@@ -313,8 +314,8 @@ __device__ __forceinline__ void cand_finish(const DS& S, int k, int f, const Dev
         S.icnt[k] = ni;
         S.acc[A_SHA * SS + k] = sha;
         S.acc[A_SHW * SS + k] = shw;
-        if (!qc) S.hcnt_new[f] = 0;
-        if (!qb) S.bcnt_new[f] = 0;
+        if (!qc) S.hcnt_new[fs] = 0;
+        if (!qb) S.bcnt_new[fs] = 0;
         S.ovw[k] = -1;
         S.wflag[k] = 0;
         S.bgat[k] = 0;
@@ -339,8 +340,9 @@ __device__ __forceinline__ void cand_thread(const DS& S, int k, const GeoRec& me
     const ShelRec msh = S.shel[k];
     const T uwi = T(P.uw), vwi = T(P.vw);
     T sha = T(1), shw = T(1);
-    qc = S.hcnt_old[f] > 0;
-    qb = S.bcnt_old[f] > 0;
+    const int fs = S.cval[k];
+    qc = S.hcnt_old[fs] > 0;
+    qb = S.bcnt_old[fs] > 0;
     qw = me.y - me.r < 0 || me.y + me.r > P.ny || me.x - me.r < 0 || me.x + me.r > P.nx;
     int no = 0, ni = 0;
     uint32_t over = 0;
@@ -358,72 +360,94 @@ __device__ __forceinline__ void cand_thread(const DS& S, int k, const GeoRec& me
             const int2 rg = S.crange[P.cell_base + b * P.gx + a];
             if (rg.y > rg.x) { lo = min(lo, rg.x); hi = max(hi, rg.y); }
         }
-        {
-            for (int q = lo; q < hi; ++q) {
-                if (q == k) continue;
-                const GeoRec* o = &S.geo[q];
-                double dx = me.x - o->x, dy = me.y - o->y;
-                double d = sqrt(dx * dx + dy * dy);
-                int j = o->id;
-                if (j > f) {
-                    if (!(d <= rad)) continue;
-                    if (no >= S.CO) { over |= ERR_OUTER_CAP; continue; }
-                    int p = no;
-                    while (p > 0) {
-                        double dp = S.od[(p - 1) * SS + k];
-                        int64_t ep = oj[(p - 1) * SS + k];
-                        if (dp < d || (dp == d && ent_id(ep) < j)) break;
-                        S.od[p * SS + k] = dp;
-                        oj[p * SS + k] = ep;
-                        --p;
-                    }
-                    S.od[p * SS + k] = d;
-                    oj[p * SS + k] = ent(j, q);
-                    ++no;
-                    // pair (f, j): f is the outer floe i of the event
-                    const double deltan = me.r + o->r - d;
-                    if (deltan > 0) qc = true;
-                    if (!qb && bond_step && deltan >= -P.bond_lim &&
-                        bond_keep(f - first, j - first, P.bond_seed, P.bond_thr24))
-                        qb = true;
-                    if (P.shelter) {
-                        // floe f takes hsf*(j,f), evaluated with -cosa, -sina
-                        const double cosa = (o->x - me.x) / d, sina = (o->y - me.y) / d;
-                        const ShelRec oh = S.shel[q];
-                        const T dn = T(deltan), c = T(cosa), sn = T(sina);
-                        sha = dmin(sha, S_shelter(T(oh.hfa), T(msh.hfa), dn, -c, -sn, T(msh.wx), T(msh.wy)));
-                        shw = dmin(shw, S_shelter(T(oh.hfw), T(msh.hfw), dn, -c, -sn, uwi, vwi));
-                    }
-                } else {
-                    if (!(d <= o->r + P.rtree)) continue;
-                    if (ni >= S.CI) { over |= ERR_INNER_CAP; continue; }
-                    int p = ni;
-                    const int64_t e = ent(j, q);
-                    while (p > 0) {
-                        int64_t ep = ik[(p - 1) * SS + k];
-                        if (ep < e) break;
-                        ik[p * SS + k] = ep;
-                        --p;
-                    }
-                    ik[p * SS + k] = e;
-                    ++ni;
-                    // pair (j, f): f is the inner floe j of the event
-                    const double deltan = o->r + me.r - d;
-                    if (deltan > 0) qc = true;
-                    if (P.shelter) {
-                        // cosa/sina point from j to f; floe f takes hsf*(j,f) with j's wind
-                        const double cosa = (me.x - o->x) / d, sina = (me.y - o->y) / d;
-                        const ShelRec oh = S.shel[q];
-                        const T dn = T(deltan), c = T(cosa), sn = T(sina);
-                        sha = dmin(sha, S_shelter(T(oh.hfa), T(msh.hfa), dn, c, sn, T(oh.wx), T(oh.wy)));
-                        shw = dmin(shw, S_shelter(T(oh.hfw), T(msh.hfw), dn, c, sn, uwi, vwi));
-                    }
+        // the next row entry is loaded while this one is tested
+        GeoRec nxt;
+        if (lo < hi) nxt = S.geo[lo];
+        for (int q = lo; q < hi; ++q) {
+            const GeoRec cur = nxt;
+            if (q + 1 < hi) nxt = S.geo[q + 1];
+            if (q == k) continue;
+            const GeoRec* o = &cur;
+            const double dx = me.x - o->x, dy = me.y - o->y;
+            const double dist2 = dx * dx + dy * dy;
+            const int j = o->id;
+            const double thr = j > f ? rad : o->r + P.rtree;
+            // far beyond the radius: no square root needed (sqrt is
+            // correctly rounded, so it cannot cross a margin of 1e-12)
+            if (dist2 > thr * thr * (1.0 + 1e-12)) continue;
+            const double d = sqrt(dist2);
+            if (!(d <= thr)) continue;
+            if (j > f) {
+                if (no >= S.CO) { over |= ERR_OUTER_CAP; continue; }
+                int p = no;
+                while (p > 0) {
+                    double dp = S.od[(p - 1) * SS + k];
+                    int64_t ep = oj[(p - 1) * SS + k];
+                    if (dp < d || (dp == d && ent_id(ep) < j)) break;
+                    S.od[p * SS + k] = dp;
+                    oj[p * SS + k] = ep;
+                    --p;
                 }
+                S.od[p * SS + k] = d;
+                oj[p * SS + k] = ent(j, q);
+                ++no;
+            } else {
+                if (ni >= S.CI) { over |= ERR_INNER_CAP; continue; }
+                int p = ni;
+                const int64_t e = ent(j, q);
+                while (p > 0) {
+                    int64_t ep = ik[(p - 1) * SS + k];
+                    if (ep < e) break;
+                    ik[p * SS + k] = ep;
+                    --p;
+                }
+                ik[p * SS + k] = e;
+                ++ni;
             }
         }
     }
+    // sheltering and classification over the lists just built (separate
+    // from the scan, so that a wavefront's lanes only diverge over their own
+    // few candidates)
+    for (int s2 = 0; s2 < no; ++s2) {
+        const int64_t e = oj[s2 * SS + k];
+        const int j = ent_id(e), q = ent_pos(e);
+        const double d = S.od[s2 * SS + k];
+        const GeoRec* o = &S.geo[q];
+        // pair (f, j): f is the outer floe i of the event
+        const double deltan = me.r + o->r - d;
+        if (deltan > 0) qc = true;
+        if (!qb && bond_step && deltan >= -P.bond_lim &&
+            bond_keep(f - first, j - first, P.bond_seed, P.bond_thr24))
+            qb = true;
+        if (P.shelter) {
+            // floe f takes hsf*(j,f), evaluated with -cosa, -sina
+            const double cosa = (o->x - me.x) / d, sina = (o->y - me.y) / d;
+            const ShelRec oh = S.shel[q];
+            const T dn = T(deltan), c = T(cosa), sn = T(sina);
+            sha = dmin(sha, S_shelter(T(oh.hfa), T(msh.hfa), dn, -c, -sn, T(msh.wx), T(msh.wy)));
+            shw = dmin(shw, S_shelter(T(oh.hfw), T(msh.hfw), dn, -c, -sn, uwi, vwi));
+        }
+    }
+    for (int s2 = 0; s2 < ni; ++s2) {
+        const int q = ent_pos(ik[s2 * SS + k]);
+        const GeoRec* o = &S.geo[q];
+        const double dx = me.x - o->x, dy = me.y - o->y;
+        const double d = sqrt(dx * dx + dy * dy);
+        // pair (j, f): f is the inner floe j of the event
+        const double deltan = o->r + me.r - d;
+        if (deltan > 0) qc = true;
+        if (P.shelter) {
+            // cosa/sina point from j to f; floe f takes hsf*(j,f) with j's wind
+            const double cosa = (me.x - o->x) / d, sina = (me.y - o->y) / d;
+            const ShelRec oh = S.shel[q];
+            const T dn = T(deltan), c = T(cosa), sn = T(sina);
+            sha = dmin(sha, S_shelter(T(oh.hfa), T(msh.hfa), dn, c, sn, T(oh.wx), T(oh.wy)));
+            shw = dmin(shw, S_shelter(T(oh.hfw), T(msh.hfw), dn, c, sn, uwi, vwi));
+        }
+    }
     if (over) set_err(S, over);
-    cand_finish<MIXED>(S, k, f, P, no, ni, double(sha), double(shw), qc, qb, 0, 1);
+    cand_finish<MIXED>(S, k, fs, P, no, ni, double(sha), double(shw), qc, qb, 0, 1);
 }
 
 template <bool MIXED>
@@ -550,10 +574,15 @@ __global__ __launch_bounds__(64) void k_cand_dense(DS S) {
                 if (q < hi && q != k) {
                     const GeoRec o = S.geo[q];
                     const double dx = me.x - o.x, dy = me.y - o.y;
-                    d = sqrt(dx * dx + dy * dy);
+                    const double dist2 = dx * dx + dy * dy;
                     oid = o.id;
-                    if (o.id > f) vout = d <= rad;
-                    else vin = d <= o.r + P.rtree;
+                    const double thr = o.id > f ? rad : o.r + P.rtree;
+                    // far beyond the radius: no square root needed
+                    if (dist2 <= thr * thr * (1.0 + 1e-12)) {
+                        d = sqrt(dist2);
+                        if (o.id > f) vout = d <= thr;
+                        else vin = d <= thr;
+                    }
                 }
                 const uint64_t mo = __ballot(vout), mi = __ballot(vin);
                 if (vout) {
@@ -643,10 +672,11 @@ __global__ __launch_bounds__(64) void k_cand_dense(DS S) {
         }
         sha = wave_min(sha);
         shw = wave_min(shw);
-        const bool qc = __ballot(lqc) != 0 || S.hcnt_old[f] > 0;
-        const bool qb = __ballot(lqb) != 0 || S.bcnt_old[f] > 0;
+        const int fs = S.cval[k];
+        const bool qc = __ballot(lqc) != 0 || S.hcnt_old[fs] > 0;
+        const bool qb = __ballot(lqb) != 0 || S.bcnt_old[fs] > 0;
         const bool qw = me.y - me.r < 0 || me.y + me.r > P.ny || me.x - me.r < 0 || me.x + me.r > P.nx;
-        cand_finish<MIXED>(S, k, f, P, no, ni, double(sha), double(shw), qc, qb, lane, 64);
+        cand_finish<MIXED>(S, k, fs, P, no, ni, double(sha), double(shw), qc, qb, lane, 64);
         if (lane == 0) {
             if (qc) S.qlist[atomicAdd(&S.qn[0], 1)] = k;
             if (qb) S.qlist[SS + atomicAdd(&S.qn[1], 1)] = k;
@@ -659,8 +689,8 @@ __global__ __launch_bounds__(64) void k_cand_dense(DS S) {
 // ---------------------------------------------------------------------
 // 3. speculative events + per-floe gather + owner state writes
 // ---------------------------------------------------------------------
-// Contact state of pair (i, j), owner i: history, last contact forces and
-// whether it is bonded.
+// Contact state of pair (i, j), owner i by storage slot, partner j by
+// identity: history, last contact forces and whether it is bonded.
 __device__ __forceinline__ void load_contact_state(const DS& S, int i, int j, PairState& ps) {
     ps.deltat = 0.0; ps.thetarelc = 0.0; ps.fcn = 0.0; ps.fct = 0.0;
     ps.bond = 0; ps.has_bond_rec = false;
@@ -693,7 +723,8 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_contact(DS S) {
     const DevParams& P = S.params[m];
     const int tstep = S.members[m].tstep;
     const int first = S.members[m].first;
-    const int f = me.id;
+    const int f = me.id;            // identity (ordering, bond mask)
+    const int fs = S.cval[k];       // storage slot of the floe's pair state
     const int64_t SS = S.S;
     const int64_t* oj = S.oj;
     const int64_t* ik = S.ik;
@@ -713,7 +744,7 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_contact(DS S) {
         kinematics(Fi, Ff, kn);
         if (!(kn.deltan > 0)) continue;
         PairState ps;
-        load_contact_state(S, i, f, ps);
+        load_contact_state(S, S.cval[qi], f, ps);
         bond_create(P, tstep, i - first, f - first, Fi, Ff, kn, ps);
         EventOut o;
         o.flags = 0;
@@ -725,7 +756,7 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_contact(DS S) {
     // pairs (f, j), f outer (owner), distance order
     const int no = S.ocnt[k];
     int nh = 0;
-    const int hco = S.hcnt_old[f];
+    const int hco = S.hcnt_old[fs];
     for (int s = 0; s < no; ++s) {
         const int64_t e = oj[s * SS + k];
         const int j = ent_id(e), qj = ent_pos(e);
@@ -737,13 +768,13 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_contact(DS S) {
             // reset_contact (reset.f90:1-15): the history goes, the last
             // contact forces stay
             for (int q = 0; q < hco; ++q) {
-                const HistRec& h0 = S.hist_old[(int64_t)q * SS + f];
+                const HistRec& h0 = S.hist_old[(int64_t)q * SS + fs];
                 if (h0.j != j) continue;
                 if (h0.fcn != 0.0 || h0.fct != 0.0) {
                     if (nh < S.HC) {
                         HistRec hr; hr.j = j; hr.pad_ = 0; hr.deltat = 0.0; hr.thetarelc = 0.0;
                         hr.fcn = h0.fcn; hr.fct = h0.fct;
-                        S.hist_new[nh * SS + f] = hr;
+                        S.hist_new[nh * SS + fs] = hr;
                         ++nh;
                     } else {
                         over |= ERR_HIST_CAP;
@@ -759,7 +790,7 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_contact(DS S) {
             continue;
         }
         PairState ps;
-        load_contact_state(S, f, j, ps);
+        load_contact_state(S, fs, j, ps);
         bond_create(P, tstep, f - first, j - first, Fi, Fj, kn, ps);
         EventOut o;
         o.flags = 0;
@@ -772,7 +803,7 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_contact(DS S) {
             if (nh < S.HC) {
                 HistRec hr; hr.j = j; hr.pad_ = 0; hr.deltat = ps.deltat; hr.thetarelc = ps.thetarelc;
                 hr.fcn = ps.fcn; hr.fct = ps.fct;
-                S.hist_new[nh * SS + f] = hr;
+                S.hist_new[nh * SS + fs] = hr;
                 ++nh;
             } else {
                 over |= ERR_HIST_CAP;
@@ -789,17 +820,17 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_contact(DS S) {
     // contact histories of pairs that are not candidates this step persist
     // (the reference only resets a candidate pair, reset_contact)
     for (int q = 0; q < hco; ++q) {
-        HistRec hr = S.hist_old[q * SS + f];
+        HistRec hr = S.hist_old[q * SS + fs];
         if (hr.j < 0) continue;
         bool cand = false;
         for (int s = 0; s < no; ++s)
             if (ent_id(oj[s * SS + k]) == hr.j) { cand = true; break; }
         if (!cand) {
-            if (nh < S.HC) S.hist_new[nh * SS + f] = hr, ++nh;
+            if (nh < S.HC) S.hist_new[nh * SS + fs] = hr, ++nh;
             else over |= ERR_HIST_CAP;
         }
     }
-    S.hcnt_new[f] = nh;
+    S.hcnt_new[fs] = nh;
     S.acc[A_FCX * SS + k] = acc[A_FCX];
     S.acc[A_FCY * SS + k] = acc[A_FCY];
     S.acc[A_MC * SS + k] = acc[A_MC];
@@ -823,7 +854,8 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_bond(DS S) {
     const int tstep = S.members[m].tstep;
     const int first = S.members[m].first;
     const bool bond_step = tstep == 1 && P.restart != 1 && P.cohesion;
-    const int f = me.id;
+    const int f = me.id;            // identity
+    const int fs = S.cval[k];       // storage slot of the owner's records
     const int64_t SS = S.S;
     const int64_t* oj = S.oj;
     const Floe F0 = floe_at(S, k);
@@ -831,10 +863,10 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_bond(DS S) {
 
     // the owner's records carry over (cleared entries dropped)
     int nb = 0;
-    const int bco = S.bcnt_old[f];
+    const int bco = S.bcnt_old[fs];
     for (int q = 0; q < bco; ++q) {
-        const BondRec* b = &S.bond_old[q * SS + f];
-        if (b->j >= 0) copy_bond(&S.bond_new[(nb++) * SS + f], b);
+        const BondRec* b = &S.bond_old[q * SS + fs];
+        if (b->j >= 0) copy_bond(&S.bond_new[(nb++) * SS + fs], b);
     }
     const int nb_old = nb;
     bool any = false;
@@ -846,7 +878,7 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_bond(DS S) {
         // the old list (the copy above holds the same values)
         int rq = -1, ro = -1;
         for (int q = 0, c2 = 0; q < bco; ++q) {
-            const int jj = S.bond_old[(int64_t)q * SS + f].j;
+            const int jj = S.bond_old[(int64_t)q * SS + fs].j;
             if (jj < 0) continue;
             if (jj == j) { rq = c2; ro = q; break; }
             ++c2;
@@ -854,7 +886,7 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_bond(DS S) {
         if (rq < 0) {
             // a record created earlier in this loop (step-1 bonds)
             for (int q = nb_old; q < nb; ++q)
-                if (S.bond_new[(int64_t)q * SS + f].j == j) { rq = q; break; }
+                if (S.bond_new[(int64_t)q * SS + fs].j == j) { rq = q; break; }
         }
         if (rq < 0 && !bond_step) continue;
         Floe Fi = F0;
@@ -867,7 +899,7 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_bond(DS S) {
         ps.damage = ps.deltanb = ps.deltatb = ps.trb_ji = ps.trb_ij = 0.0;
         ps.rb = ps.hb = ps.lb = ps.sb = ps.ib = 0.0;
         if (rq >= 0) {
-            const BondRec& b = ro >= 0 ? S.bond_old[(int64_t)ro * SS + f] : S.bond_new[(int64_t)rq * SS + f];
+            const BondRec& b = ro >= 0 ? S.bond_old[(int64_t)ro * SS + fs] : S.bond_new[(int64_t)rq * SS + fs];
             ps.has_bond_rec = true; ps.bond = b.bond; ps.damage = b.damage;
             ps.deltanb = b.deltanb; ps.deltatb = b.deltatb; ps.trb_ji = b.trb_ji; ps.trb_ij = b.trb_ij;
             ps.rb = b.rb; ps.hb = b.hb; ps.lb = b.lb; ps.sb = b.sb; ps.ib = b.ib;
@@ -884,13 +916,13 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_bond(DS S) {
             if (nb < S.BC) rq = nb++;
             else { over |= ERR_BOND_CAP; continue; }
         }
-        S.bond_new[(int64_t)rq * SS + f] = bond_from(ps, j);
+        S.bond_new[(int64_t)rq * SS + fs] = bond_from(ps, j);
         if (o.flags & EV_BONDED) {
             BondTerms bt;
             bt.fbn_c = o.fbn_c; bt.fbt_s = o.fbt_s; bt.fbn_s = o.fbn_s; bt.fbt_c = o.fbt_c;
             bt.lri_fbt = o.lri_fbt; bt.lrj_fbt = o.lrj_fbt; bt.mbb_ji = o.mbb_ji; bt.mbb_ij = o.mbb_ij;
             bt.sb = o.sb; bt.fbn_sb = o.fbn_sb;
-            S.bterm[(int64_t)rq * SS + f] = bt;
+            S.bterm[(int64_t)rq * SS + fs] = bt;
             S.evf[s * SS + k] |= EV_BONDED | ((uint32_t)(rq + 1) << 16);
             // tell the partner where its terms are: its inner slot of f
             const int nij = S.icnt[qj];
@@ -901,7 +933,7 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_bond(DS S) {
         }
     }
     if (any) S.bgat[k] = 1;
-    S.bcnt_new[f] = nb;
+    S.bcnt_new[fs] = nb;
     if (over) set_err(S, over);
 }
 
@@ -911,20 +943,19 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_bond(DS S) {
 __global__ __launch_bounds__(FLOE_BLOCK) void k_bond_gather(DS S) {
     int k = xcd_block() * blockDim.x + threadIdx.x;
     if (k >= S.N || *S.err || !S.bgat[k]) return;
-    const int f = S.geo[k].id;
+    const int fs = S.cval[k];
     const int64_t SS = S.S;
-    const int64_t* oj = S.oj;
     double a[NACC];
     acc_init(a);
     const int ni = S.icnt[k];
     for (int q = 0; q < ni; ++q) {
         const int rq = S.ibond[q * SS + k];
-        if (rq) add_bond_j(a, S.bterm[(int64_t)(rq - 1) * SS + ent_id(S.ik[q * SS + k])]);
+        if (rq) add_bond_j(a, S.bterm[(int64_t)(rq - 1) * SS + S.cval[ent_pos(S.ik[q * SS + k])]]);
     }
     const int no = S.ocnt[k];
     for (int s = 0; s < no; ++s) {
         const uint32_t fl = S.evf[s * SS + k];
-        if (fl & EV_BONDED) add_bond_i(a, S.bterm[(int64_t)((fl >> 16) - 1) * SS + f]);
+        if (fl & EV_BONDED) add_bond_i(a, S.bterm[(int64_t)((fl >> 16) - 1) * SS + fs]);
     }
     S.acc[A_FBX * SS + k] = a[A_FBX];
     S.acc[A_FBY * SS + k] = a[A_FBY];
@@ -945,9 +976,10 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_walls(DS S) {
     const DevParams& P = S.params[m];
     const int first = S.members[m].first;
     const int f = me.id;
+    const int fs = S.cval[k];
     const int64_t SS = S.S;
     Floe Fw = floe_at(S, k);
-    double hist[4] = {S.wbc[0 * SS + f], S.wbc[1 * SS + f], S.wbc[2 * SS + f], S.wbc[3 * SS + f]};
+    double hist[4] = {S.wbc[0 * SS + fs], S.wbc[1 * SS + fs], S.wbc[2 * SS + fs], S.wbc[3 * SS + fs]};
     WallOut W;
     walls<MIXED>(P, Fw, hist, W);
     S.wflag[k] = (uint8_t)(W.flags & 1u);
@@ -1087,39 +1119,41 @@ __global__ void k_repair(DS S, int nmembers) {
         if (key == last) continue;
         last = key;
         int b = (int)(key >> 16), s = (int)(key & 0xFFFF);
-        int f = first + b;
-        int kf = S.pos_of[f];
+        int f = first + b;              // identity
+        const int fs = S.sid_of[f];     // storage slot
+        int kf = S.pos_of[fs];
         ++repaired;
         if (s == WALL_SLOT) {
-            Floe F = load_floe_cur(S, f);
-            double hist[4] = {S.wbc[0 * SS + f], S.wbc[1 * SS + f], S.wbc[2 * SS + f], S.wbc[3 * SS + f]};
+            Floe F = load_floe_cur(S, fs);
+            double hist[4] = {S.wbc[0 * SS + fs], S.wbc[1 * SS + fs], S.wbc[2 * SS + fs], S.wbc[3 * SS + fs]};
             WallOut W;
             walls<MIXED>(P, F, hist, W);
             if (nwt >= nmem) { over |= ERR_REPAIR_CAP; break; }
             wtab[nwt] = W;
             S.ovw[kf] = nwt++;
-            S.affected[f] = 1;
+            S.affected[fs] = 1;
             if (W.flags & 4u) {
-                set_cur(S, f, W.nr, W.nh, W.nmass, W.ninertia, W.nhfa, W.nhfw);
-                S.dirty[f] = 1;
+                set_cur(S, fs, W.nr, W.nh, W.nmass, W.ninertia, W.nhfa, W.nhfw);
+                S.dirty[fs] = 1;
             }
             continue;
         }
         if (b != cur_block) {
             cur_block = b;
-            rq = S.dirty[f] ? S.cur[f] : S.rec[f].r;   // query radius at the top of iteration f
+            rq = S.dirty[fs] ? S.cur[fs] : S.rec[fs].r;   // query radius at the top of iteration f
         }
         const int64_t e = oj[(int64_t)s * SS + kf];
         const int j = ent_id(e);
-        Floe Fi = load_floe_cur(S, f);
-        Floe Fj = load_floe_cur(S, j);
-        S.affected[f] = 1;
-        S.affected[j] = 1;
+        const int js = S.cval[ent_pos(e)];
+        Floe Fi = load_floe_cur(S, fs);
+        Floe Fj = load_floe_cur(S, js);
+        S.affected[fs] = 1;
+        S.affected[js] = 1;
         EventOut o;
         double dx = Fi.x - Fj.x, dy = Fi.y - Fj.y;
         double d = sqrt(dx * dx + dy * dy);
         PairState ps;
-        load_pair_state(S, f, j, ps);
+        load_pair_state(S, fs, j, ps);
         if (!(d <= rq + P.rtree)) {
             // no longer a candidate: the pair keeps its old state
             o.flags = EV_REMOVED;
@@ -1128,17 +1162,18 @@ __global__ void k_repair(DS S, int nmembers) {
             const double2 wf = wind_at(S, kf);
             pair_event<MIXED>(P, tstep, b, j - first, Fi, Fj, wf.x, wf.y, P.uw, P.vw, ps, o, 0);
         }
-        bool okh = put_hist(S, f, j, ps);
-        bool okb = put_bond(S, f, j, ps.has_bond_rec, bond_from(ps, j));
+        bool okh = put_hist(S, fs, j, ps);
+        bool okb = put_bond(S, fs, j, ps.has_bond_rec, bond_from(ps, j));
         if (!okh) over |= ERR_HIST_CAP;
         if (!okb) over |= ERR_BOND_CAP;
         if (o.flags & EV_RIDGED) {
             int g = o.ridged == 0 ? f : j;
-            set_cur(S, g, o.nr, o.nh, o.nmass, o.ninertia, o.nhfa, o.nhfw);
-            if (!S.dirty[g]) {
-                S.dirty[g] = 1;
+            const int gs = o.ridged == 0 ? fs : js;
+            set_cur(S, gs, o.nr, o.nh, o.nmass, o.ninertia, o.nhfa, o.nhfw);
+            if (!S.dirty[gs]) {
+                S.dirty[gs] = 1;
                 int gb = g - first;
-                int kg = S.pos_of[g];
+                int kg = S.pos_of[gs];
                 // every later event of g is recomputed
                 if (g != f) {
                     int gi = S.icnt[kg];
@@ -1181,7 +1216,8 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_fixup(DS S) {
     const int m = me.member;
     if (m < 0) return;
     const int f = me.id;
-    if (!S.affected[f]) return;
+    const int fs = S.cval[k];
+    if (!S.affected[fs]) return;
     const DevParams& P = S.params[m];
     const int tstep = S.members[m].tstep;
     const int first = S.members[m].first;
@@ -1205,7 +1241,7 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_fixup(DS S) {
             Floe Fi = floe_at(S, qi);
             Floe Ff = F0;
             PairState ps;
-            load_pair_state(S, i, f, ps);
+            load_pair_state(S, S.cval[qi], f, ps);
             const double2 wi = wind_at(S, qi);
             EventOut o;
             pair_event<MIXED>(P, tstep, i - first, f - first, Fi, Ff, wi.x, wi.y, P.uw, P.vw, ps, o, 2);
@@ -1225,7 +1261,7 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_fixup(DS S) {
             Floe Fi = F0;
             Floe Fj = floe_at(S, qj);
             PairState ps;
-            load_pair_state(S, f, j, ps);
+            load_pair_state(S, fs, j, ps);
             EventOut o;
             pair_event<MIXED>(P, tstep, f - first, j - first, Fi, Fj, wf.x, wf.y, P.uw, P.vw, ps, o, 1);
             apply_i(acc, o);
@@ -1322,7 +1358,7 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_final(DS S) {
     const GeoRec me = S.geo[k];
     const int m = me.member;
     if (m < 0) return;
-    const int f = me.id;
+    const int f = S.cval[k];        // storage slot
     const DevParams& P = S.params[m];
     const int64_t SS = S.S;
     Floe F = floe_at(S, k);
@@ -1395,7 +1431,7 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_final(DS S) {
     // flags set by the repair are cleared (most floes have none: no write)
     if (dirty) S.dirty[f] = 0;
     if (S.affected[f]) S.affected[f] = 0;
-    if (f == S.members[m].first) S.members[m].tstep += 1;
+    if (f == S.members[m].first) S.members[m].tstep += 1;   // once per member
 }
 
 // per-member totals of the last step: candidates, contacts (evf flags) and
@@ -1411,7 +1447,7 @@ __global__ void k_totals(DS S, int64_t* out) {
             ++c;
             if (fl & EV_CONTACT) ++ct;
         }
-        int f = S.geo[k].id;
+        int f = S.cval[k];
         int nb = S.bcnt_old[f];
         for (int q = 0; q < nb; ++q) {
             const BondRec& b = S.bond_old[(int64_t)q * S.S + f];
@@ -1432,10 +1468,10 @@ __global__ void k_totals(DS S, int64_t* out) {
 }
 
 // explicit instantiations used by the host launcher
-template __global__ void k_cand<false>(DS);
-template __global__ void k_cand<true>(DS);
 template __global__ void k_cand_dense<false>(DS);
 template __global__ void k_cand_dense<true>(DS);
+template __global__ void k_cand<false>(DS);
+template __global__ void k_cand<true>(DS);
 template __global__ void k_contact<false>(DS);
 template __global__ void k_contact<true>(DS);
 template __global__ void k_bond<false>(DS);

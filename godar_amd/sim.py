@@ -100,6 +100,24 @@ class Context:
             self.set_wind_grid(m, field.wind_grid)
         return m
 
+    def checkpoint(self, m: int, params: GodarParams) -> bytes:
+        """Complete state of member m (floes, walls, contact and bond pair
+        state, step number, parameters) as a byte block."""
+        n = ctypes.c_int64()
+        self._chk(self.L.godar_checkpoint_size(self.h, int(m), ctypes.byref(n)))
+        buf = ctypes.create_string_buffer(n.value)
+        self._chk(self.L.godar_checkpoint_save(self.h, int(m), ctypes.byref(params), buf, n.value))
+        return buf.raw
+
+    def restore(self, block: bytes) -> int:
+        """Add a member from a checkpoint block; it continues exactly."""
+        m = ctypes.c_int32()
+        buf = ctypes.create_string_buffer(block, len(block))
+        self._chk(self.L.godar_member_restore(self.h, buf, len(block), ctypes.byref(m)))
+        n = int(np.frombuffer(block[12:16], dtype=np.int32)[0])   # floe count in the header
+        self.sizes[m.value] = n
+        return m.value
+
     def remove(self, m: int):
         self._chk(self.L.godar_member_remove(self.h, int(m)))
         self.sizes.pop(m, None)

@@ -108,6 +108,20 @@ int godar_set_bonds(godar_ctx *ctx, int32_t member, int32_t nb, const int32_t *b
                     const int32_t *bj, const double *state);
 
 /*
+ * Checkpoints (no counterpart in the reference, whose restart reads back
+ * positions, velocities, radii, thicknesses and bonds, ini_get.f90:36-39): the complete
+ * state of a member -- floe state, wall histories, contact histories with the
+ * last contact forces, bond records with their strain state, next step
+ * number and its parameters -- as one byte block.  A member restored from it
+ * continues exactly as the saved one would have (a gridded wind is not part
+ * of the block: set it again).
+ */
+int godar_checkpoint_size(godar_ctx *ctx, int32_t member, int64_t *bytes);
+int godar_checkpoint_save(godar_ctx *ctx, int32_t member, const godar_params *p, void *buf,
+                          int64_t cap);
+int godar_member_restore(godar_ctx *ctx, const void *buf, int64_t size, int32_t *member);
+
+/*
  * Advance every active member by nsteps steps: the replacement of
  * nsteps calls of stepper(tstep, restart) (stepper.f90:1-345).  A step that
  * overflows a per-floe list or the ridging-repair pools is rolled back; the

@@ -312,6 +312,36 @@ def test_dense_search_matches_thread_search():
             assert np.array_equal(a, b)
 
 
+def test_checkpoint_restore_continues_exactly():
+    """A member restored from a checkpoint (floes, walls, contact histories
+    with last contact forces, bond strain state, step number) continues
+    bitwise as the uninterrupted run, alone or next to another member, in FP64
+    and mixed precision; forces and stresses of the next steps match too."""
+    fa = kick(squeeze(packed_box(64, 3)), 1)
+    pa = config0_params(fa, sigmanc_crit=1e5)
+    fb = g.config0_member(51, 700)
+    pb = config0_params(fb)
+    for prec in ("fp64", "mixed"):
+        ref = _ctx(fa.n, 1, prec)
+        m0 = ref.add_field(pa, fa)
+        ref.step(30)
+        block = ref.checkpoint(m0, pa)
+        ref.step(25)
+        want, wantf = ref.floes(m0), ref.forces(m0)
+        ref.close()
+        ctx = _ctx(fa.n + fb.n, 2, prec)
+        ctx.add_field(pb, fb)
+        m1 = ctx.restore(block)
+        assert ctx.tstep(m1) == 31
+        ctx.step(25)
+        got, gotf = ctx.floes(m1), ctx.forces(m1)
+        ctx.close()
+        for k in STATE + ["theta"]:
+            assert np.array_equal(got[k], want[k]), (prec, k)
+        for k in ("tfx", "tfy", "m", *STRESS):
+            assert np.array_equal(gotf[k], wantf[k]), (prec, k)
+
+
 def test_capacity_overflow_rolls_back():
     """A too-small candidate list aborts the step without advancing the state;
     after reserving more room the run continues exactly as an unhindered one.
