@@ -48,6 +48,18 @@ __device__ __forceinline__ int xcd_block() {
     return (int)(blockIdx.x % XCD_COUNT) * per + (int)(blockIdx.x / XCD_COUNT);
 }
 
+// Entry of a work queue of length qn for this thread, or -1.  Each XCD takes
+// one contiguous share of the queue (the grid, sized for all floes, always
+// has enough threads per XCD), so that short queues still spread over all
+// XCDs and each XCD's share sits together in its L2.
+__device__ __forceinline__ int queue_slot(int qn) {
+    const int per = (qn + XCD_COUNT - 1) / XCD_COUNT;
+    const int local = (int)(blockIdx.x / XCD_COUNT) * (int)blockDim.x + (int)threadIdx.x;
+    if (local >= per) return -1;
+    const int t = (int)(blockIdx.x % XCD_COUNT) * per + local;
+    return t < qn ? t : -1;
+}
+
 __device__ __forceinline__ int ent_id(int64_t e) { return (int)(e >> 32); }
 __device__ __forceinline__ int ent_pos(int64_t e) { return (int)(uint32_t)(e & 0xffffffffll); }
 __device__ __forceinline__ int64_t ent(int id, int pos) {
@@ -715,8 +727,8 @@ __device__ __forceinline__ void load_contact_state(const DS& S, int i, int j, Pa
 // update_shape, coulomb; stepper.f90:103-146) + the owner's new histories.
 template <bool MIXED>
 __global__ __launch_bounds__(FLOE_BLOCK) void k_contact(DS S) {
-    int t = xcd_block() * blockDim.x + threadIdx.x;
-    if (*S.err || t >= S.qn[0]) return;
+    const int t = queue_slot(S.qn[0]);
+    if (*S.err || t < 0) return;
     const int k = S.qlist[t];
     const GeoRec me = S.geo[k];
     const int m = me.member;
@@ -845,8 +857,8 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_contact(DS S) {
 // k_bond_gather then sums every floe's terms in the reference's order.
 template <bool MIXED>
 __global__ __launch_bounds__(FLOE_BLOCK) void k_bond(DS S) {
-    int t = xcd_block() * blockDim.x + threadIdx.x;
-    if (*S.err || t >= S.qn[1]) return;
+    const int t = queue_slot(S.qn[1]);
+    if (*S.err || t < 0) return;
     const int k = S.qlist[S.S + t];
     const GeoRec me = S.geo[k];
     const int m = me.member;
@@ -968,8 +980,8 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_bond_gather(DS S) {
 // Walls of the queued floes (verify_bc, boundaries.f90:1-97).
 template <bool MIXED>
 __global__ __launch_bounds__(FLOE_BLOCK) void k_walls(DS S) {
-    int t = xcd_block() * blockDim.x + threadIdx.x;
-    if (*S.err || t >= S.qn[2]) return;
+    const int t = queue_slot(S.qn[2]);
+    if (*S.err || t < 0) return;
     const int k = S.qlist[2 * S.S + t];
     const GeoRec me = S.geo[k];
     const int m = me.member;
