@@ -337,11 +337,19 @@ __device__ __forceinline__ void cand_finish(const DS& S, int k, int fs, const De
 
 // Neighbour search of one floe by one thread: candidate lists with
 // insertion into the slot-major rows, sheltering minima, classification and
-// the per-step resets of the floe's rows.
-template <bool MIXED>
+// the per-step resets of the floe's rows.  SPAN is the member's span (1 for
+// sparse members, 2 for dense ones).  Every cell range of the block is read
+// at once, the rows' positions, taken as one sequence, SCAN_BATCH records at
+// a time, and the head of each candidate list is sorted in registers, so
+// that the search waits on few dependent loads.
+constexpr int SCAN_BATCH = 4;
+constexpr int LIST_REGS = 4;
+
+template <bool MIXED, int SPAN>
 __device__ __forceinline__ void cand_thread(const DS& S, int k, const GeoRec& me, bool& qc,
                                             bool& qb, bool& qw) {
     using T = typename Prec<MIXED>::T;
+    constexpr int R = 2 * SPAN + 1;
     const int m = me.member;
     const DevParams& P = S.params[m];
     const double rad = me.r + P.rtree;
@@ -361,25 +369,69 @@ __device__ __forceinline__ void cand_thread(const DS& S, int k, const GeoRec& me
     const int64_t SS = S.S;
     int64_t* oj = S.oj;
     int64_t* ik = S.ik;
-    const int sp = P.span;
-    const int a0 = max(cx - sp, 0), a1 = min(cx + sp, P.gx - 1);
-    for (int b = cy - sp; b <= cy + sp; ++b) {
-        if (b < 0 || b >= P.gy) continue;
-        // the cells of one row of the block are consecutive in the cell
-        // order: one contiguous range of positions
-        int lo = INT_MAX, hi = 0;
-        for (int a = a0; a <= a1; ++a) {
-            const int2 rg = S.crange[P.cell_base + b * P.gx + a];
-            if (rg.y > rg.x) { lo = min(lo, rg.x); hi = max(hi, rg.y); }
+    // the cells of one row of the block are consecutive in the cell order:
+    // each row is one contiguous range of positions.  Cell indices are
+    // clamped into the grid so that every load is valid; cells outside it
+    // are masked out afterwards.
+    int2 rg[R][R];
+#pragma unroll
+    for (int t = 0; t < R; ++t) {
+        const int b = min(max(cy - SPAN + t, 0), P.gy - 1);
+#pragma unroll
+        for (int u = 0; u < R; ++u) {
+            const int a = min(max(cx - SPAN + u, 0), P.gx - 1);
+            rg[t][u] = S.crange[P.cell_base + b * P.gx + a];
         }
-        // the next row entry is loaded while this one is tested
-        GeoRec nxt;
-        if (lo < hi) nxt = S.geo[lo];
-        for (int q = lo; q < hi; ++q) {
-            const GeoRec cur = nxt;
-            if (q + 1 < hi) nxt = S.geo[q + 1];
-            if (q == k) continue;
-            const GeoRec* o = &cur;
+    }
+    int rlo[R], rlen[R];
+    int L = 0;
+#pragma unroll
+    for (int t = 0; t < R; ++t) {
+        const int b = cy - SPAN + t;
+        int lo = INT_MAX, hi = 0;
+#pragma unroll
+        for (int u = 0; u < R; ++u) {
+            const int a = cx - SPAN + u;
+            const bool in = b >= 0 && b < P.gy && a >= 0 && a < P.gx && rg[t][u].y > rg[t][u].x;
+            lo = in ? min(lo, rg[t][u].x) : lo;
+            hi = in ? max(hi, rg[t][u].y) : hi;
+        }
+        rlo[t] = lo;
+        rlen[t] = lo < hi ? hi - lo : 0;
+        L += rlen[t];
+    }
+    // the first LIST_REGS entries of each list stay in registers while the
+    // lists are built (sorted; empty slots hold keys above every entry), the
+    // rest are sorted in their slots in HBM
+    double rd[LIST_REGS];
+    int64_t re[LIST_REGS], ri[LIST_REGS];
+#pragma unroll
+    for (int t = 0; t < LIST_REGS; ++t) {
+        rd[t] = __longlong_as_double(0x7ff0000000000000ll);
+        re[t] = INT64_MAX;
+        ri[t] = INT64_MAX;
+    }
+    for (int v0 = 0; v0 < L; v0 += SCAN_BATCH) {
+        GeoRec g[SCAN_BATCH];
+        int qq[SCAN_BATCH];
+#pragma unroll
+        for (int u = 0; u < SCAN_BATCH; ++u) {
+            // position of entry v of the concatenated rows (the last entry
+            // stands in past the end; it is not tested twice)
+            int rem = min(v0 + u, L - 1), q = 0;
+#pragma unroll
+            for (int t = 0; t < R; ++t) {
+                q = (rem >= 0 && rem < rlen[t]) ? rlo[t] + rem : q;
+                rem -= rlen[t];
+            }
+            qq[u] = q;
+            g[u] = S.geo[q];
+        }
+#pragma unroll
+        for (int u = 0; u < SCAN_BATCH; ++u) {
+            const int q = qq[u];
+            if (v0 + u >= L || q == k) continue;
+            const GeoRec* o = &g[u];
             const double dx = me.x - o->x, dy = me.y - o->y;
             const double dist2 = dx * dx + dy * dy;
             const int j = o->id;
@@ -391,72 +443,113 @@ __device__ __forceinline__ void cand_thread(const DS& S, int k, const GeoRec& me
             if (!(d <= thr)) continue;
             if (j > f) {
                 if (no >= S.CO) { over |= ERR_OUTER_CAP; continue; }
-                int p = no;
-                while (p > 0) {
-                    double dp = S.od[(p - 1) * SS + k];
-                    int64_t ep = oj[(p - 1) * SS + k];
-                    if (dp < d || (dp == d && ent_id(ep) < j)) break;
-                    S.od[p * SS + k] = dp;
-                    oj[p * SS + k] = ep;
-                    --p;
+                // insertion by (distance, id) into the register part; the
+                // entry that leaves it is the largest so far
+                double xd = d;
+                int64_t xe = ent(j, q);
+#pragma unroll
+                for (int t = 0; t < LIST_REGS; ++t) {
+                    const bool lt = xd < rd[t] || (xd == rd[t] && ent_id(xe) < ent_id(re[t]));
+                    const double td = rd[t];
+                    const int64_t te = re[t];
+                    rd[t] = lt ? xd : td;
+                    re[t] = lt ? xe : te;
+                    xd = lt ? td : xd;
+                    xe = lt ? te : xe;
                 }
-                S.od[p * SS + k] = d;
-                oj[p * SS + k] = ent(j, q);
+                if (no >= LIST_REGS) {
+                    int p = no;
+                    while (p > LIST_REGS) {
+                        double dp = S.od[(p - 1) * SS + k];
+                        int64_t ep = oj[(p - 1) * SS + k];
+                        if (dp < xd || (dp == xd && ent_id(ep) < ent_id(xe))) break;
+                        S.od[p * SS + k] = dp;
+                        oj[p * SS + k] = ep;
+                        --p;
+                    }
+                    S.od[p * SS + k] = xd;
+                    oj[p * SS + k] = xe;
+                }
                 ++no;
             } else {
                 if (ni >= S.CI) { over |= ERR_INNER_CAP; continue; }
-                int p = ni;
-                const int64_t e = ent(j, q);
-                while (p > 0) {
-                    int64_t ep = ik[(p - 1) * SS + k];
-                    if (ep < e) break;
-                    ik[p * SS + k] = ep;
-                    --p;
+                int64_t xe = ent(j, q);
+#pragma unroll
+                for (int t = 0; t < LIST_REGS; ++t) {
+                    const bool lt = xe < ri[t];
+                    const int64_t te = ri[t];
+                    ri[t] = lt ? xe : te;
+                    xe = lt ? te : xe;
                 }
-                ik[p * SS + k] = e;
+                if (ni >= LIST_REGS) {
+                    int p = ni;
+                    while (p > LIST_REGS) {
+                        int64_t ep = ik[(p - 1) * SS + k];
+                        if (ep < xe) break;
+                        ik[p * SS + k] = ep;
+                        --p;
+                    }
+                    ik[p * SS + k] = xe;
+                }
                 ++ni;
             }
         }
     }
     // sheltering and classification over the lists just built (separate
     // from the scan, so that a wavefront's lanes only diverge over their own
-    // few candidates)
-    for (int s2 = 0; s2 < no; ++s2) {
-        const int64_t e = oj[s2 * SS + k];
+    // few candidates).  A minimum and an "any" do not depend on the order of
+    // their terms.
+    auto outer_terms = [&](int64_t e, double d) {
         const int j = ent_id(e), q = ent_pos(e);
-        const double d = S.od[s2 * SS + k];
-        const GeoRec* o = &S.geo[q];
+        const GeoRec o = S.geo[q];
         // pair (f, j): f is the outer floe i of the event
-        const double deltan = me.r + o->r - d;
+        const double deltan = me.r + o.r - d;
         if (deltan > 0) qc = true;
         if (!qb && bond_step && deltan >= -P.bond_lim &&
             bond_keep(f - first, j - first, P.bond_seed, P.bond_thr24))
             qb = true;
         if (P.shelter) {
             // floe f takes hsf*(j,f), evaluated with -cosa, -sina
-            const double cosa = (o->x - me.x) / d, sina = (o->y - me.y) / d;
+            const double cosa = (o.x - me.x) / d, sina = (o.y - me.y) / d;
             const ShelRec oh = S.shel[q];
             const T dn = T(deltan), c = T(cosa), sn = T(sina);
             sha = dmin(sha, S_shelter(T(oh.hfa), T(msh.hfa), dn, -c, -sn, T(msh.wx), T(msh.wy)));
             shw = dmin(shw, S_shelter(T(oh.hfw), T(msh.hfw), dn, -c, -sn, uwi, vwi));
         }
-    }
-    for (int s2 = 0; s2 < ni; ++s2) {
-        const int q = ent_pos(ik[s2 * SS + k]);
-        const GeoRec* o = &S.geo[q];
-        const double dx = me.x - o->x, dy = me.y - o->y;
+    };
+    auto inner_terms = [&](int64_t e) {
+        const int q = ent_pos(e);
+        const GeoRec o = S.geo[q];
+        const double dx = me.x - o.x, dy = me.y - o.y;
         const double d = sqrt(dx * dx + dy * dy);
         // pair (j, f): f is the inner floe j of the event
-        const double deltan = o->r + me.r - d;
+        const double deltan = o.r + me.r - d;
         if (deltan > 0) qc = true;
         if (P.shelter) {
             // cosa/sina point from j to f; floe f takes hsf*(j,f) with j's wind
-            const double cosa = (me.x - o->x) / d, sina = (me.y - o->y) / d;
+            const double cosa = (me.x - o.x) / d, sina = (me.y - o.y) / d;
             const ShelRec oh = S.shel[q];
             const T dn = T(deltan), c = T(cosa), sn = T(sina);
             sha = dmin(sha, S_shelter(T(oh.hfa), T(msh.hfa), dn, c, sn, T(oh.wx), T(oh.wy)));
             shw = dmin(shw, S_shelter(T(oh.hfw), T(msh.hfw), dn, c, sn, uwi, vwi));
         }
+    };
+#pragma unroll
+    for (int s2 = 0; s2 < LIST_REGS; ++s2)
+        if (s2 < no) outer_terms(re[s2], rd[s2]);
+    for (int s2 = LIST_REGS; s2 < no; ++s2) outer_terms(oj[s2 * SS + k], S.od[s2 * SS + k]);
+#pragma unroll
+    for (int s2 = 0; s2 < LIST_REGS; ++s2)
+        if (s2 < ni) inner_terms(ri[s2]);
+    for (int s2 = LIST_REGS; s2 < ni; ++s2) inner_terms(ik[s2 * SS + k]);
+    // the register part of the lists goes to its slots
+#pragma unroll
+    for (int s2 = 0; s2 < LIST_REGS; ++s2) {
+        if (s2 < no) {
+            S.od[s2 * SS + k] = rd[s2];
+            oj[s2 * SS + k] = re[s2];
+        }
+        if (s2 < ni) ik[s2 * SS + k] = ri[s2];
     }
     if (over) set_err(S, over);
     cand_finish<MIXED>(S, k, fs, P, no, ni, double(sha), double(shw), qc, qb, 0, 1);
@@ -478,7 +571,8 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_cand(DS S) {
     }
     if (live) {
         if (S.params[me.member].dense) qd = true;   // k_cand_dense searches it
-        else cand_thread<MIXED>(S, k, me, qc, qb, qw);
+        // members that are not dense have span 1 (godar_member_add)
+        else cand_thread<MIXED, 1>(S, k, me, qc, qb, qw);
     }
     enqueue(qc, S.qlist, &S.qn[0], k);
     enqueue(qb, S.qlist + S.S, &S.qn[1], k);
@@ -617,7 +711,7 @@ __global__ __launch_bounds__(64) void k_cand_dense(DS S) {
             // rare: more candidates than LDS holds, the thread search does it
             if (lane == 0) {
                 bool c2, b2, w2;
-                cand_thread<MIXED>(S, k, me, c2, b2, w2);
+                cand_thread<MIXED, 2>(S, k, me, c2, b2, w2);
                 if (c2) S.qlist[atomicAdd(&S.qn[0], 1)] = k;
                 if (b2) S.qlist[SS + atomicAdd(&S.qn[1], 1)] = k;
                 if (w2) S.qlist[2 * SS + atomicAdd(&S.qn[2], 1)] = k;
