@@ -967,51 +967,20 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_bond(DS S) {
     const Floe F0 = floe_at(S, k);
     uint32_t over = 0;
 
-    // the owner's records carry over (cleared entries dropped)
-    int nb = 0;
-    const int bco = S.bcnt_old[fs];
-    for (int q = 0; q < bco; ++q) {
-        const BondRec* b = &S.bond_old[q * SS + fs];
-        if (b->j >= 0) copy_bond(&S.bond_new[(nb++) * SS + fs], b);
-    }
-    const int nb_old = nb;
-    bool any = false;
     const int no = S.ocnt[k];
-    for (int s = 0; s < no; ++s) {
+    bool any = false;
+    int nb = 0;                     // records written so far
+    // one bond event: pair (f, j) at outer slot s, record index rq (new
+    // records: -1, appended if the event leaves one)
+    auto event = [&](int s, int rq, PairState& ps) {
         const int64_t e = oj[s * SS + k];
         const int j = ent_id(e), qj = ent_pos(e);
-        // record of j: its index among the carried-over records, found in
-        // the old list (the copy above holds the same values)
-        int rq = -1, ro = -1;
-        for (int q = 0, c2 = 0; q < bco; ++q) {
-            const int jj = S.bond_old[(int64_t)q * SS + fs].j;
-            if (jj < 0) continue;
-            if (jj == j) { rq = c2; ro = q; break; }
-            ++c2;
-        }
-        if (rq < 0) {
-            // a record created earlier in this loop (step-1 bonds)
-            for (int q = nb_old; q < nb; ++q)
-                if (S.bond_new[(int64_t)q * SS + fs].j == j) { rq = q; break; }
-        }
-        if (rq < 0 && !bond_step) continue;
         Floe Fi = F0;
         Floe Fj = floe_at(S, qj);
         Kin kn;
         kinematics(Fi, Fj, kn);
-        PairState ps;
-        ps.deltat = 0.0; ps.thetarelc = 0.0; ps.fcn = 0.0; ps.fct = 0.0;
-        ps.bond = 0; ps.has_bond_rec = false;
-        ps.damage = ps.deltanb = ps.deltatb = ps.trb_ji = ps.trb_ij = 0.0;
-        ps.rb = ps.hb = ps.lb = ps.sb = ps.ib = 0.0;
-        if (rq >= 0) {
-            const BondRec& b = ro >= 0 ? S.bond_old[(int64_t)ro * SS + fs] : S.bond_new[(int64_t)rq * SS + fs];
-            ps.has_bond_rec = true; ps.bond = b.bond; ps.damage = b.damage;
-            ps.deltanb = b.deltanb; ps.deltatb = b.deltatb; ps.trb_ji = b.trb_ji; ps.trb_ij = b.trb_ij;
-            ps.rb = b.rb; ps.hb = b.hb; ps.lb = b.lb; ps.sb = b.sb; ps.ib = b.ib;
-        }
         bond_create(P, tstep, f - first, j - first, Fi, Fj, kn, ps);
-        if (!ps.has_bond_rec) continue;
+        if (!ps.has_bond_rec) return;
         EventOut o;
         o.flags = 0;
         o.ridged = -1;
@@ -1020,7 +989,7 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_bond(DS S) {
         if (S.record_forces) S.ssb[s * SS + k] = o.sbd;
         if (rq < 0) {
             if (nb < S.BC) rq = nb++;
-            else { over |= ERR_BOND_CAP; continue; }
+            else { over |= ERR_BOND_CAP; return; }
         }
         S.bond_new[(int64_t)rq * SS + fs] = bond_from(ps, j);
         if (o.flags & EV_BONDED) {
@@ -1036,6 +1005,45 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_bond(DS S) {
                 if (ent_id(S.ik[(int64_t)q * SS + qj]) == f) { S.ibond[(int64_t)q * SS + qj] = rq + 1; break; }
             S.bgat[qj] = 1;
             any = true;
+        }
+    };
+    // the owner's records in order, cleared entries dropped: a record whose
+    // partner is a candidate this step takes part in an event (its new
+    // values replace it), the others carry over unchanged
+    const int bco = S.bcnt_old[fs];
+    for (int q = 0; q < bco; ++q) {
+        const BondRec b = S.bond_old[q * SS + fs];
+        if (b.j < 0) continue;
+        const int rq = nb++;
+        int s = -1;
+        for (int s2 = 0; s2 < no; ++s2)
+            if (ent_id(oj[s2 * SS + k]) == b.j) { s = s2; break; }
+        if (s < 0) {
+            S.bond_new[(int64_t)rq * SS + fs] = b;
+            continue;
+        }
+        PairState ps;
+        ps.deltat = 0.0; ps.thetarelc = 0.0; ps.fcn = 0.0; ps.fct = 0.0;
+        ps.has_bond_rec = true; ps.bond = b.bond; ps.damage = b.damage;
+        ps.deltanb = b.deltanb; ps.deltatb = b.deltatb; ps.trb_ji = b.trb_ji; ps.trb_ij = b.trb_ij;
+        ps.rb = b.rb; ps.hb = b.hb; ps.lb = b.lb; ps.sb = b.sb; ps.ib = b.ib;
+        event(s, rq, ps);
+    }
+    // bonds that form this step (bond_create), in candidate order, appended
+    if (bond_step) {
+        const int nb_old = nb;
+        for (int s = 0; s < no; ++s) {
+            const int j = ent_id(oj[s * SS + k]);
+            bool has = false;
+            for (int q = 0; q < nb_old; ++q)
+                if (S.bond_new[(int64_t)q * SS + fs].j == j) { has = true; break; }
+            if (has) continue;
+            PairState ps;
+            ps.deltat = 0.0; ps.thetarelc = 0.0; ps.fcn = 0.0; ps.fct = 0.0;
+            ps.bond = 0; ps.has_bond_rec = false;
+            ps.damage = ps.deltanb = ps.deltatb = ps.trb_ji = ps.trb_ij = 0.0;
+            ps.rb = ps.hb = ps.lb = ps.sb = ps.ib = 0.0;
+            event(s, -1, ps);
         }
     }
     if (any) S.bgat[k] = 1;
