@@ -192,6 +192,10 @@ void sync_ds(godar_ctx* c) {
     d.SEEDK = c->SEEDK; d.HEAPK = c->HEAPK; d.TABK = c->TABK;
     d.record_forces = c->record;
     d.ncells = c->ncells_cap;
+    // XCD share of the candidate search (k_cand's grid, XCD-aware order)
+    const int bc = c->sblock[3];
+    const int gc = ((std::max(c->N_used, 1) + bc - 1) / bc + XCD_COUNT - 1) / XCD_COUNT * XCD_COUNT;
+    d.qseg = gc / XCD_COUNT * bc;
     bind_parity(c);
 }
 
@@ -317,7 +321,8 @@ int launch_step(godar_ctx* c) {
     }
     {
         StageTimer t(c, 2);
-        hipLaunchKernelGGL(k_clear_cells, dim3((c->ncells_cap + 1 + 255) / 256), dim3(256), 0, s, d);
+        hipLaunchKernelGGL(k_clear_cells, dim3((std::max(c->ncells_cap + 1, NQUEUE * XCD_COUNT) + 255) / 256),
+                           dim3(256), 0, s, d);
         hipLaunchKernelGGL(k_ranges, grid_for(N, B[2], false), dim3(B[2]), 0, s, d);
     }
     int nm = c->Mcap;
@@ -524,7 +529,7 @@ int godar_create(int64_t max_floes, int32_t max_members, int32_t precision, goda
     ACHK(dalloc(&d.cls, S));
     ACHK(dalloc(&d.qlist, 4 * S));
     ACHK(dalloc(&d.bgat, S));
-    ACHK(dalloc(&d.qn, 4));
+    ACHK(dalloc(&d.qn, (int64_t)NQUEUE * XCD_COUNT * QN_STRIDE));
     ACHK(dalloc(&d.seeds, (int64_t)c->SEEDK * S));
     ACHK(dalloc(&d.nseeds, max_members));
     ACHK(dalloc(&d.heap, (int64_t)c->HEAPK * S));
@@ -1449,9 +1454,13 @@ int godar_get_totals(godar_ctx* c, int64_t* out) {
         hipLaunchKernelGGL(k_totals, dim3((c->N_used + 255) / 256), dim3(256), 0, c->stream, c->d, dout);
     hipError_t e = hipStreamSynchronize(c->stream);
     if (e == hipSuccess) e = hipMemcpy(out, dout, 3 * sizeof(int64_t), hipMemcpyDeviceToHost);
-    int32_t q[3] = {0, 0, 0};
-    if (e == hipSuccess) e = hipMemcpy(q, c->d.qn, sizeof(q), hipMemcpyDeviceToHost);
-    for (int k = 0; k < 3; ++k) out[3 + k] = q[k];
+    std::vector<int32_t> q((size_t)NQUEUE * XCD_COUNT * QN_STRIDE, 0);
+    if (e == hipSuccess)
+        e = hipMemcpy(q.data(), c->d.qn, q.size() * sizeof(int32_t), hipMemcpyDeviceToHost);
+    for (int k = 0; k < 3; ++k) {
+        out[3 + k] = 0;
+        for (int x = 0; x < XCD_COUNT; ++x) out[3 + k] += q[(size_t)(k * XCD_COUNT + x) * QN_STRIDE];
+    }
     (void)hipFree(dout);
     HIPCHK(c, e);
     return GODAR_OK;

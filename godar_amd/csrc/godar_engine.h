@@ -20,6 +20,8 @@ constexpr int WALL_SLOT = 0xFFFF;
 constexpr uint8_t CLS_CONTACT = 1, CLS_BOND = 2;
 constexpr int FLOE_BLOCK = 256;   // upper bound of the per-floe kernels' block size
 constexpr int XCD_COUNT = 8;      // accelerator complex dies of an MI355X
+constexpr int NQUEUE = 4;         // work queues: contact, bond, wall, dense search
+constexpr int QN_STRIDE = 64;     // ints between queue counters (256 B)
 
 // Persistent per-floe record (96 bytes), indexed by floe id.
 struct FloeRec {
@@ -50,6 +52,8 @@ struct DS {
     int32_t SEEDK, HEAPK, TABK;  // repair pool entries per floe slot
     int32_t record_forces;
     int32_t ncells;            // cell slots (sentinel key = ncells)
+    int32_t qseg;              // floe positions per XCD share of the candidate search
+    int32_t pad_q_;
     // persistent floe state (id)
     FloeRec* rec;
     double* wbc;               // [4][S] deltat_bc1, deltat_bc2, theta_bc1, theta_bc2
@@ -96,8 +100,13 @@ struct DS {
     uint8_t* wflag;            // [S] 1: the floe touched a wall this step
     int32_t* ovw;              // [S]
     uint8_t* cls;              // [S] CLS_* bits
-    int32_t* qlist;            // [3][S] positions queued for contact / bond / wall work
-    int32_t* qn;               // [3] queue lengths
+    // Work queues (contact, bond, wall, dense search), each in XCD_COUNT
+    // segments: the candidate search's blocks on XCD x cover positions
+    // [x qseg, (x+1) qseg) and queue them into segment x, at
+    // qlist[c S + x qseg + i], counted by their own counter (one per
+    // QN_STRIDE ints), so that the counters' atomics spread over many lines.
+    int32_t* qlist;            // [4][S]
+    int32_t* qn;               // [4][XCD_COUNT] x QN_STRIDE
     // ordered repair (id)
     uint64_t* seeds;           // SEEDK per floe slot, member region starts at SEEDK*first
     int32_t* nseeds;           // [members]

@@ -60,6 +60,48 @@ __device__ __forceinline__ int queue_slot(int qn) {
     return t < qn ? t : -1;
 }
 
+__device__ __forceinline__ int32_t* qcount(const DS& S, int c, int x) {
+    return S.qn + (c * XCD_COUNT + x) * QN_STRIDE;
+}
+
+// Queue c read as one sequence (its segments in XCD order): the segment
+// counts and the total.
+struct QueueView {
+    int cnt[XCD_COUNT];
+    int total;
+};
+__device__ __forceinline__ QueueView queue_view(const DS& S, int c) {
+    QueueView v;
+    v.total = 0;
+#pragma unroll
+    for (int x = 0; x < XCD_COUNT; ++x) {
+        v.cnt[x] = *qcount(S, c, x);
+        v.total += v.cnt[x];
+    }
+    return v;
+}
+// position held by entry t (0 <= t < total) of the sequence
+__device__ __forceinline__ int queue_at(const DS& S, int c, const QueueView& v, int t) {
+    int idx = 0;
+#pragma unroll
+    for (int x = 0; x < XCD_COUNT; ++x) {
+        idx = (t >= 0 && t < v.cnt[x]) ? x * S.qseg + t : idx;
+        t -= v.cnt[x];
+    }
+    return S.qlist[(int64_t)c * S.S + idx];
+}
+// the queue entry of this thread (split over the XCDs as queue_slot), or -1
+__device__ __forceinline__ int queue_entry(const DS& S, int c) {
+    const QueueView v = queue_view(S, c);
+    const int t = queue_slot(v.total);
+    return t < 0 ? -1 : queue_at(S, c, v, t);
+}
+// appends position k to queue c from a single lane
+__device__ __forceinline__ void enqueue_one(const DS& S, int c, int k) {
+    const int x = k / S.qseg;
+    S.qlist[(int64_t)c * S.S + x * S.qseg + atomicAdd(qcount(S, c, x), 1)] = k;
+}
+
 __device__ __forceinline__ int ent_id(int64_t e) { return (int)(e >> 32); }
 __device__ __forceinline__ int ent_pos(int64_t e) { return (int)(uint32_t)(e & 0xffffffffll); }
 __device__ __forceinline__ int64_t ent(int id, int pos) {
@@ -242,7 +284,7 @@ __global__ void k_clear_cells(DS S) {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c > S.ncells || *S.err) return;
     S.crange[c] = make_int2(0, 0);
-    if (c < 4) S.qn[c] = 0;
+    if (c < NQUEUE * XCD_COUNT) S.qn[c * QN_STRIDE] = 0;
 }
 
 // cell ranges over the sorted keys, the sorted floe records and the wind at
@@ -275,19 +317,21 @@ __global__ void k_ranges(DS S) {
     S.shel[k] = h;
 }
 
-// Appends position k to a work queue, one atomic per wavefront.  Every lane
-// of the wavefront must call it.
-__device__ __forceinline__ void enqueue(bool want, int32_t* q, int32_t* cnt, int k) {
+// Appends position k to work queue c, one atomic per wavefront.  Every lane
+// of the wavefront must call it; the wavefront's positions lie in one XCD
+// share of the candidate search.
+__device__ __forceinline__ void enqueue(const DS& S, bool want, int c, int k) {
     const uint64_t mask = __ballot(want);
     if (!mask) return;
     const int lane = threadIdx.x & 63;
     const int leader = __ffsll((long long)mask) - 1;
+    const int x = __shfl(k, leader) / S.qseg;
     int base = 0;
-    if (lane == leader) base = atomicAdd(cnt, __popcll(mask));
+    if (lane == leader) base = atomicAdd(qcount(S, c, x), __popcll(mask));
     base = __shfl(base, leader);
     if (want) {
         const uint64_t below = lane ? mask & ((~0ull) >> (64 - lane)) : 0ull;
-        q[base + __popcll(below)] = k;
+        S.qlist[(int64_t)c * S.S + x * S.qseg + base + __popcll(below)] = k;
     }
 }
 
@@ -574,10 +618,10 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_cand(DS S) {
         // members that are not dense have span 1 (godar_member_add)
         else cand_thread<MIXED, 1>(S, k, me, qc, qb, qw);
     }
-    enqueue(qc, S.qlist, &S.qn[0], k);
-    enqueue(qb, S.qlist + S.S, &S.qn[1], k);
-    enqueue(qw, S.qlist + 2 * S.S, &S.qn[2], k);
-    enqueue(qd, S.qlist + 3 * S.S, &S.qn[3], k);
+    enqueue(S, qc, 0, k);
+    enqueue(S, qb, 1, k);
+    enqueue(S, qw, 2, k);
+    enqueue(S, qd, 3, k);
 }
 
 // Dense members (many floes per cell): one wavefront per floe.  The lanes
@@ -645,7 +689,8 @@ __global__ __launch_bounds__(64) void k_cand_dense(DS S) {
     __shared__ int64_t si[DENSE_CAP];
     if (*S.err) return;
     const int lane = threadIdx.x;
-    const int nq = S.qn[3];
+    const QueueView qv = queue_view(S, 3);
+    const int nq = qv.total;
     const int nb = (int)(gridDim.x / XCD_COUNT);
     const int xcd = (int)(blockIdx.x % XCD_COUNT), bi = (int)(blockIdx.x / XCD_COUNT);
     const int per = (nq + XCD_COUNT - 1) / XCD_COUNT;
@@ -653,7 +698,7 @@ __global__ __launch_bounds__(64) void k_cand_dense(DS S) {
     const uint64_t lt = lane ? ((~0ull) >> (64 - lane)) : 0ull;
     const int64_t SS = S.S;
     for (int t = xcd * per + bi; t < t1; t += nb) {
-        const int k = S.qlist[3 * SS + t];
+        const int k = queue_at(S, 3, qv, t);
         const GeoRec me = S.geo[k];
         const int m = me.member;
         const DevParams& P = S.params[m];
@@ -712,9 +757,9 @@ __global__ __launch_bounds__(64) void k_cand_dense(DS S) {
             if (lane == 0) {
                 bool c2, b2, w2;
                 cand_thread<MIXED, 2>(S, k, me, c2, b2, w2);
-                if (c2) S.qlist[atomicAdd(&S.qn[0], 1)] = k;
-                if (b2) S.qlist[SS + atomicAdd(&S.qn[1], 1)] = k;
-                if (w2) S.qlist[2 * SS + atomicAdd(&S.qn[2], 1)] = k;
+                if (c2) enqueue_one(S, 0, k);
+                if (b2) enqueue_one(S, 1, k);
+                if (w2) enqueue_one(S, 2, k);
             }
             __syncthreads();
             continue;
@@ -784,9 +829,9 @@ __global__ __launch_bounds__(64) void k_cand_dense(DS S) {
         const bool qw = me.y - me.r < 0 || me.y + me.r > P.ny || me.x - me.r < 0 || me.x + me.r > P.nx;
         cand_finish<MIXED>(S, k, fs, P, no, ni, double(sha), double(shw), qc, qb, lane, 64);
         if (lane == 0) {
-            if (qc) S.qlist[atomicAdd(&S.qn[0], 1)] = k;
-            if (qb) S.qlist[SS + atomicAdd(&S.qn[1], 1)] = k;
-            if (qw) S.qlist[2 * SS + atomicAdd(&S.qn[2], 1)] = k;
+            if (qc) enqueue_one(S, 0, k);
+            if (qb) enqueue_one(S, 1, k);
+            if (qw) enqueue_one(S, 2, k);
         }
         __syncthreads();
     }
@@ -821,9 +866,9 @@ __device__ __forceinline__ void load_contact_state(const DS& S, int i, int j, Pa
 // update_shape, coulomb; stepper.f90:103-146) + the owner's new histories.
 template <bool MIXED>
 __global__ __launch_bounds__(FLOE_BLOCK) void k_contact(DS S) {
-    const int t = queue_slot(S.qn[0]);
-    if (*S.err || t < 0) return;
-    const int k = S.qlist[t];
+    if (*S.err) return;
+    const int k = queue_entry(S, 0);
+    if (k < 0) return;
     const GeoRec me = S.geo[k];
     const int m = me.member;
     const DevParams& P = S.params[m];
@@ -951,9 +996,9 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_contact(DS S) {
 // k_bond_gather then sums every floe's terms in the reference's order.
 template <bool MIXED>
 __global__ __launch_bounds__(FLOE_BLOCK) void k_bond(DS S) {
-    const int t = queue_slot(S.qn[1]);
-    if (*S.err || t < 0) return;
-    const int k = S.qlist[S.S + t];
+    if (*S.err) return;
+    const int k = queue_entry(S, 1);
+    if (k < 0) return;
     const GeoRec me = S.geo[k];
     const int m = me.member;
     const DevParams& P = S.params[m];
@@ -1082,9 +1127,9 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_bond_gather(DS S) {
 // Walls of the queued floes (verify_bc, boundaries.f90:1-97).
 template <bool MIXED>
 __global__ __launch_bounds__(FLOE_BLOCK) void k_walls(DS S) {
-    const int t = queue_slot(S.qn[2]);
-    if (*S.err || t < 0) return;
-    const int k = S.qlist[2 * S.S + t];
+    if (*S.err) return;
+    const int k = queue_entry(S, 2);
+    if (k < 0) return;
     const GeoRec me = S.geo[k];
     const int m = me.member;
     const DevParams& P = S.params[m];
