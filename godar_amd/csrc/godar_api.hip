@@ -89,6 +89,8 @@ struct godar_ctx {
     // captured for
     bool use_graphs = true;
     int dense_scan = 96;                // floes per 3x3 search above which a member is dense
+    int repair_lanes = 64;              // repair events evaluated at once
+    int repair_cap = 1024;              // pending repair keys held in LDS
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
     std::vector<unsigned char> gsig[2];
 };
@@ -196,6 +198,8 @@ void sync_ds(godar_ctx* c) {
     const int bc = c->sblock[3];
     const int gc = ((std::max(c->N_used, 1) + bc - 1) / bc + XCD_COUNT - 1) / XCD_COUNT * XCD_COUNT;
     d.qseg = gc / XCD_COUNT * bc;
+    d.repair_lanes = c->repair_lanes;
+    d.repair_cap = c->repair_cap;
     bind_parity(c);
 }
 
@@ -1129,6 +1133,13 @@ int godar_set_block(godar_ctx* c, int32_t threads) {
 int godar_set_dense_search(godar_ctx* c, int32_t floes) {
     if (!c || floes < 0) return GODAR_ERR_ARG;
     c->dense_scan = floes;
+    return GODAR_OK;
+}
+
+int godar_set_repair(godar_ctx* c, int32_t lanes, int32_t pending) {
+    if (!c || lanes < 1 || lanes > 64 || pending < 1 || pending > 1024) return GODAR_ERR_ARG;
+    c->repair_lanes = lanes;
+    c->repair_cap = pending;
     return GODAR_OK;
 }
 

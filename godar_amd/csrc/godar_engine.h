@@ -53,7 +53,9 @@ struct DS {
     int32_t record_forces;
     int32_t ncells;            // cell slots (sentinel key = ncells)
     int32_t qseg;              // floe positions per XCD share of the candidate search
-    int32_t pad_q_;
+    int32_t repair_lanes;      // events the repair evaluates at once (1: one at a time)
+    int32_t repair_cap;        // pending repair keys held in LDS
+    int32_t pad_r_;
     // persistent floe state (id)
     FloeRec* rec;
     double* wbc;               // [4][S] deltat_bc1, deltat_bc2, theta_bc1, theta_bc2

@@ -1153,7 +1153,7 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_walls(DS S) {
 }
 
 // ---------------------------------------------------------------------
-// 4. ordered repair, one member per block, serial in lane 0
+// 4. ordered repair, one member per block (one wavefront)
 // ---------------------------------------------------------------------
 struct Heap {
     uint64_t* a;
@@ -1243,10 +1243,198 @@ __device__ __forceinline__ bool put_bond(const DS& S, int i, int j, bool want, c
     return true;
 }
 
+// put_hist / put_bond for the events of several lanes at once.  Lanes may
+// share the owner i (never the pair): every lane finds and updates its own
+// entry, then the lanes that append do so one at a time, in lane order.
+__device__ __forceinline__ void put_hist_lanes(const DS& S, bool on, int i, int j,
+                                               const PairState& ps, uint32_t& over) {
+    const bool want = on && (ps.deltat != 0.0 || ps.thetarelc != 0.0 || ps.fcn != 0.0 || ps.fct != 0.0);
+    HistRec hr;
+    hr.j = j; hr.pad_ = 0; hr.deltat = ps.deltat; hr.thetarelc = ps.thetarelc; hr.fcn = ps.fcn; hr.fct = ps.fct;
+    int q = -1;
+    if (on) {
+        const int n = S.hcnt_new[i];
+        for (int p = 0; p < n; ++p)
+            if (S.hist_new[(int64_t)p * S.S + i].j == j) { q = p; break; }
+        if (q >= 0) {
+            if (want) S.hist_new[(int64_t)q * S.S + i] = hr;
+            else S.hist_new[(int64_t)q * S.S + i].j = -1;
+        }
+    }
+    uint64_t ma = __ballot(want && q < 0);
+    while (ma) {
+        const int l = __ffsll((long long)ma) - 1;
+        if ((int)(threadIdx.x & 63) == l) {
+            const int n = S.hcnt_new[i];
+            if (n >= S.HC) over |= ERR_HIST_CAP;
+            else { S.hist_new[(int64_t)n * S.S + i] = hr; S.hcnt_new[i] = n + 1; }
+        }
+        ma &= ma - 1;
+    }
+}
+
+__device__ __forceinline__ void put_bond_lanes(const DS& S, bool on, int i, int j, bool want,
+                                               const BondRec& b, uint32_t& over) {
+    want = want && on;
+    int q = -1;
+    if (on) {
+        const int n = S.bcnt_new[i];
+        for (int p = 0; p < n; ++p)
+            if (S.bond_new[(int64_t)p * S.S + i].j == j) { q = p; break; }
+        if (q >= 0) {
+            if (want) S.bond_new[(int64_t)q * S.S + i] = b;
+            else S.bond_new[(int64_t)q * S.S + i].j = -1;
+        }
+    }
+    uint64_t ma = __ballot(want && q < 0);
+    while (ma) {
+        const int l = __ffsll((long long)ma) - 1;
+        if ((int)(threadIdx.x & 63) == l) {
+            const int n = S.bcnt_new[i];
+            if (n >= S.BC) over |= ERR_BOND_CAP;
+            else { S.bond_new[(int64_t)n * S.S + i] = b; S.bcnt_new[i] = n + 1; }
+        }
+        ma &= ma - 1;
+    }
+}
+
+// Progress of one member's repair (the same in every lane).
+struct RepairState {
+    int64_t last;        // key of the last event recomputed (-1: none)
+    int cur_block;       // f - first of the last pair event
+    double rq;           // query radius of that floe at the top of its iteration
+    int nt, nwt;         // repair table entries used
+    int64_t repaired;
+};
+
+// Every later event of floe g, reshaped by event (f, s), goes onto the heap:
+// the pairs (i2, g) with f < i2 < g, then g's own events after s (all of
+// them if g != f) and g's walls.
+__device__ __forceinline__ void push_later_events(const DS& S, Heap& H, int first, int f, int s,
+                                                  int g, int gs, uint32_t& over) {
+    const int64_t SS = S.S;
+    const int gb = g - first;
+    const int kg = S.pos_of[gs];
+    if (g != f) {
+        const int gi = S.icnt[kg];
+        for (int q = 0; q < gi; ++q) {
+            const int64_t ei = S.ik[(int64_t)q * SS + kg];
+            const int i2 = ent_id(ei);
+            if (i2 <= f) continue;
+            const int sk = slot_of(S, ent_pos(ei), g);
+            if (sk >= 0 && !H.push(((uint64_t)(i2 - first) << 16) | (uint64_t)sk)) over |= ERR_REPAIR_CAP;
+        }
+    }
+    const int s0 = (g == f) ? s + 1 : 0;
+    const int gno = S.ocnt[kg];
+    for (int s2 = s0; s2 < gno; ++s2)
+        if (!H.push(((uint64_t)gb << 16) | (uint64_t)s2)) over |= ERR_REPAIR_CAP;
+    if (!H.push(((uint64_t)gb << 16) | WALL_SLOT)) over |= ERR_REPAIR_CAP;
+}
+
+// One event at a time off the binary heap in HBM (single lane).
 template <bool MIXED>
-__global__ void k_repair(DS S, int nmembers) {
-    int m = blockIdx.x;
-    if (m >= nmembers || threadIdx.x != 0 || *S.err) return;
+__device__ void repair_serial(const DS& S, const DevParams& P, int first, int nmem, int tstep,
+                              Heap& H, EventOut* etab, int tcap, WallOut* wtab, RepairState& st,
+                              uint32_t& over) {
+    const int64_t SS = S.S;
+    while (H.n > 0 && !over) {
+        const uint64_t key = H.pop();
+        if ((int64_t)key == st.last) continue;
+        st.last = (int64_t)key;
+        const int b = (int)(key >> 16), s = (int)(key & 0xFFFF);
+        const int f = first + b;              // identity
+        const int fs = S.sid_of[f];           // storage slot
+        const int kf = S.pos_of[fs];
+        ++st.repaired;
+        if (s == WALL_SLOT) {
+            Floe F = load_floe_cur(S, fs);
+            double hist[4] = {S.wbc[0 * SS + fs], S.wbc[1 * SS + fs], S.wbc[2 * SS + fs], S.wbc[3 * SS + fs]};
+            WallOut W;
+            walls<MIXED>(P, F, hist, W);
+            if (st.nwt >= nmem) { over |= ERR_REPAIR_CAP; break; }
+            wtab[st.nwt] = W;
+            S.ovw[kf] = st.nwt++;
+            S.affected[fs] = 1;
+            if (W.flags & 4u) {
+                set_cur(S, fs, W.nr, W.nh, W.nmass, W.ninertia, W.nhfa, W.nhfw);
+                S.dirty[fs] = 1;
+            }
+            continue;
+        }
+        if (b != st.cur_block) {
+            st.cur_block = b;
+            st.rq = S.dirty[fs] ? S.cur[fs] : S.rec[fs].r;   // query radius at the top of iteration f
+        }
+        const int64_t e = S.oj[(int64_t)s * SS + kf];
+        const int j = ent_id(e);
+        const int js = S.cval[ent_pos(e)];
+        Floe Fi = load_floe_cur(S, fs);
+        Floe Fj = load_floe_cur(S, js);
+        S.affected[fs] = 1;
+        S.affected[js] = 1;
+        EventOut o;
+        const double dx = Fi.x - Fj.x, dy = Fi.y - Fj.y;
+        const double d = sqrt(dx * dx + dy * dy);
+        PairState ps;
+        load_pair_state(S, fs, j, ps);
+        if (!(d <= st.rq + P.rtree)) {
+            // no longer a candidate: the pair keeps its old state
+            o.flags = EV_REMOVED;
+            o.ridged = -1;
+        } else {
+            const double2 wf = wind_at(S, kf);
+            pair_event<MIXED>(P, tstep, b, j - first, Fi, Fj, wf.x, wf.y, P.uw, P.vw, ps, o, 0);
+        }
+        if (!put_hist(S, fs, j, ps)) over |= ERR_HIST_CAP;
+        if (!put_bond(S, fs, j, ps.has_bond_rec, bond_from(ps, j))) over |= ERR_BOND_CAP;
+        if (o.flags & EV_RIDGED) {
+            const int g = o.ridged == 0 ? f : j;
+            const int gs = o.ridged == 0 ? fs : js;
+            set_cur(S, gs, o.nr, o.nh, o.nmass, o.ninertia, o.nhfa, o.nhfw);
+            if (!S.dirty[gs]) {
+                S.dirty[gs] = 1;
+                push_later_events(S, H, first, f, s, g, gs, over);
+            }
+        }
+        if (st.nt >= tcap) { over |= ERR_REPAIR_CAP; break; }
+        etab[st.nt] = o;
+        S.ov[(int64_t)s * SS + kf] = st.nt++;
+        S.evf[(int64_t)s * SS + kf] = o.flags;
+    }
+}
+
+// number of a[0..n) below v (strictly, or at most v)
+__device__ __forceinline__ int count_below(const int64_t* a, int n, int64_t v, bool or_equal) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        const bool left = or_equal ? a[mid] <= v : a[mid] < v;
+        if (left) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+// The recomputation follows the reference's loop order, key
+// (f - first) << 16 | slot.  Between two reshapes the recomputed events are
+// independent: an event reads the radius, thickness, mass, inertia and
+// drafts of its floes, which only a reshape changes, and writes the state of
+// its own pair.  So the wavefront evaluates the next `repair_lanes` pending
+// keys at once, one per lane, and commits them in key order up to and
+// including the first that reshapes a floe; the rest stay pending and are
+// evaluated again with the new state.  The pending keys are a sorted array in
+// LDS; those a reshape adds (every later event of the reshaped floe, produced
+// in order) are merged in.  A member whose pending keys outgrow `repair_cap`
+// continues one event at a time on its binary heap in HBM.
+constexpr int REPAIR_PCAP = 1024;    // pending keys held in LDS
+constexpr int REPAIR_NCAP = 512;     // keys one reshape adds, at most
+
+template <bool MIXED>
+__global__ __launch_bounds__(64) void k_repair(DS S, int nmembers) {
+    __shared__ int64_t pk[2][REPAIR_PCAP];
+    __shared__ int64_t nk[REPAIR_NCAP];
+    const int m = blockIdx.x;
+    if (m >= nmembers || *S.err) return;
     if (!member_live(S.members[m])) return;
     int ns = S.nseeds[m];
     if (ns == 0) return;
@@ -1255,112 +1443,191 @@ __global__ void k_repair(DS S, int nmembers) {
     const int nmem = S.members[m].n;
     const int tstep = S.members[m].tstep;
     const int64_t SS = S.S;
-    const int64_t* oj = S.oj;
-    const int64_t* ik = S.ik;
     if (ns > S.SEEDK * nmem) ns = S.SEEDK * nmem;
+    const int lane = (int)threadIdx.x;
+    const uint64_t lt = lane ? ((~0ull) >> (64 - lane)) : 0ull;
+    const uint64_t* seeds = S.seeds + (int64_t)S.SEEDK * first;
+    EventOut* etab = S.etab + (int64_t)S.TABK * first;
+    const int tcap = S.TABK * nmem;
+    WallOut* wtab = S.wtab + first;
+    RepairState st;
+    st.last = -1; st.cur_block = -1; st.rq = 0.0; st.nt = 0; st.nwt = 0; st.repaired = 0;
+    uint32_t over = 0;
     Heap H;
     H.a = S.heap + (int64_t)S.HEAPK * first;
     H.n = 0;
     H.cap = S.HEAPK * nmem;
-    uint32_t over = 0;
-    for (int p = 0; p < ns; ++p)
-        if (!H.push(S.seeds[(int64_t)S.SEEDK * first + p])) over |= ERR_REPAIR_CAP;
-    EventOut* etab = S.etab + (int64_t)S.TABK * first;
-    const int tcap = S.TABK * nmem;
-    WallOut* wtab = S.wtab + first;
-    int nt = 0, nwt = 0;
-    uint64_t last = ~0ull;
-    int cur_block = -1;
-    double rq = 0.0;
-    int64_t repaired = 0;
-    while (H.n > 0 && !over) {
-        uint64_t key = H.pop();
-        if (key == last) continue;
-        last = key;
-        int b = (int)(key >> 16), s = (int)(key & 0xFFFF);
-        int f = first + b;              // identity
-        const int fs = S.sid_of[f];     // storage slot
-        int kf = S.pos_of[fs];
-        ++repaired;
-        if (s == WALL_SLOT) {
+    const int lanes = min(max(S.repair_lanes, 1), 64);
+    const int cap = min(max(S.repair_cap, 1), REPAIR_PCAP);
+    bool serial = lanes == 1 || ns > cap;
+    int cb = 0, head = 0, np = 0;
+    if (serial) {
+        if (lane == 0)
+            for (int p = 0; p < ns; ++p)
+                if (!H.push(seeds[p])) over |= ERR_REPAIR_CAP;
+    } else {
+        const int p2 = pow2_at_least(ns);
+        for (int i = lane; i < p2; i += 64) pk[0][i] = i < ns ? (int64_t)seeds[i] : INT64_MAX;
+        __syncthreads();
+        bitonic_inner(pk[0], p2, lane);
+        np = ns;
+    }
+    while (!serial && np > 0) {
+        const int nb = min(np, lanes);
+        const int64_t key = lane < nb ? pk[cb][head + lane] : INT64_MAX;
+        const int64_t prev = lane == 0 ? st.last : (lane < nb ? pk[cb][head + lane - 1] : INT64_MAX);
+        const bool act = lane < nb && key != prev;      // repeated keys are done once
+        const int64_t kk = act ? key : 0;
+        const int b = (int)(kk >> 16), s = (int)(kk & 0xFFFF);
+        const int f = first + b;
+        const bool wall = act && s == WALL_SLOT;
+        const bool pair = act && s != WALL_SLOT;
+        const uint64_t mw = __ballot(wall), me = __ballot(pair);
+        const int rank_w = __popcll(mw & lt), rank_e = __popcll(me & lt);
+        if (st.nt + __popcll(me) > tcap || st.nwt + __popcll(mw) > nmem) { over |= ERR_REPAIR_CAP; break; }
+        int fs = 0, kf = 0, j = 0, js = 0, ridged = -1;
+        bool reshape = false;
+        uint32_t oflags = 0;
+        double rql = 0.0, nr = 0.0, nh = 0.0, nmass = 0.0, ninertia = 0.0, nhfa = 0.0, nhfw = 0.0;
+        PairState ps;
+        if (act) {
+            fs = S.sid_of[f];
+            kf = S.pos_of[fs];
+        }
+        if (wall) {
             Floe F = load_floe_cur(S, fs);
             double hist[4] = {S.wbc[0 * SS + fs], S.wbc[1 * SS + fs], S.wbc[2 * SS + fs], S.wbc[3 * SS + fs]};
             WallOut W;
             walls<MIXED>(P, F, hist, W);
-            if (nwt >= nmem) { over |= ERR_REPAIR_CAP; break; }
-            wtab[nwt] = W;
-            S.ovw[kf] = nwt++;
-            S.affected[fs] = 1;
-            if (W.flags & 4u) {
-                set_cur(S, fs, W.nr, W.nh, W.nmass, W.ninertia, W.nhfa, W.nhfw);
-                S.dirty[fs] = 1;
+            wtab[st.nwt + rank_w] = W;
+            reshape = (W.flags & 4u) != 0;
+            nr = W.nr; nh = W.nh; nmass = W.nmass; ninertia = W.ninertia; nhfa = W.nhfa; nhfw = W.nhfw;
+        }
+        if (pair) {
+            rql = b == st.cur_block ? st.rq : (S.dirty[fs] ? S.cur[fs] : S.rec[fs].r);
+            const int64_t e = S.oj[(int64_t)s * SS + kf];
+            j = ent_id(e);
+            js = S.cval[ent_pos(e)];
+            Floe Fi = load_floe_cur(S, fs);
+            Floe Fj = load_floe_cur(S, js);
+            const double dx = Fi.x - Fj.x, dy = Fi.y - Fj.y;
+            const double d = sqrt(dx * dx + dy * dy);
+            load_pair_state(S, fs, j, ps);
+            EventOut o;
+            if (!(d <= rql + P.rtree)) {
+                o.flags = EV_REMOVED;
+                o.ridged = -1;
+            } else {
+                const double2 wf = wind_at(S, kf);
+                pair_event<MIXED>(P, tstep, b, j - first, Fi, Fj, wf.x, wf.y, P.uw, P.vw, ps, o, 0);
             }
+            etab[st.nt + rank_e] = o;
+            oflags = o.flags;
+            reshape = (o.flags & EV_RIDGED) != 0;
+            ridged = o.ridged;
+            nr = o.nr; nh = o.nh; nmass = o.nmass; ninertia = o.ninertia; nhfa = o.nhfa; nhfw = o.nhfw;
+        }
+        // commit every event up to and including the first reshape
+        const uint64_t mres = __ballot(reshape);
+        const int c = mres ? __ffsll((long long)mres) - 1 : nb - 1;
+        const bool com = act && lane <= c;
+        if (com && pair) {
+            S.affected[fs] = 1;
+            S.affected[js] = 1;
+            S.ov[(int64_t)s * SS + kf] = st.nt + rank_e;
+            S.evf[(int64_t)s * SS + kf] = oflags;
+        }
+        if (com && wall) {
+            S.ovw[kf] = st.nwt + rank_w;
+            S.affected[fs] = 1;
+        }
+        put_hist_lanes(S, com && pair, fs, j, ps, over);
+        put_bond_lanes(S, com && pair, fs, j, ps.has_bond_rec, bond_from(ps, j), over);
+        int g = 0, gs = 0;
+        bool adds = false;
+        if (com && reshape) {
+            g = (wall || ridged == 0) ? f : j;
+            gs = (wall || ridged == 0) ? fs : js;
+            set_cur(S, gs, nr, nh, nmass, ninertia, nhfa, nhfw);
+            if (wall) S.dirty[gs] = 1;
+            else if (!S.dirty[gs]) { S.dirty[gs] = 1; adds = true; }
+        }
+        const uint64_t mce = __ballot(com && pair), mcw = __ballot(com && wall);
+        st.nt += __popcll(mce);
+        st.nwt += __popcll(mcw);
+        st.repaired += __popcll(mce | mcw);
+        if (mce | mcw) st.last = __shfl(key, 63 - __clzll((long long)(mce | mcw)));
+        if (mce) {
+            const int hl = 63 - __clzll((long long)mce);
+            const int bl = __shfl(b, hl);
+            const double rl = __shfl(rql, hl);
+            if (bl != st.cur_block) { st.cur_block = bl; st.rq = rl; }
+        }
+        const int consumed = c + 1;
+        if (__ballot(over != 0)) break;
+        if (!__ballot(adds)) {
+            head += consumed;
+            np -= consumed;
             continue;
         }
-        if (b != cur_block) {
-            cur_block = b;
-            rq = S.dirty[fs] ? S.cur[fs] : S.rec[fs].r;   // query radius at the top of iteration f
-        }
-        const int64_t e = oj[(int64_t)s * SS + kf];
-        const int j = ent_id(e);
-        const int js = S.cval[ent_pos(e)];
-        Floe Fi = load_floe_cur(S, fs);
-        Floe Fj = load_floe_cur(S, js);
-        S.affected[fs] = 1;
-        S.affected[js] = 1;
-        EventOut o;
-        double dx = Fi.x - Fj.x, dy = Fi.y - Fj.y;
-        double d = sqrt(dx * dx + dy * dy);
-        PairState ps;
-        load_pair_state(S, fs, j, ps);
-        if (!(d <= rq + P.rtree)) {
-            // no longer a candidate: the pair keeps its old state
-            o.flags = EV_REMOVED;
-            o.ridged = -1;
-        } else {
-            const double2 wf = wind_at(S, kf);
-            pair_event<MIXED>(P, tstep, b, j - first, Fi, Fj, wf.x, wf.y, P.uw, P.vw, ps, o, 0);
-        }
-        bool okh = put_hist(S, fs, j, ps);
-        bool okb = put_bond(S, fs, j, ps.has_bond_rec, bond_from(ps, j));
-        if (!okh) over |= ERR_HIST_CAP;
-        if (!okb) over |= ERR_BOND_CAP;
-        if (o.flags & EV_RIDGED) {
-            int g = o.ridged == 0 ? f : j;
-            const int gs = o.ridged == 0 ? fs : js;
-            set_cur(S, gs, o.nr, o.nh, o.nmass, o.ninertia, o.nhfa, o.nhfw);
-            if (!S.dirty[gs]) {
-                S.dirty[gs] = 1;
-                int gb = g - first;
-                int kg = S.pos_of[gs];
-                // every later event of g is recomputed
-                if (g != f) {
-                    int gi = S.icnt[kg];
-                    for (int q = 0; q < gi; ++q) {
-                        const int64_t ei = ik[(int64_t)q * SS + kg];
-                        int i2 = ent_id(ei);
-                        if (i2 <= f) continue;
-                        int sk = slot_of(S, ent_pos(ei), g);
-                        if (sk >= 0 && !H.push(((uint64_t)(i2 - first) << 16) | (uint64_t)sk))
-                            over |= ERR_REPAIR_CAP;
-                    }
-                }
-                int s0 = (g == f) ? s + 1 : 0;
-                int gno = S.ocnt[kg];
-                for (int s2 = s0; s2 < gno; ++s2)
-                    if (!H.push(((uint64_t)gb << 16) | (uint64_t)s2)) over |= ERR_REPAIR_CAP;
-                if (!H.push(((uint64_t)gb << 16) | WALL_SLOT)) over |= ERR_REPAIR_CAP;
+        // the reshape of lane c adds every later event of its floe g
+        const int gc = __shfl(g, c), gsc = __shfl(gs, c), fc = __shfl(f, c), sc = __shfl(s, c);
+        const int kg = S.pos_of[gsc];
+        const int gi = gc != fc ? S.icnt[kg] : 0;
+        const int s0 = gc == fc ? sc + 1 : 0;
+        const int gno = S.ocnt[kg];
+        const int rest = np - consumed;
+        const int kmax = gi + max(gno - s0, 0) + 1;
+        if (kmax > REPAIR_NCAP || rest + kmax > cap) {
+            // too many pending keys for LDS: on to the heap in HBM
+            if (lane == 0) {
+                for (int i = 0; i < rest; ++i)
+                    if (!H.push((uint64_t)pk[cb][head + consumed + i])) over |= ERR_REPAIR_CAP;
+                push_later_events(S, H, first, fc, sc, gc, gsc, over);
             }
+            serial = true;
+            break;
         }
-        if (nt >= tcap) { over |= ERR_REPAIR_CAP; break; }
-        etab[nt] = o;
-        S.ov[(int64_t)s * SS + kf] = nt++;
-        S.evf[(int64_t)s * SS + kf] = o.flags;
+        int K = 0;
+        for (int q0 = 0; q0 < gi; q0 += 64) {
+            const int q = q0 + lane;
+            bool ok = false;
+            int64_t nkey = 0;
+            if (q < gi) {
+                const int64_t ei = S.ik[(int64_t)q * SS + kg];
+                const int i2 = ent_id(ei);
+                if (i2 > fc) {
+                    const int sk = slot_of(S, ent_pos(ei), gc);
+                    if (sk >= 0) { ok = true; nkey = ((int64_t)(i2 - first) << 16) | sk; }
+                }
+            }
+            const uint64_t mk = __ballot(ok);
+            if (ok) nk[K + __popcll(mk & lt)] = nkey;
+            K += __popcll(mk);
+        }
+        for (int s2 = s0 + lane; s2 < gno; s2 += 64) nk[K + s2 - s0] = ((int64_t)(gc - first) << 16) | s2;
+        K += max(gno - s0, 0);
+        if (lane == 0) nk[K] = ((int64_t)(gc - first) << 16) | WALL_SLOT;
+        K += 1;
+        __syncthreads();
+        // merge the two sorted runs (pending keys first among equals)
+        const int64_t* A = &pk[cb][head + consumed];
+        int64_t* O = pk[cb ^ 1];
+        for (int i = lane; i < rest; i += 64) O[i + count_below(nk, K, A[i], false)] = A[i];
+        for (int i = lane; i < K; i += 64) O[i + count_below(A, rest, nk[i], true)] = nk[i];
+        __syncthreads();
+        cb ^= 1;
+        head = 0;
+        np = rest + K;
     }
-    S.netab[m] = nt;
-    S.nwtab[m] = nwt;
-    S.stats[4 * m + 2] = ns;
-    S.stats[4 * m + 3] = repaired;
+    if (serial && lane == 0)
+        repair_serial<MIXED>(S, P, first, nmem, tstep, H, etab, tcap, wtab, st, over);
+    if (lane == 0) {
+        S.netab[m] = st.nt;
+        S.nwtab[m] = st.nwt;
+        S.stats[4 * m + 2] = ns;
+        S.stats[4 * m + 3] = st.repaired;
+    }
     if (over) set_err(S, over);
 }
 

@@ -162,6 +162,11 @@ class Context:
         """Members added afterwards examining >= floes per search use one wavefront per floe."""
         self._chk(self.L.godar_set_dense_search(self.h, int(floes)))
 
+    def set_repair(self, lanes: int = 64, pending: int = 1024):
+        """Ridging repair: events evaluated at once (1: one at a time) and
+        pending events kept on chip.  Results do not depend on either."""
+        self._chk(self.L.godar_set_repair(self.h, int(lanes), int(pending)))
+
     def set_stage_block(self, stage: int, threads: int):
         self._chk(self.L.godar_set_stage_block(self.h, int(stage), int(threads)))
 

@@ -191,6 +191,11 @@ int godar_set_stage_block(godar_ctx *ctx, int32_t stage, int32_t threads);
  * searched by one wavefront per floe instead of one thread (0: every member,
  * a huge value: none).  Results do not depend on it. */
 int godar_set_dense_search(godar_ctx *ctx, int32_t floes);
+/* Ridging repair: the number of pending events one member evaluates at once
+ * (1-64, default 64; 1: one at a time) and the pending events kept in on-chip
+ * memory (1-1024, default 1024) before the member continues one event at a
+ * time.  Results do not depend on either. */
+int godar_set_repair(godar_ctx *ctx, int32_t lanes, int32_t pending);
 int godar_get_stage_blocks(const godar_ctx *ctx, int32_t *threads);   /* GODAR_NSTAGES entries */
 /* Autotuner: for each tunable stage in turn, time `trial` real steps of the
  * current members with each block size and keep the fastest.  The members

@@ -163,6 +163,41 @@ def test_dense_ridging_repair_fp64():
     ctx.close()
 
 
+def test_repair_modes_bitwise():
+    """The ridging repair gives the same trajectory, bitwise, whether each
+    member evaluates 64, 5 or 1 pending events at once, and when its pending
+    events outgrow on-chip memory (8 or 3 kept) and it continues on the heap
+    in HBM; heavy in-loop ridging, FP64 and mixed precision."""
+    fd, pd, _ = load("dense64.npz")
+    fb = kick(squeeze(packed_box(64, 3)), 1)
+    pb = config0_params(fb, sigmanc_crit=1e5)
+    for prec in ("fp64", "mixed"):
+        runs = {}
+        for mode in ((64, 1024), (1, 1024), (5, 1024), (64, 8), (64, 3)):
+            ctx = _ctx(fd.n + fb.n, 2, prec)
+            ctx.set_repair(*mode)
+            ms = [ctx.add_field(pd, fd), ctx.add_field(pb, fb)]
+            repaired = 0
+            snaps = []
+            for t in range(1, 31):
+                ctx.step(1)
+                repaired += sum(ctx.stats(m)["repaired"] for m in ms)
+                if t in (1, 5, 30):
+                    snaps.append([(ctx.floes(m), _pairs(ctx, m, "contact"), ctx.forces(m)["tfx"]) for m in ms])
+            runs[mode] = (snaps, repaired)
+            ctx.close()
+        ref, rep0 = runs[(64, 1024)]
+        assert rep0 > 300, rep0
+        for mode, (snaps, rep) in runs.items():
+            assert rep == rep0, (prec, mode, rep, rep0)
+            for sa, sb in zip(snaps, ref):
+                for (fa, ca, ta), (fb_, cb, tb) in zip(sa, sb):
+                    for key in STATE:
+                        assert np.array_equal(fa[key], fb_[key]), (prec, mode, key)
+                    assert ca == cb, (prec, mode)
+                    assert np.array_equal(ta, tb), (prec, mode)
+
+
 def test_stiff_bond_breaking_fp64():
     f = kick(squeeze(packed_box(64, 5), 0.995), 2, speed=0.5)
     p = config0_params(f, sigmanc_crit=1e5, e_modul=6e9, eb=6e9, bond_lim=50.0, bond_frac=1.0)
