@@ -17,6 +17,11 @@ __global__ void k_bin(DS);
 __global__ void k_clear_cells(DS);
 __global__ void k_ranges(DS);
 template <bool M> __global__ void k_cand(DS);
+template <bool F> __global__ void k_sup(DS);
+template <bool F> __global__ void k_sup_dense(DS);
+__global__ void k_scan_tiles(DS);
+__global__ void k_scan_total(DS, int);
+__global__ void k_scan_offsets(DS);
 template <bool M> __global__ void k_cand_dense(DS);
 template <bool M> __global__ void k_contact(DS);
 template <bool M> __global__ void k_bond(DS);
@@ -91,6 +96,9 @@ struct godar_ctx {
     int dense_scan = 96;                // floes per 3x3 search above which a member is dense
     int repair_lanes = 64;              // repair events evaluated at once
     int repair_cap = 1024;              // pending repair keys held in LDS
+    double skin_frac = 0.1;             // superset skin of new members, x (max r + rtree)
+    int64_t sup_cap = 0;                // superset pool entries
+    int32_t built_qseg = -1;            // queue share the dense list was built with
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
     std::vector<unsigned char> gsig[2];
 };
@@ -187,6 +195,17 @@ void bind_parity(godar_ctx* c) {
     c->d.bcnt_old = c->bcnt[o]; c->d.bcnt_new = c->bcnt[n];
 }
 
+// The next step builds the candidate supersets (and the cell order) anew.
+void force_rebuild(godar_ctx* c) {
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, c->stream, c->d.rebuild, 1, 1);
+}
+
+int32_t qseg_of(const godar_ctx* c) {
+    const int bc = c->sblock[3];
+    const int gc = ((std::max(c->N_used, 1) + bc - 1) / bc + XCD_COUNT - 1) / XCD_COUNT * XCD_COUNT;
+    return gc / XCD_COUNT * bc;
+}
+
 void sync_ds(godar_ctx* c) {
     DS& d = c->d;
     d.S = c->S; d.N = c->N_used;
@@ -194,12 +213,10 @@ void sync_ds(godar_ctx* c) {
     d.SEEDK = c->SEEDK; d.HEAPK = c->HEAPK; d.TABK = c->TABK;
     d.record_forces = c->record;
     d.ncells = c->ncells_cap;
-    // XCD share of the candidate search (k_cand's grid, XCD-aware order)
-    const int bc = c->sblock[3];
-    const int gc = ((std::max(c->N_used, 1) + bc - 1) / bc + XCD_COUNT - 1) / XCD_COUNT * XCD_COUNT;
-    d.qseg = gc / XCD_COUNT * bc;
+    d.qseg = qseg_of(c);   // XCD share of the candidate search (k_cand's grid, XCD-aware order)
     d.repair_lanes = c->repair_lanes;
     d.repair_cap = c->repair_cap;
+    d.sup_cap = c->sup_cap;
     bind_parity(c);
 }
 
@@ -316,30 +333,43 @@ int launch_step(godar_ctx* c) {
     if (N == 0) return GODAR_OK;
     const int* B = c->sblock;
     hipStream_t s = c->stream;
+    // Steps that start with d.rebuild set bin, sort and search the cells and
+    // build the candidate supersets; the other steps' build kernels return at
+    // once (the launch sequence, and a captured graph, is the same).
     { StageTimer t(c, 0); hipLaunchKernelGGL(k_bin, grid_for(N, B[0], false), dim3(B[0]), 0, s, d); }
     int bits = 1;
     while ((1u << bits) <= (uint32_t)c->ncells_cap && bits < 32) ++bits;
     {
         StageTimer t(c, 1);
-        HIPCHK(c, radix_sort_pairs(c->rs, d.ckey, d.cval, N, bits, d.err, s));
+        HIPCHK(c, radix_sort_pairs(c->rs, d.ckey, d.cval, N, bits, d.err, d.rebuild, s));
     }
+    int nm = c->Mcap;
+    const bool mixed = c->precision == GODAR_MIXED;
+    bool any_dense = false;
+    for (int k = 0; k < c->Mcap; ++k)
+        if (c->members[k].active && c->dparams[k].dense) any_dense = true;
     {
         StageTimer t(c, 2);
         hipLaunchKernelGGL(k_clear_cells, dim3((std::max(c->ncells_cap + 1, NQUEUE * XCD_COUNT) + 255) / 256),
                            dim3(256), 0, s, d);
         hipLaunchKernelGGL(k_ranges, grid_for(N, B[2], false), dim3(B[2]), 0, s, d);
+        // superset build: counts, offsets, entries (k_sup shares k_cand's
+        // grid, whose XCD shares lay out the dense-search list)
+        const int ntiles = (N + SCAN_TILE - 1) / SCAN_TILE;
+        hipLaunchKernelGGL(k_sup<false>, grid_for(N, B[3], true), dim3(B[3]), 0, s, d);
+        if (any_dense) hipLaunchKernelGGL(k_sup_dense<false>, dim3(XCD_COUNT * 1024), dim3(64), 0, s, d);
+        hipLaunchKernelGGL(k_scan_tiles, dim3(ntiles), dim3(SCAN_TILE), 0, s, d);
+        hipLaunchKernelGGL(k_scan_total, dim3(1), dim3(SCAN_TILE), 0, s, d, ntiles);
+        hipLaunchKernelGGL(k_scan_offsets, dim3(ntiles), dim3(SCAN_TILE), 0, s, d);
+        hipLaunchKernelGGL(k_sup<true>, grid_for(N, B[3], true), dim3(B[3]), 0, s, d);
+        if (any_dense) hipLaunchKernelGGL(k_sup_dense<true>, dim3(XCD_COUNT * 1024), dim3(64), 0, s, d);
     }
-    int nm = c->Mcap;
-    const bool mixed = c->precision == GODAR_MIXED;
     {
         StageTimer t(c, 3);
         hipLaunchKernelGGL(mixed ? k_cand<true> : k_cand<false>, grid_for(N, B[3], true),
                            dim3(B[3]), 0, s, d);
-        bool any_dense = false;
-        for (int k = 0; k < c->Mcap; ++k)
-            if (c->members[k].active && c->dparams[k].dense) any_dense = true;
         if (any_dense) {
-            // one wavefront per dense floe, a contiguous share of the queue per XCD
+            // one wavefront per dense floe, a contiguous share of the list per XCD
             hipLaunchKernelGGL(mixed ? k_cand_dense<true> : k_cand_dense<false>,
                                dim3(XCD_COUNT * 1024), dim3(64), 0, s, d);
         }
@@ -450,6 +480,7 @@ std::string err_text(uint32_t e) {
     if (e & ERR_BOND_CAP) s += " bond list full;";
     if (e & ERR_REPAIR_CAP) s += " ridging repair table full;";
     if (e & ERR_SEED_CAP) s += " ridging seed list full;";
+    if (e & ERR_SUP_CAP) s += " candidate superset pool full;";
     return s;
 }
 
@@ -549,10 +580,20 @@ int godar_create(int64_t max_floes, int32_t max_members, int32_t precision, goda
     ACHK(dalloc(&d.fout, 33 * S));
     ACHK(dalloc(&d.err, 1));
     ACHK(dalloc(&d.steps_done, 1));
+    c->sup_cap = 8 * S;
+    ACHK(dalloc(&d.sup, c->sup_cap));
+    ACHK(dalloc(&d.scnt, S));
+    ACHK(dalloc(&d.soff, S));
+    ACHK(dalloc(&d.sblk, S / SCAN_TILE + 2));
+    ACHK(dalloc(&d.x0, S));
+    ACHK(dalloc(&d.rebuild, 1));
     ACHK(radix_alloc(c->rs, S));
 #undef ACHK
     (void)hipMemsetAsync(d.err, 0, sizeof(uint32_t), c->stream);
     (void)hipMemsetAsync(d.steps_done, 0, sizeof(int64_t), c->stream);
+    (void)hipMemsetAsync(d.scnt, 0, sizeof(int32_t) * S, c->stream);
+    (void)hipMemsetAsync(d.qn, 0, sizeof(int32_t) * NQUEUE * XCD_COUNT * QN_STRIDE, c->stream);
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, c->stream, d.rebuild, 1, 1);
     (void)hipMemsetAsync(d.dirty, 0, S, c->stream);
     (void)hipMemsetAsync(d.affected, 0, S, c->stream);
     (void)hipMemsetAsync(d.stats, 0, sizeof(int64_t) * 4 * max_members, c->stream);
@@ -589,6 +630,7 @@ int godar_destroy(godar_ctx* c) {
     dfree(d.wout); dfree(d.wflag); dfree(d.cls); dfree(d.qlist); dfree(d.qn); dfree(d.bgat); dfree(d.seeds); dfree(d.nseeds); dfree(d.heap); dfree(d.etab);
     dfree(d.netab); dfree(d.wtab); dfree(d.nwtab); dfree(d.ovw); dfree(d.dirty);
     dfree(d.affected); dfree(d.stats); dfree(d.err); dfree(d.steps_done);
+    dfree(d.sup); dfree(d.scnt); dfree(d.soff); dfree(d.sblk); dfree(d.x0); dfree(d.rebuild);
     radix_free(c->rs);
     collect_events(c);
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
@@ -614,6 +656,8 @@ static int64_t bytes_per_floe(const godar_ctx* c) {
     per += (int64_t)c->SEEDK * 8 + (int64_t)c->HEAPK * 8 + (int64_t)c->TABK * sizeof(EventOut);
     per += sizeof(WallOut) /*wtab*/ + 4 /*ovw*/ + 6 * 8 /*cur*/ + 1 + 1 /*dirty, affected*/;
     per += 33 * 8 /*fout*/ + 4 + 4 /*radix keys, values*/;
+    per += (c->S > 0 ? c->sup_cap / c->S : 0) * 8 /*superset pool share*/ + 4 + 8 + 16 /*scnt soff x0*/;
+    per += 1 /*sblk, per SCAN_TILE floes*/;
     return per;
 }
 
@@ -662,10 +706,12 @@ int godar_member_add(godar_ctx* c, const godar_params* p, int32_t n, const doubl
     }
     DevParams dp;
     derive_params(*p, dp);
-    // cell >= (max query reach) / span, so that (2 span + 1)^2 cells around a
-    // floe hold every candidate.  Grids are coarsened to at most 4n+64 cells.
+    dp.skin = c->skin_frac * (rmax + p->rtree);
+    // cell >= (max query reach + skin) / span, so that (2 span + 1)^2 cells
+    // around a floe hold its whole superset.  Grids are coarsened to at most
+    // 4n+64 cells.
     auto grid = [&](int span, double& cell, int64_t& gx, int64_t& gy) {
-        cell = (rmax + p->rtree) * 1.0000001 / span + 1e-9;
+        cell = (rmax + p->rtree + dp.skin) * 1.0000001 / span + 1e-9;
         gx = std::max<int64_t>(1, (int64_t)std::floor(p->nx / cell));
         gy = std::max<int64_t>(1, (int64_t)std::floor(p->ny / cell));
         while (gx * gy > 4 * (int64_t)n + 64) { gx = (gx + 1) / 2; gy = (gy + 1) / 2; cell *= 2; }
@@ -749,6 +795,7 @@ int godar_member_add(godar_ctx* c, const godar_params* p, int32_t n, const doubl
     c->N_used = std::max<int32_t>(c->N_used, (int32_t)(first + n));
     rc = upload_tables(c);
     if (rc != GODAR_OK) return rc;
+    force_rebuild(c);
     HIPCHK(c, hipStreamSynchronize(c->stream));
     *member = m;
     return GODAR_OK;
@@ -1015,6 +1062,7 @@ int godar_member_remove(godar_ctx* c, int32_t m) {
     c->N_used = hw;
     rc = upload_members_only(c);
     if (rc != GODAR_OK) return rc;
+    force_rebuild(c);
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return GODAR_OK;
 }
@@ -1136,6 +1184,12 @@ int godar_set_dense_search(godar_ctx* c, int32_t floes) {
     return GODAR_OK;
 }
 
+int godar_set_skin(godar_ctx* c, double frac) {
+    if (!c || !(frac >= 0.0) || frac > 1.0) return GODAR_ERR_ARG;
+    c->skin_frac = frac;
+    return GODAR_OK;
+}
+
 int godar_set_repair(godar_ctx* c, int32_t lanes, int32_t pending) {
     if (!c || lanes < 1 || lanes > 64 || pending < 1 || pending > 1024) return GODAR_ERR_ARG;
     c->repair_lanes = lanes;
@@ -1203,6 +1257,13 @@ int godar_step_async(godar_ctx* c, int32_t nsteps) {
     }
     const bool graphs = c->use_graphs && !c->prof;
     for (int k = 0; k < nsteps; ++k) {
+        // the dense-search list is laid out by the candidate search's XCD
+        // shares: a new block size or floe count needs a build
+        const int32_t q = qseg_of(c);
+        if (q != c->built_qseg) {
+            force_rebuild(c);
+            c->built_qseg = q;
+        }
         int rc = graphs ? graph_step(c) : launch_step(c);
         if (rc != GODAR_OK) return rc;
     }
@@ -1268,6 +1329,15 @@ static bool grow_for(godar_ctx* c, uint32_t bits) {
         if (dalloc(&c->d.etab, (int64_t)c->TABK * c->S) != hipSuccess) return false;
         grew = true;
     }
+    if (bits & ERR_SUP_CAP) {
+        if (c->sup_cap >= ((int64_t)1 << 34)) return false;
+        if (hipStreamSynchronize(c->stream) != hipSuccess) return false;
+        c->sup_cap *= 2;
+        dfree(c->d.sup);
+        if (dalloc(&c->d.sup, c->sup_cap) != hipSuccess) return false;
+        grew = true;
+    }
+    if (grew) force_rebuild(c);
     return grew;
 }
 
@@ -1301,6 +1371,7 @@ int godar_member_run_until(godar_ctx* c, int32_t m, int32_t tlast) {
     c->members[m].tstop = tlast > 0 ? tlast + 1 : 0;
     rc = upload_members_only(c);
     if (rc != GODAR_OK) return rc;
+    force_rebuild(c);   // a member that runs again needs its floes back in the cell order
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return GODAR_OK;
 }

@@ -22,6 +22,7 @@ constexpr int FLOE_BLOCK = 256;   // upper bound of the per-floe kernels' block 
 constexpr int XCD_COUNT = 8;      // accelerator complex dies of an MI355X
 constexpr int NQUEUE = 4;         // work queues: contact, bond, wall, dense search
 constexpr int QN_STRIDE = 64;     // ints between queue counters (256 B)
+constexpr int SCAN_TILE = 1024;   // positions per tile of the superset offset scan
 
 // Persistent per-floe record (96 bytes), indexed by floe id.
 struct FloeRec {
@@ -102,6 +103,21 @@ struct DS {
     uint8_t* wflag;            // [S] 1: the floe touched a wall this step
     int32_t* ovw;              // [S]
     uint8_t* cls;              // [S] CLS_* bits
+    // Candidate supersets: every floe within its query radius plus the
+    // member's skin at the last build, (id << 32 | pos), floe k's at
+    // sup[soff[k] .. soff[k] + scnt[k]).  They are rebuilt (cell binning,
+    // sort, search) only on steps that start with *rebuild set: after a
+    // change of members, or once some floe has moved more than 0.45 skin from
+    // where it was at the build (x0); in between, each step's candidate lists
+    // are the superset entries within the query radius, the same lists a full
+    // search gives (radii only shrink).
+    int64_t* sup;              // [sup_cap]
+    int64_t sup_cap;
+    int32_t* scnt;             // [S] by pos
+    int64_t* soff;             // [S] by pos
+    int64_t* sblk;             // [S / SCAN_TILE + 2] scan of per-tile totals
+    double2* x0;               // [S] by storage slot
+    int32_t* rebuild;
     // Work queues (contact, bond, wall, dense search), each in XCD_COUNT
     // segments: the candidate search's blocks on XCD x cover positions
     // [x qseg, (x+1) qseg) and queue them into segment x, at

@@ -265,36 +265,43 @@ __device__ __forceinline__ void acc_init(double* a) {
 __global__ void k_bin(DS S) {
     int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= S.N || *S.err) return;
-    int m = S.member_of[f];
+    const int m = S.member_of[f];
+    const bool live = m >= 0 && member_live(S.members[m]);
+    if (live && f == S.members[m].first) {
+        S.nseeds[m] = 0; S.netab[m] = 0; S.nwtab[m] = 0;
+        S.stats[4 * m + 2] = 0; S.stats[4 * m + 3] = 0;
+    }
+    if (!*S.rebuild) return;      // the cell order of the last build stands
     S.cval[f] = (uint32_t)f;
     // floes of free slots and of members past their stop step sort last and
     // take no part in the step
-    if (m < 0 || !member_live(S.members[m])) { S.ckey[f] = (uint32_t)S.ncells; return; }
+    if (!live) { S.ckey[f] = (uint32_t)S.ncells; return; }
     const DevParams& P = S.params[m];
     double x = S.rec[f].x, y = S.rec[f].y;
     int cx = cell_coord(x, P.cell, P.gx), cy = cell_coord(y, P.cell, P.gy);
     S.ckey[f] = (uint32_t)(P.cell_base + cy * P.gx + cx);
-    if (f == S.members[m].first) {
-        S.nseeds[m] = 0; S.netab[m] = 0; S.nwtab[m] = 0;
-        S.stats[4 * m + 2] = 0; S.stats[4 * m + 3] = 0;
-    }
 }
 
+// Per step: the queue counters (the dense-search list, queue 3, only on build
+// steps: it lists positions, which change only then); on build steps the cell
+// ranges too.
 __global__ void k_clear_cells(DS S) {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c > S.ncells || *S.err) return;
-    S.crange[c] = make_int2(0, 0);
-    if (c < NQUEUE * XCD_COUNT) S.qn[c * QN_STRIDE] = 0;
+    if (*S.err) return;
+    const bool build = *S.rebuild != 0;
+    if (c < NQUEUE * XCD_COUNT && (build || c < 3 * XCD_COUNT)) S.qn[c * QN_STRIDE] = 0;
+    if (build && c <= S.ncells) S.crange[c] = make_int2(0, 0);
 }
 
-// cell ranges over the sorted keys, the sorted floe records and the wind at
-// each floe (winds_currents, forcing.f90:1-29)
+// The floe records in the current order and the wind at each floe
+// (winds_currents, forcing.f90:1-29); on build steps the cell ranges over the
+// sorted keys.
 __global__ void k_ranges(DS S) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= S.N || *S.err) return;
-    uint32_t key = S.ckey[k];
     int f = (int)S.cval[k];
-    int m = key < (uint32_t)S.ncells ? S.member_of[f] : -1;
+    int m = S.member_of[f];
+    if (m >= 0 && !member_live(S.members[m])) m = -1;
     const FloeRec fr = S.rec[f];
     GeoRec g;
     g.x = fr.x; g.y = fr.y; g.r = fr.r; g.id = S.gid_of[f]; g.member = m;
@@ -306,8 +313,11 @@ __global__ void k_ranges(DS S) {
     S.pos_of[f] = k;
     double ua = 0.0, va = 0.0;
     if (m >= 0) {
-        if (k == 0 || S.ckey[k - 1] != key) S.crange[key].x = k;
-        if (k == S.N - 1 || S.ckey[k + 1] != key) S.crange[key].y = k + 1;
+        if (*S.rebuild) {
+            const uint32_t key = S.ckey[k];
+            if (k == 0 || S.ckey[k - 1] != key) S.crange[key].x = k;
+            if (k == S.N - 1 || S.ckey[k + 1] != key) S.crange[key].y = k + 1;
+        }
         const DevParams& P = S.params[m];
         ua = P.ua; va = P.va;
         if (P.wind_grid >= 0) sample_wind(S.wgrids[P.wind_grid], S.wpool, fr.x, fr.y, ua, va);
@@ -379,44 +389,21 @@ __device__ __forceinline__ void cand_finish(const DS& S, int k, int fs, const De
     }
 }
 
-// Neighbour search of one floe by one thread: candidate lists with
-// insertion into the slot-major rows, sheltering minima, classification and
-// the per-step resets of the floe's rows.  SPAN is the member's span (1 for
-// sparse members, 2 for dense ones).  Every cell range of the block is read
-// at once, the rows' positions, taken as one sequence, SCAN_BATCH records at
-// a time, and the head of each candidate list is sorted in registers, so
-// that the search waits on few dependent loads.
 constexpr int SCAN_BATCH = 4;
 constexpr int LIST_REGS = 4;
 
-template <bool MIXED, int SPAN>
-__device__ __forceinline__ void cand_thread(const DS& S, int k, const GeoRec& me, bool& qc,
-                                            bool& qb, bool& qw) {
-    using T = typename Prec<MIXED>::T;
+// Candidate superset of one floe by one thread over its (2 SPAN + 1)^2 cell
+// block: every floe j within (j > f ? r(f) : r(j)) + rtree + skin.  The cells
+// of one row of the block are consecutive in the cell order, so each row is
+// one contiguous range of positions; every cell range is read at once
+// (indices clamped into the grid, cells outside it masked out afterwards) and
+// the rows, taken as one sequence, SCAN_BATCH records at a time.  FILL
+// writes the entries at `out`; either way the count is returned.
+template <int SPAN, bool FILL>
+__device__ __forceinline__ int sup_thread(const DS& S, int k, const GeoRec& me, const DevParams& P,
+                                          int64_t* out) {
     constexpr int R = 2 * SPAN + 1;
-    const int m = me.member;
-    const DevParams& P = S.params[m];
-    const double rad = me.r + P.rtree;
-    const int f = me.id;
-    const int first = S.members[m].first;
-    const bool bond_step = S.members[m].tstep == 1 && P.restart != 1 && P.cohesion;
     const int cx = cell_coord(me.x, P.cell, P.gx), cy = cell_coord(me.y, P.cell, P.gy);
-    const ShelRec msh = S.shel[k];
-    const T uwi = T(P.uw), vwi = T(P.vw);
-    T sha = T(1), shw = T(1);
-    const int fs = S.cval[k];
-    qc = S.hcnt_old[fs] > 0;
-    qb = S.bcnt_old[fs] > 0;
-    qw = me.y - me.r < 0 || me.y + me.r > P.ny || me.x - me.r < 0 || me.x + me.r > P.nx;
-    int no = 0, ni = 0;
-    uint32_t over = 0;
-    const int64_t SS = S.S;
-    int64_t* oj = S.oj;
-    int64_t* ik = S.ik;
-    // the cells of one row of the block are consecutive in the cell order:
-    // each row is one contiguous range of positions.  Cell indices are
-    // clamped into the grid so that every load is valid; cells outside it
-    // are masked out afterwards.
     int2 rg[R][R];
 #pragma unroll
     for (int t = 0; t < R; ++t) {
@@ -444,17 +431,9 @@ __device__ __forceinline__ void cand_thread(const DS& S, int k, const GeoRec& me
         rlen[t] = lo < hi ? hi - lo : 0;
         L += rlen[t];
     }
-    // the first LIST_REGS entries of each list stay in registers while the
-    // lists are built (sorted; empty slots hold keys above every entry), the
-    // rest are sorted in their slots in HBM
-    double rd[LIST_REGS];
-    int64_t re[LIST_REGS], ri[LIST_REGS];
-#pragma unroll
-    for (int t = 0; t < LIST_REGS; ++t) {
-        rd[t] = __longlong_as_double(0x7ff0000000000000ll);
-        re[t] = INT64_MAX;
-        ri[t] = INT64_MAX;
-    }
+    const int f = me.id;
+    const double rad = me.r + P.rtree + P.skin, reach = P.rtree + P.skin;
+    int n = 0;
     for (int v0 = 0; v0 < L; v0 += SCAN_BATCH) {
         GeoRec g[SCAN_BATCH];
         int qq[SCAN_BATCH];
@@ -475,6 +454,71 @@ __device__ __forceinline__ void cand_thread(const DS& S, int k, const GeoRec& me
         for (int u = 0; u < SCAN_BATCH; ++u) {
             const int q = qq[u];
             if (v0 + u >= L || q == k) continue;
+            const double dx = me.x - g[u].x, dy = me.y - g[u].y;
+            const double thr = g[u].id > f ? rad : g[u].r + reach;
+            // a superset: the margin only adds entries
+            if (dx * dx + dy * dy <= thr * thr * (1.0 + 1e-12)) {
+                if (FILL) out[n] = ent(g[u].id, q);
+                ++n;
+            }
+        }
+    }
+    return n;
+}
+
+// Neighbour search of one floe by one thread: its candidate lists are the
+// entries of its superset within the query radius, inserted into the
+// slot-major rows, then sheltering minima, classification and the per-step
+// resets of the floe's rows.  Superset entries are read SCAN_BATCH at a time
+// and the head of each candidate list is sorted in registers, so that the
+// search waits on few dependent loads.
+template <bool MIXED>
+__device__ __forceinline__ void cand_thread(const DS& S, int k, const GeoRec& me, bool& qc,
+                                            bool& qb, bool& qw) {
+    using T = typename Prec<MIXED>::T;
+    const int m = me.member;
+    const DevParams& P = S.params[m];
+    const double rad = me.r + P.rtree;
+    const int f = me.id;
+    const int first = S.members[m].first;
+    const bool bond_step = S.members[m].tstep == 1 && P.restart != 1 && P.cohesion;
+    const ShelRec msh = S.shel[k];
+    const T uwi = T(P.uw), vwi = T(P.vw);
+    T sha = T(1), shw = T(1);
+    const int fs = S.cval[k];
+    qc = S.hcnt_old[fs] > 0;
+    qb = S.bcnt_old[fs] > 0;
+    qw = me.y - me.r < 0 || me.y + me.r > P.ny || me.x - me.r < 0 || me.x + me.r > P.nx;
+    int no = 0, ni = 0;
+    uint32_t over = 0;
+    const int64_t SS = S.S;
+    int64_t* oj = S.oj;
+    int64_t* ik = S.ik;
+    const int L = S.scnt[k];
+    const int64_t* se = S.sup + S.soff[k];
+    // the first LIST_REGS entries of each list stay in registers while the
+    // lists are built (sorted; empty slots hold keys above every entry), the
+    // rest are sorted in their slots in HBM
+    double rd[LIST_REGS];
+    int64_t re[LIST_REGS], ri[LIST_REGS];
+#pragma unroll
+    for (int t = 0; t < LIST_REGS; ++t) {
+        rd[t] = __longlong_as_double(0x7ff0000000000000ll);
+        re[t] = INT64_MAX;
+        ri[t] = INT64_MAX;
+    }
+    for (int v0 = 0; v0 < L; v0 += SCAN_BATCH) {
+        GeoRec g[SCAN_BATCH];
+        int qq[SCAN_BATCH];
+#pragma unroll
+        for (int u = 0; u < SCAN_BATCH; ++u)
+            qq[u] = ent_pos(se[min(v0 + u, L - 1)]);   // the last entry stands in past the end
+#pragma unroll
+        for (int u = 0; u < SCAN_BATCH; ++u) g[u] = S.geo[qq[u]];
+#pragma unroll
+        for (int u = 0; u < SCAN_BATCH; ++u) {
+            const int q = qq[u];
+            if (v0 + u >= L) continue;
             const GeoRec* o = &g[u];
             const double dx = me.x - o->x, dy = me.y - o->y;
             const double dist2 = dx * dx + dy * dy;
@@ -599,10 +643,92 @@ __device__ __forceinline__ void cand_thread(const DS& S, int k, const GeoRec& me
     cand_finish<MIXED>(S, k, fs, P, no, ni, double(sha), double(shw), qc, qb, 0, 1);
 }
 
+// Superset build, thread per floe (members that are not dense; they have
+// span 1, godar_member_add): counts (FILL false) or entries and the build
+// positions x0 (FILL true).  The count pass lists the dense members' floes in
+// queue 3 for k_sup_dense and k_cand_dense.  Only on build steps.
+template <bool FILL>
+__global__ __launch_bounds__(FLOE_BLOCK) void k_sup(DS S) {
+    if (!*S.rebuild || *S.err) return;
+    const int k = xcd_block() * blockDim.x + threadIdx.x;
+    bool qd = false;
+    if (k < S.N) {
+        const GeoRec me = S.geo[k];
+        if (me.member < 0) {
+            if (!FILL) S.scnt[k] = 0;
+        } else {
+            const DevParams& P = S.params[me.member];
+            if (P.dense) {
+                qd = !FILL;
+            } else if (FILL) {
+                sup_thread<1, true>(S, k, me, P, S.sup + S.soff[k]);
+                S.x0[S.cval[k]] = make_double2(me.x, me.y);
+            } else {
+                S.scnt[k] = sup_thread<1, false>(S, k, me, P, nullptr);
+            }
+        }
+    }
+    enqueue(S, qd, 3, k);
+}
+
+// Superset offsets: per-tile totals, their exclusive scan (one workgroup;
+// the total must fit the superset pool), then offsets within each tile.
+__device__ __forceinline__ int64_t block_exclusive_scan(int64_t v, int64_t* tmp, int64_t& total) {
+    const int t = (int)threadIdx.x, n = (int)blockDim.x;
+    tmp[t] = v;
+    __syncthreads();
+    for (int off = 1; off < n; off <<= 1) {
+        const int64_t add = t >= off ? tmp[t - off] : 0;
+        __syncthreads();
+        tmp[t] += add;
+        __syncthreads();
+    }
+    total = tmp[n - 1];
+    const int64_t incl = tmp[t];
+    __syncthreads();
+    return incl - v;
+}
+
+__global__ __launch_bounds__(SCAN_TILE) void k_scan_tiles(DS S) {
+    __shared__ int64_t tmp[SCAN_TILE];
+    if (!*S.rebuild || *S.err) return;
+    const int k = blockIdx.x * SCAN_TILE + threadIdx.x;
+    int64_t tot;
+    block_exclusive_scan(k < S.N ? S.scnt[k] : 0, tmp, tot);
+    if (threadIdx.x == 0) S.sblk[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(SCAN_TILE) void k_scan_total(DS S, int ntiles) {
+    __shared__ int64_t tmp[SCAN_TILE];
+    if (!*S.rebuild || *S.err) return;
+    int64_t base = 0;
+    for (int t0 = 0; t0 < ntiles; t0 += SCAN_TILE) {
+        const int t = t0 + (int)threadIdx.x;
+        const int64_t v = t < ntiles ? S.sblk[t] : 0;
+        int64_t tot;
+        const int64_t ex = block_exclusive_scan(v, tmp, tot);
+        if (t < ntiles) S.sblk[t] = base + ex;
+        base += tot;
+    }
+    if (threadIdx.x == 0) {
+        S.sblk[ntiles] = base;
+        if (base > S.sup_cap) set_err(S, ERR_SUP_CAP);
+    }
+}
+
+__global__ __launch_bounds__(SCAN_TILE) void k_scan_offsets(DS S) {
+    __shared__ int64_t tmp[SCAN_TILE];
+    if (!*S.rebuild || *S.err) return;
+    const int k = blockIdx.x * SCAN_TILE + threadIdx.x;
+    int64_t tot;
+    const int64_t ex = block_exclusive_scan(k < S.N ? S.scnt[k] : 0, tmp, tot);
+    if (k < S.N) S.soff[k] = S.sblk[blockIdx.x] + ex;
+}
+
 template <bool MIXED>
 __global__ __launch_bounds__(FLOE_BLOCK) void k_cand(DS S) {
     const int k = xcd_block() * blockDim.x + threadIdx.x;
-    bool qc = false, qb = false, qw = false, qd = false;
+    bool qc = false, qb = false, qw = false;
     bool live = k < S.N && !*S.err;
     GeoRec me;
     if (live) {
@@ -613,15 +739,11 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_cand(DS S) {
             live = false;
         }
     }
-    if (live) {
-        if (S.params[me.member].dense) qd = true;   // k_cand_dense searches it
-        // members that are not dense have span 1 (godar_member_add)
-        else cand_thread<MIXED, 1>(S, k, me, qc, qb, qw);
-    }
+    // floes of dense members: k_cand_dense
+    if (live && !S.params[me.member].dense) cand_thread<MIXED>(S, k, me, qc, qb, qw);
     enqueue(S, qc, 0, k);
     enqueue(S, qb, 1, k);
     enqueue(S, qw, 2, k);
-    enqueue(S, qd, 3, k);
 }
 
 // Dense members (many floes per cell): one wavefront per floe.  The lanes
@@ -681,6 +803,62 @@ __device__ __forceinline__ int pow2_at_least(int n) {
     return p;
 }
 
+// Superset build of the dense members' floes (queue 3), one wavefront per
+// floe over its (2 span + 1)^2 cell block, 64 entries at a time: counts, or
+// entries in scan order and the build positions.  Only on build steps.
+template <bool FILL>
+__global__ __launch_bounds__(64) void k_sup_dense(DS S) {
+    if (!*S.rebuild || *S.err) return;
+    const int lane = threadIdx.x;
+    const QueueView qv = queue_view(S, 3);
+    const int nq = qv.total;
+    const int nb = (int)(gridDim.x / XCD_COUNT);
+    const int xcd = (int)(blockIdx.x % XCD_COUNT), bi = (int)(blockIdx.x / XCD_COUNT);
+    const int per = (nq + XCD_COUNT - 1) / XCD_COUNT;
+    const int t1 = min(nq, xcd * per + per);
+    const uint64_t lt = lane ? ((~0ull) >> (64 - lane)) : 0ull;
+    for (int t = xcd * per + bi; t < t1; t += nb) {
+        const int k = queue_at(S, 3, qv, t);
+        const GeoRec me = S.geo[k];
+        const DevParams& P = S.params[me.member];
+        const double rad = me.r + P.rtree + P.skin, reach = P.rtree + P.skin;
+        const int f = me.id;
+        const int sp = P.span;
+        const int cx = cell_coord(me.x, P.cell, P.gx), cy = cell_coord(me.y, P.cell, P.gy);
+        int64_t* out = FILL ? S.sup + S.soff[k] : nullptr;
+        int n = 0;
+        const int a0 = max(cx - sp, 0), a1 = min(cx + sp, P.gx - 1);
+        for (int b = cy - sp; b <= cy + sp; ++b) {
+            if (b < 0 || b >= P.gy) continue;
+            // the cells of one row are consecutive in the cell order
+            int lo = INT_MAX, hi = 0;
+            for (int a = a0; a <= a1; ++a) {
+                const int2 rg = S.crange[P.cell_base + b * P.gx + a];
+                if (rg.y > rg.x) { lo = min(lo, rg.x); hi = max(hi, rg.y); }
+            }
+            for (int base = lo; base < hi; base += 64) {
+                const int q = base + lane;
+                bool in = false;
+                int oid = 0;
+                if (q < hi && q != k) {
+                    const GeoRec o = S.geo[q];
+                    const double dx = me.x - o.x, dy = me.y - o.y;
+                    oid = o.id;
+                    const double thr = o.id > f ? rad : o.r + reach;
+                    in = dx * dx + dy * dy <= thr * thr * (1.0 + 1e-12);
+                }
+                const uint64_t mi = __ballot(in);
+                if (FILL && in) out[n + (int)__popcll(mi & lt)] = ent(oid, q);
+                n += (int)__popcll(mi);
+            }
+        }
+        if (lane == 0) {
+            if (FILL) S.x0[S.cval[k]] = make_double2(me.x, me.y);
+            else S.scnt[k] = n;
+        }
+    }
+}
+
 template <bool MIXED>
 __global__ __launch_bounds__(64) void k_cand_dense(DS S) {
     using T = typename Prec<MIXED>::T;
@@ -701,62 +879,54 @@ __global__ __launch_bounds__(64) void k_cand_dense(DS S) {
         const int k = queue_at(S, 3, qv, t);
         const GeoRec me = S.geo[k];
         const int m = me.member;
+        if (m < 0) continue;        // member past its stop step (k_cand cleared its counts)
         const DevParams& P = S.params[m];
         const double rad = me.r + P.rtree;
         const int f = me.id;
-        const int sp = P.span;
-        const int cx = cell_coord(me.x, P.cell, P.gx), cy = cell_coord(me.y, P.cell, P.gy);
-        // 1. scan: the cells of one row of the block are consecutive in the
-        //    cell order, so each row is one contiguous range of positions
+        // 1. the superset entries within the query radius, 64 at a time
         int no = 0, ni = 0;
-        const int a0 = max(cx - sp, 0), a1 = min(cx + sp, P.gx - 1);
-        for (int b = cy - sp; b <= cy + sp; ++b) {
-            if (b < 0 || b >= P.gy) continue;
-            int lo = INT_MAX, hi = 0;
-            for (int a = a0; a <= a1; ++a) {
-                const int2 rg = S.crange[P.cell_base + b * P.gx + a];
-                if (rg.y > rg.x) { lo = min(lo, rg.x); hi = max(hi, rg.y); }
+        const int L = S.scnt[k];
+        const int64_t* sup = S.sup + S.soff[k];
+        for (int base = 0; base < L; base += 64) {
+            const int v = base + lane;
+            bool vout = false, vin = false;
+            int oid = 0, q = 0;
+            double d = 0.0;
+            if (v < L) {
+                q = ent_pos(sup[v]);
+                const GeoRec o = S.geo[q];
+                const double dx = me.x - o.x, dy = me.y - o.y;
+                const double dist2 = dx * dx + dy * dy;
+                oid = o.id;
+                const double thr = o.id > f ? rad : o.r + P.rtree;
+                // far beyond the radius: no square root needed
+                if (dist2 <= thr * thr * (1.0 + 1e-12)) {
+                    d = sqrt(dist2);
+                    if (o.id > f) vout = d <= thr;
+                    else vin = d <= thr;
+                }
             }
-            for (int base = lo; base < hi; base += 64) {
-                const int q = base + lane;
-                bool vout = false, vin = false;
-                int oid = 0;
-                double d = 0.0;
-                if (q < hi && q != k) {
-                    const GeoRec o = S.geo[q];
-                    const double dx = me.x - o.x, dy = me.y - o.y;
-                    const double dist2 = dx * dx + dy * dy;
-                    oid = o.id;
-                    const double thr = o.id > f ? rad : o.r + P.rtree;
-                    // far beyond the radius: no square root needed
-                    if (dist2 <= thr * thr * (1.0 + 1e-12)) {
-                        d = sqrt(dist2);
-                        if (o.id > f) vout = d <= thr;
-                        else vin = d <= thr;
-                    }
+            const uint64_t mo = __ballot(vout), mi = __ballot(vin);
+            if (vout) {
+                const int slot = no + (int)__popcll(mo & lt);
+                if (slot < DENSE_CAP) {
+                    sd[slot] = (uint64_t)__double_as_longlong(d);
+                    se[slot] = ent(oid, q);
                 }
-                const uint64_t mo = __ballot(vout), mi = __ballot(vin);
-                if (vout) {
-                    const int slot = no + (int)__popcll(mo & lt);
-                    if (slot < DENSE_CAP) {
-                        sd[slot] = (uint64_t)__double_as_longlong(d);
-                        se[slot] = ent(oid, q);
-                    }
-                }
-                if (vin) {
-                    const int slot = ni + (int)__popcll(mi & lt);
-                    if (slot < DENSE_CAP) si[slot] = ent(oid, q);
-                }
-                no += (int)__popcll(mo);
-                ni += (int)__popcll(mi);
             }
+            if (vin) {
+                const int slot = ni + (int)__popcll(mi & lt);
+                if (slot < DENSE_CAP) si[slot] = ent(oid, q);
+            }
+            no += (int)__popcll(mo);
+            ni += (int)__popcll(mi);
         }
         __syncthreads();
         if (no > DENSE_CAP || ni > DENSE_CAP) {
             // rare: more candidates than LDS holds, the thread search does it
             if (lane == 0) {
                 bool c2, b2, w2;
-                cand_thread<MIXED, 2>(S, k, me, c2, b2, w2);
+                cand_thread<MIXED>(S, k, me, c2, b2, w2);
                 if (c2) enqueue_one(S, 0, k);
                 if (b2) enqueue_one(S, 1, k);
                 if (w2) enqueue_one(S, 2, k);
@@ -1128,6 +1298,9 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_bond_gather(DS S) {
 template <bool MIXED>
 __global__ __launch_bounds__(FLOE_BLOCK) void k_walls(DS S) {
     if (*S.err) return;
+    // no later kernel of the step reads the build flag; k_final raises it
+    // again when a floe has moved too far for the supersets
+    if (blockIdx.x == 0 && threadIdx.x == 0) *S.rebuild = 0;
     const int k = queue_entry(S, 2);
     if (k < 0) return;
     const GeoRec me = S.geo[k];
@@ -1838,6 +2011,15 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_final(DS S) {
     out.r = F.r; out.h = F.h; out.mass = F.mass; out.inertia = F.inertia; out.hfa = F.hfa; out.hfw = F.hfw;
     out.theta = S.kin[k].theta + om * P.dt;
     S.rec[f] = out;
+    {
+        // next step builds the supersets anew if this floe has moved more
+        // than 0.45 skin since the last build (a pair not in a superset was
+        // more than its query radius + skin apart then, so it stays outside
+        // the query radius while both floes stay within half the skin)
+        const double2 p0 = S.x0[f];
+        const double ex = out.x - p0.x, ey = out.y - p0.y, lim = 0.45 * P.skin;
+        if (ex * ex + ey * ey > lim * lim) *S.rebuild = 1;
+    }
     if (S.record_forces) {
         double tp = (a[A_TAC] != 0.0 ? a[A_PC] / a[A_TAC] : 0.0)
                   + (a[A_TAB] != 0.0 ? a[A_PB] / a[A_TAB] : 0.0)
@@ -1894,6 +2076,10 @@ __global__ void k_totals(DS S, int64_t* out) {
 }
 
 // explicit instantiations used by the host launcher
+template __global__ void k_sup<false>(DS);
+template __global__ void k_sup<true>(DS);
+template __global__ void k_sup_dense<false>(DS);
+template __global__ void k_sup_dense<true>(DS);
 template __global__ void k_cand_dense<false>(DS);
 template __global__ void k_cand_dense<true>(DS);
 template __global__ void k_cand<false>(DS);

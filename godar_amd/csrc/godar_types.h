@@ -31,8 +31,9 @@ struct DevParams {
     uint64_t bond_seed;
     int32_t wind_grid;     // -1: uniform ua/va, else index into the wind-grid table
     int32_t dense;         // 1: many floes per cell; one wavefront searches each floe
-    int32_t span;          // cells searched on each side (cell >= (max r + rtree) / span)
+    int32_t span;          // cells searched on each side (cell >= (max r + rtree + skin) / span)
     int32_t pad2_;
+    double skin;           // candidate supersets reach this far beyond the query radius
 };
 
 struct WindGridDesc {
@@ -64,6 +65,7 @@ enum : uint32_t {
     ERR_REPAIR_CAP = 1u << 4,
     ERR_SEED_CAP   = 1u << 5,
     ERR_NONFINITE  = 1u << 6,
+    ERR_SUP_CAP    = 1u << 7,
 };
 
 // Bond record (owner = lower floe index), bonds.f90 state of pair (j,i).

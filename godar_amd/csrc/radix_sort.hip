@@ -7,8 +7,8 @@ namespace godar {
 __global__ __launch_bounds__(RS_THREADS) void rs_hist_kernel(const uint32_t* __restrict__ keys,
                                                              int64_t n, int shift, int ntiles,
                                                              uint32_t* __restrict__ hist,
-                                                             const uint32_t* err) {
-    if (err && *err) return;
+                                                             const uint32_t* err, const int32_t* run) {
+    if ((err && *err) || (run && !*run)) return;
     __shared__ uint32_t h[256];
     int t = threadIdx.x;
     h[t] = 0;
@@ -26,8 +26,9 @@ __global__ __launch_bounds__(RS_THREADS) void rs_hist_kernel(const uint32_t* __r
 // Exclusive scan of each digit's row of per-tile counts (one workgroup per
 // digit); the row total goes to dtot[digit].
 __global__ __launch_bounds__(256) void rs_scan_kernel(uint32_t* __restrict__ hist, int ntiles,
-                                                      uint32_t* __restrict__ dtot, const uint32_t* err) {
-    if (err && *err) return;
+                                                      uint32_t* __restrict__ dtot, const uint32_t* err,
+                                                      const int32_t* go) {
+    if ((err && *err) || (go && !*go)) return;
     __shared__ uint32_t part[256];
     const int t = threadIdx.x;
     uint32_t* row = hist + (int64_t)blockIdx.x * ntiles;
@@ -58,8 +59,8 @@ __global__ __launch_bounds__(256) void rs_scan_kernel(uint32_t* __restrict__ his
 __global__ __launch_bounds__(RS_THREADS) void rs_scatter_kernel(
     const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin, uint32_t* __restrict__ kout,
     uint32_t* __restrict__ vout, int64_t n, int shift, int ntiles, const uint32_t* __restrict__ hist,
-    const uint32_t* __restrict__ dtot, const uint32_t* err) {
-    if (err && *err) return;
+    const uint32_t* __restrict__ dtot, const uint32_t* err, const int32_t* run) {
+    if ((err && *err) || (run && !*run)) return;
     constexpr int NW = RS_THREADS / 64;
     __shared__ uint32_t base_off[256];
     __shared__ uint32_t wcount[NW][256];
@@ -139,8 +140,19 @@ void radix_free(RadixScratch& s) {
     s = RadixScratch{};
 }
 
+// Result of an odd number of passes back into the caller's arrays.
+__global__ __launch_bounds__(256) void rs_copy_kernel(const uint32_t* __restrict__ ka,
+                                                      const uint32_t* __restrict__ va,
+                                                      uint32_t* __restrict__ keys,
+                                                      uint32_t* __restrict__ vals, int64_t n,
+                                                      const uint32_t* err, const int32_t* run) {
+    if ((err && *err) || (run && !*run)) return;
+    const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (p < n) { keys[p] = ka[p]; vals[p] = va[p]; }
+}
+
 hipError_t radix_sort_pairs(RadixScratch& s, uint32_t* keys, uint32_t* vals, int64_t n, int bits,
-                            const uint32_t* err, hipStream_t stream) {
+                            const uint32_t* err, const int32_t* run, hipStream_t stream) {
     if (n <= 0) return hipSuccess;
     if (n > s.capacity) return hipErrorInvalidValue;
     int passes = (bits + 7) / 8;
@@ -150,20 +162,17 @@ hipError_t radix_sort_pairs(RadixScratch& s, uint32_t* keys, uint32_t* vals, int
     for (int ps = 0; ps < passes; ++ps) {
         int shift = ps * 8;
         hipLaunchKernelGGL(rs_hist_kernel, dim3(ntiles), dim3(RS_THREADS), 0, stream, ka, n, shift,
-                           ntiles, s.hist, err);
+                           ntiles, s.hist, err, run);
         hipLaunchKernelGGL(rs_scan_kernel, dim3(256), dim3(256), 0, stream, s.hist, ntiles, s.dtot,
-                           err);
+                           err, run);
         hipLaunchKernelGGL(rs_scatter_kernel, dim3(ntiles), dim3(RS_THREADS), 0, stream, ka, va, kb,
-                           vb, n, shift, ntiles, s.hist, s.dtot, err);
+                           vb, n, shift, ntiles, s.hist, s.dtot, err, run);
         uint32_t* tk = ka; ka = kb; kb = tk;
         uint32_t* tv = va; va = vb; vb = tv;
     }
-    if (ka != keys) {
-        hipError_t e = hipMemcpyAsync(keys, ka, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, stream);
-        if (e != hipSuccess) return e;
-        e = hipMemcpyAsync(vals, va, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, stream);
-        if (e != hipSuccess) return e;
-    }
+    if (ka != keys)
+        hipLaunchKernelGGL(rs_copy_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, ka, va,
+                           keys, vals, n, err, run);
     return hipGetLastError();
 }
 

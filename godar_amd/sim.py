@@ -162,6 +162,11 @@ class Context:
         """Members added afterwards examining >= floes per search use one wavefront per floe."""
         self._chk(self.L.godar_set_dense_search(self.h, int(floes)))
 
+    def set_skin(self, frac: float = 0.1):
+        """Superset skin of members added afterwards, x (max r + rtree).
+        Results do not depend on it."""
+        self._chk(self.L.godar_set_skin(self.h, float(frac)))
+
     def set_repair(self, lanes: int = 64, pending: int = 1024):
         """Ridging repair: events evaluated at once (1: one at a time) and
         pending events kept on chip.  Results do not depend on either."""

@@ -196,6 +196,12 @@ int godar_set_dense_search(godar_ctx *ctx, int32_t floes);
  * memory (1-1024, default 1024) before the member continues one event at a
  * time.  Results do not depend on either. */
 int godar_set_repair(godar_ctx *ctx, int32_t lanes, int32_t pending);
+/* Candidate supersets of members added afterwards reach `frac` x (max r +
+ * rtree) beyond each floe's query radius (0-1, default 0.1).  The cell sort
+ * and search run only on steps after some floe has moved more than 0.45 of
+ * that skin (or after members change); the other steps take their candidate
+ * lists from the supersets.  Results do not depend on it. */
+int godar_set_skin(godar_ctx *ctx, double frac);
 int godar_get_stage_blocks(const godar_ctx *ctx, int32_t *threads);   /* GODAR_NSTAGES entries */
 /* Autotuner: for each tunable stage in turn, time `trial` real steps of the
  * current members with each block size and keep the fastest.  The members

@@ -198,6 +198,40 @@ def test_repair_modes_bitwise():
                     assert np.array_equal(ta, tb), (prec, mode)
 
 
+def test_superset_skin_bitwise():
+    """Candidate lists taken from skinned supersets (rebuilt only after a floe
+    has moved far enough) give the same trajectory, bitwise, as a full cell
+    search every step (skin 0): sparse and dense (wavefront) searches, moving
+    fields with ridging, FP64 and mixed."""
+    fa = kick(g.config0_member(41, 600), 3, speed=4.0)   # rebuilds every few steps at small skins
+    fd, pd, _ = load("dense64.npz")
+    pa = config0_params(fa)
+    for prec in ("fp64", "mixed"):
+        runs = {}
+        for skin in (0.0, 0.02, 0.1, 0.5):
+            ctx = _ctx(fa.n + 2 * fd.n, 3, prec)
+            ctx.set_skin(skin)
+            ms = [ctx.add_field(pa, fa), ctx.add_field(pd, fd)]
+            ctx.set_dense_search(0)          # this member: wavefront search
+            ms.append(ctx.add_field(pd, fd))
+            snaps = []
+            for t in range(1, 41):
+                ctx.step(1)
+                if t in (1, 7, 40):
+                    snaps.append([(ctx.floes(m), _pairs(ctx, m, "candidate"), ctx.forces(m)["tfx"])
+                                  for m in ms])
+            runs[skin] = snaps
+            ctx.close()
+        ref = runs[0.0]
+        for skin, snaps in runs.items():
+            for sa, sb in zip(snaps, ref):
+                for (fa_, ca, ta), (fb_, cb, tb) in zip(sa, sb):
+                    for key in STATE:
+                        assert np.array_equal(fa_[key], fb_[key]), (prec, skin, key)
+                    assert ca == cb, (prec, skin)
+                    assert np.array_equal(ta, tb), (prec, skin)
+
+
 def test_stiff_bond_breaking_fp64():
     f = kick(squeeze(packed_box(64, 5), 0.995), 2, speed=0.5)
     p = config0_params(f, sigmanc_crit=1e5, e_modul=6e9, eb=6e9, bond_lim=50.0, bond_frac=1.0)
