@@ -200,6 +200,12 @@ void force_rebuild(godar_ctx* c) {
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, c->stream, c->d.rebuild, 1, 1);
 }
 
+// The next step copies the floe records into the current order anew (the
+// host changed what the copies hold, e.g. the wind).
+void force_refresh(godar_ctx* c) {
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, c->stream, c->d.rebuild + 1, 1, 1);
+}
+
 int32_t qseg_of(const godar_ctx* c) {
     const int bc = c->sblock[3];
     const int gc = ((std::max(c->N_used, 1) + bc - 1) / bc + XCD_COUNT - 1) / XCD_COUNT * XCD_COUNT;
@@ -586,14 +592,14 @@ int godar_create(int64_t max_floes, int32_t max_members, int32_t precision, goda
     ACHK(dalloc(&d.soff, S));
     ACHK(dalloc(&d.sblk, S / SCAN_TILE + 2));
     ACHK(dalloc(&d.x0, S));
-    ACHK(dalloc(&d.rebuild, 1));
+    ACHK(dalloc(&d.rebuild, 2));
     ACHK(radix_alloc(c->rs, S));
 #undef ACHK
     (void)hipMemsetAsync(d.err, 0, sizeof(uint32_t), c->stream);
     (void)hipMemsetAsync(d.steps_done, 0, sizeof(int64_t), c->stream);
     (void)hipMemsetAsync(d.scnt, 0, sizeof(int32_t) * S, c->stream);
     (void)hipMemsetAsync(d.qn, 0, sizeof(int32_t) * NQUEUE * XCD_COUNT * QN_STRIDE, c->stream);
-    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, c->stream, d.rebuild, 1, 1);
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, c->stream, d.rebuild, 2, 1);
     (void)hipMemsetAsync(d.dirty, 0, S, c->stream);
     (void)hipMemsetAsync(d.affected, 0, S, c->stream);
     (void)hipMemsetAsync(d.stats, 0, sizeof(int64_t) * 4 * max_members, c->stream);
@@ -1073,6 +1079,7 @@ int godar_set_forcing(godar_ctx* c, int32_t m, double ua, double va, double uw, 
     d.ua = ua; d.va = va; d.uw = uw; d.vw = vw;
     HIPCHK(c, hipMemcpyAsync((void*)(c->d.params + m), &d, sizeof(DevParams), hipMemcpyHostToDevice,
                              c->stream));
+    force_refresh(c);
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return GODAR_OK;
 }
@@ -1109,6 +1116,8 @@ int godar_set_wind_grid(godar_ctx* c, int32_t m, int32_t nxn, int32_t nyn, doubl
                         hipMemcpyHostToDevice));
     c->dparams[m].wind_grid = (int32_t)c->wgrids.size() - 1;
     HIPCHK(c, hipMemcpy((void*)(c->d.params + m), &c->dparams[m], sizeof(DevParams), hipMemcpyHostToDevice));
+    force_refresh(c);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return GODAR_OK;
 }
 

@@ -86,7 +86,7 @@ struct DS {
     int32_t* icnt;
     int64_t* ik;
     uint32_t* evf;
-    int32_t* ov;
+    int32_t* ov;               // repair table entry, valid where evf has EV_TABLE
     double *ssc, *ssb;         // [CO][S] pos: the pair's |(fcn,fct)| and |(fbn,fbt)|
                                // (written only when forces are recorded)
     // pair state, double-buffered (id rows; partner stored as id)
@@ -100,8 +100,9 @@ struct DS {
     // speculative gather (pos)
     double* acc;               // [NACC][S]
     WallOut* wout;             // [S], valid where wflag is set
-    uint8_t* wflag;            // [S] 1: the floe touched a wall this step
-    int32_t* ovw;              // [S]
+    uint8_t* wflag;            // [S] bit 0: the floe touched a wall this step; bit 1: the
+                               //     repair recomputed its walls (wtab entry in ovw)
+    int32_t* ovw;              // [S] repair wall-table entry, valid where wflag bit 1 is set
     uint8_t* cls;              // [S] CLS_* bits
     // Candidate supersets: every floe within its query radius plus the
     // member's skin at the last build, (id << 32 | pos), floe k's at

@@ -295,10 +295,12 @@ __global__ void k_clear_cells(DS S) {
 
 // The floe records in the current order and the wind at each floe
 // (winds_currents, forcing.f90:1-29); on build steps the cell ranges over the
-// sorted keys.
+// sorted keys.  Between builds the order stands and k_final writes these
+// copies for the next step itself, so this runs only on build steps and after
+// the copies went stale (a member stopped, the host changed the wind).
 __global__ void k_ranges(DS S) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= S.N || *S.err) return;
+    if (k >= S.N || *S.err || (!S.rebuild[0] && !S.rebuild[1])) return;
     int f = (int)S.cval[k];
     int m = S.member_of[f];
     if (m >= 0 && !member_live(S.members[m])) m = -1;
@@ -368,7 +370,6 @@ __device__ __forceinline__ void cand_finish(const DS& S, int k, int fs, const De
     const uint32_t fl = P.shelter ? EV_SHELTER : 0u;
     for (int s2 = lane0; s2 < no; s2 += lanes) {
         S.evf[s2 * SS + k] = fl;
-        S.ov[s2 * SS + k] = -1;
         if (S.record_forces) {
             S.ssc[s2 * SS + k] = 0.0;
             S.ssb[s2 * SS + k] = 0.0;
@@ -382,7 +383,6 @@ __device__ __forceinline__ void cand_finish(const DS& S, int k, int fs, const De
         S.acc[A_SHW * SS + k] = shw;
         if (!qc) S.hcnt_new[fs] = 0;
         if (!qb) S.bcnt_new[fs] = 0;
-        S.ovw[k] = -1;
         S.wflag[k] = 0;
         S.bgat[k] = 0;
         S.cls[k] = qc ? CLS_CONTACT : 0;   // k_bond_gather adds CLS_BOND
@@ -634,8 +634,7 @@ __device__ __forceinline__ void cand_thread(const DS& S, int k, const GeoRec& me
 #pragma unroll
     for (int s2 = 0; s2 < LIST_REGS; ++s2) {
         if (s2 < no) {
-            S.od[s2 * SS + k] = rd[s2];
-            oj[s2 * SS + k] = re[s2];
+            oj[s2 * SS + k] = re[s2];   // (distances are only read back beyond the register part)
         }
         if (s2 < ni) ik[s2 * SS + k] = ri[s2];
     }
@@ -1300,7 +1299,7 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_walls(DS S) {
     if (*S.err) return;
     // no later kernel of the step reads the build flag; k_final raises it
     // again when a floe has moved too far for the supersets
-    if (blockIdx.x == 0 && threadIdx.x == 0) *S.rebuild = 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) { S.rebuild[0] = 0; S.rebuild[1] = 0; }
     const int k = queue_entry(S, 2);
     if (k < 0) return;
     const GeoRec me = S.geo[k];
@@ -1528,6 +1527,7 @@ __device__ void repair_serial(const DS& S, const DevParams& P, int first, int nm
             if (st.nwt >= nmem) { over |= ERR_REPAIR_CAP; break; }
             wtab[st.nwt] = W;
             S.ovw[kf] = st.nwt++;
+            S.wflag[kf] |= 2;
             S.affected[fs] = 1;
             if (W.flags & 4u) {
                 set_cur(S, fs, W.nr, W.nh, W.nmass, W.ninertia, W.nhfa, W.nhfw);
@@ -1573,7 +1573,7 @@ __device__ void repair_serial(const DS& S, const DevParams& P, int first, int nm
         if (st.nt >= tcap) { over |= ERR_REPAIR_CAP; break; }
         etab[st.nt] = o;
         S.ov[(int64_t)s * SS + kf] = st.nt++;
-        S.evf[(int64_t)s * SS + kf] = o.flags;
+        S.evf[(int64_t)s * SS + kf] = o.flags | EV_TABLE;
     }
 }
 
@@ -1708,10 +1708,11 @@ __global__ __launch_bounds__(64) void k_repair(DS S, int nmembers) {
             S.affected[fs] = 1;
             S.affected[js] = 1;
             S.ov[(int64_t)s * SS + kf] = st.nt + rank_e;
-            S.evf[(int64_t)s * SS + kf] = oflags;
+            S.evf[(int64_t)s * SS + kf] = oflags | EV_TABLE;
         }
         if (com && wall) {
             S.ovw[kf] = st.nwt + rank_w;
+            S.wflag[kf] |= 2;
             S.affected[fs] = 1;
         }
         put_hist_lanes(S, com && pair, fs, j, ps, over);
@@ -1832,7 +1833,7 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_fixup(DS S) {
         const int64_t e = ik[q * SS + k];
         const int i = ent_id(e), qi = ent_pos(e);
         int sk = slot_of(S, qi, f);
-        int idx = sk >= 0 ? S.ov[(int64_t)sk * SS + qi] : -1;
+        int idx = sk >= 0 && (S.evf[(int64_t)sk * SS + qi] & EV_TABLE) ? S.ov[(int64_t)sk * SS + qi] : -1;
         if (idx >= 0) {
             EventOut o = etab[idx];
             if (!(o.flags & EV_REMOVED)) apply_j(acc, o);
@@ -1850,7 +1851,7 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_fixup(DS S) {
     const int no = S.ocnt[k];
     const double2 wf = wind_at(S, k);
     for (int s = 0; s < no; ++s) {
-        int idx = S.ov[s * SS + k];
+        int idx = (S.evf[s * SS + k] & EV_TABLE) ? S.ov[s * SS + k] : -1;
         if (idx >= 0) {
             EventOut o = etab[idx];
             if (!(o.flags & EV_REMOVED)) apply_i(acc, o);
@@ -1903,7 +1904,7 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_stress(DS S) {
         const int qi = ent_pos(e);
         const int s = slot_of(S, qi, f);
         if (s < 0) continue;
-        const int idx = S.ov[(int64_t)s * SS + qi];
+        const int idx = (S.evf[(int64_t)s * SS + qi] & EV_TABLE) ? S.ov[(int64_t)s * SS + qi] : -1;
         const GeoRec gi = S.geo[qi];
         double sc, sb, ri;
         if (idx >= 0) {
@@ -1923,7 +1924,7 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_stress(DS S) {
     for (int s = 0; s < no; ++s) {
         const int64_t e = S.oj[s * SS + k];
         const int qj = ent_pos(e);
-        const int idx = S.ov[s * SS + k];
+        const int idx = (S.evf[s * SS + k] & EV_TABLE) ? S.ov[s * SS + k] : -1;
         double sc, sb, ri;
         if (idx >= 0) {
             const EventOut& o = etab[idx];
@@ -1966,11 +1967,11 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_final(DS S) {
         F.r = S.cur[0 * SS + f]; F.h = S.cur[1 * SS + f]; F.mass = S.cur[2 * SS + f];
         F.inertia = S.cur[3 * SS + f]; F.hfa = S.cur[4 * SS + f]; F.hfw = S.cur[5 * SS + f];
     }
-    int wi = S.ovw[k];
+    const uint8_t wf = S.wflag[k];
     WallOut W;
-    if (wi >= 0) {
-        W = S.wtab[S.members[m].first + wi];
-    } else if (S.wflag[k]) {
+    if (wf & 2) {
+        W = S.wtab[S.members[m].first + S.ovw[k]];   // recomputed by the repair
+    } else if (wf & 1) {
         W = S.wout[k];
     } else {
         // no wall touched: reset_boundary(i, 0) cleared every wall history
@@ -2039,7 +2040,26 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_final(DS S) {
     // flags set by the repair are cleared (most floes have none: no write)
     if (dirty) S.dirty[f] = 0;
     if (S.affected[f]) S.affected[f] = 0;
-    if (f == S.members[m].first) S.members[m].tstep += 1;   // once per member
+    // the next step's copies in the current order (k_ranges), unless it
+    // rebuilds them: position, kinematics, the wind at the new position
+    {
+        GeoRec g;
+        g.x = out.x; g.y = out.y; g.r = out.r; g.id = me.id; g.member = m;
+        S.geo[k] = g;
+        KinRec q;
+        q.u = out.u; q.v = out.v; q.omega = out.omega; q.h = out.h; q.mass = out.mass;
+        q.inertia = out.inertia; q.theta = out.theta;
+        S.kin[k] = q;
+        double ua = P.ua, va = P.va;
+        if (P.wind_grid >= 0) sample_wind(S.wgrids[P.wind_grid], S.wpool, out.x, out.y, ua, va);
+        ShelRec h;
+        h.hfa = out.hfa; h.hfw = out.hfw; h.wx = ua; h.wy = va;
+        S.shel[k] = h;
+    }
+    if (f == S.members[m].first) {   // once per member
+        S.members[m].tstep += 1;
+        if (!member_live(S.members[m])) S.rebuild[1] = 1;   // its floes leave the next step
+    }
 }
 
 // per-member totals of the last step: candidates, contacts (evf flags) and

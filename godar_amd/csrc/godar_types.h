@@ -118,6 +118,7 @@ enum : uint32_t {
     EV_RIDGED  = 1u << 2,
     EV_REMOVED = 1u << 3,   // no longer a candidate (query radius shrank)
     EV_SHELTER = 1u << 4,
+    EV_TABLE   = 1u << 5,   // (evf only) the repair table holds the event: ov is its entry
 };
 
 // Wall result of one floe (verify_bc, boundaries.f90:1-97).

@@ -411,6 +411,33 @@ def test_checkpoint_restore_continues_exactly():
             assert np.array_equal(gotf[k], wantf[k]), (prec, k)
 
 
+def test_forcing_change_takes_effect_next_step():
+    """A wind or current change between steps (godar_set_forcing) acts from
+    the next step on: the run continues bitwise like a member restored from a
+    checkpoint that carries the new forcing."""
+    import copy
+    f = kick(g.config0_member(61, 500), 5)
+    p = config0_params(f)
+    p2 = copy.copy(p)
+    p2.ua, p2.va, p2.uw, p2.vw = -6.0, 4.0, 0.05, -0.2
+    for prec in ("fp64", "mixed"):
+        a = _ctx(f.n, 1, prec)
+        ma = a.add_field(p, f)
+        a.step(12)
+        a.set_forcing(ma, p2.ua, p2.va, p2.uw, p2.vw)
+        block = a.checkpoint(ma, p2)
+        a.step(9)
+        b = _ctx(f.n, 1, prec)
+        mb = b.restore(block)
+        b.step(9)
+        fa, fb = a.floes(ma), b.floes(mb)
+        for k in STATE:
+            assert np.array_equal(fa[k], fb[k]), (prec, k)
+        assert np.array_equal(a.forces(ma)["tfx"], b.forces(mb)["tfx"]), prec
+        a.close()
+        b.close()
+
+
 def test_capacity_overflow_rolls_back():
     """A too-small candidate list aborts the step without advancing the state;
     after reserving more room the run continues exactly as an unhindered one.
