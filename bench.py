@@ -30,6 +30,30 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+PROFILE = os.path.join(ROOT, "profiles", "step_kernels_batched_512fields.md")
+STAGE_KERNEL = {"candidates": "k_cand", "contacts": "k_contact", "bonds": "k_bond",
+                "bond_gather": "k_bond_gather", "walls": "k_walls", "repair": "k_repair",
+                "fixup": "k_fixup", "integrate": "k_final"}
+
+
+def profile_traffic(stage: str, precision: str):
+    """HBM bytes per launch (read + write, counter-scaled) of a stage's kernel
+    from the committed rocprofv3 summary, or None."""
+    name = STAGE_KERNEL.get(stage)
+    if not name or not os.path.exists(PROFILE):
+        return None
+    tmpl = "<true>" if precision == "mixed" else "<false>"
+    for line in open(PROFILE):
+        cols = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cols) != 5 or not cols[0].startswith("`godar::"):
+            continue
+        kname = cols[0].strip("`")[len("godar::"):]
+        if kname in (name + tmpl, name):
+            try:
+                return (float(cols[3]) + float(cols[4])) * 1e6
+            except ValueError:
+                return None
+    return None
 
 
 def log(*a):
@@ -265,7 +289,10 @@ def main():
         },
         "roofline": {
             "bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+            "traffic": profile_traffic(dom, args.precision),
+            "traffic_source": "profiles/step_kernels_batched_512fields.md (rocprofv3 --pmc "
+                              "FETCH_SIZE x2.000, WRITE_SIZE x1.000, bytes per launch)",
             "kernel_avg_ms": dom_ms,
             "step_algorithmic_GBs": step_gbs, "step_frac": step_gbs / HBM_PEAK_GBS,
         },
