@@ -802,6 +802,28 @@ __device__ __forceinline__ int pow2_at_least(int n) {
     return p;
 }
 
+// Bitonic sort of one element per lane across the wavefront, ascending in
+// lane order, of two lists at once: outer keys (distance bits, entry) and
+// inner keys (entry).
+__device__ __forceinline__ void wave_sort_lists(uint64_t& od, int64_t& oe, int64_t& ie, int lane) {
+#pragma unroll
+    for (int size = 2; size <= 64; size <<= 1) {
+#pragma unroll
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            const uint64_t pd = (uint64_t)__shfl_xor((long long)od, stride);
+            const int64_t pe = __shfl_xor((long long)oe, stride);
+            const int64_t pi = __shfl_xor((long long)ie, stride);
+            // the lower lane of a pair keeps the smaller element in ascending
+            // blocks, the larger in descending ones
+            const bool keep_min = ((lane & stride) == 0) == ((lane & size) == 0);
+            const bool o_less = od < pd || (od == pd && oe < pe);
+            if (keep_min != o_less) { od = pd; oe = pe; }
+            const bool i_less = ie < pi;
+            if (keep_min != i_less) ie = pi;
+        }
+    }
+}
+
 // Superset build of the dense members' floes (queue 3), one wavefront per
 // floe over its (2 span + 1)^2 cell block, 64 entries at a time: counts, or
 // entries in scan order and the build positions.  Only on build steps.
@@ -882,87 +904,20 @@ __global__ __launch_bounds__(64) void k_cand_dense(DS S) {
         const DevParams& P = S.params[m];
         const double rad = me.r + P.rtree;
         const int f = me.id;
-        // 1. the superset entries within the query radius, 64 at a time
-        int no = 0, ni = 0;
-        const int L = S.scnt[k];
-        const int64_t* sup = S.sup + S.soff[k];
-        for (int base = 0; base < L; base += 64) {
-            const int v = base + lane;
-            bool vout = false, vin = false;
-            int oid = 0, q = 0;
-            double d = 0.0;
-            if (v < L) {
-                q = ent_pos(sup[v]);
-                const GeoRec o = S.geo[q];
-                const double dx = me.x - o.x, dy = me.y - o.y;
-                const double dist2 = dx * dx + dy * dy;
-                oid = o.id;
-                const double thr = o.id > f ? rad : o.r + P.rtree;
-                // far beyond the radius: no square root needed
-                if (dist2 <= thr * thr * (1.0 + 1e-12)) {
-                    d = sqrt(dist2);
-                    if (o.id > f) vout = d <= thr;
-                    else vin = d <= thr;
-                }
-            }
-            const uint64_t mo = __ballot(vout), mi = __ballot(vin);
-            if (vout) {
-                const int slot = no + (int)__popcll(mo & lt);
-                if (slot < DENSE_CAP) {
-                    sd[slot] = (uint64_t)__double_as_longlong(d);
-                    se[slot] = ent(oid, q);
-                }
-            }
-            if (vin) {
-                const int slot = ni + (int)__popcll(mi & lt);
-                if (slot < DENSE_CAP) si[slot] = ent(oid, q);
-            }
-            no += (int)__popcll(mo);
-            ni += (int)__popcll(mi);
-        }
-        __syncthreads();
-        if (no > DENSE_CAP || ni > DENSE_CAP) {
-            // rare: more candidates than LDS holds, the thread search does it
-            if (lane == 0) {
-                bool c2, b2, w2;
-                cand_thread<MIXED>(S, k, me, c2, b2, w2);
-                if (c2) enqueue_one(S, 0, k);
-                if (b2) enqueue_one(S, 1, k);
-                if (w2) enqueue_one(S, 2, k);
-            }
-            __syncthreads();
-            continue;
-        }
-        uint32_t over = 0;
-        if (no > S.CO) { over |= ERR_OUTER_CAP; no = S.CO; }
-        if (ni > S.CI) { over |= ERR_INNER_CAP; ni = S.CI; }
-        if (over && lane == 0) set_err(S, over);
-        // 2. sort: outer by (distance, id), inner by id
-        const int po = pow2_at_least(no), pi = pow2_at_least(ni);
-        for (int i = no + lane; i < po; i += 64) { sd[i] = ~0ull; se[i] = INT64_MAX; }
-        for (int i = ni + lane; i < pi; i += 64) si[i] = INT64_MAX;
-        __syncthreads();
-        bitonic_outer(sd, se, po, lane);
-        bitonic_inner(si, pi, lane);
-        // 3. sheltering and classification over the compacted candidates
         const int first = S.members[m].first;
         const bool bond_step = S.members[m].tstep == 1 && P.restart != 1 && P.cohesion;
         const ShelRec msh = S.shel[k];
         const T uwi = T(P.uw), vwi = T(P.vw);
         T sha = T(1), shw = T(1);
         bool lqc = false, lqb = false;
-        for (int s2 = lane; s2 < no; s2 += 64) {
-            const int64_t e = se[s2];
-            const int j = ent_id(e), q = ent_pos(e);
-            const double d = __longlong_as_double((long long)sd[s2]);
-            S.od[s2 * SS + k] = d;
-            S.oj[s2 * SS + k] = e;
-            const GeoRec o = S.geo[q];
+        // sheltering and classification terms of one outer / inner candidate
+        // (lane-parallel; a minimum and an "any" do not depend on the order)
+        auto outer_math = [&](const GeoRec& o, int q, double d) {
             // pair (f, j): f is the outer floe i of the event
             const double deltan = me.r + o.r - d;
             if (deltan > 0) lqc = true;
             if (bond_step && deltan >= -P.bond_lim &&
-                bond_keep(f - first, j - first, P.bond_seed, P.bond_thr24))
+                bond_keep(f - first, o.id - first, P.bond_seed, P.bond_thr24))
                 lqb = true;
             if (P.shelter) {
                 const double cosa = (o.x - me.x) / d, sina = (o.y - me.y) / d;
@@ -971,14 +926,8 @@ __global__ __launch_bounds__(64) void k_cand_dense(DS S) {
                 sha = dmin(sha, S_shelter(T(oh.hfa), T(msh.hfa), dn, -c, -sn, T(msh.wx), T(msh.wy)));
                 shw = dmin(shw, S_shelter(T(oh.hfw), T(msh.hfw), dn, -c, -sn, uwi, vwi));
             }
-        }
-        for (int s2 = lane; s2 < ni; s2 += 64) {
-            const int64_t e = si[s2];
-            const int q = ent_pos(e);
-            S.ik[s2 * SS + k] = e;
-            const GeoRec o = S.geo[q];
-            const double dx = me.x - o.x, dy = me.y - o.y;
-            const double d = sqrt(dx * dx + dy * dy);
+        };
+        auto inner_math = [&](const GeoRec& o, int q, double d) {
             // pair (j, f): f is the inner floe j of the event
             const double deltan = o.r + me.r - d;
             if (deltan > 0) lqc = true;
@@ -989,7 +938,112 @@ __global__ __launch_bounds__(64) void k_cand_dense(DS S) {
                 sha = dmin(sha, S_shelter(T(oh.hfa), T(msh.hfa), dn, c, sn, T(oh.wx), T(oh.wy)));
                 shw = dmin(shw, S_shelter(T(oh.hfw), T(msh.hfw), dn, c, sn, uwi, vwi));
             }
+        };
+        const int L = S.scnt[k];
+        const int64_t* sup = S.sup + S.soff[k];
+        int no = 0, ni = 0;
+        uint32_t over = 0;
+        if (L <= 64) {
+            // one entry per lane, which also takes the terms of a candidate
+            // it finds (the partner's row is at hand); both lists are then
+            // sorted across the wavefront in registers (sentinels above every
+            // entry fill the other lanes)
+            bool vout = false, vin = false;
+            int64_t e = 0;
+            double d = 0.0;
+            if (lane < L) {
+                const int q = ent_pos(sup[lane]);
+                const GeoRec o = S.geo[q];
+                const double dx = me.x - o.x, dy = me.y - o.y;
+                const double dist2 = dx * dx + dy * dy;
+                e = ent(o.id, q);
+                const double thr = o.id > f ? rad : o.r + P.rtree;
+                // far beyond the radius: no square root needed
+                if (dist2 <= thr * thr * (1.0 + 1e-12)) {
+                    d = sqrt(dist2);
+                    if (o.id > f) vout = d <= thr;
+                    else vin = d <= thr;
+                }
+                if (vout) outer_math(o, q, d);
+                if (vin) inner_math(o, q, d);
+            }
+            no = (int)__popcll(__ballot(vout));
+            ni = (int)__popcll(__ballot(vin));
+            uint64_t okey = vout ? (uint64_t)__double_as_longlong(d) : ~0ull;   // d >= 0: bits order as values
+            int64_t oe = vout ? e : INT64_MAX;
+            int64_t ie = vin ? e : INT64_MAX;
+            wave_sort_lists(okey, oe, ie, lane);
+            if (no > S.CO) { over |= ERR_OUTER_CAP; no = S.CO; }
+            if (ni > S.CI) { over |= ERR_INNER_CAP; ni = S.CI; }
+            // (distances are only read back by the thread search's tail)
+            if (lane < no) S.oj[lane * SS + k] = oe;
+            if (lane < ni) S.ik[lane * SS + k] = ie;
+        } else {
+            // 64 entries at a time, survivors compacted into LDS and sorted there
+            for (int base = 0; base < L; base += 64) {
+                const int v = base + lane;
+                bool vout = false, vin = false;
+                int oid = 0, q = 0;
+                double d = 0.0;
+                if (v < L) {
+                    q = ent_pos(sup[v]);
+                    const GeoRec o = S.geo[q];
+                    const double dx = me.x - o.x, dy = me.y - o.y;
+                    const double dist2 = dx * dx + dy * dy;
+                    oid = o.id;
+                    const double thr = o.id > f ? rad : o.r + P.rtree;
+                    if (dist2 <= thr * thr * (1.0 + 1e-12)) {
+                        d = sqrt(dist2);
+                        if (o.id > f) vout = d <= thr;
+                        else vin = d <= thr;
+                    }
+                    if (vout) outer_math(o, q, d);
+                    if (vin) inner_math(o, q, d);
+                }
+                const uint64_t mo = __ballot(vout), mi = __ballot(vin);
+                if (vout) {
+                    const int slot = no + (int)__popcll(mo & lt);
+                    if (slot < DENSE_CAP) {
+                        sd[slot] = (uint64_t)__double_as_longlong(d);
+                        se[slot] = ent(oid, q);
+                    }
+                }
+                if (vin) {
+                    const int slot = ni + (int)__popcll(mi & lt);
+                    if (slot < DENSE_CAP) si[slot] = ent(oid, q);
+                }
+                no += (int)__popcll(mo);
+                ni += (int)__popcll(mi);
+            }
+            __syncthreads();
+            if (no > DENSE_CAP || ni > DENSE_CAP) {
+                // rare: more candidates than LDS holds, the thread search does it
+                if (lane == 0) {
+                    bool c2, b2, w2;
+                    cand_thread<MIXED>(S, k, me, c2, b2, w2);
+                    if (c2) enqueue_one(S, 0, k);
+                    if (b2) enqueue_one(S, 1, k);
+                    if (w2) enqueue_one(S, 2, k);
+                }
+                __syncthreads();
+                continue;
+            }
+            if (no > S.CO) { over |= ERR_OUTER_CAP; no = S.CO; }
+            if (ni > S.CI) { over |= ERR_INNER_CAP; ni = S.CI; }
+            // sort: outer by (distance, id), inner by id
+            const int po = pow2_at_least(no), pi = pow2_at_least(ni);
+            for (int i = no + lane; i < po; i += 64) { sd[i] = ~0ull; se[i] = INT64_MAX; }
+            for (int i = ni + lane; i < pi; i += 64) si[i] = INT64_MAX;
+            __syncthreads();
+            bitonic_outer(sd, se, po, lane);
+            bitonic_inner(si, pi, lane);
+            for (int s2 = lane; s2 < no; s2 += 64) {
+                S.od[s2 * SS + k] = __longlong_as_double((long long)sd[s2]);
+                S.oj[s2 * SS + k] = se[s2];
+            }
+            for (int s2 = lane; s2 < ni; s2 += 64) S.ik[s2 * SS + k] = si[s2];
         }
+        if (over && lane == 0) set_err(S, over);
         sha = wave_min(sha);
         shw = wave_min(shw);
         const int fs = S.cval[k];
@@ -1515,6 +1569,9 @@ __device__ void repair_serial(const DS& S, const DevParams& P, int first, int nm
         if ((int64_t)key == st.last) continue;
         st.last = (int64_t)key;
         const int b = (int)(key >> 16), s = (int)(key & 0xFFFF);
+        // keys name an event of this member (a broken key is an internal
+        // error, never an out-of-range access)
+        if (key >> 16 >= (uint64_t)nmem || (s != WALL_SLOT && s >= S.CO)) { over |= ERR_REPAIR_CAP; break; }
         const int f = first + b;              // identity
         const int fs = S.sid_of[f];           // storage slot
         const int kf = S.pos_of[fs];
@@ -1649,7 +1706,14 @@ __global__ __launch_bounds__(64) void k_repair(DS S, int nmembers) {
         const int nb = min(np, lanes);
         const int64_t key = lane < nb ? pk[cb][head + lane] : INT64_MAX;
         const int64_t prev = lane == 0 ? st.last : (lane < nb ? pk[cb][head + lane - 1] : INT64_MAX);
-        const bool act = lane < nb && key != prev;      // repeated keys are done once
+        bool act = lane < nb && key != prev;            // repeated keys are done once
+        // keys name an event of this member (a broken key is an internal
+        // error, never an out-of-range access)
+        if (act && (key < 0 || (key >> 16) >= nmem || ((key & 0xFFFF) != WALL_SLOT && (key & 0xFFFF) >= S.CO))) {
+            over |= ERR_REPAIR_CAP;
+            act = false;
+        }
+        if (__ballot(over != 0)) break;
         const int64_t kk = act ? key : 0;
         const int b = (int)(kk >> 16), s = (int)(kk & 0xFFFF);
         const int f = first + b;
