@@ -601,8 +601,8 @@ __device__ __forceinline__ void cand_thread(const DS& S, int k, const GeoRec& me
             const double cosa = (o.x - me.x) / d, sina = (o.y - me.y) / d;
             const ShelRec oh = S.shel[q];
             const T dn = T(deltan), c = T(cosa), sn = T(sina);
-            sha = dmin(sha, S_shelter(T(oh.hfa), T(msh.hfa), dn, -c, -sn, T(msh.wx), T(msh.wy)));
-            shw = dmin(shw, S_shelter(T(oh.hfw), T(msh.hfw), dn, -c, -sn, uwi, vwi));
+            sha = shelter_min(sha, T(oh.hfa), T(msh.hfa), dn, -c, -sn, T(msh.wx), T(msh.wy));
+            shw = shelter_min(shw, T(oh.hfw), T(msh.hfw), dn, -c, -sn, uwi, vwi);
         }
     };
     auto inner_terms = [&](int64_t e) {
@@ -618,8 +618,8 @@ __device__ __forceinline__ void cand_thread(const DS& S, int k, const GeoRec& me
             const double cosa = (me.x - o.x) / d, sina = (me.y - o.y) / d;
             const ShelRec oh = S.shel[q];
             const T dn = T(deltan), c = T(cosa), sn = T(sina);
-            sha = dmin(sha, S_shelter(T(oh.hfa), T(msh.hfa), dn, c, sn, T(oh.wx), T(oh.wy)));
-            shw = dmin(shw, S_shelter(T(oh.hfw), T(msh.hfw), dn, c, sn, uwi, vwi));
+            sha = shelter_min(sha, T(oh.hfa), T(msh.hfa), dn, c, sn, T(oh.wx), T(oh.wy));
+            shw = shelter_min(shw, T(oh.hfw), T(msh.hfw), dn, c, sn, uwi, vwi);
         }
     };
 #pragma unroll
@@ -896,11 +896,20 @@ __global__ __launch_bounds__(64) void k_cand_dense(DS S) {
     const int t1 = min(nq, xcd * per + per);
     const uint64_t lt = lane ? ((~0ull) >> (64 - lane)) : 0ull;
     const int64_t SS = S.S;
-    for (int t = xcd * per + bi; t < t1; t += nb) {
-        const int k = queue_at(S, 3, qv, t);
-        const GeoRec me = S.geo[k];
+    // the next floe's list entry and row are loaded a floe ahead
+    int t = xcd * per + bi;
+    int k_next = t < t1 ? queue_at(S, 3, qv, t) : 0;
+    GeoRec me_next;
+    if (t < t1) me_next = S.geo[k_next];
+    for (; t < t1; t += nb) {
+        const int k = k_next;
+        const GeoRec me = me_next;
+        if (t + nb < t1) k_next = queue_at(S, 3, qv, t + nb);
         const int m = me.member;
-        if (m < 0) continue;        // member past its stop step (k_cand cleared its counts)
+        if (m < 0) {                // member past its stop step (k_cand cleared its counts)
+            if (t + nb < t1) me_next = S.geo[k_next];
+            continue;
+        }
         const DevParams& P = S.params[m];
         const double rad = me.r + P.rtree;
         const int f = me.id;
@@ -910,6 +919,10 @@ __global__ __launch_bounds__(64) void k_cand_dense(DS S) {
         const T uwi = T(P.uw), vwi = T(P.vw);
         T sha = T(1), shw = T(1);
         bool lqc = false, lqb = false;
+        // the floe's own pair-state counts, read early (the search's stores
+        // would otherwise hold these loads back)
+        const int fs = S.cval[k];
+        const bool has_hist = S.hcnt_old[fs] > 0, has_bonds = S.bcnt_old[fs] > 0;
         // sheltering and classification terms of one outer / inner candidate
         // (lane-parallel; a minimum and an "any" do not depend on the order)
         auto outer_math = [&](const GeoRec& o, int q, double d) {
@@ -923,8 +936,8 @@ __global__ __launch_bounds__(64) void k_cand_dense(DS S) {
                 const double cosa = (o.x - me.x) / d, sina = (o.y - me.y) / d;
                 const ShelRec oh = S.shel[q];
                 const T dn = T(deltan), c = T(cosa), sn = T(sina);
-                sha = dmin(sha, S_shelter(T(oh.hfa), T(msh.hfa), dn, -c, -sn, T(msh.wx), T(msh.wy)));
-                shw = dmin(shw, S_shelter(T(oh.hfw), T(msh.hfw), dn, -c, -sn, uwi, vwi));
+                sha = shelter_min(sha, T(oh.hfa), T(msh.hfa), dn, -c, -sn, T(msh.wx), T(msh.wy));
+                shw = shelter_min(shw, T(oh.hfw), T(msh.hfw), dn, -c, -sn, uwi, vwi);
             }
         };
         auto inner_math = [&](const GeoRec& o, int q, double d) {
@@ -935,8 +948,8 @@ __global__ __launch_bounds__(64) void k_cand_dense(DS S) {
                 const double cosa = (me.x - o.x) / d, sina = (me.y - o.y) / d;
                 const ShelRec oh = S.shel[q];
                 const T dn = T(deltan), c = T(cosa), sn = T(sina);
-                sha = dmin(sha, S_shelter(T(oh.hfa), T(msh.hfa), dn, c, sn, T(oh.wx), T(oh.wy)));
-                shw = dmin(shw, S_shelter(T(oh.hfw), T(msh.hfw), dn, c, sn, uwi, vwi));
+                sha = shelter_min(sha, T(oh.hfa), T(msh.hfa), dn, c, sn, T(oh.wx), T(oh.wy));
+                shw = shelter_min(shw, T(oh.hfw), T(msh.hfw), dn, c, sn, uwi, vwi);
             }
         };
         const int L = S.scnt[k];
@@ -1026,6 +1039,7 @@ __global__ __launch_bounds__(64) void k_cand_dense(DS S) {
                     if (w2) enqueue_one(S, 2, k);
                 }
                 __syncthreads();
+                if (t + nb < t1) me_next = S.geo[k_next];
                 continue;
             }
             if (no > S.CO) { over |= ERR_OUTER_CAP; no = S.CO; }
@@ -1046,9 +1060,9 @@ __global__ __launch_bounds__(64) void k_cand_dense(DS S) {
         if (over && lane == 0) set_err(S, over);
         sha = wave_min(sha);
         shw = wave_min(shw);
-        const int fs = S.cval[k];
-        const bool qc = __ballot(lqc) != 0 || S.hcnt_old[fs] > 0;
-        const bool qb = __ballot(lqb) != 0 || S.bcnt_old[fs] > 0;
+        const bool qc = __ballot(lqc) != 0 || has_hist;
+        const bool qb = __ballot(lqb) != 0 || has_bonds;
+        if (t + nb < t1) me_next = S.geo[k_next];
         const bool qw = me.y - me.r < 0 || me.y + me.r > P.ny || me.x - me.r < 0 || me.x + me.r > P.nx;
         cand_finish<MIXED>(S, k, fs, P, no, ni, double(sha), double(shw), qc, qb, lane, 64);
         if (lane == 0) {
@@ -1056,7 +1070,11 @@ __global__ __launch_bounds__(64) void k_cand_dense(DS S) {
             if (qb) enqueue_one(S, 1, k);
             if (qw) enqueue_one(S, 2, k);
         }
-        __syncthreads();
+        // The block is this one wavefront, whose LDS accesses complete in
+        // order: the next floe may reuse the LDS lists without a barrier
+        // (which would also wait for this floe's stores to complete).  Only
+        // the compiler is kept from moving accesses across.
+        __builtin_amdgcn_wave_barrier();
     }
 }
 

@@ -82,6 +82,19 @@ __device__ __forceinline__ T S_shelter(T hfi, T hfj, T deltan, T cosa, T sina, T
     return T(1) - D * G;
 }
 
+// min(acc, S_shelter(...)) for the per-floe sheltering minima.  An open-water
+// pair (deltan < 0) has Hd = 0 and D = 1 - (1 - e) with e = exp(-0.18 |deltan
+// / costheta| / hfj) and |costheta| <= 1; once 0.18 |deltan| / hfj >= 20
+// (FP32) or 40 (FP64), e is below half an ulp of 1, 1 - e rounds to 1, D = 0
+// and the term is exactly 1.  Every term is at most 1, so such a term leaves
+// the minimum unchanged and is not evaluated.
+template <typename T>
+__device__ __forceinline__ T shelter_min(T acc, T hfi, T hfj, T deltan, T cosa, T sina, T uf, T vf) {
+    const T lim = sizeof(T) == 4 ? T(20) : T(40);
+    if (deltan < T(0) && T((double)0.18f) * -deltan >= lim * hfj) return acc;
+    return dmin(acc, S_shelter(hfi, hfj, deltan, cosa, sina, uf, vf));
+}
+
 // Old pair state of one event (contact history + bond record).
 struct PairState {
     double deltat, thetarelc;
