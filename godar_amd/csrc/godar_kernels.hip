@@ -38,6 +38,15 @@ namespace godar {
 
 __device__ __forceinline__ void set_err(const DS& S, uint32_t bit) { atomicOr(S.err, bit); }
 
+// LDS written by some lanes of a one-wavefront block, then read by others:
+// the wavefront's LDS operations complete in order, so waiting for its own
+// outstanding ones (and keeping the compiler from moving accesses across)
+// is enough; no block barrier, which would also wait for global stores.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_s_waitcnt(0xc07f);   // lgkmcnt(0); vmcnt, expcnt unchanged (gfx9 encoding)
+    __builtin_amdgcn_wave_barrier();
+}
+
 // Workgroups are dispatched to the 8 XCDs round-robin, and each XCD has its
 // own L2.  Kernels that read their neighbours' rows take this logical block
 // index instead of blockIdx.x, so that each XCD works on one contiguous eighth
@@ -133,11 +142,14 @@ __device__ __forceinline__ Floe floe_of(const FloeRec& s) {
 
 // floe by storage slot with the properties current during the ordered repair
 __device__ __forceinline__ Floe load_floe_cur(const DS& S, int f) {
+    // the current rows are read with the record (one round of loads) and
+    // taken where the floe was reshaped
     Floe F = floe_of(S.rec[f]);
-    if (S.dirty[f]) {
-        F.r = S.cur[0 * S.S + f]; F.h = S.cur[1 * S.S + f]; F.mass = S.cur[2 * S.S + f];
-        F.inertia = S.cur[3 * S.S + f]; F.hfa = S.cur[4 * S.S + f]; F.hfw = S.cur[5 * S.S + f];
-    }
+    const bool d = S.dirty[f] != 0;
+    const double r = S.cur[0 * S.S + f], h = S.cur[1 * S.S + f], mass = S.cur[2 * S.S + f];
+    const double inertia = S.cur[3 * S.S + f], hfa = S.cur[4 * S.S + f], hfw = S.cur[5 * S.S + f];
+    F.r = d ? r : F.r; F.h = d ? h : F.h; F.mass = d ? mass : F.mass;
+    F.inertia = d ? inertia : F.inertia; F.hfa = d ? hfa : F.hfa; F.hfw = d ? hfw : F.hfw;
     return F;
 }
 
@@ -1865,13 +1877,13 @@ __global__ __launch_bounds__(64) void k_repair(DS S, int nmembers) {
         K += max(gno - s0, 0);
         if (lane == 0) nk[K] = ((int64_t)(gc - first) << 16) | WALL_SLOT;
         K += 1;
-        __syncthreads();
+        wave_lds_sync();
         // merge the two sorted runs (pending keys first among equals)
         const int64_t* A = &pk[cb][head + consumed];
         int64_t* O = pk[cb ^ 1];
         for (int i = lane; i < rest; i += 64) O[i + count_below(nk, K, A[i], false)] = A[i];
         for (int i = lane; i < K; i += 64) O[i + count_below(A, rest, nk[i], true)] = nk[i];
-        __syncthreads();
+        wave_lds_sync();
         cb ^= 1;
         head = 0;
         np = rest + K;
