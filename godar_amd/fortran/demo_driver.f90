@@ -4,7 +4,11 @@
 ! stepper.  It is compiled against the reference's COMMON blocks (with a
 ! parameter.h for its n, nx, ny), like stepper_gpu itself.
 !
-!     demo_driver <input> <output>
+!     demo_driver <input> <output> [expno]
+!
+! With expno, the reference's own output routine sea_ice_post
+! (diagnostics_outputs.f90, compiled from the reference sources) writes
+! output/<name>.<expno> on the output steps, as godar.f90:195-197 does.
 !
 ! input (stream, native byte order): int32 nt, restart, comp, precision;
 ! 33 doubles dt rtree rhoair rhoice rhowater Cdair Cdwater Csair Cswater
@@ -28,12 +32,16 @@ program demo_driver
     include "CB_diagnostics.h"
 
     character(len=512) :: fin, fout
+    character(len=4) :: expno
+    logical :: post
     integer(c_int32_t) :: hdr(4), opt(4), nbonds
     real(c_double) :: pv(33)
     integer :: tstep, restart_, ii, jj, u_in, u_out
 
     call get_command_argument(1, fin)
     call get_command_argument(2, fout)
+    post = command_argument_count() >= 3
+    if (post) call get_command_argument(3, expno)
     open(newunit=u_in, file=trim(fin), access='stream', form='unformatted', status='old')
     read(u_in) hdr, pv, opt
     read(u_in) x, y, u, v, theta, omega, r, h
@@ -48,12 +56,16 @@ program demo_driver
     sigmatb_crit = pv(20); sigmacb_crit = pv(21); tau_crit = pv(22); bond_lim = pv(23)
     dmax = pv(24); dtd = pv(25); dth = pv(26); ua = pv(27); va = pv(28); uw = pv(29); vw = pv(30)
     ntree = pv(31)
+    pi = 4d0 * datan(1d0)        ! par_get.f90:35 (the output routine reads it)
     slipping = opt(1) /= 0; cohesion = opt(2) /= 0; ridging = opt(3) /= 0; shelter = opt(4) /= 0
     bond = 0
     damageb = 0d0
 
     do tstep = 1, int(nt)
         call stepper_gpu(tstep, restart_)
+        if (post .and. (modulo(tstep, int(comp)) .eq. 0 .or. tstep + restart_ == 1)) then
+            call sea_ice_post(tstep, expno)
+        end if
     end do
 
     nbonds = 0
