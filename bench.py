@@ -11,7 +11,9 @@ precision.  Reported beside it, on the same box:
 * the relative position error of the mixed-precision path after 10^4 steps
   of the 1,000-floe field against the fixture made by GODAR's own routines;
 * the CPU baseline: GODAR's own physics routines compiled from the reference
-  sources (oracle/_ref, 1 core), or the C oracle where that is absent.
+  sources (oracle/_ref), or the C oracle where that is absent: one core, and
+  one run per core on up to 16 cores at once (the aggregate is
+  `cpu_baseline`).
 
 `value` is whole-job floe-steps/s of the batched workload with all state
 resident in HBM; the timed region is K steps bracketed by device syncs.
@@ -168,12 +170,22 @@ def pos_error_1e4(precision="mixed"):
     rel = np.abs(xy - ref) / np.abs(ref)
     env_max = max(float(L["ulp_env_max_10000_0"]), float(L["ulp_env_max_10000_1"]))
     env_med = max(float(L["ulp_env_median_10000_0"]), float(L["ulp_env_median_10000_1"]))
+    uv = np.concatenate([a["u"], a["v"]])
+    uv_ref = np.concatenate([L["u_10000"], L["v_10000"]])
+    vol = lambda r, h: float(np.sum(np.pi * r * r * h))        # ridging conserves each floe's pi r^2 h
+    v0 = vol(f.r, f.h)
     return dict(max=float(rel.max()), median=float(np.median(rel)),
+                rel_l2=float(np.linalg.norm(xy - ref) / np.linalg.norm(ref)),
+                vel_rel_l2=float(np.linalg.norm(uv - uv_ref) / np.linalg.norm(uv_ref)),
+                vel_max_abs=float(np.abs(uv - uv_ref).max()),
+                volume_drift=(vol(a["r"], a["h"]) - v0) / v0,
+                volume_drift_reference=(vol(L["r_10000"], L["h_10000"]) - v0) / v0,
                 ulp_envelope_max=env_max, ulp_envelope_median=env_med)
 
 
 def cpu_baseline(target_seconds: float):
-    """GODAR's own routines (oracle/_ref) on the 1,000-floe field, one core."""
+    """GODAR's own routines (oracle/_ref) on the 1,000-floe field, one core
+    (a plain dict, so that worker processes can return it)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from godar_amd import workloads as W
@@ -185,7 +197,7 @@ def cpu_baseline(target_seconds: float):
         step = H.step
         kind = "reference"
         what = ("GODAR physics routines compiled from the reference sources (amdflang -O2), "
-                "stepper control flow restated, brute-force radius search")
+                "stepper control flow restated, cell-list radius search")
     else:
         from oracle_py import OracleSim
         O = OracleSim(p, f)
@@ -200,8 +212,30 @@ def cpu_baseline(target_seconds: float):
         el = time.perf_counter() - t0
         if el >= target_seconds:
             break
-    return dict(value=f.n * t / el, unit="floe-steps/s", cores=1, kind=kind,
+    return dict(value=f.n * t / el, unit="floe-steps/s", cores=1, kind=kind, steps=t, seconds=el,
                 sample=f"config 0 (1,000 floes, 100 km box), first {t} steps, {el:.1f} s; {what}")
+
+
+def _cpu_worker(target_seconds):
+    return cpu_baseline(target_seconds)
+
+
+def cpu_baseline_all_cores(target_seconds: float, workers: int):
+    """The same run in `workers` processes at once (the batched workload's
+    members are independent, so a CPU runs one per core): aggregate
+    floe-steps/s.  Called before the GPU is touched (worker processes are
+    forked)."""
+    import multiprocessing as mp
+    with mp.get_context("fork").Pool(workers) as pool:
+        runs = pool.map(_cpu_worker, [target_seconds] * workers)
+    one = runs[0]
+    total = sum(r["value"] for r in runs)
+    return dict(value=total, unit="floe-steps/s", cores=workers, kind=one["kind"],
+                sample=(f"{workers} processes, each config 0 (1,000 floes, 100 km box) for "
+                        f"{target_seconds:.0f} s ({min(r['steps'] for r in runs)}-"
+                        f"{max(r['steps'] for r in runs)} steps); "
+                        + one["sample"].split("; ", 1)[1]),
+                per_core=[r["value"] for r in runs])
 
 
 def main():
@@ -237,6 +271,12 @@ def main():
     with open(os.path.join(ROOT, "BASELINE.json")) as fh:
         base = json.load(fh)
 
+    cb = cb1 = None
+    if not args.quick:
+        # CPU baselines first: the worker processes are forked before any GPU use
+        workers = max(1, min(16, os.cpu_count() or 1))
+        cb1 = cpu_baseline(args.cpu_seconds)
+        cb = cpu_baseline_all_cores(args.cpu_seconds, workers) if workers > 1 else cb1
     B = run_batched(args)
     value = B["nfloes"] * args.steps / B["seconds"]
     ms_step = B["seconds"] / args.steps * 1e3
@@ -310,9 +350,10 @@ def main():
         out["pos_err_1e4"] = {"mixed": pos_error_1e4("mixed"), "fp64": pos_error_1e4("fp64")}
         if args.serve_requests > 0:
             out["serving"] = run_serving(args.serve_requests)
-        cb = cpu_baseline(args.cpu_seconds)
         out["cpu_baseline"] = cb
+        out["cpu_baseline_one_core"] = {k: cb1[k] for k in ("value", "unit", "cores", "kind", "sample")}
         out["vs_cpu_baseline"] = value / cb["value"]
+        out["vs_cpu_baseline_one_core"] = value / cb1["value"]
     print(json.dumps(out), flush=True)
 
 

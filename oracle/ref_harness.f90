@@ -8,7 +8,8 @@
 ! needs the coretran k-d tree and an MPI mpi_f08 module, neither of which is
 ! in the image.  This file therefore restates only stepper.f90's control flow
 ! for one MPI rank and one OpenMP thread and calls the reference routines for
-! all arithmetic, with a brute-force radius search in place of coretran:
+! all arithmetic, with a cell-list radius search in place of coretran (3 x 3
+! cells of side >= max r + rtree around each floe, exact distances):
 !
 !   candidates of i = { j : sqrt((x(i)-x(j))**2 + (y(i)-y(j))**2) <= r(i)+rtree }
 !   sorted by ascending distance, ties by ascending index; the first entry
@@ -227,8 +228,27 @@ subroutine ref_step(tstep, restart) bind(C, name="ref_step")
     double precision, allocatable :: cd(:), thr_a(:), thr_w(:)
     double precision :: hi0, hj0
     integer :: tj, ci, cjj
+    integer :: gx, gy, cx, cy, bx, by
+    double precision :: cs
+    integer, allocatable :: head(:), nxt(:)
 
     allocate(cj(n), cd(n), thr_a(n), thr_w(n))
+    ! cell list of the step's start positions: side >= max r + rtree, so the
+    ! 3 x 3 cells around a floe hold every floe within its query radius
+    ! (cells are clamped into the grid; distances are exact)
+    cs = maxval(r) + rtree
+    gx = max(1, int(nx / cs)); gy = max(1, int(ny / cs))
+    do while (dble(gx) * dble(gy) > 4d0 * n + 64d0)
+        gx = max(1, (gx + 1) / 2); gy = max(1, (gy + 1) / 2)
+    end do
+    allocate(head(gx * gy), nxt(n))
+    head = 0
+    do j = n, 1, -1
+        cx = min(max(int(floor(x(j) / (nx / gx))), 0), gx - 1)
+        cy = min(max(int(floor(y(j) / (ny / gy))), 0), gy - 1)
+        nxt(j) = head(cy * gx + cx + 1)
+        head(cy * gx + cx + 1) = j
+    end do
     thr_a = 1d0
     thr_w = 1d0
     nlog = 0
@@ -250,13 +270,21 @@ subroutine ref_step(tstep, restart) bind(C, name="ref_step")
         ! radius search around the current position and radius of i
         rad = r(i) + rtree
         cnt = 0
-        do j = 1, n
-            d = sqrt( ( x(i) - x(j) ) ** 2 + ( y(i) - y(j) ) ** 2 )
-            if (d <= rad) then
-                cnt = cnt + 1
-                cj(cnt) = j
-                cd(cnt) = d
-            end if
+        cx = min(max(int(floor(x(i) / (nx / gx))), 0), gx - 1)
+        cy = min(max(int(floor(y(i) / (ny / gy))), 0), gy - 1)
+        do by = max(cy - 1, 0), min(cy + 1, gy - 1)
+            do bx = max(cx - 1, 0), min(cx + 1, gx - 1)
+                j = head(by * gx + bx + 1)
+                do while (j /= 0)
+                    d = sqrt( ( x(i) - x(j) ) ** 2 + ( y(i) - y(j) ) ** 2 )
+                    if (d <= rad) then
+                        cnt = cnt + 1
+                        cj(cnt) = j
+                        cd(cnt) = d
+                    end if
+                    j = nxt(j)
+                end do
+            end do
         end do
         ! insertion sort by (distance, index)
         do a = 2, cnt
@@ -419,7 +447,7 @@ subroutine ref_step(tstep, restart) bind(C, name="ref_step")
     call velocity
     call position
 
-    deallocate(cj, cd, thr_a, thr_w)
+    deallocate(cj, cd, thr_a, thr_w, head, nxt)
 end subroutine ref_step
 
 
