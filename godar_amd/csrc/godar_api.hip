@@ -226,6 +226,16 @@ void sync_ds(godar_ctx* c) {
     bind_parity(c);
 }
 
+// Floe records up to date with the ordered copies (a no-op unless a step ran
+// since the last spill), before the host reads or writes them or changes the
+// floe count; the next step then reads rec as it stands.
+void spill(godar_ctx* c) {
+    if (c->N_used == 0) return;
+    sync_ds(c);
+    hipLaunchKernelGGL(k_spill, dim3((c->N_used + 255) / 256), dim3(256), 0, c->stream, c->d, 1);
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, c->stream, c->d.rebuild + 2, 1, 0);
+}
+
 int ensure_cells(godar_ctx* c, int32_t need) {
     if (need <= c->ncells_cap) return GODAR_OK;
     int32_t cap = std::max(need, c->ncells_cap * 2);
@@ -342,7 +352,11 @@ int launch_step(godar_ctx* c) {
     // Steps that start with d.rebuild set bin, sort and search the cells and
     // build the candidate supersets; the other steps' build kernels return at
     // once (the launch sequence, and a captured graph, is the same).
-    { StageTimer t(c, 0); hipLaunchKernelGGL(k_bin, grid_for(N, B[0], false), dim3(B[0]), 0, s, d); }
+    {
+        StageTimer t(c, 0);
+        hipLaunchKernelGGL(k_spill, grid_for(N, B[0], false), dim3(B[0]), 0, s, d, 0);
+        hipLaunchKernelGGL(k_bin, grid_for(N, B[0], false), dim3(B[0]), 0, s, d);
+    }
     int bits = 1;
     while ((1u << bits) <= (uint32_t)c->ncells_cap && bits < 32) ++bits;
     {
@@ -589,10 +603,11 @@ int godar_create(int64_t max_floes, int32_t max_members, int32_t precision, goda
     c->sup_cap = 8 * S;
     ACHK(dalloc(&d.sup, c->sup_cap));
     ACHK(dalloc(&d.scnt, S));
+    ACHK(dalloc(&d.sno, S));
     ACHK(dalloc(&d.soff, S));
     ACHK(dalloc(&d.sblk, S / SCAN_TILE + 2));
     ACHK(dalloc(&d.x0, S));
-    ACHK(dalloc(&d.rebuild, 2));
+    ACHK(dalloc(&d.rebuild, 3));
     ACHK(radix_alloc(c->rs, S));
 #undef ACHK
     (void)hipMemsetAsync(d.err, 0, sizeof(uint32_t), c->stream);
@@ -600,6 +615,7 @@ int godar_create(int64_t max_floes, int32_t max_members, int32_t precision, goda
     (void)hipMemsetAsync(d.scnt, 0, sizeof(int32_t) * S, c->stream);
     (void)hipMemsetAsync(d.qn, 0, sizeof(int32_t) * NQUEUE * XCD_COUNT * QN_STRIDE, c->stream);
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, c->stream, d.rebuild, 2, 1);
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, c->stream, d.rebuild + 2, 1, 0);
     (void)hipMemsetAsync(d.dirty, 0, S, c->stream);
     (void)hipMemsetAsync(d.affected, 0, S, c->stream);
     (void)hipMemsetAsync(d.stats, 0, sizeof(int64_t) * 4 * max_members, c->stream);
@@ -636,7 +652,7 @@ int godar_destroy(godar_ctx* c) {
     dfree(d.wout); dfree(d.wflag); dfree(d.cls); dfree(d.qlist); dfree(d.qn); dfree(d.bgat); dfree(d.seeds); dfree(d.nseeds); dfree(d.heap); dfree(d.etab);
     dfree(d.netab); dfree(d.wtab); dfree(d.nwtab); dfree(d.ovw); dfree(d.dirty);
     dfree(d.affected); dfree(d.stats); dfree(d.err); dfree(d.steps_done);
-    dfree(d.sup); dfree(d.scnt); dfree(d.soff); dfree(d.sblk); dfree(d.x0); dfree(d.rebuild);
+    dfree(d.sup); dfree(d.scnt); dfree(d.sno); dfree(d.soff); dfree(d.sblk); dfree(d.x0); dfree(d.rebuild);
     radix_free(c->rs);
     collect_events(c);
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
@@ -662,7 +678,7 @@ static int64_t bytes_per_floe(const godar_ctx* c) {
     per += (int64_t)c->SEEDK * 8 + (int64_t)c->HEAPK * 8 + (int64_t)c->TABK * sizeof(EventOut);
     per += sizeof(WallOut) /*wtab*/ + 4 /*ovw*/ + 6 * 8 /*cur*/ + 1 + 1 /*dirty, affected*/;
     per += 33 * 8 /*fout*/ + 4 + 4 /*radix keys, values*/;
-    per += (c->S > 0 ? c->sup_cap / c->S : 0) * 8 /*superset pool share*/ + 4 + 8 + 16 /*scnt soff x0*/;
+    per += (c->S > 0 ? c->sup_cap / c->S : 0) * 8 /*superset pool share*/ + 4 + 4 + 8 + 16 /*scnt sno soff x0*/;
     per += 1 /*sblk, per SCAN_TILE floes*/;
     return per;
 }
@@ -777,6 +793,7 @@ int godar_member_add(godar_ctx* c, const godar_params* p, int32_t n, const doubl
         gid[t] = (int32_t)(first + i);
         sid[i] = (int32_t)(first + t);
     }
+    spill(c);
     HIPCHK(c, hipStreamSynchronize(c->stream));
     DS& d = c->d;
     HIPCHK(c, hipMemcpy(d.rec + first, recs.data(), sizeof(FloeRec) * n, hipMemcpyHostToDevice));
@@ -832,6 +849,7 @@ struct MemberState {
 };
 
 int read_member_state(godar_ctx* c, int32_t m, MemberState& out) {
+    spill(c);
     HIPCHK(c, hipStreamSynchronize(c->stream));
     const Member& mm = c->members[m];
     const int n = mm.n;
@@ -1055,6 +1073,7 @@ int godar_member_restore(godar_ctx* c, const void* buf, int64_t size, int32_t* m
 
 int godar_member_remove(godar_ctx* c, int32_t m) {
     if (!valid_member(c, m)) return fail(c, GODAR_ERR_ARG, "member_remove: bad member");
+    spill(c);                       // before the floe count changes
     int rc = download_members(c);   // keep device-side step counters
     if (rc != GODAR_OK) return rc;
     Member& mm = c->members[m];
@@ -1406,6 +1425,7 @@ int godar_get_floes(godar_ctx* c, int32_t m, double* x, double* y, double* u, do
                     double* theta, double* omega, double* r, double* h, double* mass,
                     double* inertia, double* hfa, double* hfw) {
     if (!valid_member(c, m)) return fail(c, GODAR_ERR_ARG, "get_floes: bad member");
+    spill(c);
     HIPCHK(c, hipStreamSynchronize(c->stream));
     const Member& mm = c->members[m];
     std::vector<FloeRec> recs(mm.n);

@@ -24,14 +24,18 @@ constexpr int NQUEUE = 4;         // work queues: contact, bond, wall, dense sea
 constexpr int QN_STRIDE = 64;     // ints between queue counters (256 B)
 constexpr int SCAN_TILE = 1024;   // positions per tile of the superset offset scan
 
-// Persistent per-floe record (96 bytes), indexed by floe id.
+// Per-floe record (96 bytes) by storage slot.  Between builds of the cell
+// order the floe state lives in the ordered copies below, which the step
+// reads and writes; the records are brought up to date from them (k_spill)
+// before anything reads or writes them: on steps that build or refresh the
+// copies, and before the host reads floes, adds floes or removes a member.
 struct FloeRec {
     double x, y, u, v, omega;
     double r, h, mass, inertia, hfa, hfw;
     double theta;
 };
-// Each step the floes are copied in cell order ("pos"), so that a floe's
-// neighbours are close in memory, and split by what the kernels read:
+// The floes are held in cell order ("pos"), so that a floe's neighbours are
+// close in memory, and split by what the kernels read:
 //   GeoRec  - neighbour search and every pair test (32 B)
 //   ShelRec - sheltering: draft coefficients and the wind at the floe (32 B)
 //   KinRec  - contact, bond, wall and forcing terms, integration (56 B)
@@ -57,7 +61,8 @@ struct DS {
     int32_t repair_lanes;      // events the repair evaluates at once (1: one at a time)
     int32_t repair_cap;        // pending repair keys held in LDS
     int32_t pad_r_;
-    // persistent floe state (id)
+    // floe state by storage slot (current where rebuild[2] is clear, else
+    // in the ordered copies)
     FloeRec* rec;
     double* wbc;               // [4][S] deltat_bc1, deltat_bc2, theta_bc1, theta_bc2
     int32_t* member_of;        // [S], -1 = free slot
@@ -115,10 +120,13 @@ struct DS {
     int64_t* sup;              // [sup_cap]
     int64_t sup_cap;
     int32_t* scnt;             // [S] by pos
+    int32_t* sno;              // [S] by pos: a thread-searched floe's superset holds its
+                               //     sno entries j > f first, then those with j < f
     int64_t* soff;             // [S] by pos
     int64_t* sblk;             // [S / SCAN_TILE + 2] scan of per-tile totals
     double2* x0;               // [S] by storage slot
-    int32_t* rebuild;
+    int32_t* rebuild;          // [3] 0: build the supersets; 1: refresh the ordered copies;
+                               //     2: the copies are newer than rec
     // Work queues (contact, bond, wall, dense search), each in XCD_COUNT
     // segments: the candidate search's blocks on XCD x cover positions
     // [x qseg, (x+1) qseg) and queue them into segment x, at

@@ -31,6 +31,8 @@
 // a sorted copy of the 96-byte floe records, so that a floe's neighbours sit
 // close in memory.  Pair state persists per floe id.  List entries pack the
 // partner as (id << 32 | pos).
+#include <type_traits>
+
 #include "godar_engine.h"
 #include "godar_physics.h"
 
@@ -133,18 +135,12 @@ __device__ __forceinline__ double2 wind_at(const DS& S, int k) {
     return make_double2(h.wx, h.wy);
 }
 
-__device__ __forceinline__ Floe floe_of(const FloeRec& s) {
-    Floe F;
-    F.x = s.x; F.y = s.y; F.u = s.u; F.v = s.v; F.omega = s.omega;
-    F.r = s.r; F.h = s.h; F.mass = s.mass; F.inertia = s.inertia; F.hfa = s.hfa; F.hfw = s.hfw;
-    return F;
-}
-
-// floe by storage slot with the properties current during the ordered repair
-__device__ __forceinline__ Floe load_floe_cur(const DS& S, int f) {
-    // the current rows are read with the record (one round of loads) and
+// floe at storage slot f and position k with the properties current during
+// the ordered repair
+__device__ __forceinline__ Floe load_floe_cur(const DS& S, int f, int k) {
+    // the current rows are read with the records (one round of loads) and
     // taken where the floe was reshaped
-    Floe F = floe_of(S.rec[f]);
+    Floe F = floe_at(S, k);
     const bool d = S.dirty[f] != 0;
     const double r = S.cur[0 * S.S + f], h = S.cur[1 * S.S + f], mass = S.cur[2 * S.S + f];
     const double inertia = S.cur[3 * S.S + f], hfa = S.cur[4 * S.S + f], hfw = S.cur[5 * S.S + f];
@@ -274,8 +270,30 @@ __device__ __forceinline__ void acc_init(double* a) {
 // ---------------------------------------------------------------------
 // 1. cell keys (one per floe id)
 // ---------------------------------------------------------------------
+// rec from the ordered copies (k_final's state) where they are newer: at the
+// start of steps that rebuild or refresh the copies from rec (then k_bin
+// clears the flag), and for the host (`host`, which clears it itself) before
+// it reads or writes floe records or changes the floe count.  Positions whose
+// copy is of no live member are skipped: their records are current.
+__global__ void k_spill(DS S, int host) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= S.N || !S.rebuild[2] || (!host && !S.rebuild[0] && !S.rebuild[1])) return;
+    const GeoRec g = S.geo[k];
+    const uint32_t f = S.cval[k];
+    if (g.member < 0 || f >= (uint32_t)S.N) return;
+    const KinRec q = S.kin[k];
+    const ShelRec h = S.shel[k];
+    FloeRec r;
+    r.x = g.x; r.y = g.y; r.u = q.u; r.v = q.v; r.omega = q.omega;
+    r.r = g.r; r.h = q.h; r.mass = q.mass; r.inertia = q.inertia; r.hfa = h.hfa; r.hfw = h.hfw;
+    r.theta = q.theta;
+    S.rec[f] = r;
+}
+
 __global__ void k_bin(DS S) {
     int f = blockIdx.x * blockDim.x + threadIdx.x;
+    // k_spill has brought rec up to date where this step reads it
+    if (f == 0 && (S.rebuild[0] || S.rebuild[1])) S.rebuild[2] = 0;
     if (f >= S.N || *S.err) return;
     const int m = S.member_of[f];
     const bool live = m >= 0 && member_live(S.members[m]);
@@ -410,10 +428,12 @@ constexpr int LIST_REGS = 4;
 // one contiguous range of positions; every cell range is read at once
 // (indices clamped into the grid, cells outside it masked out afterwards) and
 // the rows, taken as one sequence, SCAN_BATCH records at a time.  FILL
-// writes the entries at `out`; either way the count is returned.
+// writes the entries at `out`, the `total` (the count of the counting pass)
+// entries j > f from the front and the others from the back, and stores the
+// number of the former in *nout; either way the count is returned.
 template <int SPAN, bool FILL>
 __device__ __forceinline__ int sup_thread(const DS& S, int k, const GeoRec& me, const DevParams& P,
-                                          int64_t* out) {
+                                          int64_t* out, int total = 0, int32_t* nout = nullptr) {
     constexpr int R = 2 * SPAN + 1;
     const int cx = cell_coord(me.x, P.cell, P.gx), cy = cell_coord(me.y, P.cell, P.gy);
     int2 rg[R][R];
@@ -445,7 +465,7 @@ __device__ __forceinline__ int sup_thread(const DS& S, int k, const GeoRec& me, 
     }
     const int f = me.id;
     const double rad = me.r + P.rtree + P.skin, reach = P.rtree + P.skin;
-    int n = 0;
+    int n = 0, no = 0;
     for (int v0 = 0; v0 < L; v0 += SCAN_BATCH) {
         GeoRec g[SCAN_BATCH];
         int qq[SCAN_BATCH];
@@ -470,11 +490,16 @@ __device__ __forceinline__ int sup_thread(const DS& S, int k, const GeoRec& me, 
             const double thr = g[u].id > f ? rad : g[u].r + reach;
             // a superset: the margin only adds entries
             if (dx * dx + dy * dy <= thr * thr * (1.0 + 1e-12)) {
-                if (FILL) out[n] = ent(g[u].id, q);
+                if (FILL && n < total) {   // (the two passes count alike)
+                    const bool outer = g[u].id > f;
+                    out[outer ? no : total - 1 - (n - no)] = ent(g[u].id, q);
+                    no += outer;
+                }
                 ++n;
             }
         }
     }
+    if (FILL) *nout = no;
     return n;
 }
 
@@ -519,82 +544,90 @@ __device__ __forceinline__ void cand_thread(const DS& S, int k, const GeoRec& me
         re[t] = INT64_MAX;
         ri[t] = INT64_MAX;
     }
-    for (int v0 = 0; v0 < L; v0 += SCAN_BATCH) {
-        GeoRec g[SCAN_BATCH];
-        int qq[SCAN_BATCH];
+    // the outer entries (j > f) first, then the inner ones: each pass runs
+    // one insertion network, whichever side each lane's entry falls on
+    auto scan = [&](auto outer_tag, int lo, int hi) {
+        constexpr bool OUTER = decltype(outer_tag)::value;
+        for (int v0 = lo; v0 < hi; v0 += SCAN_BATCH) {
+            GeoRec g[SCAN_BATCH];
+            int qq[SCAN_BATCH];
 #pragma unroll
-        for (int u = 0; u < SCAN_BATCH; ++u)
-            qq[u] = ent_pos(se[min(v0 + u, L - 1)]);   // the last entry stands in past the end
+            for (int u = 0; u < SCAN_BATCH; ++u)
+                qq[u] = ent_pos(se[min(v0 + u, hi - 1)]);   // the last entry stands in past the end
 #pragma unroll
-        for (int u = 0; u < SCAN_BATCH; ++u) g[u] = S.geo[qq[u]];
+            for (int u = 0; u < SCAN_BATCH; ++u) g[u] = S.geo[qq[u]];
 #pragma unroll
-        for (int u = 0; u < SCAN_BATCH; ++u) {
-            const int q = qq[u];
-            if (v0 + u >= L) continue;
-            const GeoRec* o = &g[u];
-            const double dx = me.x - o->x, dy = me.y - o->y;
-            const double dist2 = dx * dx + dy * dy;
-            const int j = o->id;
-            const double thr = j > f ? rad : o->r + P.rtree;
-            // far beyond the radius: no square root needed (sqrt is
-            // correctly rounded, so it cannot cross a margin of 1e-12)
-            if (dist2 > thr * thr * (1.0 + 1e-12)) continue;
-            const double d = sqrt(dist2);
-            if (!(d <= thr)) continue;
-            if (j > f) {
-                if (no >= S.CO) { over |= ERR_OUTER_CAP; continue; }
-                // insertion by (distance, id) into the register part; the
-                // entry that leaves it is the largest so far
-                double xd = d;
-                int64_t xe = ent(j, q);
+            for (int u = 0; u < SCAN_BATCH; ++u) {
+                const int q = qq[u];
+                if (v0 + u >= hi) continue;
+                const GeoRec* o = &g[u];
+                const double dx = me.x - o->x, dy = me.y - o->y;
+                const double dist2 = dx * dx + dy * dy;
+                const int j = o->id;
+                const double thr = OUTER ? rad : o->r + P.rtree;
+                // far beyond the radius: no square root needed (sqrt is
+                // correctly rounded, so it cannot cross a margin of 1e-12)
+                if (dist2 > thr * thr * (1.0 + 1e-12)) continue;
+                const double d = sqrt(dist2);
+                if (!(d <= thr)) continue;
+                if (OUTER) {
+                    if (no >= S.CO) { over |= ERR_OUTER_CAP; continue; }
+                    // insertion by (distance, id) into the register part; the
+                    // entry that leaves it is the largest so far
+                    double xd = d;
+                    int64_t xe = ent(j, q);
 #pragma unroll
-                for (int t = 0; t < LIST_REGS; ++t) {
-                    const bool lt = xd < rd[t] || (xd == rd[t] && ent_id(xe) < ent_id(re[t]));
-                    const double td = rd[t];
-                    const int64_t te = re[t];
-                    rd[t] = lt ? xd : td;
-                    re[t] = lt ? xe : te;
-                    xd = lt ? td : xd;
-                    xe = lt ? te : xe;
-                }
-                if (no >= LIST_REGS) {
-                    int p = no;
-                    while (p > LIST_REGS) {
-                        double dp = S.od[(p - 1) * SS + k];
-                        int64_t ep = oj[(p - 1) * SS + k];
-                        if (dp < xd || (dp == xd && ent_id(ep) < ent_id(xe))) break;
-                        S.od[p * SS + k] = dp;
-                        oj[p * SS + k] = ep;
-                        --p;
+                    for (int t = 0; t < LIST_REGS; ++t) {
+                        const bool lt = xd < rd[t] || (xd == rd[t] && ent_id(xe) < ent_id(re[t]));
+                        const double td = rd[t];
+                        const int64_t te = re[t];
+                        rd[t] = lt ? xd : td;
+                        re[t] = lt ? xe : te;
+                        xd = lt ? td : xd;
+                        xe = lt ? te : xe;
                     }
-                    S.od[p * SS + k] = xd;
-                    oj[p * SS + k] = xe;
-                }
-                ++no;
-            } else {
-                if (ni >= S.CI) { over |= ERR_INNER_CAP; continue; }
-                int64_t xe = ent(j, q);
-#pragma unroll
-                for (int t = 0; t < LIST_REGS; ++t) {
-                    const bool lt = xe < ri[t];
-                    const int64_t te = ri[t];
-                    ri[t] = lt ? xe : te;
-                    xe = lt ? te : xe;
-                }
-                if (ni >= LIST_REGS) {
-                    int p = ni;
-                    while (p > LIST_REGS) {
-                        int64_t ep = ik[(p - 1) * SS + k];
-                        if (ep < xe) break;
-                        ik[p * SS + k] = ep;
-                        --p;
+                    if (no >= LIST_REGS) {
+                        int p = no;
+                        while (p > LIST_REGS) {
+                            double dp = S.od[(p - 1) * SS + k];
+                            int64_t ep = oj[(p - 1) * SS + k];
+                            if (dp < xd || (dp == xd && ent_id(ep) < ent_id(xe))) break;
+                            S.od[p * SS + k] = dp;
+                            oj[p * SS + k] = ep;
+                            --p;
+                        }
+                        S.od[p * SS + k] = xd;
+                        oj[p * SS + k] = xe;
                     }
-                    ik[p * SS + k] = xe;
+                    ++no;
+                } else {
+                    if (ni >= S.CI) { over |= ERR_INNER_CAP; continue; }
+                    int64_t xe = ent(j, q);
+#pragma unroll
+                    for (int t = 0; t < LIST_REGS; ++t) {
+                        const bool lt = xe < ri[t];
+                        const int64_t te = ri[t];
+                        ri[t] = lt ? xe : te;
+                        xe = lt ? te : xe;
+                    }
+                    if (ni >= LIST_REGS) {
+                        int p = ni;
+                        while (p > LIST_REGS) {
+                            int64_t ep = ik[(p - 1) * SS + k];
+                            if (ep < xe) break;
+                            ik[p * SS + k] = ep;
+                            --p;
+                        }
+                        ik[p * SS + k] = xe;
+                    }
+                    ++ni;
                 }
-                ++ni;
             }
         }
-    }
+    };
+    const int L_out = S.sno[k];
+    scan(std::true_type{}, 0, L_out);
+    scan(std::false_type{}, L_out, L);
     // sheltering and classification over the lists just built (separate
     // from the scan, so that a wavefront's lanes only diverge over their own
     // few candidates).  A minimum and an "any" do not depend on the order of
@@ -672,7 +705,7 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_sup(DS S) {
             if (P.dense) {
                 qd = !FILL;
             } else if (FILL) {
-                sup_thread<1, true>(S, k, me, P, S.sup + S.soff[k]);
+                sup_thread<1, true>(S, k, me, P, S.sup + S.soff[k], S.scnt[k], S.sno + k);
                 S.x0[S.cval[k]] = make_double2(me.x, me.y);
             } else {
                 S.scnt[k] = sup_thread<1, false>(S, k, me, P, nullptr);
@@ -1607,7 +1640,7 @@ __device__ void repair_serial(const DS& S, const DevParams& P, int first, int nm
         const int kf = S.pos_of[fs];
         ++st.repaired;
         if (s == WALL_SLOT) {
-            Floe F = load_floe_cur(S, fs);
+            Floe F = load_floe_cur(S, fs, kf);
             double hist[4] = {S.wbc[0 * SS + fs], S.wbc[1 * SS + fs], S.wbc[2 * SS + fs], S.wbc[3 * SS + fs]};
             WallOut W;
             walls<MIXED>(P, F, hist, W);
@@ -1624,13 +1657,13 @@ __device__ void repair_serial(const DS& S, const DevParams& P, int first, int nm
         }
         if (b != st.cur_block) {
             st.cur_block = b;
-            st.rq = S.dirty[fs] ? S.cur[fs] : S.rec[fs].r;   // query radius at the top of iteration f
+            st.rq = S.dirty[fs] ? S.cur[fs] : S.geo[kf].r;   // query radius at the top of iteration f
         }
         const int64_t e = S.oj[(int64_t)s * SS + kf];
         const int j = ent_id(e);
         const int js = S.cval[ent_pos(e)];
-        Floe Fi = load_floe_cur(S, fs);
-        Floe Fj = load_floe_cur(S, js);
+        Floe Fi = load_floe_cur(S, fs, kf);
+        Floe Fj = load_floe_cur(S, js, ent_pos(e));
         S.affected[fs] = 1;
         S.affected[js] = 1;
         EventOut o;
@@ -1762,7 +1795,7 @@ __global__ __launch_bounds__(64) void k_repair(DS S, int nmembers) {
             kf = S.pos_of[fs];
         }
         if (wall) {
-            Floe F = load_floe_cur(S, fs);
+            Floe F = load_floe_cur(S, fs, kf);
             double hist[4] = {S.wbc[0 * SS + fs], S.wbc[1 * SS + fs], S.wbc[2 * SS + fs], S.wbc[3 * SS + fs]};
             WallOut W;
             walls<MIXED>(P, F, hist, W);
@@ -1771,12 +1804,12 @@ __global__ __launch_bounds__(64) void k_repair(DS S, int nmembers) {
             nr = W.nr; nh = W.nh; nmass = W.nmass; ninertia = W.ninertia; nhfa = W.nhfa; nhfw = W.nhfw;
         }
         if (pair) {
-            rql = b == st.cur_block ? st.rq : (S.dirty[fs] ? S.cur[fs] : S.rec[fs].r);
+            rql = b == st.cur_block ? st.rq : (S.dirty[fs] ? S.cur[fs] : S.geo[kf].r);
             const int64_t e = S.oj[(int64_t)s * SS + kf];
             j = ent_id(e);
             js = S.cval[ent_pos(e)];
-            Floe Fi = load_floe_cur(S, fs);
-            Floe Fj = load_floe_cur(S, js);
+            Floe Fi = load_floe_cur(S, fs, kf);
+            Floe Fj = load_floe_cur(S, js, ent_pos(e));
             const double dx = Fi.x - Fj.x, dy = Fi.y - Fj.y;
             const double d = sqrt(dx * dx + dy * dy);
             load_pair_state(S, fs, j, ps);
@@ -2105,7 +2138,6 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_final(DS S) {
     out.u = u; out.v = v; out.omega = om;
     out.r = F.r; out.h = F.h; out.mass = F.mass; out.inertia = F.inertia; out.hfa = F.hfa; out.hfw = F.hfw;
     out.theta = S.kin[k].theta + om * P.dt;
-    S.rec[f] = out;
     {
         // next step builds the supersets anew if this floe has moved more
         // than 0.45 skin since the last build (a pair not in a superset was
@@ -2134,8 +2166,9 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_final(DS S) {
     // flags set by the repair are cleared (most floes have none: no write)
     if (dirty) S.dirty[f] = 0;
     if (S.affected[f]) S.affected[f] = 0;
-    // the next step's copies in the current order (k_ranges), unless it
-    // rebuilds them: position, kinematics, the wind at the new position
+    // the new state, in the current order (the next step's copies unless it
+    // rebuilds them from rec): position, kinematics, the wind at the new
+    // position
     {
         GeoRec g;
         g.x = out.x; g.y = out.y; g.r = out.r; g.id = me.id; g.member = m;
@@ -2151,6 +2184,7 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_final(DS S) {
         S.shel[k] = h;
     }
     if (f == S.members[m].first) {   // once per member
+        S.rebuild[2] = 1;                // the copies hold the new state
         S.members[m].tstep += 1;
         if (!member_live(S.members[m])) S.rebuild[1] = 1;   // its floes leave the next step
     }
