@@ -596,6 +596,7 @@ int godar_create(int64_t max_floes, int32_t max_members, int32_t precision, goda
     ACHK(dalloc(&d.cur, 6 * S));
     ACHK(dalloc(&d.dirty, S));
     ACHK(dalloc(&d.affected, S));
+    ACHK(dalloc(&d.wnz, S));
     ACHK(dalloc(&d.stats, 4 * (int64_t)max_members));
     ACHK(dalloc(&d.fout, 33 * S));
     ACHK(dalloc(&d.err, 1));
@@ -618,6 +619,7 @@ int godar_create(int64_t max_floes, int32_t max_members, int32_t precision, goda
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, c->stream, d.rebuild + 2, 1, 0);
     (void)hipMemsetAsync(d.dirty, 0, S, c->stream);
     (void)hipMemsetAsync(d.affected, 0, S, c->stream);
+    (void)hipMemsetAsync(d.wnz, 1, S, c->stream);
     (void)hipMemsetAsync(d.stats, 0, sizeof(int64_t) * 4 * max_members, c->stream);
     hipLaunchKernelGGL(k_fill_i32, dim3((S + 255) / 256), dim3(256), 0, c->stream, d.member_of, S, -1);
     for (int k = 0; k < 2; ++k) {
@@ -651,7 +653,7 @@ int godar_destroy(godar_ctx* c) {
     free_lists(c);
     dfree(d.wout); dfree(d.wflag); dfree(d.cls); dfree(d.qlist); dfree(d.qn); dfree(d.bgat); dfree(d.seeds); dfree(d.nseeds); dfree(d.heap); dfree(d.etab);
     dfree(d.netab); dfree(d.wtab); dfree(d.nwtab); dfree(d.ovw); dfree(d.dirty);
-    dfree(d.affected); dfree(d.stats); dfree(d.err); dfree(d.steps_done);
+    dfree(d.affected); dfree(d.wnz); dfree(d.stats); dfree(d.err); dfree(d.steps_done);
     dfree(d.sup); dfree(d.scnt); dfree(d.sno); dfree(d.soff); dfree(d.sblk); dfree(d.x0); dfree(d.rebuild);
     radix_free(c->rs);
     collect_events(c);
@@ -676,7 +678,7 @@ static int64_t bytes_per_floe(const godar_ctx* c) {
     per += (int64_t)c->BC * sizeof(BondTerms) + 1 /*bgat*/;
     per += NACC * 8 + sizeof(WallOut) + 1 /*wflag*/ + 1 /*cls*/ + 4 * 4 /*qlist*/;
     per += (int64_t)c->SEEDK * 8 + (int64_t)c->HEAPK * 8 + (int64_t)c->TABK * sizeof(EventOut);
-    per += sizeof(WallOut) /*wtab*/ + 4 /*ovw*/ + 6 * 8 /*cur*/ + 1 + 1 /*dirty, affected*/;
+    per += sizeof(WallOut) /*wtab*/ + 4 /*ovw*/ + 6 * 8 /*cur*/ + 1 + 1 + 1 /*dirty, affected, wnz*/;
     per += 33 * 8 /*fout*/ + 4 + 4 /*radix keys, values*/;
     per += (c->S > 0 ? c->sup_cap / c->S : 0) * 8 /*superset pool share*/ + 4 + 4 + 8 + 16 /*scnt sno soff x0*/;
     per += 1 /*sblk, per SCAN_TILE floes*/;
@@ -807,6 +809,7 @@ int godar_member_add(godar_ctx* c, const godar_params* p, int32_t n, const doubl
     }
     HIPCHK(c, hipMemset(d.dirty + first, 0, n));
     HIPCHK(c, hipMemset(d.affected + first, 0, n));
+    HIPCHK(c, hipMemset(d.wnz + first, 0, n));        // wall history rows zeroed above
     std::vector<int32_t> mo(n, m);
     HIPCHK(c, hipMemcpy(d.member_of + first, mo.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
     c->dparams[m] = dp;
@@ -1027,6 +1030,7 @@ int godar_member_restore(godar_ctx* c, const void* buf, int64_t size, int32_t* m
         for (int i = 0; i < n; ++i) row[slot[i]] = wbc[(size_t)4 * i + k];
         HIPCHK(c, hipMemcpy(c->d.wbc + k * S + mm.first, row.data(), sizeof(double) * n, hipMemcpyHostToDevice));
     }
+    HIPCHK(c, hipMemset(c->d.wnz + mm.first, 1, n));   // restored rows may be nonzero
     std::vector<int32_t> cnt(n);
     for (int i = 0; i < n; ++i) cnt[slot[i]] = hc[i];
     HIPCHK(c, hipMemcpy(c->hcnt[o] + mm.first, cnt.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));

@@ -144,6 +144,8 @@ struct DS {
     int32_t* nwtab;
     double* cur;               // [6][S] r h mass inertia hfa hfw during repair
     uint8_t *dirty, *affected; // [S]
+    uint8_t* wnz;              // [S] by storage slot: 0 = the floe's wall history rows (wbc)
+                               //     are all +0.0 (k_final then skips rewriting them)
     int64_t* stats;            // [members][4]
     double* fout;              // [33][S] (id)
     uint32_t* err;

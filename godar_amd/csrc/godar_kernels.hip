@@ -2106,8 +2106,17 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_final(DS S) {
         W.deltat_bc1 = W.deltat_bc2 = W.theta_bc1 = W.theta_bc2 = 0.0;
         W.sxx_bc = W.syy_bc = W.sxy_bc = W.syx_bc = 0.0;
     }
-    S.wbc[0 * SS + f] = W.deltat_bc1; S.wbc[1 * SS + f] = W.deltat_bc2;
-    S.wbc[2 * SS + f] = W.theta_bc1; S.wbc[3 * SS + f] = W.theta_bc2;
+    {
+        // wall histories: rows that are all +0.0 and stay so are not rewritten
+        const bool had = S.wnz[f] != 0;
+        const bool now = (__double_as_longlong(W.deltat_bc1) | __double_as_longlong(W.deltat_bc2) |
+                          __double_as_longlong(W.theta_bc1) | __double_as_longlong(W.theta_bc2)) != 0;
+        if (had || now) {
+            S.wbc[0 * SS + f] = W.deltat_bc1; S.wbc[1 * SS + f] = W.deltat_bc2;
+            S.wbc[2 * SS + f] = W.theta_bc1; S.wbc[3 * SS + f] = W.theta_bc2;
+        }
+        if (had != now) S.wnz[f] = now;
+    }
     double a[NACC];
     const uint8_t cl = S.cls[k];
 #pragma unroll
@@ -2177,11 +2186,16 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_final(DS S) {
         q.u = out.u; q.v = out.v; q.omega = out.omega; q.h = out.h; q.mass = out.mass;
         q.inertia = out.inertia; q.theta = out.theta;
         S.kin[k] = q;
-        double ua = P.ua, va = P.va;
-        if (P.wind_grid >= 0) sample_wind(S.wgrids[P.wind_grid], S.wpool, out.x, out.y, ua, va);
-        ShelRec h;
-        h.hfa = out.hfa; h.hfw = out.hfw; h.wx = ua; h.wy = va;
-        S.shel[k] = h;
+        // the sheltering copy changes only with a reshape or a gridded wind
+        // (the uniform wind stands until the host changes it, which
+        // refreshes the copies)
+        if (dirty || P.wind_grid >= 0) {
+            double ua = P.ua, va = P.va;
+            if (P.wind_grid >= 0) sample_wind(S.wgrids[P.wind_grid], S.wpool, out.x, out.y, ua, va);
+            ShelRec h;
+            h.hfa = out.hfa; h.hfw = out.hfw; h.wx = ua; h.wy = va;
+            S.shel[k] = h;
+        }
     }
     if (f == S.members[m].first) {   // once per member
         S.rebuild[2] = 1;                // the copies hold the new state
