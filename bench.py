@@ -90,7 +90,9 @@ def run_batched(args):
     else:
         blocks = ctx.autotune(2)      # real steps: part of the warm-up
     ctx.sync()
-    # timed region: graph-replayed steps, nothing else on the stream
+    launch = "graph replay" if ctx.graphs_on() else "one-by-one launches, bond gather beside the repair"
+    # timed region: steps in the launch mode the autotuner chose (graph
+    # replay or one-by-one launches), nothing else on the GPU
     t0 = time.perf_counter()
     ctx.step_async(args.steps)
     ctx.sync()
@@ -105,7 +107,7 @@ def run_batched(args):
     ctx.profile(False)
     ncand, ncont, nbond, tot = pair_totals(ctx, mids)
     ctx.close()
-    return dict(nfloes=nfloes, members=len(batch), seconds=dt, stages=stages, blocks=blocks,
+    return dict(nfloes=nfloes, members=len(batch), seconds=dt, stages=stages, blocks=blocks, launch=launch,
                 ncand=ncand, ncont=ncont, nbond=nbond, totals=tot,
                 alg_bytes_step=algorithmic_bytes(nfloes, ncand, ncont, nbond))
 
@@ -339,6 +341,7 @@ def main():
         "stages_ms_per_step": {k: v[0] / v[1] for k, v in stages.items()},
         "pair_totals_last_step": B["totals"],
         "block_sizes": B["blocks"],
+        "launch_mode": B["launch"],
     }
     if not args.quick:
         p0, ms0 = run_plain("config0", 500, 20)

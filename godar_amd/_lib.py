@@ -32,7 +32,7 @@ EXPORTS = [
     "godar_step", "godar_step_async", "godar_sync", "godar_member_tstep", "godar_reserve",
     "godar_member_run_until", "godar_member_tsteps",
     "godar_get_floes", "godar_record_forces", "godar_get_forces", "godar_get_walls",
-    "godar_get_pairs", "godar_get_bond_state", "godar_get_stats", "godar_set_block", "godar_use_graphs",
+    "godar_get_pairs", "godar_get_bond_state", "godar_get_stats", "godar_set_block", "godar_use_graphs", "godar_graphs_on",
     "godar_set_stage_block", "godar_get_stage_blocks", "godar_autotune",
     "godar_set_dense_search", "godar_set_repair", "godar_set_skin", "godar_checkpoint_size", "godar_checkpoint_save",
     "godar_member_restore",
@@ -95,6 +95,7 @@ def load(path: str = LIB_PATH):
         "godar_get_stats": (_i32, [_vp, _i32, _lp]),
         "godar_set_block": (_i32, [_vp, _i32]),
         "godar_use_graphs": (_i32, [_vp, _i32]),
+        "godar_graphs_on": (_i32, [_vp, _ip]),
         "godar_set_stage_block": (_i32, [_vp, _i32, _i32]),
         "godar_get_stage_blocks": (_i32, [_vp, _ip]),
         "godar_autotune": (_i32, [_vp, _i32]),

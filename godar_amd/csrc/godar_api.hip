@@ -6,6 +6,7 @@
 #include "radix_sort.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -49,6 +50,10 @@ struct Range { int64_t first, n; };
 
 struct godar_ctx {
     hipStream_t stream = nullptr;
+    // the bond gather runs beside the ordered repair on a second stream
+    // (fork and join events; default priority like the main stream)
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     int precision = GODAR_FP64;
     int64_t S = 0;
     int Mcap = 0;
@@ -93,6 +98,7 @@ struct godar_ctx {
     // device-state pointers by value), each with the launch signature it was
     // captured for
     bool use_graphs = true;
+    bool capturing = false;             // launch_step is being captured into a graph
     int dense_scan = 96;                // floes per 3x3 search above which a member is dense
     int repair_lanes = 64;              // repair events evaluated at once
     int repair_cap = 1024;              // pending repair keys held in LDS
@@ -405,17 +411,33 @@ int launch_step(godar_ctx* c) {
                            dim3(B[5]), 0, s, d);
     }
     {
-        StageTimer t(c, 6);
-        hipLaunchKernelGGL(k_bond_gather, grid_for(N, B[6], true), dim3(B[6]), 0, s, d);
-    }
-    {
         StageTimer t(c, 7);
         hipLaunchKernelGGL(mixed ? k_walls<true> : k_walls<false>, grid_for(N, B[7], true),
                            dim3(B[7]), 0, s, d);
     }
-    {
-        StageTimer t(c, 8);
+    if (!c->prof && !c->capturing) {
+        // The ordered repair is one wavefront per member and leaves most of
+        // the GPU idle; the bond gather runs beside it.  They share no data
+        // but the event flags of repaired pairs, whose owners the fixup
+        // recomputes whole (k_bond_gather's reading of them is bounded).
+        // A captured graph keeps the two in sequence: replayed with a fork
+        // it ran slower than either.
+        HIPCHK(c, hipEventRecord(c->ev_fork, s));
+        HIPCHK(c, hipStreamWaitEvent(c->side, c->ev_fork, 0));
+        hipLaunchKernelGGL(k_bond_gather, grid_for(N, B[6], true), dim3(B[6]), 0, c->side, d);
+        HIPCHK(c, hipEventRecord(c->ev_join, c->side));
         hipLaunchKernelGGL(mixed ? k_repair<true> : k_repair<false>, dim3(nm), dim3(64), 0, s, d, nm);
+        HIPCHK(c, hipStreamWaitEvent(s, c->ev_join, 0));
+    } else {
+        // timed stage by stage: one after the other (same results)
+        {
+            StageTimer t(c, 6);
+            hipLaunchKernelGGL(k_bond_gather, grid_for(N, B[6], true), dim3(B[6]), 0, s, d);
+        }
+        {
+            StageTimer t(c, 8);
+            hipLaunchKernelGGL(mixed ? k_repair<true> : k_repair<false>, dim3(nm), dim3(64), 0, s, d, nm);
+        }
     }
     {
         StageTimer t(c, 9);
@@ -470,7 +492,9 @@ int graph_step(godar_ctx* c) {
         c->gexec[par] = nullptr;
         c->gsig[par].clear();
         HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        c->capturing = true;
         int rc = launch_step(c);
+        c->capturing = false;
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(c->stream, &g);
         c->parity = par;   // the captured step has not run yet
@@ -556,6 +580,9 @@ int godar_create(int64_t max_floes, int32_t max_members, int32_t precision, goda
         }                                              \
     } while (0)
     if (e != hipSuccess) { delete c; return GODAR_ERR_HIP; }
+    ACHK(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+    ACHK(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+    ACHK(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
     ACHK(dalloc(&d.rec, S));
     ACHK(dalloc(&d.geo, S));
     ACHK(dalloc(&d.shel, S));
@@ -658,6 +685,9 @@ int godar_destroy(godar_ctx* c) {
     radix_free(c->rs);
     collect_events(c);
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
+    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+    if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+    if (c->side) (void)hipStreamDestroy(c->side);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return GODAR_OK;
@@ -1270,7 +1300,24 @@ int godar_autotune(godar_ctx* c, int32_t trial) {
         c->sblock[st] = best;
     }
     int r2 = godar_profile(c, prof0 ? 1 : 0);
-    return r2;
+    if (r2 != GODAR_OK || prof0 || !c->use_graphs) return r2;
+    // launch mode: graph replay or one-by-one launches (these run the bond
+    // gather beside the ordered repair), whichever steps faster
+    const int n = std::max(4, 4 * trial);
+    double t[2] = {0.0, 0.0};
+    for (int mode = 0; mode < 2; ++mode) {
+        c->use_graphs = mode == 0;
+        rc = godar_step(c, 1);                     // capture / settle
+        if (rc != GODAR_OK) { c->use_graphs = true; return rc; }
+        const auto t0 = std::chrono::steady_clock::now();
+        rc = godar_step(c, n);
+        if (rc == GODAR_OK) rc = godar_sync(c);
+        if (rc != GODAR_OK) { c->use_graphs = true; return rc; }
+        t[mode] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+    c->use_graphs = t[0] <= t[1];
+    if (!c->use_graphs) drop_graphs(c);
+    return GODAR_OK;
 }
 
 int godar_record_forces(godar_ctx* c, int32_t on) {
@@ -1299,6 +1346,12 @@ int godar_step_async(godar_ctx* c, int32_t nsteps) {
         int rc = graphs ? graph_step(c) : launch_step(c);
         if (rc != GODAR_OK) return rc;
     }
+    return GODAR_OK;
+}
+
+int godar_graphs_on(const godar_ctx* c, int32_t* on) {
+    if (!c || !on) return GODAR_ERR_ARG;
+    *on = c->use_graphs ? 1 : 0;
     return GODAR_OK;
 }
 

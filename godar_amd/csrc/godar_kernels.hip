@@ -1399,8 +1399,12 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_bond_gather(DS S) {
     }
     const int no = S.ocnt[k];
     for (int s = 0; s < no; ++s) {
+        // (the ordered repair may rewrite the flags of a floe's pairs
+        // meanwhile, without a record index; the fixup then recomputes the
+        // floe whole, and no index is formed from them here)
         const uint32_t fl = S.evf[s * SS + k];
-        if (fl & EV_BONDED) add_bond_i(a, S.bterm[(int64_t)((fl >> 16) - 1) * SS + fs]);
+        const int rq = (int)(fl >> 16);
+        if ((fl & EV_BONDED) && rq >= 1 && rq <= S.BC) add_bond_i(a, S.bterm[(int64_t)(rq - 1) * SS + fs]);
     }
     S.acc[A_FBX * SS + k] = a[A_FBX];
     S.acc[A_FBY * SS + k] = a[A_FBY];

@@ -181,13 +181,20 @@ class Context:
         return {self.L.godar_stage_name(k).decode(): int(t[k]) for k in range(_lib.NSTAGES)}
 
     def autotune(self, trial: int = 3) -> dict:
-        """Pick each stage's block size by timing real steps (members advance)."""
+        """Pick each stage's block size, then the launch mode (graph replay or
+        one-by-one launches), by timing real steps (members advance)."""
         self._chk(self.L.godar_autotune(self.h, int(trial)))
         return self.stage_blocks()
 
     def use_graphs(self, on: bool = True):
         """Replay a captured HIP graph per step (default) or launch kernels one by one."""
         self._chk(self.L.godar_use_graphs(self.h, int(bool(on))))
+
+    def graphs_on(self) -> bool:
+        """The current launch mode (godar_autotune may change it)."""
+        v = ctypes.c_int32(0)
+        self._chk(self.L.godar_graphs_on(self.h, ctypes.byref(v)))
+        return bool(v.value)
 
     def profile(self, on: bool = True):
         self._chk(self.L.godar_profile(self.h, 1 if on else 0))

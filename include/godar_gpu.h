@@ -204,15 +204,20 @@ int godar_set_repair(godar_ctx *ctx, int32_t lanes, int32_t pending);
 int godar_set_skin(godar_ctx *ctx, double frac);
 int godar_get_stage_blocks(const godar_ctx *ctx, int32_t *threads);   /* GODAR_NSTAGES entries */
 /* Autotuner: for each tunable stage in turn, time `trial` real steps of the
- * current members with each block size and keep the fastest.  The members
- * advance by 1 + 3 * trial * (tunable stages) steps. */
+ * current members with each block size and keep the fastest.  Then, unless
+ * graphs are off (godar_use_graphs) or steps are being profiled, time
+ * max(4, 4 trial) steps in each launch mode and keep the faster one.  The
+ * members advance by 1 + 3 * trial * (tunable stages) steps, plus
+ * 2 * (1 + max(4, 4 trial)) for the launch mode. */
 int godar_autotune(godar_ctx *ctx, int32_t trial);
 /* Step launch mode (no counterpart in the reference): on (the default), each
  * step replays a captured HIP graph of the whole kernel sequence, recaptured
  * whenever members, capacities or the block size change; off, the kernels are
- * launched one by one.  Results are identical.  Steps timed by godar_profile
- * are always launched one by one. */
+ * launched one by one, and the bond gather runs on a second stream beside the
+ * ordered repair.  Results are identical.  Steps timed by godar_profile are
+ * always launched one by one on one stream. */
 int godar_use_graphs(godar_ctx *ctx, int32_t on);
+int godar_graphs_on(const godar_ctx *ctx, int32_t *on);   /* the current launch mode */
 
 /*
  * Stage timing (tooling; no counterpart in the reference, which only times
