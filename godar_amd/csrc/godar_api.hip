@@ -588,6 +588,7 @@ int godar_create(int64_t max_floes, int32_t max_members, int32_t precision, goda
     ACHK(dalloc(&d.shel, S));
     ACHK(dalloc(&d.kin, S));
     ACHK(dalloc(&d.pos_of, S));
+    ACHK(dalloc(&d.pos_id, S));
     ACHK(dalloc(&d.wbc, 4 * S));
     ACHK(dalloc(&d.member_of, S));
     ACHK(dalloc(&d.gid_of, S));
@@ -669,7 +670,7 @@ int godar_destroy(godar_ctx* c) {
     DS& d = c->d;
     double* dbl[] = {d.wbc, d.acc, d.cur, d.fout};
     for (double* p : dbl) dfree(p);
-    dfree(d.rec); dfree(d.geo); dfree(d.shel); dfree(d.kin); dfree(d.pos_of);
+    dfree(d.rec); dfree(d.geo); dfree(d.shel); dfree(d.kin); dfree(d.pos_of); dfree(d.pos_id);
     dfree(d.member_of); dfree(d.gid_of); dfree(d.sid_of); dfree(d.members);
     DevParams* pp = const_cast<DevParams*>(d.params); dfree(pp);
     WindGridDesc* wg = const_cast<WindGridDesc*>(d.wgrids); dfree(wg);
@@ -701,7 +702,7 @@ const char* godar_last_error(const godar_ctx* c) { return c ? c->err.c_str() : "
 static int64_t bytes_per_floe(const godar_ctx* c) {
     int64_t per = 0;
     per += sizeof(FloeRec) + sizeof(GeoRec) + sizeof(ShelRec) + sizeof(KinRec);
-    per += 4 /*pos_of*/ + 4 * 8 /*wbc*/ + 4 /*member_of*/ + 4 + 4 /*ckey, cval*/;
+    per += 4 + 4 /*pos_of, pos_id*/ + 4 * 8 /*wbc*/ + 4 /*member_of*/ + 4 + 4 /*ckey, cval*/;
     per += 4 + 4 /*ocnt, icnt*/ + 2 * (4 + 4) /*hcnt, bcnt*/;
     per += (int64_t)c->CO * (8 + 8 + 4 + 4 + 8 + 8) + (int64_t)c->CI * (8 + 4);   // oj od evf ov ssc ssb; ik ibond
     per += 2 * ((int64_t)c->HC * sizeof(HistRec) + (int64_t)c->BC * sizeof(BondRec));

@@ -84,6 +84,7 @@ struct DS {
     ShelRec* shel;             // [S] pos (wind at the floe this step)
     KinRec* kin;               // [S] pos
     int32_t* pos_of;           // [S] storage slot -> pos
+    int32_t* pos_id;           // [S] identity -> pos (floes of live members)
     // candidate lists (pos rows; entries are partner positions)
     int32_t* ocnt;
     int64_t* oj;

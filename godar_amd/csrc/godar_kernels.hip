@@ -157,28 +157,45 @@ __device__ __forceinline__ int cell_coord(double p, double cell, int g) {
 }
 
 // Old pair state of (i, j): owner i by storage slot, partner j by identity.
+// Lists are searched with their first LIST_PEEK entries read at once (a
+// floe's lists are short): one round of loads instead of one per entry.
+constexpr int LIST_PEEK = 4;
+
+// Index of the first of owner i's n records (slot-major rows of stride SS)
+// whose partner is j (j >= 0; empty records hold -1), or -1.
+template <int PEEK = LIST_PEEK, typename Rec>
+__device__ __forceinline__ int find_partner(const Rec* list, int64_t SS, int i, int n, int j) {
+    int id[PEEK];
+#pragma unroll
+    for (int p = 0; p < PEEK; ++p) id[p] = p < n ? list[(int64_t)p * SS + i].j : -1;
+    int q = -1;
+#pragma unroll
+    for (int p = PEEK - 1; p >= 0; --p) q = id[p] == j ? p : q;
+    for (int p = PEEK; q < 0 && p < n; ++p)
+        if (list[(int64_t)p * SS + i].j == j) q = p;
+    return q;
+}
+
+// (PEEK 1: one entry at a time, for kernels short of registers)
+template <int PEEK = LIST_PEEK>
 __device__ __forceinline__ void load_pair_state(const DS& S, int i, int j, PairState& ps) {
     ps.deltat = 0.0; ps.thetarelc = 0.0; ps.fcn = 0.0; ps.fct = 0.0;
     ps.bond = 0; ps.has_bond_rec = false;
     ps.damage = ps.deltanb = ps.deltatb = ps.trb_ji = ps.trb_ij = 0.0;
     ps.rb = ps.hb = ps.lb = ps.sb = ps.ib = 0.0;
-    int hc = S.hcnt_old[i];
-    for (int q = 0; q < hc; ++q) {
-        const HistRec& hr = S.hist_old[(int64_t)q * S.S + i];
-        if (hr.j == j) {
-            ps.deltat = hr.deltat; ps.thetarelc = hr.thetarelc; ps.fcn = hr.fcn; ps.fct = hr.fct;
-            break;
-        }
+    // both lists are searched at once
+    const int hc = S.hcnt_old[i], bc = S.bcnt_old[i];
+    const int hq = find_partner<PEEK>(S.hist_old, S.S, i, hc, j);
+    const int bq = find_partner<PEEK>(S.bond_old, S.S, i, bc, j);
+    if (hq >= 0) {
+        const HistRec& hr = S.hist_old[(int64_t)hq * S.S + i];
+        ps.deltat = hr.deltat; ps.thetarelc = hr.thetarelc; ps.fcn = hr.fcn; ps.fct = hr.fct;
     }
-    int bc = S.bcnt_old[i];
-    for (int q = 0; q < bc; ++q) {
-        const BondRec& b = S.bond_old[(int64_t)q * S.S + i];
-        if (b.j == j) {
-            ps.has_bond_rec = true; ps.bond = b.bond; ps.damage = b.damage;
-            ps.deltanb = b.deltanb; ps.deltatb = b.deltatb; ps.trb_ji = b.trb_ji; ps.trb_ij = b.trb_ij;
-            ps.rb = b.rb; ps.hb = b.hb; ps.lb = b.lb; ps.sb = b.sb; ps.ib = b.ib;
-            break;
-        }
+    if (bq >= 0) {
+        const BondRec& b = S.bond_old[(int64_t)bq * S.S + i];
+        ps.has_bond_rec = true; ps.bond = b.bond; ps.damage = b.damage;
+        ps.deltanb = b.deltanb; ps.deltatb = b.deltatb; ps.trb_ji = b.trb_ji; ps.trb_ij = b.trb_ij;
+        ps.rb = b.rb; ps.hb = b.hb; ps.lb = b.lb; ps.sb = b.sb; ps.ib = b.ib;
     }
 }
 
@@ -345,6 +362,7 @@ __global__ void k_ranges(DS S) {
     S.pos_of[f] = k;
     double ua = 0.0, va = 0.0;
     if (m >= 0) {
+        S.pos_id[g.id] = k;
         if (*S.rebuild) {
             const uint32_t key = S.ckey[k];
             if (k == 0 || S.ckey[k - 1] != key) S.crange[key].x = k;
@@ -1489,20 +1507,25 @@ __device__ __forceinline__ void set_cur(const DS& S, int g, double r, double h, 
 // slot of partner id g in the outer list of the floe at position kp
 __device__ __forceinline__ int slot_of(const DS& S, int kp, int g) {
     const int64_t* oj = S.oj;
-    int no = S.ocnt[kp];
-    for (int s = 0; s < no; ++s)
-        if (ent_id(oj[(int64_t)s * S.S + kp]) == g) return s;
-    return -1;
+    const int no = S.ocnt[kp];
+    // the first LIST_PEEK entries are read at once, the rest one by one
+    int id[LIST_PEEK];
+#pragma unroll
+    for (int s = 0; s < LIST_PEEK; ++s) id[s] = s < no ? ent_id(oj[(int64_t)s * S.S + kp]) : -1;
+    int r = -1;
+#pragma unroll
+    for (int s = LIST_PEEK - 1; s >= 0; --s) r = id[s] == g ? s : r;
+    for (int s = LIST_PEEK; r < 0 && s < no; ++s)
+        if (ent_id(oj[(int64_t)s * S.S + kp]) == g) r = s;
+    return r;
 }
 
 // write (or clear) owner i's new contact history entry for partner j; the
 // entry lives while any of its four values is non-zero
 __device__ __forceinline__ bool put_hist(const DS& S, int i, int j, const PairState& ps) {
     const bool want = ps.deltat != 0.0 || ps.thetarelc != 0.0 || ps.fcn != 0.0 || ps.fct != 0.0;
-    int n = S.hcnt_new[i];
-    int q = -1;
-    for (int p = 0; p < n; ++p)
-        if (S.hist_new[(int64_t)p * S.S + i].j == j) { q = p; break; }
+    const int n = S.hcnt_new[i];
+    int q = find_partner(S.hist_new, S.S, i, n, j);
     if (want) {
         if (q < 0) {
             if (n >= S.HC) return false;
@@ -1519,10 +1542,8 @@ __device__ __forceinline__ bool put_hist(const DS& S, int i, int j, const PairSt
 }
 
 __device__ __forceinline__ bool put_bond(const DS& S, int i, int j, bool want, const BondRec& b) {
-    int n = S.bcnt_new[i];
-    int q = -1;
-    for (int p = 0; p < n; ++p)
-        if (S.bond_new[(int64_t)p * S.S + i].j == j) { q = p; break; }
+    const int n = S.bcnt_new[i];
+    int q = find_partner(S.bond_new, S.S, i, n, j);
     if (want) {
         if (q < 0) {
             if (n >= S.BC) return false;
@@ -1536,59 +1557,70 @@ __device__ __forceinline__ bool put_bond(const DS& S, int i, int j, bool want, c
     return true;
 }
 
-// put_hist / put_bond for the events of several lanes at once.  Lanes may
-// share the owner i (never the pair): every lane finds and updates its own
-// entry, then the lanes that append do so one at a time, in lane order.
+// Appends record r of every lane in `ma` to its owner i's list (count cnt[i],
+// capacity cap) in lane order.  The lanes of one owner are adjacent in the
+// batch (the keys are sorted by owner), so each takes the owner's count plus
+// its rank among them, and the last of them writes the new count: no lane
+// waits for another.  Every lane of the wavefront calls it.
+template <typename Rec>
+__device__ __forceinline__ void append_lanes(Rec* list, int32_t* cnt, int64_t SS, int cap,
+                                             uint64_t ma, int i, int n, const Rec& r,
+                                             uint32_t& over, uint32_t err) {
+    const int lane = (int)(threadIdx.x & 63);
+    const bool mine = (ma >> lane) & 1ull;
+    const uint64_t below = ma & ((1ull << lane) - 1ull);
+    const uint64_t above = ma & ~((2ull << lane) - 1ull);
+    const int prev = below ? 63 - __clzll((long long)below) : lane;
+    const int next = above ? __ffsll((long long)above) - 1 : lane;
+    const int iprev = __shfl(i, prev), inext = __shfl(i, next);
+    const bool head = mine && (!below || iprev != i);
+    const bool last = mine && (!above || inext != i);
+    const uint64_t hb = __ballot(head) & ((2ull << lane) - 1ull);
+    const int start = hb ? 63 - __clzll((long long)hb) : 0;
+    const int rank = __popcll(below & ~((1ull << start) - 1ull));
+    if (mine) {
+        const int at = n + rank;
+        if (at >= cap) over |= err;
+        else list[(int64_t)at * SS + i] = r;
+        if (last) cnt[i] = min(at + 1, cap);
+    }
+}
+
+// Lane versions of put_hist / put_bond for the batched repair: the lanes in
+// `on` hold committed events, in key order.  Records found are replaced (or
+// cleared) in place; new ones are appended in lane order.
 __device__ __forceinline__ void put_hist_lanes(const DS& S, bool on, int i, int j,
                                                const PairState& ps, uint32_t& over) {
     const bool want = on && (ps.deltat != 0.0 || ps.thetarelc != 0.0 || ps.fcn != 0.0 || ps.fct != 0.0);
     HistRec hr;
     hr.j = j; hr.pad_ = 0; hr.deltat = ps.deltat; hr.thetarelc = ps.thetarelc; hr.fcn = ps.fcn; hr.fct = ps.fct;
-    int q = -1;
+    int q = -1, n = 0;
     if (on) {
-        const int n = S.hcnt_new[i];
-        for (int p = 0; p < n; ++p)
-            if (S.hist_new[(int64_t)p * S.S + i].j == j) { q = p; break; }
+        n = S.hcnt_new[i];
+        q = find_partner(S.hist_new, S.S, i, n, j);
         if (q >= 0) {
             if (want) S.hist_new[(int64_t)q * S.S + i] = hr;
             else S.hist_new[(int64_t)q * S.S + i].j = -1;
         }
     }
-    uint64_t ma = __ballot(want && q < 0);
-    while (ma) {
-        const int l = __ffsll((long long)ma) - 1;
-        if ((int)(threadIdx.x & 63) == l) {
-            const int n = S.hcnt_new[i];
-            if (n >= S.HC) over |= ERR_HIST_CAP;
-            else { S.hist_new[(int64_t)n * S.S + i] = hr; S.hcnt_new[i] = n + 1; }
-        }
-        ma &= ma - 1;
-    }
+    append_lanes(S.hist_new, S.hcnt_new, S.S, S.HC, __ballot(want && q < 0), i, n, hr, over,
+                 ERR_HIST_CAP);
 }
 
 __device__ __forceinline__ void put_bond_lanes(const DS& S, bool on, int i, int j, bool want,
                                                const BondRec& b, uint32_t& over) {
     want = want && on;
-    int q = -1;
+    int q = -1, n = 0;
     if (on) {
-        const int n = S.bcnt_new[i];
-        for (int p = 0; p < n; ++p)
-            if (S.bond_new[(int64_t)p * S.S + i].j == j) { q = p; break; }
+        n = S.bcnt_new[i];
+        q = find_partner(S.bond_new, S.S, i, n, j);
         if (q >= 0) {
             if (want) S.bond_new[(int64_t)q * S.S + i] = b;
             else S.bond_new[(int64_t)q * S.S + i].j = -1;
         }
     }
-    uint64_t ma = __ballot(want && q < 0);
-    while (ma) {
-        const int l = __ffsll((long long)ma) - 1;
-        if ((int)(threadIdx.x & 63) == l) {
-            const int n = S.bcnt_new[i];
-            if (n >= S.BC) over |= ERR_BOND_CAP;
-            else { S.bond_new[(int64_t)n * S.S + i] = b; S.bcnt_new[i] = n + 1; }
-        }
-        ma &= ma - 1;
-    }
+    append_lanes(S.bond_new, S.bcnt_new, S.S, S.BC, __ballot(want && q < 0), i, n, b, over,
+                 ERR_BOND_CAP);
 }
 
 // Progress of one member's repair (the same in every lane).
@@ -1604,10 +1636,9 @@ struct RepairState {
 // the pairs (i2, g) with f < i2 < g, then g's own events after s (all of
 // them if g != f) and g's walls.
 __device__ __forceinline__ void push_later_events(const DS& S, Heap& H, int first, int f, int s,
-                                                  int g, int gs, uint32_t& over) {
+                                                  int g, int kg, uint32_t& over) {
     const int64_t SS = S.S;
     const int gb = g - first;
-    const int kg = S.pos_of[gs];
     if (g != f) {
         const int gi = S.icnt[kg];
         for (int q = 0; q < gi; ++q) {
@@ -1641,7 +1672,7 @@ __device__ void repair_serial(const DS& S, const DevParams& P, int first, int nm
         if (key >> 16 >= (uint64_t)nmem || (s != WALL_SLOT && s >= S.CO)) { over |= ERR_REPAIR_CAP; break; }
         const int f = first + b;              // identity
         const int fs = S.sid_of[f];           // storage slot
-        const int kf = S.pos_of[fs];
+        const int kf = S.pos_id[f];           // position (loaded beside the slot)
         ++st.repaired;
         if (s == WALL_SLOT) {
             Floe F = load_floe_cur(S, fs, kf);
@@ -1688,10 +1719,11 @@ __device__ void repair_serial(const DS& S, const DevParams& P, int first, int nm
         if (o.flags & EV_RIDGED) {
             const int g = o.ridged == 0 ? f : j;
             const int gs = o.ridged == 0 ? fs : js;
+            const int kg = o.ridged == 0 ? kf : ent_pos(e);
             set_cur(S, gs, o.nr, o.nh, o.nmass, o.ninertia, o.nhfa, o.nhfw);
             if (!S.dirty[gs]) {
                 S.dirty[gs] = 1;
-                push_later_events(S, H, first, f, s, g, gs, over);
+                push_later_events(S, H, first, f, s, g, kg, over);
             }
         }
         if (st.nt >= tcap) { over |= ERR_REPAIR_CAP; break; }
@@ -1789,14 +1821,14 @@ __global__ __launch_bounds__(64) void k_repair(DS S, int nmembers) {
         const uint64_t mw = __ballot(wall), me = __ballot(pair);
         const int rank_w = __popcll(mw & lt), rank_e = __popcll(me & lt);
         if (st.nt + __popcll(me) > tcap || st.nwt + __popcll(mw) > nmem) { over |= ERR_REPAIR_CAP; break; }
-        int fs = 0, kf = 0, j = 0, js = 0, ridged = -1;
+        int fs = 0, kf = 0, j = 0, js = 0, kj = 0, ridged = -1;
         bool reshape = false;
         uint32_t oflags = 0;
         double rql = 0.0, nr = 0.0, nh = 0.0, nmass = 0.0, ninertia = 0.0, nhfa = 0.0, nhfw = 0.0;
         PairState ps;
         if (act) {
             fs = S.sid_of[f];
-            kf = S.pos_of[fs];
+            kf = S.pos_id[f];
         }
         if (wall) {
             Floe F = load_floe_cur(S, fs, kf);
@@ -1811,9 +1843,10 @@ __global__ __launch_bounds__(64) void k_repair(DS S, int nmembers) {
             rql = b == st.cur_block ? st.rq : (S.dirty[fs] ? S.cur[fs] : S.geo[kf].r);
             const int64_t e = S.oj[(int64_t)s * SS + kf];
             j = ent_id(e);
-            js = S.cval[ent_pos(e)];
+            kj = ent_pos(e);
+            js = S.cval[kj];
             Floe Fi = load_floe_cur(S, fs, kf);
-            Floe Fj = load_floe_cur(S, js, ent_pos(e));
+            Floe Fj = load_floe_cur(S, js, kj);
             const double dx = Fi.x - Fj.x, dy = Fi.y - Fj.y;
             const double d = sqrt(dx * dx + dy * dy);
             load_pair_state(S, fs, j, ps);
@@ -1848,11 +1881,12 @@ __global__ __launch_bounds__(64) void k_repair(DS S, int nmembers) {
         }
         put_hist_lanes(S, com && pair, fs, j, ps, over);
         put_bond_lanes(S, com && pair, fs, j, ps.has_bond_rec, bond_from(ps, j), over);
-        int g = 0, gs = 0;
+        int g = 0, gs = 0, kg = 0;
         bool adds = false;
         if (com && reshape) {
             g = (wall || ridged == 0) ? f : j;
             gs = (wall || ridged == 0) ? fs : js;
+            kg = (wall || ridged == 0) ? kf : kj;
             set_cur(S, gs, nr, nh, nmass, ninertia, nhfa, nhfw);
             if (wall) S.dirty[gs] = 1;
             else if (!S.dirty[gs]) { S.dirty[gs] = 1; adds = true; }
@@ -1876,11 +1910,10 @@ __global__ __launch_bounds__(64) void k_repair(DS S, int nmembers) {
             continue;
         }
         // the reshape of lane c adds every later event of its floe g
-        const int gc = __shfl(g, c), gsc = __shfl(gs, c), fc = __shfl(f, c), sc = __shfl(s, c);
-        const int kg = S.pos_of[gsc];
-        const int gi = gc != fc ? S.icnt[kg] : 0;
+        const int gc = __shfl(g, c), fc = __shfl(f, c), sc = __shfl(s, c), kgc = __shfl(kg, c);
+        const int gi = gc != fc ? S.icnt[kgc] : 0;
         const int s0 = gc == fc ? sc + 1 : 0;
-        const int gno = S.ocnt[kg];
+        const int gno = S.ocnt[kgc];
         const int rest = np - consumed;
         const int kmax = gi + max(gno - s0, 0) + 1;
         if (kmax > REPAIR_NCAP || rest + kmax > cap) {
@@ -1888,7 +1921,7 @@ __global__ __launch_bounds__(64) void k_repair(DS S, int nmembers) {
             if (lane == 0) {
                 for (int i = 0; i < rest; ++i)
                     if (!H.push((uint64_t)pk[cb][head + consumed + i])) over |= ERR_REPAIR_CAP;
-                push_later_events(S, H, first, fc, sc, gc, gsc, over);
+                push_later_events(S, H, first, fc, sc, gc, kgc, over);
             }
             serial = true;
             break;
@@ -1899,7 +1932,7 @@ __global__ __launch_bounds__(64) void k_repair(DS S, int nmembers) {
             bool ok = false;
             int64_t nkey = 0;
             if (q < gi) {
-                const int64_t ei = S.ik[(int64_t)q * SS + kg];
+                const int64_t ei = S.ik[(int64_t)q * SS + kgc];
                 const int i2 = ent_id(ei);
                 if (i2 > fc) {
                     const int sk = slot_of(S, ent_pos(ei), gc);
@@ -1972,7 +2005,7 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_fixup(DS S) {
             Floe Fi = floe_at(S, qi);
             Floe Ff = F0;
             PairState ps;
-            load_pair_state(S, S.cval[qi], f, ps);
+            load_pair_state<1>(S, S.cval[qi], f, ps);
             const double2 wi = wind_at(S, qi);
             EventOut o;
             pair_event<MIXED>(P, tstep, i - first, f - first, Fi, Ff, wi.x, wi.y, P.uw, P.vw, ps, o, 2);
@@ -1992,7 +2025,7 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_fixup(DS S) {
             Floe Fi = F0;
             Floe Fj = floe_at(S, qj);
             PairState ps;
-            load_pair_state(S, fs, j, ps);
+            load_pair_state<1>(S, fs, j, ps);
             EventOut o;
             pair_event<MIXED>(P, tstep, f - first, j - first, Fi, Fj, wf.x, wf.y, P.uw, P.vw, ps, o, 1);
             apply_i(acc, o);
