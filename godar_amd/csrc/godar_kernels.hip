@@ -1864,9 +1864,44 @@ __global__ __launch_bounds__(64) void k_repair(DS S, int nmembers) {
             ridged = o.ridged;
             nr = o.nr; nh = o.nh; nmass = o.nmass; ninertia = o.ninertia; nhfa = o.nhfa; nhfw = o.nhfw;
         }
-        // commit every event up to and including the first reshape
-        const uint64_t mres = __ballot(reshape);
-        const int c = mres ? __ffsll((long long)mres) - 1 : nb - 1;
+        // Commit a prefix of the batch.  A reshape of floe g makes the later
+        // events of g (keys above the reshape's) run again, and only what
+        // those events change can change what follows them: every event with
+        // a key below the least of them stands.  That key follows at once
+        // for a reshape of the owner (and a wall reshape, which ends the
+        // round too); for a reshape of the partner j it is the first of j's
+        // own events or of the events (i2, j) with i2 above the owner, i2
+        // from j's inner list (ascending).
+        int64_t reach = INT64_MAX;
+        if (reshape) {
+            if (wall || ridged == 0) {
+                reach = key + 1;
+            } else {
+                reach = (int64_t)(j - first) << 16;
+                const int nij = S.icnt[kj];
+                int64_t ids[LIST_PEEK];
+#pragma unroll
+                for (int q = 0; q < LIST_PEEK; ++q) ids[q] = q < nij ? S.ik[(int64_t)q * SS + kj] : INT64_MAX;
+                int i2 = INT_MAX;
+#pragma unroll
+                for (int q = LIST_PEEK - 1; q >= 0; --q)
+                    if (ids[q] != INT64_MAX && ent_id(ids[q]) > f) i2 = ent_id(ids[q]);
+                for (int q = LIST_PEEK; i2 == INT_MAX && q < nij; ++q) {
+                    const int id = ent_id(S.ik[(int64_t)q * SS + kj]);
+                    if (id > f) i2 = id;
+                }
+                if (i2 != INT_MAX) reach = min(reach, (int64_t)(i2 - first) << 16);
+            }
+        }
+        int end = nb;
+        for (uint64_t rm = __ballot(reshape); rm; rm &= rm - 1) {
+            const int cl = __ffsll((long long)rm) - 1;
+            if (cl >= end) break;
+            const int64_t ka = __shfl(reach, cl);
+            const uint64_t cut = __ballot(lane > cl && lane < end && key >= ka);
+            if (cut) end = __ffsll((long long)cut) - 1;
+        }
+        const int c = end - 1;                  // the last lane committed
         const bool com = act && lane <= c;
         if (com && pair) {
             S.affected[fs] = 1;
@@ -1909,45 +1944,65 @@ __global__ __launch_bounds__(64) void k_repair(DS S, int nmembers) {
             np -= consumed;
             continue;
         }
-        // the reshape of lane c adds every later event of its floe g
-        const int gc = __shfl(g, c), fc = __shfl(f, c), sc = __shfl(s, c), kgc = __shfl(kg, c);
-        const int gi = gc != fc ? S.icnt[kgc] : 0;
-        const int s0 = gc == fc ? sc + 1 : 0;
-        const int gno = S.ocnt[kgc];
+        // each committed reshape (of a floe not reshaped before) adds every
+        // later event of its floe g: one sorted run per reshape, the runs
+        // sorted together when there are several
+        const uint64_t madd = __ballot(adds);
         const int rest = np - consumed;
-        const int kmax = gi + max(gno - s0, 0) + 1;
-        if (kmax > REPAIR_NCAP || rest + kmax > cap) {
+        int K = 0;
+        bool spill_heap = false;
+        for (uint64_t ma = madd; ma; ma &= ma - 1) {
+            const int cl = __ffsll((long long)ma) - 1;
+            const int gc = __shfl(g, cl), fc = __shfl(f, cl), kgc = __shfl(kg, cl);
+            const int sc = __shfl(s, cl);
+            const int gi = gc != fc ? S.icnt[kgc] : 0;
+            const int s0 = gc == fc ? sc + 1 : 0;
+            const int gno = S.ocnt[kgc];
+            const int kmax = gi + max(gno - s0, 0) + 1;
+            if (K + kmax > REPAIR_NCAP || rest + K + kmax > cap) { spill_heap = true; break; }
+            for (int q0 = 0; q0 < gi; q0 += 64) {
+                const int q = q0 + lane;
+                bool ok = false;
+                int64_t nkey = 0;
+                if (q < gi) {
+                    const int64_t ei = S.ik[(int64_t)q * SS + kgc];
+                    const int i2 = ent_id(ei);
+                    if (i2 > fc) {
+                        const int sk = slot_of(S, ent_pos(ei), gc);
+                        if (sk >= 0) { ok = true; nkey = ((int64_t)(i2 - first) << 16) | sk; }
+                    }
+                }
+                const uint64_t mk = __ballot(ok);
+                if (ok) nk[K + __popcll(mk & lt)] = nkey;
+                K += __popcll(mk);
+            }
+            for (int s2 = s0 + lane; s2 < gno; s2 += 64) nk[K + s2 - s0] = ((int64_t)(gc - first) << 16) | s2;
+            K += max(gno - s0, 0);
+            if (lane == 0) nk[K] = ((int64_t)(gc - first) << 16) | WALL_SLOT;
+            K += 1;
+        }
+        if (spill_heap) {
             // too many pending keys for LDS: on to the heap in HBM
             if (lane == 0) {
                 for (int i = 0; i < rest; ++i)
                     if (!H.push((uint64_t)pk[cb][head + consumed + i])) over |= ERR_REPAIR_CAP;
-                push_later_events(S, H, first, fc, sc, gc, kgc, over);
+            }
+            for (uint64_t ma = madd; ma; ma &= ma - 1) {
+                const int cl = __ffsll((long long)ma) - 1;
+                const int gc = __shfl(g, cl), fc = __shfl(f, cl), kgc = __shfl(kg, cl);
+                const int sc = __shfl(s, cl);
+                if (lane == 0) push_later_events(S, H, first, fc, sc, gc, kgc, over);
             }
             serial = true;
             break;
         }
-        int K = 0;
-        for (int q0 = 0; q0 < gi; q0 += 64) {
-            const int q = q0 + lane;
-            bool ok = false;
-            int64_t nkey = 0;
-            if (q < gi) {
-                const int64_t ei = S.ik[(int64_t)q * SS + kgc];
-                const int i2 = ent_id(ei);
-                if (i2 > fc) {
-                    const int sk = slot_of(S, ent_pos(ei), gc);
-                    if (sk >= 0) { ok = true; nkey = ((int64_t)(i2 - first) << 16) | sk; }
-                }
-            }
-            const uint64_t mk = __ballot(ok);
-            if (ok) nk[K + __popcll(mk & lt)] = nkey;
-            K += __popcll(mk);
-        }
-        for (int s2 = s0 + lane; s2 < gno; s2 += 64) nk[K + s2 - s0] = ((int64_t)(gc - first) << 16) | s2;
-        K += max(gno - s0, 0);
-        if (lane == 0) nk[K] = ((int64_t)(gc - first) << 16) | WALL_SLOT;
-        K += 1;
         wave_lds_sync();
+        if (__popcll(madd) > 1) {
+            const int p2 = pow2_at_least(K);
+            for (int i = K + lane; i < p2; i += 64) nk[i] = INT64_MAX;
+            wave_lds_sync();
+            bitonic_inner(nk, p2, lane);
+        }
         // merge the two sorted runs (pending keys first among equals)
         const int64_t* A = &pk[cb][head + consumed];
         int64_t* O = pk[cb ^ 1];
