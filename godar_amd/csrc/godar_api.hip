@@ -608,7 +608,7 @@ int godar_create(int64_t max_floes, int32_t max_members, int32_t precision, goda
     ACHK(alloc_lists(c));
     ACHK(dalloc(&d.acc, (int64_t)NACC * S));
     ACHK(dalloc(&d.wout, S));
-    ACHK(dalloc(&d.wflag, S));
+    ACHK(dalloc(&d.wflag, (S + 3) / 4 * 4));   // whole 32-bit words (wflag_set)
     ACHK(dalloc(&d.cls, S));
     ACHK(dalloc(&d.qlist, 4 * S));
     ACHK(dalloc(&d.bgat, S));
@@ -623,7 +623,6 @@ int godar_create(int64_t max_floes, int32_t max_members, int32_t precision, goda
     ACHK(dalloc(&d.ovw, S));
     ACHK(dalloc(&d.cur, 6 * S));
     ACHK(dalloc(&d.dirty, S));
-    ACHK(dalloc(&d.affected, S));
     ACHK(dalloc(&d.wnz, S));
     ACHK(dalloc(&d.stats, 4 * (int64_t)max_members));
     ACHK(dalloc(&d.fout, 33 * S));
@@ -646,7 +645,6 @@ int godar_create(int64_t max_floes, int32_t max_members, int32_t precision, goda
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, c->stream, d.rebuild, 2, 1);
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, c->stream, d.rebuild + 2, 1, 0);
     (void)hipMemsetAsync(d.dirty, 0, S, c->stream);
-    (void)hipMemsetAsync(d.affected, 0, S, c->stream);
     (void)hipMemsetAsync(d.wnz, 1, S, c->stream);
     (void)hipMemsetAsync(d.stats, 0, sizeof(int64_t) * 4 * max_members, c->stream);
     hipLaunchKernelGGL(k_fill_i32, dim3((S + 255) / 256), dim3(256), 0, c->stream, d.member_of, S, -1);
@@ -681,7 +679,7 @@ int godar_destroy(godar_ctx* c) {
     free_lists(c);
     dfree(d.wout); dfree(d.wflag); dfree(d.cls); dfree(d.qlist); dfree(d.qn); dfree(d.bgat); dfree(d.seeds); dfree(d.nseeds); dfree(d.heap); dfree(d.etab);
     dfree(d.netab); dfree(d.wtab); dfree(d.nwtab); dfree(d.ovw); dfree(d.dirty);
-    dfree(d.affected); dfree(d.wnz); dfree(d.stats); dfree(d.err); dfree(d.steps_done);
+    dfree(d.wnz); dfree(d.stats); dfree(d.err); dfree(d.steps_done);
     dfree(d.sup); dfree(d.scnt); dfree(d.sno); dfree(d.soff); dfree(d.sblk); dfree(d.x0); dfree(d.rebuild);
     radix_free(c->rs);
     collect_events(c);
@@ -709,7 +707,7 @@ static int64_t bytes_per_floe(const godar_ctx* c) {
     per += (int64_t)c->BC * sizeof(BondTerms) + 1 /*bgat*/;
     per += NACC * 8 + sizeof(WallOut) + 1 /*wflag*/ + 1 /*cls*/ + 4 * 4 /*qlist*/;
     per += (int64_t)c->SEEDK * 8 + (int64_t)c->HEAPK * 8 + (int64_t)c->TABK * sizeof(EventOut);
-    per += sizeof(WallOut) /*wtab*/ + 4 /*ovw*/ + 6 * 8 /*cur*/ + 1 + 1 + 1 /*dirty, affected, wnz*/;
+    per += sizeof(WallOut) /*wtab*/ + 4 /*ovw*/ + 6 * 8 /*cur*/ + 1 + 1 /*dirty, wnz*/;
     per += 33 * 8 /*fout*/ + 4 + 4 /*radix keys, values*/;
     per += (c->S > 0 ? c->sup_cap / c->S : 0) * 8 /*superset pool share*/ + 4 + 4 + 8 + 16 /*scnt sno soff x0*/;
     per += 1 /*sblk, per SCAN_TILE floes*/;
@@ -839,7 +837,6 @@ int godar_member_add(godar_ctx* c, const godar_params* p, int32_t n, const doubl
         HIPCHK(c, hipMemset(c->bcnt[k] + first, 0, sizeof(int32_t) * n));
     }
     HIPCHK(c, hipMemset(d.dirty + first, 0, n));
-    HIPCHK(c, hipMemset(d.affected + first, 0, n));
     HIPCHK(c, hipMemset(d.wnz + first, 0, n));        // wall history rows zeroed above
     std::vector<int32_t> mo(n, m);
     HIPCHK(c, hipMemcpy(d.member_of + first, mo.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
@@ -1387,7 +1384,6 @@ int godar_sync(godar_ctx* c) {
             bind_parity(c);
         }
         HIPCHK(c, hipMemset(c->d.dirty, 0, c->S));
-        HIPCHK(c, hipMemset(c->d.affected, 0, c->S));
         HIPCHK(c, hipMemset(c->d.err, 0, sizeof(uint32_t)));
         c->last_err_bits = e;
         return fail(c, GODAR_ERR_CAPACITY, "step aborted:" + err_text(e));

@@ -47,6 +47,11 @@ struct GeoRec {
 struct ShelRec { double hfa, hfw, wx, wy; };
 struct KinRec { double u, v, omega, h, mass, inertia, theta; };
 
+// Per-position step flags (wflag): the floe touched a wall this step (wout
+// holds its terms); the repair recomputed its walls (wtab entry in ovw); the
+// repair touched the floe, so k_fixup re-gathers it.
+constexpr uint32_t WF_WALL = 1, WF_REPAIR_WALL = 2, WF_FIXUP = 4;
+
 // Everything the kernels touch.  "id" arrays are indexed by a floe's storage
 // slot; "pos" arrays by the floe's position in this step's cell order.  Per-floe lists
 // are slot-major, entry s of row r at [s*S + r].
@@ -105,10 +110,9 @@ struct DS {
     int32_t* ibond;            // [CI][S] pos: 1 + owner's record slot of the inner pair's bond
     // speculative gather (pos)
     double* acc;               // [NACC][S]
-    WallOut* wout;             // [S], valid where wflag is set
-    uint8_t* wflag;            // [S] bit 0: the floe touched a wall this step; bit 1: the
-                               //     repair recomputed its walls (wtab entry in ovw)
-    int32_t* ovw;              // [S] repair wall-table entry, valid where wflag bit 1 is set
+    WallOut* wout;             // [S], valid where WF_WALL is set
+    uint8_t* wflag;            // [S rounded up to 4] WF_* bits
+    int32_t* ovw;              // [S] repair wall-table entry, valid where WF_REPAIR_WALL is set
     uint8_t* cls;              // [S] CLS_* bits
     // Candidate supersets: every floe within its query radius plus the
     // member's skin at the last build, (id << 32 | pos), floe k's at
@@ -144,7 +148,7 @@ struct DS {
     WallOut* wtab;             // one per floe slot
     int32_t* nwtab;
     double* cur;               // [6][S] r h mass inertia hfa hfw during repair
-    uint8_t *dirty, *affected; // [S]
+    uint8_t* dirty;            // [S]
     uint8_t* wnz;              // [S] by storage slot: 0 = the floe's wall history rows (wbc)
                                //     are all +0.0 (k_final then skips rewriting them)
     int64_t* stats;            // [members][4]

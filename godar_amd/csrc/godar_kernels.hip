@@ -1,7 +1,8 @@
 // Kernels of the GODAR step pipeline (one launch sequence per step):
 //
-//   k_bin -> radix sort -> k_clear_cells -> k_ranges -> k_cand -> k_contact
-//        -> k_bond -> k_bond_gather -> k_walls -> k_repair -> k_fixup -> k_final
+//   [build steps: k_spill -> k_bin -> radix sort -> k_clear_cells -> k_ranges
+//    -> k_sup ...] -> k_cand -> k_contact -> k_bond -> k_walls
+//    -> k_repair (beside it: k_bond_gather) -> k_fixup -> k_final
 //
 // Sequential semantics of the reference (stepper.f90:71-280 visits floes i
 // in index order and, for each, its candidates j > i in ascending distance;
@@ -133,6 +134,13 @@ __device__ __forceinline__ Floe floe_at(const DS& S, int k) {
 __device__ __forceinline__ double2 wind_at(const DS& S, int k) {
     const ShelRec h = S.shel[k];
     return make_double2(h.wx, h.wy);
+}
+
+// Sets flag bits of position k's wflag byte from any lane or wavefront: the
+// byte shares a 32-bit word with three neighbours whose bits others may be
+// setting at the same time (the array is allocated in whole words).
+__device__ __forceinline__ void wflag_set(const DS& S, int k, uint32_t bits) {
+    atomicOr(reinterpret_cast<uint32_t*>(S.wflag + (k & ~3)), bits << (8 * (k & 3)));
 }
 
 // floe at storage slot f and position k with the properties current during
@@ -1452,7 +1460,7 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_walls(DS S) {
     double hist[4] = {S.wbc[0 * SS + fs], S.wbc[1 * SS + fs], S.wbc[2 * SS + fs], S.wbc[3 * SS + fs]};
     WallOut W;
     walls<MIXED>(P, Fw, hist, W);
-    S.wflag[k] = (uint8_t)(W.flags & 1u);
+    S.wflag[k] = (uint8_t)((W.flags & 1u) ? WF_WALL : 0u);
     if (W.flags & 1u) S.wout[k] = W;
     if (W.flags & 4u) {
         int p = atomicAdd(&S.nseeds[m], 1);
@@ -1682,8 +1690,7 @@ __device__ void repair_serial(const DS& S, const DevParams& P, int first, int nm
             if (st.nwt >= nmem) { over |= ERR_REPAIR_CAP; break; }
             wtab[st.nwt] = W;
             S.ovw[kf] = st.nwt++;
-            S.wflag[kf] |= 2;
-            S.affected[fs] = 1;
+            wflag_set(S, kf, WF_REPAIR_WALL | WF_FIXUP);
             if (W.flags & 4u) {
                 set_cur(S, fs, W.nr, W.nh, W.nmass, W.ninertia, W.nhfa, W.nhfw);
                 S.dirty[fs] = 1;
@@ -1699,8 +1706,8 @@ __device__ void repair_serial(const DS& S, const DevParams& P, int first, int nm
         const int js = S.cval[ent_pos(e)];
         Floe Fi = load_floe_cur(S, fs, kf);
         Floe Fj = load_floe_cur(S, js, ent_pos(e));
-        S.affected[fs] = 1;
-        S.affected[js] = 1;
+        wflag_set(S, kf, WF_FIXUP);
+        wflag_set(S, ent_pos(e), WF_FIXUP);
         EventOut o;
         const double dx = Fi.x - Fj.x, dy = Fi.y - Fj.y;
         const double d = sqrt(dx * dx + dy * dy);
@@ -1904,15 +1911,14 @@ __global__ __launch_bounds__(64) void k_repair(DS S, int nmembers) {
         const int c = end - 1;                  // the last lane committed
         const bool com = act && lane <= c;
         if (com && pair) {
-            S.affected[fs] = 1;
-            S.affected[js] = 1;
+            wflag_set(S, kf, WF_FIXUP);
+            wflag_set(S, kj, WF_FIXUP);
             S.ov[(int64_t)s * SS + kf] = st.nt + rank_e;
             S.evf[(int64_t)s * SS + kf] = oflags | EV_TABLE;
         }
         if (com && wall) {
             S.ovw[kf] = st.nwt + rank_w;
-            S.wflag[kf] |= 2;
-            S.affected[fs] = 1;
+            wflag_set(S, kf, WF_REPAIR_WALL | WF_FIXUP);
         }
         put_hist_lanes(S, com && pair, fs, j, ps, over);
         put_bond_lanes(S, com && pair, fs, j, ps.has_bond_rec, bond_from(ps, j), over);
@@ -2030,13 +2036,13 @@ __global__ __launch_bounds__(64) void k_repair(DS S, int nmembers) {
 template <bool MIXED>
 __global__ __launch_bounds__(FLOE_BLOCK) void k_fixup(DS S) {
     int k = xcd_block() * blockDim.x + threadIdx.x;
-    if (k >= S.N || *S.err) return;
+    // floes the repair touched (one byte per position decides)
+    if (k >= S.N || *S.err || !(S.wflag[k] & WF_FIXUP)) return;
     const GeoRec me = S.geo[k];
     const int m = me.member;
     if (m < 0) return;
     const int f = me.id;
     const int fs = S.cval[k];
-    if (!S.affected[fs]) return;
     const DevParams& P = S.params[m];
     const int tstep = S.members[m].tstep;
     const int first = S.members[m].first;
@@ -2188,9 +2194,9 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_final(DS S) {
     }
     const uint8_t wf = S.wflag[k];
     WallOut W;
-    if (wf & 2) {
+    if (wf & WF_REPAIR_WALL) {
         W = S.wtab[S.members[m].first + S.ovw[k]];   // recomputed by the repair
-    } else if (wf & 1) {
+    } else if (wf & WF_WALL) {
         W = S.wout[k];
     } else {
         // no wall touched: reset_boundary(i, 0) cleared every wall history
@@ -2266,7 +2272,6 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_final(DS S) {
     }
     // flags set by the repair are cleared (most floes have none: no write)
     if (dirty) S.dirty[f] = 0;
-    if (S.affected[f]) S.affected[f] = 0;
     // the new state, in the current order (the next step's copies unless it
     // rebuilds them from rec): position, kinematics, the wind at the new
     // position
