@@ -184,6 +184,21 @@ __device__ __forceinline__ int find_partner(const Rec* list, int64_t SS, int i, 
     return q;
 }
 
+// Slot of partner g among the first n entries of the slot-major list row kp
+// (outer oj or inner ik), or -1; the first LIST_PEEK entries are read at
+// once, the rest one by one.
+__device__ __forceinline__ int find_entry(const int64_t* list, int64_t SS, int kp, int n, int g) {
+    int id[LIST_PEEK];
+#pragma unroll
+    for (int s = 0; s < LIST_PEEK; ++s) id[s] = s < n ? ent_id(list[(int64_t)s * SS + kp]) : -1;
+    int r = -1;
+#pragma unroll
+    for (int s = LIST_PEEK - 1; s >= 0; --s) r = id[s] == g ? s : r;
+    for (int s = LIST_PEEK; r < 0 && s < n; ++s)
+        if (ent_id(list[(int64_t)s * SS + kp]) == g) r = s;
+    return r;
+}
+
 // (PEEK 1: one entry at a time, for kernels short of registers)
 template <int PEEK = LIST_PEEK>
 __device__ __forceinline__ void load_pair_state(const DS& S, int i, int j, PairState& ps) {
@@ -1357,9 +1372,8 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_bond(DS S) {
             S.bterm[(int64_t)rq * SS + fs] = bt;
             S.evf[s * SS + k] |= EV_BONDED | ((uint32_t)(rq + 1) << 16);
             // tell the partner where its terms are: its inner slot of f
-            const int nij = S.icnt[qj];
-            for (int q = 0; q < nij; ++q)
-                if (ent_id(S.ik[(int64_t)q * SS + qj]) == f) { S.ibond[(int64_t)q * SS + qj] = rq + 1; break; }
+            const int qi = find_entry(S.ik, SS, qj, S.icnt[qj], f);
+            if (qi >= 0) S.ibond[(int64_t)qi * SS + qj] = rq + 1;
             S.bgat[qj] = 1;
             any = true;
         }
@@ -1372,9 +1386,7 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_bond(DS S) {
         const BondRec b = S.bond_old[q * SS + fs];
         if (b.j < 0) continue;
         const int rq = nb++;
-        int s = -1;
-        for (int s2 = 0; s2 < no; ++s2)
-            if (ent_id(oj[s2 * SS + k]) == b.j) { s = s2; break; }
+        const int s = find_entry(oj, SS, k, no, b.j);
         if (s < 0) {
             S.bond_new[(int64_t)rq * SS + fs] = b;
             continue;
@@ -1514,18 +1526,7 @@ __device__ __forceinline__ void set_cur(const DS& S, int g, double r, double h, 
 
 // slot of partner id g in the outer list of the floe at position kp
 __device__ __forceinline__ int slot_of(const DS& S, int kp, int g) {
-    const int64_t* oj = S.oj;
-    const int no = S.ocnt[kp];
-    // the first LIST_PEEK entries are read at once, the rest one by one
-    int id[LIST_PEEK];
-#pragma unroll
-    for (int s = 0; s < LIST_PEEK; ++s) id[s] = s < no ? ent_id(oj[(int64_t)s * S.S + kp]) : -1;
-    int r = -1;
-#pragma unroll
-    for (int s = LIST_PEEK - 1; s >= 0; --s) r = id[s] == g ? s : r;
-    for (int s = LIST_PEEK; r < 0 && s < no; ++s)
-        if (ent_id(oj[(int64_t)s * S.S + kp]) == g) r = s;
-    return r;
+    return find_entry(S.oj, S.S, kp, S.ocnt[kp], g);
 }
 
 // write (or clear) owner i's new contact history entry for partner j; the
