@@ -392,8 +392,9 @@ int launch_step(godar_ctx* c) {
     }
     {
         StageTimer t(c, 3);
+        // LDS: one list of (floe, candidate) pairs per wavefront
         hipLaunchKernelGGL(mixed ? k_cand<true> : k_cand<false>, grid_for(N, B[3], true),
-                           dim3(B[3]), 0, s, d);
+                           dim3(B[3]), (B[3] / 64) * CAND_LDS_PER_WAVE, s, d);
         if (any_dense) {
             // one wavefront per dense floe, a contiguous share of the list per XCD
             hipLaunchKernelGGL(mixed ? k_cand_dense<true> : k_cand_dense<false>,
@@ -631,7 +632,6 @@ int godar_create(int64_t max_floes, int32_t max_members, int32_t precision, goda
     c->sup_cap = 8 * S;
     ACHK(dalloc(&d.sup, c->sup_cap));
     ACHK(dalloc(&d.scnt, S));
-    ACHK(dalloc(&d.sno, S));
     ACHK(dalloc(&d.soff, S));
     ACHK(dalloc(&d.sblk, S / SCAN_TILE + 2));
     ACHK(dalloc(&d.x0, S));
@@ -680,7 +680,7 @@ int godar_destroy(godar_ctx* c) {
     dfree(d.wout); dfree(d.wflag); dfree(d.cls); dfree(d.qlist); dfree(d.qn); dfree(d.bgat); dfree(d.seeds); dfree(d.nseeds); dfree(d.heap); dfree(d.etab);
     dfree(d.netab); dfree(d.wtab); dfree(d.nwtab); dfree(d.ovw); dfree(d.dirty);
     dfree(d.wnz); dfree(d.stats); dfree(d.err); dfree(d.steps_done);
-    dfree(d.sup); dfree(d.scnt); dfree(d.sno); dfree(d.soff); dfree(d.sblk); dfree(d.x0); dfree(d.rebuild);
+    dfree(d.sup); dfree(d.scnt); dfree(d.soff); dfree(d.sblk); dfree(d.x0); dfree(d.rebuild);
     radix_free(c->rs);
     collect_events(c);
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
@@ -709,7 +709,7 @@ static int64_t bytes_per_floe(const godar_ctx* c) {
     per += (int64_t)c->SEEDK * 8 + (int64_t)c->HEAPK * 8 + (int64_t)c->TABK * sizeof(EventOut);
     per += sizeof(WallOut) /*wtab*/ + 4 /*ovw*/ + 6 * 8 /*cur*/ + 1 + 1 /*dirty, wnz*/;
     per += 33 * 8 /*fout*/ + 4 + 4 /*radix keys, values*/;
-    per += (c->S > 0 ? c->sup_cap / c->S : 0) * 8 /*superset pool share*/ + 4 + 4 + 8 + 16 /*scnt sno soff x0*/;
+    per += (c->S > 0 ? c->sup_cap / c->S : 0) * 8 /*superset pool share*/ + 4 + 8 + 16 /*scnt soff x0*/;
     per += 1 /*sblk, per SCAN_TILE floes*/;
     return per;
 }

@@ -125,8 +125,6 @@ struct DS {
     int64_t* sup;              // [sup_cap]
     int64_t sup_cap;
     int32_t* scnt;             // [S] by pos
-    int32_t* sno;              // [S] by pos: a thread-searched floe's superset holds its
-                               //     sno entries j > f first, then those with j < f
     int64_t* soff;             // [S] by pos
     int64_t* sblk;             // [S / SCAN_TILE + 2] scan of per-tile totals
     double2* x0;               // [S] by storage slot

@@ -32,8 +32,6 @@
 // a sorted copy of the 96-byte floe records, so that a floe's neighbours sit
 // close in memory.  Pair state persists per floe id.  List entries pack the
 // partner as (id << 32 | pos).
-#include <type_traits>
-
 #include "godar_engine.h"
 #include "godar_physics.h"
 
@@ -469,12 +467,10 @@ constexpr int LIST_REGS = 4;
 // one contiguous range of positions; every cell range is read at once
 // (indices clamped into the grid, cells outside it masked out afterwards) and
 // the rows, taken as one sequence, SCAN_BATCH records at a time.  FILL
-// writes the entries at `out`, the `total` (the count of the counting pass)
-// entries j > f from the front and the others from the back, and stores the
-// number of the former in *nout; either way the count is returned.
+// writes the entries at `out`; either way the count is returned.
 template <int SPAN, bool FILL>
 __device__ __forceinline__ int sup_thread(const DS& S, int k, const GeoRec& me, const DevParams& P,
-                                          int64_t* out, int total = 0, int32_t* nout = nullptr) {
+                                          int64_t* out) {
     constexpr int R = 2 * SPAN + 1;
     const int cx = cell_coord(me.x, P.cell, P.gx), cy = cell_coord(me.y, P.cell, P.gy);
     int2 rg[R][R];
@@ -506,7 +502,7 @@ __device__ __forceinline__ int sup_thread(const DS& S, int k, const GeoRec& me, 
     }
     const int f = me.id;
     const double rad = me.r + P.rtree + P.skin, reach = P.rtree + P.skin;
-    int n = 0, no = 0;
+    int n = 0;
     for (int v0 = 0; v0 < L; v0 += SCAN_BATCH) {
         GeoRec g[SCAN_BATCH];
         int qq[SCAN_BATCH];
@@ -531,18 +527,19 @@ __device__ __forceinline__ int sup_thread(const DS& S, int k, const GeoRec& me, 
             const double thr = g[u].id > f ? rad : g[u].r + reach;
             // a superset: the margin only adds entries
             if (dx * dx + dy * dy <= thr * thr * (1.0 + 1e-12)) {
-                if (FILL && n < total) {   // (the two passes count alike)
-                    const bool outer = g[u].id > f;
-                    out[outer ? no : total - 1 - (n - no)] = ent(g[u].id, q);
-                    no += outer;
-                }
+                if (FILL) out[n] = ent(g[u].id, q);
                 ++n;
             }
         }
     }
-    if (FILL) *nout = no;
     return n;
 }
+
+// (floe, candidate) pairs of one wavefront whose sheltering and
+// classification terms its lanes evaluate together (LDS per wavefront:
+// entry, distance, owner lane)
+constexpr int CAND_TCAP = 512;
+constexpr int CAND_LDS_PER_WAVE = CAND_TCAP * (8 + 8 + 1);
 
 // Neighbour search of one floe by one thread: its candidate lists are the
 // entries of its superset within the query radius, inserted into the
@@ -550,30 +547,34 @@ __device__ __forceinline__ int sup_thread(const DS& S, int k, const GeoRec& me, 
 // resets of the floe's rows.  Superset entries are read SCAN_BATCH at a time
 // and the head of each candidate list is sorted in registers, so that the
 // search waits on few dependent loads.
-template <bool MIXED>
-__device__ __forceinline__ void cand_thread(const DS& S, int k, const GeoRec& me, bool& qc,
-                                            bool& qb, bool& qw) {
+//
+// WAVE: every lane of the wavefront calls it (act: the lane has a floe), and
+// the sheltering and classification terms of all the wavefront's (floe,
+// candidate) pairs are evaluated together, 64 at a time, from a list in LDS
+// (`lds`, CAND_LDS_PER_WAVE bytes for this wavefront); each floe then takes
+// the minimum of its own terms, in its list order (a minimum of fmin terms
+// does not depend on which lane computed them).  Otherwise, or when the
+// pairs outnumber CAND_TCAP, each lane evaluates its own.
+template <bool MIXED, bool WAVE>
+__device__ __forceinline__ void cand_thread(const DS& S, int k, const GeoRec& me, bool act, bool& qc,
+                                            bool& qb, bool& qw, unsigned char* lds) {
     using T = typename Prec<MIXED>::T;
-    const int m = me.member;
+    const int m = act ? me.member : 0;
     const DevParams& P = S.params[m];
-    const double rad = me.r + P.rtree;
-    const int f = me.id;
+    const double rad = act ? me.r + P.rtree : 0.0;
+    const int f = act ? me.id : 0;
     const int first = S.members[m].first;
-    const bool bond_step = S.members[m].tstep == 1 && P.restart != 1 && P.cohesion;
-    const ShelRec msh = S.shel[k];
+    const bool bond_step = act && S.members[m].tstep == 1 && P.restart != 1 && P.cohesion;
+    ShelRec msh;
+    msh.hfa = msh.hfw = msh.wx = msh.wy = 0.0;
     const T uwi = T(P.uw), vwi = T(P.vw);
     T sha = T(1), shw = T(1);
-    const int fs = S.cval[k];
-    qc = S.hcnt_old[fs] > 0;
-    qb = S.bcnt_old[fs] > 0;
-    qw = me.y - me.r < 0 || me.y + me.r > P.ny || me.x - me.r < 0 || me.x + me.r > P.nx;
+    int fs = 0;
     int no = 0, ni = 0;
     uint32_t over = 0;
     const int64_t SS = S.S;
     int64_t* oj = S.oj;
     int64_t* ik = S.ik;
-    const int L = S.scnt[k];
-    const int64_t* se = S.sup + S.soff[k];
     // the first LIST_REGS entries of each list stay in registers while the
     // lists are built (sorted; empty slots hold keys above every entry), the
     // rest are sorted in their slots in HBM
@@ -585,33 +586,37 @@ __device__ __forceinline__ void cand_thread(const DS& S, int k, const GeoRec& me
         re[t] = INT64_MAX;
         ri[t] = INT64_MAX;
     }
-    // the outer entries (j > f) first, then the inner ones: each pass runs
-    // one insertion network, whichever side each lane's entry falls on
-    auto scan = [&](auto outer_tag, int lo, int hi) {
-        constexpr bool OUTER = decltype(outer_tag)::value;
-        for (int v0 = lo; v0 < hi; v0 += SCAN_BATCH) {
+    if (act) {
+        msh = S.shel[k];
+        fs = S.cval[k];
+        qc = S.hcnt_old[fs] > 0;
+        qb = S.bcnt_old[fs] > 0;
+        qw = me.y - me.r < 0 || me.y + me.r > P.ny || me.x - me.r < 0 || me.x + me.r > P.nx;
+        const int L = S.scnt[k];
+        const int64_t* se = S.sup + S.soff[k];
+        for (int v0 = 0; v0 < L; v0 += SCAN_BATCH) {
             GeoRec g[SCAN_BATCH];
             int qq[SCAN_BATCH];
 #pragma unroll
             for (int u = 0; u < SCAN_BATCH; ++u)
-                qq[u] = ent_pos(se[min(v0 + u, hi - 1)]);   // the last entry stands in past the end
+                qq[u] = ent_pos(se[min(v0 + u, L - 1)]);   // the last entry stands in past the end
 #pragma unroll
             for (int u = 0; u < SCAN_BATCH; ++u) g[u] = S.geo[qq[u]];
 #pragma unroll
             for (int u = 0; u < SCAN_BATCH; ++u) {
                 const int q = qq[u];
-                if (v0 + u >= hi) continue;
+                if (v0 + u >= L) continue;
                 const GeoRec* o = &g[u];
                 const double dx = me.x - o->x, dy = me.y - o->y;
                 const double dist2 = dx * dx + dy * dy;
                 const int j = o->id;
-                const double thr = OUTER ? rad : o->r + P.rtree;
+                const double thr = j > f ? rad : o->r + P.rtree;
                 // far beyond the radius: no square root needed (sqrt is
                 // correctly rounded, so it cannot cross a margin of 1e-12)
                 if (dist2 > thr * thr * (1.0 + 1e-12)) continue;
                 const double d = sqrt(dist2);
                 if (!(d <= thr)) continue;
-                if (OUTER) {
+                if (j > f) {
                     if (no >= S.CO) { over |= ERR_OUTER_CAP; continue; }
                     // insertion by (distance, id) into the register part; the
                     // entry that leaves it is the largest so far
@@ -665,57 +670,155 @@ __device__ __forceinline__ void cand_thread(const DS& S, int k, const GeoRec& me
                 }
             }
         }
-    };
-    const int L_out = S.sno[k];
-    scan(std::true_type{}, 0, L_out);
-    scan(std::false_type{}, L_out, L);
-    // sheltering and classification over the lists just built (separate
-    // from the scan, so that a wavefront's lanes only diverge over their own
-    // few candidates).  A minimum and an "any" do not depend on the order of
-    // their terms.
-    auto outer_terms = [&](int64_t e, double d) {
-        const int j = ent_id(e), q = ent_pos(e);
-        const GeoRec o = S.geo[q];
-        // pair (f, j): f is the outer floe i of the event
-        const double deltan = me.r + o.r - d;
-        if (deltan > 0) qc = true;
-        if (!qb && bond_step && deltan >= -P.bond_lim &&
-            bond_keep(f - first, j - first, P.bond_seed, P.bond_thr24))
-            qb = true;
-        if (P.shelter) {
-            // floe f takes hsf*(j,f), evaluated with -cosa, -sina
-            const double cosa = (o.x - me.x) / d, sina = (o.y - me.y) / d;
-            const ShelRec oh = S.shel[q];
-            const T dn = T(deltan), c = T(cosa), sn = T(sina);
-            sha = shelter_min(sha, T(oh.hfa), T(msh.hfa), dn, -c, -sn, T(msh.wx), T(msh.wy));
-            shw = shelter_min(shw, T(oh.hfw), T(msh.hfw), dn, -c, -sn, uwi, vwi);
-        }
-    };
-    auto inner_terms = [&](int64_t e) {
-        const int q = ent_pos(e);
-        const GeoRec o = S.geo[q];
-        const double dx = me.x - o.x, dy = me.y - o.y;
-        const double d = sqrt(dx * dx + dy * dy);
-        // pair (j, f): f is the inner floe j of the event
-        const double deltan = o.r + me.r - d;
-        if (deltan > 0) qc = true;
-        if (P.shelter) {
-            // cosa/sina point from j to f; floe f takes hsf*(j,f) with j's wind
-            const double cosa = (me.x - o.x) / d, sina = (me.y - o.y) / d;
-            const ShelRec oh = S.shel[q];
-            const T dn = T(deltan), c = T(cosa), sn = T(sina);
-            sha = shelter_min(sha, T(oh.hfa), T(msh.hfa), dn, c, sn, T(oh.wx), T(oh.wy));
-            shw = shelter_min(shw, T(oh.hfw), T(msh.hfw), dn, c, sn, uwi, vwi);
-        }
-    };
+    }
+    // sheltering and classification over the lists just built.  A minimum
+    // and an "any" do not depend on the order of their terms.
+    int total = CAND_TCAP + 1, base = 0;
+    const int cnt = act ? no + ni : 0;
+    if (WAVE) {
+        const int lane = (int)(threadIdx.x & 63);
+        int incl = cnt;
 #pragma unroll
-    for (int s2 = 0; s2 < LIST_REGS; ++s2)
-        if (s2 < no) outer_terms(re[s2], rd[s2]);
-    for (int s2 = LIST_REGS; s2 < no; ++s2) outer_terms(oj[s2 * SS + k], S.od[s2 * SS + k]);
+        for (int off = 1; off < 64; off <<= 1) {
+            const int v = __shfl_up(incl, off);
+            if (lane >= off) incl += v;
+        }
+        base = incl - cnt;
+        total = __shfl(incl, 63);
+    }
+    if (WAVE && total <= CAND_TCAP) {
+        const int lane = (int)(threadIdx.x & 63);
+        int64_t* te = reinterpret_cast<int64_t*>(lds);                     // entry, then the water term
+        double* td = reinterpret_cast<double*>(lds + 8 * CAND_TCAP);        // distance (inner: -1), then the air term
+        uint8_t* to = lds + 16 * CAND_TCAP;                                  // owner lane, then flags
+        if (act) {
 #pragma unroll
-    for (int s2 = 0; s2 < LIST_REGS; ++s2)
-        if (s2 < ni) inner_terms(ri[s2]);
-    for (int s2 = LIST_REGS; s2 < ni; ++s2) inner_terms(ik[s2 * SS + k]);
+            for (int s2 = 0; s2 < LIST_REGS; ++s2)
+                if (s2 < no) { te[base + s2] = re[s2]; td[base + s2] = rd[s2]; to[base + s2] = (uint8_t)lane; }
+            for (int s2 = LIST_REGS; s2 < no; ++s2) {
+                te[base + s2] = oj[s2 * SS + k];
+                td[base + s2] = S.od[s2 * SS + k];
+                to[base + s2] = (uint8_t)lane;
+            }
+#pragma unroll
+            for (int s2 = 0; s2 < LIST_REGS; ++s2)
+                if (s2 < ni) { te[base + no + s2] = ri[s2]; td[base + no + s2] = -1.0; to[base + no + s2] = (uint8_t)lane; }
+            for (int s2 = LIST_REGS; s2 < ni; ++s2) {
+                te[base + no + s2] = ik[s2 * SS + k];
+                td[base + no + s2] = -1.0;
+                to[base + no + s2] = (uint8_t)lane;
+            }
+        }
+        wave_lds_sync();
+        for (int c0 = 0; c0 < total; c0 += 64) {
+            const int idx = c0 + lane;
+            const bool valid = idx < total;
+            int64_t e = 0;
+            double d = -1.0;
+            int own = lane;
+            if (valid) { e = te[idx]; d = td[idx]; own = to[idx]; }
+            const int ko = __shfl(k, own);
+            const int mo_m = __shfl(m, own);
+            T ta = T(1), tw = T(1);
+            uint32_t fl = 0;
+            if (valid) {
+                const GeoRec mo = S.geo[ko];
+                const DevParams& Po = S.params[mo_m];
+                const int q = ent_pos(e);
+                const GeoRec o = S.geo[q];
+                const bool outer = d >= 0.0;
+                double dd = d;
+                if (!outer) {
+                    const double dx = mo.x - o.x, dy = mo.y - o.y;
+                    dd = sqrt(dx * dx + dy * dy);
+                }
+                // outer: pair (f, j), f the outer floe i of the event; inner:
+                // pair (j, f), f the inner floe j
+                const double deltan = outer ? mo.r + o.r - dd : o.r + mo.r - dd;
+                if (deltan > 0) fl |= 1u;
+                if (outer) {
+                    const int first_o = S.members[mo_m].first;
+                    const bool bstep = S.members[mo_m].tstep == 1 && Po.restart != 1 && Po.cohesion;
+                    if (bstep && deltan >= -Po.bond_lim &&
+                        bond_keep(mo.id - first_o, ent_id(e) - first_o, Po.bond_seed, Po.bond_thr24))
+                        fl |= 2u;
+                }
+                if (Po.shelter) {
+                    // outer: floe f takes hsf*(j,f), evaluated with -cosa,
+                    // -sina of the direction f -> j; inner: cosa/sina point
+                    // from j to f and f takes hsf*(j,f) with j's wind
+                    const ShelRec oh = S.shel[q];
+                    const ShelRec mh = S.shel[ko];
+                    const double cosa = outer ? (o.x - mo.x) / dd : (mo.x - o.x) / dd;
+                    const double sina = outer ? (o.y - mo.y) / dd : (mo.y - o.y) / dd;
+                    const T dn = T(deltan), c0t = T(cosa), s0t = T(sina);
+                    const T c = outer ? -c0t : c0t, sn = outer ? -s0t : s0t;
+                    const T ux = outer ? T(mh.wx) : T(oh.wx), uy = outer ? T(mh.wy) : T(oh.wy);
+                    ta = shelter_min(T(1), T(oh.hfa), T(mh.hfa), dn, c, sn, ux, uy);
+                    tw = shelter_min(T(1), T(oh.hfw), T(mh.hfw), dn, c, sn, T(Po.uw), T(Po.vw));
+                }
+            }
+            if (valid) {
+                td[idx] = (double)ta;
+                te[idx] = __double_as_longlong((double)tw);
+                to[idx] = (uint8_t)fl;
+            }
+        }
+        wave_lds_sync();
+        if (act) {
+            for (int i = base; i < base + cnt; ++i) {
+                sha = dmin(sha, T(td[i]));
+                shw = dmin(shw, T(__longlong_as_double(te[i])));
+                qc = qc || (to[i] & 1u);
+                qb = qb || (to[i] & 2u);
+            }
+        }
+    } else if (act) {
+        auto outer_terms = [&](int64_t e, double d) {
+            const int j = ent_id(e), q = ent_pos(e);
+            const GeoRec o = S.geo[q];
+            // pair (f, j): f is the outer floe i of the event
+            const double deltan = me.r + o.r - d;
+            if (deltan > 0) qc = true;
+            if (!qb && bond_step && deltan >= -P.bond_lim &&
+                bond_keep(f - first, j - first, P.bond_seed, P.bond_thr24))
+                qb = true;
+            if (P.shelter) {
+                // floe f takes hsf*(j,f), evaluated with -cosa, -sina
+                const double cosa = (o.x - me.x) / d, sina = (o.y - me.y) / d;
+                const ShelRec oh = S.shel[q];
+                const T dn = T(deltan), c = T(cosa), sn = T(sina);
+                sha = shelter_min(sha, T(oh.hfa), T(msh.hfa), dn, -c, -sn, T(msh.wx), T(msh.wy));
+                shw = shelter_min(shw, T(oh.hfw), T(msh.hfw), dn, -c, -sn, uwi, vwi);
+            }
+        };
+        auto inner_terms = [&](int64_t e) {
+            const int q = ent_pos(e);
+            const GeoRec o = S.geo[q];
+            const double dx = me.x - o.x, dy = me.y - o.y;
+            const double d = sqrt(dx * dx + dy * dy);
+            // pair (j, f): f is the inner floe j of the event
+            const double deltan = o.r + me.r - d;
+            if (deltan > 0) qc = true;
+            if (P.shelter) {
+                // cosa/sina point from j to f; floe f takes hsf*(j,f) with j's wind
+                const double cosa = (me.x - o.x) / d, sina = (me.y - o.y) / d;
+                const ShelRec oh = S.shel[q];
+                const T dn = T(deltan), c = T(cosa), sn = T(sina);
+                sha = shelter_min(sha, T(oh.hfa), T(msh.hfa), dn, c, sn, T(oh.wx), T(oh.wy));
+                shw = shelter_min(shw, T(oh.hfw), T(msh.hfw), dn, c, sn, uwi, vwi);
+            }
+        };
+#pragma unroll
+        for (int s2 = 0; s2 < LIST_REGS; ++s2)
+            if (s2 < no) outer_terms(re[s2], rd[s2]);
+        for (int s2 = LIST_REGS; s2 < no; ++s2) outer_terms(oj[s2 * SS + k], S.od[s2 * SS + k]);
+#pragma unroll
+        for (int s2 = 0; s2 < LIST_REGS; ++s2)
+            if (s2 < ni) inner_terms(ri[s2]);
+        for (int s2 = LIST_REGS; s2 < ni; ++s2) inner_terms(ik[s2 * SS + k]);
+    }
+    if (!act) return;
     // the register part of the lists goes to its slots
 #pragma unroll
     for (int s2 = 0; s2 < LIST_REGS; ++s2) {
@@ -746,7 +849,7 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_sup(DS S) {
             if (P.dense) {
                 qd = !FILL;
             } else if (FILL) {
-                sup_thread<1, true>(S, k, me, P, S.sup + S.soff[k], S.scnt[k], S.sno + k);
+                sup_thread<1, true>(S, k, me, P, S.sup + S.soff[k]);
                 S.x0[S.cval[k]] = make_double2(me.x, me.y);
             } else {
                 S.scnt[k] = sup_thread<1, false>(S, k, me, P, nullptr);
@@ -825,7 +928,10 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_cand(DS S) {
         }
     }
     // floes of dense members: k_cand_dense
-    if (live && !S.params[me.member].dense) cand_thread<MIXED>(S, k, me, qc, qb, qw);
+    extern __shared__ unsigned char cand_lds[];
+    const bool act = live && !S.params[me.member].dense;
+    cand_thread<MIXED, true>(S, k, me, act, qc, qb, qw,
+                             cand_lds + (threadIdx.x >> 6) * CAND_LDS_PER_WAVE);
     enqueue(S, qc, 0, k);
     enqueue(S, qb, 1, k);
     enqueue(S, qw, 2, k);
@@ -1119,7 +1225,8 @@ __global__ __launch_bounds__(64) void k_cand_dense(DS S) {
                 // rare: more candidates than LDS holds, the thread search does it
                 if (lane == 0) {
                     bool c2, b2, w2;
-                    cand_thread<MIXED>(S, k, me, c2, b2, w2);
+                    c2 = b2 = w2 = false;
+                    cand_thread<MIXED, false>(S, k, me, true, c2, b2, w2, nullptr);
                     if (c2) enqueue_one(S, 0, k);
                     if (b2) enqueue_one(S, 1, k);
                     if (w2) enqueue_one(S, 2, k);
