@@ -1424,106 +1424,185 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_contact(DS S) {
     if (over) set_err(S, over);
 }
 
+// One bond event of owner f (position k, storage slot fs, floe F0): pair
+// (f, j) at outer slot s, written to record rq (new records: rq < 0, appended
+// at nb if the event leaves one).  Returns whether the pair is bonded after
+// it (the floes then have bond terms to gather).
+template <bool MIXED>
+__device__ __forceinline__ bool bond_event(const DS& S, const DevParams& P, int tstep, int first,
+                                           int f, int k, int fs, const Floe& F0, int s, int rq,
+                                           int& nb, PairState& ps, uint32_t& over) {
+    const int64_t SS = S.S;
+    const int64_t e = S.oj[s * SS + k];
+    const int j = ent_id(e), qj = ent_pos(e);
+    Floe Fi = F0;
+    Floe Fj = floe_at(S, qj);
+    Kin kn;
+    kinematics(Fi, Fj, kn);
+    bond_create(P, tstep, f - first, j - first, Fi, Fj, kn, ps);
+    if (!ps.has_bond_rec) return false;
+    EventOut o;
+    o.flags = 0;
+    o.ridged = -1;
+    o.sbd = 0.0;
+    bond_part<MIXED>(P, Fi, Fj, kn, ps, o);
+    if (S.record_forces) S.ssb[s * SS + k] = o.sbd;
+    if (rq < 0) {
+        if (nb < S.BC) rq = nb++;
+        else { over |= ERR_BOND_CAP; return false; }
+    }
+    S.bond_new[(int64_t)rq * SS + fs] = bond_from(ps, j);
+    if (!(o.flags & EV_BONDED)) return false;
+    BondTerms bt;
+    bt.fbn_c = o.fbn_c; bt.fbt_s = o.fbt_s; bt.fbn_s = o.fbn_s; bt.fbt_c = o.fbt_c;
+    bt.lri_fbt = o.lri_fbt; bt.lrj_fbt = o.lrj_fbt; bt.mbb_ji = o.mbb_ji; bt.mbb_ij = o.mbb_ij;
+    bt.sb = o.sb; bt.fbn_sb = o.fbn_sb;
+    S.bterm[(int64_t)rq * SS + fs] = bt;
+    S.evf[s * SS + k] |= EV_BONDED | ((uint32_t)(rq + 1) << 16);
+    // tell the partner where its terms are: its inner slot of f
+    const int qi = find_entry(S.ik, SS, qj, S.icnt[qj], f);
+    if (qi >= 0) S.ibond[(int64_t)qi * SS + qj] = rq + 1;
+    S.bgat[qj] = 1;
+    return true;
+}
+
+// Old record b of the owner at record index rq of the new list: a record
+// whose partner is a candidate this step takes part in an event (its new
+// values replace it), the others carry over unchanged.
+template <bool MIXED>
+__device__ __forceinline__ bool bond_record(const DS& S, const DevParams& P, int tstep, int first,
+                                            int f, int k, int fs, const Floe& F0, int no,
+                                            const BondRec& b, int rq, int& nb, uint32_t& over) {
+    const int s = find_entry(S.oj, S.S, k, no, b.j);
+    if (s < 0) {
+        S.bond_new[(int64_t)rq * S.S + fs] = b;
+        return false;
+    }
+    PairState ps;
+    ps.deltat = 0.0; ps.thetarelc = 0.0; ps.fcn = 0.0; ps.fct = 0.0;
+    ps.has_bond_rec = true; ps.bond = b.bond; ps.damage = b.damage;
+    ps.deltanb = b.deltanb; ps.deltatb = b.deltatb; ps.trb_ji = b.trb_ji; ps.trb_ij = b.trb_ij;
+    ps.rb = b.rb; ps.hb = b.hb; ps.lb = b.lb; ps.sb = b.sb; ps.ib = b.ib;
+    return bond_event<MIXED>(S, P, tstep, first, f, k, fs, F0, s, rq, nb, ps, over);
+}
+
+// (owner lane, record, new record index) triples of one wavefront, evaluated
+// one per lane
+constexpr int BOND_TCAP = 256;
+
 // Bond events of the queued owners (bond_forces_euler, bond_breaking;
 // stepper.f90:148-198): each bond is evaluated once, by its owner (the lower
 // index), which writes the new record and the event's terms for both floes;
-// k_bond_gather then sums every floe's terms in the reference's order.
+// k_bond_gather then sums every floe's terms in the reference's order.  An
+// owner's records keep their order, cleared ones dropped: record q goes to
+// the new index rq = the number of live records before it.  The wavefront's
+// records are listed in LDS and evaluated one per lane, so that an owner with
+// several bonds does not hold its lane through all of them; owners whose
+// bonds form this step (step 1) append new records in candidate order and
+// walk their own lists.
 template <bool MIXED>
 __global__ __launch_bounds__(FLOE_BLOCK) void k_bond(DS S) {
+    __shared__ uint32_t recs[FLOE_BLOCK / 64][BOND_TCAP];
     if (*S.err) return;
     const int k = queue_entry(S, 1);
-    if (k < 0) return;
-    const GeoRec me = S.geo[k];
-    const int m = me.member;
-    const DevParams& P = S.params[m];
-    const int tstep = S.members[m].tstep;
-    const int first = S.members[m].first;
-    const bool bond_step = tstep == 1 && P.restart != 1 && P.cohesion;
-    const int f = me.id;            // identity
-    const int fs = S.cval[k];       // storage slot of the owner's records
+    const bool act = k >= 0;
+    const int lane = (int)(threadIdx.x & 63);
+    uint32_t* rl = recs[threadIdx.x >> 6];
     const int64_t SS = S.S;
-    const int64_t* oj = S.oj;
-    const Floe F0 = floe_at(S, k);
+    int m = 0, fs = 0, no = 0, bco = 0;
+    bool bond_step = false;
+    if (act) {
+        m = S.geo[k].member;
+        const DevParams& P = S.params[m];
+        bond_step = S.members[m].tstep == 1 && P.restart != 1 && P.cohesion;
+        fs = S.cval[k];
+        no = S.ocnt[k];
+        bco = S.bcnt_old[fs];
+    }
+    // the live records of the owners that form no bonds this step
+    int nlive = 0;
     uint32_t over = 0;
-
-    const int no = S.ocnt[k];
-    bool any = false;
-    int nb = 0;                     // records written so far
-    // one bond event: pair (f, j) at outer slot s, record index rq (new
-    // records: -1, appended if the event leaves one)
-    auto event = [&](int s, int rq, PairState& ps) {
-        const int64_t e = oj[s * SS + k];
-        const int j = ent_id(e), qj = ent_pos(e);
-        Floe Fi = F0;
-        Floe Fj = floe_at(S, qj);
-        Kin kn;
-        kinematics(Fi, Fj, kn);
-        bond_create(P, tstep, f - first, j - first, Fi, Fj, kn, ps);
-        if (!ps.has_bond_rec) return;
-        EventOut o;
-        o.flags = 0;
-        o.ridged = -1;
-        o.sbd = 0.0;
-        bond_part<MIXED>(P, Fi, Fj, kn, ps, o);
-        if (S.record_forces) S.ssb[s * SS + k] = o.sbd;
-        if (rq < 0) {
-            if (nb < S.BC) rq = nb++;
-            else { over |= ERR_BOND_CAP; return; }
-        }
-        S.bond_new[(int64_t)rq * SS + fs] = bond_from(ps, j);
-        if (o.flags & EV_BONDED) {
-            BondTerms bt;
-            bt.fbn_c = o.fbn_c; bt.fbt_s = o.fbt_s; bt.fbn_s = o.fbn_s; bt.fbt_c = o.fbt_c;
-            bt.lri_fbt = o.lri_fbt; bt.lrj_fbt = o.lrj_fbt; bt.mbb_ji = o.mbb_ji; bt.mbb_ij = o.mbb_ij;
-            bt.sb = o.sb; bt.fbn_sb = o.fbn_sb;
-            S.bterm[(int64_t)rq * SS + fs] = bt;
-            S.evf[s * SS + k] |= EV_BONDED | ((uint32_t)(rq + 1) << 16);
-            // tell the partner where its terms are: its inner slot of f
-            const int qi = find_entry(S.ik, SS, qj, S.icnt[qj], f);
-            if (qi >= 0) S.ibond[(int64_t)qi * SS + qj] = rq + 1;
-            S.bgat[qj] = 1;
-            any = true;
-        }
-    };
-    // the owner's records in order, cleared entries dropped: a record whose
-    // partner is a candidate this step takes part in an event (its new
-    // values replace it), the others carry over unchanged
-    const int bco = S.bcnt_old[fs];
-    for (int q = 0; q < bco; ++q) {
-        const BondRec b = S.bond_old[q * SS + fs];
-        if (b.j < 0) continue;
-        const int rq = nb++;
-        const int s = find_entry(oj, SS, k, no, b.j);
-        if (s < 0) {
-            S.bond_new[(int64_t)rq * SS + fs] = b;
-            continue;
-        }
-        PairState ps;
-        ps.deltat = 0.0; ps.thetarelc = 0.0; ps.fcn = 0.0; ps.fct = 0.0;
-        ps.has_bond_rec = true; ps.bond = b.bond; ps.damage = b.damage;
-        ps.deltanb = b.deltanb; ps.deltatb = b.deltatb; ps.trb_ji = b.trb_ji; ps.trb_ij = b.trb_ij;
-        ps.rb = b.rb; ps.hb = b.hb; ps.lb = b.lb; ps.sb = b.sb; ps.ib = b.ib;
-        event(s, rq, ps);
+    const bool listed = act && !bond_step;
+    if (listed) {
+        int id[LIST_PEEK];
+#pragma unroll
+        for (int q = 0; q < LIST_PEEK; ++q) id[q] = q < bco ? S.bond_old[(int64_t)q * SS + fs].j : -1;
+#pragma unroll
+        for (int q = 0; q < LIST_PEEK; ++q) nlive += id[q] >= 0;
+        for (int q = LIST_PEEK; q < bco; ++q) nlive += S.bond_old[(int64_t)q * SS + fs].j >= 0;
     }
-    // bonds that form this step (bond_create), in candidate order, appended
-    if (bond_step) {
-        const int nb_old = nb;
-        for (int s = 0; s < no; ++s) {
-            const int j = ent_id(oj[s * SS + k]);
-            bool has = false;
-            for (int q = 0; q < nb_old; ++q)
-                if (S.bond_new[(int64_t)q * SS + fs].j == j) { has = true; break; }
-            if (has) continue;
-            PairState ps;
-            ps.deltat = 0.0; ps.thetarelc = 0.0; ps.fcn = 0.0; ps.fct = 0.0;
-            ps.bond = 0; ps.has_bond_rec = false;
-            ps.damage = ps.deltanb = ps.deltatb = ps.trb_ji = ps.trb_ij = 0.0;
-            ps.rb = ps.hb = ps.lb = ps.sb = ps.ib = 0.0;
-            event(s, -1, ps);
+    int incl = nlive;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int v = __shfl_up(incl, off);
+        if (lane >= off) incl += v;
+    }
+    const int total = __shfl(incl, 63);
+    const bool fits = total <= BOND_TCAP;
+    if (fits) {
+        if (listed) {
+            int at = incl - nlive, rq = 0;
+            for (int q = 0; q < bco; ++q)
+                if (S.bond_old[(int64_t)q * SS + fs].j >= 0) {
+                    rl[at++] = (uint32_t)lane | ((uint32_t)q << 6) | ((uint32_t)rq << 19);
+                    ++rq;
+                }
+            S.bcnt_new[fs] = nlive;
+        }
+        wave_lds_sync();
+        for (int c0 = 0; c0 < total; c0 += 64) {
+            const int idx = c0 + lane;
+            const uint32_t w = idx < total ? rl[idx] : (uint32_t)lane;
+            const int own = (int)(w & 63);
+            const int ko = __shfl(k, own), mo = __shfl(m, own), fso = __shfl(fs, own);
+            const int noo = __shfl(no, own);
+            if (idx < total) {
+                const int q = (int)((w >> 6) & 0x1FFF), rq = (int)(w >> 19);
+                const DevParams& P = S.params[mo];
+                const Floe F0 = floe_at(S, ko);
+                const BondRec b = S.bond_old[(int64_t)q * SS + fso];
+                int nb = 0;
+                if (bond_record<MIXED>(S, P, S.members[mo].tstep, S.members[mo].first, S.geo[ko].id,
+                                       ko, fso, F0, noo, b, rq, nb, over))
+                    S.bgat[ko] = 1;
+            }
         }
     }
-    if (any) S.bgat[k] = 1;
-    S.bcnt_new[fs] = nb;
+    if (act && (bond_step || !fits)) {
+        const GeoRec me = S.geo[k];
+        const DevParams& P = S.params[m];
+        const int tstep = S.members[m].tstep;
+        const int first = S.members[m].first;
+        const int f = me.id;
+        const Floe F0 = floe_at(S, k);
+        bool any = false;
+        int nb = 0;                     // records written so far
+        for (int q = 0; q < bco; ++q) {
+            const BondRec b = S.bond_old[q * SS + fs];
+            if (b.j < 0) continue;
+            const int rq = nb++;
+            any |= bond_record<MIXED>(S, P, tstep, first, f, k, fs, F0, no, b, rq, nb, over);
+        }
+        // bonds that form this step (bond_create), in candidate order, appended
+        if (bond_step) {
+            const int nb_old = nb;
+            for (int s = 0; s < no; ++s) {
+                const int j = ent_id(S.oj[s * SS + k]);
+                bool has = false;
+                for (int q = 0; q < nb_old; ++q)
+                    if (S.bond_new[(int64_t)q * SS + fs].j == j) { has = true; break; }
+                if (has) continue;
+                PairState ps;
+                ps.deltat = 0.0; ps.thetarelc = 0.0; ps.fcn = 0.0; ps.fct = 0.0;
+                ps.bond = 0; ps.has_bond_rec = false;
+                ps.damage = ps.deltanb = ps.deltatb = ps.trb_ji = ps.trb_ij = 0.0;
+                ps.rb = ps.hb = ps.lb = ps.sb = ps.ib = 0.0;
+                any |= bond_event<MIXED>(S, P, tstep, first, f, k, fs, F0, s, -1, nb, ps, over);
+            }
+        }
+        if (any) S.bgat[k] = 1;
+        S.bcnt_new[fs] = nb;
+    }
     if (over) set_err(S, over);
 }
 
