@@ -34,7 +34,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 PROFILE = os.path.join(ROOT, "profiles", "step_kernels_batched_512fields.md")
 STAGE_KERNEL = {"candidates": "k_cand", "contacts": "k_contact", "bonds": "k_bond",
-                "bond_gather": "k_bond_gather", "walls": "k_walls", "repair": "k_repair",
+                "walls": "k_walls", "repair": "k_repair",
                 "fixup": "k_fixup", "integrate": "k_final"}
 
 
@@ -90,7 +90,7 @@ def run_batched(args):
     else:
         blocks = ctx.autotune(2)      # real steps: part of the warm-up
     ctx.sync()
-    launch = "graph replay" if ctx.graphs_on() else "one-by-one launches, bond gather beside the repair"
+    launch = "graph replay" if ctx.graphs_on() else "one-by-one launches"
     # timed region: steps in the launch mode the autotuner chose (graph
     # replay or one-by-one launches), nothing else on the GPU
     t0 = time.perf_counter()

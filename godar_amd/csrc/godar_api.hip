@@ -26,7 +26,6 @@ __global__ void k_scan_offsets(DS);
 template <bool M> __global__ void k_cand_dense(DS);
 template <bool M> __global__ void k_contact(DS);
 template <bool M> __global__ void k_bond(DS);
-__global__ void k_bond_gather(DS);
 __global__ void k_stress(DS);
 template <bool M> __global__ void k_walls(DS);
 template <bool M> __global__ void k_repair(DS, int);
@@ -50,10 +49,6 @@ struct Range { int64_t first, n; };
 
 struct godar_ctx {
     hipStream_t stream = nullptr;
-    // the bond gather runs beside the ordered repair on a second stream
-    // (fork and join events; default priority like the main stream)
-    hipStream_t side = nullptr;
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     int precision = GODAR_FP64;
     int64_t S = 0;
     int Mcap = 0;
@@ -98,7 +93,6 @@ struct godar_ctx {
     // device-state pointers by value), each with the launch signature it was
     // captured for
     bool use_graphs = true;
-    bool capturing = false;             // launch_step is being captured into a graph
     int dense_scan = 96;                // floes per 3x3 search above which a member is dense
     int repair_lanes = 64;              // repair events evaluated at once
     int repair_cap = 1024;              // pending repair keys held in LDS
@@ -416,29 +410,9 @@ int launch_step(godar_ctx* c) {
         hipLaunchKernelGGL(mixed ? k_walls<true> : k_walls<false>, grid_for(N, B[7], true),
                            dim3(B[7]), 0, s, d);
     }
-    if (!c->prof && !c->capturing) {
-        // The ordered repair is one wavefront per member and leaves most of
-        // the GPU idle; the bond gather runs beside it.  They share no data
-        // but the event flags of repaired pairs, whose owners the fixup
-        // recomputes whole (k_bond_gather's reading of them is bounded).
-        // A captured graph keeps the two in sequence: replayed with a fork
-        // it ran slower than either.
-        HIPCHK(c, hipEventRecord(c->ev_fork, s));
-        HIPCHK(c, hipStreamWaitEvent(c->side, c->ev_fork, 0));
-        hipLaunchKernelGGL(k_bond_gather, grid_for(N, B[6], true), dim3(B[6]), 0, c->side, d);
-        HIPCHK(c, hipEventRecord(c->ev_join, c->side));
+    {
+        StageTimer t(c, 8);
         hipLaunchKernelGGL(mixed ? k_repair<true> : k_repair<false>, dim3(nm), dim3(64), 0, s, d, nm);
-        HIPCHK(c, hipStreamWaitEvent(s, c->ev_join, 0));
-    } else {
-        // timed stage by stage: one after the other (same results)
-        {
-            StageTimer t(c, 6);
-            hipLaunchKernelGGL(k_bond_gather, grid_for(N, B[6], true), dim3(B[6]), 0, s, d);
-        }
-        {
-            StageTimer t(c, 8);
-            hipLaunchKernelGGL(mixed ? k_repair<true> : k_repair<false>, dim3(nm), dim3(64), 0, s, d, nm);
-        }
     }
     {
         StageTimer t(c, 9);
@@ -493,9 +467,7 @@ int graph_step(godar_ctx* c) {
         c->gexec[par] = nullptr;
         c->gsig[par].clear();
         HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        c->capturing = true;
         int rc = launch_step(c);
-        c->capturing = false;
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(c->stream, &g);
         c->parity = par;   // the captured step has not run yet
@@ -581,9 +553,6 @@ int godar_create(int64_t max_floes, int32_t max_members, int32_t precision, goda
         }                                              \
     } while (0)
     if (e != hipSuccess) { delete c; return GODAR_ERR_HIP; }
-    ACHK(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
-    ACHK(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
-    ACHK(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
     ACHK(dalloc(&d.rec, S));
     ACHK(dalloc(&d.geo, S));
     ACHK(dalloc(&d.shel, S));
@@ -684,9 +653,6 @@ int godar_destroy(godar_ctx* c) {
     radix_free(c->rs);
     collect_events(c);
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
-    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
-    if (c->ev_join) (void)hipEventDestroy(c->ev_join);
-    if (c->side) (void)hipStreamDestroy(c->side);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return GODAR_OK;
@@ -1228,7 +1194,8 @@ int godar_reserve(godar_ctx* c, int32_t outer, int32_t inner, int32_t histc, int
 }
 
 // stages whose per-floe kernels take a tunable block size
-static bool tunable_stage(int st) { return st == 0 || (st >= 2 && st <= 7) || st >= 9; }
+// (stage 6 has no kernel: the bond gather is part of the integration)
+static bool tunable_stage(int st) { return st == 0 || (st >= 2 && st <= 5) || st == 7 || st >= 9; }
 
 int godar_set_block(godar_ctx* c, int32_t threads) {
     if (!c || threads < 64 || threads > FLOE_BLOCK || (threads & 63)) return GODAR_ERR_ARG;
@@ -1299,8 +1266,7 @@ int godar_autotune(godar_ctx* c, int32_t trial) {
     }
     int r2 = godar_profile(c, prof0 ? 1 : 0);
     if (r2 != GODAR_OK || prof0 || !c->use_graphs) return r2;
-    // launch mode: graph replay or one-by-one launches (these run the bond
-    // gather beside the ordered repair), whichever steps faster
+    // launch mode: graph replay or one-by-one launches, whichever steps faster
     const int n = std::max(4, 4 * trial);
     double t[2] = {0.0, 0.0};
     for (int mode = 0; mode < 2; ++mode) {
@@ -1632,7 +1598,7 @@ int godar_get_totals(godar_ctx* c, int64_t* out) {
 }
 
 static const char* kStageNames[GODAR_NSTAGES] = {
-    "bin", "sort", "cells", "candidates", "contacts", "bonds", "bond_gather", "walls", "repair",
+    "bin", "sort", "cells", "candidates", "contacts", "bonds", "unused", "walls", "repair",
     "fixup", "stress", "integrate"};
 
 int godar_profile(godar_ctx* c, int32_t on) {

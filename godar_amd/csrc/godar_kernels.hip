@@ -2,7 +2,7 @@
 //
 //   [build steps: k_spill -> k_bin -> radix sort -> k_clear_cells -> k_ranges
 //    -> k_sup ...] -> k_cand -> k_contact -> k_bond -> k_walls
-//    -> k_repair (beside it: k_bond_gather) -> k_fixup -> k_final
+//    -> k_repair -> k_fixup -> k_final (which also gathers the bond terms)
 //
 // Sequential semantics of the reference (stepper.f90:71-280 visits floes i
 // in index order and, for each, its candidates j > i in ascending distance;
@@ -18,7 +18,7 @@
 //     first the pairs where it is the inner index (ascending outer index),
 //     then its own candidates in distance order.  Contact and sheltering
 //     terms are evaluated by both floes of a pair; a bond is evaluated once,
-//     by its owner, which stores the terms for k_bond_gather.  No atomics,
+//     by its owner, which stores the terms; k_final gathers them.  No atomics,
 //     and FP64 sums stay identical to the reference.
 //   * An event's speculative result is exact unless one of its floes was
 //     reshaped earlier in the sequence.  Events that ridge are seeds; the
@@ -454,7 +454,7 @@ __device__ __forceinline__ void cand_finish(const DS& S, int k, int fs, const De
         if (!qb) S.bcnt_new[fs] = 0;
         S.wflag[k] = 0;
         S.bgat[k] = 0;
-        S.cls[k] = qc ? CLS_CONTACT : 0;   // k_bond_gather adds CLS_BOND
+        S.cls[k] = qc ? CLS_CONTACT : 0;   // the fix-up sets CLS_BOND
     }
 }
 
@@ -1493,7 +1493,7 @@ constexpr int BOND_TCAP = 256;
 // Bond events of the queued owners (bond_forces_euler, bond_breaking;
 // stepper.f90:148-198): each bond is evaluated once, by its owner (the lower
 // index), which writes the new record and the event's terms for both floes;
-// k_bond_gather then sums every floe's terms in the reference's order.  An
+// k_final then sums every floe's terms in the reference's order.  An
 // owner's records keep their order, cleared ones dropped: record q goes to
 // the new index rq = the number of live records before it.  The wavefront's
 // records are listed in LDS and evaluated one per lane, so that an owner with
@@ -1606,16 +1606,13 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_bond(DS S) {
     if (over) set_err(S, over);
 }
 
-// Bond rows of every floe with a bonded event: the terms its owners wrote,
-// first the pairs where it is the inner index (ascending outer index), then
-// its own, in distance order (stepper.f90:165-198).
-__global__ __launch_bounds__(FLOE_BLOCK) void k_bond_gather(DS S) {
-    int k = xcd_block() * blockDim.x + threadIdx.x;
-    if (k >= S.N || *S.err || !S.bgat[k]) return;
-    const int fs = S.cval[k];
+// Bond rows of a floe with a bonded event (position k, storage slot fs),
+// added to a[] (rows at zero): the terms its owners wrote, first the pairs
+// where it is the inner index (ascending outer index), then its own, in
+// distance order (stepper.f90:165-198).  k_final calls it for the floes the
+// repair left alone: their flags and terms are k_bond's.
+__device__ __forceinline__ void gather_bonds(const DS& S, int k, int fs, double* a) {
     const int64_t SS = S.S;
-    double a[NACC];
-    acc_init(a);
     const int ni = S.icnt[k];
     for (int q = 0; q < ni; ++q) {
         const int rq = S.ibond[q * SS + k];
@@ -1623,19 +1620,10 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_bond_gather(DS S) {
     }
     const int no = S.ocnt[k];
     for (int s = 0; s < no; ++s) {
-        // (the ordered repair may rewrite the flags of a floe's pairs
-        // meanwhile, without a record index; the fixup then recomputes the
-        // floe whole, and no index is formed from them here)
         const uint32_t fl = S.evf[s * SS + k];
         const int rq = (int)(fl >> 16);
         if ((fl & EV_BONDED) && rq >= 1 && rq <= S.BC) add_bond_i(a, S.bterm[(int64_t)(rq - 1) * SS + fs]);
     }
-    S.acc[A_FBX * SS + k] = a[A_FBX];
-    S.acc[A_FBY * SS + k] = a[A_FBY];
-    S.acc[A_MB * SS + k] = a[A_MB];
-    S.acc[A_TAB * SS + k] = a[A_TAB];
-    S.acc[A_PB * SS + k] = a[A_PB];
-    S.cls[k] |= CLS_BOND;
 }
 
 // Walls of the queued floes (verify_bc, boundaries.f90:1-97).
@@ -2413,6 +2401,8 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_final(DS S) {
                          : shel_row || (bond_row ? (cl & CLS_BOND) : (cl & CLS_CONTACT));
         a[c] = valid ? S.acc[c * SS + k] : 0.0;
     }
+    // bond terms: rows the fix-up wrote (CLS_BOND), else gathered here
+    if (!(cl & CLS_BOND) && S.bgat[k]) gather_bonds(S, k, f, a);
     // shelter minima (reset_shelter + thread reduction); with shelter off
     // the reference's minima stay at their zero initial value
     double sha = P.shelter ? a[A_SHA] : 0.0;
