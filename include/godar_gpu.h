@@ -213,9 +213,8 @@ int godar_autotune(godar_ctx *ctx, int32_t trial);
 /* Step launch mode (no counterpart in the reference): on (the default), each
  * step replays a captured HIP graph of the whole kernel sequence, recaptured
  * whenever members, capacities or the block size change; off, the kernels are
- * launched one by one, and the bond gather runs on a second stream beside the
- * ordered repair.  Results are identical.  Steps timed by godar_profile are
- * always launched one by one on one stream. */
+ * launched one by one.  Results are identical.  Steps timed by godar_profile
+ * are always launched one by one. */
 int godar_use_graphs(godar_ctx *ctx, int32_t on);
 int godar_graphs_on(const godar_ctx *ctx, int32_t *on);   /* the current launch mode */
 
@@ -224,7 +223,8 @@ int godar_graphs_on(const godar_ctx *ctx, int32_t *on);   /* the current launch 
  * the whole loop with omp_get_wtime, godar.f90:185-219).  When on, every
  * step records HIP events around its stages on the context's stream;
  * godar_get_profile returns the summed milliseconds and the number of timed
- * launches per stage.  godar_profile resets the totals.
+ * launches per stage.  godar_profile resets the totals.  Stage 6 ("unused")
+ * launches nothing: the bond gather is part of the integration.
  */
 #define GODAR_NSTAGES 12
 int godar_profile(godar_ctx *ctx, int32_t on);
