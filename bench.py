@@ -298,12 +298,15 @@ def main():
     #    per contact two list entries read, history (16 B) read and written
     #  bonds: same per queued floe; per bond two list entries read, the
     #    88-byte record read and written
+    #  integrate: the floe state (12 words) read, the 6 evolving words
+    #    written, 2 sheltering minima read; 5 force accumulators per contact
+    #    floe; per bond its 10 terms read by each floe (the bond gather)
     per_stage_bytes = {
         "shelter": nf * (7 * 8 + 2 * 8 + 2 * 4 + 1) + nc * (2 * 8 + 4 + 4),
         "contacts": qc * (11 * 8 + 5 * 8 + 4) + nk * (2 * 8 + 32),
         "bonds": qb * (11 * 8 + 5 * 8 + 4) + nb * (2 * 8 + 2 * 88),
         "candidates": nf * (3 * 8 + 8) + 12.0 * 2 * nc,
-        "integrate": nf * (12 * 8 + 12 * 8 + 6 * 8 + 4 * 8),
+        "integrate": nf * (12 * 8 + 6 * 8 + 2 * 8) + qc * 5 * 8 + nb * 2 * 10 * 8,
         "sort": nf * 8.0 * 2 * 3,
     }
     alg = per_stage_bytes.get(dom, B["alg_bytes_step"])

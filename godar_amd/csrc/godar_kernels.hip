@@ -749,8 +749,9 @@ __device__ __forceinline__ void cand_thread(const DS& S, int k, const GeoRec& me
                     // from j to f and f takes hsf*(j,f) with j's wind
                     const ShelRec oh = S.shel[q];
                     const ShelRec mh = S.shel[ko];
-                    const double cosa = outer ? (o.x - mo.x) / dd : (mo.x - o.x) / dd;
-                    const double sina = outer ? (o.y - mo.y) / dd : (mo.y - o.y) / dd;
+                    const double nx = outer ? o.x - mo.x : mo.x - o.x;
+                    const double ny = outer ? o.y - mo.y : mo.y - o.y;
+                    const double cosa = nx / dd, sina = ny / dd;
                     const T dn = T(deltan), c0t = T(cosa), s0t = T(sina);
                     const T c = outer ? -c0t : c0t, sn = outer ? -s0t : s0t;
                     const T ux = outer ? T(mh.wx) : T(oh.wx), uy = outer ? T(mh.wy) : T(oh.wy);
@@ -1088,18 +1089,25 @@ __global__ __launch_bounds__(64) void k_cand_dense(DS S) {
     const int t1 = min(nq, xcd * per + per);
     const uint64_t lt = lane ? ((~0ull) >> (64 - lane)) : 0ull;
     const int64_t SS = S.S;
-    // the next floe's list entry and row are loaded a floe ahead
+    // the next floe's list entry, row and superset extent are loaded a floe
+    // ahead
     int t = xcd * per + bi;
     int k_next = t < t1 ? queue_at(S, 3, qv, t) : 0;
     GeoRec me_next;
-    if (t < t1) me_next = S.geo[k_next];
+    int L_next = 0;
+    int64_t off_next = 0;
+    if (t < t1) { me_next = S.geo[k_next]; L_next = S.scnt[k_next]; off_next = S.soff[k_next]; }
     for (; t < t1; t += nb) {
         const int k = k_next;
         const GeoRec me = me_next;
+        const int L = L_next;
+        const int64_t* sup = S.sup + off_next;
         if (t + nb < t1) k_next = queue_at(S, 3, qv, t + nb);
         const int m = me.member;
         if (m < 0) {                // member past its stop step (k_cand cleared its counts)
-            if (t + nb < t1) me_next = S.geo[k_next];
+            if (t + nb < t1) {
+                me_next = S.geo[k_next]; L_next = S.scnt[k_next]; off_next = S.soff[k_next];
+            }
             continue;
         }
         const DevParams& P = S.params[m];
@@ -1115,37 +1123,30 @@ __global__ __launch_bounds__(64) void k_cand_dense(DS S) {
         // would otherwise hold these loads back)
         const int fs = S.cval[k];
         const bool has_hist = S.hcnt_old[fs] > 0, has_bonds = S.bcnt_old[fs] > 0;
-        // sheltering and classification terms of one outer / inner candidate
-        // (lane-parallel; a minimum and an "any" do not depend on the order)
-        auto outer_math = [&](const GeoRec& o, int q, double d) {
-            // pair (f, j): f is the outer floe i of the event
-            const double deltan = me.r + o.r - d;
+        // sheltering and classification terms of one candidate (lane-
+        // parallel; a minimum and an "any" do not depend on the order), one
+        // code path for both sides: outer, pair (f, j) with f the outer floe
+        // i of the event, f takes hsf*(j,f) with -cosa, -sina of f -> j and
+        // its own wind; inner, pair (j, f) with f the inner floe, cosa/sina
+        // from j to f and j's wind
+        auto cand_math = [&](const GeoRec& o, int q, double d, bool outer) {
+            const double deltan = outer ? me.r + o.r - d : o.r + me.r - d;
             if (deltan > 0) lqc = true;
-            if (bond_step && deltan >= -P.bond_lim &&
+            if (outer && bond_step && deltan >= -P.bond_lim &&
                 bond_keep(f - first, o.id - first, P.bond_seed, P.bond_thr24))
                 lqb = true;
             if (P.shelter) {
-                const double cosa = (o.x - me.x) / d, sina = (o.y - me.y) / d;
+                const double nx = outer ? o.x - me.x : me.x - o.x;
+                const double ny = outer ? o.y - me.y : me.y - o.y;
+                const double cosa = nx / d, sina = ny / d;
                 const ShelRec oh = S.shel[q];
-                const T dn = T(deltan), c = T(cosa), sn = T(sina);
-                sha = shelter_min(sha, T(oh.hfa), T(msh.hfa), dn, -c, -sn, T(msh.wx), T(msh.wy));
-                shw = shelter_min(shw, T(oh.hfw), T(msh.hfw), dn, -c, -sn, uwi, vwi);
-            }
-        };
-        auto inner_math = [&](const GeoRec& o, int q, double d) {
-            // pair (j, f): f is the inner floe j of the event
-            const double deltan = o.r + me.r - d;
-            if (deltan > 0) lqc = true;
-            if (P.shelter) {
-                const double cosa = (me.x - o.x) / d, sina = (me.y - o.y) / d;
-                const ShelRec oh = S.shel[q];
-                const T dn = T(deltan), c = T(cosa), sn = T(sina);
-                sha = shelter_min(sha, T(oh.hfa), T(msh.hfa), dn, c, sn, T(oh.wx), T(oh.wy));
+                const T dn = T(deltan), c0 = T(cosa), s0 = T(sina);
+                const T c = outer ? -c0 : c0, sn = outer ? -s0 : s0;
+                const T ux = outer ? T(msh.wx) : T(oh.wx), uy = outer ? T(msh.wy) : T(oh.wy);
+                sha = shelter_min(sha, T(oh.hfa), T(msh.hfa), dn, c, sn, ux, uy);
                 shw = shelter_min(shw, T(oh.hfw), T(msh.hfw), dn, c, sn, uwi, vwi);
             }
         };
-        const int L = S.scnt[k];
-        const int64_t* sup = S.sup + S.soff[k];
         int no = 0, ni = 0;
         uint32_t over = 0;
         if (L <= 64) {
@@ -1169,8 +1170,7 @@ __global__ __launch_bounds__(64) void k_cand_dense(DS S) {
                     if (o.id > f) vout = d <= thr;
                     else vin = d <= thr;
                 }
-                if (vout) outer_math(o, q, d);
-                if (vin) inner_math(o, q, d);
+                if (vout || vin) cand_math(o, q, d, vout);
             }
             no = (int)__popcll(__ballot(vout));
             ni = (int)__popcll(__ballot(vin));
@@ -1202,8 +1202,7 @@ __global__ __launch_bounds__(64) void k_cand_dense(DS S) {
                         if (o.id > f) vout = d <= thr;
                         else vin = d <= thr;
                     }
-                    if (vout) outer_math(o, q, d);
-                    if (vin) inner_math(o, q, d);
+                    if (vout || vin) cand_math(o, q, d, vout);
                 }
                 const uint64_t mo = __ballot(vout), mi = __ballot(vin);
                 if (vout) {
@@ -1232,7 +1231,9 @@ __global__ __launch_bounds__(64) void k_cand_dense(DS S) {
                     if (w2) enqueue_one(S, 2, k);
                 }
                 __syncthreads();
-                if (t + nb < t1) me_next = S.geo[k_next];
+                if (t + nb < t1) {
+                    me_next = S.geo[k_next]; L_next = S.scnt[k_next]; off_next = S.soff[k_next];
+                }
                 continue;
             }
             if (no > S.CO) { over |= ERR_OUTER_CAP; no = S.CO; }
@@ -1255,7 +1256,9 @@ __global__ __launch_bounds__(64) void k_cand_dense(DS S) {
         shw = wave_min(shw);
         const bool qc = __ballot(lqc) != 0 || has_hist;
         const bool qb = __ballot(lqb) != 0 || has_bonds;
-        if (t + nb < t1) me_next = S.geo[k_next];
+        if (t + nb < t1) {
+            me_next = S.geo[k_next]; L_next = S.scnt[k_next]; off_next = S.soff[k_next];
+        }
         const bool qw = me.y - me.r < 0 || me.y + me.r > P.ny || me.x - me.r < 0 || me.x + me.r > P.nx;
         cand_finish<MIXED>(S, k, fs, P, no, ni, double(sha), double(shw), qc, qb, lane, 64);
         if (lane == 0) {
