@@ -105,7 +105,7 @@ struct DS {
     int32_t *hcnt_old, *hcnt_new;
     BondRec *bond_old, *bond_new;
     int32_t *bcnt_old, *bcnt_new;
-    BondTerms* bterm;          // [BC][S] (id): this step's terms of owner record q
+    BondTerms* bterm;          // [BC][S] (pos): this step's terms of owner record q
     uint8_t* bgat;             // [S] pos: 1 if the floe has a bonded event this step
     int32_t* ibond;            // [CI][S] pos: 1 + owner's record slot of the inner pair's bond
     // speculative gather (pos)

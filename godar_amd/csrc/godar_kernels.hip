@@ -1460,7 +1460,7 @@ __device__ __forceinline__ bool bond_event(const DS& S, const DevParams& P, int 
     bt.fbn_c = o.fbn_c; bt.fbt_s = o.fbt_s; bt.fbn_s = o.fbn_s; bt.fbt_c = o.fbt_c;
     bt.lri_fbt = o.lri_fbt; bt.lrj_fbt = o.lrj_fbt; bt.mbb_ji = o.mbb_ji; bt.mbb_ij = o.mbb_ij;
     bt.sb = o.sb; bt.fbn_sb = o.fbn_sb;
-    S.bterm[(int64_t)rq * SS + fs] = bt;
+    S.bterm[(int64_t)rq * SS + k] = bt;          // (by the owner's position)
     S.evf[s * SS + k] |= EV_BONDED | ((uint32_t)(rq + 1) << 16);
     // tell the partner where its terms are: its inner slot of f
     const int qi = find_entry(S.ik, SS, qj, S.icnt[qj], f);
@@ -1609,23 +1609,23 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_bond(DS S) {
     if (over) set_err(S, over);
 }
 
-// Bond rows of a floe with a bonded event (position k, storage slot fs),
-// added to a[] (rows at zero): the terms its owners wrote, first the pairs
+// Bond rows of a floe with a bonded event (position k), added to a[] (rows
+// at zero): the terms its owners wrote, first the pairs
 // where it is the inner index (ascending outer index), then its own, in
 // distance order (stepper.f90:165-198).  k_final calls it for the floes the
 // repair left alone: their flags and terms are k_bond's.
-__device__ __forceinline__ void gather_bonds(const DS& S, int k, int fs, double* a) {
+__device__ __forceinline__ void gather_bonds(const DS& S, int k, double* a) {
     const int64_t SS = S.S;
     const int ni = S.icnt[k];
     for (int q = 0; q < ni; ++q) {
         const int rq = S.ibond[q * SS + k];
-        if (rq) add_bond_j(a, S.bterm[(int64_t)(rq - 1) * SS + S.cval[ent_pos(S.ik[q * SS + k])]]);
+        if (rq) add_bond_j(a, S.bterm[(int64_t)(rq - 1) * SS + ent_pos(S.ik[q * SS + k])]);
     }
     const int no = S.ocnt[k];
     for (int s = 0; s < no; ++s) {
         const uint32_t fl = S.evf[s * SS + k];
         const int rq = (int)(fl >> 16);
-        if ((fl & EV_BONDED) && rq >= 1 && rq <= S.BC) add_bond_i(a, S.bterm[(int64_t)(rq - 1) * SS + fs]);
+        if ((fl & EV_BONDED) && rq >= 1 && rq <= S.BC) add_bond_i(a, S.bterm[(int64_t)(rq - 1) * SS + k]);
     }
 }
 
@@ -2405,7 +2405,7 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_final(DS S) {
         a[c] = valid ? S.acc[c * SS + k] : 0.0;
     }
     // bond terms: rows the fix-up wrote (CLS_BOND), else gathered here
-    if (!(cl & CLS_BOND) && S.bgat[k]) gather_bonds(S, k, f, a);
+    if (!(cl & CLS_BOND) && S.bgat[k]) gather_bonds(S, k, a);
     // shelter minima (reset_shelter + thread reduction); with shelter off
     // the reference's minima stay at their zero initial value
     double sha = P.shelter ? a[A_SHA] : 0.0;
