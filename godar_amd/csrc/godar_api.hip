@@ -14,8 +14,7 @@
 #include <vector>
 
 namespace godar {
-__global__ void k_bin(DS);
-__global__ void k_clear_cells(DS);
+__global__ void k_begin(DS);
 __global__ void k_ranges(DS);
 template <bool M> __global__ void k_cand(DS);
 template <bool F> __global__ void k_sup(DS);
@@ -223,6 +222,8 @@ void sync_ds(godar_ctx* c) {
     d.repair_lanes = c->repair_lanes;
     d.repair_cap = c->repair_cap;
     d.sup_cap = c->sup_cap;
+    d.bkey = c->rs.keys_alt;
+    d.bval = c->rs.vals_alt;
     bind_parity(c);
 }
 
@@ -354,8 +355,8 @@ int launch_step(godar_ctx* c) {
     // once (the launch sequence, and a captured graph, is the same).
     {
         StageTimer t(c, 0);
-        hipLaunchKernelGGL(k_spill, grid_for(N, B[0], false), dim3(B[0]), 0, s, d, 0);
-        hipLaunchKernelGGL(k_bin, grid_for(N, B[0], false), dim3(B[0]), 0, s, d);
+        const int nt = std::max({N, c->ncells_cap + 1, NQUEUE * XCD_COUNT});
+        hipLaunchKernelGGL(k_begin, grid_for(nt, B[0], false), dim3(B[0]), 0, s, d);
     }
     int bits = 1;
     while ((1u << bits) <= (uint32_t)c->ncells_cap && bits < 32) ++bits;
@@ -370,8 +371,6 @@ int launch_step(godar_ctx* c) {
         if (c->members[k].active && c->dparams[k].dense) any_dense = true;
     {
         StageTimer t(c, 2);
-        hipLaunchKernelGGL(k_clear_cells, dim3((std::max(c->ncells_cap + 1, NQUEUE * XCD_COUNT) + 255) / 256),
-                           dim3(256), 0, s, d);
         hipLaunchKernelGGL(k_ranges, grid_for(N, B[2], false), dim3(B[2]), 0, s, d);
         // superset build: counts, offsets, entries (k_sup shares k_cand's
         // grid, whose XCD shares lay out the dense-search list)

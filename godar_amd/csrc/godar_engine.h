@@ -83,7 +83,8 @@ struct DS {
     const WindGridDesc* wgrids;
     const double* wpool;
     // cell list
-    uint32_t *ckey, *cval;     // sort keys / floe ids
+    uint32_t *ckey, *cval;     // sort keys / floe ids, sorted (cval[pos] = storage slot)
+    uint32_t *bkey, *bval;     // the sort's input, written by k_begin (the sort's scratch)
     int2* crange;              // [ncells+1] (start, end) in cell order
     GeoRec* geo;               // [S] pos
     ShelRec* shel;             // [S] pos (wind at the floe this step)

@@ -1,7 +1,7 @@
 // Kernels of the GODAR step pipeline (one launch sequence per step):
 //
-//   [build steps: k_spill -> k_bin -> radix sort -> k_clear_cells -> k_ranges
-//    -> k_sup ...] -> k_cand -> k_contact -> k_bond -> k_walls
+//   k_begin -> [build steps: radix sort -> k_ranges -> k_sup ...] -> k_cand
+//    -> k_contact -> k_bond -> k_walls
 //    -> k_repair -> k_fixup -> k_final (which also gathers the bond terms)
 //
 // Sequential semantics of the reference (stepper.f90:71-280 visits floes i
@@ -308,11 +308,11 @@ __device__ __forceinline__ void acc_init(double* a) {
 // ---------------------------------------------------------------------
 // 1. cell keys (one per floe id)
 // ---------------------------------------------------------------------
-// rec from the ordered copies (k_final's state) where they are newer: at the
-// start of steps that rebuild or refresh the copies from rec (then k_bin
-// clears the flag), and for the host (`host`, which clears it itself) before
-// it reads or writes floe records or changes the floe count.  Positions whose
-// copy is of no live member are skipped: their records are current.
+// rec from the ordered copies (k_final's state) where they are newer, for
+// the host (`host`, which clears the flag itself) before it reads or writes
+// floe records or changes the floe count; k_begin does the same at the start
+// of build and refresh steps.  Positions whose copy is of no live member are
+// skipped: their records are current.
 __global__ void k_spill(DS S, int host) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= S.N || !S.rebuild[2] || (!host && !S.rebuild[0] && !S.rebuild[1])) return;
@@ -328,37 +328,56 @@ __global__ void k_spill(DS S, int host) {
     S.rec[f] = r;
 }
 
-__global__ void k_bin(DS S) {
-    int f = blockIdx.x * blockDim.x + threadIdx.x;
-    // k_spill has brought rec up to date where this step reads it
-    if (f == 0 && (S.rebuild[0] || S.rebuild[1])) S.rebuild[2] = 0;
-    if (f >= S.N || *S.err) return;
+// First kernel of every step, over max(N, cells) threads:
+//  * the queue counters (the dense-search list, queue 3, only on build steps:
+//    it lists positions, which change only then) and, on build steps, the
+//    cell ranges;
+//  * thread i as position: rec from the ordered copies where they are newer,
+//    on build and refresh steps (k_spill's work; k_ranges clears the flag);
+//  * thread i as storage slot: the per-member resets of the step and, on
+//    build steps, the floe's cell key for the sort, written to the sort's
+//    input buffers (bkey, bval).  The position is read from the ordered copy
+//    where that is newer than rec (it is the same value the spill writes),
+//    so the spill and the keys need no order between them.
+__global__ void k_begin(DS S) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (*S.err) return;
+    const bool build = S.rebuild[0] != 0, refresh = S.rebuild[1] != 0, newer = S.rebuild[2] != 0;
+    if (i < NQUEUE * XCD_COUNT && (build || i < 3 * XCD_COUNT)) S.qn[i * QN_STRIDE] = 0;
+    if (build && i <= S.ncells) S.crange[i] = make_int2(0, 0);
+    if (i >= S.N) return;
+    if (newer && (build || refresh)) {
+        const GeoRec g = S.geo[i];
+        const uint32_t fs = S.cval[i];
+        if (g.member >= 0 && fs < (uint32_t)S.N) {
+            const KinRec q = S.kin[i];
+            const ShelRec h = S.shel[i];
+            FloeRec r;
+            r.x = g.x; r.y = g.y; r.u = q.u; r.v = q.v; r.omega = q.omega;
+            r.r = g.r; r.h = q.h; r.mass = q.mass; r.inertia = q.inertia; r.hfa = h.hfa; r.hfw = h.hfw;
+            r.theta = q.theta;
+            S.rec[fs] = r;
+        }
+    }
+    const int f = i;
     const int m = S.member_of[f];
     const bool live = m >= 0 && member_live(S.members[m]);
     if (live && f == S.members[m].first) {
         S.nseeds[m] = 0; S.netab[m] = 0; S.nwtab[m] = 0;
         S.stats[4 * m + 2] = 0; S.stats[4 * m + 3] = 0;
     }
-    if (!*S.rebuild) return;      // the cell order of the last build stands
-    S.cval[f] = (uint32_t)f;
+    if (!build) return;           // the cell order of the last build stands
+    S.bval[f] = (uint32_t)f;
     // floes of free slots and of members past their stop step sort last and
     // take no part in the step
-    if (!live) { S.ckey[f] = (uint32_t)S.ncells; return; }
+    if (!live) { S.bkey[f] = (uint32_t)S.ncells; return; }
     const DevParams& P = S.params[m];
-    double x = S.rec[f].x, y = S.rec[f].y;
+    double x, y;
+    const GeoRec g = newer ? S.geo[S.pos_of[f]] : GeoRec{};
+    if (newer && g.member >= 0) { x = g.x; y = g.y; }
+    else { x = S.rec[f].x; y = S.rec[f].y; }
     int cx = cell_coord(x, P.cell, P.gx), cy = cell_coord(y, P.cell, P.gy);
-    S.ckey[f] = (uint32_t)(P.cell_base + cy * P.gx + cx);
-}
-
-// Per step: the queue counters (the dense-search list, queue 3, only on build
-// steps: it lists positions, which change only then); on build steps the cell
-// ranges too.
-__global__ void k_clear_cells(DS S) {
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (*S.err) return;
-    const bool build = *S.rebuild != 0;
-    if (c < NQUEUE * XCD_COUNT && (build || c < 3 * XCD_COUNT)) S.qn[c * QN_STRIDE] = 0;
-    if (build && c <= S.ncells) S.crange[c] = make_int2(0, 0);
+    S.bkey[f] = (uint32_t)(P.cell_base + cy * P.gx + cx);
 }
 
 // The floe records in the current order and the wind at each floe
@@ -368,6 +387,8 @@ __global__ void k_clear_cells(DS S) {
 // the copies went stale (a member stopped, the host changed the wind).
 __global__ void k_ranges(DS S) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
+    // k_begin has brought rec up to date where this step reads it
+    if (k == 0 && !*S.err && (S.rebuild[0] || S.rebuild[1])) S.rebuild[2] = 0;
     if (k >= S.N || *S.err || (!S.rebuild[0] && !S.rebuild[1])) return;
     int f = (int)S.cval[k];
     int m = S.member_of[f];

@@ -23,9 +23,10 @@ constexpr int RS_TILE = RS_THREADS * RS_IPT;
 
 hipError_t radix_alloc(RadixScratch& s, int64_t n);
 void radix_free(RadixScratch& s);
-// Sorts keys/vals in place (n items, keys < 2^bits).  The error word and the
-// run word are checked by every kernel: a nonzero error word or a zero run
-// word (when given) turns the sort into a no-op.
+// Sorts the n pairs held in the scratch (s.keys_alt, s.vals_alt; keys <
+// 2^bits) into keys/vals, stably.  The error word and the run word are
+// checked by every kernel: a nonzero error word or a zero run word (when
+// given) turns the sort into a no-op.
 hipError_t radix_sort_pairs(RadixScratch& s, uint32_t* keys, uint32_t* vals, int64_t n, int bits,
                             const uint32_t* err, const int32_t* run, hipStream_t stream);
 

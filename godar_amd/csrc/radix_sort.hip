@@ -140,27 +140,20 @@ void radix_free(RadixScratch& s) {
     s = RadixScratch{};
 }
 
-// Result of an odd number of passes back into the caller's arrays.
-__global__ __launch_bounds__(256) void rs_copy_kernel(const uint32_t* __restrict__ ka,
-                                                      const uint32_t* __restrict__ va,
-                                                      uint32_t* __restrict__ keys,
-                                                      uint32_t* __restrict__ vals, int64_t n,
-                                                      const uint32_t* err, const int32_t* run) {
-    if ((err && *err) || (run && !*run)) return;
-    const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (p < n) { keys[p] = ka[p]; vals[p] = va[p]; }
-}
-
 hipError_t radix_sort_pairs(RadixScratch& s, uint32_t* keys, uint32_t* vals, int64_t n, int bits,
                             const uint32_t* err, const int32_t* run, hipStream_t stream) {
     if (n <= 0) return hipSuccess;
     if (n > s.capacity) return hipErrorInvalidValue;
     int passes = (bits + 7) / 8;
     if (passes < 1) passes = 1;
+    // an odd number of passes ends in the caller's arrays (the input is in
+    // the scratch pair); an even count gets one more pass on the top digit,
+    // which leaves a sequence already sorted by it as it is (stable)
+    const int total = passes | 1;
     int ntiles = (int)((n + RS_TILE - 1) / RS_TILE);
-    uint32_t *ka = keys, *va = vals, *kb = s.keys_alt, *vb = s.vals_alt;
-    for (int ps = 0; ps < passes; ++ps) {
-        int shift = ps * 8;
+    uint32_t *ka = s.keys_alt, *va = s.vals_alt, *kb = keys, *vb = vals;
+    for (int ps = 0; ps < total; ++ps) {
+        const int shift = (ps < passes ? ps : passes - 1) * 8;
         hipLaunchKernelGGL(rs_hist_kernel, dim3(ntiles), dim3(RS_THREADS), 0, stream, ka, n, shift,
                            ntiles, s.hist, err, run);
         hipLaunchKernelGGL(rs_scan_kernel, dim3(256), dim3(256), 0, stream, s.hist, ntiles, s.dtot,
@@ -170,9 +163,6 @@ hipError_t radix_sort_pairs(RadixScratch& s, uint32_t* keys, uint32_t* vals, int
         uint32_t* tk = ka; ka = kb; kb = tk;
         uint32_t* tv = va; va = vb; vb = tv;
     }
-    if (ka != keys)
-        hipLaunchKernelGGL(rs_copy_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, ka, va,
-                           keys, vals, n, err, run);
     return hipGetLastError();
 }
 
