@@ -1637,17 +1637,27 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_bond(DS S) {
 // repair left alone: their flags and terms are k_bond's.
 __device__ __forceinline__ void gather_bonds(const DS& S, int k, double* a) {
     const int64_t SS = S.S;
-    const int ni = S.icnt[k];
-    for (int q = 0; q < ni; ++q) {
-        const int rq = S.ibond[q * SS + k];
-        if (rq) add_bond_j(a, S.bterm[(int64_t)(rq - 1) * SS + ent_pos(S.ik[q * SS + k])]);
+    const int ni = S.icnt[k], no = S.ocnt[k];
+    // the first LIST_PEEK slots' bond words of both sides are read at once
+    int rqi[LIST_PEEK], rqo[LIST_PEEK];
+#pragma unroll
+    for (int q = 0; q < LIST_PEEK; ++q) {
+        rqi[q] = q < ni ? S.ibond[q * SS + k] : 0;
+        rqo[q] = q < no ? (int)S.evf[q * SS + k] : 0;
     }
-    const int no = S.ocnt[k];
-    for (int s = 0; s < no; ++s) {
-        const uint32_t fl = S.evf[s * SS + k];
+    auto inner = [&](int q, int rq) {
+        if (rq) add_bond_j(a, S.bterm[(int64_t)(rq - 1) * SS + ent_pos(S.ik[q * SS + k])]);
+    };
+    auto outer = [&](uint32_t fl) {
         const int rq = (int)(fl >> 16);
         if ((fl & EV_BONDED) && rq >= 1 && rq <= S.BC) add_bond_i(a, S.bterm[(int64_t)(rq - 1) * SS + k]);
-    }
+    };
+#pragma unroll
+    for (int q = 0; q < LIST_PEEK; ++q) inner(q, rqi[q]);
+    for (int q = LIST_PEEK; q < ni; ++q) inner(q, S.ibond[q * SS + k]);
+#pragma unroll
+    for (int s = 0; s < LIST_PEEK; ++s) outer((uint32_t)rqo[s]);
+    for (int s = LIST_PEEK; s < no; ++s) outer(S.evf[s * SS + k]);
 }
 
 // Walls of the queued floes (verify_bc, boundaries.f90:1-97).
