@@ -556,33 +556,37 @@ __device__ __forceinline__ int sup_thread(const DS& S, int k, const GeoRec& me, 
     return n;
 }
 
-// (floe, candidate) pairs of one wavefront whose sheltering and
-// classification terms its lanes evaluate together (LDS per wavefront:
-// entry, distance, owner lane)
+// (floe, candidate) pairs of one wavefront in LDS: entry, distance (inner:
+// -1) and owner lane, then the per-lane pair counts
 constexpr int CAND_TCAP = 512;
-constexpr int CAND_LDS_PER_WAVE = CAND_TCAP * (8 + 8 + 1);
+constexpr int CAND_LDS_PER_WAVE = CAND_TCAP * (8 + 8 + 1) + 64 * 4;
 
 // Neighbour search of one floe by one thread: its candidate lists are the
 // entries of its superset within the query radius, inserted into the
 // slot-major rows, then sheltering minima, classification and the per-step
-// resets of the floe's rows.  Superset entries are read SCAN_BATCH at a time
-// and the head of each candidate list is sorted in registers, so that the
-// search waits on few dependent loads.
+// resets of the floe's rows.  The head of each candidate list is sorted in
+// registers while it is built, the rest in its slots in HBM.
 //
-// WAVE: every lane of the wavefront calls it (act: the lane has a floe), and
-// the sheltering and classification terms of all the wavefront's (floe,
-// candidate) pairs are evaluated together, 64 at a time, from a list in LDS
-// (`lds`, CAND_LDS_PER_WAVE bytes for this wavefront); each floe then takes
-// the minimum of its own terms, in its list order (a minimum of fmin terms
-// does not depend on which lane computed them).  Otherwise, or when the
-// pairs outnumber CAND_TCAP, each lane evaluates its own.
+// WAVE: every lane of the wavefront calls it (act: the lane has a floe).  The
+// wavefront's supersets lie one after the other in the pool; they are tested
+// together, 64 consecutive entries a round (each lane finds the entry's
+// owner among the lanes), and the entries within the query radius are listed
+// in LDS (`lds`, CAND_LDS_PER_WAVE bytes for this wavefront) in superset
+// order, so each owner's are contiguous.  Each floe inserts its own into its
+// lists in that order (as its own scan would), and the sheltering and
+// classification terms of all the listed pairs are evaluated together, 64 at
+// a time; each floe then takes the minimum of its own terms (a minimum of
+// fmin terms does not depend on the order or on which lane computed them).
+// Otherwise, or when the pairs outnumber CAND_TCAP, each lane scans its own
+// superset SCAN_BATCH records at a time and evaluates its own terms.
 template <bool MIXED, bool WAVE>
 __device__ __forceinline__ void cand_thread(const DS& S, int k, const GeoRec& me, bool act, bool& qc,
                                             bool& qb, bool& qw, unsigned char* lds) {
     using T = typename Prec<MIXED>::T;
     const int m = act ? me.member : 0;
     const DevParams& P = S.params[m];
-    const double rad = act ? me.r + P.rtree : 0.0;
+    const double mx = act ? me.x : 0.0, my = act ? me.y : 0.0, mr = act ? me.r : 0.0;
+    const double rad = mr + P.rtree;
     const int f = act ? me.id : 0;
     const int first = S.members[m].first;
     const bool bond_step = act && S.members[m].tstep == 1 && P.restart != 1 && P.cohesion;
@@ -607,133 +611,186 @@ __device__ __forceinline__ void cand_thread(const DS& S, int k, const GeoRec& me
         re[t] = INT64_MAX;
         ri[t] = INT64_MAX;
     }
+    int L = 0;
+    int64_t soff = 0;
     if (act) {
         msh = S.shel[k];
         fs = S.cval[k];
         qc = S.hcnt_old[fs] > 0;
         qb = S.bcnt_old[fs] > 0;
         qw = me.y - me.r < 0 || me.y + me.r > P.ny || me.x - me.r < 0 || me.x + me.r > P.nx;
-        const int L = S.scnt[k];
-        const int64_t* se = S.sup + S.soff[k];
-        for (int v0 = 0; v0 < L; v0 += SCAN_BATCH) {
-            GeoRec g[SCAN_BATCH];
-            int qq[SCAN_BATCH];
-#pragma unroll
-            for (int u = 0; u < SCAN_BATCH; ++u)
-                qq[u] = ent_pos(se[min(v0 + u, L - 1)]);   // the last entry stands in past the end
-#pragma unroll
-            for (int u = 0; u < SCAN_BATCH; ++u) g[u] = S.geo[qq[u]];
-#pragma unroll
-            for (int u = 0; u < SCAN_BATCH; ++u) {
-                const int q = qq[u];
-                if (v0 + u >= L) continue;
-                const GeoRec* o = &g[u];
-                const double dx = me.x - o->x, dy = me.y - o->y;
-                const double dist2 = dx * dx + dy * dy;
-                const int j = o->id;
-                const double thr = j > f ? rad : o->r + P.rtree;
-                // far beyond the radius: no square root needed (sqrt is
-                // correctly rounded, so it cannot cross a margin of 1e-12)
-                if (dist2 > thr * thr * (1.0 + 1e-12)) continue;
-                const double d = sqrt(dist2);
-                if (!(d <= thr)) continue;
-                if (j > f) {
-                    if (no >= S.CO) { over |= ERR_OUTER_CAP; continue; }
-                    // insertion by (distance, id) into the register part; the
-                    // entry that leaves it is the largest so far
-                    double xd = d;
-                    int64_t xe = ent(j, q);
-#pragma unroll
-                    for (int t = 0; t < LIST_REGS; ++t) {
-                        const bool lt = xd < rd[t] || (xd == rd[t] && ent_id(xe) < ent_id(re[t]));
-                        const double td = rd[t];
-                        const int64_t te = re[t];
-                        rd[t] = lt ? xd : td;
-                        re[t] = lt ? xe : te;
-                        xd = lt ? td : xd;
-                        xe = lt ? te : xe;
-                    }
-                    if (no >= LIST_REGS) {
-                        int p = no;
-                        while (p > LIST_REGS) {
-                            double dp = S.od[(p - 1) * SS + k];
-                            int64_t ep = oj[(p - 1) * SS + k];
-                            if (dp < xd || (dp == xd && ent_id(ep) < ent_id(xe))) break;
-                            S.od[p * SS + k] = dp;
-                            oj[p * SS + k] = ep;
-                            --p;
-                        }
-                        S.od[p * SS + k] = xd;
-                        oj[p * SS + k] = xe;
-                    }
-                    ++no;
-                } else {
-                    if (ni >= S.CI) { over |= ERR_INNER_CAP; continue; }
-                    int64_t xe = ent(j, q);
-#pragma unroll
-                    for (int t = 0; t < LIST_REGS; ++t) {
-                        const bool lt = xe < ri[t];
-                        const int64_t te = ri[t];
-                        ri[t] = lt ? xe : te;
-                        xe = lt ? te : xe;
-                    }
-                    if (ni >= LIST_REGS) {
-                        int p = ni;
-                        while (p > LIST_REGS) {
-                            int64_t ep = ik[(p - 1) * SS + k];
-                            if (ep < xe) break;
-                            ik[p * SS + k] = ep;
-                            --p;
-                        }
-                        ik[p * SS + k] = xe;
-                    }
-                    ++ni;
-                }
-            }
-        }
+        L = S.scnt[k];
+        soff = S.soff[k];
     }
-    // sheltering and classification over the lists just built.  A minimum
-    // and an "any" do not depend on the order of their terms.
-    int total = CAND_TCAP + 1, base = 0;
-    const int cnt = act ? no + ni : 0;
+    // one candidate j at position q and distance d into the floe's lists
+    auto insert = [&](int j, int q, double d) {
+        if (j > f) {
+            if (no >= S.CO) { over |= ERR_OUTER_CAP; return; }
+            // insertion by (distance, id) into the register part; the entry
+            // that leaves it is the largest so far
+            double xd = d;
+            int64_t xe = ent(j, q);
+#pragma unroll
+            for (int t = 0; t < LIST_REGS; ++t) {
+                const bool lt = xd < rd[t] || (xd == rd[t] && ent_id(xe) < ent_id(re[t]));
+                const double td = rd[t];
+                const int64_t te = re[t];
+                rd[t] = lt ? xd : td;
+                re[t] = lt ? xe : te;
+                xd = lt ? td : xd;
+                xe = lt ? te : xe;
+            }
+            if (no >= LIST_REGS) {
+                int p = no;
+                while (p > LIST_REGS) {
+                    double dp = S.od[(p - 1) * SS + k];
+                    int64_t ep = oj[(p - 1) * SS + k];
+                    if (dp < xd || (dp == xd && ent_id(ep) < ent_id(xe))) break;
+                    S.od[p * SS + k] = dp;
+                    oj[p * SS + k] = ep;
+                    --p;
+                }
+                S.od[p * SS + k] = xd;
+                oj[p * SS + k] = xe;
+            }
+            ++no;
+        } else {
+            if (ni >= S.CI) { over |= ERR_INNER_CAP; return; }
+            int64_t xe = ent(j, q);
+#pragma unroll
+            for (int t = 0; t < LIST_REGS; ++t) {
+                const bool lt = xe < ri[t];
+                const int64_t te = ri[t];
+                ri[t] = lt ? xe : te;
+                xe = lt ? te : xe;
+            }
+            if (ni >= LIST_REGS) {
+                int p = ni;
+                while (p > LIST_REGS) {
+                    int64_t ep = ik[(p - 1) * SS + k];
+                    if (ep < xe) break;
+                    ik[p * SS + k] = ep;
+                    --p;
+                }
+                ik[p * SS + k] = xe;
+            }
+            ++ni;
+        }
+    };
+    bool flat = false;          // the wavefront's pairs are listed in LDS
+    int abase = 0, acount = 0, atotal = 0;
+    int64_t* te = reinterpret_cast<int64_t*>(lds);                      // entry, then the water term
+    double* td = reinterpret_cast<double*>(lds + 8 * CAND_TCAP);         // distance (inner: -1), then the air term
+    uint8_t* to = lds + 16 * CAND_TCAP;                                   // owner lane, then flags
+    int32_t* tn = reinterpret_cast<int32_t*>(lds + 17 * CAND_TCAP);      // pairs per lane
     if (WAVE) {
         const int lane = (int)(threadIdx.x & 63);
-        int incl = cnt;
+        const uint64_t lt = lane ? ((~0ull) >> (64 - lane)) : 0ull;
+        int incl = L;
 #pragma unroll
         for (int off = 1; off < 64; off <<= 1) {
             const int v = __shfl_up(incl, off);
             if (lane >= off) incl += v;
         }
-        base = incl - cnt;
-        total = __shfl(incl, 63);
-    }
-    if (WAVE && total <= CAND_TCAP) {
-        const int lane = (int)(threadIdx.x & 63);
-        int64_t* te = reinterpret_cast<int64_t*>(lds);                     // entry, then the water term
-        double* td = reinterpret_cast<double*>(lds + 8 * CAND_TCAP);        // distance (inner: -1), then the air term
-        uint8_t* to = lds + 16 * CAND_TCAP;                                  // owner lane, then flags
-        if (act) {
+        const int start = incl - L, totalL = __shfl(incl, 63);
+        tn[lane] = 0;
+        wave_lds_sync();
+        const double rtree = P.rtree;
+        bool ovf = false;
+        for (int c0 = 0; c0 < totalL; c0 += 64) {
+            const int idx = c0 + lane;
+            // the entry's owner: the last lane whose superset starts at or
+            // before it (lanes without entries share the next one's start)
+            int own = 0;
 #pragma unroll
-            for (int s2 = 0; s2 < LIST_REGS; ++s2)
-                if (s2 < no) { te[base + s2] = re[s2]; td[base + s2] = rd[s2]; to[base + s2] = (uint8_t)lane; }
-            for (int s2 = LIST_REGS; s2 < no; ++s2) {
-                te[base + s2] = oj[s2 * SS + k];
-                td[base + s2] = S.od[s2 * SS + k];
-                to[base + s2] = (uint8_t)lane;
+            for (int step = 32; step > 0; step >>= 1)
+                if (__shfl(start, own + step) <= idx) own += step;
+            const int ostart = __shfl(start, own);
+            const int64_t ooff = __shfl(soff, own);
+            const double ox = __shfl(mx, own), oy = __shfl(my, own), orad = __shfl(rad, own);
+            const double ort = __shfl(rtree, own);
+            const int of = __shfl(f, own);
+            bool hit = false;
+            double d = 0.0;
+            int j = 0, q = 0;
+            if (idx < totalL) {
+                q = ent_pos(S.sup[ooff + (idx - ostart)]);
+                const GeoRec o = S.geo[q];
+                const double dx = ox - o.x, dy = oy - o.y;
+                const double dist2 = dx * dx + dy * dy;
+                j = o.id;
+                const double thr = j > of ? orad : o.r + ort;
+                // far beyond the radius: no square root needed (sqrt is
+                // correctly rounded, so it cannot cross a margin of 1e-12)
+                if (dist2 <= thr * thr * (1.0 + 1e-12)) {
+                    d = sqrt(dist2);
+                    hit = d <= thr;
+                }
             }
-#pragma unroll
-            for (int s2 = 0; s2 < LIST_REGS; ++s2)
-                if (s2 < ni) { te[base + no + s2] = ri[s2]; td[base + no + s2] = -1.0; to[base + no + s2] = (uint8_t)lane; }
-            for (int s2 = LIST_REGS; s2 < ni; ++s2) {
-                te[base + no + s2] = ik[s2 * SS + k];
-                td[base + no + s2] = -1.0;
-                to[base + no + s2] = (uint8_t)lane;
+            const uint64_t mh = __ballot(hit);
+            if (atotal + __popcll(mh) > CAND_TCAP) { ovf = true; break; }
+            if (hit) {
+                const int at = atotal + __popcll(mh & lt);
+                te[at] = ent(j, q);
+                td[at] = j > of ? d : -1.0;
+                to[at] = (uint8_t)own;
+                atomicAdd(&tn[own], 1);
             }
+            atotal += __popcll(mh);
         }
         wave_lds_sync();
-        for (int c0 = 0; c0 < total; c0 += 64) {
+        if (!ovf) {
+            flat = true;
+            acount = tn[lane];
+            int ai = acount;
+#pragma unroll
+            for (int off = 1; off < 64; off <<= 1) {
+                const int v = __shfl_up(ai, off);
+                if (lane >= off) ai += v;
+            }
+            abase = ai - acount;
+            if (act)
+                for (int i = abase; i < abase + acount; ++i) {
+                    const double dv = td[i];
+                    insert(ent_id(te[i]), ent_pos(te[i]), dv);
+                }
+        }
+    }
+    if (!flat && act) {
+        // (the wavefront search is the common path: this one keeps fewer
+        // records in flight, which keeps the kernel's registers down)
+        constexpr int B = WAVE ? 2 : SCAN_BATCH;
+        const int64_t* se = S.sup + soff;
+        for (int v0 = 0; v0 < L; v0 += B) {
+            GeoRec g[B];
+            int qq[B];
+#pragma unroll
+            for (int u = 0; u < B; ++u)
+                qq[u] = ent_pos(se[min(v0 + u, L - 1)]);   // the last entry stands in past the end
+#pragma unroll
+            for (int u = 0; u < B; ++u) g[u] = S.geo[qq[u]];
+#pragma unroll
+            for (int u = 0; u < B; ++u) {
+                if (v0 + u >= L) continue;
+                const GeoRec* o = &g[u];
+                const double dx = me.x - o->x, dy = me.y - o->y;
+                const double dist2 = dx * dx + dy * dy;
+                const double thr = o->id > f ? rad : o->r + P.rtree;
+                if (dist2 > thr * thr * (1.0 + 1e-12)) continue;
+                const double d = sqrt(dist2);
+                if (!(d <= thr)) continue;
+                insert(o->id, qq[u], d);
+            }
+        }
+    }
+    // sheltering and classification over the pairs found.  A minimum and an
+    // "any" do not depend on the order of their terms.
+    if (flat) {
+        const int lane = (int)(threadIdx.x & 63);
+        wave_lds_sync();
+        for (int c0 = 0; c0 < atotal; c0 += 64) {
             const int idx = c0 + lane;
-            const bool valid = idx < total;
+            const bool valid = idx < atotal;
             int64_t e = 0;
             double d = -1.0;
             int own = lane;
@@ -788,7 +845,7 @@ __device__ __forceinline__ void cand_thread(const DS& S, int k, const GeoRec& me
         }
         wave_lds_sync();
         if (act) {
-            for (int i = base; i < base + cnt; ++i) {
+            for (int i = abase; i < abase + acount; ++i) {
                 sha = dmin(sha, T(td[i]));
                 shw = dmin(shw, T(__longlong_as_double(te[i])));
                 qc = qc || (to[i] & 1u);
