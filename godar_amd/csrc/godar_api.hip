@@ -416,7 +416,7 @@ int launch_step(godar_ctx* c) {
     {
         StageTimer t(c, 9);
         hipLaunchKernelGGL(mixed ? k_fixup<true> : k_fixup<false>, grid_for(N, B[9], true),
-                           dim3(B[9]), 0, s, d);
+                           dim3(B[9]), (B[9] / 64) * FIX_LDS_PER_WAVE, s, d);
     }
     if (c->record) {
         StageTimer t(c, 10);

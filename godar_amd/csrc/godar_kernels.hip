@@ -2299,65 +2299,111 @@ __global__ __launch_bounds__(64) void k_repair(DS S, int nmembers) {
 // ---------------------------------------------------------------------
 // 5. re-gather floes touched by the repair
 // ---------------------------------------------------------------------
+// Events of one wavefront's touched floes evaluated together (LDS per
+// wavefront: one pass of 64 results).
+constexpr int FIX_LDS_PER_WAVE = 64 * (int)sizeof(EventOut);
+static_assert(sizeof(EventOut) % 8 == 0, "EventOut is copied in 8-byte words");
+
+__device__ __forceinline__ void copy_event(EventOut* dst, const EventOut* src) {
+    const uint64_t* s = reinterpret_cast<const uint64_t*>(src);
+    uint64_t* d = reinterpret_cast<uint64_t*>(dst);
+#pragma unroll
+    for (int w = 0; w < (int)(sizeof(EventOut) / 8); ++w) d[w] = s[w];
+}
+
+// The floes the repair touched re-gather every event of theirs, in the
+// reference's order (inner pairs by ascending outer index, then their own
+// candidates in distance order): the repair's result where the table holds
+// one, else the event evaluated again (its inputs are final).  All the
+// wavefront's events are evaluated together, 64 a pass (each lane finds its
+// event's floe among the lanes), and after each pass every floe adds its own
+// in order, so that a floe with many candidates does not hold one lane
+// through all of them.
 template <bool MIXED>
 __global__ __launch_bounds__(FLOE_BLOCK) void k_fixup(DS S) {
-    int k = xcd_block() * blockDim.x + threadIdx.x;
+    extern __shared__ unsigned char fix_lds[];
+    EventOut* ev = reinterpret_cast<EventOut*>(fix_lds + (threadIdx.x >> 6) * FIX_LDS_PER_WAVE);
+    const int k = xcd_block() * blockDim.x + threadIdx.x;
+    const int lane = (int)(threadIdx.x & 63);
     // floes the repair touched (one byte per position decides)
-    if (k >= S.N || *S.err || !(S.wflag[k] & WF_FIXUP)) return;
-    const GeoRec me = S.geo[k];
-    const int m = me.member;
-    if (m < 0) return;
-    const int f = me.id;
-    const int fs = S.cval[k];
-    const DevParams& P = S.params[m];
-    const int tstep = S.members[m].tstep;
-    const int first = S.members[m].first;
+    bool act = k < S.N && !*S.err && (S.wflag[k] & WF_FIXUP);
+    int m = 0;
+    if (act) {
+        m = S.geo[k].member;
+        act = m >= 0;
+    }
+    if (!__ballot(act)) return;
+    if (!act) m = 0;
+    const int f = act ? S.geo[k].id : 0;
+    const int fs = act ? (int)S.cval[k] : 0;
+    const int ni = act ? S.icnt[k] : 0, no = act ? S.ocnt[k] : 0;
+    const int cnt = ni + no;
+    int incl = cnt;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int v = __shfl_up(incl, off);
+        if (lane >= off) incl += v;
+    }
+    const int start = incl - cnt, total = __shfl(incl, 63);
     const int64_t SS = S.S;
-    const int64_t* oj = S.oj;
-    const int64_t* ik = S.ik;
-    const EventOut* etab = S.etab + (int64_t)S.TABK * first;
-    const Floe F0 = floe_at(S, k);
     double acc[NACC];
     acc_init(acc);
-    const int ni = S.icnt[k];
-    for (int q = 0; q < ni; ++q) {
-        const int64_t e = ik[q * SS + k];
-        const int i = ent_id(e), qi = ent_pos(e);
-        int sk = slot_of(S, qi, f);
-        int idx = sk >= 0 && (S.evf[(int64_t)sk * SS + qi] & EV_TABLE) ? S.ov[(int64_t)sk * SS + qi] : -1;
-        if (idx >= 0) {
-            EventOut o = etab[idx];
-            if (!(o.flags & EV_REMOVED)) apply_j(acc, o);
-        } else {
-            Floe Fi = floe_at(S, qi);
-            Floe Ff = F0;
-            PairState ps;
-            load_pair_state<1>(S, S.cval[qi], f, ps);
-            const double2 wi = wind_at(S, qi);
-            EventOut o;
-            pair_event<MIXED>(P, tstep, i - first, f - first, Fi, Ff, wi.x, wi.y, P.uw, P.vw, ps, o, 2);
-            apply_j(acc, o);
+    for (int c0 = 0; c0 < total; c0 += 64) {
+        const int idx = c0 + lane;
+        int own = 0;
+#pragma unroll
+        for (int step = 32; step > 0; step >>= 1)
+            if (__shfl(start, own + step) <= idx) own += step;
+        const int ko = __shfl(k, own), om = __shfl(m, own), of = __shfl(f, own);
+        const int ofs = __shfl(fs, own), oni = __shfl(ni, own), ostart = __shfl(start, own);
+        if (idx < total) {
+            const DevParams& P = S.params[om];
+            const int tstep = S.members[om].tstep;
+            const int first = S.members[om].first;
+            const EventOut* etab = S.etab + (int64_t)S.TABK * first;
+            const int local = idx - ostart;
+            const bool inner = local < oni;
+            int tab = -1, qp = 0, pid = 0;
+            if (inner) {
+                const int64_t e = S.ik[(int64_t)local * SS + ko];
+                pid = ent_id(e);
+                qp = ent_pos(e);
+                const int sk = slot_of(S, qp, of);
+                tab = sk >= 0 && (S.evf[(int64_t)sk * SS + qp] & EV_TABLE) ? S.ov[(int64_t)sk * SS + qp] : -1;
+            } else {
+                const int s2 = local - oni;
+                tab = (S.evf[(int64_t)s2 * SS + ko] & EV_TABLE) ? S.ov[(int64_t)s2 * SS + ko] : -1;
+                const int64_t e = S.oj[(int64_t)s2 * SS + ko];
+                pid = ent_id(e);
+                qp = ent_pos(e);
+            }
+            if (tab >= 0) {
+                copy_event(&ev[lane], &etab[tab]);
+                if (ev[lane].flags & EV_REMOVED) ev[lane].flags = 0;   // adds nothing
+            } else {
+                // inner: pair (i, f) with i's wind; outer: pair (f, j) with f's wind
+                PairState ps;
+                load_pair_state<1>(S, inner ? (int)S.cval[qp] : ofs, inner ? of : pid, ps);
+                const double2 w = wind_at(S, inner ? qp : ko);
+                Floe Fa = floe_at(S, inner ? qp : ko), Fb = floe_at(S, inner ? ko : qp);
+                EventOut o;
+                pair_event<MIXED>(P, tstep, (inner ? pid : of) - first, (inner ? of : pid) - first,
+                                  Fa, Fb, w.x, w.y, P.uw, P.vw, ps, o, inner ? 2 : 1);
+                copy_event(&ev[lane], &o);
+            }
         }
-    }
-    const int no = S.ocnt[k];
-    const double2 wf = wind_at(S, k);
-    for (int s = 0; s < no; ++s) {
-        int idx = (S.evf[s * SS + k] & EV_TABLE) ? S.ov[s * SS + k] : -1;
-        if (idx >= 0) {
-            EventOut o = etab[idx];
-            if (!(o.flags & EV_REMOVED)) apply_i(acc, o);
-        } else {
-            const int64_t e = oj[s * SS + k];
-            const int j = ent_id(e), qj = ent_pos(e);
-            Floe Fi = F0;
-            Floe Fj = floe_at(S, qj);
-            PairState ps;
-            load_pair_state<1>(S, fs, j, ps);
-            EventOut o;
-            pair_event<MIXED>(P, tstep, f - first, j - first, Fi, Fj, wf.x, wf.y, P.uw, P.vw, ps, o, 1);
-            apply_i(acc, o);
+        wave_lds_sync();
+        // every floe adds this pass's events of its own, in order
+        if (act) {
+            const int lo = max(start, c0), hi = min(start + cnt, c0 + 64);
+            for (int i2 = lo; i2 < hi; ++i2) {
+                if (i2 - start < ni) apply_j(acc, ev[i2 - c0]);
+                else apply_i(acc, ev[i2 - c0]);
+            }
         }
+        wave_lds_sync();
     }
+    if (!act) return;
 #pragma unroll
     for (int a = 0; a < NACC; ++a) S.acc[a * SS + k] = acc[a];
     S.cls[k] = CLS_CONTACT | CLS_BOND;
