@@ -2323,8 +2323,12 @@ template <bool MIXED>
 __global__ __launch_bounds__(FLOE_BLOCK) void k_fixup(DS S) {
     extern __shared__ unsigned char fix_lds[];
     EventOut* ev = reinterpret_cast<EventOut*>(fix_lds + (threadIdx.x >> 6) * FIX_LDS_PER_WAVE);
-    const int k = xcd_block() * blockDim.x + threadIdx.x;
+    // positions strided over the wavefronts (lane l of wavefront w takes
+    // l W + w): touched floes sit together in ridging zones, and this spreads
+    // them over many wavefronts instead of a few long ones
     const int lane = (int)(threadIdx.x & 63);
+    const int nwave = (int)(gridDim.x * blockDim.x) >> 6;
+    const int k = lane * nwave + (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
     // floes the repair touched (one byte per position decides)
     bool act = k < S.N && !*S.err && (S.wflag[k] & WF_FIXUP);
     int m = 0;
