@@ -2570,7 +2570,10 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_final(DS S) {
         // the query radius while both floes stay within half the skin)
         const double2 p0 = S.x0[f];
         const double ex = out.x - p0.x, ey = out.y - p0.y, lim = 0.45 * P.skin;
-        if (ex * ex + ey * ey > lim * lim) *S.rebuild = 1;
+        // one store per wavefront (every such floe writing the one word
+        // would queue the writes behind each other)
+        const uint64_t mv = __ballot(ex * ex + ey * ey > lim * lim);
+        if (mv && (int)(threadIdx.x & 63) == __ffsll((long long)mv) - 1) *S.rebuild = 1;
     }
     if (S.record_forces) {
         double tp = (a[A_TAC] != 0.0 ? a[A_PC] / a[A_TAC] : 0.0)
