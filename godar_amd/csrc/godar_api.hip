@@ -23,6 +23,7 @@ __global__ void k_scan_tiles(DS);
 __global__ void k_scan_total(DS, int);
 __global__ void k_scan_offsets(DS);
 template <bool M> __global__ void k_cand_dense(DS);
+template <bool M> __global__ void k_cand_dense_over(DS);
 template <bool M> __global__ void k_contact(DS);
 template <bool M> __global__ void k_bond(DS);
 __global__ void k_stress(DS);
@@ -389,9 +390,12 @@ int launch_step(godar_ctx* c) {
         hipLaunchKernelGGL(mixed ? k_cand<true> : k_cand<false>, grid_for(N, B[3], true),
                            dim3(B[3]), (B[3] / 64) * CAND_LDS_PER_WAVE, s, d);
         if (any_dense) {
-            // one wavefront per dense floe, a contiguous share of the list per XCD
+            // one wavefront per dense floe, a contiguous share of the list per
+            // XCD; 1280 wavefronts per XCD are two rounds of 5 per SIMD
             hipLaunchKernelGGL(mixed ? k_cand_dense<true> : k_cand_dense<false>,
-                               dim3(XCD_COUNT * 1024), dim3(64), 0, s, d);
+                               dim3(XCD_COUNT * 1280), dim3(64), 0, s, d);
+            hipLaunchKernelGGL(mixed ? k_cand_dense_over<true> : k_cand_dense_over<false>,
+                               grid_for(N, FLOE_BLOCK, true), dim3(FLOE_BLOCK), 0, s, d);
         }
     }
     {

@@ -1022,9 +1022,11 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_cand(DS S) {
 // outer candidates by (distance, id), inner by id -- the orders of the thread
 // search.  Sheltering minima are reduced over the lanes (a minimum is exact in
 // any order).  A floe with more than DENSE_CAP candidates on a side falls
-// back to the thread search (lane 0).  Blocks are one wavefront; the queue of
-// dense floes is split into one contiguous share per XCD.
+// back to the thread search (k_cand_dense_over: its registers, inlined here,
+// would cost the kernel a wavefront per SIMD).  Blocks are one wavefront; the
+// queue of dense floes is split into one contiguous share per XCD.
 constexpr int DENSE_CAP = 256;
+constexpr int DENSE_OVER = -1;   // ocnt of a dense floe left to the thread search
 
 __device__ __forceinline__ void bitonic_outer(uint64_t* kd, int64_t* ke, int n, int lane) {
     for (int size = 2; size <= n; size <<= 1) {
@@ -1151,8 +1153,9 @@ __global__ __launch_bounds__(64) void k_sup_dense(DS S) {
     }
 }
 
+// mixed precision fits 5 wavefronts per SIMD without spilling, FP64 does not
 template <bool MIXED>
-__global__ __launch_bounds__(64) void k_cand_dense(DS S) {
+__global__ __launch_bounds__(64, MIXED ? 5 : 1) void k_cand_dense(DS S) {
     using T = typename Prec<MIXED>::T;
     __shared__ uint64_t sd[DENSE_CAP];
     __shared__ int64_t se[DENSE_CAP];
@@ -1299,15 +1302,9 @@ __global__ __launch_bounds__(64) void k_cand_dense(DS S) {
             }
             __syncthreads();
             if (no > DENSE_CAP || ni > DENSE_CAP) {
-                // rare: more candidates than LDS holds, the thread search does it
-                if (lane == 0) {
-                    bool c2, b2, w2;
-                    c2 = b2 = w2 = false;
-                    cand_thread<MIXED, false>(S, k, me, true, c2, b2, w2, nullptr);
-                    if (c2) enqueue_one(S, 0, k);
-                    if (b2) enqueue_one(S, 1, k);
-                    if (w2) enqueue_one(S, 2, k);
-                }
+                // rare: more candidates than LDS holds; k_cand_dense_over
+                // runs the thread search for this floe
+                if (lane == 0) S.ocnt[k] = DENSE_OVER;
                 __syncthreads();
                 if (t + nb < t1) {
                     me_next = S.geo[k_next]; L_next = S.scnt[k_next]; off_next = S.soff[k_next];
@@ -1350,6 +1347,22 @@ __global__ __launch_bounds__(64) void k_cand_dense(DS S) {
         // the compiler is kept from moving accesses across.
         __builtin_amdgcn_wave_barrier();
     }
+}
+
+// The thread search for the dense floes whose candidates overflow the dense
+// search's LDS lists (marked DENSE_OVER in ocnt, which every dense floe's
+// search writes each step).  One thread per entry of the dense queue.
+template <bool MIXED>
+__global__ __launch_bounds__(FLOE_BLOCK) void k_cand_dense_over(DS S) {
+    if (*S.err) return;
+    const int k = queue_entry(S, 3);
+    if (k < 0 || S.ocnt[k] != DENSE_OVER) return;
+    const GeoRec me = S.geo[k];
+    bool qc = false, qb = false, qw = false;
+    cand_thread<MIXED, false>(S, k, me, true, qc, qb, qw, nullptr);
+    if (qc) enqueue_one(S, 0, k);
+    if (qb) enqueue_one(S, 1, k);
+    if (qw) enqueue_one(S, 2, k);
 }
 
 // ---------------------------------------------------------------------
@@ -2662,6 +2675,8 @@ template __global__ void k_sup_dense<false>(DS);
 template __global__ void k_sup_dense<true>(DS);
 template __global__ void k_cand_dense<false>(DS);
 template __global__ void k_cand_dense<true>(DS);
+template __global__ void k_cand_dense_over<false>(DS);
+template __global__ void k_cand_dense_over<true>(DS);
 template __global__ void k_cand<false>(DS);
 template __global__ void k_cand<true>(DS);
 template __global__ void k_contact<false>(DS);
