@@ -368,8 +368,16 @@ int launch_step(godar_ctx* c) {
     int nm = c->Mcap;
     const bool mixed = c->precision == GODAR_MIXED;
     bool any_dense = false;
+    int64_t dense_n = 0;
     for (int k = 0; k < c->Mcap; ++k)
-        if (c->members[k].active && c->dparams[k].dense) any_dense = true;
+        if (c->members[k].active && c->dparams[k].dense) { any_dense = true; dense_n += c->members[k].n; }
+    // The dense search runs one wavefront per block, each block over a fixed
+    // share of the dense floes; the work per floe is uneven, and blocks of
+    // about 4 floes let the dispatcher balance it (100 served requests,
+    // 1.45 M dense floes: 1,024 wavefronts per XCD 4.44 ms, 40,960 3.40 ms).
+    // The superset builds keep 1,024: they return at once on most steps, and
+    // dispatching that many more wavefronts would cost more than they save.
+    const dim3 dense_grid(XCD_COUNT * (int)std::min<int64_t>(65536, std::max<int64_t>(1024, (dense_n + 31) / 32)));
     {
         StageTimer t(c, 2);
         hipLaunchKernelGGL(k_ranges, grid_for(N, B[2], false), dim3(B[2]), 0, s, d);
@@ -390,10 +398,9 @@ int launch_step(godar_ctx* c) {
         hipLaunchKernelGGL(mixed ? k_cand<true> : k_cand<false>, grid_for(N, B[3], true),
                            dim3(B[3]), (B[3] / 64) * CAND_LDS_PER_WAVE, s, d);
         if (any_dense) {
-            // one wavefront per dense floe, a contiguous share of the list per
-            // XCD; 1280 wavefronts per XCD are two rounds of 5 per SIMD
+            // one wavefront per dense floe, a contiguous share of the list per XCD
             hipLaunchKernelGGL(mixed ? k_cand_dense<true> : k_cand_dense<false>,
-                               dim3(XCD_COUNT * 1280), dim3(64), 0, s, d);
+                               dense_grid, dim3(64), 0, s, d);
             hipLaunchKernelGGL(mixed ? k_cand_dense_over<true> : k_cand_dense_over<false>,
                                grid_for(N, FLOE_BLOCK, true), dim3(FLOE_BLOCK), 0, s, d);
         }
@@ -441,12 +448,17 @@ int launch_step(godar_ctx* c) {
 std::vector<unsigned char> launch_signature(godar_ctx* c) {
     sync_ds(c);
     std::vector<unsigned char> sig(sizeof(DS) + sizeof(RadixScratch) + sizeof(c->sblock) +
-                                   3 * sizeof(int));
+                                   5 * sizeof(int));
     unsigned char* p = sig.data();
     std::memcpy(p, &c->d, sizeof(DS)); p += sizeof(DS);
     std::memcpy(p, &c->rs, sizeof(RadixScratch)); p += sizeof(RadixScratch);
     std::memcpy(p, c->sblock, sizeof(c->sblock)); p += sizeof(c->sblock);
-    int extra[3] = {c->precision, c->Mcap, c->N_used};
+    // the dense kernels are launched, and their grid sized, by the members'
+    // dense floe count
+    int dense_members = 0, dense_n = 0;
+    for (int k = 0; k < c->Mcap; ++k)
+        if (c->members[k].active && c->dparams[k].dense) { ++dense_members; dense_n += c->members[k].n; }
+    int extra[5] = {c->precision, c->Mcap, c->N_used, dense_members, dense_n};
     std::memcpy(p, extra, sizeof(extra));
     return sig;
 }
