@@ -135,6 +135,35 @@ def run_serving(nreq: int, seed: int = 4, hbm_gib: float = 16.0, chunk: int = 25
                 peak_members=out["peak_members"], peak_floes=out["peak_floes"], chunk=chunk)
 
 
+def run_served_mix(nreq: int, steps: int, seed: int = 4):
+    """The served requests' fields stepped together as one batch (every
+    request admitted at once, mixed precision): 5 warm-up steps, then `steps`
+    timed steps and the per-stage times of as many more."""
+    from godar_amd import workloads as W
+    from godar_amd.sim import Context
+    reqs = W.config4_requests(nreq, seed)
+    n = sum(f.n for f, _, _, _ in reqs)
+    ctx = Context(n, len(reqs), "mixed")
+    for f, p, _, _ in reqs:
+        ctx.add_field(p, f)
+    ctx.step(5)
+    ctx.sync()
+    t0 = time.perf_counter()
+    ctx.step(steps)
+    ctx.sync()
+    ms = (time.perf_counter() - t0) / steps * 1e3
+    ctx.profile(True)
+    ctx.step(steps)
+    ctx.sync()
+    st = ctx.stage_times()
+    ctx.profile(False)
+    out = {"workload": f"{nreq} served requests at once", "floes": n, "steps": steps,
+           "ms_per_step": ms, "stages_ms_per_step": {k: v[0] / v[1] for k, v in st.items() if v[1]},
+           "pairs": ctx.totals()}
+    ctx.close()
+    return out
+
+
 def run_plain(which: str, steps: int, warmup: int):
     from godar_amd import workloads as W
     from godar_amd.sim import Context
@@ -255,7 +284,12 @@ def main():
                     help="served requests of the continuous-batching leg (0: skip)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (calibrates rocprof byte counters)")
+    ap.add_argument("--served-mix", action="store_true",
+                    help="only step all served requests at once (for profilers)")
     args = ap.parse_args()
+    if args.served_mix:
+        print(json.dumps(run_served_mix(args.serve_requests, args.steps)))
+        return
     if args.copy_calibration:
         import torch
         a = torch.ones(1 << 27, dtype=torch.float64, device="cuda")   # 1 GiB
