@@ -160,31 +160,54 @@ void derive_params(const godar_params& p, DevParams& d) {
     d.wind_grid = -1;
 }
 
-void free_lists(godar_ctx* c) {
-    DS& d = c->d;
-    dfree(d.oj); dfree(d.od); dfree(d.ik); dfree(d.evf); dfree(d.ov); dfree(d.bterm);
-    dfree(d.ssc); dfree(d.ssb); dfree(d.ibond);
-    for (int k = 0; k < 2; ++k) { dfree(c->hist[k]); dfree(c->bond[k]); }
+// The per-floe lists sized by the list capacities (CO, CI, HC, BC).
+struct PairLists {
+    int64_t* oj = nullptr;
+    double* od = nullptr;
+    int64_t* ik = nullptr;
+    uint32_t* evf = nullptr;
+    int32_t* ov = nullptr;
+    BondTerms* bterm = nullptr;
+    double *ssc = nullptr, *ssb = nullptr;
+    int32_t* ibond = nullptr;
+    HistRec* hist[2] = {nullptr, nullptr};
+    BondRec* bond[2] = {nullptr, nullptr};
+};
+
+void free_lists(PairLists& L) {
+    dfree(L.oj); dfree(L.od); dfree(L.ik); dfree(L.evf); dfree(L.ov); dfree(L.bterm);
+    dfree(L.ssc); dfree(L.ssb); dfree(L.ibond);
+    for (int k = 0; k < 2; ++k) { dfree(L.hist[k]); dfree(L.bond[k]); }
 }
 
-hipError_t alloc_lists(godar_ctx* c) {
+// All or nothing: on failure nothing stays allocated.
+hipError_t alloc_lists(PairLists& L, int64_t S, int CO, int CI, int HC, int BC) {
+    hipError_t e = hipSuccess;
+    auto ok = [&](hipError_t x) { e = x; return x == hipSuccess; };
+    bool good = ok(dalloc(&L.oj, S * CO)) && ok(dalloc(&L.od, S * CO)) && ok(dalloc(&L.ik, S * CI)) &&
+                ok(dalloc(&L.evf, S * CO)) && ok(dalloc(&L.ov, S * CO)) &&
+                ok(dalloc(&L.bterm, S * BC)) && ok(dalloc(&L.ssc, S * CO)) &&
+                ok(dalloc(&L.ibond, S * CI)) && ok(dalloc(&L.ssb, S * CO));
+    for (int k = 0; good && k < 2; ++k)
+        good = ok(dalloc(&L.hist[k], S * HC)) && ok(dalloc(&L.bond[k], S * BC));
+    if (!good) free_lists(L);
+    return e;
+}
+
+PairLists lists_of(const godar_ctx* c) {
+    const DS& d = c->d;
+    PairLists L;
+    L.oj = d.oj; L.od = d.od; L.ik = d.ik; L.evf = d.evf; L.ov = d.ov; L.bterm = d.bterm;
+    L.ssc = d.ssc; L.ssb = d.ssb; L.ibond = d.ibond;
+    for (int k = 0; k < 2; ++k) { L.hist[k] = c->hist[k]; L.bond[k] = c->bond[k]; }
+    return L;
+}
+
+void set_lists(godar_ctx* c, const PairLists& L) {
     DS& d = c->d;
-    int64_t S = c->S;
-    hipError_t e;
-    if ((e = dalloc(&d.oj, S * c->CO)) != hipSuccess) return e;
-    if ((e = dalloc(&d.od, S * c->CO)) != hipSuccess) return e;
-    if ((e = dalloc(&d.ik, S * c->CI)) != hipSuccess) return e;
-    if ((e = dalloc(&d.evf, S * c->CO)) != hipSuccess) return e;
-    if ((e = dalloc(&d.ov, S * c->CO)) != hipSuccess) return e;
-    if ((e = dalloc(&d.bterm, S * c->BC)) != hipSuccess) return e;
-    if ((e = dalloc(&d.ssc, S * c->CO)) != hipSuccess) return e;
-    if ((e = dalloc(&d.ibond, S * c->CI)) != hipSuccess) return e;
-    if ((e = dalloc(&d.ssb, S * c->CO)) != hipSuccess) return e;
-    for (int k = 0; k < 2; ++k) {
-        if ((e = dalloc(&c->hist[k], S * c->HC)) != hipSuccess) return e;
-        if ((e = dalloc(&c->bond[k], S * c->BC)) != hipSuccess) return e;
-    }
-    return hipSuccess;
+    d.oj = L.oj; d.od = L.od; d.ik = L.ik; d.evf = L.evf; d.ov = L.ov; d.bterm = L.bterm;
+    d.ssc = L.ssc; d.ssb = L.ssb; d.ibond = L.ibond;
+    for (int k = 0; k < 2; ++k) { c->hist[k] = L.hist[k]; c->bond[k] = L.bond[k]; }
 }
 
 void bind_parity(godar_ctx* c) {
@@ -242,8 +265,13 @@ int ensure_cells(godar_ctx* c, int32_t need) {
     if (need <= c->ncells_cap) return GODAR_OK;
     int32_t cap = std::max(need, c->ncells_cap * 2);
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    int2* cr = nullptr;
+    if (dalloc(&cr, (int64_t)cap + 1) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(c, GODAR_ERR_NOMEM, "out of device memory for the cell table");
+    }
     dfree(c->d.crange);
-    HIPCHK(c, dalloc(&c->d.crange, (int64_t)cap + 1));
+    c->d.crange = cr;
     c->ncells_cap = cap;
     return GODAR_OK;
 }
@@ -590,7 +618,11 @@ int godar_create(int64_t max_floes, int32_t max_members, int32_t precision, goda
         ACHK(dalloc(&c->hcnt[k], S));
         ACHK(dalloc(&c->bcnt[k], S));
     }
-    ACHK(alloc_lists(c));
+    {
+        PairLists L;
+        ACHK(alloc_lists(L, S, c->CO, c->CI, c->HC, c->BC));
+        set_lists(c, L);
+    }
     ACHK(dalloc(&d.acc, (int64_t)NACC * S));
     ACHK(dalloc(&d.wout, S));
     ACHK(dalloc(&d.wflag, (S + 3) / 4 * 4));   // whole 32-bit words (wflag_set)
@@ -660,7 +692,11 @@ int godar_destroy(godar_ctx* c) {
     dfree(d.ckey); dfree(d.cval); dfree(d.ocnt); dfree(d.icnt);
     dfree(d.crange);
     for (int k = 0; k < 2; ++k) { dfree(c->hcnt[k]); dfree(c->bcnt[k]); }
-    free_lists(c);
+    {
+        PairLists L = lists_of(c);
+        free_lists(L);
+        set_lists(c, L);
+    }
     dfree(d.wout); dfree(d.wflag); dfree(d.cls); dfree(d.qlist); dfree(d.qn); dfree(d.bgat); dfree(d.seeds); dfree(d.nseeds); dfree(d.heap); dfree(d.etab);
     dfree(d.netab); dfree(d.wtab); dfree(d.nwtab); dfree(d.ovw); dfree(d.dirty);
     dfree(d.wnz); dfree(d.stats); dfree(d.err); dfree(d.steps_done);
@@ -721,6 +757,13 @@ int godar_member_add(godar_ctx* c, const godar_params* p, int32_t n, const doubl
     if (n > 0xFFFF * 64) return fail(c, GODAR_ERR_ARG, "member_add: member too large");
     if (!(p->dt > 0) || !(p->rtree >= 0) || !(p->nx > 0) || !(p->ny > 0))
         return fail(c, GODAR_ERR_ARG, "member_add: dt, nx, ny must be positive, rtree >= 0");
+    // The reference searches a tree of the positions at the last rebuild,
+    // rebuilt when tstep == 1 or mod(tstep, ntree) == 0 (stepper.f90:44-50);
+    // the search here always uses the current positions, which is ntree = 1
+    // (the default, par_get.f90:28).
+    if (p->ntree != 1)
+        return fail(c, GODAR_ERR_ARG, "member_add: ntree must be 1 (the neighbour search uses the "
+                                      "current positions every step)");
     int m = -1;
     for (int k = 0; k < c->Mcap; ++k)
         if (!c->members[k].active) { m = k; break; }
@@ -987,6 +1030,9 @@ int godar_member_restore(godar_ctx* c, const void* buf, int64_t size, int32_t* m
     if (std::memcmp(h.magic, "GODARCK1", 8) != 0 || h.version != CK_VERSION || h.n <= 0 ||
         h.nhist < 0 || h.nbond < 0)
         return fail(c, GODAR_ERR_ARG, "member_restore: not a checkpoint block");
+    if (h.precision != c->precision)
+        return fail(c, GODAR_ERR_ARG, "member_restore: the block was saved in a context of the other "
+                                      "precision (it would not continue exactly)");
     const int n = h.n;
     const int64_t need = (int64_t)sizeof(CkHeader) + (int64_t)sizeof(godar_params) + (int64_t)n * 12 * 8 +
                          (int64_t)n * 4 * 8 + (int64_t)n * 4 + h.nhist * (int64_t)CK_HIST +
@@ -1018,6 +1064,23 @@ int godar_member_restore(godar_ctx* c, const void* buf, int64_t size, int32_t* m
         sh += hc[i]; sb += bc[i];
     }
     if (sh != h.nhist || sb != h.nbond) return fail(c, GODAR_ERR_ARG, "member_restore: bad counts");
+    // every record is owned by the lower index of its pair: partner in (i, n)
+    {
+        const char* q = hist_at;
+        for (int i = 0; i < n; ++i)
+            for (int k = 0; k < hc[i]; ++k, q += CK_HIST) {
+                int32_t j;
+                std::memcpy(&j, q, 4);
+                if (j <= i || j >= n) return fail(c, GODAR_ERR_ARG, "member_restore: history partner out of range");
+            }
+        q = bond_at;
+        for (int i = 0; i < n; ++i)
+            for (int k = 0; k < bc[i]; ++k, q += CK_BOND) {
+                int32_t j;
+                std::memcpy(&j, q, 4);
+                if (j <= i || j >= n) return fail(c, GODAR_ERR_ARG, "member_restore: bond partner out of range");
+            }
+    }
     if (maxh > c->HC || maxb > c->BC) {
         int rc = godar_reserve(c, 0, 0, std::max(maxh, c->HC), std::max(maxb, c->BC));
         if (rc != GODAR_OK) return rc;
@@ -1183,28 +1246,70 @@ int godar_set_bonds(godar_ctx* c, int32_t m, int32_t nb, const int32_t* bi, cons
 }
 
 int godar_reserve(godar_ctx* c, int32_t outer, int32_t inner, int32_t histc, int32_t bonds) {
-    if (!c || outer < 0 || inner < 0 || histc < 0 || bonds < 0 || outer > 0xFFFE)
+    if (!c || outer < 0 || inner < 0 || histc < 0 || bonds < 0 || outer > 0xFFFE ||
+        inner > (1 << 15) || histc > (1 << 12) || bonds > (1 << 12))
         return fail(c, GODAR_ERR_ARG, "reserve: bad argument");
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    int nCO = outer ? outer : c->CO, nCI = inner ? inner : c->CI;
-    int nHC = histc ? histc : c->HC, nBC = bonds ? bonds : c->BC;
+    const int nCO = outer ? outer : c->CO, nCI = inner ? inner : c->CI;
+    const int nHC = histc ? histc : c->HC, nBC = bonds ? bonds : c->BC;
     if (nCO == c->CO && nCI == c->CI && nHC == c->HC && nBC == c->BC) return GODAR_OK;
-    // keep the pair state of the current parity
-    int64_t S = c->S;
-    int o = c->parity;
-    HistRec* oh = c->hist[o];
-    BondRec* ob = c->bond[o];
-    c->hist[o] = nullptr;
-    c->bond[o] = nullptr;
-    int oHC = c->HC, oBC = c->BC;
-    free_lists(c);
+    const int64_t S = c->S;
+    const int o = c->parity;
+    // The kernels walk a floe's records up to its count, and the last step's
+    // candidate lists stay readable (godar_get_pairs): a list may shrink only
+    // down to the longest one in use.
+    auto longest = [&](const int32_t* cnt, int32_t& mx) -> int {
+        mx = 0;
+        if (c->N_used == 0) return GODAR_OK;
+        std::vector<int32_t> h(c->N_used);
+        HIPCHK(c, hipMemcpy(h.data(), cnt, sizeof(int32_t) * c->N_used, hipMemcpyDeviceToHost));
+        for (int32_t v : h) mx = std::max(mx, v);
+        return GODAR_OK;
+    };
+    int32_t mo = 0, mi = 0, mh = 0, mb = 0;
+    int rc;
+    if (nCO < c->CO && (rc = longest(c->d.ocnt, mo)) != GODAR_OK) return rc;
+    if (nCI < c->CI && (rc = longest(c->d.icnt, mi)) != GODAR_OK) return rc;
+    if (nHC < c->HC && (rc = longest(c->hcnt[o], mh)) != GODAR_OK) return rc;
+    if (nBC < c->BC && (rc = longest(c->bcnt[o], mb)) != GODAR_OK) return rc;
+    if (nCO < mo || nCI < mi || nHC < mh || nBC < mb)
+        return fail(c, GODAR_ERR_ARG,
+                    "reserve: a capacity is below the entries a floe holds (outer " + std::to_string(mo) +
+                        ", inner " + std::to_string(mi) + ", histories " + std::to_string(mh) +
+                        ", bonds " + std::to_string(mb) + ")");
+    // New lists first: if an allocation fails the context keeps its lists
+    // and stays usable.
+    PairLists nl;
+    if (alloc_lists(nl, S, nCO, nCI, nHC, nBC) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(c, GODAR_ERR_NOMEM, "reserve: out of device memory (the current capacities stay)");
+    }
+    PairLists ol = lists_of(c);
+    // slot-major lists: the first rows of each carry over
+    auto rows = [&](void* dst, const void* src, size_t elem, int oc, int nc) {
+        return hipMemcpy(dst, src, elem * (size_t)S * (size_t)std::min(oc, nc), hipMemcpyDeviceToDevice);
+    };
+    hipError_t e = hipSuccess;
+    auto ok = [&](hipError_t x) { e = x; return x == hipSuccess; };
+    bool good = ok(rows(nl.oj, ol.oj, sizeof(int64_t), c->CO, nCO)) &&
+                ok(rows(nl.od, ol.od, sizeof(double), c->CO, nCO)) &&
+                ok(rows(nl.evf, ol.evf, sizeof(uint32_t), c->CO, nCO)) &&
+                ok(rows(nl.ov, ol.ov, sizeof(int32_t), c->CO, nCO)) &&
+                ok(rows(nl.ssc, ol.ssc, sizeof(double), c->CO, nCO)) &&
+                ok(rows(nl.ssb, ol.ssb, sizeof(double), c->CO, nCO)) &&
+                ok(rows(nl.ik, ol.ik, sizeof(int64_t), c->CI, nCI)) &&
+                ok(rows(nl.ibond, ol.ibond, sizeof(int32_t), c->CI, nCI)) &&
+                ok(rows(nl.bterm, ol.bterm, sizeof(BondTerms), c->BC, nBC)) &&
+                ok(rows(nl.hist[o], ol.hist[o], sizeof(HistRec), c->HC, nHC)) &&
+                ok(rows(nl.bond[o], ol.bond[o], sizeof(BondRec), c->BC, nBC));
+    if (!good) {
+        free_lists(nl);
+        return fail(c, GODAR_ERR_HIP, std::string("reserve: ") + hipGetErrorString(e));
+    }
+    free_lists(ol);
+    set_lists(c, nl);
     c->CO = nCO; c->CI = nCI; c->HC = nHC; c->BC = nBC;
-    hipError_t e = alloc_lists(c);
-    if (e != hipSuccess) { dfree(oh); dfree(ob); return fail(c, GODAR_ERR_NOMEM, "reserve: out of device memory"); }
-    HIPCHK(c, hipMemcpy(c->hist[o], oh, sizeof(HistRec) * S * std::min(oHC, nHC), hipMemcpyDeviceToDevice));
-    HIPCHK(c, hipMemcpy(c->bond[o], ob, sizeof(BondRec) * S * std::min(oBC, nBC), hipMemcpyDeviceToDevice));
-    dfree(oh);
-    dfree(ob);
+    sync_ds(c);
     return GODAR_OK;
 }
 
@@ -1385,19 +1490,32 @@ static bool grow_for(godar_ctx* c, uint32_t bits) {
     if (bits & (ERR_REPAIR_CAP | ERR_SEED_CAP)) {
         if (c->HEAPK >= 1024) return false;
         if (hipStreamSynchronize(c->stream) != hipSuccess) return false;
-        c->SEEDK *= 2; c->HEAPK *= 2; c->TABK *= 2;
+        // the larger pools first: on failure the current ones stay
+        uint64_t *seeds = nullptr, *heap = nullptr;
+        EventOut* etab = nullptr;
+        if (dalloc(&seeds, (int64_t)2 * c->SEEDK * c->S) != hipSuccess ||
+            dalloc(&heap, (int64_t)2 * c->HEAPK * c->S) != hipSuccess ||
+            dalloc(&etab, (int64_t)2 * c->TABK * c->S) != hipSuccess) {
+            dfree(seeds); dfree(heap); dfree(etab);
+            (void)hipGetLastError();
+            return false;
+        }
         dfree(c->d.seeds); dfree(c->d.heap); dfree(c->d.etab);
-        if (dalloc(&c->d.seeds, (int64_t)c->SEEDK * c->S) != hipSuccess) return false;
-        if (dalloc(&c->d.heap, (int64_t)c->HEAPK * c->S) != hipSuccess) return false;
-        if (dalloc(&c->d.etab, (int64_t)c->TABK * c->S) != hipSuccess) return false;
+        c->d.seeds = seeds; c->d.heap = heap; c->d.etab = etab;
+        c->SEEDK *= 2; c->HEAPK *= 2; c->TABK *= 2;
         grew = true;
     }
     if (bits & ERR_SUP_CAP) {
         if (c->sup_cap >= ((int64_t)1 << 34)) return false;
         if (hipStreamSynchronize(c->stream) != hipSuccess) return false;
-        c->sup_cap *= 2;
+        int64_t* sup = nullptr;
+        if (dalloc(&sup, 2 * c->sup_cap) != hipSuccess) {
+            (void)hipGetLastError();
+            return false;
+        }
         dfree(c->d.sup);
-        if (dalloc(&c->d.sup, c->sup_cap) != hipSuccess) return false;
+        c->d.sup = sup;
+        c->sup_cap *= 2;
         grew = true;
     }
     if (grew) force_rebuild(c);

@@ -63,7 +63,8 @@ def config3(seed: int = 3, members: int = 512, lo: int = 500, hi: int = 8000, wo
 
 def config4_request(rng: np.random.Generator, seed: int):
     """One served request: 1,000-50,000 floes (log-uniform) from the config-1
-    generator (power-law radii, packing 0.7), 500-5,000 steps, dt = 1 s."""
+    generator (power-law radii, packing 0.7, Gaussian wind field), 500-5,000
+    steps, dt = 1 s."""
     n = int(math.floor(math.exp(rng.uniform(math.log(1e3), math.log(5e4)))))
     steps = int(rng.integers(500, 5001))
     f, p = config4_field((n, seed))
@@ -71,11 +72,10 @@ def config4_request(rng: np.random.Generator, seed: int):
 
 
 def config4_field(args):
-    """Field and parameters of a served request of n floes (seeded)."""
+    """Field and parameters of a served request of n floes (seeded): the
+    config-1 generator, Gaussian wind field included."""
     n, seed = args
-    f = g.make_field(n, seed, radius="powerlaw", r_lo=200.0, r_hi=5000.0, h_median=1.5,
-                     h_sigma=0.3, packing=0.7, box=None, wind_speed=8.0, current_speed=0.1,
-                     placement="rsa")
+    f = g.config1(seed, n=n)
     return f, params_for(f, bond_frac=0.3, bond_seed=seed)
 
 

@@ -83,7 +83,10 @@ const char *godar_last_error(const godar_ctx *ctx);
  * derives mass, inertia and freeboards (floe_properties, physics.f90:179-197)
  * as ini_get does (ini_get.f90:170-247).  tstep0 is the reference step
  * number of the member's next step (1 for a fresh run; bonds form in step 1
- * unless p->restart == 1).  *member receives the member id.
+ * unless p->restart == 1).  *member receives the member id.  p->ntree must
+ * be 1: the neighbour search uses the current positions every step, where
+ * the reference with ntree > 1 searches a tree of older positions
+ * (stepper.f90:44-50); other values return GODAR_ERR_ARG.
  */
 int godar_member_add(godar_ctx *ctx, const godar_params *p, int32_t n,
                      const double *x, const double *y, const double *u, const double *v,
@@ -116,6 +119,9 @@ int godar_set_bonds(godar_ctx *ctx, int32_t member, int32_t nb, const int32_t *b
  * continues exactly as the saved one would have (a gridded wind is not part
  * of the block: set it again).
  */
+/* godar_member_restore refuses (GODAR_ERR_ARG) a block saved in a context of
+ * the other precision and one whose pair records name a partner outside
+ * (owner, n). */
 int godar_checkpoint_size(godar_ctx *ctx, int32_t member, int64_t *bytes);
 int godar_checkpoint_save(godar_ctx *ctx, int32_t member, const godar_params *p, void *buf,
                           int64_t cap);
@@ -145,7 +151,10 @@ int godar_member_run_until(godar_ctx *ctx, int32_t member, int32_t tlast);
 /* Next step number of every member slot (max_members entries; -1 = free). */
 int godar_member_tsteps(godar_ctx *ctx, int32_t *tsteps);
 /* Per-floe list capacities (outer candidates, inner candidates, contact
- * histories, bonds per floe); 0 keeps the current value. */
+ * histories, bonds per floe); 0 keeps the current value.  A capacity may
+ * shrink only down to the longest list in use (else GODAR_ERR_ARG); if the
+ * new lists cannot be allocated the call returns GODAR_ERR_NOMEM and the
+ * context keeps its current lists. */
 int godar_reserve(godar_ctx *ctx, int32_t outer, int32_t inner, int32_t hist, int32_t bonds);
 
 /* Copy a member's floe state back: any pointer may be NULL. */

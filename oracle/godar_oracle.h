@@ -62,6 +62,8 @@ int or_get_bonds(or_sim *s, int cap, int32_t *bi, int32_t *bj);
 void or_get_pairs(or_sim *s, int np, const int32_t *pi, const int32_t *pj, double *out);
 void or_get_stats(or_sim *s, int32_t *st);   /* ridge_pair, ridge_wall, ncand, ncontact */
 uint32_t or_bond_threshold(double frac);
+/* fit (contact_forces.f90:247-265) by itself. */
+double or_fit(double xi);
 int or_bond_keep(int i, int j, uint64_t seed, uint32_t thr24);
 
 #ifdef __cplusplus

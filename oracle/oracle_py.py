@@ -66,6 +66,8 @@ def lib():
         L.or_bond_threshold.restype = ctypes.c_uint32
         L.or_bond_keep.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint32]
         L.or_bond_keep.restype = ctypes.c_int
+        L.or_fit.argtypes = [ctypes.c_double]
+        L.or_fit.restype = ctypes.c_double
         _lib = L
     return _lib
 

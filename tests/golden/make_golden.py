@@ -12,11 +12,12 @@ holds the inputs (initial floe field, parameters) and the expected outputs:
   after 1, 10 and 50 steps and the pair sets of every step.
 * config0_1e4.npz -- the same 1,000-floe field after 2,000 and 10,000 steps,
   plus the reference's sensitivity envelope: the max and median relative
-  position deviation of runs whose initial centres were moved by one ulp
-  (the dynamics are chaotic; any result that is not bit-identical is judged
+  position deviation, and the relative L2 deviation of positions and of
+  velocities, of runs whose initial centres were moved by one ulp (the
+  dynamics are chaotic; any result that is not bit-identical is judged
   against this envelope).
 
-Usage:  python tests/golden/make_golden.py [--long]
+Usage:  python tests/golden/make_golden.py [--long | --long-only]
 """
 from __future__ import annotations
 
@@ -82,12 +83,16 @@ def run(field, params, checkpoints, engine="ref", cfg=None, pair_steps=None):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--long", action="store_true", help="also the 10,000-step fixture")
+    ap.add_argument("--long-only", action="store_true", help="only the 10,000-step fixture")
     a = ap.parse_args()
     import ref_harness as rh
     eng = "ref" if rh.available("n1000") and rh.available("n64") else "oracle"
     print("engine:", eng)
     f0 = g.config0(0)
     p0 = config0_params(f0)
+    if a.long_only:
+        long_fixture(f0, p0, eng)
+        return
     t = time.time()
     out = run(f0, p0, {1, 100, 1000}, eng, "n1000")
     np.savez_compressed(os.path.join(HERE, "config0_seed0.npz"), engine=eng,
@@ -104,26 +109,35 @@ def main():
                         **field_arrays(fb), **out)
     print("dense64.npz")
     if a.long:
-        t = time.time()
-        out = run(f0, p0, {2000, 10000}, eng, "n1000")
-        keep = {k: v for k, v in out.items() if k.startswith(("x_", "y_", "u_", "v_", "r_", "h_"))}
-        # sensitivity envelope: the same run with every floe centre moved by
-        # one ulp (two seeds); max and median relative position deviation
-        env = {}
-        for k, seed in enumerate((1, 2)):
-            fp = g.config0(0)
-            rng = np.random.default_rng(seed)
-            fp.x = np.where(rng.random(fp.n) < 0.5, np.nextafter(fp.x, np.inf), np.nextafter(fp.x, -np.inf))
-            fp.y = np.where(rng.random(fp.n) < 0.5, np.nextafter(fp.y, np.inf), np.nextafter(fp.y, -np.inf))
-            o = run(fp, p0, {2000, 10000}, "oracle")
-            for T in (2000, 10000):
-                xy = np.concatenate([o[f"x_{T}"], o[f"y_{T}"]])
-                ref = np.concatenate([out[f"x_{T}"], out[f"y_{T}"]])
-                rel = np.abs(xy - ref) / np.abs(ref)
-                env[f"ulp_env_max_{T}_{k}"] = np.float64(rel.max())
-                env[f"ulp_env_median_{T}_{k}"] = np.float64(np.median(rel))
-        np.savez_compressed(os.path.join(HERE, "config0_1e4.npz"), engine=eng, **keep, **env)
-        print("config0_1e4.npz", time.time() - t, {k: float(v) for k, v in env.items()})
+        long_fixture(f0, p0, eng)
+
+
+def long_fixture(f0, p0, eng):
+    t = time.time()
+    out = run(f0, p0, {2000, 10000}, eng, "n1000")
+    keep = {k: v for k, v in out.items() if k.startswith(("x_", "y_", "u_", "v_", "r_", "h_"))}
+    # sensitivity envelope: the same run with every floe centre moved by
+    # one ulp (two seeds): max and median relative position deviation, and
+    # relative L2 deviation of positions and of velocities
+    env = {}
+    for k, seed in enumerate((1, 2)):
+        fp = g.config0(0)
+        rng = np.random.default_rng(seed)
+        fp.x = np.where(rng.random(fp.n) < 0.5, np.nextafter(fp.x, np.inf), np.nextafter(fp.x, -np.inf))
+        fp.y = np.where(rng.random(fp.n) < 0.5, np.nextafter(fp.y, np.inf), np.nextafter(fp.y, -np.inf))
+        o = run(fp, p0, {2000, 10000}, "oracle")
+        for T in (2000, 10000):
+            xy = np.concatenate([o[f"x_{T}"], o[f"y_{T}"]])
+            ref = np.concatenate([out[f"x_{T}"], out[f"y_{T}"]])
+            rel = np.abs(xy - ref) / np.abs(ref)
+            env[f"ulp_env_max_{T}_{k}"] = np.float64(rel.max())
+            env[f"ulp_env_median_{T}_{k}"] = np.float64(np.median(rel))
+            env[f"ulp_env_pos_l2_{T}_{k}"] = np.float64(np.linalg.norm(xy - ref) / np.linalg.norm(ref))
+            uv = np.concatenate([o[f"u_{T}"], o[f"v_{T}"]])
+            uref = np.concatenate([out[f"u_{T}"], out[f"v_{T}"]])
+            env[f"ulp_env_vel_l2_{T}_{k}"] = np.float64(np.linalg.norm(uv - uref) / np.linalg.norm(uref))
+    np.savez_compressed(os.path.join(HERE, "config0_1e4.npz"), engine=eng, **keep, **env)
+    print("config0_1e4.npz", time.time() - t, {k: float(v) for k, v in env.items()})
 
 
 if __name__ == "__main__":

@@ -90,16 +90,17 @@ def test_one_step_identical_inputs():
         for k in STATE:
             assert _rel(a[k], b[k]) <= 1e-13, k
         for k in ("fcx", "fcy", "mc", "fbx", "fby", "mb", "fx_bc", "fy_bc", "m_bc", "tac", "pc",
-                  *STRESS):
+                  "tab", "pb", "ta_bc", "p_bc", "tp", "hsfa_min", "hsfw_min", *STRESS):
             scale = float(np.max(np.abs(fb[k]))) + 1e-300
             assert float(np.max(np.abs(fa[k] - fb[k]))) / scale <= 1e-13, k
         ctx.close()
 
 
 def test_stress_diagnostics_match_oracle():
-    """Stress diagnostics (stepper.f90:238-271) in a colliding, ridging box:
-    candidate pairs that separated keep contributing their last contact
-    forces; FP64 vs the oracle every step, within the FP64 tolerance."""
+    """Stress and pressure diagnostics (stepper.f90:131-146,184-199,238-271,
+    328-331; boundaries.f90:141-144) in a colliding, ridging box: candidate
+    pairs that separated keep contributing their last contact forces; FP64
+    vs the oracle every step, within the FP64 tolerance."""
     from oracle_py import OracleSim
     f = kick(squeeze(packed_box(64, 3)), 1)
     p = config0_params(f, sigmanc_crit=1e5)
@@ -114,7 +115,7 @@ def test_stress_diagnostics_match_oracle():
         ever |= set(cont)
         stale_seen += len((set(cand) - set(cont)) & ever)
         fa, fb = ctx.forces(m), O.forces()
-        for k in STRESS:
+        for k in (*STRESS, "tp", "tac", "pc", "tab", "pb", "ta_bc", "p_bc"):
             scale = float(np.max(np.abs(fb[k]))) + 1e-300
             worst = max(worst, float(np.max(np.abs(fa[k] - fb[k]))) / scale)
     ctx.close()
