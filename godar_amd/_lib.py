@@ -37,6 +37,7 @@ EXPORTS = [
     "godar_set_dense_search", "godar_set_repair", "godar_set_skin", "godar_checkpoint_size", "godar_checkpoint_save",
     "godar_member_restore",
     "godar_profile", "godar_get_profile", "godar_stage_name", "godar_get_totals",
+    "godar_get_totals_noalloc",
 ]
 NSTAGES = 12
 
@@ -109,6 +110,7 @@ def load(path: str = LIB_PATH):
         "godar_get_profile": (_i32, [_vp, _dp, _lp]),
         "godar_stage_name": (ctypes.c_char_p, [_i32]),
         "godar_get_totals": (_i32, [_vp, _lp]),
+        "godar_get_totals_noalloc": (_i32, [_vp, _lp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

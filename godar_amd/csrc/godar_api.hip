@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -40,6 +41,30 @@ __global__ void k_fill_u8(uint8_t* p, int64_t n, uint8_t v) {
 __global__ void k_fill_i32(int32_t* p, int64_t n, int32_t v) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
+}
+
+// Working memory of godar_get_totals_noalloc: the six totals it returns (pair
+// totals summed by k_totals, then the three queue totals), one copy on each
+// GPU the library runs on, held until the process ends.  Host code never
+// passes this name to a kernel or a copy: it uses the device address that
+// hipGetSymbolAddress gives for the current device (totals_scratch_of).
+constexpr int TOTALS_SCRATCH = 32;   // entries: 256 bytes, six of them used
+static __device__ __align__(256) int64_t scratch[TOTALS_SCRATCH];
+
+// The three queue totals (contact, bond, wall work): each is the sum of its
+// per-XCD segment counters.  One wavefront.
+__global__ void k_queue_totals(const int32_t* qn, int64_t* out) {
+    const int t = threadIdx.x;
+    if (t < 3) {
+        int64_t sum = 0;
+        for (int x = 0; x < XCD_COUNT; ++x) sum += qn[(t * XCD_COUNT + x) * QN_STRIDE];
+        out[t] = sum;
+    }
+}
+
+__global__ void k_zero_i64(int64_t* p, int n) {
+    const int t = threadIdx.x;
+    if (t < n) p[t] = 0;
 }
 }  // namespace godar
 
@@ -101,6 +126,8 @@ struct godar_ctx {
     int32_t built_qseg = -1;            // queue share the dense list was built with
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
     std::vector<unsigned char> gsig[2];
+    int device = 0;                     // the GPU the context lives on
+    int64_t* totals_scratch = nullptr;  // device address of that GPU's `scratch`
 };
 
 namespace {
@@ -317,6 +344,24 @@ bool find_range(const godar_ctx* c, int64_t n, int64_t cap, const std::vector<Ra
     if (cap - pos >= n) { first = pos; return true; }
     (void)c;
     return false;
+}
+
+// Device address of `scratch` on the current GPU, resolved once per GPU
+// (each GPU holds its own copy of the array at its own address).
+std::mutex g_scratch_mu;
+std::vector<int64_t*> g_scratch_addr;
+
+hipError_t totals_scratch_of(int dev, int64_t** out) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if ((int)g_scratch_addr.size() <= dev) g_scratch_addr.resize(dev + 1, nullptr);
+    if (!g_scratch_addr[dev]) {
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_addr[dev] = static_cast<int64_t*>(p);
+    }
+    *out = g_scratch_addr[dev];
+    return hipSuccess;
 }
 
 bool valid_member(const godar_ctx* c, int32_t m) {
@@ -596,6 +641,11 @@ int godar_create(int64_t max_floes, int32_t max_members, int32_t precision, goda
         }                                              \
     } while (0)
     if (e != hipSuccess) { delete c; return GODAR_ERR_HIP; }
+    if (hipGetDevice(&c->device) != hipSuccess ||
+        totals_scratch_of(c->device, &c->totals_scratch) != hipSuccess) {
+        godar_destroy(c);
+        return GODAR_ERR_HIP;
+    }
     ACHK(dalloc(&d.rec, S));
     ACHK(dalloc(&d.geo, S));
     ACHK(dalloc(&d.shel, S));
@@ -1727,6 +1777,31 @@ int godar_get_totals(godar_ctx* c, int64_t* out) {
     }
     (void)hipFree(dout);
     HIPCHK(c, e);
+    return GODAR_OK;
+}
+
+// godar_get_totals without allocating: the same kernel sums the pair totals
+// into this GPU's `scratch` (through its device address), a second kernel the
+// queue totals; both are cleared by a kernel first (no memset node), and one
+// copy brings all six back.
+int godar_get_totals_noalloc(godar_ctx* c, int64_t* out) {
+    if (!c || !out) return GODAR_ERR_ARG;
+    if (!c->totals_scratch) return fail(c, GODAR_ERR_STATE, "get_totals_noalloc: no scratch address");
+    int cur = -1;
+    HIPCHK(c, hipGetDevice(&cur));
+    if (cur != c->device)
+        return fail(c, GODAR_ERR_STATE, "get_totals_noalloc: call with the context's GPU current (" +
+                                            std::to_string(c->device) + ")");
+    int64_t* w = c->totals_scratch;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    sync_ds(c);
+    hipLaunchKernelGGL(k_zero_i64, dim3(1), dim3(64), 0, c->stream, w, GODAR_NTOTALS);
+    if (c->N_used > 0)
+        hipLaunchKernelGGL(k_totals, dim3((c->N_used + 255) / 256), dim3(256), 0, c->stream, c->d, w);
+    hipLaunchKernelGGL(k_queue_totals, dim3(1), dim3(64), 0, c->stream, c->d.qn, w + 3);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(out, w, GODAR_NTOTALS * sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return GODAR_OK;
 }
 

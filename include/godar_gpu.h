@@ -188,6 +188,15 @@ int godar_get_stats(godar_ctx *ctx, int32_t member, int64_t *stats);
  * wall work.  `totals` holds GODAR_NTOTALS entries. */
 #define GODAR_NTOTALS 6
 int godar_get_totals(godar_ctx *ctx, int64_t *totals);
+/* godar_get_totals without any device allocation (no counterpart in the
+ * reference).  For callers that cannot allocate: serving loops that poll
+ * the totals between batched steps, code that must not call hipMalloc.  Its
+ * working memory is a fixed 256-byte array in the library, one copy per GPU,
+ * held until the process ends; every call on a GPU shares it, so calls that
+ * overlap on one GPU (contexts used from several host threads) race.  Call
+ * it with the context's GPU current (else GODAR_ERR_STATE).  Same result as
+ * godar_get_totals. */
+int godar_get_totals_noalloc(godar_ctx *ctx, int64_t *totals);
 
 /* Pipeline tuning (no counterpart in the reference).  Threads per block
  * (64, 128 or 256) of the per-floe kernels of every stage, or of one stage
