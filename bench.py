@@ -40,19 +40,23 @@ STAGE_KERNEL = {"candidates": "k_cand", "contacts": "k_contact", "bonds": "k_bon
 
 def profile_traffic(stage: str, precision: str):
     """HBM bytes per launch (read + write, counter-scaled) of a stage's kernel
-    from the committed rocprofv3 summary, or None."""
+    from the committed rocprofv3 summary, or None.  The table gives MB
+    (10^6 bytes) or MiB per launch, as its header says."""
     name = STAGE_KERNEL.get(stage)
     if not name or not os.path.exists(PROFILE):
         return None
     tmpl = "<true>" if precision == "mixed" else "<false>"
+    unit = 1e6
     for line in open(PROFILE):
+        if line.startswith("| kernel") and "MiB" in line:
+            unit = float(1 << 20)
         cols = [c.strip() for c in line.strip().strip("|").split("|")]
         if len(cols) != 5 or not cols[0].startswith("`godar::"):
             continue
         kname = cols[0].strip("`")[len("godar::"):]
         if kname in (name + tmpl, name):
             try:
-                return (float(cols[3]) + float(cols[4])) * 1e6
+                return (float(cols[3]) + float(cols[4])) * unit
             except ValueError:
                 return None
     return None
@@ -76,10 +80,16 @@ def pair_totals(ctx, mids):
     return t["candidates"], t["contacts"], t["bonds"], t
 
 
-def run_batched(args):
+STAGE_INDEX = ["bin", "sort", "cells", "candidates", "contacts", "bonds", "unused", "walls",
+               "repair", "fixup", "stress", "integrate"]
+
+
+def run_batched(args, batch):
+    """The headline leg: W warm-up steps and the autotuner, then (untimed)
+    on to the middle of the configured run, K timed steps there, and K more
+    with HIP events around every stage."""
     from godar_amd.sim import Context
     from godar_amd.workloads import algorithmic_bytes
-    batch = build_batch(args.members, args.seed)
     nfloes = sum(f.n for f, _ in batch)
     ctx = Context(nfloes, len(batch), args.precision)
     mids = [ctx.add_field(p, f) for f, p in batch]
@@ -90,7 +100,16 @@ def run_batched(args):
     else:
         blocks = ctx.autotune(2)      # real steps: part of the warm-up
     ctx.sync()
-    launch = "graph replay" if ctx.graphs_on() else "one-by-one launches"
+    graphs = ctx.graphs_on()
+    launch = "graph replay" if graphs else "one-by-one launches"
+    # the timed window starts at step `window_start` of the run (the steps
+    # before it are untimed warm-up); later steps of the configured run hold
+    # more contacts than the first ones
+    t_next = ctx.tstep(mids[0])
+    if args.window_start > t_next:
+        ctx.step(args.window_start - t_next)
+    ctx.sync()
+    w0 = ctx.tstep(mids[0])
     # timed region: steps in the launch mode the autotuner chose (graph
     # replay or one-by-one launches), nothing else on the GPU
     t0 = time.perf_counter()
@@ -98,6 +117,7 @@ def run_batched(args):
     ctx.sync()
     t1 = time.perf_counter()
     dt = t1 - t0
+    ncand, ncont, nbond, tot = pair_totals(ctx, mids)
     # the same number of steps again with HIP events around every stage
     # (kernels launched one by one) for the per-kernel durations
     ctx.profile(True)
@@ -105,11 +125,51 @@ def run_batched(args):
     ctx.sync()
     stages = ctx.stage_times()
     ctx.profile(False)
-    ncand, ncont, nbond, tot = pair_totals(ctx, mids)
     ctx.close()
-    return dict(nfloes=nfloes, members=len(batch), seconds=dt, stages=stages, blocks=blocks, launch=launch,
+    return dict(nfloes=nfloes, members=len(batch), seconds=dt, stages=stages, blocks=blocks,
+                launch=launch, graphs=graphs, window=(w0, w0 + args.steps - 1),
                 ncand=ncand, ncont=ncont, nbond=nbond, totals=tot,
                 alg_bytes_step=algorithmic_bytes(nfloes, ncand, ncont, nbond))
+
+
+def run_full(batch, blocks, graphs, precision, total=5000, chunk=500):
+    """The configured run from step 1 on a fresh ensemble, with the block
+    sizes and launch mode the autotuner chose: wall time of every `chunk`
+    steps (device syncs around each) and the pairs per floe at its end."""
+    from godar_amd.sim import Context
+    nfloes = sum(f.n for f, _ in batch)
+    ctx = Context(nfloes, len(batch), precision)
+    for f, p in batch:
+        ctx.add_field(p, f)
+    for k, name in enumerate(STAGE_INDEX):
+        if name in blocks and name != "unused":
+            try:
+                ctx.set_stage_block(k, blocks[name])
+            except Exception:
+                pass
+    ctx.use_graphs(graphs)
+    ctx.sync()
+    phases = []
+    whole = 0.0
+    for a in range(1, total + 1, chunk):
+        t0 = time.perf_counter()
+        ctx.step_async(chunk)
+        ctx.sync()
+        el = time.perf_counter() - t0
+        whole += el
+        tot = ctx.totals()
+        phases.append(dict(steps=f"{a}-{a + chunk - 1}", ms_per_step=el / chunk * 1e3,
+                           floe_steps_per_s=nfloes * chunk / el,
+                           candidates_per_floe=tot["candidates"] / nfloes,
+                           contacts_per_floe=tot["contacts"] / nfloes,
+                           bonds_per_floe=tot["bonds"] / nfloes))
+        log(f"[bench] full run steps {a}-{a + chunk - 1}: {el / chunk * 1e3:.3f} ms/step, "
+            f"{tot['contacts']} contacts")
+    ctx.close()
+    return dict(steps=total, floes=nfloes, seconds=whole, ms_per_step=whole / total * 1e3,
+                floe_steps_per_s=nfloes * total / whole, launch_mode=("graph replay" if graphs
+                                                                      else "one-by-one launches"),
+                phases=phases)
 
 
 def run_serving(nreq: int, seed: int = 4, hbm_gib: float = 16.0, chunk: int = 25):
@@ -222,12 +282,15 @@ def cpu_baseline(target_seconds: float):
     from godar_amd import workloads as W
     f, p = W.config0()
     import ref_harness as rh
-    if rh.available("n1000"):
-        H = rh.RefHarness("n1000")
+    cfg = "n1000fast" if rh.available("n1000fast") else "n1000"
+    if rh.available(cfg):
+        H = rh.RefHarness(cfg)
         H.setup(p, f, bondmask=rh.dense_bondmask(f.n, p.bond_seed, p.bond_frac), logcap=1)
         step = H.step
         kind = "reference"
-        what = ("GODAR physics routines compiled from the reference sources (amdflang -O2), "
+        flags = ("-O3 -funroll-loops -ffast-math, the reference's Release flags"
+                 if cfg == "n1000fast" else "-O2 -ffp-contract=off")
+        what = (f"GODAR physics routines compiled from the reference sources (amdflang {flags}), "
                 "stepper control flow restated, cell-list radius search")
     else:
         from oracle_py import OracleSim
@@ -245,6 +308,24 @@ def cpu_baseline(target_seconds: float):
             break
     return dict(value=f.n * t / el, unit="floe-steps/s", cores=1, kind=kind, steps=t, seconds=el,
                 sample=f"config 0 (1,000 floes, 100 km box), first {t} steps, {el:.1f} s; {what}")
+
+
+def host_info() -> dict:
+    """What the host offers: logical CPUs, the CPUs this process may run on,
+    and the processor model."""
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except AttributeError:
+        allowed = os.cpu_count() or 1
+    return dict(logical_cpus=os.cpu_count(), cpus_allowed=allowed, model=model)
 
 
 def _cpu_worker(target_seconds):
@@ -278,6 +359,10 @@ def main():
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
     ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--window-start", type=int, default=2501,
+                    help="step of the configured 5,000-step run at which the timed window starts")
+    ap.add_argument("--full-steps", type=int, default=5000,
+                    help="steps of the full-run leg (0: skip)")
     ap.add_argument("--quick", action="store_true", help="batched run only (for profilers)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--serve-requests", type=int, default=100,
@@ -309,11 +394,15 @@ def main():
 
     cb = cb1 = None
     if not args.quick:
-        # CPU baselines first: the worker processes are forked before any GPU use
-        workers = max(1, min(16, os.cpu_count() or 1))
+        # CPU baselines first: the worker processes are forked before any GPU use.
+        # A GPU box allots 16 host cores to its GPU: one process per core, at
+        # most 16, whatever the machine has in total.
+        host = host_info()
+        workers = max(1, min(16, host["cpus_allowed"]))
         cb1 = cpu_baseline(args.cpu_seconds)
         cb = cpu_baseline_all_cores(args.cpu_seconds, workers) if workers > 1 else cb1
-    B = run_batched(args)
+    batch = build_batch(args.members, args.seed)
+    B = run_batched(args, batch)
     value = B["nfloes"] * args.steps / B["seconds"]
     ms_step = B["seconds"] / args.steps * 1e3
     # dominant stage by measured time; roofline on its algorithmic bytes
@@ -339,7 +428,7 @@ def main():
         "shelter": nf * (7 * 8 + 2 * 8 + 2 * 4 + 1) + nc * (2 * 8 + 4 + 4),
         "contacts": qc * (11 * 8 + 5 * 8 + 4) + nk * (2 * 8 + 32),
         "bonds": qb * (11 * 8 + 5 * 8 + 4) + nb * (2 * 8 + 2 * 88),
-        "candidates": nf * (3 * 8 + 8) + 12.0 * 2 * nc,
+        "candidates": nf * (4 * 8 + 8) + 24.0 * nc,
         "integrate": nf * (12 * 8 + 6 * 8 + 2 * 8) + qc * 5 * 8 + nb * 2 * 10 * 8,
         "sort": nf * 8.0 * 2 * 3,
     }
@@ -363,6 +452,8 @@ def main():
             "workload": (f"config 3: {B['members']} independent floe fields, "
                          f"{nf} floes (log-uniform 500-8000 per field), {args.precision} precision, "
                          "one step pipeline"),
+            "window": (f"steps {B['window'][0]}-{B['window'][1]} of the configured 5,000-step run "
+                       "(earlier steps: untimed warm-up and autotuning)"),
             "members": B["members"], "floes": nf, "precision": args.precision,
             "candidates_per_floe": kbar, "contacts_per_floe": kc, "bonds_per_floe": kb,
         },
@@ -380,6 +471,9 @@ def main():
         "block_sizes": B["blocks"],
         "launch_mode": B["launch"],
     }
+    if not args.quick and args.full_steps > 0:
+        out["full_run"] = run_full(batch, B["blocks"], B["graphs"], args.precision,
+                                   total=args.full_steps)
     if not args.quick:
         p0, ms0 = run_plain("config0", 500, 20)
         p1, ms1 = run_plain("config1", 20, 3)
@@ -390,7 +484,9 @@ def main():
         out["pos_err_1e4"] = {"mixed": pos_error_1e4("mixed"), "fp64": pos_error_1e4("fp64")}
         if args.serve_requests > 0:
             out["serving"] = run_serving(args.serve_requests)
-        out["cpu_baseline"] = cb
+        out["cpu_baseline"] = dict(cb, host=host, share=(
+            f"{workers} processes: the host cores a GPU box allots to its one GPU (16) "
+            f"of {host['cpus_allowed']} this process may use ({host['logical_cpus']} logical CPUs)"))
         out["cpu_baseline_one_core"] = {k: cb1[k] for k in ("value", "unit", "cores", "kind", "sample")}
         out["vs_cpu_baseline"] = value / cb["value"]
         out["vs_cpu_baseline_one_core"] = value / cb1["value"]

@@ -444,13 +444,13 @@ int launch_step(godar_ctx* c) {
     int64_t dense_n = 0;
     for (int k = 0; k < c->Mcap; ++k)
         if (c->members[k].active && c->dparams[k].dense) { any_dense = true; dense_n += c->members[k].n; }
-    // The dense search runs one wavefront per block, each block over a fixed
-    // share of the dense floes; the work per floe is uneven, and blocks of
-    // about 4 floes let the dispatcher balance it (100 served requests,
-    // 1.45 M dense floes: 1,024 wavefronts per XCD 4.44 ms, 40,960 3.40 ms).
-    // The superset builds keep 1,024: they return at once on most steps, and
-    // dispatching that many more wavefronts would cost more than they save.
-    const dim3 dense_grid(XCD_COUNT * (int)std::min<int64_t>(65536, std::max<int64_t>(1024, (dense_n + 31) / 32)));
+    // The dense search runs one wavefront per block, each block over
+    // DENSE_CHUNK entries of the dense queue; the work per floe is uneven,
+    // and small shares let the dispatcher balance it.  The superset builds
+    // keep 1,024 wavefronts per XCD: they return at once on most steps, and
+    // dispatching many more would cost more than they save.
+    const dim3 dense_grid(XCD_COUNT * (int)std::min<int64_t>(
+        65536, std::max<int64_t>(64, (dense_n + XCD_COUNT * DENSE_CHUNK - 1) / (XCD_COUNT * DENSE_CHUNK))));
     {
         StageTimer t(c, 2);
         hipLaunchKernelGGL(k_ranges, grid_for(N, B[2], false), dim3(B[2]), 0, s, d);
@@ -471,7 +471,8 @@ int launch_step(godar_ctx* c) {
         hipLaunchKernelGGL(mixed ? k_cand<true> : k_cand<false>, grid_for(N, B[3], true),
                            dim3(B[3]), (B[3] / 64) * CAND_LDS_PER_WAVE, s, d);
         if (any_dense) {
-            // one wavefront per dense floe, a contiguous share of the list per XCD
+            // groups of dense floes per wavefront, a contiguous share of the
+            // list per XCD
             hipLaunchKernelGGL(mixed ? k_cand_dense<true> : k_cand_dense<false>,
                                dense_grid, dim3(64), 0, s, d);
             hipLaunchKernelGGL(mixed ? k_cand_dense_over<true> : k_cand_dense_over<false>,

@@ -1016,35 +1016,26 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_cand(DS S) {
     enqueue(S, qw, 2, k);
 }
 
-// Dense members (many floes per cell): one wavefront per floe.  The lanes
-// test 64 cell entries at a time, survivors are compacted into LDS with
-// ballots, then sorted there (bitonic network over the padded power of two):
-// outer candidates by (distance, id), inner by id -- the orders of the thread
-// search.  Sheltering minima are reduced over the lanes (a minimum is exact in
-// any order).  A floe with more than DENSE_CAP candidates on a side falls
-// back to the thread search (k_cand_dense_over: its registers, inlined here,
-// would cost the kernel a wavefront per SIMD).  Blocks are one wavefront; the
-// queue of dense floes is split into one contiguous share per XCD.
-constexpr int DENSE_CAP = 256;
+// Dense members (many floes per cell): a wavefront searches a group of
+// consecutive floes of the dense queue together.  Their supersets lie one
+// after the other in the pool; the lanes test them 64 entries a round (each
+// lane finds its entry's floe among the group's by a binary search over
+// their starts), evaluate the sheltering and classification terms of every
+// pair they find at once (minima and flags go to the floe's slot in LDS by
+// atomics: a minimum and an "any" do not depend on the order of their
+// terms), and list the pairs in LDS, where each floe's are contiguous.  Each
+// pair's slot in its floe's list is then its rank there -- outer candidates
+// by (distance, id), inner by id, the orders of the thread search -- counted
+// over the floe's pairs, and the lists are written straight to their slots.
+// A group holds at most GRP_MAX floes and GRP_SUP superset entries; a floe
+// whose superset alone is larger goes to the thread search
+// (k_cand_dense_over).  Blocks are one wavefront, each over DENSE_CHUNK
+// entries of the dense queue (the host sizes the grid by it), the queue split
+// into one contiguous share per XCD; the work per floe is uneven, and small
+// shares let the dispatcher balance it.
 constexpr int DENSE_OVER = -1;   // ocnt of a dense floe left to the thread search
-
-__device__ __forceinline__ void bitonic_outer(uint64_t* kd, int64_t* ke, int n, int lane) {
-    for (int size = 2; size <= n; size <<= 1) {
-        for (int stride = size >> 1; stride > 0; stride >>= 1) {
-            for (int i = lane; i < n; i += 64) {
-                const int pj = i ^ stride;
-                if (pj > i) {
-                    const bool up = (i & size) == 0;
-                    const uint64_t d1 = kd[i], d2 = kd[pj];
-                    const int64_t e1 = ke[i], e2 = ke[pj];
-                    const bool gt = d1 > d2 || (d1 == d2 && e1 > e2);
-                    if (gt == up) { kd[i] = d2; kd[pj] = d1; ke[i] = e2; ke[pj] = e1; }
-                }
-            }
-            __syncthreads();
-        }
-    }
-}
+constexpr int GRP_MAX = 16;      // floes per group
+constexpr int GRP_SUP = 448;     // superset entries per group (the LDS pair list)
 
 __device__ __forceinline__ void bitonic_inner(int64_t* ke, int n, int lane) {
     for (int size = 2; size <= n; size <<= 1) {
@@ -1073,28 +1064,6 @@ __device__ __forceinline__ int pow2_at_least(int n) {
     int p = 1;
     while (p < n) p <<= 1;
     return p;
-}
-
-// Bitonic sort of one element per lane across the wavefront, ascending in
-// lane order, of two lists at once: outer keys (distance bits, entry) and
-// inner keys (entry).
-__device__ __forceinline__ void wave_sort_lists(uint64_t& od, int64_t& oe, int64_t& ie, int lane) {
-#pragma unroll
-    for (int size = 2; size <= 64; size <<= 1) {
-#pragma unroll
-        for (int stride = size >> 1; stride > 0; stride >>= 1) {
-            const uint64_t pd = (uint64_t)__shfl_xor((long long)od, stride);
-            const int64_t pe = __shfl_xor((long long)oe, stride);
-            const int64_t pi = __shfl_xor((long long)ie, stride);
-            // the lower lane of a pair keeps the smaller element in ascending
-            // blocks, the larger in descending ones
-            const bool keep_min = ((lane & stride) == 0) == ((lane & size) == 0);
-            const bool o_less = od < pd || (od == pd && oe < pe);
-            if (keep_min != o_less) { od = pd; oe = pe; }
-            const bool i_less = ie < pi;
-            if (keep_min != i_less) ie = pi;
-        }
-    }
 }
 
 // Superset build of the dense members' floes (queue 3), one wavefront per
@@ -1153,13 +1122,46 @@ __global__ __launch_bounds__(64) void k_sup_dense(DS S) {
     }
 }
 
-// mixed precision fits 5 wavefronts per SIMD without spilling, FP64 does not
+// One floe of a dense-search group in LDS.
+struct GroupFloe {
+    double x, y, r, rt;          // position, radius, rtree of its member
+    double hfa, hfw, wx, wy;     // its sheltering record (drafts, wind at the floe)
+    double uw, vw;               // current of its member
+    int64_t off;                 // first entry of its superset in the pool
+    int32_t k, id, start, mem;   // position, identity, first entry in the group's sequence, member
+    int32_t flags;               // bit0 sheltering on, bit1 bonds form in this step
+    int32_t pad_;
+};
+// What the lanes gather for a floe of the group (LDS atomics).
+struct GroupAcc {
+    uint64_t mina, minw;         // ordered keys of the sheltering minima
+    int32_t no, ni;              // outer and inner pairs found
+    uint32_t fl;                 // bit0 a partner overlaps, bit1 a bond forms
+    int32_t hb;                  // first of its pairs in the group's list
+};
+struct GroupLds {
+    uint64_t key[GRP_SUP];       // outer: distance bits (d >= 0); inner: entry | 1 << 63
+    int64_t ent[GRP_SUP];        // (id << 32 | pos)
+    uint8_t own[GRP_SUP];        // floe of the group
+    GroupFloe fl[GRP_MAX];
+    GroupAcc acc[GRP_MAX];
+};
+static_assert(GRP_MAX <= 64 && GRP_SUP < (1 << 16), "group shape");
+
+// Order-preserving map of doubles to unsigned keys (a < b iff key(a) < key(b)
+// for non-NaN values), for minima taken with integer atomics.
+__device__ __forceinline__ uint64_t ord_key(double v) {
+    const uint64_t u = (uint64_t)__double_as_longlong(v);
+    return (u >> 63) ? ~u : (u | (1ull << 63));
+}
+__device__ __forceinline__ double ord_val(uint64_t k) {
+    return __longlong_as_double((long long)((k >> 63) ? (k & ~(1ull << 63)) : ~k));
+}
+
 template <bool MIXED>
-__global__ __launch_bounds__(64, MIXED ? 5 : 1) void k_cand_dense(DS S) {
+__global__ __launch_bounds__(64) void k_cand_dense(DS S) {
     using T = typename Prec<MIXED>::T;
-    __shared__ uint64_t sd[DENSE_CAP];
-    __shared__ int64_t se[DENSE_CAP];
-    __shared__ int64_t si[DENSE_CAP];
+    __shared__ GroupLds A;
     if (*S.err) return;
     const int lane = threadIdx.x;
     const QueueView qv = queue_view(S, 3);
@@ -1167,185 +1169,216 @@ __global__ __launch_bounds__(64, MIXED ? 5 : 1) void k_cand_dense(DS S) {
     const int nb = (int)(gridDim.x / XCD_COUNT);
     const int xcd = (int)(blockIdx.x % XCD_COUNT), bi = (int)(blockIdx.x / XCD_COUNT);
     const int per = (nq + XCD_COUNT - 1) / XCD_COUNT;
-    const int t1 = min(nq, xcd * per + per);
+    const int s1 = min(nq, xcd * per + per);
+    const int chunk = (per + nb - 1) / nb;
+    int t = xcd * per + bi * chunk;
+    const int t_end = min(s1, t + chunk);
     const uint64_t lt = lane ? ((~0ull) >> (64 - lane)) : 0ull;
     const int64_t SS = S.S;
-    // the next floe's list entry, row and superset extent are loaded a floe
-    // ahead
-    int t = xcd * per + bi;
-    int k_next = t < t1 ? queue_at(S, 3, qv, t) : 0;
-    GeoRec me_next;
-    int L_next = 0;
-    int64_t off_next = 0;
-    if (t < t1) { me_next = S.geo[k_next]; L_next = S.scnt[k_next]; off_next = S.soff[k_next]; }
-    for (; t < t1; t += nb) {
-        const int k = k_next;
-        const GeoRec me = me_next;
-        const int L = L_next;
-        const int64_t* sup = S.sup + off_next;
-        if (t + nb < t1) k_next = queue_at(S, 3, qv, t + nb);
-        const int m = me.member;
-        if (m < 0) {                // member past its stop step (k_cand cleared its counts)
-            if (t + nb < t1) {
-                me_next = S.geo[k_next]; L_next = S.scnt[k_next]; off_next = S.soff[k_next];
-            }
+    const uint64_t one_key = ord_key(1.0);
+    while (t < t_end) {
+        // 1. the group: the leading queue entries whose supersets fit
+        const bool valid = lane < GRP_MAX && t + lane < t_end;
+        int k = 0, L = 0, mem = -1;
+        GeoRec me;
+        me.x = me.y = me.r = 0.0;
+        me.id = 0;
+        me.member = -1;
+        if (valid) {
+            k = queue_at(S, 3, qv, t + lane);
+            me = S.geo[k];
+            mem = me.member;            // -1: member past its stop step (nothing to do)
+            if (mem >= 0) L = S.scnt[k];
+        }
+        int incl = L;
+#pragma unroll
+        for (int off = 1; off < GRP_MAX; off <<= 1) {
+            const int v = __shfl_up(incl, off);
+            if (lane >= off) incl += v;
+        }
+        const uint64_t fm = __ballot(valid && L <= GRP_SUP && incl <= GRP_SUP);
+        const int G = __builtin_ctzll(~fm);
+        if (G == 0) {
+            // the first floe's superset alone outgrows the list: thread search
+            if (lane == 0 && mem >= 0) S.ocnt[k] = DENSE_OVER;
+            t += 1;
             continue;
         }
-        const DevParams& P = S.params[m];
-        const double rad = me.r + P.rtree;
-        const int f = me.id;
-        const int first = S.members[m].first;
-        const bool bond_step = S.members[m].tstep == 1 && P.restart != 1 && P.cohesion;
-        const ShelRec msh = S.shel[k];
-        const T uwi = T(P.uw), vwi = T(P.vw);
-        T sha = T(1), shw = T(1);
-        bool lqc = false, lqb = false;
-        // the floe's own pair-state counts, read early (the search's stores
-        // would otherwise hold these loads back)
-        const int fs = S.cval[k];
-        const bool has_hist = S.hcnt_old[fs] > 0, has_bonds = S.bcnt_old[fs] > 0;
-        // sheltering and classification terms of one candidate (lane-
-        // parallel; a minimum and an "any" do not depend on the order), one
-        // code path for both sides: outer, pair (f, j) with f the outer floe
-        // i of the event, f takes hsf*(j,f) with -cosa, -sina of f -> j and
-        // its own wind; inner, pair (j, f) with f the inner floe, cosa/sina
-        // from j to f and j's wind
-        auto cand_math = [&](const GeoRec& o, int q, double d, bool outer) {
-            const double deltan = outer ? me.r + o.r - d : o.r + me.r - d;
-            if (deltan > 0) lqc = true;
-            if (outer && bond_step && deltan >= -P.bond_lim &&
-                bond_keep(f - first, o.id - first, P.bond_seed, P.bond_thr24))
-                lqb = true;
-            if (P.shelter) {
-                const double nx = outer ? o.x - me.x : me.x - o.x;
-                const double ny = outer ? o.y - me.y : me.y - o.y;
-                const double cosa = nx / d, sina = ny / d;
-                const ShelRec oh = S.shel[q];
-                const T dn = T(deltan), c0 = T(cosa), s0 = T(sina);
-                const T c = outer ? -c0 : c0, sn = outer ? -s0 : s0;
-                const T ux = outer ? T(msh.wx) : T(oh.wx), uy = outer ? T(msh.wy) : T(oh.wy);
-                sha = shelter_min(sha, T(oh.hfa), T(msh.hfa), dn, c, sn, ux, uy);
-                shw = shelter_min(shw, T(oh.hfw), T(msh.hfw), dn, c, sn, uwi, vwi);
+        const int start = incl - L;
+        const int total = __shfl(incl, G - 1);
+        int fs = 0;
+        bool has_hist = false, has_bonds = false;
+        if (lane < G) {
+            GroupFloe g;
+            g.k = k; g.mem = mem; g.start = start; g.id = me.id;
+            g.x = me.x; g.y = me.y; g.r = me.r;
+            g.rt = g.hfa = g.hfw = g.wx = g.wy = g.uw = g.vw = 0.0;
+            g.off = 0;
+            g.flags = 0;
+            g.pad_ = 0;
+            if (mem >= 0) {
+                const DevParams& P = S.params[mem];
+                const ShelRec sh = S.shel[k];
+                fs = S.cval[k];
+                g.rt = P.rtree;
+                g.hfa = sh.hfa; g.hfw = sh.hfw; g.wx = sh.wx; g.wy = sh.wy;
+                g.uw = P.uw; g.vw = P.vw;
+                g.off = S.soff[k];
+                g.flags = (P.shelter ? 1 : 0) |
+                          ((S.members[mem].tstep == 1 && P.restart != 1 && P.cohesion) ? 2 : 0);
+                has_hist = S.hcnt_old[fs] > 0;
+                has_bonds = S.bcnt_old[fs] > 0;
             }
-        };
-        int no = 0, ni = 0;
-        uint32_t over = 0;
-        if (L <= 64) {
-            // one entry per lane, which also takes the terms of a candidate
-            // it finds (the partner's row is at hand); both lists are then
-            // sorted across the wavefront in registers (sentinels above every
-            // entry fill the other lanes)
-            bool vout = false, vin = false;
-            int64_t e = 0;
+            A.fl[lane] = g;
+            GroupAcc z;
+            z.mina = z.minw = one_key;
+            z.no = z.ni = 0;
+            z.fl = 0;
+            z.hb = 0;
+            A.acc[lane] = z;
+        }
+        const int my_start = lane < G ? start : INT_MAX;
+        wave_lds_sync();
+        // 2. the group's supersets, 64 entries a round: pair test, terms, list
+        int atotal = 0;
+        for (int c0 = 0; c0 < total; c0 += 64) {
+            const int idx = c0 + lane;
+            // the entry's floe: the last one whose superset starts at or
+            // before it (floes without entries share the next one's start)
+            int own = 0;
+#pragma unroll
+            for (int step = GRP_MAX / 2; step > 0; step >>= 1)
+                if (__shfl(my_start, own + step) <= idx) own += step;
+            bool hit = false, outer = false;
             double d = 0.0;
-            if (lane < L) {
-                const int q = ent_pos(sup[lane]);
+            int64_t e = 0;
+            if (idx < total) {
+                const GroupFloe& g = A.fl[own];
+                const int q = ent_pos(S.sup[g.off + (idx - g.start)]);
                 const GeoRec o = S.geo[q];
-                const double dx = me.x - o.x, dy = me.y - o.y;
+                const double dx = g.x - o.x, dy = g.y - o.y;
                 const double dist2 = dx * dx + dy * dy;
-                e = ent(o.id, q);
-                const double thr = o.id > f ? rad : o.r + P.rtree;
+                outer = o.id > g.id;
+                const double thr = outer ? g.r + g.rt : o.r + g.rt;
                 // far beyond the radius: no square root needed
                 if (dist2 <= thr * thr * (1.0 + 1e-12)) {
                     d = sqrt(dist2);
-                    if (o.id > f) vout = d <= thr;
-                    else vin = d <= thr;
+                    hit = d <= thr;
                 }
-                if (vout || vin) cand_math(o, q, d, vout);
+                e = ent(o.id, q);
+                if (hit) {
+                    // outer: pair (f, j) with f the outer floe i of the event;
+                    // inner: pair (j, f) with f the inner floe j
+                    const double deltan = outer ? g.r + o.r - d : o.r + g.r - d;
+                    uint32_t fl = deltan > 0 ? 1u : 0u;
+                    if (outer && (g.flags & 2)) {
+                        const DevParams& P = S.params[g.mem];
+                        const int first = S.members[g.mem].first;
+                        if (deltan >= -P.bond_lim &&
+                            bond_keep(g.id - first, o.id - first, P.bond_seed, P.bond_thr24))
+                            fl |= 2u;
+                    }
+                    if (fl) atomicOr(&A.acc[own].fl, fl);
+                    if (g.flags & 1) {
+                        // f takes hsf*(j,f): outer with -cosa, -sina of f -> j
+                        // and its own wind; inner with cosa/sina from j to f
+                        // and j's wind
+                        const double nx = outer ? o.x - g.x : g.x - o.x;
+                        const double ny = outer ? o.y - g.y : g.y - o.y;
+                        const double cosa = nx / d, sina = ny / d;
+                        const ShelRec oh = S.shel[q];
+                        const T dn = T(deltan), c0t = T(cosa), s0t = T(sina);
+                        const T c = outer ? -c0t : c0t, sn = outer ? -s0t : s0t;
+                        const T ux = outer ? T(g.wx) : T(oh.wx), uy = outer ? T(g.wy) : T(oh.wy);
+                        const T ta = shelter_min(T(1), T(oh.hfa), T(g.hfa), dn, c, sn, ux, uy);
+                        const T tw = shelter_min(T(1), T(oh.hfw), T(g.hfw), dn, c, sn, T(g.uw), T(g.vw));
+                        if (ta < T(1)) atomicMin((unsigned long long*)&A.acc[own].mina,
+                                                 (unsigned long long)ord_key((double)ta));
+                        if (tw < T(1)) atomicMin((unsigned long long*)&A.acc[own].minw,
+                                                 (unsigned long long)ord_key((double)tw));
+                    }
+                    atomicAdd(outer ? &A.acc[own].no : &A.acc[own].ni, 1);
+                }
             }
-            no = (int)__popcll(__ballot(vout));
-            ni = (int)__popcll(__ballot(vin));
-            uint64_t okey = vout ? (uint64_t)__double_as_longlong(d) : ~0ull;   // d >= 0: bits order as values
-            int64_t oe = vout ? e : INT64_MAX;
-            int64_t ie = vin ? e : INT64_MAX;
-            wave_sort_lists(okey, oe, ie, lane);
+            const uint64_t hm = __ballot(hit);
+            if (hit) {
+                const int at = atotal + (int)__popcll(hm & lt);
+                A.key[at] = outer ? (uint64_t)__double_as_longlong(d) : ((uint64_t)e | (1ull << 63));
+                A.ent[at] = e;
+                A.own[at] = (uint8_t)own;
+            }
+            atotal += (int)__popcll(hm);
+        }
+        wave_lds_sync();
+        // 3. each floe's pairs start where the previous floe's end
+        GroupAcc a;
+        a.no = a.ni = 0;
+        a.fl = 0;
+        a.mina = a.minw = one_key;
+        if (lane < G) a = A.acc[lane];
+        const int cnt = a.no + a.ni;
+        int cincl = cnt;
+#pragma unroll
+        for (int off = 1; off < GRP_MAX; off <<= 1) {
+            const int v = __shfl_up(cincl, off);
+            if (lane >= off) cincl += v;
+        }
+        if (lane < G) A.acc[lane].hb = cincl - cnt;
+        wave_lds_sync();
+        // 4. every pair to its rank in its floe's list
+        for (int h = lane; h < atotal; h += 64) {
+            const int g = A.own[h];
+            const uint64_t kh = A.key[h];
+            const int64_t eh = A.ent[h];
+            const int b0 = A.acc[g].hb, b1 = b0 + A.acc[g].no + A.acc[g].ni;
+            int rank = 0;
+            for (int j = b0; j < b1; ++j) {
+                const uint64_t kj = A.key[j];
+                const int64_t ej = A.ent[j];
+                rank += (((kj ^ kh) >> 63) == 0 && (kj < kh || (kj == kh && ej < eh))) ? 1 : 0;
+            }
+            const int kk = A.fl[g].k;
+            if (kh >> 63) {
+                if (rank < S.CI) {
+                    S.ik[rank * SS + kk] = eh;
+                    S.ibond[rank * SS + kk] = 0;
+                }
+            } else if (rank < S.CO) {
+                S.oj[rank * SS + kk] = eh;
+                S.evf[rank * SS + kk] = (A.fl[g].flags & 1) ? EV_SHELTER : 0u;
+                if (S.record_forces) {
+                    S.ssc[rank * SS + kk] = 0.0;
+                    S.ssb[rank * SS + kk] = 0.0;
+                }
+            }
+        }
+        // 5. each floe's counts, minima, per-step resets and queues
+        if (lane < G && mem >= 0) {
+            int no = a.no, ni = a.ni;
+            uint32_t over = 0;
             if (no > S.CO) { over |= ERR_OUTER_CAP; no = S.CO; }
             if (ni > S.CI) { over |= ERR_INNER_CAP; ni = S.CI; }
-            // (distances are only read back by the thread search's tail)
-            if (lane < no) S.oj[lane * SS + k] = oe;
-            if (lane < ni) S.ik[lane * SS + k] = ie;
-        } else {
-            // 64 entries at a time, survivors compacted into LDS and sorted there
-            for (int base = 0; base < L; base += 64) {
-                const int v = base + lane;
-                bool vout = false, vin = false;
-                int oid = 0, q = 0;
-                double d = 0.0;
-                if (v < L) {
-                    q = ent_pos(sup[v]);
-                    const GeoRec o = S.geo[q];
-                    const double dx = me.x - o.x, dy = me.y - o.y;
-                    const double dist2 = dx * dx + dy * dy;
-                    oid = o.id;
-                    const double thr = o.id > f ? rad : o.r + P.rtree;
-                    if (dist2 <= thr * thr * (1.0 + 1e-12)) {
-                        d = sqrt(dist2);
-                        if (o.id > f) vout = d <= thr;
-                        else vin = d <= thr;
-                    }
-                    if (vout || vin) cand_math(o, q, d, vout);
-                }
-                const uint64_t mo = __ballot(vout), mi = __ballot(vin);
-                if (vout) {
-                    const int slot = no + (int)__popcll(mo & lt);
-                    if (slot < DENSE_CAP) {
-                        sd[slot] = (uint64_t)__double_as_longlong(d);
-                        se[slot] = ent(oid, q);
-                    }
-                }
-                if (vin) {
-                    const int slot = ni + (int)__popcll(mi & lt);
-                    if (slot < DENSE_CAP) si[slot] = ent(oid, q);
-                }
-                no += (int)__popcll(mo);
-                ni += (int)__popcll(mi);
-            }
-            __syncthreads();
-            if (no > DENSE_CAP || ni > DENSE_CAP) {
-                // rare: more candidates than LDS holds; k_cand_dense_over
-                // runs the thread search for this floe
-                if (lane == 0) S.ocnt[k] = DENSE_OVER;
-                __syncthreads();
-                if (t + nb < t1) {
-                    me_next = S.geo[k_next]; L_next = S.scnt[k_next]; off_next = S.soff[k_next];
-                }
-                continue;
-            }
-            if (no > S.CO) { over |= ERR_OUTER_CAP; no = S.CO; }
-            if (ni > S.CI) { over |= ERR_INNER_CAP; ni = S.CI; }
-            // sort: outer by (distance, id), inner by id
-            const int po = pow2_at_least(no), pi = pow2_at_least(ni);
-            for (int i = no + lane; i < po; i += 64) { sd[i] = ~0ull; se[i] = INT64_MAX; }
-            for (int i = ni + lane; i < pi; i += 64) si[i] = INT64_MAX;
-            __syncthreads();
-            bitonic_outer(sd, se, po, lane);
-            bitonic_inner(si, pi, lane);
-            for (int s2 = lane; s2 < no; s2 += 64) {
-                S.od[s2 * SS + k] = __longlong_as_double((long long)sd[s2]);
-                S.oj[s2 * SS + k] = se[s2];
-            }
-            for (int s2 = lane; s2 < ni; s2 += 64) S.ik[s2 * SS + k] = si[s2];
-        }
-        if (over && lane == 0) set_err(S, over);
-        sha = wave_min(sha);
-        shw = wave_min(shw);
-        const bool qc = __ballot(lqc) != 0 || has_hist;
-        const bool qb = __ballot(lqb) != 0 || has_bonds;
-        if (t + nb < t1) {
-            me_next = S.geo[k_next]; L_next = S.scnt[k_next]; off_next = S.soff[k_next];
-        }
-        const bool qw = me.y - me.r < 0 || me.y + me.r > P.ny || me.x - me.r < 0 || me.x + me.r > P.nx;
-        cand_finish<MIXED>(S, k, fs, P, no, ni, double(sha), double(shw), qc, qb, lane, 64);
-        if (lane == 0) {
+            if (over) set_err(S, over);
+            const DevParams& P = S.params[mem];
+            const bool qc = (a.fl & 1u) || has_hist;
+            const bool qb = (a.fl & 2u) || has_bonds;
+            const bool qw = me.y - me.r < 0 || me.y + me.r > P.ny || me.x - me.r < 0 || me.x + me.r > P.nx;
+            S.ocnt[k] = no;
+            S.icnt[k] = ni;
+            S.acc[A_SHA * SS + k] = ord_val(a.mina);
+            S.acc[A_SHW * SS + k] = ord_val(a.minw);
+            if (!qc) S.hcnt_new[fs] = 0;
+            if (!qb) S.bcnt_new[fs] = 0;
+            S.wflag[k] = 0;
+            S.bgat[k] = 0;
+            S.cls[k] = qc ? CLS_CONTACT : 0;
             if (qc) enqueue_one(S, 0, k);
             if (qb) enqueue_one(S, 1, k);
             if (qw) enqueue_one(S, 2, k);
         }
-        // The block is this one wavefront, whose LDS accesses complete in
-        // order: the next floe may reuse the LDS lists without a barrier
-        // (which would also wait for this floe's stores to complete).  Only
-        // the compiler is kept from moving accesses across.
-        __builtin_amdgcn_wave_barrier();
+        t += G;
+        // the next group rewrites the LDS lists: the reads above come first
+        wave_lds_sync();
     }
 }
 

@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 #include "godar_types.h"
+#include "cr_math.h"
 
 namespace godar {
 
@@ -23,9 +24,11 @@ __device__ __forceinline__ float dmin(float a, float b) { return fminf(a, b); }
 __device__ __forceinline__ float dmax(float a, float b) { return fmaxf(a, b); }
 __device__ __forceinline__ double gsqrt(double a) { return sqrt(a); }
 __device__ __forceinline__ float gsqrt(float a) { return sqrtf(a); }
-__device__ __forceinline__ double gexp(double a) { return exp(a); }
+// FP64: correctly rounded (cr_math.h), so that they agree with the host libm
+// the reference calls wherever that libm rounds correctly
+__device__ __forceinline__ double gexp(double a) { return crm::cr_exp(a); }
 __device__ __forceinline__ float gexp(float a) { return expf(a); }
-__device__ __forceinline__ double glog(double a) { return log(a); }
+__device__ __forceinline__ double glog(double a) { return crm::cr_log(a); }
 __device__ __forceinline__ float glog(float a) { return logf(a); }
 __device__ __forceinline__ double gabs(double a) { return fabs(a); }
 __device__ __forceinline__ float gabs(float a) { return fabsf(a); }
@@ -187,8 +190,8 @@ __device__ __forceinline__ void contact_part(const DevParams& P, Floe& fi, Floe&
             double ri = fi.r, rj = fj.r;
             double dij = (dist * dist - rj * rj + ri * ri) / (2.0 * dist);
             double dji = (dist * dist - ri * ri + rj * rj) / (2.0 * dist);
-            double area = ri * ri * acos(dij / ri) - dij * delt_ridge / 2.0
-                        + rj * rj * acos(dji / rj) - dji * delt_ridge / 2.0;
+            double area = ri * ri * crm::cr_acos(dij / ri) - dij * delt_ridge / 2.0
+                        + rj * rj * crm::cr_acos(dji / rj) - dji * delt_ridge / 2.0;
             double vol = fmax(area * hmin64 / 2.0, 0.0);
             // the thinner floe is reshaped; its six properties are selected
             // field by field (a pointer to either floe, or a select of whole
@@ -440,7 +443,7 @@ __device__ __forceinline__ void contact_bc(const DevParams& P, Floe& f, int dir1
             // update_shape_bc (FP64)
             double drb = delt_ridge_bc;
             double hh = f.h;
-            double area = ri * ri * asin(drb / 2.0 / ri) - drb * (ri - deltan_bc) / 2.0;
+            double area = ri * ri * crm::cr_asin(drb / 2.0 / ri) - drb * (ri - deltan_bc) / 2.0;
             double vol = fmax(area * hh, 0.0);
             double dh = vol / (P.pi * (ri * ri));
             f.r = ri * sqrt(hh / (hh + dh));
