@@ -431,8 +431,18 @@ def main():
         "candidates": nf * (4 * 8 + 8) + 24.0 * nc,
         "integrate": nf * (12 * 8 + 6 * 8 + 2 * 8) + qc * 5 * 8 + nb * 2 * 10 * 8,
         "sort": nf * 8.0 * 2 * 3,
+        #  repair: per event re-evaluated, both floes' current properties and
+        #    rows (2 x (48 + 32 + 56) B) and the pair state (40 B history, 88 B
+        #    bond record) read, its 264-byte result written
+        "repair": B["totals"]["repaired_events"] * (2 * (48 + 32 + 56) + 40 + 88 + 264),
     }
-    alg = per_stage_bytes.get(dom, B["alg_bytes_step"])
+    # the roofline is the dominant kernel's; a stage without a byte model
+    # (fix-up, walls, build) passes the place to the next one
+    modelled = {k: v for k, v in stages.items() if k in per_stage_bytes}
+    top = dom
+    dom = max(modelled, key=lambda k: modelled[k][0] / modelled[k][1])
+    dom_ms = stages[dom][0] / stages[dom][1]
+    alg = per_stage_bytes[dom]
     achieved = alg / (dom_ms * 1e-3) / 1e9
     step_gbs = B["alg_bytes_step"] / (ms_step * 1e-3) / 1e9
     out = {
@@ -463,7 +473,7 @@ def main():
             "traffic": profile_traffic(dom, args.precision),
             "traffic_source": "profiles/step_kernels_batched_512fields.md (rocprofv3 --pmc "
                               "FETCH_SIZE x2.000, WRITE_SIZE x1.000, bytes per launch)",
-            "kernel_avg_ms": dom_ms,
+            "kernel_avg_ms": dom_ms, "algorithmic_bytes": alg, "dominant_stage": top,
             "step_algorithmic_GBs": step_gbs, "step_frac": step_gbs / HBM_PEAK_GBS,
         },
         "stages_ms_per_step": {k: v[0] / v[1] for k, v in stages.items()},

@@ -43,12 +43,13 @@ __global__ void k_fill_i32(int32_t* p, int64_t n, int32_t v) {
     if (i < n) p[i] = v;
 }
 
-// Working memory of godar_get_totals_noalloc: the six totals it returns (pair
-// totals summed by k_totals, then the three queue totals), one copy on each
+// Working memory of godar_get_totals_noalloc: the totals it returns (pair
+// totals summed by k_totals, the three queue totals, the repair totals),
+// one copy on each
 // GPU the library runs on, held until the process ends.  Host code never
 // passes this name to a kernel or a copy: it uses the device address that
 // hipGetSymbolAddress gives for the current device (totals_scratch_of).
-constexpr int TOTALS_SCRATCH = 32;   // entries: 256 bytes, six of them used
+constexpr int TOTALS_SCRATCH = 32;   // entries: 256 bytes, GODAR_NTOTALS of them used
 static __device__ __align__(256) int64_t scratch[TOTALS_SCRATCH];
 
 // The three queue totals (contact, bond, wall work): each is the sum of its
@@ -60,6 +61,26 @@ __global__ void k_queue_totals(const int32_t* qn, int64_t* out) {
         for (int x = 0; x < XCD_COUNT; ++x) sum += qn[(t * XCD_COUNT + x) * QN_STRIDE];
         out[t] = sum;
     }
+}
+
+// Ridging seeds and events re-evaluated by the repair in the last step each
+// active member took, summed over the members (one block).
+__global__ void k_member_totals(DS S, int mcap, int64_t* out) {
+    __shared__ int64_t part[2][256];
+    int64_t a = 0, b = 0;
+    for (int m = threadIdx.x; m < mcap; m += blockDim.x)
+        if (S.members[m].active) { a += S.stats[4 * m + 2]; b += S.stats[4 * m + 3]; }
+    part[0][threadIdx.x] = a;
+    part[1][threadIdx.x] = b;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            part[0][threadIdx.x] += part[0][threadIdx.x + w];
+            part[1][threadIdx.x] += part[1][threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) { out[0] = part[0][0]; out[1] = part[1][0]; }
 }
 
 __global__ void k_zero_i64(int64_t* p, int n) {
@@ -1762,13 +1783,15 @@ int godar_get_totals(godar_ctx* c, int64_t* out) {
     if (!c || !out) return GODAR_ERR_ARG;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     int64_t* dout = nullptr;
-    HIPCHK(c, hipMalloc(&dout, 3 * sizeof(int64_t)));
-    HIPCHK(c, hipMemset(dout, 0, 3 * sizeof(int64_t)));
+    HIPCHK(c, hipMalloc(&dout, 5 * sizeof(int64_t)));
+    HIPCHK(c, hipMemset(dout, 0, 5 * sizeof(int64_t)));
     sync_ds(c);
     if (c->N_used > 0)
         hipLaunchKernelGGL(k_totals, dim3((c->N_used + 255) / 256), dim3(256), 0, c->stream, c->d, dout);
+    hipLaunchKernelGGL(k_member_totals, dim3(1), dim3(256), 0, c->stream, c->d, c->Mcap, dout + 3);
     hipError_t e = hipStreamSynchronize(c->stream);
     if (e == hipSuccess) e = hipMemcpy(out, dout, 3 * sizeof(int64_t), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(out + 6, dout + 3, 2 * sizeof(int64_t), hipMemcpyDeviceToHost);
     std::vector<int32_t> q((size_t)NQUEUE * XCD_COUNT * QN_STRIDE, 0);
     if (e == hipSuccess)
         e = hipMemcpy(q.data(), c->d.qn, q.size() * sizeof(int32_t), hipMemcpyDeviceToHost);
@@ -1800,6 +1823,7 @@ int godar_get_totals_noalloc(godar_ctx* c, int64_t* out) {
     if (c->N_used > 0)
         hipLaunchKernelGGL(k_totals, dim3((c->N_used + 255) / 256), dim3(256), 0, c->stream, c->d, w);
     hipLaunchKernelGGL(k_queue_totals, dim3(1), dim3(64), 0, c->stream, c->d.qn, w + 3);
+    hipLaunchKernelGGL(k_member_totals, dim3(1), dim3(256), 0, c->stream, c->d, c->Mcap, w + 6);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipMemcpyAsync(out, w, GODAR_NTOTALS * sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));

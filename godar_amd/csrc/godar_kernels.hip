@@ -1020,22 +1020,25 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_cand(DS S) {
 // consecutive floes of the dense queue together.  Their supersets lie one
 // after the other in the pool; the lanes test them 64 entries a round (each
 // lane finds its entry's floe among the group's by a binary search over
-// their starts), evaluate the sheltering and classification terms of every
-// pair they find at once (minima and flags go to the floe's slot in LDS by
-// atomics: a minimum and an "any" do not depend on the order of their
-// terms), and list the pairs in LDS, where each floe's are contiguous.  Each
-// pair's slot in its floe's list is then its rank there -- outer candidates
-// by (distance, id), inner by id, the orders of the thread search -- counted
-// over the floe's pairs, and the lists are written straight to their slots.
-// A group holds at most GRP_MAX floes and GRP_SUP superset entries; a floe
-// whose superset alone is larger goes to the thread search
-// (k_cand_dense_over).  Blocks are one wavefront, each over DENSE_CHUNK
-// entries of the dense queue (the host sizes the grid by it), the queue split
-// into one contiguous share per XCD; the work per floe is uneven, and small
-// shares let the dispatcher balance it.
+// their starts) and list the pairs they find in LDS, outer candidates from
+// the front of one array and inner ones from its back, each floe's pairs
+// contiguous.  Each pair's slot in its floe's list is its rank among that
+// floe's pairs of the same side -- outer by (distance, id), inner by id, the
+// orders of the thread search -- and the lists are written straight to
+// their slots.  Sheltering terms are needed only for pairs close enough
+// that exp(-0.18 |dn/cos| / hf) still counts (shelter_min); those pairs are
+// listed apart and their terms evaluated 64 at a time, each floe taking the
+// minimum of its own by an LDS atomic (a minimum does not depend on the
+// order of its terms).  A group holds at most GRP_MAX floes and GRP_SUP
+// superset entries; a floe whose superset alone is larger goes to the
+// thread search (k_cand_dense_over).  Blocks are one wavefront, each over
+// DENSE_CHUNK entries of the dense queue (the host sizes the grid by it),
+// the queue split into one contiguous share per XCD; the work per floe is
+// uneven, and small shares let the dispatcher balance it.
 constexpr int DENSE_OVER = -1;   // ocnt of a dense floe left to the thread search
 constexpr int GRP_MAX = 16;      // floes per group
-constexpr int GRP_SUP = 448;     // superset entries per group (the LDS pair list)
+constexpr int GRP_SUP = 448;     // superset entries per group (bounds its pairs)
+constexpr int GRP_NEAR = 128;    // near pairs listed before their terms are evaluated
 
 __device__ __forceinline__ void bitonic_inner(int64_t* ke, int n, int lane) {
     for (int size = 2; size <= n; size <<= 1) {
@@ -1132,21 +1135,28 @@ struct GroupFloe {
     int32_t flags;               // bit0 sheltering on, bit1 bonds form in this step
     int32_t pad_;
 };
-// What the lanes gather for a floe of the group (LDS atomics).
-struct GroupAcc {
-    uint64_t mina, minw;         // ordered keys of the sheltering minima
-    int32_t no, ni;              // outer and inner pairs found
-    uint32_t fl;                 // bit0 a partner overlaps, bit1 a bond forms
-    int32_t hb;                  // first of its pairs in the group's list
+// A pair of the group: outer (key = distance bits, d >= 0) or inner
+// (key = entry); ent = (id << 32 | pos) of the partner.
+struct GroupPair {
+    uint64_t key;
+    int64_t ent;
+};
+// A pair whose sheltering terms are needed.
+struct GroupNear {
+    double d;
+    int32_t q;                   // partner position
+    uint8_t own, outer, pad0_, pad1_;
 };
 struct GroupLds {
-    uint64_t key[GRP_SUP];       // outer: distance bits (d >= 0); inner: entry | 1 << 63
-    int64_t ent[GRP_SUP];        // (id << 32 | pos)
-    uint8_t own[GRP_SUP];        // floe of the group
+    GroupPair pr[GRP_SUP];       // outer pairs from index 0 up, inner from GRP_SUP - 1 down
+    uint8_t own[GRP_SUP];        // floe of the group of each pair
+    GroupNear nr[GRP_NEAR];
     GroupFloe fl[GRP_MAX];
-    GroupAcc acc[GRP_MAX];
+    uint64_t mina[GRP_MAX], minw[GRP_MAX];   // ordered keys of the sheltering minima
+    uint32_t cfl[GRP_MAX];                   // bit0 a partner overlaps, bit1 a bond forms
+    int32_t seg[GRP_MAX][4];                 // first and count of its outer, then inner pairs
 };
-static_assert(GRP_MAX <= 64 && GRP_SUP < (1 << 16), "group shape");
+static_assert(GRP_MAX <= 64 && GRP_SUP <= 65536, "group shape");
 
 // Order-preserving map of doubles to unsigned keys (a < b iff key(a) < key(b)
 // for non-NaN values), for minima taken with integer atomics.
@@ -1156,6 +1166,47 @@ __device__ __forceinline__ uint64_t ord_key(double v) {
 }
 __device__ __forceinline__ double ord_val(uint64_t k) {
     return __longlong_as_double((long long)((k >> 63) ? (k & ~(1ull << 63)) : ~k));
+}
+
+// first index i in [0, n) with own(i) >= g, own non-decreasing
+template <typename F>
+__device__ __forceinline__ int lower_bound_own(F own, int n, int g) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (own(mid) < g) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Sheltering terms of the listed near pairs (every lane of the wavefront
+// calls it): each floe takes the minimum of its own.
+template <bool MIXED>
+__device__ __forceinline__ void group_near_terms(const DS& S, GroupLds& A, int nnear, int lane) {
+    using T = typename Prec<MIXED>::T;
+    for (int i = lane; i < nnear; i += 64) {
+        const GroupNear nr = A.nr[i];
+        const GroupFloe& g = A.fl[nr.own];
+        const GeoRec o = S.geo[nr.q];
+        const ShelRec oh = S.shel[nr.q];
+        const double d = nr.d;
+        const bool outer = nr.outer != 0;
+        // outer: pair (f, j), f the outer floe i of the event, which takes
+        // hsf*(j,f) with -cosa, -sina of f -> j and its own wind; inner:
+        // pair (j, f), cosa/sina from j to f and j's wind
+        const double deltan = outer ? g.r + o.r - d : o.r + g.r - d;
+        const double nx = outer ? o.x - g.x : g.x - o.x;
+        const double ny = outer ? o.y - g.y : g.y - o.y;
+        const double cosa = nx / d, sina = ny / d;
+        const T dn = T(deltan), c0t = T(cosa), s0t = T(sina);
+        const T c = outer ? -c0t : c0t, sn = outer ? -s0t : s0t;
+        const T ux = outer ? T(g.wx) : T(oh.wx), uy = outer ? T(g.wy) : T(oh.wy);
+        const T ta = shelter_min(T(1), T(oh.hfa), T(g.hfa), dn, c, sn, ux, uy);
+        const T tw = shelter_min(T(1), T(oh.hfw), T(g.hfw), dn, c, sn, T(g.uw), T(g.vw));
+        if (ta < T(1)) atomicMin((unsigned long long*)&A.mina[nr.own], (unsigned long long)ord_key((double)ta));
+        if (tw < T(1)) atomicMin((unsigned long long*)&A.minw[nr.own], (unsigned long long)ord_key((double)tw));
+    }
 }
 
 template <bool MIXED>
@@ -1176,6 +1227,9 @@ __global__ __launch_bounds__(64) void k_cand_dense(DS S) {
     const uint64_t lt = lane ? ((~0ull) >> (64 - lane)) : 0ull;
     const int64_t SS = S.S;
     const uint64_t one_key = ord_key(1.0);
+    // a near pair needs its sheltering terms unless both are skipped by
+    // shelter_min's own test (deltan < 0 and 0.18 |deltan| >= lim hf_j)
+    const T lim = sizeof(T) == 4 ? T(20) : T(40);
     while (t < t_end) {
         // 1. the group: the leading queue entries whose supersets fit
         const bool valid = lane < GRP_MAX && t + lane < t_end;
@@ -1230,17 +1284,13 @@ __global__ __launch_bounds__(64) void k_cand_dense(DS S) {
                 has_bonds = S.bcnt_old[fs] > 0;
             }
             A.fl[lane] = g;
-            GroupAcc z;
-            z.mina = z.minw = one_key;
-            z.no = z.ni = 0;
-            z.fl = 0;
-            z.hb = 0;
-            A.acc[lane] = z;
+            A.mina[lane] = A.minw[lane] = one_key;
+            A.cfl[lane] = 0;
         }
         const int my_start = lane < G ? start : INT_MAX;
         wave_lds_sync();
-        // 2. the group's supersets, 64 entries a round: pair test, terms, list
-        int atotal = 0;
+        // 2. the group's supersets, 64 entries a round
+        int no_all = 0, ni_all = 0, nnear = 0;
         for (int c0 = 0; c0 < total; c0 += 64) {
             const int idx = c0 + lane;
             // the entry's floe: the last one whose superset starts at or
@@ -1249,12 +1299,13 @@ __global__ __launch_bounds__(64) void k_cand_dense(DS S) {
 #pragma unroll
             for (int step = GRP_MAX / 2; step > 0; step >>= 1)
                 if (__shfl(my_start, own + step) <= idx) own += step;
-            bool hit = false, outer = false;
+            bool hit = false, outer = false, near = false;
             double d = 0.0;
             int64_t e = 0;
+            int q = 0;
             if (idx < total) {
                 const GroupFloe& g = A.fl[own];
-                const int q = ent_pos(S.sup[g.off + (idx - g.start)]);
+                q = ent_pos(S.sup[g.off + (idx - g.start)]);
                 const GeoRec o = S.geo[q];
                 const double dx = g.x - o.x, dy = g.y - o.y;
                 const double dist2 = dx * dx + dy * dy;
@@ -1267,8 +1318,6 @@ __global__ __launch_bounds__(64) void k_cand_dense(DS S) {
                 }
                 e = ent(o.id, q);
                 if (hit) {
-                    // outer: pair (f, j) with f the outer floe i of the event;
-                    // inner: pair (j, f) with f the inner floe j
                     const double deltan = outer ? g.r + o.r - d : o.r + g.r - d;
                     uint32_t fl = deltan > 0 ? 1u : 0u;
                     if (outer && (g.flags & 2)) {
@@ -1278,73 +1327,89 @@ __global__ __launch_bounds__(64) void k_cand_dense(DS S) {
                             bond_keep(g.id - first, o.id - first, P.bond_seed, P.bond_thr24))
                             fl |= 2u;
                     }
-                    if (fl) atomicOr(&A.acc[own].fl, fl);
+                    if (fl) atomicOr(&A.cfl[own], fl);
                     if (g.flags & 1) {
-                        // f takes hsf*(j,f): outer with -cosa, -sina of f -> j
-                        // and its own wind; inner with cosa/sina from j to f
-                        // and j's wind
-                        const double nx = outer ? o.x - g.x : g.x - o.x;
-                        const double ny = outer ? o.y - g.y : g.y - o.y;
-                        const double cosa = nx / d, sina = ny / d;
-                        const ShelRec oh = S.shel[q];
-                        const T dn = T(deltan), c0t = T(cosa), s0t = T(sina);
-                        const T c = outer ? -c0t : c0t, sn = outer ? -s0t : s0t;
-                        const T ux = outer ? T(g.wx) : T(oh.wx), uy = outer ? T(g.wy) : T(oh.wy);
-                        const T ta = shelter_min(T(1), T(oh.hfa), T(g.hfa), dn, c, sn, ux, uy);
-                        const T tw = shelter_min(T(1), T(oh.hfw), T(g.hfw), dn, c, sn, T(g.uw), T(g.vw));
-                        if (ta < T(1)) atomicMin((unsigned long long*)&A.acc[own].mina,
-                                                 (unsigned long long)ord_key((double)ta));
-                        if (tw < T(1)) atomicMin((unsigned long long*)&A.acc[own].minw,
-                                                 (unsigned long long)ord_key((double)tw));
+                        const T dn = T(deltan);
+                        near = !(dn < T(0) && T((double)0.18f) * -dn >= lim * T(g.hfa)) ||
+                               !(dn < T(0) && T((double)0.18f) * -dn >= lim * T(g.hfw));
                     }
-                    atomicAdd(outer ? &A.acc[own].no : &A.acc[own].ni, 1);
                 }
             }
-            const uint64_t hm = __ballot(hit);
+            const uint64_t om = __ballot(hit && outer), im = __ballot(hit && !outer);
             if (hit) {
-                const int at = atotal + (int)__popcll(hm & lt);
-                A.key[at] = outer ? (uint64_t)__double_as_longlong(d) : ((uint64_t)e | (1ull << 63));
-                A.ent[at] = e;
-                A.own[at] = (uint8_t)own;
+                if (outer) {
+                    const int at = no_all + (int)__popcll(om & lt);
+                    A.pr[at].key = (uint64_t)__double_as_longlong(d);
+                    A.pr[at].ent = e;
+                    A.own[at] = (uint8_t)own;
+                } else {
+                    const int at = GRP_SUP - 1 - (ni_all + (int)__popcll(im & lt));
+                    A.pr[at].key = (uint64_t)e;
+                    A.pr[at].ent = e;
+                    A.own[at] = (uint8_t)own;
+                }
             }
-            atotal += (int)__popcll(hm);
+            no_all += (int)__popcll(om);
+            ni_all += (int)__popcll(im);
+            const uint64_t nm = __ballot(near);
+            if (nm) {
+                if (nnear + (int)__popcll(nm) > GRP_NEAR) {
+                    wave_lds_sync();
+                    group_near_terms<MIXED>(S, A, nnear, lane);
+                    nnear = 0;
+                    wave_lds_sync();
+                }
+                if (near) {
+                    GroupNear r;
+                    r.d = d; r.q = q; r.own = (uint8_t)own; r.outer = outer ? 1 : 0;
+                    r.pad0_ = r.pad1_ = 0;
+                    A.nr[nnear + (int)__popcll(nm & lt)] = r;
+                }
+                nnear += (int)__popcll(nm);
+            }
         }
         wave_lds_sync();
-        // 3. each floe's pairs start where the previous floe's end
-        GroupAcc a;
-        a.no = a.ni = 0;
-        a.fl = 0;
-        a.mina = a.minw = one_key;
-        if (lane < G) a = A.acc[lane];
-        const int cnt = a.no + a.ni;
-        int cincl = cnt;
-#pragma unroll
-        for (int off = 1; off < GRP_MAX; off <<= 1) {
-            const int v = __shfl_up(cincl, off);
-            if (lane >= off) cincl += v;
+        if (nnear) {
+            group_near_terms<MIXED>(S, A, nnear, lane);
+            wave_lds_sync();
         }
-        if (lane < G) A.acc[lane].hb = cincl - cnt;
+        // 3. each floe's pairs: outer at [ob, ob + no), inner (counted from
+        // the back) at [ib, ib + ni)
+        int ob = 0, no = 0, ib = 0, ni = 0;
+        if (lane < G) {
+            ob = lower_bound_own([&](int i) { return (int)A.own[i]; }, no_all, lane);
+            no = lower_bound_own([&](int i) { return (int)A.own[i]; }, no_all, lane + 1) - ob;
+            ib = lower_bound_own([&](int i) { return (int)A.own[GRP_SUP - 1 - i]; }, ni_all, lane);
+            ni = lower_bound_own([&](int i) { return (int)A.own[GRP_SUP - 1 - i]; }, ni_all, lane + 1) - ib;
+            A.seg[lane][0] = ob; A.seg[lane][1] = no; A.seg[lane][2] = ib; A.seg[lane][3] = ni;
+        }
         wave_lds_sync();
-        // 4. every pair to its rank in its floe's list
-        for (int h = lane; h < atotal; h += 64) {
-            const int g = A.own[h];
-            const uint64_t kh = A.key[h];
-            const int64_t eh = A.ent[h];
-            const int b0 = A.acc[g].hb, b1 = b0 + A.acc[g].no + A.acc[g].ni;
+        // 4. every pair to its rank among its floe's pairs of the same side
+        for (int h = lane; h < no_all + ni_all; h += 64) {
+            const bool out_h = h < no_all;
+            const int hi = out_h ? h : h - no_all;               // index on its side
+            const int at = out_h ? hi : GRP_SUP - 1 - hi;
+            const int g = A.own[at];
+            const int b0 = A.seg[g][out_h ? 0 : 2], n0 = A.seg[g][out_h ? 1 : 3];
+            const GroupPair ph = A.pr[at];
             int rank = 0;
-            for (int j = b0; j < b1; ++j) {
-                const uint64_t kj = A.key[j];
-                const int64_t ej = A.ent[j];
-                rank += (((kj ^ kh) >> 63) == 0 && (kj < kh || (kj == kh && ej < eh))) ? 1 : 0;
+            if (out_h) {
+                for (int j = b0; j < b0 + n0; ++j) {
+                    const GroupPair pj = A.pr[j];
+                    rank += (pj.key < ph.key || (pj.key == ph.key && pj.ent < ph.ent)) ? 1 : 0;
+                }
+            } else {
+                for (int j = b0; j < b0 + n0; ++j)
+                    rank += A.pr[GRP_SUP - 1 - j].key < ph.key ? 1 : 0;
             }
             const int kk = A.fl[g].k;
-            if (kh >> 63) {
+            if (!out_h) {
                 if (rank < S.CI) {
-                    S.ik[rank * SS + kk] = eh;
+                    S.ik[rank * SS + kk] = ph.ent;
                     S.ibond[rank * SS + kk] = 0;
                 }
             } else if (rank < S.CO) {
-                S.oj[rank * SS + kk] = eh;
+                S.oj[rank * SS + kk] = ph.ent;
                 S.evf[rank * SS + kk] = (A.fl[g].flags & 1) ? EV_SHELTER : 0u;
                 if (S.record_forces) {
                     S.ssc[rank * SS + kk] = 0.0;
@@ -1354,19 +1419,19 @@ __global__ __launch_bounds__(64) void k_cand_dense(DS S) {
         }
         // 5. each floe's counts, minima, per-step resets and queues
         if (lane < G && mem >= 0) {
-            int no = a.no, ni = a.ni;
             uint32_t over = 0;
             if (no > S.CO) { over |= ERR_OUTER_CAP; no = S.CO; }
             if (ni > S.CI) { over |= ERR_INNER_CAP; ni = S.CI; }
             if (over) set_err(S, over);
             const DevParams& P = S.params[mem];
-            const bool qc = (a.fl & 1u) || has_hist;
-            const bool qb = (a.fl & 2u) || has_bonds;
+            const uint32_t cf = A.cfl[lane];
+            const bool qc = (cf & 1u) || has_hist;
+            const bool qb = (cf & 2u) || has_bonds;
             const bool qw = me.y - me.r < 0 || me.y + me.r > P.ny || me.x - me.r < 0 || me.x + me.r > P.nx;
             S.ocnt[k] = no;
             S.icnt[k] = ni;
-            S.acc[A_SHA * SS + k] = ord_val(a.mina);
-            S.acc[A_SHW * SS + k] = ord_val(a.minw);
+            S.acc[A_SHA * SS + k] = ord_val(A.mina[lane]);
+            S.acc[A_SHW * SS + k] = ord_val(A.minw[lane]);
             if (!qc) S.hcnt_new[fs] = 0;
             if (!qb) S.bcnt_new[fs] = 0;
             S.wflag[k] = 0;

@@ -273,12 +273,13 @@ class Context:
 
     def totals(self) -> dict:
         """Candidate pairs, contacts (last step) and bonded pairs over all
-        members, and the floes the last step queued for contact, bond and wall
-        work."""
-        t = np.zeros(6, dtype=np.int64)
+        members, the floes the last step queued for contact, bond and wall
+        work, and the ridging seeds and events the repair re-evaluated."""
+        t = np.zeros(8, dtype=np.int64)
         self._chk(self.L.godar_get_totals(self.h, t.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))))
         return dict(candidates=int(t[0]), contacts=int(t[1]), bonds=int(t[2]),
-                    queued_contact=int(t[3]), queued_bond=int(t[4]), queued_wall=int(t[5]))
+                    queued_contact=int(t[3]), queued_bond=int(t[4]), queued_wall=int(t[5]),
+                    ridging_seeds=int(t[6]), repaired_events=int(t[7]))
 
     def stats(self, m: int) -> dict:
         s = np.zeros(4, dtype=np.int64)

@@ -185,8 +185,10 @@ int godar_get_bond_state(godar_ctx *ctx, int32_t member, int32_t cap, double *st
 int godar_get_stats(godar_ctx *ctx, int32_t member, int64_t *stats);
 /* Totals over all members of the last step: [0] candidate pairs,
  * [1] contacts, [2] bonded pairs, [3..5] floes queued for contact, bond and
- * wall work.  `totals` holds GODAR_NTOTALS entries. */
-#define GODAR_NTOTALS 6
+ * wall work, [6] ridging seeds and [7] events re-evaluated by the ridging
+ * repair (of the last step each active member took).  `totals` holds
+ * GODAR_NTOTALS entries. */
+#define GODAR_NTOTALS 8
 int godar_get_totals(godar_ctx *ctx, int64_t *totals);
 /* godar_get_totals without any device allocation (no counterpart in the
  * reference).  For callers that cannot allocate: serving loops that poll
