@@ -465,13 +465,11 @@ int launch_step(godar_ctx* c) {
     int64_t dense_n = 0;
     for (int k = 0; k < c->Mcap; ++k)
         if (c->members[k].active && c->dparams[k].dense) { any_dense = true; dense_n += c->members[k].n; }
-    // The dense search runs one wavefront per block, each block over
-    // DENSE_CHUNK entries of the dense queue; the work per floe is uneven,
-    // and small shares let the dispatcher balance it.  The superset builds
-    // keep 1,024 wavefronts per XCD: they return at once on most steps, and
-    // dispatching many more would cost more than they save.
-    const dim3 dense_grid(XCD_COUNT * (int)std::min<int64_t>(
-        65536, std::max<int64_t>(64, (dense_n + XCD_COUNT * DENSE_CHUNK - 1) / (XCD_COUNT * DENSE_CHUNK))));
+    // The dense search runs a thread per dense floe (blocks of one
+    // wavefront, each with its list of near pairs in LDS); the superset
+    // builds keep 1,024 wavefronts per XCD: they return at once on most
+    // steps, and dispatching many more would cost more than they save.
+    const dim3 dense_grid(XCD_COUNT * (int)std::max<int64_t>(1, (dense_n + XCD_COUNT * 64 - 1) / (XCD_COUNT * 64)));
     {
         StageTimer t(c, 2);
         hipLaunchKernelGGL(k_ranges, grid_for(N, B[2], false), dim3(B[2]), 0, s, d);
@@ -492,8 +490,7 @@ int launch_step(godar_ctx* c) {
         hipLaunchKernelGGL(mixed ? k_cand<true> : k_cand<false>, grid_for(N, B[3], true),
                            dim3(B[3]), (B[3] / 64) * CAND_LDS_PER_WAVE, s, d);
         if (any_dense) {
-            // groups of dense floes per wavefront, a contiguous share of the
-            // list per XCD
+            // a thread per dense floe, a contiguous share of the list per XCD
             hipLaunchKernelGGL(mixed ? k_cand_dense<true> : k_cand_dense<false>,
                                dense_grid, dim3(64), 0, s, d);
             hipLaunchKernelGGL(mixed ? k_cand_dense_over<true> : k_cand_dense_over<false>,
@@ -720,6 +717,7 @@ int godar_create(int64_t max_floes, int32_t max_members, int32_t precision, goda
     c->sup_cap = 8 * S;
     ACHK(dalloc(&d.sup, c->sup_cap));
     ACHK(dalloc(&d.scnt, S));
+    ACHK(dalloc(&d.sout, S));
     ACHK(dalloc(&d.soff, S));
     ACHK(dalloc(&d.sblk, S / SCAN_TILE + 2));
     ACHK(dalloc(&d.x0, S));
@@ -772,7 +770,7 @@ int godar_destroy(godar_ctx* c) {
     dfree(d.wout); dfree(d.wflag); dfree(d.cls); dfree(d.qlist); dfree(d.qn); dfree(d.bgat); dfree(d.seeds); dfree(d.nseeds); dfree(d.heap); dfree(d.etab);
     dfree(d.netab); dfree(d.wtab); dfree(d.nwtab); dfree(d.ovw); dfree(d.dirty);
     dfree(d.wnz); dfree(d.stats); dfree(d.err); dfree(d.steps_done);
-    dfree(d.sup); dfree(d.scnt); dfree(d.soff); dfree(d.sblk); dfree(d.x0); dfree(d.rebuild);
+    dfree(d.sup); dfree(d.scnt); dfree(d.sout); dfree(d.soff); dfree(d.sblk); dfree(d.x0); dfree(d.rebuild);
     radix_free(c->rs);
     collect_events(c);
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
@@ -798,7 +796,7 @@ static int64_t bytes_per_floe(const godar_ctx* c) {
     per += (int64_t)c->SEEDK * 8 + (int64_t)c->HEAPK * 8 + (int64_t)c->TABK * sizeof(EventOut);
     per += sizeof(WallOut) /*wtab*/ + 4 /*ovw*/ + 6 * 8 /*cur*/ + 1 + 1 /*dirty, wnz*/;
     per += 33 * 8 /*fout*/ + 4 + 4 /*radix keys, values*/;
-    per += (c->S > 0 ? c->sup_cap / c->S : 0) * 8 /*superset pool share*/ + 4 + 8 + 16 /*scnt soff x0*/;
+    per += (c->S > 0 ? c->sup_cap / c->S : 0) * 8 /*superset pool share*/ + 4 + 4 + 8 + 16 /*scnt sout soff x0*/;
     per += 1 /*sblk, per SCAN_TILE floes*/;
     return per;
 }

@@ -23,7 +23,6 @@ constexpr int XCD_COUNT = 8;      // accelerator complex dies of an MI355X
 constexpr int NQUEUE = 4;         // work queues: contact, bond, wall, dense search
 constexpr int QN_STRIDE = 64;     // ints between queue counters (256 B)
 constexpr int SCAN_TILE = 1024;   // positions per tile of the superset offset scan
-constexpr int DENSE_CHUNK = 16;   // dense-queue entries per wavefront of the dense search
 
 // Per-floe record (96 bytes) by storage slot.  Between builds of the cell
 // order the floe state lives in the ordered copies below, which the step
@@ -127,6 +126,10 @@ struct DS {
     int64_t* sup;              // [sup_cap]
     int64_t sup_cap;
     int32_t* scnt;             // [S] by pos
+    int32_t* sout;             // [S] by pos, dense members: the superset holds its outer
+                               // entries (partner id above the floe's) first, by (distance
+                               // at the build, id), then the inner ones by id; sout is the
+                               // outer count (-1: left in scan order, too long to order)
     int64_t* soff;             // [S] by pos
     int64_t* sblk;             // [S / SCAN_TILE + 2] scan of per-tile totals
     double2* x0;               // [S] by storage slot

@@ -1016,29 +1016,25 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_cand(DS S) {
     enqueue(S, qw, 2, k);
 }
 
-// Dense members (many floes per cell): a wavefront searches a group of
-// consecutive floes of the dense queue together.  Their supersets lie one
-// after the other in the pool; the lanes test them 64 entries a round (each
-// lane finds its entry's floe among the group's by a binary search over
-// their starts) and list the pairs they find in LDS, outer candidates from
-// the front of one array and inner ones from its back, each floe's pairs
-// contiguous.  Each pair's slot in its floe's list is its rank among that
-// floe's pairs of the same side -- outer by (distance, id), inner by id, the
-// orders of the thread search -- and the lists are written straight to
-// their slots.  Sheltering terms are needed only for pairs close enough
-// that exp(-0.18 |dn/cos| / hf) still counts (shelter_min); those pairs are
-// listed apart and their terms evaluated 64 at a time, each floe taking the
-// minimum of its own by an LDS atomic (a minimum does not depend on the
-// order of its terms).  A group holds at most GRP_MAX floes and GRP_SUP
-// superset entries; a floe whose superset alone is larger goes to the
-// thread search (k_cand_dense_over).  Blocks are one wavefront, each over
-// DENSE_CHUNK entries of the dense queue (the host sizes the grid by it),
-// the queue split into one contiguous share per XCD; the work per floe is
-// uneven, and small shares let the dispatcher balance it.
+// Dense members (many floes per cell): a thread per floe, over its superset
+// as the build ordered it (outer entries by distance at the build, then inner
+// ones by id).  Inner candidates therefore come in list order and are
+// appended; outer ones come in nearly the order of their current distance
+// (floes move little between builds), so each is inserted among the last
+// DENSE_WIN entries, which stay in registers, and only the rare entry that
+// belongs below them is inserted into the part of the list in HBM.
+// Sheltering terms are needed only for pairs close enough that
+// exp(-0.18 |dn/cos| / hf) still counts (shelter_min); the threads list those
+// pairs in LDS and the wavefront evaluates them 64 at a time, each floe
+// taking the minimum of its own (an LDS atomic: a minimum does not depend on
+// the order of its terms).  A superset left in scan order (longer than
+// SUP_ORDER_CAP) goes to the thread search (k_cand_dense_over).
 constexpr int DENSE_OVER = -1;   // ocnt of a dense floe left to the thread search
-constexpr int GRP_MAX = 16;      // floes per group
-constexpr int GRP_SUP = 448;     // superset entries per group (bounds its pairs)
-constexpr int GRP_NEAR = 128;    // near pairs listed before their terms are evaluated
+constexpr int DENSE_WIN = 4;     // outer candidates in registers
+constexpr int DENSE_NEAR = 512;  // near pairs listed per wavefront
+#ifndef DENSE_BATCH
+#define DENSE_BATCH 4              // superset entries a thread loads at once
+#endif
 
 __device__ __forceinline__ void bitonic_inner(int64_t* ke, int n, int lane) {
     for (int size = 2; size <= n; size <<= 1) {
@@ -1071,9 +1067,17 @@ __device__ __forceinline__ int pow2_at_least(int n) {
 
 // Superset build of the dense members' floes (queue 3), one wavefront per
 // floe over its (2 span + 1)^2 cell block, 64 entries at a time: counts, or
-// entries in scan order and the build positions.  Only on build steps.
+// the entries and the build positions.  The filled superset is then ordered
+// as the dense search reads it: outer entries (partner id above the floe's)
+// first, by (distance at the build, id), then inner entries by id; each
+// entry goes to its rank, counted over the floe's entries in LDS.  A
+// superset longer than SUP_ORDER_CAP stays in scan order (sout = -1).  Only
+// on build steps.
+constexpr int SUP_ORDER_CAP = 1024;
 template <bool FILL>
 __global__ __launch_bounds__(64) void k_sup_dense(DS S) {
+    __shared__ double od_[FILL ? SUP_ORDER_CAP : 1];
+    __shared__ int64_t oe_[FILL ? SUP_ORDER_CAP : 1];
     if (!*S.rebuild || *S.err) return;
     const int lane = threadIdx.x;
     const QueueView qv = queue_view(S, 3);
@@ -1118,6 +1122,42 @@ __global__ __launch_bounds__(64) void k_sup_dense(DS S) {
                 n += (int)__popcll(mi);
             }
         }
+        if (FILL) {
+            if (n <= SUP_ORDER_CAP) {
+                // outer entries carry their build distance, inner ones -1
+                int nout = 0;
+                for (int v = lane; v < n; v += 64) {
+                    const int64_t e = out[v];
+                    const GeoRec o = S.geo[ent_pos(e)];
+                    const double dx = me.x - o.x, dy = me.y - o.y;
+                    const bool outer = ent_id(e) > f;
+                    od_[v] = outer ? sqrt(dx * dx + dy * dy) : -1.0;
+                    oe_[v] = e;
+                    nout += (int)__popcll(__ballot(outer));
+                }
+                nout = __shfl(nout, 0);
+                wave_lds_sync();
+                for (int v = lane; v < n; v += 64) {
+                    const double dv = od_[v];
+                    const int64_t ev = oe_[v];
+                    int rank = 0;
+                    if (dv >= 0.0) {
+                        for (int u = 0; u < n; ++u) {
+                            const double du = od_[u];
+                            rank += (du >= 0.0 && (du < dv || (du == dv && oe_[u] < ev))) ? 1 : 0;
+                        }
+                    } else {
+                        rank = nout;
+                        for (int u = 0; u < n; ++u) rank += (od_[u] < 0.0 && oe_[u] < ev) ? 1 : 0;
+                    }
+                    out[rank] = ev;
+                }
+                if (lane == 0) S.sout[k] = nout;
+                wave_lds_sync();
+            } else if (lane == 0) {
+                S.sout[k] = -1;
+            }
+        }
         if (lane == 0) {
             if (FILL) S.x0[S.cval[k]] = make_double2(me.x, me.y);
             else S.scnt[k] = n;
@@ -1125,38 +1165,12 @@ __global__ __launch_bounds__(64) void k_sup_dense(DS S) {
     }
 }
 
-// One floe of a dense-search group in LDS.
-struct GroupFloe {
-    double x, y, r, rt;          // position, radius, rtree of its member
-    double hfa, hfw, wx, wy;     // its sheltering record (drafts, wind at the floe)
-    double uw, vw;               // current of its member
-    int64_t off;                 // first entry of its superset in the pool
-    int32_t k, id, start, mem;   // position, identity, first entry in the group's sequence, member
-    int32_t flags;               // bit0 sheltering on, bit1 bonds form in this step
-    int32_t pad_;
-};
-// A pair of the group: outer (key = distance bits, d >= 0) or inner
-// (key = entry); ent = (id << 32 | pos) of the partner.
-struct GroupPair {
-    uint64_t key;
-    int64_t ent;
-};
 // A pair whose sheltering terms are needed.
-struct GroupNear {
+struct DenseNear {
     double d;
     int32_t q;                   // partner position
-    uint8_t own, outer, pad0_, pad1_;
+    uint8_t lane, outer, pad0_, pad1_;
 };
-struct GroupLds {
-    GroupPair pr[GRP_SUP];       // outer pairs from index 0 up, inner from GRP_SUP - 1 down
-    uint8_t own[GRP_SUP];        // floe of the group of each pair
-    GroupNear nr[GRP_NEAR];
-    GroupFloe fl[GRP_MAX];
-    uint64_t mina[GRP_MAX], minw[GRP_MAX];   // ordered keys of the sheltering minima
-    uint32_t cfl[GRP_MAX];                   // bit0 a partner overlaps, bit1 a bond forms
-    int32_t seg[GRP_MAX][4];                 // first and count of its outer, then inner pairs
-};
-static_assert(GRP_MAX <= 64 && GRP_SUP <= 65536, "group shape");
 
 // Order-preserving map of doubles to unsigned keys (a < b iff key(a) < key(b)
 // for non-NaN values), for minima taken with integer atomics.
@@ -1168,283 +1182,230 @@ __device__ __forceinline__ double ord_val(uint64_t k) {
     return __longlong_as_double((long long)((k >> 63) ? (k & ~(1ull << 63)) : ~k));
 }
 
-// first index i in [0, n) with own(i) >= g, own non-decreasing
-template <typename F>
-__device__ __forceinline__ int lower_bound_own(F own, int n, int g) {
-    int lo = 0, hi = n;
-    while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (own(mid) < g) lo = mid + 1;
-        else hi = mid;
-    }
-    return lo;
-}
-
-// Sheltering terms of the listed near pairs (every lane of the wavefront
-// calls it): each floe takes the minimum of its own.
-template <bool MIXED>
-__device__ __forceinline__ void group_near_terms(const DS& S, GroupLds& A, int nnear, int lane) {
-    using T = typename Prec<MIXED>::T;
-    for (int i = lane; i < nnear; i += 64) {
-        const GroupNear nr = A.nr[i];
-        const GroupFloe& g = A.fl[nr.own];
-        const GeoRec o = S.geo[nr.q];
-        const ShelRec oh = S.shel[nr.q];
-        const double d = nr.d;
-        const bool outer = nr.outer != 0;
-        // outer: pair (f, j), f the outer floe i of the event, which takes
-        // hsf*(j,f) with -cosa, -sina of f -> j and its own wind; inner:
-        // pair (j, f), cosa/sina from j to f and j's wind
-        const double deltan = outer ? g.r + o.r - d : o.r + g.r - d;
-        const double nx = outer ? o.x - g.x : g.x - o.x;
-        const double ny = outer ? o.y - g.y : g.y - o.y;
-        const double cosa = nx / d, sina = ny / d;
-        const T dn = T(deltan), c0t = T(cosa), s0t = T(sina);
-        const T c = outer ? -c0t : c0t, sn = outer ? -s0t : s0t;
-        const T ux = outer ? T(g.wx) : T(oh.wx), uy = outer ? T(g.wy) : T(oh.wy);
-        const T ta = shelter_min(T(1), T(oh.hfa), T(g.hfa), dn, c, sn, ux, uy);
-        const T tw = shelter_min(T(1), T(oh.hfw), T(g.hfw), dn, c, sn, T(g.uw), T(g.vw));
-        if (ta < T(1)) atomicMin((unsigned long long*)&A.mina[nr.own], (unsigned long long)ord_key((double)ta));
-        if (tw < T(1)) atomicMin((unsigned long long*)&A.minw[nr.own], (unsigned long long)ord_key((double)tw));
-    }
-}
-
 template <bool MIXED>
 __global__ __launch_bounds__(64) void k_cand_dense(DS S) {
     using T = typename Prec<MIXED>::T;
-    __shared__ GroupLds A;
+    __shared__ DenseNear nr[DENSE_NEAR];
+    __shared__ uint64_t mina[64], minw[64];
+    __shared__ int nnear;
     if (*S.err) return;
     const int lane = threadIdx.x;
-    const QueueView qv = queue_view(S, 3);
-    const int nq = qv.total;
-    const int nb = (int)(gridDim.x / XCD_COUNT);
-    const int xcd = (int)(blockIdx.x % XCD_COUNT), bi = (int)(blockIdx.x / XCD_COUNT);
-    const int per = (nq + XCD_COUNT - 1) / XCD_COUNT;
-    const int s1 = min(nq, xcd * per + per);
-    const int chunk = (per + nb - 1) / nb;
-    int t = xcd * per + bi * chunk;
-    const int t_end = min(s1, t + chunk);
-    const uint64_t lt = lane ? ((~0ull) >> (64 - lane)) : 0ull;
-    const int64_t SS = S.S;
-    const uint64_t one_key = ord_key(1.0);
-    // a near pair needs its sheltering terms unless both are skipped by
-    // shelter_min's own test (deltan < 0 and 0.18 |deltan| >= lim hf_j)
-    const T lim = sizeof(T) == 4 ? T(20) : T(40);
-    while (t < t_end) {
-        // 1. the group: the leading queue entries whose supersets fit
-        const bool valid = lane < GRP_MAX && t + lane < t_end;
-        int k = 0, L = 0, mem = -1;
-        GeoRec me;
-        me.x = me.y = me.r = 0.0;
-        me.id = 0;
-        me.member = -1;
-        if (valid) {
-            k = queue_at(S, 3, qv, t + lane);
-            me = S.geo[k];
-            mem = me.member;            // -1: member past its stop step (nothing to do)
-            if (mem >= 0) L = S.scnt[k];
+    const int k = queue_entry(S, 3);
+    GeoRec me;
+    me.x = me.y = me.r = 0.0;
+    me.id = 0;
+    me.member = -1;
+    int L = 0, nout = 0;
+    int64_t soff = 0;
+    if (k >= 0) {
+        me = S.geo[k];
+        if (me.member >= 0) {   // (-1: member past its stop step, nothing to do)
+            L = S.scnt[k];
+            nout = S.sout[k];
+            soff = S.soff[k];
         }
-        int incl = L;
-#pragma unroll
-        for (int off = 1; off < GRP_MAX; off <<= 1) {
-            const int v = __shfl_up(incl, off);
-            if (lane >= off) incl += v;
-        }
-        const uint64_t fm = __ballot(valid && L <= GRP_SUP && incl <= GRP_SUP);
-        const int G = __builtin_ctzll(~fm);
-        if (G == 0) {
-            // the first floe's superset alone outgrows the list: thread search
-            if (lane == 0 && mem >= 0) S.ocnt[k] = DENSE_OVER;
-            t += 1;
-            continue;
-        }
-        const int start = incl - L;
-        const int total = __shfl(incl, G - 1);
-        int fs = 0;
-        bool has_hist = false, has_bonds = false;
-        if (lane < G) {
-            GroupFloe g;
-            g.k = k; g.mem = mem; g.start = start; g.id = me.id;
-            g.x = me.x; g.y = me.y; g.r = me.r;
-            g.rt = g.hfa = g.hfw = g.wx = g.wy = g.uw = g.vw = 0.0;
-            g.off = 0;
-            g.flags = 0;
-            g.pad_ = 0;
-            if (mem >= 0) {
-                const DevParams& P = S.params[mem];
-                const ShelRec sh = S.shel[k];
-                fs = S.cval[k];
-                g.rt = P.rtree;
-                g.hfa = sh.hfa; g.hfw = sh.hfw; g.wx = sh.wx; g.wy = sh.wy;
-                g.uw = P.uw; g.vw = P.vw;
-                g.off = S.soff[k];
-                g.flags = (P.shelter ? 1 : 0) |
-                          ((S.members[mem].tstep == 1 && P.restart != 1 && P.cohesion) ? 2 : 0);
-                has_hist = S.hcnt_old[fs] > 0;
-                has_bonds = S.bcnt_old[fs] > 0;
-            }
-            A.fl[lane] = g;
-            A.mina[lane] = A.minw[lane] = one_key;
-            A.cfl[lane] = 0;
-        }
-        const int my_start = lane < G ? start : INT_MAX;
-        wave_lds_sync();
-        // 2. the group's supersets, 64 entries a round
-        int no_all = 0, ni_all = 0, nnear = 0;
-        for (int c0 = 0; c0 < total; c0 += 64) {
-            const int idx = c0 + lane;
-            // the entry's floe: the last one whose superset starts at or
-            // before it (floes without entries share the next one's start)
-            int own = 0;
-#pragma unroll
-            for (int step = GRP_MAX / 2; step > 0; step >>= 1)
-                if (__shfl(my_start, own + step) <= idx) own += step;
-            bool hit = false, outer = false, near = false;
-            double d = 0.0;
-            int64_t e = 0;
-            int q = 0;
-            if (idx < total) {
-                const GroupFloe& g = A.fl[own];
-                q = ent_pos(S.sup[g.off + (idx - g.start)]);
-                const GeoRec o = S.geo[q];
-                const double dx = g.x - o.x, dy = g.y - o.y;
-                const double dist2 = dx * dx + dy * dy;
-                outer = o.id > g.id;
-                const double thr = outer ? g.r + g.rt : o.r + g.rt;
-                // far beyond the radius: no square root needed
-                if (dist2 <= thr * thr * (1.0 + 1e-12)) {
-                    d = sqrt(dist2);
-                    hit = d <= thr;
-                }
-                e = ent(o.id, q);
-                if (hit) {
-                    const double deltan = outer ? g.r + o.r - d : o.r + g.r - d;
-                    uint32_t fl = deltan > 0 ? 1u : 0u;
-                    if (outer && (g.flags & 2)) {
-                        const DevParams& P = S.params[g.mem];
-                        const int first = S.members[g.mem].first;
-                        if (deltan >= -P.bond_lim &&
-                            bond_keep(g.id - first, o.id - first, P.bond_seed, P.bond_thr24))
-                            fl |= 2u;
-                    }
-                    if (fl) atomicOr(&A.cfl[own], fl);
-                    if (g.flags & 1) {
-                        const T dn = T(deltan);
-                        near = !(dn < T(0) && T((double)0.18f) * -dn >= lim * T(g.hfa)) ||
-                               !(dn < T(0) && T((double)0.18f) * -dn >= lim * T(g.hfw));
-                    }
-                }
-            }
-            const uint64_t om = __ballot(hit && outer), im = __ballot(hit && !outer);
-            if (hit) {
-                if (outer) {
-                    const int at = no_all + (int)__popcll(om & lt);
-                    A.pr[at].key = (uint64_t)__double_as_longlong(d);
-                    A.pr[at].ent = e;
-                    A.own[at] = (uint8_t)own;
-                } else {
-                    const int at = GRP_SUP - 1 - (ni_all + (int)__popcll(im & lt));
-                    A.pr[at].key = (uint64_t)e;
-                    A.pr[at].ent = e;
-                    A.own[at] = (uint8_t)own;
-                }
-            }
-            no_all += (int)__popcll(om);
-            ni_all += (int)__popcll(im);
-            const uint64_t nm = __ballot(near);
-            if (nm) {
-                if (nnear + (int)__popcll(nm) > GRP_NEAR) {
-                    wave_lds_sync();
-                    group_near_terms<MIXED>(S, A, nnear, lane);
-                    nnear = 0;
-                    wave_lds_sync();
-                }
-                if (near) {
-                    GroupNear r;
-                    r.d = d; r.q = q; r.own = (uint8_t)own; r.outer = outer ? 1 : 0;
-                    r.pad0_ = r.pad1_ = 0;
-                    A.nr[nnear + (int)__popcll(nm & lt)] = r;
-                }
-                nnear += (int)__popcll(nm);
-            }
-        }
-        wave_lds_sync();
-        if (nnear) {
-            group_near_terms<MIXED>(S, A, nnear, lane);
-            wave_lds_sync();
-        }
-        // 3. each floe's pairs: outer at [ob, ob + no), inner (counted from
-        // the back) at [ib, ib + ni)
-        int ob = 0, no = 0, ib = 0, ni = 0;
-        if (lane < G) {
-            ob = lower_bound_own([&](int i) { return (int)A.own[i]; }, no_all, lane);
-            no = lower_bound_own([&](int i) { return (int)A.own[i]; }, no_all, lane + 1) - ob;
-            ib = lower_bound_own([&](int i) { return (int)A.own[GRP_SUP - 1 - i]; }, ni_all, lane);
-            ni = lower_bound_own([&](int i) { return (int)A.own[GRP_SUP - 1 - i]; }, ni_all, lane + 1) - ib;
-            A.seg[lane][0] = ob; A.seg[lane][1] = no; A.seg[lane][2] = ib; A.seg[lane][3] = ni;
-        }
-        wave_lds_sync();
-        // 4. every pair to its rank among its floe's pairs of the same side
-        for (int h = lane; h < no_all + ni_all; h += 64) {
-            const bool out_h = h < no_all;
-            const int hi = out_h ? h : h - no_all;               // index on its side
-            const int at = out_h ? hi : GRP_SUP - 1 - hi;
-            const int g = A.own[at];
-            const int b0 = A.seg[g][out_h ? 0 : 2], n0 = A.seg[g][out_h ? 1 : 3];
-            const GroupPair ph = A.pr[at];
-            int rank = 0;
-            if (out_h) {
-                for (int j = b0; j < b0 + n0; ++j) {
-                    const GroupPair pj = A.pr[j];
-                    rank += (pj.key < ph.key || (pj.key == ph.key && pj.ent < ph.ent)) ? 1 : 0;
-                }
-            } else {
-                for (int j = b0; j < b0 + n0; ++j)
-                    rank += A.pr[GRP_SUP - 1 - j].key < ph.key ? 1 : 0;
-            }
-            const int kk = A.fl[g].k;
-            if (!out_h) {
-                if (rank < S.CI) {
-                    S.ik[rank * SS + kk] = ph.ent;
-                    S.ibond[rank * SS + kk] = 0;
-                }
-            } else if (rank < S.CO) {
-                S.oj[rank * SS + kk] = ph.ent;
-                S.evf[rank * SS + kk] = (A.fl[g].flags & 1) ? EV_SHELTER : 0u;
-                if (S.record_forces) {
-                    S.ssc[rank * SS + kk] = 0.0;
-                    S.ssb[rank * SS + kk] = 0.0;
-                }
-            }
-        }
-        // 5. each floe's counts, minima, per-step resets and queues
-        if (lane < G && mem >= 0) {
-            uint32_t over = 0;
-            if (no > S.CO) { over |= ERR_OUTER_CAP; no = S.CO; }
-            if (ni > S.CI) { over |= ERR_INNER_CAP; ni = S.CI; }
-            if (over) set_err(S, over);
-            const DevParams& P = S.params[mem];
-            const uint32_t cf = A.cfl[lane];
-            const bool qc = (cf & 1u) || has_hist;
-            const bool qb = (cf & 2u) || has_bonds;
-            const bool qw = me.y - me.r < 0 || me.y + me.r > P.ny || me.x - me.r < 0 || me.x + me.r > P.nx;
-            S.ocnt[k] = no;
-            S.icnt[k] = ni;
-            S.acc[A_SHA * SS + k] = ord_val(A.mina[lane]);
-            S.acc[A_SHW * SS + k] = ord_val(A.minw[lane]);
-            if (!qc) S.hcnt_new[fs] = 0;
-            if (!qb) S.bcnt_new[fs] = 0;
-            S.wflag[k] = 0;
-            S.bgat[k] = 0;
-            S.cls[k] = qc ? CLS_CONTACT : 0;
-            if (qc) enqueue_one(S, 0, k);
-            if (qb) enqueue_one(S, 1, k);
-            if (qw) enqueue_one(S, 2, k);
-        }
-        t += G;
-        // the next group rewrites the LDS lists: the reads above come first
-        wave_lds_sync();
     }
+    bool act = k >= 0 && me.member >= 0;
+    if (act && nout < 0) {
+        S.ocnt[k] = DENSE_OVER;   // superset in scan order: the thread search
+        act = false;
+    }
+    const uint64_t one_key = ord_key(1.0);
+    mina[lane] = minw[lane] = one_key;
+    if (lane == 0) nnear = 0;
+    const int m = act ? me.member : 0;
+    const DevParams& P = S.params[m];
+    const double rad = me.r + P.rtree;
+    const int f = me.id;
+    const int first = S.members[m].first;
+    const bool bond_step = act && S.members[m].tstep == 1 && P.restart != 1 && P.cohesion;
+    ShelRec msh;
+    msh.hfa = msh.hfw = msh.wx = msh.wy = 0.0;
+    int fs = 0;
+    bool qc = false, qb = false;
+    if (act) {
+        msh = S.shel[k];
+        fs = S.cval[k];
+        qc = S.hcnt_old[fs] > 0;
+        qb = S.bcnt_old[fs] > 0;
+    }
+    if (!act) L = 0;
+    const T uwi = T(P.uw), vwi = T(P.vw);
+    const T lim = sizeof(T) == 4 ? T(20) : T(40);
+    T sha = T(1), shw = T(1);
+    wave_lds_sync();
+    const int64_t SS = S.S;
+    int64_t* oj = S.oj;
+    double* od = S.od;
+    // the last min(no, DENSE_WIN) outer entries, ascending: slots [no - nw, no)
+    double wd[DENSE_WIN];
+    int64_t we[DENSE_WIN];
+#pragma unroll
+    for (int w = 0; w < DENSE_WIN; ++w) {
+        wd[w] = __longlong_as_double(0x7ff0000000000000ll);
+        we[w] = INT64_MAX;
+    }
+    int no = 0, ni = 0, nw = 0;
+    uint32_t over = 0;
+    auto less = [](double ad, int64_t ae, double bd, int64_t be) {
+        return ad < bd || (ad == bd && ent_id(ae) < ent_id(be));
+    };
+    auto insert_outer = [&](double xd, int64_t xe) {
+        if (no >= S.CO) { over |= ERR_OUTER_CAP; return; }
+        if (nw == DENSE_WIN && less(xd, xe, wd[0], we[0])) {
+            // below the register part: into the part in HBM, whose slots
+            // [0, no - DENSE_WIN) move up by one from x's place on
+            int p = no - DENSE_WIN;
+            while (p > 0) {
+                const double dp = od[(int64_t)(p - 1) * SS + k];
+                const int64_t ep = oj[(int64_t)(p - 1) * SS + k];
+                if (!less(xd, xe, dp, ep)) break;
+                od[(int64_t)p * SS + k] = dp;
+                oj[(int64_t)p * SS + k] = ep;
+                --p;
+            }
+            od[(int64_t)p * SS + k] = xd;
+            oj[(int64_t)p * SS + k] = xe;
+            ++no;
+            return;
+        }
+        if (nw == DENSE_WIN) {
+            // the smallest register entry is final: to its slot
+            const int sl = no - DENSE_WIN;
+            od[(int64_t)sl * SS + k] = wd[0];
+            oj[(int64_t)sl * SS + k] = we[0];
+#pragma unroll
+            for (int w = 0; w < DENSE_WIN - 1; ++w) { wd[w] = wd[w + 1]; we[w] = we[w + 1]; }
+            wd[DENSE_WIN - 1] = __longlong_as_double(0x7ff0000000000000ll);
+            we[DENSE_WIN - 1] = INT64_MAX;
+            --nw;
+        }
+        // insertion into the sorted register part (empty slots hold keys
+        // above every entry)
+#pragma unroll
+        for (int w = 0; w < DENSE_WIN; ++w) {
+            const bool lt = less(xd, xe, wd[w], we[w]);
+            const double td = wd[w];
+            const int64_t te = we[w];
+            wd[w] = lt ? xd : td;
+            we[w] = lt ? xe : te;
+            xd = lt ? td : xd;
+            xe = lt ? te : xe;
+        }
+        ++nw;
+        ++no;
+    };
+    constexpr int B = DENSE_BATCH;
+    const int64_t* se = S.sup + soff;
+    for (int v0 = 0; v0 < L; v0 += B) {
+        int64_t ev[B];
+        GeoRec g[B];
+#pragma unroll
+        for (int u = 0; u < B; ++u) ev[u] = se[min(v0 + u, L - 1)];   // the last entry stands in past the end
+#pragma unroll
+        for (int u = 0; u < B; ++u) g[u] = S.geo[ent_pos(ev[u])];
+#pragma unroll
+        for (int u = 0; u < B; ++u) {
+            if (v0 + u >= L) continue;
+            const GeoRec& o = g[u];
+            const bool outer = v0 + u < nout;
+            const double dx = me.x - o.x, dy = me.y - o.y;
+            const double dist2 = dx * dx + dy * dy;
+            const double thr = outer ? rad : o.r + P.rtree;
+            // far beyond the radius: no square root needed
+            if (dist2 > thr * thr * (1.0 + 1e-12)) continue;
+            const double d = sqrt(dist2);
+            if (!(d <= thr)) continue;
+            // outer: pair (f, j) with f the outer floe i of the event;
+            // inner: pair (j, f) with f the inner floe j
+            const double deltan = outer ? me.r + o.r - d : o.r + me.r - d;
+            if (deltan > 0) qc = true;
+            if (outer && bond_step && deltan >= -P.bond_lim &&
+                bond_keep(f - first, ent_id(ev[u]) - first, P.bond_seed, P.bond_thr24))
+                qb = true;
+            if (P.shelter) {
+                const T dn = T(deltan);
+                const bool near = !(dn < T(0) && T((double)0.18f) * -dn >= lim * T(msh.hfa)) ||
+                                  !(dn < T(0) && T((double)0.18f) * -dn >= lim * T(msh.hfw));
+                if (near) {
+                    const int slot = atomicAdd(&nnear, 1);
+                    if (slot < DENSE_NEAR) {
+                        DenseNear r;
+                        r.d = d; r.q = ent_pos(ev[u]); r.lane = (uint8_t)lane; r.outer = outer ? 1 : 0;
+                        r.pad0_ = r.pad1_ = 0;
+                        nr[slot] = r;
+                    } else {
+                        // list full: this thread takes the terms itself
+                        const double nx = outer ? o.x - me.x : me.x - o.x;
+                        const double ny = outer ? o.y - me.y : me.y - o.y;
+                        const double cosa = nx / d, sina = ny / d;
+                        const ShelRec oh = S.shel[ent_pos(ev[u])];
+                        const T c0t = T(cosa), s0t = T(sina);
+                        const T c = outer ? -c0t : c0t, sn = outer ? -s0t : s0t;
+                        const T ux = outer ? T(msh.wx) : T(oh.wx), uy = outer ? T(msh.wy) : T(oh.wy);
+                        sha = shelter_min(sha, T(oh.hfa), T(msh.hfa), dn, c, sn, ux, uy);
+                        shw = shelter_min(shw, T(oh.hfw), T(msh.hfw), dn, c, sn, uwi, vwi);
+                    }
+                }
+            }
+            if (outer) {
+                insert_outer(d, ev[u]);
+            } else if (ni < S.CI) {
+                S.ik[(int64_t)ni * SS + k] = ev[u];
+                ++ni;
+            } else {
+                over |= ERR_INNER_CAP;
+            }
+        }
+    }
+    // the register part to its slots
+#pragma unroll
+    for (int w = 0; w < DENSE_WIN; ++w)
+        if (w < nw) oj[(int64_t)(no - nw + w) * SS + k] = we[w];
+    // sheltering terms of the listed near pairs, across the lanes (every lane
+    // takes part in each round; the listing thread's floe comes by shuffle)
+    wave_lds_sync();
+    const int nn = min(nnear, DENSE_NEAR);
+    for (int i0 = 0; i0 < nn; i0 += 64) {
+        const int i = i0 + lane;
+        DenseNear r;
+        r.d = 1.0; r.q = 0; r.lane = 0; r.outer = 0; r.pad0_ = r.pad1_ = 0;
+        if (i < nn) r = nr[i];
+        const int ow = r.lane;
+        const double ox = __shfl(me.x, ow), oy = __shfl(me.y, ow), orr = __shfl(me.r, ow);
+        const double ohfa = __shfl(msh.hfa, ow), ohfw = __shfl(msh.hfw, ow);
+        const double owx = __shfl(msh.wx, ow), owy = __shfl(msh.wy, ow);
+        const double ouw = __shfl(P.uw, ow), ovw = __shfl(P.vw, ow);
+        if (i < nn) {
+            const GeoRec o = S.geo[r.q];
+            const ShelRec oh = S.shel[r.q];
+            const bool outer = r.outer != 0;
+            const double d = r.d;
+            // outer: f takes hsf*(j,f) with -cosa, -sina of f -> j and its own
+            // wind; inner: cosa/sina from j to f and j's wind
+            const double deltan = outer ? orr + o.r - d : o.r + orr - d;
+            const double nx = outer ? o.x - ox : ox - o.x;
+            const double ny = outer ? o.y - oy : oy - o.y;
+            const double cosa = nx / d, sina = ny / d;
+            const T dn = T(deltan), c0t = T(cosa), s0t = T(sina);
+            const T c = outer ? -c0t : c0t, sn = outer ? -s0t : s0t;
+            const T ux = outer ? T(owx) : T(oh.wx), uy = outer ? T(owy) : T(oh.wy);
+            const T ta = shelter_min(T(1), T(oh.hfa), T(ohfa), dn, c, sn, ux, uy);
+            const T tw = shelter_min(T(1), T(oh.hfw), T(ohfw), dn, c, sn, T(ouw), T(ovw));
+            if (ta < T(1)) atomicMin((unsigned long long*)&mina[ow], (unsigned long long)ord_key((double)ta));
+            if (tw < T(1)) atomicMin((unsigned long long*)&minw[ow], (unsigned long long)ord_key((double)tw));
+        }
+    }
+    wave_lds_sync();
+    sha = dmin(sha, T(ord_val(mina[lane])));
+    shw = dmin(shw, T(ord_val(minw[lane])));
+    if (!act) return;
+    if (over) set_err(S, over);
+    const bool qw = me.y - me.r < 0 || me.y + me.r > P.ny || me.x - me.r < 0 || me.x + me.r > P.nx;
+    cand_finish<MIXED>(S, k, fs, P, no, ni, double(sha), double(shw), qc, qb, 0, 1);
+    if (qc) enqueue_one(S, 0, k);
+    if (qb) enqueue_one(S, 1, k);
+    if (qw) enqueue_one(S, 2, k);
 }
 
 // The thread search for the dense floes whose candidates overflow the dense

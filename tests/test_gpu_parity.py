@@ -383,33 +383,36 @@ def test_dense_search_matches_thread_search():
             assert np.array_equal(a, b)
 
 
-def test_dense_search_overflow_takes_thread_search():
-    """Dense floes with more candidates on one side than the wavefront
-    search's LDS lists hold (256) are handed to the thread search: a member
-    with a search radius of 8 km (about 9 % of its floes over 256 on a side)
-    gives the same lists, sheltering and trajectories through the dense
-    search as through the thread search, bitwise, in FP64 and mixed
-    precision."""
-    f = packed_box(600, 5, r_lo=200.0, r_hi=400.0)
-    p = config0_params(f, rtree=8.0e3)
-    for prec in ("fp64", "mixed"):
-        runs = []
-        for thr in (0, 1 << 30):
-            ctx = _ctx(f.n, 1, prec)
-            ctx.set_dense_search(thr)
-            m = ctx.add_field(p, f)
-            ctx.step(6)
-            runs.append((ctx.floes(m), _pairs(ctx, m, "candidate"), ctx.forces(m)["hsfa_min"]))
-            ctx.close()
-        (sa, ca, ha), (sb, cb, hb) = runs
-        outer = np.bincount([i for i, _ in ca], minlength=f.n)
-        inner = np.bincount([j for _, j in ca], minlength=f.n)
-        over = (outer > 256) | (inner > 256)
-        assert 0.02 < over.mean() < 0.5, over.mean()
-        for k in STATE:
-            assert np.array_equal(sa[k], sb[k]), (prec, k)
-        assert ca == cb
-        assert np.array_equal(ha, hb)
+def test_dense_search_long_lists_and_unordered_supersets():
+    """Dense floes with long candidate lists give the same lists, sheltering
+    and trajectories through the dense search as through the thread search,
+    bitwise, in FP64 and mixed precision: a member with a search radius of
+    8 km (many floes with more than 256 candidates on a side, so that
+    entries often go below the register part of the outer list), and one
+    with a radius of 12 km whose supersets exceed 1,024 entries (left in
+    scan order at the build and handed to the thread search)."""
+    cases = [(packed_box(600, 5, r_lo=200.0, r_hi=400.0), 8.0e3, 256, 6),
+             (packed_box(1500, 8, box=1.4e4, r_lo=100.0, r_hi=200.0), 1.2e4, 1024, 3)]
+    for f, rtree, big, steps in cases:
+        p = config0_params(f, rtree=rtree)
+        for prec in ("fp64", "mixed"):
+            runs = []
+            for thr in (0, 1 << 30):
+                ctx = _ctx(f.n, 1, prec)
+                ctx.set_dense_search(thr)
+                m = ctx.add_field(p, f)
+                ctx.step(steps)
+                runs.append((ctx.floes(m), _pairs(ctx, m, "candidate"), ctx.forces(m)["hsfa_min"]))
+                ctx.close()
+            (sa, ca, ha), (sb, cb, hb) = runs
+            outer = np.bincount([i for i, _ in ca], minlength=f.n)
+            inner = np.bincount([j for _, j in ca], minlength=f.n)
+            many = ((outer > big) | (inner > big)) if big == 256 else (outer + inner > big)
+            assert 0.02 < many.mean(), (rtree, many.mean())
+            for k in STATE:
+                assert np.array_equal(sa[k], sb[k]), (prec, rtree, k)
+            assert ca == cb
+            assert np.array_equal(ha, hb)
 
 
 def test_checkpoint_restore_continues_exactly():
