@@ -28,7 +28,8 @@ PAIRS_BOND = 2
 EXPORTS = [
     "godar_params_default", "godar_create", "godar_destroy", "godar_last_error",
     "godar_member_add", "godar_member_remove", "godar_member_footprint",
-    "godar_capacity_left", "godar_set_forcing", "godar_set_wind_grid", "godar_set_bonds",
+    "godar_capacity_left", "godar_set_forcing", "godar_set_wind_grid", "godar_set_wind_frames",
+    "godar_set_bonds",
     "godar_step", "godar_step_async", "godar_sync", "godar_member_tstep", "godar_reserve",
     "godar_member_run_until", "godar_member_tsteps",
     "godar_get_floes", "godar_record_forces", "godar_get_forces", "godar_get_walls",
@@ -79,6 +80,8 @@ def load(path: str = LIB_PATH):
         "godar_capacity_left": (_i32, [_vp, _lp, _ip]),
         "godar_set_forcing": (_i32, [_vp, _i32, _d, _d, _d, _d]),
         "godar_set_wind_grid": (_i32, [_vp, _i32, _i32, _i32, _d, _d, _d, _d, _dp, _dp]),
+        "godar_set_wind_frames": (_i32, [_vp, _i32, _i32, _i32, _d, _d, _d, _d, _i32, _d, _d, _i32,
+                                         _dp, _dp]),
         "godar_set_bonds": (_i32, [_vp, _i32, _i32, _ip, _ip, _dp]),
         "godar_step": (_i32, [_vp, _i32]),
         "godar_step_async": (_i32, [_vp, _i32]),

@@ -711,6 +711,7 @@ int godar_create(int64_t max_floes, int32_t max_members, int32_t precision, goda
     ACHK(dalloc(&d.dirty, S));
     ACHK(dalloc(&d.wnz, S));
     ACHK(dalloc(&d.stats, 4 * (int64_t)max_members));
+    ACHK(dalloc(&d.tcur, max_members));
     ACHK(dalloc(&d.fout, 33 * S));
     ACHK(dalloc(&d.err, 1));
     ACHK(dalloc(&d.steps_done, 1));
@@ -733,6 +734,7 @@ int godar_create(int64_t max_floes, int32_t max_members, int32_t precision, goda
     (void)hipMemsetAsync(d.dirty, 0, S, c->stream);
     (void)hipMemsetAsync(d.wnz, 1, S, c->stream);
     (void)hipMemsetAsync(d.stats, 0, sizeof(int64_t) * 4 * max_members, c->stream);
+    (void)hipMemsetAsync(d.tcur, 0, sizeof(int32_t) * max_members, c->stream);
     hipLaunchKernelGGL(k_fill_i32, dim3((S + 255) / 256), dim3(256), 0, c->stream, d.member_of, S, -1);
     for (int k = 0; k < 2; ++k) {
         (void)hipMemsetAsync(c->hcnt[k], 0, sizeof(int32_t) * S, c->stream);
@@ -769,7 +771,7 @@ int godar_destroy(godar_ctx* c) {
     }
     dfree(d.wout); dfree(d.wflag); dfree(d.cls); dfree(d.qlist); dfree(d.qn); dfree(d.bgat); dfree(d.seeds); dfree(d.nseeds); dfree(d.heap); dfree(d.etab);
     dfree(d.netab); dfree(d.wtab); dfree(d.nwtab); dfree(d.ovw); dfree(d.dirty);
-    dfree(d.wnz); dfree(d.stats); dfree(d.err); dfree(d.steps_done);
+    dfree(d.wnz); dfree(d.stats); dfree(d.tcur); dfree(d.err); dfree(d.steps_done);
     dfree(d.sup); dfree(d.scnt); dfree(d.sout); dfree(d.soff); dfree(d.sblk); dfree(d.x0); dfree(d.rebuild);
     radix_free(c->rs);
     collect_events(c);
@@ -1249,31 +1251,51 @@ int godar_set_forcing(godar_ctx* c, int32_t m, double ua, double va, double uw, 
     return GODAR_OK;
 }
 
-int godar_set_wind_grid(godar_ctx* c, int32_t m, int32_t nxn, int32_t nyn, double x0, double y0,
-                        double dx, double dy, const double* ua, const double* va) {
-    if (!valid_member(c, m) || nxn < 2 || nyn < 2 || !(dx > 0) || !(dy > 0) || !ua || !va)
-        return fail(c, GODAR_ERR_ARG, "set_wind_grid: bad argument");
+int godar_set_wind_frames(godar_ctx* c, int32_t m, int32_t nxn, int32_t nyn, double x0, double y0,
+                          double dx, double dy, int32_t nframes, double t0, double frame_dt,
+                          int32_t disc_average, const double* ua, const double* va) {
+    if (!valid_member(c, m) || nxn < 2 || nyn < 2 || !(dx > 0) || !(dy > 0) || !ua || !va ||
+        nframes < 1 || !std::isfinite(t0) || (nframes > 1 && !(frame_dt > 0)))
+        return fail(c, GODAR_ERR_ARG, "set_wind_frames: bad argument");
     WindGridDesc g;
     g.nxn = nxn; g.nyn = nyn; g.x0 = x0; g.y0 = y0; g.dx = dx; g.dy = dy;
     g.offset = (int64_t)c->wpool.size();
-    int64_t cnt = (int64_t)nxn * nyn;
-    c->wpool.insert(c->wpool.end(), ua, ua + cnt);
-    c->wpool.insert(c->wpool.end(), va, va + cnt);
+    g.nframes = nframes; g.avg = disc_average ? 1 : 0;
+    g.t0 = t0; g.fdt = nframes > 1 ? frame_dt : 1.0;
+    const int64_t cnt = (int64_t)nxn * nyn;
+    for (int32_t f = 0; f < nframes; ++f) {
+        c->wpool.insert(c->wpool.end(), ua + f * cnt, ua + (f + 1) * cnt);
+        c->wpool.insert(c->wpool.end(), va + f * cnt, va + (f + 1) * cnt);
+    }
     c->wgrids.push_back(g);
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if ((int64_t)c->wpool.size() > c->wpool_dev_cap) {
-        double* wp = const_cast<double*>(c->d.wpool);
-        dfree(wp);
-        c->wpool_dev_cap = std::max<int64_t>((int64_t)c->wpool.size(), 2 * c->wpool_dev_cap);
-        HIPCHK(c, dalloc(&wp, c->wpool_dev_cap));
+        double* wp = nullptr;
+        const int64_t cap = std::max<int64_t>((int64_t)c->wpool.size(), 2 * c->wpool_dev_cap);
+        if (dalloc(&wp, cap) != hipSuccess) {
+            (void)hipGetLastError();
+            c->wgrids.pop_back();
+            c->wpool.resize((size_t)g.offset);
+            return fail(c, GODAR_ERR_NOMEM, "set_wind_frames: out of device memory");
+        }
+        double* old = const_cast<double*>(c->d.wpool);
+        dfree(old);
         c->d.wpool = wp;
+        c->wpool_dev_cap = cap;
     }
     if ((int)c->wgrids.size() > c->wgrids_dev_cap) {
-        WindGridDesc* wg = const_cast<WindGridDesc*>(c->d.wgrids);
-        dfree(wg);
-        c->wgrids_dev_cap = std::max<int>((int)c->wgrids.size(), 2 * c->wgrids_dev_cap);
-        HIPCHK(c, dalloc(&wg, c->wgrids_dev_cap));
+        WindGridDesc* wg = nullptr;
+        const int cap = std::max<int>((int)c->wgrids.size(), 2 * c->wgrids_dev_cap);
+        if (dalloc(&wg, cap) != hipSuccess) {
+            (void)hipGetLastError();
+            c->wgrids.pop_back();
+            c->wpool.resize((size_t)g.offset);
+            return fail(c, GODAR_ERR_NOMEM, "set_wind_frames: out of device memory");
+        }
+        WindGridDesc* old = const_cast<WindGridDesc*>(c->d.wgrids);
+        dfree(old);
         c->d.wgrids = wg;
+        c->wgrids_dev_cap = cap;
     }
     HIPCHK(c, hipMemcpy((void*)c->d.wpool, c->wpool.data(), sizeof(double) * c->wpool.size(),
                         hipMemcpyHostToDevice));
@@ -1284,6 +1306,11 @@ int godar_set_wind_grid(godar_ctx* c, int32_t m, int32_t nxn, int32_t nyn, doubl
     force_refresh(c);
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return GODAR_OK;
+}
+
+int godar_set_wind_grid(godar_ctx* c, int32_t m, int32_t nxn, int32_t nyn, double x0, double y0,
+                        double dx, double dy, const double* ua, const double* va) {
+    return godar_set_wind_frames(c, m, nxn, nyn, x0, y0, dx, dy, 1, 0.0, 1.0, 0, ua, va);
 }
 
 int godar_set_bonds(godar_ctx* c, int32_t m, int32_t nb, const int32_t* bi, const int32_t* bj,

@@ -155,6 +155,7 @@ struct DS {
     uint8_t* wnz;              // [S] by storage slot: 0 = the floe's wall history rows (wbc)
                                //     are all +0.0 (k_final then skips rewriting them)
     int64_t* stats;            // [members][4]
+    int32_t* tcur;             // [members] number of the step in progress (set at its start)
     double* fout;              // [33][S] (id)
     uint32_t* err;
     int64_t* steps_done;       // steps completed without error (whole context)

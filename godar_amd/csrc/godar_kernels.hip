@@ -365,6 +365,7 @@ __global__ void k_begin(DS S) {
     if (live && f == S.members[m].first) {
         S.nseeds[m] = 0; S.netab[m] = 0; S.nwtab[m] = 0;
         S.stats[4 * m + 2] = 0; S.stats[4 * m + 3] = 0;
+        S.tcur[m] = S.members[m].tstep;
     }
     if (!build) return;           // the cell order of the last build stands
     S.bval[f] = (uint32_t)f;
@@ -412,7 +413,10 @@ __global__ void k_ranges(DS S) {
         }
         const DevParams& P = S.params[m];
         ua = P.ua; va = P.va;
-        if (P.wind_grid >= 0) sample_wind(S.wgrids[P.wind_grid], S.wpool, fr.x, fr.y, ua, va);
+        // the wind at the start of step tcur: model time (tcur - 1) dt
+        if (P.wind_grid >= 0)
+            sample_wind(S.wgrids[P.wind_grid], S.wpool, fr.x, fr.y, fr.r,
+                        (double)(S.tcur[m] - 1) * P.dt, ua, va);
     }
     ShelRec h;
     h.hfa = fr.hfa; h.hfw = fr.hfw; h.wx = ua; h.wy = va;
@@ -2681,7 +2685,10 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_final(DS S) {
         // refreshes the copies)
         if (dirty || P.wind_grid >= 0) {
             double ua = P.ua, va = P.va;
-            if (P.wind_grid >= 0) sample_wind(S.wgrids[P.wind_grid], S.wpool, out.x, out.y, ua, va);
+            // at the start of the next step: model time tcur dt
+            if (P.wind_grid >= 0)
+                sample_wind(S.wgrids[P.wind_grid], S.wpool, out.x, out.y, out.r,
+                            (double)S.tcur[m] * P.dt, ua, va);
             ShelRec h;
             h.hfa = out.hfa; h.hfw = out.hfw; h.wx = ua; h.wy = va;
             S.shel[k] = h;

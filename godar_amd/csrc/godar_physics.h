@@ -576,15 +576,15 @@ __device__ __forceinline__ void forcing(const DevParams& P, const Floe& f, doubl
     o.fcory = -(f.mass * 2.0 * 7.2921e-5 * P.sinlat * f.u);
 }
 
-// Bilinear wind sample; the same expression as the generator's sample_grid.
-__device__ __forceinline__ void sample_wind(const WindGridDesc& g, const double* pool, double x,
-                                            double y, double& ua, double& va) {
+// Bilinear sample of one frame; the same expression as the generator's
+// sample_grid.
+__device__ __forceinline__ void wind_bilinear(const WindGridDesc& g, const double* gu, double x,
+                                              double y, double& ua, double& va) {
     double fx = (x - g.x0) / g.dx, fy = (y - g.y0) / g.dy;
     fx = fmin(fmax(fx, 0.0), (double)g.nxn - 1.000001);
     fy = fmin(fmax(fy, 0.0), (double)g.nyn - 1.000001);
     int a = (int)floor(fx), b = (int)floor(fy);
     double tx = fx - a, ty = fy - b;
-    const double* gu = pool + g.offset;
     const double* gv = gu + (int64_t)g.nxn * g.nyn;
     int64_t k00 = (int64_t)b * g.nxn + a;
     int64_t k01 = k00 + g.nxn;
@@ -594,6 +594,57 @@ __device__ __forceinline__ void sample_wind(const WindGridDesc& g, const double*
     lo = gv[k00] + (gv[k00 + 1] - gv[k00]) * tx;
     hi = gv[k01] + (gv[k01 + 1] - gv[k01]) * tx;
     va = lo + (hi - lo) * ty;
+}
+
+// The field at (x, y) at model time t: bilinear in space, linear in time
+// between the two frames around t.
+__device__ __forceinline__ void wind_point(const WindGridDesc& g, const double* pool, int f0,
+                                           double w, double x, double y, double& ua, double& va) {
+    const double* gu = pool + g.offset + (int64_t)f0 * 2 * g.nxn * g.nyn;
+    wind_bilinear(g, gu, x, y, ua, va);
+    if (w > 0.0) {
+        double ub, vb;
+        wind_bilinear(g, gu + (int64_t)2 * g.nxn * g.nyn, x, y, ub, vb);
+        ua = ua + (ub - ua) * w;
+        va = va + (vb - va) * w;
+    }
+}
+
+// The wind a floe of radius r at (x, y) takes at model time t: the field
+// at its centre, or (avg) its mean over the floe's disc by the 7-point rule
+// of degree 5 for the disc (the centre with weight 1/4, six points at
+// radius r sqrt(2/3) and angles k 60 degrees with weight 1/8 each; exact for
+// polynomials of degree 5, so for the bilinear field wherever the disc lies
+// in one grid cell).  The winds_currents hook of the reference
+// (forcing.f90:1-29) plans this averaging over the floe's area.
+__device__ __forceinline__ void sample_wind(const WindGridDesc& g, const double* pool, double x,
+                                            double y, double r, double t, double& ua, double& va) {
+    int f0 = 0;
+    double w = 0.0;
+    if (g.nframes > 1) {
+        const double sf = (t - g.t0) / g.fdt;
+        if (sf >= (double)(g.nframes - 1)) {
+            f0 = g.nframes - 1;
+        } else if (sf > 0.0) {
+            f0 = (int)floor(sf);
+            w = sf - (double)f0;
+        }
+    }
+    wind_point(g, pool, f0, w, x, y, ua, va);
+    if (g.avg && r > 0.0) {
+        const double s3 = 0x1.bb67ae8584caap-1;   // sqrt(3)/2
+        const double rho = r * 0x1.a20bd700c2c3ep-1;   // r sqrt(2/3)
+        const double cs[6][2] = {{1.0, 0.0}, {0.5, s3}, {-0.5, s3}, {-1.0, 0.0}, {-0.5, -s3}, {0.5, -s3}};
+        double su = 0.25 * ua, sv = 0.25 * va;
+        for (int q = 0; q < 6; ++q) {
+            double pu, pv;
+            wind_point(g, pool, f0, w, x + rho * cs[q][0], y + rho * cs[q][1], pu, pv);
+            su = su + 0.125 * pu;
+            sv = sv + 0.125 * pv;
+        }
+        ua = su;
+        va = sv;
+    }
 }
 
 }  // namespace godar

@@ -36,10 +36,18 @@ struct DevParams {
     double skin;           // candidate supersets reach this far beyond the query radius
 };
 
+// A gridded wind: nframes frames of nodes (a, b) at (x0 + a dx, y0 + b dy),
+// frame f at model time t0 + f fdt (linear in time between frames, the
+// first and last frame held outside), each frame ua then va, row-major
+// (nyn, nxn), frame f at offset + 2 f nxn nyn in the pool.  avg = 1: the
+// floe takes the field averaged over its disc (a 7-point rule), else its
+// value at the centre.
 struct WindGridDesc {
     int32_t nxn, nyn;
     double x0, y0, dx, dy;
-    int64_t offset;        // into the wind-grid value pool (ua then va, each nxn*nyn)
+    int64_t offset;
+    int32_t nframes, avg;
+    double t0, fdt;
 };
 
 // Member descriptor (device copy).

@@ -142,6 +142,21 @@ class Context:
         self._chk(self.L.godar_set_wind_grid(self.h, int(m), ua.shape[1], ua.shape[0],
                                              wg.x0, wg.y0, wg.dx, wg.dy, _ptr(ua), _ptr(va)))
 
+    def set_wind_frames(self, m, ua, va, x0, y0, dx, dy, t0=0.0, frame_dt=1.0,
+                        disc_average=True):
+        """Spatio-temporal wind: ua, va of shape (frames, nyn, nxn), frame f
+        at model time t0 + f frame_dt, averaged over each floe's disc (or
+        taken at its centre)."""
+        ua = np.ascontiguousarray(ua, dtype=np.float64)
+        va = np.ascontiguousarray(va, dtype=np.float64)
+        if ua.ndim != 3 or ua.shape != va.shape:
+            raise ValueError("ua and va must have the shape (frames, nyn, nxn)")
+        nf, nyn, nxn = ua.shape
+        self._chk(self.L.godar_set_wind_frames(self.h, int(m), nxn, nyn, float(x0), float(y0),
+                                               float(dx), float(dy), nf, float(t0),
+                                               float(frame_dt), 1 if disc_average else 0,
+                                               _ptr(ua), _ptr(va)))
+
     def set_bonds(self, m, bi, bj, state):
         bi = np.ascontiguousarray(bi, dtype=np.int32)
         bj = np.ascontiguousarray(bj, dtype=np.int32)

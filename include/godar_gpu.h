@@ -104,6 +104,18 @@ int godar_set_forcing(godar_ctx *ctx, int32_t member, double ua, double va, doub
  * (nyn, nxn); sampled bilinearly at floe centres every step. */
 int godar_set_wind_grid(godar_ctx *ctx, int32_t member, int32_t nxn, int32_t nyn, double x0,
                         double y0, double dx, double dy, const double *ua, const double *va);
+/* Spatio-temporal gridded wind (extension; the plan of winds_currents,
+ * forcing.f90:1-10): nframes frames of the grid above, frame f at model time
+ * t0 + f * frame_dt (step tstep starts at model time (tstep - 1) * dt), each
+ * frame's ua then va laid out as (nframes, nyn, nxn); linear in time between
+ * frames, the first and last frame held outside them.  disc_average = 1: a
+ * floe takes the field averaged over its disc (radius at the start of the
+ * step) by the 7-point rule of degree 5 for the disc; 0: its value at the
+ * centre.  godar_set_wind_grid is the case nframes = 1, disc_average = 0. */
+int godar_set_wind_frames(godar_ctx *ctx, int32_t member, int32_t nxn, int32_t nyn, double x0,
+                          double y0, double dx, double dy, int32_t nframes, double t0,
+                          double frame_dt, int32_t disc_average, const double *ua,
+                          const double *va);
 /* Load bonds with their state, e.g. from a restart (i<j, 0-based).
  * state[11*k ..] = bond, damage, deltanb, deltatb, thetarelb(j,i),
  * thetarelb(i,j), rb, hb, lb, sb, ib (CB_bond.h). */

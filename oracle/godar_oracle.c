@@ -66,9 +66,9 @@ struct or_sim {
     double *tac, *tab, *pc, *pb, *tp, *ta_bc, *p_bc;
     double *tsig[4];
     /* wind grid */
-    int wg_nx, wg_ny;
-    double wg_x0, wg_y0, wg_dx, wg_dy;
-    double *wg_ua, *wg_va;
+    int wg_nx, wg_ny, wg_nf, wg_avg;
+    double wg_x0, wg_y0, wg_dx, wg_dy, wg_t0, wg_fdt;
+    double *wg_ua, *wg_va;            /* (frames, ny, nx) each */
     /* hash */
     pstate *tab_p;
     size_t cap, count;
@@ -691,24 +691,35 @@ void or_set_state(or_sim *s, const double *x, const double *y, const double *r,
     s->cap = s->count = 0;
 }
 
-void or_set_wind_grid(or_sim *s, int nxn, int nyn, double x0, double y0,
-                      double dx, double dy, const double *ua, const double *va) {
+void or_set_wind_frames(or_sim *s, int nxn, int nyn, double x0, double y0, double dx,
+                        double dy, int nframes, double t0, double frame_dt, int avg,
+                        const double *ua, const double *va) {
+    size_t cnt = (size_t)nframes * nxn * nyn;
     free(s->wg_ua); free(s->wg_va);
     s->wg_nx = nxn; s->wg_ny = nyn; s->wg_x0 = x0; s->wg_y0 = y0; s->wg_dx = dx; s->wg_dy = dy;
-    s->wg_ua = xcalloc((size_t)nxn * nyn, sizeof(double));
-    s->wg_va = xcalloc((size_t)nxn * nyn, sizeof(double));
-    memcpy(s->wg_ua, ua, sizeof(double) * nxn * nyn);
-    memcpy(s->wg_va, va, sizeof(double) * nxn * nyn);
+    s->wg_nf = nframes; s->wg_avg = avg ? 1 : 0;
+    s->wg_t0 = t0; s->wg_fdt = nframes > 1 ? frame_dt : 1.0;
+    s->wg_ua = xcalloc(cnt, sizeof(double));
+    s->wg_va = xcalloc(cnt, sizeof(double));
+    memcpy(s->wg_ua, ua, sizeof(double) * cnt);
+    memcpy(s->wg_va, va, sizeof(double) * cnt);
 }
 
-/* bilinear sample; identical expression to godar_amd.generators.sample_grid */
-static void sample_wind(const or_sim *s, double x, double y, double *ua, double *va) {
+void or_set_wind_grid(or_sim *s, int nxn, int nyn, double x0, double y0,
+                      double dx, double dy, const double *ua, const double *va) {
+    or_set_wind_frames(s, nxn, nyn, x0, y0, dx, dy, 1, 0.0, 1.0, 0, ua, va);
+}
+
+/* bilinear sample of frame f; identical expression to
+ * godar_amd.generators.sample_grid */
+static void wind_bilinear(const or_sim *s, int f, double x, double y, double *ua, double *va) {
     double fx = (x - s->wg_x0) / s->wg_dx, fy = (y - s->wg_y0) / s->wg_dy;
     fx = fmin(fmax(fx, 0.0), (double)s->wg_nx - 1.000001);
     fy = fmin(fmax(fy, 0.0), (double)s->wg_ny - 1.000001);
     int a = (int)floor(fx), b = (int)floor(fy);
     double tx = fx - a, ty = fy - b;
-    const double *g[2] = {s->wg_ua, s->wg_va};
+    size_t base = (size_t)f * s->wg_nx * s->wg_ny;
+    const double *g[2] = {s->wg_ua + base, s->wg_va + base};
     double out[2];
     for (int c = 0; c < 2; ++c) {
         double g00 = g[c][b * s->wg_nx + a], g10 = g[c][b * s->wg_nx + a + 1];
@@ -719,6 +730,58 @@ static void sample_wind(const or_sim *s, double x, double y, double *ua, double 
     }
     *ua = out[0];
     *va = out[1];
+}
+
+/* the field at (x, y): frame f0, or linear in time towards frame f0 + 1 */
+static void wind_point(const or_sim *s, int f0, double w, double x, double y, double *ua,
+                       double *va) {
+    wind_bilinear(s, f0, x, y, ua, va);
+    if (w > 0.0) {
+        double ub, vb;
+        wind_bilinear(s, f0 + 1, x, y, &ub, &vb);
+        *ua = *ua + (ub - *ua) * w;
+        *va = *va + (vb - *va) * w;
+    }
+}
+
+/* Extension of winds_currents (forcing.f90:1-29), which plans a gridded
+ * field averaged over the floe's area: the field at model time t (linear
+ * between frames), at the centre or averaged over the disc of radius r by
+ * the 7-point rule of degree 5 for the disc (centre weight 1/4, six points
+ * at radius r sqrt(2/3), angles k 60 degrees, weight 1/8 each). */
+static void sample_wind(const or_sim *s, double x, double y, double r, double t, double *ua,
+                        double *va) {
+    int f0 = 0;
+    double w = 0.0;
+    if (s->wg_nf > 1) {
+        double sf = (t - s->wg_t0) / s->wg_fdt;
+        if (sf >= (double)(s->wg_nf - 1)) {
+            f0 = s->wg_nf - 1;
+        } else if (sf > 0.0) {
+            f0 = (int)floor(sf);
+            w = sf - (double)f0;
+        }
+    }
+    wind_point(s, f0, w, x, y, ua, va);
+    if (s->wg_avg && r > 0.0) {
+        const double s3 = 0x1.bb67ae8584caap-1;      /* sqrt(3)/2 */
+        const double rho = r * 0x1.a20bd700c2c3ep-1;  /* r sqrt(2/3) */
+        const double cs[6][2] = {{1.0, 0.0}, {0.5, s3}, {-0.5, s3}, {-1.0, 0.0}, {-0.5, -s3}, {0.5, -s3}};
+        double su = 0.25 * *ua, sv = 0.25 * *va;
+        for (int q = 0; q < 6; ++q) {
+            double pu, pv;
+            wind_point(s, f0, w, x + rho * cs[q][0], y + rho * cs[q][1], &pu, &pv);
+            su = su + 0.125 * pu;
+            sv = sv + 0.125 * pv;
+        }
+        *ua = su;
+        *va = sv;
+    }
+}
+
+void or_get_winds(or_sim *s, double *ua, double *va) {
+    memcpy(ua, s->ua_i, sizeof(double) * s->n);
+    memcpy(va, s->va_i, sizeof(double) * s->n);
 }
 
 static void log_push(or_sim *s, int i, int j, int f) {
@@ -754,12 +817,17 @@ int or_step(or_sim *s, int tstep) {
     }
     grid g;
     build_grid(s, &g);
+    /* winds_currents, forcing.f90:1-29; a gridded wind is taken at the
+     * start of the step (model time (tstep - 1) dt), with the radii the
+     * floes start the step with */
     for (int i = 0; i < n; ++i) {
-        /* winds_currents, forcing.f90:1-29 */
-        if (s->wg_ua) sample_wind(s, s->x[i], s->y[i], &s->ua_i[i], &s->va_i[i]);
+        if (s->wg_ua) sample_wind(s, s->x[i], s->y[i], s->r[i], (double)(tstep - 1) * p->dt,
+                                  &s->ua_i[i], &s->va_i[i]);
         else { s->ua_i[i] = p->ua; s->va_i[i] = p->va; }
         s->uw_i[i] = p->uw;
         s->vw_i[i] = p->vw;
+    }
+    for (int i = 0; i < n; ++i) {
         int cnt = candidates(s, &g, i);
         /* stepper.f90:81-87: skip the first entry, then pairs with i >= j */
         for (int k = 1; k < cnt; ++k) {

@@ -44,6 +44,15 @@ void or_set_state(or_sim *s, const double *x, const double *y, const double *r,
  * winds_currents, forcing.f90:1-29); ua/va are row-major (nyn, nxn). */
 void or_set_wind_grid(or_sim *s, int nxn, int nyn, double x0, double y0,
                       double dx, double dy, const double *ua, const double *va);
+/* Spatio-temporal wind (extension): nframes frames (frames, nyn, nxn) at
+ * model times t0 + f frame_dt, linear in time between them; avg: averaged
+ * over each floe's disc by the 7-point rule of degree 5, else at the
+ * centre.  Sampled at the start of each step ((tstep - 1) dt). */
+void or_set_wind_frames(or_sim *s, int nxn, int nyn, double x0, double y0, double dx,
+                        double dy, int nframes, double t0, double frame_dt, int avg,
+                        const double *ua, const double *va);
+/* ua_i, va_i of the last step. */
+void or_get_winds(or_sim *s, double *ua, double *va);
 int or_step(or_sim *s, int tstep);
 void or_get_state(or_sim *s, double *x, double *y, double *r, double *h,
                   double *theta, double *omega, double *u, double *v,

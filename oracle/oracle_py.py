@@ -52,6 +52,11 @@ def lib():
         L.or_set_wind_grid.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
                                        ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                        ctypes.c_double, _dp, _dp]
+        L.or_set_wind_frames.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                         ctypes.c_double, ctypes.c_int, ctypes.c_double,
+                                         ctypes.c_double, ctypes.c_int, _dp, _dp]
+        L.or_get_winds.argtypes = [ctypes.c_void_p, _dp, _dp]
         L.or_step.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.or_get_state.argtypes = [ctypes.c_void_p] + [_dp] * 12
         L.or_get_forces.argtypes = [ctypes.c_void_p, _dp]
@@ -104,6 +109,20 @@ class OracleSim:
         va = np.ascontiguousarray(wg.va, dtype=np.float64)
         self.L.or_set_wind_grid(self.h, ua.shape[1], ua.shape[0], wg.x0, wg.y0, wg.dx, wg.dy,
                                 _ptr(ua), _ptr(va))
+
+    def set_wind_frames(self, ua, va, x0, y0, dx, dy, t0=0.0, frame_dt=1.0, disc_average=True):
+        ua = np.ascontiguousarray(ua, dtype=np.float64)
+        va = np.ascontiguousarray(va, dtype=np.float64)
+        nf, nyn, nxn = ua.shape
+        self._keep_wind = (ua, va)
+        self.L.or_set_wind_frames(self.h, nxn, nyn, x0, y0, dx, dy, nf, t0, frame_dt,
+                                  1 if disc_average else 0, _ptr(ua), _ptr(va))
+
+    def winds(self):
+        """(ua_i, va_i) of the last step."""
+        a, b = np.zeros(self.n), np.zeros(self.n)
+        self.L.or_get_winds(self.h, _ptr(a), _ptr(b))
+        return a, b
 
     def step(self, k: int = 1):
         for _ in range(k):
