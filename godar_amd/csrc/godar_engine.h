@@ -132,7 +132,7 @@ struct DS {
                                // outer count (-1: left in scan order, too long to order)
     int64_t* soff;             // [S] by pos
     int64_t* sblk;             // [S / SCAN_TILE + 2] scan of per-tile totals
-    double2* x0;               // [S] by storage slot
+    double2* x0;               // [S] by pos (positions stand between builds)
     int32_t* rebuild;          // [3] 0: build the supersets; 1: refresh the ordered copies;
                                //     2: the copies are newer than rec
     // Work queues (contact, bond, wall, dense search), each in XCD_COUNT

@@ -933,7 +933,7 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_sup(DS S) {
                 qd = !FILL;
             } else if (FILL) {
                 sup_thread<1, true>(S, k, me, P, S.sup + S.soff[k]);
-                S.x0[S.cval[k]] = make_double2(me.x, me.y);
+                S.x0[k] = make_double2(me.x, me.y);
             } else {
                 S.scnt[k] = sup_thread<1, false>(S, k, me, P, nullptr);
             }
@@ -1036,9 +1036,7 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_cand(DS S) {
 constexpr int DENSE_OVER = -1;   // ocnt of a dense floe left to the thread search
 constexpr int DENSE_WIN = 4;     // outer candidates in registers
 constexpr int DENSE_NEAR = 512;  // near pairs listed per wavefront
-#ifndef DENSE_BATCH
-#define DENSE_BATCH 4              // superset entries a thread loads at once
-#endif
+constexpr int DENSE_BATCH = 6;   // superset entries a thread loads at once
 
 __device__ __forceinline__ void bitonic_inner(int64_t* ke, int n, int lane) {
     for (int size = 2; size <= n; size <<= 1) {
@@ -1163,7 +1161,7 @@ __global__ __launch_bounds__(64) void k_sup_dense(DS S) {
             }
         }
         if (lane == 0) {
-            if (FILL) S.x0[S.cval[k]] = make_double2(me.x, me.y);
+            if (FILL) S.x0[k] = make_double2(me.x, me.y);
             else S.scnt[k] = n;
         }
     }
@@ -1219,7 +1217,13 @@ __global__ __launch_bounds__(64) void k_cand_dense(DS S) {
     if (lane == 0) nnear = 0;
     const int m = act ? me.member : 0;
     const DevParams& P = S.params[m];
-    const double rad = me.r + P.rtree;
+    // the parameters the scan reads, held in registers (the list stores
+    // below could otherwise alias them)
+    const double rtree = P.rtree, bond_lim = P.bond_lim;
+    const uint64_t bseed = P.bond_seed;
+    const uint32_t bthr = P.bond_thr24;
+    const bool shelter = P.shelter != 0;
+    const double rad = me.r + rtree;
     const int f = me.id;
     const int first = S.members[m].first;
     const bool bond_step = act && S.members[m].tstep == 1 && P.restart != 1 && P.cohesion;
@@ -1315,7 +1319,7 @@ __global__ __launch_bounds__(64) void k_cand_dense(DS S) {
             const bool outer = v0 + u < nout;
             const double dx = me.x - o.x, dy = me.y - o.y;
             const double dist2 = dx * dx + dy * dy;
-            const double thr = outer ? rad : o.r + P.rtree;
+            const double thr = outer ? rad : o.r + rtree;
             // far beyond the radius: no square root needed
             if (dist2 > thr * thr * (1.0 + 1e-12)) continue;
             const double d = sqrt(dist2);
@@ -1324,10 +1328,10 @@ __global__ __launch_bounds__(64) void k_cand_dense(DS S) {
             // inner: pair (j, f) with f the inner floe j
             const double deltan = outer ? me.r + o.r - d : o.r + me.r - d;
             if (deltan > 0) qc = true;
-            if (outer && bond_step && deltan >= -P.bond_lim &&
-                bond_keep(f - first, ent_id(ev[u]) - first, P.bond_seed, P.bond_thr24))
+            if (outer && bond_step && deltan >= -bond_lim &&
+                bond_keep(f - first, ent_id(ev[u]) - first, bseed, bthr))
                 qb = true;
-            if (P.shelter) {
+            if (shelter) {
                 const T dn = T(deltan);
                 const bool near = !(dn < T(0) && T((double)0.18f) * -dn >= lim * T(msh.hfa)) ||
                                   !(dn < T(0) && T((double)0.18f) * -dn >= lim * T(msh.hfw));
@@ -2644,7 +2648,7 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_final(DS S) {
         // than 0.45 skin since the last build (a pair not in a superset was
         // more than its query radius + skin apart then, so it stays outside
         // the query radius while both floes stay within half the skin)
-        const double2 p0 = S.x0[f];
+        const double2 p0 = S.x0[k];
         const double ex = out.x - p0.x, ey = out.y - p0.y, lim = 0.45 * P.skin;
         // one store per wavefront (every such floe writing the one word
         // would queue the writes behind each other)
