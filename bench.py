@@ -100,6 +100,7 @@ def run_batched(args, batch):
     else:
         blocks = ctx.autotune(2)      # real steps: part of the warm-up
     ctx.sync()
+    search = ctx.search()
     graphs = ctx.graphs_on()
     launch = "graph replay" if graphs else "one-by-one launches"
     # the timed window starts at step `window_start` of the run (the steps
@@ -127,15 +128,16 @@ def run_batched(args, batch):
     ctx.profile(False)
     ctx.close()
     return dict(nfloes=nfloes, members=len(batch), seconds=dt, stages=stages, blocks=blocks,
-                launch=launch, graphs=graphs, window=(w0, w0 + args.steps - 1),
+                launch=launch, graphs=graphs, search=search, window=(w0, w0 + args.steps - 1),
                 ncand=ncand, ncont=ncont, nbond=nbond, totals=tot,
                 alg_bytes_step=algorithmic_bytes(nfloes, ncand, ncont, nbond))
 
 
-def run_full(batch, blocks, graphs, precision, total=5000, chunk=500):
+def run_full(batch, blocks, graphs, search, precision, total=5000, chunk=500):
     """The configured run from step 1 on a fresh ensemble, with the block
-    sizes and launch mode the autotuner chose: wall time of every `chunk`
-    steps (device syncs around each) and the pairs per floe at its end."""
+    sizes, search shape and launch mode the autotuner chose: wall time of
+    every `chunk` steps (device syncs around each) and the pairs per floe at
+    its end."""
     from godar_amd.sim import Context
     nfloes = sum(f.n for f, _ in batch)
     ctx = Context(nfloes, len(batch), precision)
@@ -147,6 +149,7 @@ def run_full(batch, blocks, graphs, precision, total=5000, chunk=500):
                 ctx.set_stage_block(k, blocks[name])
             except Exception:
                 pass
+    ctx.set_search(**search)
     ctx.use_graphs(graphs)
     ctx.sync()
     phases = []
@@ -479,10 +482,11 @@ def main():
         "stages_ms_per_step": {k: v[0] / v[1] for k, v in stages.items()},
         "pair_totals_last_step": B["totals"],
         "block_sizes": B["blocks"],
+        "search_shape": B["search"],
         "launch_mode": B["launch"],
     }
     if not args.quick and args.full_steps > 0:
-        out["full_run"] = run_full(batch, B["blocks"], B["graphs"], args.precision,
+        out["full_run"] = run_full(batch, B["blocks"], B["graphs"], B["search"], args.precision,
                                    total=args.full_steps)
     if not args.quick:
         p0, ms0 = run_plain("config0", 500, 20)

@@ -35,7 +35,7 @@ EXPORTS = [
     "godar_get_floes", "godar_record_forces", "godar_get_forces", "godar_get_walls",
     "godar_get_pairs", "godar_get_bond_state", "godar_get_stats", "godar_set_block", "godar_use_graphs", "godar_graphs_on",
     "godar_set_stage_block", "godar_get_stage_blocks", "godar_autotune",
-    "godar_set_dense_search", "godar_set_repair", "godar_set_skin", "godar_checkpoint_size", "godar_checkpoint_save",
+    "godar_set_dense_search", "godar_set_repair", "godar_set_skin", "godar_set_search", "godar_get_search", "godar_checkpoint_size", "godar_checkpoint_save",
     "godar_member_restore",
     "godar_profile", "godar_get_profile", "godar_stage_name", "godar_get_totals",
     "godar_get_totals_noalloc",
@@ -106,6 +106,8 @@ def load(path: str = LIB_PATH):
         "godar_set_dense_search": (_i32, [_vp, _i32]),
         "godar_set_repair": (_i32, [_vp, _i32, _i32]),
         "godar_set_skin": (_i32, [_vp, _d]),
+        "godar_set_search": (_i32, [_vp, _d, _i32, _i32]),
+        "godar_get_search": (_i32, [_vp, _dp, _ip, _ip]),
         "godar_checkpoint_size": (_i32, [_vp, _i32, _lp]),
         "godar_checkpoint_save": (_i32, [_vp, _i32, ctypes.POINTER(GodarParams), _vp, _i64]),
         "godar_member_restore": (_i32, [_vp, _vp, _i64, _ip]),

@@ -142,7 +142,10 @@ struct godar_ctx {
     int dense_scan = 96;                // floes per 3x3 search above which a member is dense
     int repair_lanes = 64;              // repair events evaluated at once
     int repair_cap = 1024;              // pending repair keys held in LDS
-    double skin_frac = 0.1;             // superset skin of new members, x (max r + rtree)
+    double skin_frac = 0.1;             // superset skin, x (max r + rtree)
+    int thread_span = 1;                // cells searched on each side by the thread search
+    double tuned_cost_ms = 0.0;         // the autotuner's price of the chosen search shape
+    std::vector<double> rmax;           // largest radius of each member when it was added
     int64_t sup_cap = 0;                // superset pool entries
     int32_t built_qseg = -1;            // queue share the dense list was built with
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
@@ -646,6 +649,7 @@ int godar_create(int64_t max_floes, int32_t max_members, int32_t precision, goda
     c->members.assign(max_members, Member{0, 0, 1, 0, 0});
     c->dparams.assign(max_members, DevParams{});
     c->cell_n.assign(max_members, 0);
+    c->rmax.assign(max_members, 0.0);
     c->perm.assign(max_members, {});
     for (auto& m : c->members) m.active = 0;
     DS& d = c->d;
@@ -803,6 +807,59 @@ static int64_t bytes_per_floe(const godar_ctx* c) {
     return per;
 }
 
+// Search shape of a member of n floes whose largest radius is rmax (radii
+// only shrink, by ridging): the superset skin is skin_frac (rmax + rtree);
+// the member is dense when the 3x3 cells around a floe on a grid of cells
+// >= rmax + rtree + skin hold dense_scan floes or more (on average); the
+// cells are >= (rmax + rtree + skin) / span, span = 2 for the dense search
+// and thread_span for the thread search, so that the (2 span + 1)^2 cells
+// around a floe hold its whole superset.  Grids are coarsened to at most
+// 4n + 64 cells.
+void member_search(const godar_ctx* c, DevParams& dp, int n, double rmax, int64_t& gx, int64_t& gy) {
+    dp.skin = c->skin_frac * (rmax + dp.rtree);
+    auto grid = [&](int span) {
+        dp.cell = (rmax + dp.rtree + dp.skin) * 1.0000001 / span + 1e-9;
+        gx = std::max<int64_t>(1, (int64_t)std::floor(dp.nx / dp.cell));
+        gy = std::max<int64_t>(1, (int64_t)std::floor(dp.ny / dp.cell));
+        while (gx * gy > 4 * (int64_t)n + 64) { gx = (gx + 1) / 2; gy = (gy + 1) / 2; dp.cell *= 2; }
+    };
+    grid(1);
+    dp.dense = 9.0 * (double)n / (double)(gx * gy) >= (double)c->dense_scan ? 1 : 0;
+    dp.span = dp.dense ? 2 : c->thread_span;
+    if (dp.span != 1) grid(dp.span);
+    dp.gx = (int32_t)gx;
+    dp.gy = (int32_t)gy;
+}
+
+// Every member's search shape anew from the context's settings (skin,
+// thread span, dense threshold), with cell ranges laid out one after the
+// other; the next step builds the supersets.  Nothing changes if the cell
+// table cannot grow.
+int research_all(godar_ctx* c) {
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    std::vector<DevParams> dps = c->dparams;
+    std::vector<int32_t> cn(c->Mcap, 0);
+    int64_t base = 0;
+    for (int m = 0; m < c->Mcap; ++m) {
+        if (!c->members[m].active) continue;
+        int64_t gx, gy;
+        member_search(c, dps[m], c->members[m].n, c->rmax[m], gx, gy);
+        dps[m].cell_base = (int32_t)base;
+        cn[m] = (int32_t)(gx * gy);
+        base += gx * gy;
+        if (base >= ((int64_t)1 << 30)) return fail(c, GODAR_ERR_NOMEM, "search: too many cells");
+    }
+    int rc = ensure_cells(c, (int32_t)base);
+    if (rc != GODAR_OK) return rc;
+    c->dparams = dps;
+    c->cell_n = cn;
+    rc = upload_tables(c);
+    if (rc != GODAR_OK) return rc;
+    force_rebuild(c);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return GODAR_OK;
+}
+
 int godar_member_footprint(const godar_ctx* c, int32_t n, int64_t* bytes) {
     if (!c || !bytes || n < 0) return GODAR_ERR_ARG;
     *bytes = bytes_per_floe(c) * n;
@@ -855,29 +912,8 @@ int godar_member_add(godar_ctx* c, const godar_params* p, int32_t n, const doubl
     }
     DevParams dp;
     derive_params(*p, dp);
-    dp.skin = c->skin_frac * (rmax + p->rtree);
-    // cell >= (max query reach + skin) / span, so that (2 span + 1)^2 cells
-    // around a floe hold its whole superset.  Grids are coarsened to at most
-    // 4n+64 cells.
-    auto grid = [&](int span, double& cell, int64_t& gx, int64_t& gy) {
-        cell = (rmax + p->rtree + dp.skin) * 1.0000001 / span + 1e-9;
-        gx = std::max<int64_t>(1, (int64_t)std::floor(p->nx / cell));
-        gy = std::max<int64_t>(1, (int64_t)std::floor(p->ny / cell));
-        while (gx * gy > 4 * (int64_t)n + 64) { gx = (gx + 1) / 2; gy = (gy + 1) / 2; cell *= 2; }
-    };
     int64_t gx, gy;
-    grid(1, dp.cell, gx, gy);
-    dp.span = 1;
-    // floes examined per neighbour search (3x3 cells): from about two
-    // wavefronts' worth, a wavefront per floe beats a thread per floe, and
-    // a finer grid (5x5 cells of half the size) examines 30% fewer floes
-    dp.dense = 9.0 * (double)n / (double)(gx * gy) >= (double)c->dense_scan ? 1 : 0;
-    if (dp.dense) {
-        grid(2, dp.cell, gx, gy);
-        dp.span = 2;
-    }
-    dp.gx = (int32_t)gx;
-    dp.gy = (int32_t)gy;
+    member_search(c, dp, n, rmax, gx, gy);
     int32_t ncell = (int32_t)(gx * gy);
     std::vector<Range> cused;
     for (int k = 0; k < c->Mcap; ++k)
@@ -937,6 +973,7 @@ int godar_member_add(godar_ctx* c, const godar_params* p, int32_t n, const doubl
     std::vector<int32_t> mo(n, m);
     HIPCHK(c, hipMemcpy(d.member_of + first, mo.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
     c->dparams[m] = dp;
+    c->rmax[m] = rmax;
     c->members[m].first = (int32_t)first;
     c->members[m].n = n;
     c->members[m].tstep = tstep0;
@@ -1434,6 +1471,29 @@ int godar_set_skin(godar_ctx* c, double frac) {
     return GODAR_OK;
 }
 
+int godar_set_search(godar_ctx* c, double skin_frac, int32_t thread_span, int32_t dense_scan) {
+    if (!c || !(skin_frac >= 0.0) || skin_frac > 1.0 || thread_span < 1 || thread_span > 2 ||
+        dense_scan < 0)
+        return c ? fail(c, GODAR_ERR_ARG, "set_search: skin in [0, 1], span 1 or 2, threshold >= 0")
+                 : GODAR_ERR_ARG;
+    const double s0 = c->skin_frac;
+    const int t0 = c->thread_span, d0 = c->dense_scan;
+    c->skin_frac = skin_frac;
+    c->thread_span = thread_span;
+    c->dense_scan = dense_scan;
+    const int rc = research_all(c);
+    if (rc != GODAR_OK) { c->skin_frac = s0; c->thread_span = t0; c->dense_scan = d0; }
+    return rc;
+}
+
+int godar_get_search(const godar_ctx* c, double* skin_frac, int32_t* thread_span, int32_t* dense_scan) {
+    if (!c) return GODAR_ERR_ARG;
+    if (skin_frac) *skin_frac = c->skin_frac;
+    if (thread_span) *thread_span = c->thread_span;
+    if (dense_scan) *dense_scan = c->dense_scan;
+    return GODAR_OK;
+}
+
 int godar_set_repair(godar_ctx* c, int32_t lanes, int32_t pending) {
     if (!c || lanes < 1 || lanes > 64 || pending < 1 || pending > 1024) return GODAR_ERR_ARG;
     c->repair_lanes = lanes;
@@ -1453,6 +1513,99 @@ int godar_get_stage_blocks(const godar_ctx* c, int32_t* threads) {
     if (!c || !threads) return GODAR_ERR_ARG;
     for (int k = 0; k < GODAR_NSTAGES; ++k) threads[k] = c->sblock[k];
     return GODAR_OK;
+}
+
+// Steps between superset builds the fastest floe allows at skin fraction
+// `frac` (a build follows once a floe has moved 0.45 skin from where it was
+// at the last one), from the floes' current speeds; at least 1, at most 5,000.
+int build_period(godar_ctx* c, double frac, double* period) {
+    const int N = c->N_used;
+    std::vector<KinRec> kin(std::max(N, 1));
+    std::vector<GeoRec> geo(std::max(N, 1));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(kin.data(), c->d.kin, sizeof(KinRec) * N, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(geo.data(), c->d.geo, sizeof(GeoRec) * N, hipMemcpyDeviceToHost));
+    std::vector<double> vmax(c->Mcap, 0.0);
+    for (int k = 0; k < N; ++k) {
+        const int m = geo[k].member;
+        if (m < 0 || m >= c->Mcap) continue;
+        vmax[m] = std::max(vmax[m], std::sqrt(kin[k].u * kin[k].u + kin[k].v * kin[k].v));
+    }
+    double p = 5000.0;
+    for (int m = 0; m < c->Mcap; ++m) {
+        if (!c->members[m].active || !(vmax[m] > 0.0)) continue;
+        const double skin = frac * (c->rmax[m] + c->dparams[m].rtree);
+        p = std::min(p, 0.45 * skin / (vmax[m] * c->dparams[m].dt));
+    }
+    *period = std::max(1.0, p);
+    return GODAR_OK;
+}
+
+// Search shape: the superset skin (so the cell size), the thread search's
+// cell span and the dense-search threshold, one after the other.  Each
+// candidate is priced as a build step plus `4 trial` steps after it: the
+// mean step cost over a build and the steps until the next one,
+// t_step + (t_build - t_step) / period.  The settings stay where they were
+// if no candidate is cheaper.
+int tune_search(godar_ctx* c, int trial) {
+    const int n = std::max(4, 4 * trial);
+    auto stage_sum = [&]() {
+        double t = 0.0;
+        for (int st = 0; st < GODAR_NSTAGES; ++st) t += c->stage_ms[st];
+        return t;
+    };
+    auto reset = [&]() {
+        for (int st = 0; st < GODAR_NSTAGES; ++st) { c->stage_ms[st] = 0.0; c->stage_n[st] = 0; }
+    };
+    auto price = [&](double frac, int span, int dscan, double& cost) -> int {
+        int rc = godar_set_search(c, frac, span, dscan);
+        if (rc != GODAR_OK) return rc;
+        rc = godar_profile(c, 1);
+        if (rc == GODAR_OK) { reset(); rc = godar_step(c, 1); }   // the build step
+        if (rc == GODAR_OK) rc = godar_sync(c);
+        if (rc != GODAR_OK) return rc;
+        collect_events(c);
+        const double tb = stage_sum();
+        reset();
+        rc = godar_step(c, n);
+        if (rc == GODAR_OK) rc = godar_sync(c);
+        if (rc != GODAR_OK) return rc;
+        collect_events(c);
+        const double ts = stage_sum() / n;
+        double per = 1.0;
+        rc = build_period(c, frac, &per);
+        if (rc != GODAR_OK) return rc;
+        cost = ts + std::max(0.0, tb - ts) / per;
+        return GODAR_OK;
+    };
+    double best_frac = c->skin_frac;
+    int best_span = c->thread_span, best_dscan = c->dense_scan;
+    double best = 1e300;
+    int rc = price(best_frac, best_span, best_dscan, best);
+    if (rc != GODAR_OK) return rc;
+    for (double f : {0.05, 0.1, 0.2}) {
+        if (f == best_frac) continue;
+        double cost;
+        rc = price(f, best_span, best_dscan, cost);
+        if (rc != GODAR_OK) return rc;
+        if (cost < best) { best = cost; best_frac = f; }
+    }
+    for (int sp : {1, 2}) {
+        if (sp == best_span) continue;
+        double cost;
+        rc = price(best_frac, sp, best_dscan, cost);
+        if (rc != GODAR_OK) return rc;
+        if (cost < best) { best = cost; best_span = sp; }
+    }
+    for (int ds : {48, 96, 192}) {
+        if (ds == best_dscan) continue;
+        double cost;
+        rc = price(best_frac, best_span, ds, cost);
+        if (rc != GODAR_OK) return rc;
+        if (cost < best) { best = cost; best_dscan = ds; }
+    }
+    c->tuned_cost_ms = best;
+    return godar_set_search(c, best_frac, best_span, best_dscan);
 }
 
 // Coordinate search over the block size of each tunable stage: every
@@ -1481,6 +1634,8 @@ int godar_autotune(godar_ctx* c, int32_t trial) {
         }
         c->sblock[st] = best;
     }
+    rc = tune_search(c, trial);
+    if (rc != GODAR_OK) { c->prof = prof0; return rc; }
     int r2 = godar_profile(c, prof0 ? 1 : 0);
     if (r2 != GODAR_OK || prof0 || !c->use_graphs) return r2;
     // launch mode: graph replay or one-by-one launches, whichever steps faster

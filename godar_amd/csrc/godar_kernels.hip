@@ -914,8 +914,8 @@ __device__ __forceinline__ void cand_thread(const DS& S, int k, const GeoRec& me
     cand_finish<MIXED>(S, k, fs, P, no, ni, double(sha), double(shw), qc, qb, 0, 1);
 }
 
-// Superset build, thread per floe (members that are not dense; they have
-// span 1, godar_member_add): counts (FILL false) or entries and the build
+// Superset build, thread per floe (members that are not dense; span 1 or
+// 2 cells on each side): counts (FILL false) or entries and the build
 // positions x0 (FILL true).  The count pass lists the dense members' floes in
 // queue 3 for k_sup_dense and k_cand_dense.  Only on build steps.
 template <bool FILL>
@@ -932,10 +932,12 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_sup(DS S) {
             if (P.dense) {
                 qd = !FILL;
             } else if (FILL) {
-                sup_thread<1, true>(S, k, me, P, S.sup + S.soff[k]);
+                if (P.span == 2) sup_thread<2, true>(S, k, me, P, S.sup + S.soff[k]);
+                else sup_thread<1, true>(S, k, me, P, S.sup + S.soff[k]);
                 S.x0[k] = make_double2(me.x, me.y);
             } else {
-                S.scnt[k] = sup_thread<1, false>(S, k, me, P, nullptr);
+                S.scnt[k] = P.span == 2 ? sup_thread<2, false>(S, k, me, P, nullptr)
+                                        : sup_thread<1, false>(S, k, me, P, nullptr);
             }
         }
     }

@@ -30,6 +30,20 @@ __device__ __forceinline__ double gexp(double a) { return crm::cr_exp(a); }
 __device__ __forceinline__ float gexp(float a) { return expf(a); }
 __device__ __forceinline__ double glog(double a) { return crm::cr_log(a); }
 __device__ __forceinline__ float glog(float a) { return logf(a); }
+// acos/asin of the ridging lens area (FP64 in both precisions): correctly
+// rounded in FP64 runs; in mixed precision the device library's (within an
+// ulp or two), which the FP32 force laws swamp (r^2 h stays conserved
+// whatever the area: r' = r sqrt(h / (h + dh)), h' = h + dh)
+template <bool MIXED>
+__device__ __forceinline__ double ridge_acos(double a) {
+    if constexpr (MIXED) return acos(a);
+    else return crm::cr_acos(a);
+}
+template <bool MIXED>
+__device__ __forceinline__ double ridge_asin(double a) {
+    if constexpr (MIXED) return asin(a);
+    else return crm::cr_asin(a);
+}
 __device__ __forceinline__ double gabs(double a) { return fabs(a); }
 __device__ __forceinline__ float gabs(float a) { return fabsf(a); }
 __device__ __forceinline__ double gcopysign(double a, double b) { return copysign(a, b); }
@@ -190,8 +204,8 @@ __device__ __forceinline__ void contact_part(const DevParams& P, Floe& fi, Floe&
             double ri = fi.r, rj = fj.r;
             double dij = (dist * dist - rj * rj + ri * ri) / (2.0 * dist);
             double dji = (dist * dist - ri * ri + rj * rj) / (2.0 * dist);
-            double area = ri * ri * crm::cr_acos(dij / ri) - dij * delt_ridge / 2.0
-                        + rj * rj * crm::cr_acos(dji / rj) - dji * delt_ridge / 2.0;
+            double area = ri * ri * ridge_acos<MIXED>(dij / ri) - dij * delt_ridge / 2.0
+                        + rj * rj * ridge_acos<MIXED>(dji / rj) - dji * delt_ridge / 2.0;
             double vol = fmax(area * hmin64 / 2.0, 0.0);
             // the thinner floe is reshaped; its six properties are selected
             // field by field (a pointer to either floe, or a select of whole
@@ -443,7 +457,7 @@ __device__ __forceinline__ void contact_bc(const DevParams& P, Floe& f, int dir1
             // update_shape_bc (FP64)
             double drb = delt_ridge_bc;
             double hh = f.h;
-            double area = ri * ri * crm::cr_asin(drb / 2.0 / ri) - drb * (ri - deltan_bc) / 2.0;
+            double area = ri * ri * ridge_asin<MIXED>(drb / 2.0 / ri) - drb * (ri - deltan_bc) / 2.0;
             double vol = fmax(area * hh, 0.0);
             double dh = vol / (P.pi * (ri * ri));
             f.r = ri * sqrt(hh / (hh + dh));

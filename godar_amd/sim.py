@@ -182,6 +182,20 @@ class Context:
         Results do not depend on it."""
         self._chk(self.L.godar_set_skin(self.h, float(frac)))
 
+    def set_search(self, skin_frac: float = 0.1, thread_span: int = 1, dense_scan: int = 96):
+        """Search shape of the current members (and later ones): superset
+        skin, cells the thread search scans on each side, dense-search
+        threshold.  The next step builds the supersets; results do not depend
+        on any of them."""
+        self._chk(self.L.godar_set_search(self.h, float(skin_frac), int(thread_span), int(dense_scan)))
+
+    def search(self) -> dict:
+        f = ctypes.c_double(0.0)
+        sp = ctypes.c_int32(0)
+        ds = ctypes.c_int32(0)
+        self._chk(self.L.godar_get_search(self.h, ctypes.byref(f), ctypes.byref(sp), ctypes.byref(ds)))
+        return dict(skin_frac=f.value, thread_span=sp.value, dense_scan=ds.value)
+
     def set_repair(self, lanes: int = 64, pending: int = 1024):
         """Ridging repair: events evaluated at once (1: one at a time) and
         pending events kept on chip.  Results do not depend on either."""
@@ -196,8 +210,10 @@ class Context:
         return {self.L.godar_stage_name(k).decode(): int(t[k]) for k in range(_lib.NSTAGES)}
 
     def autotune(self, trial: int = 3) -> dict:
-        """Pick each stage's block size, then the launch mode (graph replay or
-        one-by-one launches), by timing real steps (members advance)."""
+        """Pick each stage's block size, then the search shape (skin, thread
+        span, dense threshold), then the launch mode (graph replay or
+        one-by-one launches), by timing real steps (members advance).
+        Returns the stage block sizes; `search()` holds the search shape."""
         self._chk(self.L.godar_autotune(self.h, int(trial)))
         return self.stage_blocks()
 

@@ -219,9 +219,10 @@ int godar_get_totals_noalloc(godar_ctx *ctx, int64_t *totals);
 int godar_set_block(godar_ctx *ctx, int32_t threads);
 int godar_set_stage_block(godar_ctx *ctx, int32_t stage, int32_t threads);
 /* Members added afterwards whose neighbour search examines at least `floes`
- * floes per floe (3x3 cells at their grid's mean density; default 96) are
- * searched by one wavefront per floe instead of one thread (0: every member,
- * a huge value: none).  Results do not depend on it. */
+ * floes per floe (3x3 cells at their grid's mean density; default 96) take
+ * the dense search (supersets ordered by distance at each build, scanned by
+ * a thread per floe) instead of the thread search (0: every member, a huge
+ * value: none).  Results do not depend on it. */
 int godar_set_dense_search(godar_ctx *ctx, int32_t floes);
 /* Ridging repair: the number of pending events one member evaluates at once
  * (1-64, default 64; 1: one at a time) and the pending events kept in on-chip
@@ -234,13 +235,27 @@ int godar_set_repair(godar_ctx *ctx, int32_t lanes, int32_t pending);
  * that skin (or after members change); the other steps take their candidate
  * lists from the supersets.  Results do not depend on it. */
 int godar_set_skin(godar_ctx *ctx, double frac);
+/* Search shape of the current members and of those added afterwards: the
+ * superset skin `skin_frac` x (max r + rtree) (0-1), the cells the thread
+ * search scans on each side of a floe at a build (1 or 2: cells of 1/span of
+ * the superset reach) and the dense-search threshold (as
+ * godar_set_dense_search).  The next step builds the supersets.  Results do
+ * not depend on any of them. */
+int godar_set_search(godar_ctx *ctx, double skin_frac, int32_t thread_span, int32_t dense_scan);
+int godar_get_search(const godar_ctx *ctx, double *skin_frac, int32_t *thread_span,
+                     int32_t *dense_scan);
 int godar_get_stage_blocks(const godar_ctx *ctx, int32_t *threads);   /* GODAR_NSTAGES entries */
 /* Autotuner: for each tunable stage in turn, time `trial` real steps of the
- * current members with each block size and keep the fastest.  Then, unless
- * graphs are off (godar_use_graphs) or steps are being profiled, time
- * max(4, 4 trial) steps in each launch mode and keep the faster one.  The
- * members advance by 1 + 3 * trial * (tunable stages) steps, plus
- * 2 * (1 + max(4, 4 trial)) for the launch mode. */
+ * current members with each block size and keep the fastest.  Then the
+ * search shape (godar_set_search): skin 0.05, 0.1, 0.2, thread span 1, 2 and
+ * dense threshold 48, 96, 192 in turn, each priced by a build step and
+ * max(4, 4 trial) steps after it as t_step + (t_build - t_step) / period,
+ * the period being the steps until the fastest floe (at its current speed)
+ * has moved 0.45 skin, at most 5,000.  Then, unless graphs are off
+ * (godar_use_graphs) or steps are being profiled, time max(4, 4 trial) steps
+ * in each launch mode and keep the faster one.  The members advance by
+ * 1 + 3 * trial * (tunable stages) steps, at most 6 (1 + max(4, 4 trial))
+ * for the search shape and 2 (1 + max(4, 4 trial)) for the launch mode. */
 int godar_autotune(godar_ctx *ctx, int32_t trial);
 /* Step launch mode (no counterpart in the reference): on (the default), each
  * step replays a captured HIP graph of the whole kernel sequence, recaptured
