@@ -25,6 +25,7 @@ __global__ void k_scan_total(DS, int);
 __global__ void k_scan_offsets(DS);
 template <bool M> __global__ void k_cand_dense(DS);
 template <bool M> __global__ void k_cand_dense_over(DS);
+__global__ void k_comp(DS);
 template <bool M> __global__ void k_contact(DS);
 template <bool M> __global__ void k_bond(DS);
 __global__ void k_stress(DS);
@@ -517,6 +518,8 @@ int launch_step(godar_ctx* c) {
     }
     {
         StageTimer t(c, 8);
+        // contact components (roots), then the ordered repair
+        hipLaunchKernelGGL(k_comp, grid_for(N, FLOE_BLOCK, true), dim3(FLOE_BLOCK), 0, s, d);
         hipLaunchKernelGGL(mixed ? k_repair<true> : k_repair<false>, dim3(nm), dim3(64), 0, s, d, nm);
     }
     {
@@ -711,8 +714,9 @@ int godar_create(int64_t max_floes, int32_t max_members, int32_t precision, goda
     ACHK(dalloc(&d.wtab, S));
     ACHK(dalloc(&d.nwtab, max_members));
     ACHK(dalloc(&d.ovw, S));
-    ACHK(dalloc(&d.cur, 6 * S));
-    ACHK(dalloc(&d.dirty, S));
+    ACHK(dalloc(&d.cur, 7 * S));
+    ACHK(dalloc(&d.dirty, (S + 3) / 4 * 4));   // whole 32-bit words (dirty_set)
+    ACHK(dalloc(&d.comp, S));
     ACHK(dalloc(&d.wnz, S));
     ACHK(dalloc(&d.stats, 4 * (int64_t)max_members));
     ACHK(dalloc(&d.tcur, max_members));
@@ -735,7 +739,7 @@ int godar_create(int64_t max_floes, int32_t max_members, int32_t precision, goda
     (void)hipMemsetAsync(d.qn, 0, sizeof(int32_t) * NQUEUE * XCD_COUNT * QN_STRIDE, c->stream);
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, c->stream, d.rebuild, 2, 1);
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, c->stream, d.rebuild + 2, 1, 0);
-    (void)hipMemsetAsync(d.dirty, 0, S, c->stream);
+    (void)hipMemsetAsync(d.dirty, 0, (S + 3) / 4 * 4, c->stream);
     (void)hipMemsetAsync(d.wnz, 1, S, c->stream);
     (void)hipMemsetAsync(d.stats, 0, sizeof(int64_t) * 4 * max_members, c->stream);
     (void)hipMemsetAsync(d.tcur, 0, sizeof(int32_t) * max_members, c->stream);
@@ -774,7 +778,7 @@ int godar_destroy(godar_ctx* c) {
         set_lists(c, L);
     }
     dfree(d.wout); dfree(d.wflag); dfree(d.cls); dfree(d.qlist); dfree(d.qn); dfree(d.bgat); dfree(d.seeds); dfree(d.nseeds); dfree(d.heap); dfree(d.etab);
-    dfree(d.netab); dfree(d.wtab); dfree(d.nwtab); dfree(d.ovw); dfree(d.dirty);
+    dfree(d.netab); dfree(d.wtab); dfree(d.nwtab); dfree(d.ovw); dfree(d.dirty); dfree(d.comp);
     dfree(d.wnz); dfree(d.stats); dfree(d.tcur); dfree(d.err); dfree(d.steps_done);
     dfree(d.sup); dfree(d.scnt); dfree(d.sout); dfree(d.soff); dfree(d.sblk); dfree(d.x0); dfree(d.rebuild);
     radix_free(c->rs);
@@ -800,7 +804,7 @@ static int64_t bytes_per_floe(const godar_ctx* c) {
     per += (int64_t)c->BC * sizeof(BondTerms) + 1 /*bgat*/;
     per += NACC * 8 + sizeof(WallOut) + 1 /*wflag*/ + 1 /*cls*/ + 4 * 4 /*qlist*/;
     per += (int64_t)c->SEEDK * 8 + (int64_t)c->HEAPK * 8 + (int64_t)c->TABK * sizeof(EventOut);
-    per += sizeof(WallOut) /*wtab*/ + 4 /*ovw*/ + 6 * 8 /*cur*/ + 1 + 1 /*dirty, wnz*/;
+    per += sizeof(WallOut) /*wtab*/ + 4 /*ovw*/ + 7 * 8 /*cur*/ + 1 + 1 /*dirty, wnz*/ + 4 /*comp*/;
     per += 33 * 8 /*fout*/ + 4 + 4 /*radix keys, values*/;
     per += (c->S > 0 ? c->sup_cap / c->S : 0) * 8 /*superset pool share*/ + 4 + 4 + 8 + 16 /*scnt sout soff x0*/;
     per += 1 /*sblk, per SCAN_TILE floes*/;

@@ -51,6 +51,10 @@ struct KinRec { double u, v, omega, h, mass, inertia, theta; };
 // holds its terms); the repair recomputed its walls (wtab entry in ovw); the
 // repair touched the floe, so k_fixup re-gathers it.
 constexpr uint32_t WF_WALL = 1, WF_REPAIR_WALL = 2, WF_FIXUP = 4;
+// dirty bits (by storage slot, during the repair): the floe was reshaped (cur
+// rows 0-5 hold its properties); its iteration's query radius is kept (cur
+// row 6).  k_final clears them.
+constexpr uint32_t DIRTY_SHAPE = 1, DIRTY_RQ = 2;
 
 // Everything the kernels touch.  "id" arrays are indexed by a floe's storage
 // slot; "pos" arrays by the floe's position in this step's cell order.  Per-floe lists
@@ -150,8 +154,9 @@ struct DS {
     int32_t* netab;
     WallOut* wtab;             // one per floe slot
     int32_t* nwtab;
-    double* cur;               // [6][S] r h mass inertia hfa hfw during repair
-    uint8_t* dirty;            // [S]
+    double* cur;               // [7][S] r h mass inertia hfa hfw, query radius during repair
+    uint8_t* dirty;            // [S rounded up to 4] DIRTY_* bits
+    int32_t* comp;             // [S] by pos: contact component (union-find parent)
     uint8_t* wnz;              // [S] by storage slot: 0 = the floe's wall history rows (wbc)
                                //     are all +0.0 (k_final then skips rewriting them)
     int64_t* stats;            // [members][4]

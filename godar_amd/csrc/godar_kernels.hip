@@ -141,13 +141,42 @@ __device__ __forceinline__ void wflag_set(const DS& S, int k, uint32_t bits) {
     atomicOr(reinterpret_cast<uint32_t*>(S.wflag + (k & ~3)), bits << (8 * (k & 3)));
 }
 
+// Contact components of the step (union-find over the speculative contacts,
+// at positions): k_cand makes every floe its own root, k_contact unites the
+// floes of each contact, hooking the larger root under the smaller with a
+// compare-and-swap (a root's parent changes only there), and k_comp points
+// every contact floe at its root.  Parents are read and written at device
+// scope: other wavefronts change them during k_contact.
+__device__ __forceinline__ int comp_parent(const int32_t* p, int x) {
+    return __hip_atomic_load(p + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ int comp_find(int32_t* p, int x) {
+    for (;;) {
+        const int px = comp_parent(p, x);
+        if (px == x) return x;
+        const int ppx = comp_parent(p, px);
+        // path halving: any ancestor may stand in for the parent
+        if (ppx != px) __hip_atomic_store(p + x, ppx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        x = px;
+    }
+}
+__device__ __forceinline__ void comp_unite(int32_t* p, int a, int b) {
+    for (;;) {
+        a = comp_find(p, a);
+        b = comp_find(p, b);
+        if (a == b) return;
+        if (a < b) { const int t = a; a = b; b = t; }
+        if (atomicCAS(p + a, a, b) == a) return;
+    }
+}
+
 // floe at storage slot f and position k with the properties current during
 // the ordered repair
 __device__ __forceinline__ Floe load_floe_cur(const DS& S, int f, int k) {
     // the current rows are read with the records (one round of loads) and
     // taken where the floe was reshaped
     Floe F = floe_at(S, k);
-    const bool d = S.dirty[f] != 0;
+    const bool d = (S.dirty[f] & DIRTY_SHAPE) != 0;
     const double r = S.cur[0 * S.S + f], h = S.cur[1 * S.S + f], mass = S.cur[2 * S.S + f];
     const double inertia = S.cur[3 * S.S + f], hfa = S.cur[4 * S.S + f], hfw = S.cur[5 * S.S + f];
     F.r = d ? r : F.r; F.h = d ? h : F.h; F.mass = d ? mass : F.mass;
@@ -479,6 +508,7 @@ __device__ __forceinline__ void cand_finish(const DS& S, int k, int fs, const De
         if (!qb) S.bcnt_new[fs] = 0;
         S.wflag[k] = 0;
         S.bgat[k] = 0;
+        S.comp[k] = k;
         S.cls[k] = qc ? CLS_CONTACT : 0;   // the fix-up sets CLS_BOND
     }
 }
@@ -1539,6 +1569,7 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_contact(DS S) {
         }
         PairState ps;
         load_contact_state(S, fs, j, ps);
+        comp_unite(S.comp, k, qj);
         bond_create(P, tstep, f - first, j - first, Fi, Fj, kn, ps);
         EventOut o;
         o.flags = 0;
@@ -1871,6 +1902,28 @@ __device__ __forceinline__ void set_cur(const DS& S, int g, double r, double h, 
     S.cur[3 * S.S + g] = inertia; S.cur[4 * S.S + g] = hfa; S.cur[5 * S.S + g] = hfw;
 }
 
+// Sets bits of storage slot f's dirty byte (the word holds three other
+// floes' bytes, which other lanes may be setting; the array is allocated in
+// whole words).
+__device__ __forceinline__ void dirty_set(const DS& S, int f, uint32_t bits) {
+    atomicOr(reinterpret_cast<uint32_t*>(S.dirty + (f & ~3)), bits << (8 * (f & 3)));
+}
+
+// Query radius of the floe at slot fs (position kf) for the events of its
+// own iteration: its radius at the top of that iteration.  The first
+// committed event of the iteration keeps it (cur row 6); until then no event
+// of the iteration has reshaped the floe, so it is the current radius.
+__device__ __forceinline__ double block_rq(const DS& S, int fs, int kf) {
+    const uint8_t d = S.dirty[fs];
+    return (d & DIRTY_RQ) ? S.cur[6 * S.S + fs] : ((d & DIRTY_SHAPE) ? S.cur[fs] : S.geo[kf].r);
+}
+__device__ __forceinline__ void keep_block_rq(const DS& S, int fs, double rq) {
+    if (!(S.dirty[fs] & DIRTY_RQ)) {
+        S.cur[6 * S.S + fs] = rq;
+        dirty_set(S, fs, DIRTY_RQ);
+    }
+}
+
 // slot of partner id g in the outer list of the floe at position kp
 __device__ __forceinline__ int slot_of(const DS& S, int kp, int g) {
     return find_entry(S.oj, S.S, kp, S.ocnt[kp], g);
@@ -1982,8 +2035,6 @@ __device__ __forceinline__ void put_bond_lanes(const DS& S, bool on, int i, int 
 // Progress of one member's repair (the same in every lane).
 struct RepairState {
     int64_t last;        // key of the last event recomputed (-1: none)
-    int cur_block;       // f - first of the last pair event
-    double rq;           // query radius of that floe at the top of its iteration
     int nt, nwt;         // repair table entries used
     int64_t repaired;
 };
@@ -2010,6 +2061,15 @@ __device__ __forceinline__ void push_later_events(const DS& S, Heap& H, int firs
     for (int s2 = s0; s2 < gno; ++s2)
         if (!H.push(((uint64_t)gb << 16) | (uint64_t)s2)) over |= ERR_REPAIR_CAP;
     if (!H.push(((uint64_t)gb << 16) | WALL_SLOT)) over |= ERR_REPAIR_CAP;
+}
+
+// Every contact floe (queue 0) points at the root of its contact component
+// (no unions run any more).  Every floe of a contact is in the queue.
+__global__ __launch_bounds__(FLOE_BLOCK) void k_comp(DS S) {
+    if (*S.err) return;
+    const int k = queue_entry(S, 0);
+    if (k < 0) return;
+    S.comp[k] = comp_find(S.comp, k);
 }
 
 // One event at a time off the binary heap in HBM (single lane).
@@ -2041,14 +2101,12 @@ __device__ void repair_serial(const DS& S, const DevParams& P, int first, int nm
             wflag_set(S, kf, WF_REPAIR_WALL | WF_FIXUP);
             if (W.flags & 4u) {
                 set_cur(S, fs, W.nr, W.nh, W.nmass, W.ninertia, W.nhfa, W.nhfw);
-                S.dirty[fs] = 1;
+                dirty_set(S, fs, DIRTY_SHAPE);
             }
             continue;
         }
-        if (b != st.cur_block) {
-            st.cur_block = b;
-            st.rq = S.dirty[fs] ? S.cur[fs] : S.geo[kf].r;   // query radius at the top of iteration f
-        }
+        const double rq = block_rq(S, fs, kf);   // query radius at the top of iteration f
+        keep_block_rq(S, fs, rq);
         const int64_t e = S.oj[(int64_t)s * SS + kf];
         const int j = ent_id(e);
         const int js = S.cval[ent_pos(e)];
@@ -2061,7 +2119,7 @@ __device__ void repair_serial(const DS& S, const DevParams& P, int first, int nm
         const double d = sqrt(dx * dx + dy * dy);
         PairState ps;
         load_pair_state(S, fs, j, ps);
-        if (!(d <= st.rq + P.rtree)) {
+        if (!(d <= rq + P.rtree)) {
             // no longer a candidate: the pair keeps its old state
             o.flags = EV_REMOVED;
             o.ridged = -1;
@@ -2076,8 +2134,8 @@ __device__ void repair_serial(const DS& S, const DevParams& P, int first, int nm
             const int gs = o.ridged == 0 ? fs : js;
             const int kg = o.ridged == 0 ? kf : ent_pos(e);
             set_cur(S, gs, o.nr, o.nh, o.nmass, o.ninertia, o.nhfa, o.nhfw);
-            if (!S.dirty[gs]) {
-                S.dirty[gs] = 1;
+            if (!(S.dirty[gs] & DIRTY_SHAPE)) {
+                dirty_set(S, gs, DIRTY_SHAPE);
                 push_later_events(S, H, first, f, s, g, kg, over);
             }
         }
@@ -2117,6 +2175,10 @@ template <bool MIXED>
 __global__ __launch_bounds__(64) void k_repair(DS S, int nmembers) {
     __shared__ int64_t pk[2][REPAIR_PCAP];
     __shared__ int64_t nk[REPAIR_NCAP];
+    // each lane's result until the round decides whether it commits (one
+    // EventOut-sized slot per lane; a wall lane's WallOut uses its start)
+    __shared__ uint64_t stage_buf[64 * sizeof(EventOut) / 8];
+    __shared__ int tcomp[64];      // components of a round's reshaping lanes so far
     const int m = blockIdx.x;
     if (m >= nmembers || *S.err) return;
     if (!member_live(S.members[m])) return;
@@ -2135,12 +2197,14 @@ __global__ __launch_bounds__(64) void k_repair(DS S, int nmembers) {
     const int tcap = S.TABK * nmem;
     WallOut* wtab = S.wtab + first;
     RepairState st;
-    st.last = -1; st.cur_block = -1; st.rq = 0.0; st.nt = 0; st.nwt = 0; st.repaired = 0;
+    st.last = -1; st.nt = 0; st.nwt = 0; st.repaired = 0;
     uint32_t over = 0;
     Heap H;
     H.a = S.heap + (int64_t)S.HEAPK * first;
     H.n = 0;
     H.cap = S.HEAPK * nmem;
+    EventOut* stage_e = reinterpret_cast<EventOut*>(stage_buf);
+    WallOut& stage_w_mine = *reinterpret_cast<WallOut*>(stage_e + threadIdx.x);
     const int lanes = min(max(S.repair_lanes, 1), 64);
     const int cap = min(max(S.repair_cap, 1), REPAIR_PCAP);
     bool serial = lanes == 1 || ns > cap;
@@ -2173,10 +2237,7 @@ __global__ __launch_bounds__(64) void k_repair(DS S, int nmembers) {
         const int f = first + b;
         const bool wall = act && s == WALL_SLOT;
         const bool pair = act && s != WALL_SLOT;
-        const uint64_t mw = __ballot(wall), me = __ballot(pair);
-        const int rank_w = __popcll(mw & lt), rank_e = __popcll(me & lt);
-        if (st.nt + __popcll(me) > tcap || st.nwt + __popcll(mw) > nmem) { over |= ERR_REPAIR_CAP; break; }
-        int fs = 0, kf = 0, j = 0, js = 0, kj = 0, ridged = -1;
+        int fs = 0, kf = 0, j = 0, js = 0, kj = 0, ridged = -1, ca = -1, cbm = -1;
         bool reshape = false;
         uint32_t oflags = 0;
         double rql = 0.0, nr = 0.0, nh = 0.0, nmass = 0.0, ninertia = 0.0, nhfa = 0.0, nhfw = 0.0;
@@ -2186,20 +2247,23 @@ __global__ __launch_bounds__(64) void k_repair(DS S, int nmembers) {
             kf = S.pos_id[f];
         }
         if (wall) {
+            ca = cbm = S.comp[kf];
             Floe F = load_floe_cur(S, fs, kf);
             double hist[4] = {S.wbc[0 * SS + fs], S.wbc[1 * SS + fs], S.wbc[2 * SS + fs], S.wbc[3 * SS + fs]};
             WallOut W;
             walls<MIXED>(P, F, hist, W);
-            wtab[st.nwt + rank_w] = W;
+            stage_w_mine = W;
             reshape = (W.flags & 4u) != 0;
             nr = W.nr; nh = W.nh; nmass = W.nmass; ninertia = W.ninertia; nhfa = W.nhfa; nhfw = W.nhfw;
         }
         if (pair) {
-            rql = b == st.cur_block ? st.rq : (S.dirty[fs] ? S.cur[fs] : S.geo[kf].r);
+            rql = block_rq(S, fs, kf);
             const int64_t e = S.oj[(int64_t)s * SS + kf];
             j = ent_id(e);
             kj = ent_pos(e);
             js = S.cval[kj];
+            ca = S.comp[kf];
+            cbm = S.comp[kj];
             Floe Fi = load_floe_cur(S, fs, kf);
             Floe Fj = load_floe_cur(S, js, kj);
             const double dx = Fi.x - Fj.x, dy = Fi.y - Fj.y;
@@ -2213,59 +2277,77 @@ __global__ __launch_bounds__(64) void k_repair(DS S, int nmembers) {
                 const double2 wf = wind_at(S, kf);
                 pair_event<MIXED>(P, tstep, b, j - first, Fi, Fj, wf.x, wf.y, P.uw, P.vw, ps, o, 0);
             }
-            etab[st.nt + rank_e] = o;
+            stage_e[lane] = o;
             oflags = o.flags;
             reshape = (o.flags & EV_RIDGED) != 0;
             ridged = o.ridged;
             nr = o.nr; nh = o.nh; nmass = o.nmass; ninertia = o.ninertia; nhfa = o.nhfa; nhfw = o.nhfw;
         }
-        // Commit a prefix of the batch.  A reshape of floe g makes the later
-        // events of g (keys above the reshape's) run again, and only what
-        // those events change can change what follows them: every event with
-        // a key below the least of them stands.  That key follows at once
-        // for a reshape of the owner (and a wall reshape, which ends the
-        // round too); for a reshape of the partner j it is the first of j's
-        // own events or of the events (i2, j) with i2 above the owner, i2
-        // from j's inner list (ascending).
-        int64_t reach = INT64_MAX;
-        if (reshape) {
-            if (wall || ridged == 0) {
-                reach = key + 1;
-            } else {
-                reach = (int64_t)(j - first) << 16;
-                const int nij = S.icnt[kj];
-                int64_t ids[LIST_PEEK];
-#pragma unroll
-                for (int q = 0; q < LIST_PEEK; ++q) ids[q] = q < nij ? S.ik[(int64_t)q * SS + kj] : INT64_MAX;
-                int i2 = INT_MAX;
-#pragma unroll
-                for (int q = LIST_PEEK - 1; q >= 0; --q)
-                    if (ids[q] != INT64_MAX && ent_id(ids[q]) > f) i2 = ent_id(ids[q]);
-                for (int q = LIST_PEEK; i2 == INT_MAX && q < nij; ++q) {
-                    const int id = ent_id(S.ik[(int64_t)q * SS + kj]);
-                    if (id > f) i2 = id;
+        // Commit every event of the batch whose floes no uncommitted event
+        // before it (in key order) can still change, and whose reshape no
+        // such event can still read.  A floe changes only when an event of it
+        // ridges, so only through contacts: within its contact component
+        // (comp, from the step's contacts at its start radii; radii only
+        // shrink).  An event before a lane that is not committed yet (a
+        // deferred lane, one a reshape adds, one those add in turn) belongs
+        // to a component of T, the components of the reshaping lanes before
+        // it (a lane waits when a floe of it lies in T, so its own component
+        // is in T if it ridges at all), or it reads floes in T and their
+        // candidate neighbours.  So, going through the reshaping lanes in
+        // order: the component of each joins T, and the later lanes in T
+        // wait; a reshaping lane itself waits if a waiting lane before it
+        // reads the floe it reshapes or a candidate neighbour of that floe
+        // lies in T.  (A lane that waits ridges nothing new: if its floes
+        // touch, they share a component in T; if not, it cannot ridge.)
+        uint64_t defer = 0, done = 0;
+        int nT = 0;
+        const uint64_t rsh = __ballot(reshape);
+        const int kgl = (wall || ridged == 0) ? kf : kj;     // the floe a reshaping lane reshapes
+        for (;;) {
+            const uint64_t t = rsh & ~defer & ~done;
+            if (!t) break;
+            const int cl = __ffsll((long long)t) - 1;
+            done |= 1ull << cl;
+            const int cc = __shfl(ca, cl);
+            const int kgc = __shfl(kgl, cl);
+            bool exposed = __ballot(lane < cl && ((defer >> lane) & 1ull) && (kf == kgc || (pair && kj == kgc))) != 0;
+            if (!exposed && nT > 0) {
+                const int no_g = S.ocnt[kgc], ncand = no_g + S.icnt[kgc];
+                bool hit = false;
+                for (int q0 = 0; q0 < ncand; q0 += 64) {
+                    const int q = q0 + lane;
+                    if (q < ncand) {
+                        const int64_t e = q < no_g ? S.oj[(int64_t)q * SS + kgc] : S.ik[(int64_t)(q - no_g) * SS + kgc];
+                        const int cn = S.comp[ent_pos(e)];
+                        for (int u = 0; u < nT; ++u) hit = hit || cn == tcomp[u];
+                    }
+                    if (__ballot(hit)) break;
                 }
-                if (i2 != INT_MAX) reach = min(reach, (int64_t)(i2 - first) << 16);
+                exposed = __ballot(hit) != 0;
             }
+            if (exposed) defer |= 1ull << cl;
+            if (lane == 0) tcomp[nT] = cc;
+            ++nT;
+            wave_lds_sync();
+            defer |= __ballot(act && lane > cl && (ca == cc || cbm == cc));
         }
-        int end = nb;
-        for (uint64_t rm = __ballot(reshape); rm; rm &= rm - 1) {
-            const int cl = __ffsll((long long)rm) - 1;
-            if (cl >= end) break;
-            const int64_t ka = __shfl(reach, cl);
-            const uint64_t cut = __ballot(lane > cl && lane < end && key >= ka);
-            if (cut) end = __ffsll((long long)cut) - 1;
-        }
-        const int c = end - 1;                  // the last lane committed
-        const bool com = act && lane <= c;
+        const bool com = act && !((defer >> lane) & 1ull);
+        const uint64_t mce = __ballot(com && pair), mcw = __ballot(com && wall);
+        const int rank_e = __popcll(mce & lt), rank_w = __popcll(mcw & lt);
+        if (st.nt + __popcll(mce) > tcap || st.nwt + __popcll(mcw) > nmem) { over |= ERR_REPAIR_CAP; break; }
         if (com && pair) {
+            const int at = st.nt + rank_e;
+            etab[at] = stage_e[lane];
             wflag_set(S, kf, WF_FIXUP);
             wflag_set(S, kj, WF_FIXUP);
-            S.ov[(int64_t)s * SS + kf] = st.nt + rank_e;
+            S.ov[(int64_t)s * SS + kf] = at;
             S.evf[(int64_t)s * SS + kf] = oflags | EV_TABLE;
+            keep_block_rq(S, fs, rql);
         }
         if (com && wall) {
-            S.ovw[kf] = st.nwt + rank_w;
+            const int at = st.nwt + rank_w;
+            wtab[at] = stage_w_mine;
+            S.ovw[kf] = at;
             wflag_set(S, kf, WF_REPAIR_WALL | WF_FIXUP);
         }
         put_hist_lanes(S, com && pair, fs, j, ps, over);
@@ -2277,32 +2359,41 @@ __global__ __launch_bounds__(64) void k_repair(DS S, int nmembers) {
             gs = (wall || ridged == 0) ? fs : js;
             kg = (wall || ridged == 0) ? kf : kj;
             set_cur(S, gs, nr, nh, nmass, ninertia, nhfa, nhfw);
-            if (wall) S.dirty[gs] = 1;
-            else if (!S.dirty[gs]) { S.dirty[gs] = 1; adds = true; }
+            // a wall is a floe's last event: nothing of it follows
+            adds = !wall && !(S.dirty[gs] & DIRTY_SHAPE);
+            dirty_set(S, gs, DIRTY_SHAPE);
         }
-        const uint64_t mce = __ballot(com && pair), mcw = __ballot(com && wall);
         st.nt += __popcll(mce);
         st.nwt += __popcll(mcw);
         st.repaired += __popcll(mce | mcw);
-        if (mce | mcw) st.last = __shfl(key, 63 - __clzll((long long)(mce | mcw)));
-        if (mce) {
-            const int hl = 63 - __clzll((long long)mce);
-            const int bl = __shfl(b, hl);
-            const double rl = __shfl(rql, hl);
-            if (bl != st.cur_block) { st.cur_block = bl; st.rq = rl; }
-        }
-        const int consumed = c + 1;
+        // the next round starts with the deferred keys, in order, then the
+        // keys after the batch; one equal to the batch's last key is dropped
+        // if that key is done (its first lane committed, or it repeats the
+        // key done before the batch)
+        const int64_t lastkey = __shfl(key, nb - 1);
+        const uint64_t same = __ballot(lane < nb && key == lastkey);
+        const int fl = __ffsll((long long)same) - 1;
+        // (a first lane that is not active repeats the key done before)
+        const bool last_done = __shfl((int)act, fl) ? !((defer >> fl) & 1ull) : true;
+        const int nd = __popcll(defer);
+        int skip = 0;
+        if (last_done)
+            while (nb + skip < np && pk[cb][head + nb + skip] == lastkey) ++skip;
+        st.last = last_done ? lastkey : -1;
+        wave_lds_sync();
+        // deferred keys go just before the keys after the batch
+        const int nhead = head + nb + skip - nd;
+        if ((defer >> lane) & 1ull) pk[cb][nhead + __popcll(defer & lt)] = key;
+        wave_lds_sync();
         if (__ballot(over != 0)) break;
-        if (!__ballot(adds)) {
-            head += consumed;
-            np -= consumed;
-            continue;
-        }
+        head = nhead;
+        np = np - nb - skip + nd;
+        if (!__ballot(adds)) continue;
         // each committed reshape (of a floe not reshaped before) adds every
         // later event of its floe g: one sorted run per reshape, the runs
         // sorted together when there are several
         const uint64_t madd = __ballot(adds);
-        const int rest = np - consumed;
+        const int rest = np;
         int K = 0;
         bool spill_heap = false;
         for (uint64_t ma = madd; ma; ma &= ma - 1) {
@@ -2339,7 +2430,7 @@ __global__ __launch_bounds__(64) void k_repair(DS S, int nmembers) {
             // too many pending keys for LDS: on to the heap in HBM
             if (lane == 0) {
                 for (int i = 0; i < rest; ++i)
-                    if (!H.push((uint64_t)pk[cb][head + consumed + i])) over |= ERR_REPAIR_CAP;
+                    if (!H.push((uint64_t)pk[cb][head + i])) over |= ERR_REPAIR_CAP;
             }
             for (uint64_t ma = madd; ma; ma &= ma - 1) {
                 const int cl = __ffsll((long long)ma) - 1;
@@ -2358,7 +2449,7 @@ __global__ __launch_bounds__(64) void k_repair(DS S, int nmembers) {
             bitonic_inner(nk, p2, lane);
         }
         // merge the two sorted runs (pending keys first among equals)
-        const int64_t* A = &pk[cb][head + consumed];
+        const int64_t* A = &pk[cb][head];
         int64_t* O = pk[cb ^ 1];
         for (int i = lane; i < rest; i += 64) O[i + count_below(nk, K, A[i], false)] = A[i];
         for (int i = lane; i < K; i += 64) O[i + count_below(A, rest, nk[i], true)] = nk[i];
@@ -2585,7 +2676,8 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_final(DS S) {
     const DevParams& P = S.params[m];
     const int64_t SS = S.S;
     Floe F = floe_at(S, k);
-    const bool dirty = S.dirty[f] != 0;
+    const uint8_t dflags = S.dirty[f];
+    const bool dirty = (dflags & DIRTY_SHAPE) != 0;
     if (dirty) {
         F.r = S.cur[0 * SS + f]; F.h = S.cur[1 * SS + f]; F.mass = S.cur[2 * SS + f];
         F.inertia = S.cur[3 * SS + f]; F.hfa = S.cur[4 * SS + f]; F.hfw = S.cur[5 * SS + f];
@@ -2674,7 +2766,7 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_final(DS S) {
         for (int c = 0; c < 33; ++c) S.fout[c * SS + f] = cols[c];
     }
     // flags set by the repair are cleared (most floes have none: no write)
-    if (dirty) S.dirty[f] = 0;
+    if (dflags) S.dirty[f] = 0;
     // the new state, in the current order (the next step's copies unless it
     // rebuilds them from rec): position, kinematics, the wind at the new
     // position

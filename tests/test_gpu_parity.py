@@ -202,6 +202,48 @@ def test_repair_modes_bitwise():
                     assert np.array_equal(ta, tb), (prec, mode)
 
 
+def test_repair_components_bitwise():
+    """Nine separate ridging clusters in one member: a round of the repair
+    commits the events of different contact components together.  The
+    trajectory, contact sets, total forces and the number of events
+    re-evaluated equal those of the one-at-a-time repair (the reference's
+    key order) bitwise, in FP64 and mixed precision, with the pending events
+    on chip and spilling to the heap; FP64 also against the oracle."""
+    from workloads import clusters
+    f = clusters()
+    p = config0_params(f, sigmanc_crit=1e5)
+    for prec in ("fp64", "mixed"):
+        runs = {}
+        for mode in ((64, 1024), (1, 1024), (7, 1024), (64, 12)):
+            ctx = _ctx(f.n, 1, prec)
+            ctx.set_repair(*mode)
+            m = ctx.add_field(p, f)
+            repaired = seeds = 0
+            snaps = []
+            for t in range(1, 41):
+                ctx.step(1)
+                st = ctx.stats(m)
+                repaired += st["repaired"]
+                seeds += st["seeds"]
+                if t in (1, 10, 40):
+                    snaps.append((ctx.floes(m), _pairs(ctx, m, "contact"), ctx.forces(m)["tfx"]))
+            runs[mode] = (snaps, repaired, seeds)
+            ctx.close()
+        ref, rep0, seeds0 = runs[(1, 1024)]
+        print(f"[components] {prec}: {seeds0} seeds, {rep0} events re-evaluated", flush=True)
+        assert seeds0 > 50 and rep0 > 100, (seeds0, rep0)
+        for mode, (snaps, rep, _) in runs.items():
+            assert rep == rep0, (prec, mode, rep, rep0)
+            for sa, sb in zip(snaps, ref):
+                assert sa[1] == sb[1], (prec, mode)
+                assert np.array_equal(sa[2], sb[2]), (prec, mode)
+                for k in STATE:
+                    assert np.array_equal(sa[0][k], sb[0][k]), (prec, mode, k)
+    worst, seeds, _ = _lockstep(f, p, 40, check_every=5)
+    print(f"[components] fp64 vs the oracle: worst {worst:.3e}", flush=True)
+    assert worst < FP64_REL_TOL and seeds > 50, (worst, seeds)
+
+
 def test_superset_skin_bitwise():
     """Candidate lists taken from skinned supersets (rebuilt only after a floe
     has moved far enough) give the same trajectory, bitwise, as a full cell

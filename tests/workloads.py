@@ -30,3 +30,21 @@ def kick(field, seed=0, speed=0.3, spin=2e-4):
     field.v = rng.normal(0, speed, field.n)
     field.omega = rng.normal(0, spin, field.n)
     return field
+
+
+def clusters(rows=3, per=64, seed=5, box=2.0e4, gap=6.0e3):
+    """rows x rows kicked, squeezed packed boxes side by side (gap metres
+    apart) in one field: separate contact clusters, each ridging from the
+    first steps."""
+    parts = []
+    for a in range(rows * rows):
+        f = kick(squeeze(packed_box(per, seed + a, box=box)), seed + a)
+        parts.append((f, (a % rows) * (box + gap), (a // rows) * (box + gap)))
+    f0 = parts[0][0]
+    cat = lambda k: np.concatenate([getattr(f, k) + (dx if k == "x" else dy if k == "y" else 0.0)
+                                    for f, dx, dy in parts])
+    out = g.FloeField(x=cat("x"), y=cat("y"), r=cat("r"), h=cat("h"), theta=cat("theta"),
+                      omega=cat("omega"), u=cat("u"), v=cat("v"),
+                      box=rows * box + (rows - 1) * gap, seed=seed, wind=f0.wind,
+                      current=f0.current)
+    return out
