@@ -66,6 +66,9 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+
+
+
 def build_batch(members: int, seed: int):
     from godar_amd import workloads as W
     t = time.time()
@@ -93,6 +96,8 @@ def run_batched(args, batch):
     nfloes = sum(f.n for f, _ in batch)
     ctx = Context(nfloes, len(batch), args.precision)
     mids = [ctx.add_field(p, f) for f, p in batch]
+    if args.no_graphs:
+        ctx.use_graphs(False)
     ctx.step(args.warmup)
     if args.block:
         ctx.set_block(args.block)
@@ -372,6 +377,8 @@ def main():
                     help="served requests of the continuous-batching leg (0: skip)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (calibrates rocprof byte counters)")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="launch kernels one by one (no graph replay) in the batched leg")
     ap.add_argument("--served-mix", action="store_true",
                     help="only step all served requests at once (for profilers)")
     args = ap.parse_args()
@@ -447,6 +454,7 @@ def main():
     dom_ms = stages[dom][0] / stages[dom][1]
     alg = per_stage_bytes[dom]
     achieved = alg / (dom_ms * 1e-3) / 1e9
+    traffic = profile_traffic(dom, args.precision)
     step_gbs = B["alg_bytes_step"] / (ms_step * 1e-3) / 1e9
     out = {
         "metric": base["metric"],
@@ -473,9 +481,12 @@ def main():
         "roofline": {
             "bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-            "traffic": profile_traffic(dom, args.precision),
-            "traffic_source": "profiles/step_kernels_batched_512fields.md (rocprofv3 --pmc "
-                              "FETCH_SIZE x2.000, WRITE_SIZE x1.000, bytes per launch)",
+            "traffic": traffic,
+            "traffic_source": ("profiles/step_kernels_batched_512fields.md (rocprofv3 --pmc "
+                               "FETCH_SIZE x2.000, WRITE_SIZE x1.000, bytes per launch)"
+                               if traffic is not None else
+                               "null: rocprofv3 --pmc ends in a fault inside the profiler on this "
+                               "workload (profiles/step_kernels_batched_512fields.md)"),
             "kernel_avg_ms": dom_ms, "algorithmic_bytes": alg, "dominant_stage": top,
             "step_algorithmic_GBs": step_gbs, "step_frac": step_gbs / HBM_PEAK_GBS,
         },
