@@ -15,7 +15,7 @@
 #include <vector>
 
 namespace godar {
-__global__ void k_begin(DS);
+__global__ void k_begin(DS, int);
 __global__ void k_ranges(DS);
 template <bool M> __global__ void k_cand(DS);
 template <bool F> __global__ void k_sup(DS);
@@ -149,8 +149,18 @@ struct godar_ctx {
     std::vector<double> rmax;           // largest radius of each member when it was added
     int64_t sup_cap = 0;                // superset pool entries
     int32_t built_qseg = -1;            // queue share the dense list was built with
-    hipGraphExec_t gexec[2] = {nullptr, nullptr};
-    std::vector<unsigned char> gsig[2];
+    // captured step graphs by [parity][lean]
+    hipGraphExec_t gexec[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+    std::vector<unsigned char> gsig[2][2];
+    // Steps without a build launch no build kernels (lean steps): the host
+    // launches the full sequence when it forced a build or refresh itself,
+    // after LEAN_RUN lean steps in a row (the device builds early there once
+    // a floe has moved 0.3 skin), and for a while after a lean step found a
+    // build due (ERR_NEED_BUILD; godar_sync runs such steps again).
+    bool need_full = true;
+    int lean_run = 0;                   // lean steps since the last full one
+    int full_hold = 0;                  // full steps still to launch after a missed build
+    int64_t pend_launched = 0;          // steps launched since the last sync
     int device = 0;                     // the GPU the context lives on
     int64_t* totals_scratch = nullptr;  // device address of that GPU's `scratch`
 };
@@ -272,13 +282,15 @@ void bind_parity(godar_ctx* c) {
 
 // The next step builds the candidate supersets (and the cell order) anew.
 void force_rebuild(godar_ctx* c) {
-    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, c->stream, c->d.rebuild, 1, 1);
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, c->stream, c->d.rebuild, 1, BUILD_DUE);
+    c->need_full = true;
 }
 
 // The next step copies the floe records into the current order anew (the
 // host changed what the copies hold, e.g. the wind).
 void force_refresh(godar_ctx* c) {
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, c->stream, c->d.rebuild + 1, 1, 1);
+    c->need_full = true;
 }
 
 int32_t qseg_of(const godar_ctx* c) {
@@ -442,7 +454,10 @@ static dim3 grid_for(int N, int B, bool xcd) {
     return dim3(g);
 }
 
-int launch_step(godar_ctx* c) {
+constexpr int LEAN_RUN = 15;       // lean steps between two full ones
+constexpr int FULL_HOLD = 64;      // full steps after a lean step found a build due
+
+int launch_step(godar_ctx* c, bool lean) {
     DS& d = c->d;
     sync_ds(c);
     int N = c->N_used;
@@ -455,11 +470,11 @@ int launch_step(godar_ctx* c) {
     {
         StageTimer t(c, 0);
         const int nt = std::max({N, c->ncells_cap + 1, NQUEUE * XCD_COUNT});
-        hipLaunchKernelGGL(k_begin, grid_for(nt, B[0], false), dim3(B[0]), 0, s, d);
+        hipLaunchKernelGGL(k_begin, grid_for(nt, B[0], false), dim3(B[0]), 0, s, d, lean ? 1 : 0);
     }
     int bits = 1;
     while ((1u << bits) <= (uint32_t)c->ncells_cap && bits < 32) ++bits;
-    {
+    if (!lean) {
         StageTimer t(c, 1);
         HIPCHK(c, radix_sort_pairs(c->rs, d.ckey, d.cval, N, bits, d.err, d.rebuild, s));
     }
@@ -474,7 +489,7 @@ int launch_step(godar_ctx* c) {
     // builds keep 1,024 wavefronts per XCD: they return at once on most
     // steps, and dispatching many more would cost more than they save.
     const dim3 dense_grid(XCD_COUNT * (int)std::max<int64_t>(1, (dense_n + XCD_COUNT * 64 - 1) / (XCD_COUNT * 64)));
-    {
+    if (!lean) {
         StageTimer t(c, 2);
         hipLaunchKernelGGL(k_ranges, grid_for(N, B[2], false), dim3(B[2]), 0, s, d);
         // superset build: counts, offsets, entries (k_sup shares k_cand's
@@ -562,25 +577,26 @@ std::vector<unsigned char> launch_signature(godar_ctx* c) {
 }
 
 void drop_graphs(godar_ctx* c) {
-    for (int k = 0; k < 2; ++k) {
-        if (c->gexec[k]) (void)hipGraphExecDestroy(c->gexec[k]);
-        c->gexec[k] = nullptr;
-        c->gsig[k].clear();
-    }
+    for (int k = 0; k < 2; ++k)
+        for (int l = 0; l < 2; ++l) {
+            if (c->gexec[k][l]) (void)hipGraphExecDestroy(c->gexec[k][l]);
+            c->gexec[k][l] = nullptr;
+            c->gsig[k][l].clear();
+        }
 }
 
 // One step as a replay of the captured graph of the current parity; the
 // graph is (re)captured when the launch signature changed.
-int graph_step(godar_ctx* c) {
+int graph_step(godar_ctx* c, bool lean) {
     if (c->N_used == 0) return GODAR_OK;
-    const int par = c->parity;
+    const int par = c->parity, l = lean ? 1 : 0;
     std::vector<unsigned char> sig = launch_signature(c);
-    if (!c->gexec[par] || sig != c->gsig[par]) {
-        if (c->gexec[par]) (void)hipGraphExecDestroy(c->gexec[par]);
-        c->gexec[par] = nullptr;
-        c->gsig[par].clear();
+    if (!c->gexec[par][l] || sig != c->gsig[par][l]) {
+        if (c->gexec[par][l]) (void)hipGraphExecDestroy(c->gexec[par][l]);
+        c->gexec[par][l] = nullptr;
+        c->gsig[par][l].clear();
         HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        int rc = launch_step(c);
+        int rc = launch_step(c, lean);
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(c->stream, &g);
         c->parity = par;   // the captured step has not run yet
@@ -594,10 +610,10 @@ int graph_step(godar_ctx* c) {
         e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
         HIPCHK(c, e);
-        c->gexec[par] = x;
-        c->gsig[par] = std::move(sig);
+        c->gexec[par][l] = x;
+        c->gsig[par][l] = std::move(sig);
     }
-    HIPCHK(c, hipGraphLaunch(c->gexec[par], c->stream));
+    HIPCHK(c, hipGraphLaunch(c->gexec[par][l], c->stream));
     c->parity ^= 1;
     return GODAR_OK;
 }
@@ -611,6 +627,7 @@ std::string err_text(uint32_t e) {
     if (e & ERR_REPAIR_CAP) s += " ridging repair table full;";
     if (e & ERR_SEED_CAP) s += " ridging seed list full;";
     if (e & ERR_SUP_CAP) s += " candidate superset pool full;";
+    if (e & ERR_NEED_BUILD) s += " build due in a lean step;";
     return s;
 }
 
@@ -1642,10 +1659,14 @@ int godar_autotune(godar_ctx* c, int32_t trial) {
     if (rc != GODAR_OK) { c->prof = prof0; return rc; }
     int r2 = godar_profile(c, prof0 ? 1 : 0);
     if (r2 != GODAR_OK || prof0 || !c->use_graphs) return r2;
-    // launch mode: graph replay or one-by-one launches, whichever steps faster
-    const int n = std::max(4, 4 * trial);
-    double t[2] = {0.0, 0.0};
-    for (int mode = 0; mode < 2; ++mode) {
+    // launch mode: graph replay or one-by-one launches, whichever steps
+    // faster; each is timed twice, alternating, over max(32, 16 trial)
+    // steps, and its faster time counts (a short trial is at the mercy of
+    // a single slow step)
+    const int n = std::max(32, 16 * trial);
+    double t[2] = {1e300, 1e300};
+    for (int rep = 0; rep < 4; ++rep) {
+        const int mode = rep & 1;
         c->use_graphs = mode == 0;
         rc = godar_step(c, 1);                     // capture / settle
         if (rc != GODAR_OK) { c->use_graphs = true; return rc; }
@@ -1653,7 +1674,7 @@ int godar_autotune(godar_ctx* c, int32_t trial) {
         rc = godar_step(c, n);
         if (rc == GODAR_OK) rc = godar_sync(c);
         if (rc != GODAR_OK) { c->use_graphs = true; return rc; }
-        t[mode] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        t[mode] = std::min(t[mode], std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
     }
     c->use_graphs = t[0] <= t[1];
     if (!c->use_graphs) drop_graphs(c);
@@ -1673,6 +1694,7 @@ int godar_step_async(godar_ctx* c, int32_t nsteps) {
         if (rc != GODAR_OK) return rc;
         c->pend_parity = c->parity;
         c->pending = true;
+        c->pend_launched = 0;
     }
     const bool graphs = c->use_graphs && !c->prof;
     for (int k = 0; k < nsteps; ++k) {
@@ -1683,8 +1705,13 @@ int godar_step_async(godar_ctx* c, int32_t nsteps) {
             force_rebuild(c);
             c->built_qseg = q;
         }
-        int rc = graphs ? graph_step(c) : launch_step(c);
+        const bool lean = !c->need_full && c->full_hold == 0 && c->lean_run < LEAN_RUN;
+        int rc = graphs ? graph_step(c, lean) : launch_step(c, lean);
         if (rc != GODAR_OK) return rc;
+        c->need_full = false;
+        c->lean_run = lean ? c->lean_run + 1 : 0;
+        if (c->full_hold > 0 && !lean) --c->full_hold;
+        ++c->pend_launched;
     }
     return GODAR_OK;
 }
@@ -1713,6 +1740,24 @@ int godar_sync(godar_ctx* c) {
     if (rc != GODAR_OK) return rc;
     bool was_pending = c->pending;
     c->pending = false;
+    if (e == ERR_NEED_BUILD && was_pending) {
+        // a lean step found a build due: it and the steps after it did
+        // nothing; back to the last complete step, and the rest again with
+        // the build kernels
+        int64_t now = 0;
+        rc = steps_done(c, &now);
+        if (rc != GODAR_OK) return rc;
+        c->parity = c->pend_parity ^ (int)((now - c->pend_done0) & 1);
+        bind_parity(c);
+        HIPCHK(c, hipMemset(c->d.err, 0, sizeof(uint32_t)));
+        const int64_t left = c->pend_launched - (now - c->pend_done0);
+        c->need_full = true;
+        c->full_hold = FULL_HOLD;
+        if (left <= 0) return GODAR_OK;
+        rc = godar_step_async(c, (int32_t)left);
+        if (rc != GODAR_OK) return rc;
+        return godar_sync(c);
+    }
     if (e) {
         // Roll back to the last complete step.  Steps after the failing one
         // were no-ops; the device step counters tell how many completed, which

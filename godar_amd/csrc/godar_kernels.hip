@@ -357,7 +357,11 @@ __global__ void k_spill(DS S, int host) {
     S.rec[f] = r;
 }
 
-// First kernel of every step, over max(N, cells) threads:
+// First kernel of every step, over max(N, cells) threads.  A step launched
+// without the build kernels (lean) that finds a build or a refresh due sets
+// ERR_NEED_BUILD and does nothing else (every later kernel of the step then
+// returns at once and the host runs the step again with them); with the
+// build kernels, BUILD_SOON alone is enough to build.  Then:
 //  * the queue counters (the dense-search list, queue 3, only on build steps:
 //    it lists positions, which change only then) and, on build steps, the
 //    cell ranges;
@@ -368,10 +372,15 @@ __global__ void k_spill(DS S, int host) {
 //    input buffers (bkey, bval).  The position is read from the ordered copy
 //    where that is newer than rec (it is the same value the spill writes),
 //    so the spill and the keys need no order between them.
-__global__ void k_begin(DS S) {
+__global__ void k_begin(DS S, int lean) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (*S.err) return;
-    const bool build = S.rebuild[0] != 0, refresh = S.rebuild[1] != 0, newer = S.rebuild[2] != 0;
+    if (lean && ((S.rebuild[0] & BUILD_DUE) || S.rebuild[1])) {
+        if (i == 0) atomicOr(S.err, ERR_NEED_BUILD);
+        return;
+    }
+    const bool build = !lean && S.rebuild[0] != 0, refresh = !lean && S.rebuild[1] != 0;
+    const bool newer = S.rebuild[2] != 0;
     if (i < NQUEUE * XCD_COUNT && (build || i < 3 * XCD_COUNT)) S.qn[i * QN_STRIDE] = 0;
     if (build && i <= S.ncells) S.crange[i] = make_int2(0, 0);
     if (i >= S.N) return;
@@ -2741,13 +2750,17 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_final(DS S) {
         // next step builds the supersets anew if this floe has moved more
         // than 0.45 skin since the last build (a pair not in a superset was
         // more than its query radius + skin apart then, so it stays outside
-        // the query radius while both floes stay within half the skin)
+        // the query radius while both floes stay within half the skin);
+        // beyond 0.3 skin the next step with the build kernels builds early
         const double2 p0 = S.x0[k];
-        const double ex = out.x - p0.x, ey = out.y - p0.y, lim = 0.45 * P.skin;
-        // one store per wavefront (every such floe writing the one word
+        const double ex = out.x - p0.x, ey = out.y - p0.y, e2 = ex * ex + ey * ey;
+        const double lim = 0.45 * P.skin, soon = 0.3 * P.skin;
+        // one atomic per wavefront (every such floe writing the one word
         // would queue the writes behind each other)
-        const uint64_t mv = __ballot(ex * ex + ey * ey > lim * lim);
-        if (mv && (int)(threadIdx.x & 63) == __ffsll((long long)mv) - 1) *S.rebuild = 1;
+        const uint64_t md = __ballot(e2 > lim * lim), ms = __ballot(e2 > soon * soon);
+        const int bits = (md ? BUILD_DUE : 0) | (ms ? BUILD_SOON : 0);
+        const int leader = __ffsll((long long)(md | ms)) - 1;
+        if (bits && (int)(threadIdx.x & 63) == leader) atomicOr(S.rebuild, bits);
     }
     if (S.record_forces) {
         double tp = (a[A_TAC] != 0.0 ? a[A_PC] / a[A_TAC] : 0.0)

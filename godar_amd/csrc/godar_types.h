@@ -74,7 +74,16 @@ enum : uint32_t {
     ERR_SEED_CAP   = 1u << 5,
     ERR_NONFINITE  = 1u << 6,
     ERR_SUP_CAP    = 1u << 7,
+    // a step launched without the build kernels found a build or refresh
+    // due (not a failure: the host runs it again with them)
+    ERR_NEED_BUILD = 1u << 8,
 };
+
+// Build flag bits (DS::rebuild[0]): a floe has moved more than 0.45 skin
+// since the last build, so the next step must build the supersets; a floe
+// has moved more than 0.3 skin, so a step that has the build kernels builds
+// them now (steps without them carry on).
+constexpr int32_t BUILD_DUE = 1, BUILD_SOON = 2;
 
 // Bond record (owner = lower floe index), bonds.f90 state of pair (j,i).
 struct BondRec {

@@ -252,10 +252,11 @@ int godar_get_stage_blocks(const godar_ctx *ctx, int32_t *threads);   /* GODAR_N
  * max(4, 4 trial) steps after it as t_step + (t_build - t_step) / period,
  * the period being the steps until the fastest floe (at its current speed)
  * has moved 0.45 skin, at most 5,000.  Then, unless graphs are off
- * (godar_use_graphs) or steps are being profiled, time max(4, 4 trial) steps
- * in each launch mode and keep the faster one.  The members advance by
- * 1 + 3 * trial * (tunable stages) steps, at most 6 (1 + max(4, 4 trial))
- * for the search shape and 2 (1 + max(4, 4 trial)) for the launch mode. */
+ * (godar_use_graphs) or steps are being profiled, time max(32, 16 trial)
+ * steps in each launch mode twice, alternating, and keep the mode with the
+ * faster run.  The members advance by 1 + 3 * trial * (tunable stages)
+ * steps, at most 6 (1 + max(4, 4 trial)) for the search shape and
+ * 4 (1 + max(32, 16 trial)) for the launch mode. */
 int godar_autotune(godar_ctx *ctx, int32_t trial);
 /* Step launch mode (no counterpart in the reference): on (the default), each
  * step replays a captured HIP graph of the whole kernel sequence, recaptured
