@@ -518,8 +518,9 @@ int launch_step(godar_ctx* c, bool lean) {
     }
     {
         StageTimer t(c, 4);
+        // LDS: one pass of 64 pair results per wavefront
         hipLaunchKernelGGL(mixed ? k_contact<true> : k_contact<false>, grid_for(N, B[4], true),
-                           dim3(B[4]), 0, s, d);
+                           dim3(B[4]), (B[4] / 64) * CONTACT_LDS_PER_WAVE, s, d);
     }
     {
         StageTimer t(c, 5);

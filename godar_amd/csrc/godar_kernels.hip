@@ -1500,130 +1500,208 @@ __device__ __forceinline__ void load_contact_state(const DS& S, int i, int j, Pa
 
 // Contact terms of the queued floes (contact_forces, plastic_contact,
 // update_shape, coulomb; stepper.f90:103-146) + the owner's new histories.
+// The (floe, candidate) pairs of a wavefront's floes are evaluated together,
+// 64 a pass, one per lane (each lane finds its pair's floe among the lanes
+// by a binary search over their offsets): a floe's inner pairs (ascending
+// outer index) first, then its own candidates in distance order.  Each pass
+// leaves every pair's contact terms and history outcome in LDS, and every
+// floe then adds its own, in order: the same additions in the same order as
+// a floe's own walk, without one lane holding the wavefront through a long
+// list.
+struct ContactOut {
+    double fcn_c, fcr_s, fcn_s, fcr_c, r_fct, mcc, ac, fcn_ac;   // the floe's side of the terms
+    double deltat, thetarelc, fcn, fct;                           // history to append (owner side)
+    int32_t j;
+    uint32_t kind;     // bit 0: contact terms, bit 1: history record to append
+};
+constexpr int CONTACT_LDS_PER_WAVE = 64 * (int)sizeof(ContactOut);
+
 template <bool MIXED>
 __global__ __launch_bounds__(FLOE_BLOCK) void k_contact(DS S) {
+    extern __shared__ unsigned char contact_lds[];
+    ContactOut* co = reinterpret_cast<ContactOut*>(contact_lds + (threadIdx.x >> 6) * CONTACT_LDS_PER_WAVE);
     if (*S.err) return;
+    const int lane = (int)(threadIdx.x & 63);
     const int k = queue_entry(S, 0);
-    if (k < 0) return;
-    const GeoRec me = S.geo[k];
-    const int m = me.member;
-    const DevParams& P = S.params[m];
-    const int tstep = S.members[m].tstep;
-    const int first = S.members[m].first;
-    const int f = me.id;            // identity (ordering, bond mask)
-    const int fs = S.cval[k];       // storage slot of the floe's pair state
+    const bool act = k >= 0;
     const int64_t SS = S.S;
-    const int64_t* oj = S.oj;
-    const int64_t* ik = S.ik;
-    const Floe F0 = floe_at(S, k);
-    double acc[NACC];
-    acc_init(acc);
-    uint32_t over = 0;
-
-    // pairs (i, f), f inner, ascending i; only overlapping pairs contribute
-    const int ni = S.icnt[k];
-    for (int q = 0; q < ni; ++q) {
-        const int64_t e = ik[q * SS + k];
-        const int i = ent_id(e), qi = ent_pos(e);
-        Floe Fi = floe_at(S, qi);
-        Floe Ff = F0;
-        Kin kn;
-        kinematics(Fi, Ff, kn);
-        if (!(kn.deltan > 0)) continue;
-        PairState ps;
-        load_contact_state(S, S.cval[qi], f, ps);
-        bond_create(P, tstep, i - first, f - first, Fi, Ff, kn, ps);
-        EventOut o;
-        o.flags = 0;
-        o.ridged = -1;
-        contact_part<MIXED>(P, Fi, Ff, kn, ps, o);
-        apply_contact_j(acc, o);
+    int m = 0, f = 0, fs = 0, ni = 0, no = 0;
+    if (act) {
+        const GeoRec me = S.geo[k];
+        m = me.member;
+        f = me.id;                  // identity (ordering, bond mask)
+        fs = S.cval[k];             // storage slot of the floe's pair state
+        ni = S.icnt[k];
+        no = S.ocnt[k];
     }
-
-    // pairs (f, j), f outer (owner), distance order
-    const int no = S.ocnt[k];
+    const int cnt = ni + no;
+    int incl = cnt;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int v = __shfl_up(incl, off);
+        if (lane >= off) incl += v;
+    }
+    const int start = incl - cnt, total = __shfl(incl, 63);
+    double afx = 0.0, afy = 0.0, amc = 0.0, atac = 0.0, apc = 0.0;
     int nh = 0;
-    const int hco = S.hcnt_old[fs];
-    for (int s = 0; s < no; ++s) {
-        const int64_t e = oj[s * SS + k];
-        const int j = ent_id(e), qj = ent_pos(e);
-        Floe Fi = F0;
-        Floe Fj = floe_at(S, qj);
-        Kin kn;
-        kinematics(Fi, Fj, kn);
-        if (!(kn.deltan > 0)) {
-            // reset_contact (reset.f90:1-15): the history goes, the last
-            // contact forces stay
-            for (int q = 0; q < hco; ++q) {
-                const HistRec& h0 = S.hist_old[(int64_t)q * SS + fs];
-                if (h0.j != j) continue;
-                if (h0.fcn != 0.0 || h0.fct != 0.0) {
+    uint32_t over = 0;
+    for (int c0 = 0; c0 < total; c0 += 64) {
+        const int idx = c0 + lane;
+        int own = 0;
+#pragma unroll
+        for (int step = 32; step > 0; step >>= 1)
+            if (__shfl(start, own + step) <= idx) own += step;
+        const int ko = __shfl(k, own), om = __shfl(m, own), of = __shfl(f, own);
+        const int ofs = __shfl(fs, own), oni = __shfl(ni, own), ostart = __shfl(start, own);
+        ContactOut r;
+        r.kind = 0;
+        if (idx < total) {
+            const DevParams& P = S.params[om];
+            const int tstep = S.members[om].tstep;
+            const int first = S.members[om].first;
+            const int local = idx - ostart;
+            // overlap first from the position rows alone (the expression
+            // kinematics evaluates, so the same decision); the full rows
+            // only for contacts
+            const GeoRec gf = S.geo[ko];
+            if (local < oni) {
+                // pair (i, f), f the inner floe: only overlapping pairs contribute
+                const int64_t e = S.ik[(int64_t)local * SS + ko];
+                const int i = ent_id(e), qi = ent_pos(e);
+                const GeoRec gi = S.geo[qi];
+                const double dx = gi.x - gf.x, dy = gi.y - gf.y;
+                const double dn = gi.r + gf.r - sqrt(dx * dx + dy * dy);
+                if (dn > 0) {
+                    Floe Fi = floe_at(S, qi);
+                    Floe Fj = floe_at(S, ko);
+                    Kin kn;
+                    kinematics(Fi, Fj, kn);
+                    PairState ps;
+                    load_contact_state(S, S.cval[qi], of, ps);
+                    bond_create(P, tstep, i - first, of - first, Fi, Fj, kn, ps);
+                    EventOut o;
+                    o.flags = 0;
+                    o.ridged = -1;
+                    contact_part<MIXED>(P, Fi, Fj, kn, ps, o);
+                    r.kind = 1;
+                    r.fcn_c = o.fcn_c; r.fcr_s = o.fcr_s; r.fcn_s = o.fcn_s; r.fcr_c = o.fcr_c;
+                    r.r_fct = o.rj_fct; r.mcc = o.mcc; r.ac = o.ac; r.fcn_ac = o.fcn_ac;
+                }
+            } else {
+                // pair (f, j), f the outer floe (the owner)
+                const int s = local - oni;
+                const int64_t e = S.oj[(int64_t)s * SS + ko];
+                const int j = ent_id(e), qj = ent_pos(e);
+                const GeoRec gj = S.geo[qj];
+                const double dx = gf.x - gj.x, dy = gf.y - gj.y;
+                const double dn = gf.r + gj.r - sqrt(dx * dx + dy * dy);
+                r.j = j;
+                if (!(dn > 0)) {
+                    // reset_contact (reset.f90:1-15): the history goes, the
+                    // last contact forces stay
+                    const int hco = S.hcnt_old[ofs];
+                    for (int q = 0; q < hco; ++q) {
+                        const HistRec& h0 = S.hist_old[(int64_t)q * SS + ofs];
+                        if (h0.j != j) continue;
+                        if (h0.fcn != 0.0 || h0.fct != 0.0) {
+                            r.kind = 2;
+                            r.deltat = 0.0; r.thetarelc = 0.0; r.fcn = h0.fcn; r.fct = h0.fct;
+                            if (S.record_forces) {
+                                PairState pz;
+                                pz.fcn = h0.fcn; pz.fct = h0.fct;
+                                S.ssc[(int64_t)s * SS + ko] = stale_contact_magnitude<MIXED>(pz);
+                            }
+                        }
+                        break;
+                    }
+                } else {
+                    Floe Fi = floe_at(S, ko);
+                    Floe Fj = floe_at(S, qj);
+                    Kin kn;
+                    kinematics(Fi, Fj, kn);
+                    PairState ps;
+                    load_contact_state(S, ofs, j, ps);
+                    comp_unite(S.comp, ko, qj);      // the contact joins the floes' components
+                    bond_create(P, tstep, of - first, j - first, Fi, Fj, kn, ps);
+                    EventOut o;
+                    o.flags = 0;
+                    o.ridged = -1;
+                    contact_part<MIXED>(P, Fi, Fj, kn, ps, o);
+                    S.evf[(int64_t)s * SS + ko] |= o.flags;
+                    if (S.record_forces) S.ssc[(int64_t)s * SS + ko] = o.sc;
+                    r.kind = 1;
+                    r.fcn_c = o.fcn_c; r.fcr_s = o.fcr_s; r.fcn_s = o.fcn_s; r.fcr_c = o.fcr_c;
+                    r.r_fct = o.ri_fct; r.mcc = o.mcc; r.ac = o.ac; r.fcn_ac = o.fcn_ac;
+                    if (ps.deltat != 0.0 || ps.thetarelc != 0.0 || ps.fcn != 0.0 || ps.fct != 0.0) {
+                        r.kind |= 2;
+                        r.deltat = ps.deltat; r.thetarelc = ps.thetarelc; r.fcn = ps.fcn; r.fct = ps.fct;
+                    }
+                    if (o.flags & EV_RIDGED) {
+                        const int p = atomicAdd(&S.nseeds[om], 1);
+                        if (p < S.SEEDK * S.members[om].n)
+                            S.seeds[(int64_t)S.SEEDK * first + p] = ((uint64_t)(of - first) << 16) | (uint64_t)s;
+                        else
+                            over |= ERR_SEED_CAP;
+                    }
+                }
+            }
+        }
+        co[lane] = r;
+        wave_lds_sync();
+        // every floe adds this pass's terms of its own, in order
+        if (act) {
+            const int lo = max(start, c0), hi = min(start + cnt, c0 + 64);
+            for (int i2 = lo; i2 < hi; ++i2) {
+                const ContactOut& t = co[i2 - c0];
+                if (t.kind & 1) {
+                    if (i2 - start < ni) {
+                        afx = afx + t.fcn_c - t.fcr_s;
+                        afy = afy + t.fcn_s + t.fcr_c;
+                        amc = amc - t.r_fct + t.mcc;
+                    } else {
+                        afx = afx - t.fcn_c + t.fcr_s;
+                        afy = afy - t.fcn_s - t.fcr_c;
+                        amc = amc - t.r_fct - t.mcc;
+                    }
+                    atac = atac + t.ac;
+                    apc = apc - t.fcn_ac;
+                }
+                if (t.kind & 2) {
                     if (nh < S.HC) {
-                        HistRec hr; hr.j = j; hr.pad_ = 0; hr.deltat = 0.0; hr.thetarelc = 0.0;
-                        hr.fcn = h0.fcn; hr.fct = h0.fct;
-                        S.hist_new[nh * SS + fs] = hr;
+                        HistRec hr; hr.j = t.j; hr.pad_ = 0; hr.deltat = t.deltat; hr.thetarelc = t.thetarelc;
+                        hr.fcn = t.fcn; hr.fct = t.fct;
+                        S.hist_new[(int64_t)nh * SS + fs] = hr;
                         ++nh;
                     } else {
                         over |= ERR_HIST_CAP;
                     }
-                    if (S.record_forces) {
-                        PairState pz;
-                        pz.fcn = h0.fcn; pz.fct = h0.fct;
-                        S.ssc[s * SS + k] = stale_contact_magnitude<MIXED>(pz);
-                    }
                 }
-                break;
-            }
-            continue;
-        }
-        PairState ps;
-        load_contact_state(S, fs, j, ps);
-        comp_unite(S.comp, k, qj);
-        bond_create(P, tstep, f - first, j - first, Fi, Fj, kn, ps);
-        EventOut o;
-        o.flags = 0;
-        o.ridged = -1;
-        contact_part<MIXED>(P, Fi, Fj, kn, ps, o);
-        apply_contact_i(acc, o);
-        S.evf[s * SS + k] |= o.flags;
-        if (S.record_forces) S.ssc[s * SS + k] = o.sc;
-        if (ps.deltat != 0.0 || ps.thetarelc != 0.0 || ps.fcn != 0.0 || ps.fct != 0.0) {
-            if (nh < S.HC) {
-                HistRec hr; hr.j = j; hr.pad_ = 0; hr.deltat = ps.deltat; hr.thetarelc = ps.thetarelc;
-                hr.fcn = ps.fcn; hr.fct = ps.fct;
-                S.hist_new[nh * SS + fs] = hr;
-                ++nh;
-            } else {
-                over |= ERR_HIST_CAP;
             }
         }
-        if (o.flags & EV_RIDGED) {
-            int p = atomicAdd(&S.nseeds[m], 1);
-            if (p < S.SEEDK * S.members[m].n)
-                S.seeds[(int64_t)S.SEEDK * first + p] = ((uint64_t)(f - first) << 16) | (uint64_t)s;
-            else
-                over |= ERR_SEED_CAP;
-        }
+        wave_lds_sync();
     }
-    // contact histories of pairs that are not candidates this step persist
-    // (the reference only resets a candidate pair, reset_contact)
-    for (int q = 0; q < hco; ++q) {
-        HistRec hr = S.hist_old[q * SS + fs];
-        if (hr.j < 0) continue;
-        bool cand = false;
-        for (int s = 0; s < no; ++s)
-            if (ent_id(oj[s * SS + k]) == hr.j) { cand = true; break; }
-        if (!cand) {
-            if (nh < S.HC) S.hist_new[nh * SS + fs] = hr, ++nh;
-            else over |= ERR_HIST_CAP;
+    if (act) {
+        // contact histories of pairs that are not candidates this step
+        // persist (the reference only resets a candidate pair, reset_contact)
+        const int hco = S.hcnt_old[fs];
+        for (int q = 0; q < hco; ++q) {
+            HistRec hr = S.hist_old[(int64_t)q * SS + fs];
+            if (hr.j < 0) continue;
+            bool cand = false;
+            for (int s = 0; s < no; ++s)
+                if (ent_id(S.oj[(int64_t)s * SS + k]) == hr.j) { cand = true; break; }
+            if (!cand) {
+                if (nh < S.HC) S.hist_new[(int64_t)nh * SS + fs] = hr, ++nh;
+                else over |= ERR_HIST_CAP;
+            }
         }
+        S.hcnt_new[fs] = nh;
+        S.acc[A_FCX * SS + k] = afx;
+        S.acc[A_FCY * SS + k] = afy;
+        S.acc[A_MC * SS + k] = amc;
+        S.acc[A_TAC * SS + k] = atac;
+        S.acc[A_PC * SS + k] = apc;
     }
-    S.hcnt_new[fs] = nh;
-    S.acc[A_FCX * SS + k] = acc[A_FCX];
-    S.acc[A_FCY * SS + k] = acc[A_FCY];
-    S.acc[A_MC * SS + k] = acc[A_MC];
-    S.acc[A_TAC * SS + k] = acc[A_TAC];
-    S.acc[A_PC * SS + k] = acc[A_PC];
     if (over) set_err(S, over);
 }
 
