@@ -622,20 +622,46 @@ constexpr int CAND_LDS_PER_WAVE = CAND_TCAP * (8 + 8 + 1) + 64 * 4;
 // fmin terms does not depend on the order or on which lane computed them).
 // Otherwise, or when the pairs outnumber CAND_TCAP, each lane scans its own
 // superset SCAN_BATCH records at a time and evaluates its own terms.
+// The rows of position k (and its storage slot's counts) that the search
+// reads first, loaded at once as soon as k is known.
+struct CandPre {
+    ShelRec msh;
+    int64_t soff;
+    int fs, L, hc, bc;
+};
+__device__ __forceinline__ CandPre cand_pre(const DS& S, int k) {
+    CandPre c;
+    c.msh = S.shel[k];
+    c.fs = S.cval[k];
+    c.L = S.scnt[k];
+    c.soff = S.soff[k];
+    c.hc = S.hcnt_old[c.fs];
+    c.bc = S.bcnt_old[c.fs];
+    return c;
+}
+
 template <bool MIXED, bool WAVE>
-__device__ __forceinline__ void cand_thread(const DS& S, int k, const GeoRec& me, bool act, bool& qc,
-                                            bool& qb, bool& qw, unsigned char* lds) {
+__device__ __forceinline__ void cand_thread(const DS& S, int k, const GeoRec& me, const CandPre& pre,
+                                            int m, bool act, bool& qc, bool& qb, bool& qw,
+                                            unsigned char* lds) {
     using T = typename Prec<MIXED>::T;
-    const int m = act ? me.member : 0;
+    // member m's parameters and step, read at once (m is a valid member
+    // index whether or not the lane has a floe); no short-circuit tests
+    // between them, which would turn into a chain of single loads
     const DevParams& P = S.params[m];
+    const Member Mb = S.members[m];
+    const double rtree = P.rtree, pnx = P.nx, pny = P.ny, uw = P.uw, vw = P.vw, blim = P.bond_lim;
+    const int coh = P.cohesion, rst = P.restart, shl = P.shelter;
+    const uint32_t bthr = P.bond_thr24;
+    const uint64_t bseed = P.bond_seed;
     const double mx = act ? me.x : 0.0, my = act ? me.y : 0.0, mr = act ? me.r : 0.0;
-    const double rad = mr + P.rtree;
+    const double rad = mr + rtree;
     const int f = act ? me.id : 0;
-    const int first = S.members[m].first;
-    const bool bond_step = act && S.members[m].tstep == 1 && P.restart != 1 && P.cohesion;
+    const int first = Mb.first;
+    const bool bond_step = act & (Mb.tstep == 1) & (rst != 1) & (coh != 0);
     ShelRec msh;
     msh.hfa = msh.hfw = msh.wx = msh.wy = 0.0;
-    const T uwi = T(P.uw), vwi = T(P.vw);
+    const T uwi = T(uw), vwi = T(vw);
     T sha = T(1), shw = T(1);
     int fs = 0;
     int no = 0, ni = 0;
@@ -657,13 +683,13 @@ __device__ __forceinline__ void cand_thread(const DS& S, int k, const GeoRec& me
     int L = 0;
     int64_t soff = 0;
     if (act) {
-        msh = S.shel[k];
-        fs = S.cval[k];
-        qc = S.hcnt_old[fs] > 0;
-        qb = S.bcnt_old[fs] > 0;
-        qw = me.y - me.r < 0 || me.y + me.r > P.ny || me.x - me.r < 0 || me.x + me.r > P.nx;
-        L = S.scnt[k];
-        soff = S.soff[k];
+        msh = pre.msh;
+        fs = pre.fs;
+        qc = pre.hc > 0;
+        qb = pre.bc > 0;
+        qw = (me.y - me.r < 0) | (me.y + me.r > pny) | (me.x - me.r < 0) | (me.x + me.r > pnx);
+        L = pre.L;
+        soff = pre.soff;
     }
     // one candidate j at position q and distance d into the floe's lists
     auto insert = [&](int j, int q, double d) {
@@ -738,7 +764,6 @@ __device__ __forceinline__ void cand_thread(const DS& S, int k, const GeoRec& me
         const int start = incl - L, totalL = __shfl(incl, 63);
         tn[lane] = 0;
         wave_lds_sync();
-        const double rtree = P.rtree;
         bool ovf = false;
         for (int c0 = 0; c0 < totalL; c0 += 64) {
             const int idx = c0 + lane;
@@ -818,7 +843,7 @@ __device__ __forceinline__ void cand_thread(const DS& S, int k, const GeoRec& me
                 const GeoRec* o = &g[u];
                 const double dx = me.x - o->x, dy = me.y - o->y;
                 const double dist2 = dx * dx + dy * dy;
-                const double thr = o->id > f ? rad : o->r + P.rtree;
+                const double thr = o->id > f ? rad : o->r + rtree;
                 if (dist2 > thr * thr * (1.0 + 1e-12)) continue;
                 const double d = sqrt(dist2);
                 if (!(d <= thr)) continue;
@@ -838,15 +863,21 @@ __device__ __forceinline__ void cand_thread(const DS& S, int k, const GeoRec& me
             double d = -1.0;
             int own = lane;
             if (valid) { e = te[idx]; d = td[idx]; own = to[idx]; }
+            // the pair's floe's member parameters, from its lane
             const int ko = __shfl(k, own);
-            const int mo_m = __shfl(m, own);
+            const int first_o = __shfl(first, own);
+            const int bst_o = __shfl((int)bond_step, own), shl_o = __shfl(shl, own);
+            const double blim_o = __shfl(blim, own), uw_o = __shfl(uw, own), vw_o = __shfl(vw, own);
+            const uint32_t thr_o = (uint32_t)__shfl((int)bthr, own);
+            const uint64_t seed_o = __shfl(bseed, own);
             T ta = T(1), tw = T(1);
             uint32_t fl = 0;
             if (valid) {
-                const GeoRec mo = S.geo[ko];
-                const DevParams& Po = S.params[mo_m];
                 const int q = ent_pos(e);
+                const GeoRec mo = S.geo[ko];
                 const GeoRec o = S.geo[q];
+                const ShelRec oh = S.shel[q];
+                const ShelRec mh = S.shel[ko];
                 const bool outer = d >= 0.0;
                 double dd = d;
                 if (!outer) {
@@ -857,19 +888,13 @@ __device__ __forceinline__ void cand_thread(const DS& S, int k, const GeoRec& me
                 // pair (j, f), f the inner floe j
                 const double deltan = outer ? mo.r + o.r - dd : o.r + mo.r - dd;
                 if (deltan > 0) fl |= 1u;
-                if (outer) {
-                    const int first_o = S.members[mo_m].first;
-                    const bool bstep = S.members[mo_m].tstep == 1 && Po.restart != 1 && Po.cohesion;
-                    if (bstep && deltan >= -Po.bond_lim &&
-                        bond_keep(mo.id - first_o, ent_id(e) - first_o, Po.bond_seed, Po.bond_thr24))
-                        fl |= 2u;
-                }
-                if (Po.shelter) {
+                if (outer && bst_o && deltan >= -blim_o &&
+                    bond_keep(mo.id - first_o, ent_id(e) - first_o, seed_o, thr_o))
+                    fl |= 2u;
+                if (shl_o) {
                     // outer: floe f takes hsf*(j,f), evaluated with -cosa,
                     // -sina of the direction f -> j; inner: cosa/sina point
                     // from j to f and f takes hsf*(j,f) with j's wind
-                    const ShelRec oh = S.shel[q];
-                    const ShelRec mh = S.shel[ko];
                     const double nx = outer ? o.x - mo.x : mo.x - o.x;
                     const double ny = outer ? o.y - mo.y : mo.y - o.y;
                     const double cosa = nx / dd, sina = ny / dd;
@@ -877,7 +902,7 @@ __device__ __forceinline__ void cand_thread(const DS& S, int k, const GeoRec& me
                     const T c = outer ? -c0t : c0t, sn = outer ? -s0t : s0t;
                     const T ux = outer ? T(mh.wx) : T(oh.wx), uy = outer ? T(mh.wy) : T(oh.wy);
                     ta = shelter_min(T(1), T(oh.hfa), T(mh.hfa), dn, c, sn, ux, uy);
-                    tw = shelter_min(T(1), T(oh.hfw), T(mh.hfw), dn, c, sn, T(Po.uw), T(Po.vw));
+                    tw = shelter_min(T(1), T(oh.hfw), T(mh.hfw), dn, c, sn, T(uw_o), T(vw_o));
                 }
             }
             if (valid) {
@@ -902,10 +927,10 @@ __device__ __forceinline__ void cand_thread(const DS& S, int k, const GeoRec& me
             // pair (f, j): f is the outer floe i of the event
             const double deltan = me.r + o.r - d;
             if (deltan > 0) qc = true;
-            if (!qb && bond_step && deltan >= -P.bond_lim &&
-                bond_keep(f - first, j - first, P.bond_seed, P.bond_thr24))
+            if (!qb && bond_step && deltan >= -blim &&
+                bond_keep(f - first, j - first, bseed, bthr))
                 qb = true;
-            if (P.shelter) {
+            if (shl) {
                 // floe f takes hsf*(j,f), evaluated with -cosa, -sina
                 const double cosa = (o.x - me.x) / d, sina = (o.y - me.y) / d;
                 const ShelRec oh = S.shel[q];
@@ -922,7 +947,7 @@ __device__ __forceinline__ void cand_thread(const DS& S, int k, const GeoRec& me
             // pair (j, f): f is the inner floe j of the event
             const double deltan = o.r + me.r - d;
             if (deltan > 0) qc = true;
-            if (P.shelter) {
+            if (shl) {
                 // cosa/sina point from j to f; floe f takes hsf*(j,f) with j's wind
                 const double cosa = (me.x - o.x) / d, sina = (me.y - o.y) / d;
                 const ShelRec oh = S.shel[q];
@@ -1041,21 +1066,28 @@ template <bool MIXED>
 __global__ __launch_bounds__(FLOE_BLOCK) void k_cand(DS S) {
     const int k = xcd_block() * blockDim.x + threadIdx.x;
     bool qc = false, qb = false, qw = false;
-    bool live = k < S.N && !*S.err;
+    // every row the search starts from is requested before anything is
+    // tested (reads only: harmless for free slots and failed steps)
+    const bool inr = k < S.N;
     GeoRec me;
-    if (live) {
+    CandPre pre;
+    if (inr) {
         me = S.geo[k];
-        if (me.member < 0) {
-            S.ocnt[k] = 0;
-            S.icnt[k] = 0;
-            live = false;
-        }
+        pre = cand_pre(S, k);
     }
+    const bool slot_used = inr && me.member >= 0;
+    const int m = slot_used ? me.member : 0;
+    const bool dense = S.params[m].dense != 0;
+    const bool ok = !*S.err;
     // floes of dense members: k_cand_dense
     extern __shared__ unsigned char cand_lds[];
-    const bool act = live && !S.params[me.member].dense;
-    cand_thread<MIXED, true>(S, k, me, act, qc, qb, qw,
+    const bool act = slot_used & ok & !dense;
+    cand_thread<MIXED, true>(S, k, me, pre, m, act, qc, qb, qw,
                              cand_lds + (threadIdx.x >> 6) * CAND_LDS_PER_WAVE);
+    if (inr && ok && !slot_used) {
+        S.ocnt[k] = 0;
+        S.icnt[k] = 0;
+    }
     enqueue(S, qc, 0, k);
     enqueue(S, qb, 1, k);
     enqueue(S, qw, 2, k);
@@ -1466,8 +1498,9 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_cand_dense_over(DS S) {
     const int k = queue_entry(S, 3);
     if (k < 0 || S.ocnt[k] != DENSE_OVER) return;
     const GeoRec me = S.geo[k];
+    const CandPre pre = cand_pre(S, k);
     bool qc = false, qb = false, qw = false;
-    cand_thread<MIXED, false>(S, k, me, true, qc, qb, qw, nullptr);
+    cand_thread<MIXED, false>(S, k, me, pre, me.member, true, qc, qb, qw, nullptr);
     if (qc) enqueue_one(S, 0, k);
     if (qb) enqueue_one(S, 1, k);
     if (qw) enqueue_one(S, 2, k);
