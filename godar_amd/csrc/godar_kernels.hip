@@ -198,29 +198,39 @@ constexpr int LIST_PEEK = 4;
 
 // Index of the first of owner i's n records (slot-major rows of stride SS)
 // whose partner is j (j >= 0; empty records hold -1), or -1.
+// The first PEEK rows are requested together (one test of n, no test per
+// row; rows past n stand in for the last one, so no other line is read).
 template <int PEEK = LIST_PEEK, typename Rec>
 __device__ __forceinline__ int find_partner(const Rec* list, int64_t SS, int i, int n, int j) {
     int id[PEEK];
 #pragma unroll
-    for (int p = 0; p < PEEK; ++p) id[p] = p < n ? list[(int64_t)p * SS + i].j : -1;
+    for (int p = 0; p < PEEK; ++p) id[p] = -1;
+    if (n > 0) {
+#pragma unroll
+        for (int p = 0; p < PEEK; ++p) id[p] = list[(int64_t)min(p, n - 1) * SS + i].j;
+    }
     int q = -1;
 #pragma unroll
-    for (int p = PEEK - 1; p >= 0; --p) q = id[p] == j ? p : q;
+    for (int p = PEEK - 1; p >= 0; --p) q = (p < n && id[p] == j) ? p : q;
     for (int p = PEEK; q < 0 && p < n; ++p)
         if (list[(int64_t)p * SS + i].j == j) q = p;
     return q;
 }
 
 // Slot of partner g among the first n entries of the slot-major list row kp
-// (outer oj or inner ik), or -1; the first LIST_PEEK entries are read at
-// once, the rest one by one.
+// (outer oj or inner ik), or -1; the first LIST_PEEK entries are requested
+// together (as find_partner), the rest read one by one.
 __device__ __forceinline__ int find_entry(const int64_t* list, int64_t SS, int kp, int n, int g) {
     int id[LIST_PEEK];
 #pragma unroll
-    for (int s = 0; s < LIST_PEEK; ++s) id[s] = s < n ? ent_id(list[(int64_t)s * SS + kp]) : -1;
+    for (int s = 0; s < LIST_PEEK; ++s) id[s] = -1;
+    if (n > 0) {
+#pragma unroll
+        for (int s = 0; s < LIST_PEEK; ++s) id[s] = ent_id(list[(int64_t)min(s, n - 1) * SS + kp]);
+    }
     int r = -1;
 #pragma unroll
-    for (int s = LIST_PEEK - 1; s >= 0; --s) r = id[s] == g ? s : r;
+    for (int s = LIST_PEEK - 1; s >= 0; --s) r = (s < n && id[s] == g) ? s : r;
     for (int s = LIST_PEEK; r < 0 && s < n; ++s)
         if (ent_id(list[(int64_t)s * SS + kp]) == g) r = s;
     return r;
@@ -1550,7 +1560,7 @@ struct ContactOut {
 constexpr int CONTACT_LDS_PER_WAVE = 64 * (int)sizeof(ContactOut);
 
 template <bool MIXED>
-__global__ __launch_bounds__(FLOE_BLOCK) void k_contact(DS S) {
+__global__ __launch_bounds__(FLOE_BLOCK, 3) void k_contact(DS S) {
     extern __shared__ unsigned char contact_lds[];
     ContactOut* co = reinterpret_cast<ContactOut*>(contact_lds + (threadIdx.x >> 6) * CONTACT_LDS_PER_WAVE);
     if (*S.err) return;
@@ -1815,7 +1825,7 @@ constexpr int BOND_TCAP = 256;
 // bonds form this step (step 1) append new records in candidate order and
 // walk their own lists.
 template <bool MIXED>
-__global__ __launch_bounds__(FLOE_BLOCK) void k_bond(DS S) {
+__global__ __launch_bounds__(FLOE_BLOCK, 4) void k_bond(DS S) {
     __shared__ uint32_t recs[FLOE_BLOCK / 64][BOND_TCAP];
     if (*S.err) return;
     const int k = queue_entry(S, 1);
@@ -1823,24 +1833,35 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_bond(DS S) {
     const int lane = (int)(threadIdx.x & 63);
     uint32_t* rl = recs[threadIdx.x >> 6];
     const int64_t SS = S.S;
-    int m = 0, fs = 0, no = 0, bco = 0;
+    int m = 0, fs = 0, no = 0, bco = 0, fid = 0, tstep = 0, first = 0;
     bool bond_step = false;
+    // partners of the first LIST_PEEK records, requested together
+    int id[LIST_PEEK];
     if (act) {
-        m = S.geo[k].member;
-        const DevParams& P = S.params[m];
-        bond_step = S.members[m].tstep == 1 && P.restart != 1 && P.cohesion;
+        const GeoRec g = S.geo[k];
         fs = S.cval[k];
         no = S.ocnt[k];
         bco = S.bcnt_old[fs];
+        // (a queued owner has at least one record)
+#pragma unroll
+        for (int q = 0; q < LIST_PEEK; ++q) id[q] = S.bond_old[(int64_t)min(q, max(bco - 1, 0)) * SS + fs].j;
+        m = g.member;
+        fid = g.id;
+        // (no short-circuit tests: the loads go out together)
+        const DevParams& P = S.params[m];
+        const Member Mb = S.members[m];
+        const int rst = P.restart, coh = P.cohesion;
+        tstep = Mb.tstep;
+        first = Mb.first;
+        bond_step = (tstep == 1) & (rst != 1) & (coh != 0);
+#pragma unroll
+        for (int q = 0; q < LIST_PEEK; ++q) id[q] = q < bco ? id[q] : -1;
     }
     // the live records of the owners that form no bonds this step
     int nlive = 0;
     uint32_t over = 0;
     const bool listed = act && !bond_step;
     if (listed) {
-        int id[LIST_PEEK];
-#pragma unroll
-        for (int q = 0; q < LIST_PEEK; ++q) id[q] = q < bco ? S.bond_old[(int64_t)q * SS + fs].j : -1;
 #pragma unroll
         for (int q = 0; q < LIST_PEEK; ++q) nlive += id[q] >= 0;
         for (int q = LIST_PEEK; q < bco; ++q) nlive += S.bond_old[(int64_t)q * SS + fs].j >= 0;
@@ -1856,11 +1877,15 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_bond(DS S) {
     if (fits) {
         if (listed) {
             int at = incl - nlive, rq = 0;
-            for (int q = 0; q < bco; ++q)
-                if (S.bond_old[(int64_t)q * SS + fs].j >= 0) {
-                    rl[at++] = (uint32_t)lane | ((uint32_t)q << 6) | ((uint32_t)rq << 19);
-                    ++rq;
-                }
+            auto list_rec = [&](int q) {
+                rl[at++] = (uint32_t)lane | ((uint32_t)q << 6) | ((uint32_t)rq << 19);
+                ++rq;
+            };
+#pragma unroll
+            for (int q = 0; q < LIST_PEEK; ++q)
+                if (id[q] >= 0) list_rec(q);    // (-1 past the count)
+            for (int q = LIST_PEEK; q < bco; ++q)
+                if (S.bond_old[(int64_t)q * SS + fs].j >= 0) list_rec(q);
             S.bcnt_new[fs] = nlive;
         }
         wave_lds_sync();
@@ -1868,16 +1893,17 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_bond(DS S) {
             const int idx = c0 + lane;
             const uint32_t w = idx < total ? rl[idx] : (uint32_t)lane;
             const int own = (int)(w & 63);
+            // the owner's member, step and identity, from its lane
             const int ko = __shfl(k, own), mo = __shfl(m, own), fso = __shfl(fs, own);
-            const int noo = __shfl(no, own);
+            const int noo = __shfl(no, own), fo = __shfl(fid, own);
+            const int tso = __shfl(tstep, own), fro = __shfl(first, own);
             if (idx < total) {
                 const int q = (int)((w >> 6) & 0x1FFF), rq = (int)(w >> 19);
                 const DevParams& P = S.params[mo];
                 const Floe F0 = floe_at(S, ko);
                 const BondRec b = S.bond_old[(int64_t)q * SS + fso];
                 int nb = 0;
-                if (bond_record<MIXED>(S, P, S.members[mo].tstep, S.members[mo].first, S.geo[ko].id,
-                                       ko, fso, F0, noo, b, rq, nb, over))
+                if (bond_record<MIXED>(S, P, tso, fro, fo, ko, fso, F0, noo, b, rq, nb, over))
                     S.bgat[ko] = 1;
             }
         }
@@ -1925,26 +1951,40 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_bond(DS S) {
 // where it is the inner index (ascending outer index), then its own, in
 // distance order (stepper.f90:165-198).  k_final calls it for the floes the
 // repair left alone: their flags and terms are k_bond's.
-__device__ __forceinline__ void gather_bonds(const DS& S, int k, double* a) {
+__device__ __forceinline__ void gather_bonds(const DS& S, int k, int ni, int no, double* a) {
     const int64_t SS = S.S;
-    const int ni = S.icnt[k], no = S.ocnt[k];
-    // the first LIST_PEEK slots' bond words of both sides are read at once
-    int rqi[LIST_PEEK], rqo[LIST_PEEK];
+    // the first LIST_PEEK slots' bond words and inner partners of both sides
+    // are requested together (slots past a count stand in for its last one)
+    int rqi[LIST_PEEK], rqo[LIST_PEEK], pki[LIST_PEEK];
+#pragma unroll
+    for (int q = 0; q < LIST_PEEK; ++q) rqi[q] = rqo[q] = pki[q] = 0;
+    if (ni > 0) {
+#pragma unroll
+        for (int q = 0; q < LIST_PEEK; ++q) {
+            const int64_t r = (int64_t)min(q, ni - 1) * SS + k;
+            rqi[q] = S.ibond[r];
+            pki[q] = ent_pos(S.ik[r]);
+        }
+    }
+    if (no > 0) {
+#pragma unroll
+        for (int q = 0; q < LIST_PEEK; ++q) rqo[q] = (int)S.evf[(int64_t)min(q, no - 1) * SS + k];
+    }
 #pragma unroll
     for (int q = 0; q < LIST_PEEK; ++q) {
-        rqi[q] = q < ni ? S.ibond[q * SS + k] : 0;
-        rqo[q] = q < no ? (int)S.evf[q * SS + k] : 0;
+        rqi[q] = q < ni ? rqi[q] : 0;
+        rqo[q] = q < no ? rqo[q] : 0;
     }
-    auto inner = [&](int q, int rq) {
-        if (rq) add_bond_j(a, S.bterm[(int64_t)(rq - 1) * SS + ent_pos(S.ik[q * SS + k])]);
+    auto inner = [&](int pk, int rq) {
+        if (rq) add_bond_j(a, S.bterm[(int64_t)(rq - 1) * SS + pk]);
     };
     auto outer = [&](uint32_t fl) {
         const int rq = (int)(fl >> 16);
         if ((fl & EV_BONDED) && rq >= 1 && rq <= S.BC) add_bond_i(a, S.bterm[(int64_t)(rq - 1) * SS + k]);
     };
 #pragma unroll
-    for (int q = 0; q < LIST_PEEK; ++q) inner(q, rqi[q]);
-    for (int q = LIST_PEEK; q < ni; ++q) inner(q, S.ibond[q * SS + k]);
+    for (int q = 0; q < LIST_PEEK; ++q) inner(pki[q], rqi[q]);
+    for (int q = LIST_PEEK; q < ni; ++q) inner(ent_pos(S.ik[q * SS + k]), S.ibond[q * SS + k]);
 #pragma unroll
     for (int s = 0; s < LIST_PEEK; ++s) outer((uint32_t)rqo[s]);
     for (int s = LIST_PEEK; s < no; ++s) outer(S.evf[s * SS + k]);
@@ -2784,25 +2824,28 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_stress(DS S) {
 //    (stepper.f90:295-343, integration.f90:1-38)
 // ---------------------------------------------------------------------
 template <bool MIXED>
-__global__ __launch_bounds__(FLOE_BLOCK) void k_final(DS S) {
+__global__ __launch_bounds__(FLOE_BLOCK, 4) void k_final(DS S) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= S.N || *S.err) return;
+    if (k >= S.N) return;
+    // the position's rows are requested at once, before anything is tested
+    const GeoRec me = S.geo[k];
+    const int f = S.cval[k];        // storage slot
+    Floe F = floe_at(S, k);
+    const uint8_t wf = S.wflag[k], cl = S.cls[k], bg = S.bgat[k];
+    const int ni = S.icnt[k], no = S.ocnt[k];
+    if (*S.err) return;
     // every earlier kernel of the step has finished: the step is complete
     if (k == 0) *S.steps_done += 1;
-    const GeoRec me = S.geo[k];
     const int m = me.member;
     if (m < 0) return;
-    const int f = S.cval[k];        // storage slot
     const DevParams& P = S.params[m];
     const int64_t SS = S.S;
-    Floe F = floe_at(S, k);
     const uint8_t dflags = S.dirty[f];
     const bool dirty = (dflags & DIRTY_SHAPE) != 0;
     if (dirty) {
         F.r = S.cur[0 * SS + f]; F.h = S.cur[1 * SS + f]; F.mass = S.cur[2 * SS + f];
         F.inertia = S.cur[3 * SS + f]; F.hfa = S.cur[4 * SS + f]; F.hfw = S.cur[5 * SS + f];
     }
-    const uint8_t wf = S.wflag[k];
     WallOut W;
     if (wf & WF_REPAIR_WALL) {
         W = S.wtab[S.members[m].first + S.ovw[k]];   // recomputed by the repair
@@ -2826,7 +2869,6 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_final(DS S) {
         if (had != now) S.wnz[f] = now;
     }
     double a[NACC];
-    const uint8_t cl = S.cls[k];
 #pragma unroll
     for (int c = 0; c < NACC; ++c) {
         const bool bond_row = c == A_FBX || c == A_FBY || c == A_MB || c == A_TAB || c == A_PB;
@@ -2837,7 +2879,7 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_final(DS S) {
         a[c] = valid ? S.acc[c * SS + k] : 0.0;
     }
     // bond terms: rows the fix-up wrote (CLS_BOND), else gathered here
-    if (!(cl & CLS_BOND) && S.bgat[k]) gather_bonds(S, k, a);
+    if (!(cl & CLS_BOND) && bg) gather_bonds(S, k, ni, no, a);
     // shelter minima (reset_shelter + thread reduction); with shelter off
     // the reference's minima stay at their zero initial value
     double sha = P.shelter ? a[A_SHA] : 0.0;

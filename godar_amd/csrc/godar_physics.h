@@ -169,6 +169,11 @@ template <bool MIXED>
 __device__ __forceinline__ void contact_part(const DevParams& P, Floe& fi, Floe& fj, const Kin& k,
                                              PairState& ps, EventOut& o) {
     using T = typename Prec<MIXED>::T;
+    // the switches and limits are requested here, at once, and the plastic
+    // and Coulomb tests are evaluated outside their branches, so no
+    // parameter load waits behind a test
+    const int ridging = P.ridging, slipping = P.slipping;
+    const T snc = T(P.sigmanc_crit), fric = T(P.friction_coeff);
     o.flags |= EV_CONTACT;
     const double dist = k.dist;
     ps.deltat = -k.velt * P.dt + ps.deltat;
@@ -187,17 +192,18 @@ __device__ __forceinline__ void contact_part(const DevParams& P, Floe& fi, Floe&
     T gamr = gamn * dr2 / T(12);
     T fcn = dmax(knc * T(k.deltan) - gamn * T(k.veln), T(0));
     T fct = ktc * T(ps.deltat) - gamt * T(k.velt);
-    if (P.ridging) {
-        if (T(P.sigmanc_crit) * hmin <= fcn / dr / hmin) {
+    const bool plastic = (ridging != 0) & (snc * hmin <= fcn / dr / hmin);
+    {
+        if (plastic) {
             // plastic_contact, ridging.f90:1-33
             T h2 = hmin * hmin;
-            T kp = T(P.sigmanc_crit) * h2 * dr / T(k.deltan);
+            T kp = snc * h2 * dr / T(k.deltan);
             ktc = T(P.k6) * kp;
             krc = kp * dr2 / T(12);
             gamn = -(T(P.beta) * gsqrt(T(4) * kp * m_redu));
             gamt = -(T(P.two_beta) * gsqrt(T(2.0 / 3.0) * ktc * m_redu));
             gamr = gamn * dr2 / T(12);
-            fcn = dmin(fcn, T(P.sigmanc_crit) * h2 * dr);
+            fcn = dmin(fcn, snc * h2 * dr);
             fct = ktc * T(ps.deltat) - gamt * T(k.velt);
             // update_shape, ridging.f90:36-99 (FP64 volume bookkeeping)
             double hmin64 = fmin(fi.h, fj.h);
@@ -239,14 +245,11 @@ __device__ __forceinline__ void contact_part(const DevParams& P, Floe& fi, Floe&
         }
     }
     // coulomb, physics.f90:1-33
-    T fcr;
-    if (gabs(fct) > T(P.friction_coeff) * gabs(fcn)) {
-        fct = -(T(P.friction_coeff) * gabs(fcn) * gcopysign(T(1), T(k.velt)));
-        fcr = fct;
-        if (P.slipping) ps.deltat = double((fct + gamt * T(k.velt)) / ktc);
-    } else {
-        fcr = fct;
-    }
+    const bool coul = gabs(fct) > fric * gabs(fcn);
+    fct = coul ? -(fric * gabs(fcn) * gcopysign(T(1), T(k.velt))) : fct;
+    T fcr = fct;
+    const double slip_dt = double((fct + gamt * T(k.velt)) / ktc);
+    ps.deltat = (coul & (slipping != 0)) ? slip_dt : ps.deltat;
     // rolling moment: bonds only form in step 1, before any contact of the
     // pair, so a bonded pair's mcc is still its initial zero
     T mcc = T(0);
@@ -280,15 +283,20 @@ __device__ __forceinline__ void bond_part(const DevParams& P, const Floe& fi, co
                                           const Kin& k, PairState& ps, EventOut& o) {
     using T = typename Prec<MIXED>::T;
     if (ps.bond == 1) {
-        ps.deltanb = k.veln * P.dt + ps.deltanb;
-        ps.deltatb = k.veltb * P.dt + ps.deltatb;
-        ps.trb_ji = fi.omega * P.dt + ps.trb_ji;
-        ps.trb_ij = fj.omega * P.dt + ps.trb_ij;
+        // every parameter of the bond law is requested here, at once: the
+        // tests below are selects, so no parameter load waits in a branch
+        const double dt = P.dt, dtd = P.dtd, dth = P.dth, dmax = P.dmax;
+        const T eb = T(P.eb), tau_c = T(P.tau_crit), scb_c = T(P.sigmacb_crit);
+        const T stb_c = T(P.sigmatb_crit), llb = T(P.lambda_lb);
+        ps.deltanb = k.veln * dt + ps.deltanb;
+        ps.deltatb = k.veltb * dt + ps.deltatb;
+        ps.trb_ji = fi.omega * dt + ps.trb_ji;
+        ps.trb_ij = fj.omega * dt + ps.trb_ij;
         T m_redu = T(fi.mass) * T(fj.mass) / (T(fi.mass) + T(fj.mass));
         T i_redu = T(fi.inertia) * T(fj.inertia) / (T(fi.inertia) + T(fj.inertia));
         T L = T(ps.lb);
-        T EA = (T(1) - T(ps.damage)) * T(P.eb) * T(ps.sb);
-        T EI = (T(1) - T(ps.damage)) * T(P.eb) * T(ps.ib);
+        T EA = (T(1) - T(ps.damage)) * eb * T(ps.sb);
+        T EI = (T(1) - T(ps.damage)) * eb * T(ps.ib);
         T L2 = L * L, L3 = L * L * L;
         T k_axial = EA / L;
         T k_shear1 = T(12) * EI / L3;
@@ -315,34 +323,34 @@ __device__ __forceinline__ void bond_part(const DevParams& P, const Floe& fi, co
         T taub = gabs(fbt) / sb;
         T sigmatb = -fbn / sb + gabs(mbb_ji) * rb / ib;
         T sigmacb = fbn / sb + gabs(mbb_ji) * rb / ib;
-        T t = taub / (T(P.tau_crit) * hb);
+        T t = taub / (tau_c * hb);
         T phi = t * t;
-        if (sigmacb > 0) { T q = sigmacb / (T(P.sigmacb_crit) * hb); phi = phi + q * q; }
-        if (sigmatb > 0) { T q = sigmatb / (T(P.sigmatb_crit) * hb); phi = phi + q * q; }
+        const T qcb = sigmacb / (scb_c * hb), qtb = sigmatb / (stb_c * hb);
+        phi = sigmacb > 0 ? phi + qcb * qcb : phi;
+        phi = sigmatb > 0 ? phi + qtb * qtb : phi;
         T psi = dmin(T(1), T(1) / gsqrt(phi));
         double D = ps.damage;
         double psid = double(psi);
-        ps.damage = D + (1.0 - psid) * (1.0 - D) * P.dt / P.dtd - D * P.dt / P.dth;
-        if (ps.damage >= P.dmax) {
+        ps.damage = D + (1.0 - psid) * (1.0 - D) * dt / dtd - D * dt / dth;
+        if (ps.damage >= dmax) {
             ps.bond = 0;
             fbn = T(0); fbt = T(0); mbb_ji = T(0);
             ps.damage = 1.0;
         }
         o.sbd = double(gsqrt(fbn * fbn + fbt * fbt));
-        if (ps.bond == 1) {
-            o.flags |= EV_BONDED;
-            T c = T(k.cosa), s = T(k.sina);
-            o.fbn_c = double(fbn * c);
-            o.fbt_s = double(fbt * s);
-            o.fbn_s = double(fbn * s);
-            o.fbt_c = double(fbt * c);
-            o.lri_fbt = double(T(P.lambda_lb) * T(fi.r) * fbt);
-            o.lrj_fbt = double(T(P.lambda_lb) * T(fj.r) * fbt);
-            o.mbb_ji = double(mbb_ji);
-            o.mbb_ij = double(mbb_ij);
-            o.sb = ps.sb;
-            o.fbn_sb = double(fbn) * ps.sb;
-        }
+        // the terms are written either way (read only where EV_BONDED is set)
+        if (ps.bond == 1) o.flags |= EV_BONDED;
+        T c = T(k.cosa), s = T(k.sina);
+        o.fbn_c = double(fbn * c);
+        o.fbt_s = double(fbt * s);
+        o.fbn_s = double(fbn * s);
+        o.fbt_c = double(fbt * c);
+        o.lri_fbt = double(llb * T(fi.r) * fbt);
+        o.lrj_fbt = double(llb * T(fj.r) * fbt);
+        o.mbb_ji = double(mbb_ji);
+        o.mbb_ij = double(mbb_ij);
+        o.sb = ps.sb;
+        o.fbn_sb = double(fbn) * ps.sb;
     } else if (ps.has_bond_rec) {
         ps.trb_ji = ps.trb_ij = ps.deltanb = ps.deltatb = 0.0;
     }
@@ -543,51 +551,50 @@ __device__ __forceinline__ void forcing(const DevParams& P, const Floe& f, doubl
                                         double uwi, double vwi, double sca_d, double scw_d,
                                         Forcing& o) {
     using T = typename Prec<MIXED>::T;
+    // the parameters are requested at once, and the zero-speed and
+    // low-speed cases are selects (no parameter load waits in a branch)
+    const T z0w = T(P.z0w), l10 = T(P.log10z0), pi = T(P.pi);
+    const T rhoair = T(P.rhoair), rhow = T(P.rhowater);
+    const T cda = T(P.Cdair), csa = T(P.Csair), cdw = T(P.Cdwater), csw = T(P.Cswater);
+    const double sinlat = P.sinlat;
     T sca = T(sca_d), scw = T(scw_d);
     T dua = T(uai - f.u), dva = T(vai - f.v), duw = T(uwi - f.u), dvw = T(vwi - f.v);
     T avn = gsqrt(dua * dua + dva * dva);
     T ovn = gsqrt(duw * duw + dvw * dvw);
-    T la = glog(T(f.hfa) / T(P.z0w)) / T(P.log10z0);
-    T lw = glog(T(f.hfw) / T(P.z0w)) / T(P.log10z0);
+    T la = glog(T(f.hfa) / z0w) / l10;
+    T lw = glog(T(f.hfw) / z0w) / l10;
     T alog = la * la, olog = lw * lw;
     T r = T(f.r), om = T(f.omega), aom = gabs(om);
     T r2 = r * r;
-    T pi = T(P.pi);
-    T rhoair = T(P.rhoair), rhow = T(P.rhowater);
-    T fdax, fday, fsax, fsay, fdwx, fdwy, fswx, fswy;
-    if (avn == T(0)) {
-        fdax = fday = fsax = fsay = T(0);
-    } else {
-        T hfa = T(f.hfa);
-        fdax = rhoair * T(P.Cdair) * hfa * r * dua * avn * alog * sca * pi / T(2);
-        fday = rhoair * T(P.Cdair) * hfa * r * dva * avn * alog * sca * pi / T(2);
-        fsax = rhoair * pi * r2 * T(P.Csair) * dua / avn * (avn * avn + r2 * (aom * aom) / T(4));
-        fsay = rhoair * pi * r2 * T(P.Csair) * dva / avn * (avn * avn + r2 * (aom * aom) / T(4));
-    }
-    if (ovn == T(0)) {
-        fdwx = fdwy = fswx = fswy = T(0);
-    } else {
-        T hfw = T(f.hfw);
-        fdwx = rhow * T(P.Cdwater) * hfw * r * duw * ovn * olog * scw * pi / T(2);
-        fdwy = rhow * T(P.Cdwater) * hfw * r * dvw * ovn * olog * scw * pi / T(2);
-        fswx = rhow * pi * r2 * T(P.Cswater) * duw / ovn * (ovn * ovn + r2 * (aom * aom) / T(4));
-        fswy = rhow * pi * r2 * T(P.Cswater) * dvw / ovn * (ovn * ovn + r2 * (aom * aom) / T(4));
-    }
+    const T hfa = T(f.hfa), hfw = T(f.hfw);
+    // (a zero relative speed gives zero drag: the terms below are not used)
+    const bool a0 = avn == T(0), w0 = ovn == T(0);
+    T fdax = rhoair * cda * hfa * r * dua * avn * alog * sca * pi / T(2);
+    T fday = rhoair * cda * hfa * r * dva * avn * alog * sca * pi / T(2);
+    T fsax = rhoair * pi * r2 * csa * dua / avn * (avn * avn + r2 * (aom * aom) / T(4));
+    T fsay = rhoair * pi * r2 * csa * dva / avn * (avn * avn + r2 * (aom * aom) / T(4));
+    T fdwx = rhow * cdw * hfw * r * duw * ovn * olog * scw * pi / T(2);
+    T fdwy = rhow * cdw * hfw * r * dvw * ovn * olog * scw * pi / T(2);
+    T fswx = rhow * pi * r2 * csw * duw / ovn * (ovn * ovn + r2 * (aom * aom) / T(4));
+    T fswy = rhow * pi * r2 * csw * dvw / ovn * (ovn * ovn + r2 * (aom * aom) / T(4));
+    fdax = a0 ? T(0) : fdax; fday = a0 ? T(0) : fday;
+    fsax = a0 ? T(0) : fsax; fsay = a0 ? T(0) : fsay;
+    fdwx = w0 ? T(0) : fdwx; fdwy = w0 ? T(0) : fdwy;
+    fswx = w0 ? T(0) : fswx; fswy = w0 ? T(0) : fswy;
     o.fax = double(fdax + fsax);
     o.fay = double(fday + fsay);
     o.fwx = double(fdwx + fswx);
     o.fwy = double(fdwy + fswy);
     T r4 = r * r * r * r, r5 = r * r * r * r * r;
-    T ma, mw;
-    if (avn < T(1e-2)) ma = -(T(2) * pi / T(5) * r5 * rhoair * T(P.Csair) * om * aom);
-    else ma = -(T(3.0 / 4.0) * pi * rhoair * T(P.Csair) * avn * om * r4);
-    if (ovn < T(1e-3)) mw = -(T(2) * pi / T(5) * r5 * rhow * T(P.Cswater) * om * aom);
-    else mw = -(T(3.0 / 4.0) * pi * rhow * T(P.Cswater) * ovn * om * r4);
-    o.ma = double(ma);
-    o.mw = double(mw);
+    const T ma_lo = -(T(2) * pi / T(5) * r5 * rhoair * csa * om * aom);
+    const T ma_hi = -(T(3.0 / 4.0) * pi * rhoair * csa * avn * om * r4);
+    const T mw_lo = -(T(2) * pi / T(5) * r5 * rhow * csw * om * aom);
+    const T mw_hi = -(T(3.0 / 4.0) * pi * rhow * csw * ovn * om * r4);
+    o.ma = double(avn < T(1e-2) ? ma_lo : ma_hi);
+    o.mw = double(ovn < T(1e-3) ? mw_lo : mw_hi);
     // coriolis (omega_earth = 7.2921d-5), FP64
-    o.fcorx = f.mass * 2.0 * 7.2921e-5 * P.sinlat * f.v;
-    o.fcory = -(f.mass * 2.0 * 7.2921e-5 * P.sinlat * f.u);
+    o.fcorx = f.mass * 2.0 * 7.2921e-5 * sinlat * f.v;
+    o.fcory = -(f.mass * 2.0 * 7.2921e-5 * sinlat * f.u);
 }
 
 // Bilinear sample of one frame; the same expression as the generator's
