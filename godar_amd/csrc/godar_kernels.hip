@@ -2339,12 +2339,19 @@ __global__ __launch_bounds__(64) void k_repair(DS S, int nmembers) {
     // EventOut-sized slot per lane; a wall lane's WallOut uses its start)
     __shared__ uint64_t stage_buf[64 * sizeof(EventOut) / 8];
     __shared__ int tcomp[64];      // components of a round's reshaping lanes so far
+    // the member's parameters, copied to LDS: the physics reads them at LDS
+    // latency (this wavefront is the only one working on the member)
+    __shared__ uint64_t pbuf[sizeof(DevParams) / 8];
+    static_assert(sizeof(DevParams) % 8 == 0 && sizeof(DevParams) / 8 <= 64, "DevParams copied by lane");
     const int m = blockIdx.x;
     if (m >= nmembers || *S.err) return;
     if (!member_live(S.members[m])) return;
     int ns = S.nseeds[m];
     if (ns == 0) return;
-    const DevParams& P = S.params[m];
+    if ((int)threadIdx.x < (int)(sizeof(DevParams) / 8))
+        pbuf[threadIdx.x] = reinterpret_cast<const uint64_t*>(S.params + m)[threadIdx.x];
+    __syncthreads();
+    const DevParams& P = *reinterpret_cast<const DevParams*>(pbuf);
     const int first = S.members[m].first;
     const int nmem = S.members[m].n;
     const int tstep = S.members[m].tstep;
