@@ -453,6 +453,14 @@ static dim3 grid_for(int N, int B, bool xcd) {
     if (xcd) g = (g + XCD_COUNT - 1) / XCD_COUNT * XCD_COUNT;
     return dim3(g);
 }
+// Queue kernels loop over their entries: at most QUEUE_GRID_WAVES
+// wavefronts (about one round of resident wavefronts), so that short queues
+// do not launch a wavefront per 64 floes that finds nothing to do.
+static dim3 grid_queue(int N, int B) {
+    const int cap = QUEUE_GRID_WAVES * 64 / B / XCD_COUNT * XCD_COUNT;
+    const dim3 g = grid_for(N, B, true);
+    return dim3(std::max(XCD_COUNT, std::min((int)g.x, cap)));
+}
 
 constexpr int LEAN_RUN = 15;       // lean steps between two full ones
 constexpr int FULL_HOLD = 64;      // full steps after a lean step found a build due
@@ -519,23 +527,23 @@ int launch_step(godar_ctx* c, bool lean) {
     {
         StageTimer t(c, 4);
         // LDS: one pass of 64 pair results per wavefront
-        hipLaunchKernelGGL(mixed ? k_contact<true> : k_contact<false>, grid_for(N, B[4], true),
+        hipLaunchKernelGGL(mixed ? k_contact<true> : k_contact<false>, grid_queue(N, B[4]),
                            dim3(B[4]), (B[4] / 64) * CONTACT_LDS_PER_WAVE, s, d);
     }
     {
         StageTimer t(c, 5);
-        hipLaunchKernelGGL(mixed ? k_bond<true> : k_bond<false>, grid_for(N, B[5], true),
+        hipLaunchKernelGGL(mixed ? k_bond<true> : k_bond<false>, grid_queue(N, B[5]),
                            dim3(B[5]), 0, s, d);
     }
     {
         StageTimer t(c, 7);
-        hipLaunchKernelGGL(mixed ? k_walls<true> : k_walls<false>, grid_for(N, B[7], true),
+        hipLaunchKernelGGL(mixed ? k_walls<true> : k_walls<false>, grid_queue(N, B[7]),
                            dim3(B[7]), 0, s, d);
     }
     {
         StageTimer t(c, 8);
         // contact components (roots), then the ordered repair
-        hipLaunchKernelGGL(k_comp, grid_for(N, FLOE_BLOCK, true), dim3(FLOE_BLOCK), 0, s, d);
+        hipLaunchKernelGGL(k_comp, grid_queue(N, FLOE_BLOCK), dim3(FLOE_BLOCK), 0, s, d);
         hipLaunchKernelGGL(mixed ? k_repair<true> : k_repair<false>, dim3(nm), dim3(64), 0, s, d, nm);
     }
     {

@@ -23,6 +23,7 @@ constexpr int XCD_COUNT = 8;      // accelerator complex dies of an MI355X
 constexpr int NQUEUE = 4;         // work queues: contact, bond, wall, dense search
 constexpr int QN_STRIDE = 64;     // ints between queue counters (256 B)
 constexpr int SCAN_TILE = 1024;   // positions per tile of the superset offset scan
+constexpr int QUEUE_GRID_WAVES = 4096;   // wavefronts launched for a queue kernel at most
 
 // Per-floe record (96 bytes) by storage slot.  Between builds of the cell
 // order the floe state lives in the ordered copies below, which the step

@@ -100,6 +100,18 @@ __device__ __forceinline__ int queue_at(const DS& S, int c, const QueueView& v, 
     }
     return S.qlist[(int64_t)c * S.S + idx];
 }
+// Slot of this thread's it-th entry of a queue of length qn, or -1: the
+// grid may be smaller than the queue (the host caps the queue kernels' grids
+// at QUEUE_GRID_WAVES wavefronts), so each thread takes every
+// (threads per XCD)-th entry of its XCD's share.
+__device__ __forceinline__ int queue_slot_it(int qn, int it) {
+    const int per = (qn + XCD_COUNT - 1) / XCD_COUNT;
+    const int local = ((int)(blockIdx.x / XCD_COUNT) + it * (int)(gridDim.x / XCD_COUNT)) * (int)blockDim.x +
+                      (int)threadIdx.x;
+    if (local >= per) return -1;
+    const int t = (int)(blockIdx.x % XCD_COUNT) * per + local;
+    return t < qn ? t : -1;
+}
 // the queue entry of this thread (split over the XCDs as queue_slot), or -1
 __device__ __forceinline__ int queue_entry(const DS& S, int c) {
     const QueueView v = queue_view(S, c);
@@ -775,16 +787,35 @@ __device__ __forceinline__ void cand_thread(const DS& S, int k, const GeoRec& me
         tn[lane] = 0;
         wave_lds_sync();
         bool ovf = false;
-        for (int c0 = 0; c0 < totalL; c0 += 64) {
-            const int idx = c0 + lane;
-            // the entry's owner: the last lane whose superset starts at or
-            // before it (lanes without entries share the next one's start)
+        // an entry's owner: the last lane whose superset starts at or before
+        // it (lanes without entries share the next one's start); idx < totalL
+        auto owner_of = [&](int idx) {
             int own = 0;
 #pragma unroll
             for (int step = 32; step > 0; step >>= 1)
                 if (__shfl(start, own + step) <= idx) own += step;
-            const int ostart = __shfl(start, own);
-            const int64_t ooff = __shfl(soff, own);
+            return own;
+        };
+        auto entry_at = [&](int idx, int own) -> int64_t {
+            return S.sup[__shfl(soff, own) + (idx - __shfl(start, own))];
+        };
+        // each round's superset entries are requested during the round
+        // before (a lane past the end reads the last entry again)
+        int own_nxt = 0;
+        int64_t e_nxt = 0;
+        if (totalL > 0) {
+            own_nxt = owner_of(min(lane, totalL - 1));
+            e_nxt = entry_at(min(lane, totalL - 1), own_nxt);
+        }
+        for (int c0 = 0; c0 < totalL; c0 += 64) {
+            const int idx = c0 + lane;
+            const int own = own_nxt;
+            const int64_t e = e_nxt;
+            if (c0 + 64 < totalL) {
+                const int idn = min(idx + 64, totalL - 1);
+                own_nxt = owner_of(idn);
+                e_nxt = entry_at(idn, own_nxt);
+            }
             const double ox = __shfl(mx, own), oy = __shfl(my, own), orad = __shfl(rad, own);
             const double ort = __shfl(rtree, own);
             const int of = __shfl(f, own);
@@ -792,7 +823,7 @@ __device__ __forceinline__ void cand_thread(const DS& S, int k, const GeoRec& me
             double d = 0.0;
             int j = 0, q = 0;
             if (idx < totalL) {
-                q = ent_pos(S.sup[ooff + (idx - ostart)]);
+                q = ent_pos(e);
                 const GeoRec o = S.geo[q];
                 const double dx = ox - o.x, dy = oy - o.y;
                 const double dist2 = dx * dx + dy * dy;
@@ -1526,18 +1557,18 @@ __device__ __forceinline__ void load_contact_state(const DS& S, int i, int j, Pa
     ps.bond = 0; ps.has_bond_rec = false;
     ps.damage = ps.deltanb = ps.deltatb = ps.trb_ji = ps.trb_ij = 0.0;
     ps.rb = ps.hb = ps.lb = ps.sb = ps.ib = 0.0;
-    const int hc = S.hcnt_old[i];
-    for (int q = 0; q < hc; ++q) {
-        const HistRec& hr = S.hist_old[(int64_t)q * S.S + i];
-        if (hr.j == j) {
-            ps.deltat = hr.deltat; ps.thetarelc = hr.thetarelc; ps.fcn = hr.fcn; ps.fct = hr.fct;
-            break;
-        }
+    // both lists are searched at once, the first two records of each
+    // requested together
+    const int hc = S.hcnt_old[i], bc = S.bcnt_old[i];
+    const int hq = find_partner<2>(S.hist_old, S.S, i, hc, j);
+    const int bq = find_partner<2>(S.bond_old, S.S, i, bc, j);
+    if (hq >= 0) {
+        const HistRec& hr = S.hist_old[(int64_t)hq * S.S + i];
+        ps.deltat = hr.deltat; ps.thetarelc = hr.thetarelc; ps.fcn = hr.fcn; ps.fct = hr.fct;
     }
-    const int bc = S.bcnt_old[i];
-    for (int q = 0; q < bc; ++q) {
-        const BondRec& b = S.bond_old[(int64_t)q * S.S + i];
-        if (b.j == j) { ps.has_bond_rec = true; ps.bond = b.bond; break; }
+    if (bq >= 0) {
+        ps.has_bond_rec = true;
+        ps.bond = S.bond_old[(int64_t)bq * S.S + i].bond;
     }
 }
 
@@ -1565,187 +1596,190 @@ __global__ __launch_bounds__(FLOE_BLOCK, 3) void k_contact(DS S) {
     ContactOut* co = reinterpret_cast<ContactOut*>(contact_lds + (threadIdx.x >> 6) * CONTACT_LDS_PER_WAVE);
     if (*S.err) return;
     const int lane = (int)(threadIdx.x & 63);
-    const int k = queue_entry(S, 0);
-    const bool act = k >= 0;
-    const int64_t SS = S.S;
-    int m = 0, f = 0, fs = 0, ni = 0, no = 0;
-    if (act) {
-        const GeoRec me = S.geo[k];
-        m = me.member;
-        f = me.id;                  // identity (ordering, bond mask)
-        fs = S.cval[k];             // storage slot of the floe's pair state
-        ni = S.icnt[k];
-        no = S.ocnt[k];
-    }
-    const int cnt = ni + no;
-    int incl = cnt;
+    const QueueView qv = queue_view(S, 0);
+    // (the wavefront loops while any lane has an entry: its lanes share LDS
+    // and shuffles)
+    for (int it = 0;; ++it) {
+        const int qt = queue_slot_it(qv.total, it);
+        if (__ballot(qt >= 0) == 0) break;
+        const int k = qt >= 0 ? queue_at(S, 0, qv, qt) : -1;
+        const bool act = k >= 0;
+        const int64_t SS = S.S;
+        int m = 0, f = 0, fs = 0, ni = 0, no = 0;
+        if (act) {
+            const GeoRec me = S.geo[k];
+            m = me.member;
+            f = me.id;                  // identity (ordering, bond mask)
+            fs = S.cval[k];             // storage slot of the floe's pair state
+            ni = S.icnt[k];
+            no = S.ocnt[k];
+        }
+        const int cnt = ni + no;
+        int incl = cnt;
 #pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const int v = __shfl_up(incl, off);
-        if (lane >= off) incl += v;
-    }
-    const int start = incl - cnt, total = __shfl(incl, 63);
-    double afx = 0.0, afy = 0.0, amc = 0.0, atac = 0.0, apc = 0.0;
-    int nh = 0;
-    uint32_t over = 0;
-    for (int c0 = 0; c0 < total; c0 += 64) {
-        const int idx = c0 + lane;
-        int own = 0;
+        for (int off = 1; off < 64; off <<= 1) {
+            const int v = __shfl_up(incl, off);
+            if (lane >= off) incl += v;
+        }
+        const int start = incl - cnt, total = __shfl(incl, 63);
+        double afx = 0.0, afy = 0.0, amc = 0.0, atac = 0.0, apc = 0.0;
+        int nh = 0;
+        uint32_t over = 0;
+        for (int c0 = 0; c0 < total; c0 += 64) {
+            const int idx = c0 + lane;
+            int own = 0;
 #pragma unroll
-        for (int step = 32; step > 0; step >>= 1)
-            if (__shfl(start, own + step) <= idx) own += step;
-        const int ko = __shfl(k, own), om = __shfl(m, own), of = __shfl(f, own);
-        const int ofs = __shfl(fs, own), oni = __shfl(ni, own), ostart = __shfl(start, own);
-        ContactOut r;
-        r.kind = 0;
-        if (idx < total) {
-            const DevParams& P = S.params[om];
-            const int tstep = S.members[om].tstep;
-            const int first = S.members[om].first;
-            const int local = idx - ostart;
-            // overlap first from the position rows alone (the expression
-            // kinematics evaluates, so the same decision); the full rows
-            // only for contacts
-            const GeoRec gf = S.geo[ko];
-            if (local < oni) {
-                // pair (i, f), f the inner floe: only overlapping pairs contribute
-                const int64_t e = S.ik[(int64_t)local * SS + ko];
-                const int i = ent_id(e), qi = ent_pos(e);
-                const GeoRec gi = S.geo[qi];
-                const double dx = gi.x - gf.x, dy = gi.y - gf.y;
-                const double dn = gi.r + gf.r - sqrt(dx * dx + dy * dy);
-                if (dn > 0) {
-                    Floe Fi = floe_at(S, qi);
-                    Floe Fj = floe_at(S, ko);
-                    Kin kn;
-                    kinematics(Fi, Fj, kn);
-                    PairState ps;
-                    load_contact_state(S, S.cval[qi], of, ps);
-                    bond_create(P, tstep, i - first, of - first, Fi, Fj, kn, ps);
-                    EventOut o;
-                    o.flags = 0;
-                    o.ridged = -1;
-                    contact_part<MIXED>(P, Fi, Fj, kn, ps, o);
-                    r.kind = 1;
-                    r.fcn_c = o.fcn_c; r.fcr_s = o.fcr_s; r.fcn_s = o.fcn_s; r.fcr_c = o.fcr_c;
-                    r.r_fct = o.rj_fct; r.mcc = o.mcc; r.ac = o.ac; r.fcn_ac = o.fcn_ac;
-                }
-            } else {
-                // pair (f, j), f the outer floe (the owner)
-                const int s = local - oni;
-                const int64_t e = S.oj[(int64_t)s * SS + ko];
-                const int j = ent_id(e), qj = ent_pos(e);
-                const GeoRec gj = S.geo[qj];
-                const double dx = gf.x - gj.x, dy = gf.y - gj.y;
-                const double dn = gf.r + gj.r - sqrt(dx * dx + dy * dy);
-                r.j = j;
-                if (!(dn > 0)) {
-                    // reset_contact (reset.f90:1-15): the history goes, the
-                    // last contact forces stay
-                    const int hco = S.hcnt_old[ofs];
-                    for (int q = 0; q < hco; ++q) {
-                        const HistRec& h0 = S.hist_old[(int64_t)q * SS + ofs];
-                        if (h0.j != j) continue;
-                        if (h0.fcn != 0.0 || h0.fct != 0.0) {
-                            r.kind = 2;
-                            r.deltat = 0.0; r.thetarelc = 0.0; r.fcn = h0.fcn; r.fct = h0.fct;
-                            if (S.record_forces) {
-                                PairState pz;
-                                pz.fcn = h0.fcn; pz.fct = h0.fct;
-                                S.ssc[(int64_t)s * SS + ko] = stale_contact_magnitude<MIXED>(pz);
-                            }
-                        }
-                        break;
+            for (int step = 32; step > 0; step >>= 1)
+                if (__shfl(start, own + step) <= idx) own += step;
+            const int ko = __shfl(k, own), om = __shfl(m, own), of = __shfl(f, own);
+            const int ofs = __shfl(fs, own), oni = __shfl(ni, own), ostart = __shfl(start, own);
+            ContactOut r;
+            r.kind = 0;
+            if (idx < total) {
+                const DevParams& P = S.params[om];
+                const int tstep = S.members[om].tstep;
+                const int first = S.members[om].first;
+                const int local = idx - ostart;
+                // overlap first from the position rows alone (the expression
+                // kinematics evaluates, so the same decision); the full rows
+                // only for contacts
+                const GeoRec gf = S.geo[ko];
+                if (local < oni) {
+                    // pair (i, f), f the inner floe: only overlapping pairs contribute
+                    const int64_t e = S.ik[(int64_t)local * SS + ko];
+                    const int i = ent_id(e), qi = ent_pos(e);
+                    const GeoRec gi = S.geo[qi];
+                    const double dx = gi.x - gf.x, dy = gi.y - gf.y;
+                    const double dn = gi.r + gf.r - sqrt(dx * dx + dy * dy);
+                    if (dn > 0) {
+                        Floe Fi = floe_at(S, qi);
+                        Floe Fj = floe_at(S, ko);
+                        Kin kn;
+                        kinematics(Fi, Fj, kn);
+                        PairState ps;
+                        load_contact_state(S, S.cval[qi], of, ps);
+                        bond_create(P, tstep, i - first, of - first, Fi, Fj, kn, ps);
+                        EventOut o;
+                        o.flags = 0;
+                        o.ridged = -1;
+                        contact_part<MIXED>(P, Fi, Fj, kn, ps, o);
+                        r.kind = 1;
+                        r.fcn_c = o.fcn_c; r.fcr_s = o.fcr_s; r.fcn_s = o.fcn_s; r.fcr_c = o.fcr_c;
+                        r.r_fct = o.rj_fct; r.mcc = o.mcc; r.ac = o.ac; r.fcn_ac = o.fcn_ac;
                     }
                 } else {
-                    Floe Fi = floe_at(S, ko);
-                    Floe Fj = floe_at(S, qj);
-                    Kin kn;
-                    kinematics(Fi, Fj, kn);
-                    PairState ps;
-                    load_contact_state(S, ofs, j, ps);
-                    comp_unite(S.comp, ko, qj);      // the contact joins the floes' components
-                    bond_create(P, tstep, of - first, j - first, Fi, Fj, kn, ps);
-                    EventOut o;
-                    o.flags = 0;
-                    o.ridged = -1;
-                    contact_part<MIXED>(P, Fi, Fj, kn, ps, o);
-                    S.evf[(int64_t)s * SS + ko] |= o.flags;
-                    if (S.record_forces) S.ssc[(int64_t)s * SS + ko] = o.sc;
-                    r.kind = 1;
-                    r.fcn_c = o.fcn_c; r.fcr_s = o.fcr_s; r.fcn_s = o.fcn_s; r.fcr_c = o.fcr_c;
-                    r.r_fct = o.ri_fct; r.mcc = o.mcc; r.ac = o.ac; r.fcn_ac = o.fcn_ac;
-                    if (ps.deltat != 0.0 || ps.thetarelc != 0.0 || ps.fcn != 0.0 || ps.fct != 0.0) {
-                        r.kind |= 2;
-                        r.deltat = ps.deltat; r.thetarelc = ps.thetarelc; r.fcn = ps.fcn; r.fct = ps.fct;
-                    }
-                    if (o.flags & EV_RIDGED) {
-                        const int p = atomicAdd(&S.nseeds[om], 1);
-                        if (p < S.SEEDK * S.members[om].n)
-                            S.seeds[(int64_t)S.SEEDK * first + p] = ((uint64_t)(of - first) << 16) | (uint64_t)s;
-                        else
-                            over |= ERR_SEED_CAP;
+                    // pair (f, j), f the outer floe (the owner)
+                    const int s = local - oni;
+                    const int64_t e = S.oj[(int64_t)s * SS + ko];
+                    const int j = ent_id(e), qj = ent_pos(e);
+                    const GeoRec gj = S.geo[qj];
+                    const double dx = gf.x - gj.x, dy = gf.y - gj.y;
+                    const double dn = gf.r + gj.r - sqrt(dx * dx + dy * dy);
+                    r.j = j;
+                    if (!(dn > 0)) {
+                        // reset_contact (reset.f90:1-15): the history goes, the
+                        // last contact forces stay
+                        const int q = find_partner<2>(S.hist_old, SS, ofs, S.hcnt_old[ofs], j);
+                        if (q >= 0) {
+                            const HistRec& h0 = S.hist_old[(int64_t)q * SS + ofs];
+                            if (h0.fcn != 0.0 || h0.fct != 0.0) {
+                                r.kind = 2;
+                                r.deltat = 0.0; r.thetarelc = 0.0; r.fcn = h0.fcn; r.fct = h0.fct;
+                                if (S.record_forces) {
+                                    PairState pz;
+                                    pz.fcn = h0.fcn; pz.fct = h0.fct;
+                                    S.ssc[(int64_t)s * SS + ko] = stale_contact_magnitude<MIXED>(pz);
+                                }
+                            }
+                        }
+                    } else {
+                        Floe Fi = floe_at(S, ko);
+                        Floe Fj = floe_at(S, qj);
+                        Kin kn;
+                        kinematics(Fi, Fj, kn);
+                        PairState ps;
+                        load_contact_state(S, ofs, j, ps);
+                        comp_unite(S.comp, ko, qj);      // the contact joins the floes' components
+                        bond_create(P, tstep, of - first, j - first, Fi, Fj, kn, ps);
+                        EventOut o;
+                        o.flags = 0;
+                        o.ridged = -1;
+                        contact_part<MIXED>(P, Fi, Fj, kn, ps, o);
+                        S.evf[(int64_t)s * SS + ko] |= o.flags;
+                        if (S.record_forces) S.ssc[(int64_t)s * SS + ko] = o.sc;
+                        r.kind = 1;
+                        r.fcn_c = o.fcn_c; r.fcr_s = o.fcr_s; r.fcn_s = o.fcn_s; r.fcr_c = o.fcr_c;
+                        r.r_fct = o.ri_fct; r.mcc = o.mcc; r.ac = o.ac; r.fcn_ac = o.fcn_ac;
+                        if (ps.deltat != 0.0 || ps.thetarelc != 0.0 || ps.fcn != 0.0 || ps.fct != 0.0) {
+                            r.kind |= 2;
+                            r.deltat = ps.deltat; r.thetarelc = ps.thetarelc; r.fcn = ps.fcn; r.fct = ps.fct;
+                        }
+                        if (o.flags & EV_RIDGED) {
+                            const int p = atomicAdd(&S.nseeds[om], 1);
+                            if (p < S.SEEDK * S.members[om].n)
+                                S.seeds[(int64_t)S.SEEDK * first + p] = ((uint64_t)(of - first) << 16) | (uint64_t)s;
+                            else
+                                over |= ERR_SEED_CAP;
+                        }
                     }
                 }
             }
+            co[lane] = r;
+            wave_lds_sync();
+            // every floe adds this pass's terms of its own, in order
+            if (act) {
+                const int lo = max(start, c0), hi = min(start + cnt, c0 + 64);
+                for (int i2 = lo; i2 < hi; ++i2) {
+                    const ContactOut& t = co[i2 - c0];
+                    if (t.kind & 1) {
+                        if (i2 - start < ni) {
+                            afx = afx + t.fcn_c - t.fcr_s;
+                            afy = afy + t.fcn_s + t.fcr_c;
+                            amc = amc - t.r_fct + t.mcc;
+                        } else {
+                            afx = afx - t.fcn_c + t.fcr_s;
+                            afy = afy - t.fcn_s - t.fcr_c;
+                            amc = amc - t.r_fct - t.mcc;
+                        }
+                        atac = atac + t.ac;
+                        apc = apc - t.fcn_ac;
+                    }
+                    if (t.kind & 2) {
+                        if (nh < S.HC) {
+                            HistRec hr; hr.j = t.j; hr.pad_ = 0; hr.deltat = t.deltat; hr.thetarelc = t.thetarelc;
+                            hr.fcn = t.fcn; hr.fct = t.fct;
+                            S.hist_new[(int64_t)nh * SS + fs] = hr;
+                            ++nh;
+                        } else {
+                            over |= ERR_HIST_CAP;
+                        }
+                    }
+                }
+            }
+            wave_lds_sync();
         }
-        co[lane] = r;
-        wave_lds_sync();
-        // every floe adds this pass's terms of its own, in order
         if (act) {
-            const int lo = max(start, c0), hi = min(start + cnt, c0 + 64);
-            for (int i2 = lo; i2 < hi; ++i2) {
-                const ContactOut& t = co[i2 - c0];
-                if (t.kind & 1) {
-                    if (i2 - start < ni) {
-                        afx = afx + t.fcn_c - t.fcr_s;
-                        afy = afy + t.fcn_s + t.fcr_c;
-                        amc = amc - t.r_fct + t.mcc;
-                    } else {
-                        afx = afx - t.fcn_c + t.fcr_s;
-                        afy = afy - t.fcn_s - t.fcr_c;
-                        amc = amc - t.r_fct - t.mcc;
-                    }
-                    atac = atac + t.ac;
-                    apc = apc - t.fcn_ac;
-                }
-                if (t.kind & 2) {
-                    if (nh < S.HC) {
-                        HistRec hr; hr.j = t.j; hr.pad_ = 0; hr.deltat = t.deltat; hr.thetarelc = t.thetarelc;
-                        hr.fcn = t.fcn; hr.fct = t.fct;
-                        S.hist_new[(int64_t)nh * SS + fs] = hr;
-                        ++nh;
-                    } else {
-                        over |= ERR_HIST_CAP;
-                    }
+            // contact histories of pairs that are not candidates this step
+            // persist (the reference only resets a candidate pair, reset_contact)
+            const int hco = S.hcnt_old[fs];
+            for (int q = 0; q < hco; ++q) {
+                HistRec hr = S.hist_old[(int64_t)q * SS + fs];
+                if (hr.j < 0) continue;
+                const bool cand = find_entry(S.oj, SS, k, no, hr.j) >= 0;
+                if (!cand) {
+                    if (nh < S.HC) S.hist_new[(int64_t)nh * SS + fs] = hr, ++nh;
+                    else over |= ERR_HIST_CAP;
                 }
             }
+            S.hcnt_new[fs] = nh;
+            S.acc[A_FCX * SS + k] = afx;
+            S.acc[A_FCY * SS + k] = afy;
+            S.acc[A_MC * SS + k] = amc;
+            S.acc[A_TAC * SS + k] = atac;
+            S.acc[A_PC * SS + k] = apc;
         }
-        wave_lds_sync();
+        if (over) set_err(S, over);
     }
-    if (act) {
-        // contact histories of pairs that are not candidates this step
-        // persist (the reference only resets a candidate pair, reset_contact)
-        const int hco = S.hcnt_old[fs];
-        for (int q = 0; q < hco; ++q) {
-            HistRec hr = S.hist_old[(int64_t)q * SS + fs];
-            if (hr.j < 0) continue;
-            bool cand = false;
-            for (int s = 0; s < no; ++s)
-                if (ent_id(S.oj[(int64_t)s * SS + k]) == hr.j) { cand = true; break; }
-            if (!cand) {
-                if (nh < S.HC) S.hist_new[(int64_t)nh * SS + fs] = hr, ++nh;
-                else over |= ERR_HIST_CAP;
-            }
-        }
-        S.hcnt_new[fs] = nh;
-        S.acc[A_FCX * SS + k] = afx;
-        S.acc[A_FCY * SS + k] = afy;
-        S.acc[A_MC * SS + k] = amc;
-        S.acc[A_TAC * SS + k] = atac;
-        S.acc[A_PC * SS + k] = apc;
-    }
-    if (over) set_err(S, over);
 }
 
 // One bond event of owner f (position k, storage slot fs, floe F0): pair
@@ -1828,122 +1862,129 @@ template <bool MIXED>
 __global__ __launch_bounds__(FLOE_BLOCK, 4) void k_bond(DS S) {
     __shared__ uint32_t recs[FLOE_BLOCK / 64][BOND_TCAP];
     if (*S.err) return;
-    const int k = queue_entry(S, 1);
-    const bool act = k >= 0;
-    const int lane = (int)(threadIdx.x & 63);
-    uint32_t* rl = recs[threadIdx.x >> 6];
-    const int64_t SS = S.S;
-    int m = 0, fs = 0, no = 0, bco = 0, fid = 0, tstep = 0, first = 0;
-    bool bond_step = false;
-    // partners of the first LIST_PEEK records, requested together
-    int id[LIST_PEEK];
-    if (act) {
-        const GeoRec g = S.geo[k];
-        fs = S.cval[k];
-        no = S.ocnt[k];
-        bco = S.bcnt_old[fs];
-        // (a queued owner has at least one record)
+    const QueueView qv = queue_view(S, 1);
+    // (the wavefront loops while any lane has an entry: its lanes share LDS
+    // and shuffles)
+    for (int it = 0;; ++it) {
+        const int qt = queue_slot_it(qv.total, it);
+        if (__ballot(qt >= 0) == 0) break;
+        const int k = qt >= 0 ? queue_at(S, 1, qv, qt) : -1;
+        const bool act = k >= 0;
+        const int lane = (int)(threadIdx.x & 63);
+        uint32_t* rl = recs[threadIdx.x >> 6];
+        const int64_t SS = S.S;
+        int m = 0, fs = 0, no = 0, bco = 0, fid = 0, tstep = 0, first = 0;
+        bool bond_step = false;
+        // partners of the first LIST_PEEK records, requested together
+        int id[LIST_PEEK];
+        if (act) {
+            const GeoRec g = S.geo[k];
+            fs = S.cval[k];
+            no = S.ocnt[k];
+            bco = S.bcnt_old[fs];
+            // (a queued owner has at least one record)
 #pragma unroll
-        for (int q = 0; q < LIST_PEEK; ++q) id[q] = S.bond_old[(int64_t)min(q, max(bco - 1, 0)) * SS + fs].j;
-        m = g.member;
-        fid = g.id;
-        // (no short-circuit tests: the loads go out together)
-        const DevParams& P = S.params[m];
-        const Member Mb = S.members[m];
-        const int rst = P.restart, coh = P.cohesion;
-        tstep = Mb.tstep;
-        first = Mb.first;
-        bond_step = (tstep == 1) & (rst != 1) & (coh != 0);
+            for (int q = 0; q < LIST_PEEK; ++q) id[q] = S.bond_old[(int64_t)min(q, max(bco - 1, 0)) * SS + fs].j;
+            m = g.member;
+            fid = g.id;
+            // (no short-circuit tests: the loads go out together)
+            const DevParams& P = S.params[m];
+            const Member Mb = S.members[m];
+            const int rst = P.restart, coh = P.cohesion;
+            tstep = Mb.tstep;
+            first = Mb.first;
+            bond_step = (tstep == 1) & (rst != 1) & (coh != 0);
 #pragma unroll
-        for (int q = 0; q < LIST_PEEK; ++q) id[q] = q < bco ? id[q] : -1;
-    }
-    // the live records of the owners that form no bonds this step
-    int nlive = 0;
-    uint32_t over = 0;
-    const bool listed = act && !bond_step;
-    if (listed) {
-#pragma unroll
-        for (int q = 0; q < LIST_PEEK; ++q) nlive += id[q] >= 0;
-        for (int q = LIST_PEEK; q < bco; ++q) nlive += S.bond_old[(int64_t)q * SS + fs].j >= 0;
-    }
-    int incl = nlive;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const int v = __shfl_up(incl, off);
-        if (lane >= off) incl += v;
-    }
-    const int total = __shfl(incl, 63);
-    const bool fits = total <= BOND_TCAP;
-    if (fits) {
+            for (int q = 0; q < LIST_PEEK; ++q) id[q] = q < bco ? id[q] : -1;
+        }
+        // the live records of the owners that form no bonds this step
+        int nlive = 0;
+        uint32_t over = 0;
+        const bool listed = act && !bond_step;
         if (listed) {
-            int at = incl - nlive, rq = 0;
-            auto list_rec = [&](int q) {
-                rl[at++] = (uint32_t)lane | ((uint32_t)q << 6) | ((uint32_t)rq << 19);
-                ++rq;
-            };
 #pragma unroll
-            for (int q = 0; q < LIST_PEEK; ++q)
-                if (id[q] >= 0) list_rec(q);    // (-1 past the count)
-            for (int q = LIST_PEEK; q < bco; ++q)
-                if (S.bond_old[(int64_t)q * SS + fs].j >= 0) list_rec(q);
-            S.bcnt_new[fs] = nlive;
+            for (int q = 0; q < LIST_PEEK; ++q) nlive += id[q] >= 0;
+            for (int q = LIST_PEEK; q < bco; ++q) nlive += S.bond_old[(int64_t)q * SS + fs].j >= 0;
         }
-        wave_lds_sync();
-        for (int c0 = 0; c0 < total; c0 += 64) {
-            const int idx = c0 + lane;
-            const uint32_t w = idx < total ? rl[idx] : (uint32_t)lane;
-            const int own = (int)(w & 63);
-            // the owner's member, step and identity, from its lane
-            const int ko = __shfl(k, own), mo = __shfl(m, own), fso = __shfl(fs, own);
-            const int noo = __shfl(no, own), fo = __shfl(fid, own);
-            const int tso = __shfl(tstep, own), fro = __shfl(first, own);
-            if (idx < total) {
-                const int q = (int)((w >> 6) & 0x1FFF), rq = (int)(w >> 19);
-                const DevParams& P = S.params[mo];
-                const Floe F0 = floe_at(S, ko);
-                const BondRec b = S.bond_old[(int64_t)q * SS + fso];
-                int nb = 0;
-                if (bond_record<MIXED>(S, P, tso, fro, fo, ko, fso, F0, noo, b, rq, nb, over))
-                    S.bgat[ko] = 1;
+        int incl = nlive;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const int v = __shfl_up(incl, off);
+            if (lane >= off) incl += v;
+        }
+        const int total = __shfl(incl, 63);
+        const bool fits = total <= BOND_TCAP;
+        if (fits) {
+            if (listed) {
+                int at = incl - nlive, rq = 0;
+                auto list_rec = [&](int q) {
+                    rl[at++] = (uint32_t)lane | ((uint32_t)q << 6) | ((uint32_t)rq << 19);
+                    ++rq;
+                };
+#pragma unroll
+                for (int q = 0; q < LIST_PEEK; ++q)
+                    if (id[q] >= 0) list_rec(q);    // (-1 past the count)
+                for (int q = LIST_PEEK; q < bco; ++q)
+                    if (S.bond_old[(int64_t)q * SS + fs].j >= 0) list_rec(q);
+                S.bcnt_new[fs] = nlive;
+            }
+            wave_lds_sync();
+            for (int c0 = 0; c0 < total; c0 += 64) {
+                const int idx = c0 + lane;
+                const uint32_t w = idx < total ? rl[idx] : (uint32_t)lane;
+                const int own = (int)(w & 63);
+                // the owner's member, step and identity, from its lane
+                const int ko = __shfl(k, own), mo = __shfl(m, own), fso = __shfl(fs, own);
+                const int noo = __shfl(no, own), fo = __shfl(fid, own);
+                const int tso = __shfl(tstep, own), fro = __shfl(first, own);
+                if (idx < total) {
+                    const int q = (int)((w >> 6) & 0x1FFF), rq = (int)(w >> 19);
+                    const DevParams& P = S.params[mo];
+                    const Floe F0 = floe_at(S, ko);
+                    const BondRec b = S.bond_old[(int64_t)q * SS + fso];
+                    int nb = 0;
+                    if (bond_record<MIXED>(S, P, tso, fro, fo, ko, fso, F0, noo, b, rq, nb, over))
+                        S.bgat[ko] = 1;
+                }
             }
         }
-    }
-    if (act && (bond_step || !fits)) {
-        const GeoRec me = S.geo[k];
-        const DevParams& P = S.params[m];
-        const int tstep = S.members[m].tstep;
-        const int first = S.members[m].first;
-        const int f = me.id;
-        const Floe F0 = floe_at(S, k);
-        bool any = false;
-        int nb = 0;                     // records written so far
-        for (int q = 0; q < bco; ++q) {
-            const BondRec b = S.bond_old[q * SS + fs];
-            if (b.j < 0) continue;
-            const int rq = nb++;
-            any |= bond_record<MIXED>(S, P, tstep, first, f, k, fs, F0, no, b, rq, nb, over);
-        }
-        // bonds that form this step (bond_create), in candidate order, appended
-        if (bond_step) {
-            const int nb_old = nb;
-            for (int s = 0; s < no; ++s) {
-                const int j = ent_id(S.oj[s * SS + k]);
-                bool has = false;
-                for (int q = 0; q < nb_old; ++q)
-                    if (S.bond_new[(int64_t)q * SS + fs].j == j) { has = true; break; }
-                if (has) continue;
-                PairState ps;
-                ps.deltat = 0.0; ps.thetarelc = 0.0; ps.fcn = 0.0; ps.fct = 0.0;
-                ps.bond = 0; ps.has_bond_rec = false;
-                ps.damage = ps.deltanb = ps.deltatb = ps.trb_ji = ps.trb_ij = 0.0;
-                ps.rb = ps.hb = ps.lb = ps.sb = ps.ib = 0.0;
-                any |= bond_event<MIXED>(S, P, tstep, first, f, k, fs, F0, s, -1, nb, ps, over);
+        if (act && (bond_step || !fits)) {
+            const GeoRec me = S.geo[k];
+            const DevParams& P = S.params[m];
+            const int tstep = S.members[m].tstep;
+            const int first = S.members[m].first;
+            const int f = me.id;
+            const Floe F0 = floe_at(S, k);
+            bool any = false;
+            int nb = 0;                     // records written so far
+            for (int q = 0; q < bco; ++q) {
+                const BondRec b = S.bond_old[q * SS + fs];
+                if (b.j < 0) continue;
+                const int rq = nb++;
+                any |= bond_record<MIXED>(S, P, tstep, first, f, k, fs, F0, no, b, rq, nb, over);
             }
+            // bonds that form this step (bond_create), in candidate order, appended
+            if (bond_step) {
+                const int nb_old = nb;
+                for (int s = 0; s < no; ++s) {
+                    const int j = ent_id(S.oj[s * SS + k]);
+                    bool has = false;
+                    for (int q = 0; q < nb_old; ++q)
+                        if (S.bond_new[(int64_t)q * SS + fs].j == j) { has = true; break; }
+                    if (has) continue;
+                    PairState ps;
+                    ps.deltat = 0.0; ps.thetarelc = 0.0; ps.fcn = 0.0; ps.fct = 0.0;
+                    ps.bond = 0; ps.has_bond_rec = false;
+                    ps.damage = ps.deltanb = ps.deltatb = ps.trb_ji = ps.trb_ij = 0.0;
+                    ps.rb = ps.hb = ps.lb = ps.sb = ps.ib = 0.0;
+                    any |= bond_event<MIXED>(S, P, tstep, first, f, k, fs, F0, s, -1, nb, ps, over);
+                }
+            }
+            if (any) S.bgat[k] = 1;
+            S.bcnt_new[fs] = nb;
         }
-        if (any) S.bgat[k] = 1;
-        S.bcnt_new[fs] = nb;
+        if (over) set_err(S, over);
     }
-    if (over) set_err(S, over);
 }
 
 // Bond rows of a floe with a bonded event (position k), added to a[] (rows
@@ -1997,27 +2038,31 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_walls(DS S) {
     // no later kernel of the step reads the build flag; k_final raises it
     // again when a floe has moved too far for the supersets
     if (blockIdx.x == 0 && threadIdx.x == 0) { S.rebuild[0] = 0; S.rebuild[1] = 0; }
-    const int k = queue_entry(S, 2);
-    if (k < 0) return;
-    const GeoRec me = S.geo[k];
-    const int m = me.member;
-    const DevParams& P = S.params[m];
-    const int first = S.members[m].first;
-    const int f = me.id;
-    const int fs = S.cval[k];
-    const int64_t SS = S.S;
-    Floe Fw = floe_at(S, k);
-    double hist[4] = {S.wbc[0 * SS + fs], S.wbc[1 * SS + fs], S.wbc[2 * SS + fs], S.wbc[3 * SS + fs]};
-    WallOut W;
-    walls<MIXED>(P, Fw, hist, W);
-    S.wflag[k] = (uint8_t)((W.flags & 1u) ? WF_WALL : 0u);
-    if (W.flags & 1u) S.wout[k] = W;
-    if (W.flags & 4u) {
-        int p = atomicAdd(&S.nseeds[m], 1);
-        if (p < S.SEEDK * S.members[m].n)
-            S.seeds[(int64_t)S.SEEDK * first + p] = ((uint64_t)(f - first) << 16) | WALL_SLOT;
-        else
-            set_err(S, ERR_SEED_CAP);
+    const QueueView qv = queue_view(S, 2);
+    for (int it = 0;; ++it) {
+        const int t = queue_slot_it(qv.total, it);
+        if (t < 0) break;
+        const int k = queue_at(S, 2, qv, t);
+        const GeoRec me = S.geo[k];
+        const int m = me.member;
+        const DevParams& P = S.params[m];
+        const int first = S.members[m].first;
+        const int f = me.id;
+        const int fs = S.cval[k];
+        const int64_t SS = S.S;
+        Floe Fw = floe_at(S, k);
+        double hist[4] = {S.wbc[0 * SS + fs], S.wbc[1 * SS + fs], S.wbc[2 * SS + fs], S.wbc[3 * SS + fs]};
+        WallOut W;
+        walls<MIXED>(P, Fw, hist, W);
+        S.wflag[k] = (uint8_t)((W.flags & 1u) ? WF_WALL : 0u);
+        if (W.flags & 1u) S.wout[k] = W;
+        if (W.flags & 4u) {
+            int p = atomicAdd(&S.nseeds[m], 1);
+            if (p < S.SEEDK * S.members[m].n)
+                S.seeds[(int64_t)S.SEEDK * first + p] = ((uint64_t)(f - first) << 16) | WALL_SLOT;
+            else
+                set_err(S, ERR_SEED_CAP);
+        }
     }
 }
 
@@ -2227,9 +2272,13 @@ __device__ __forceinline__ void push_later_events(const DS& S, Heap& H, int firs
 // (no unions run any more).  Every floe of a contact is in the queue.
 __global__ __launch_bounds__(FLOE_BLOCK) void k_comp(DS S) {
     if (*S.err) return;
-    const int k = queue_entry(S, 0);
-    if (k < 0) return;
-    S.comp[k] = comp_find(S.comp, k);
+    const QueueView qv = queue_view(S, 0);
+    for (int it = 0;; ++it) {
+        const int t = queue_slot_it(qv.total, it);
+        if (t < 0) break;
+        const int k = queue_at(S, 0, qv, t);
+        S.comp[k] = comp_find(S.comp, k);
+    }
 }
 
 // One event at a time off the binary heap in HBM (single lane).
