@@ -31,6 +31,7 @@ template <bool M> __global__ void k_bond(DS);
 __global__ void k_stress(DS);
 template <bool M> __global__ void k_walls(DS);
 template <bool M> __global__ void k_repair(DS, int);
+__global__ void k_fixq(DS);
 template <bool M> __global__ void k_fixup(DS);
 template <bool M> __global__ void k_final(DS);
 __global__ void k_totals(DS, int64_t*);
@@ -548,7 +549,12 @@ int launch_step(godar_ctx* c, bool lean) {
     }
     {
         StageTimer t(c, 9);
-        hipLaunchKernelGGL(mixed ? k_fixup<true> : k_fixup<false>, grid_for(N, B[9], true),
+        // the touched floes into the fixup queue (laid out as the candidate
+        // search's grid, so that a wavefront's positions lie in one XCD
+        // share), then a few of them per wavefront
+        hipLaunchKernelGGL(k_fixq, grid_for(N, B[3], true), dim3(B[3]), 0, s, d);
+        const int fb = std::max(XCD_COUNT, FIXUP_GRID_WAVES * 64 / B[9] / XCD_COUNT * XCD_COUNT);
+        hipLaunchKernelGGL(mixed ? k_fixup<true> : k_fixup<false>, dim3(fb),
                            dim3(B[9]), (B[9] / 64) * FIX_LDS_PER_WAVE, s, d);
     }
     if (c->record) {
@@ -729,7 +735,7 @@ int godar_create(int64_t max_floes, int32_t max_members, int32_t precision, goda
     ACHK(dalloc(&d.wout, S));
     ACHK(dalloc(&d.wflag, (S + 3) / 4 * 4));   // whole 32-bit words (wflag_set)
     ACHK(dalloc(&d.cls, S));
-    ACHK(dalloc(&d.qlist, 4 * S));
+    ACHK(dalloc(&d.qlist, (int64_t)NQUEUE * S));
     ACHK(dalloc(&d.bgat, S));
     ACHK(dalloc(&d.qn, (int64_t)NQUEUE * XCD_COUNT * QN_STRIDE));
     ACHK(dalloc(&d.seeds, (int64_t)c->SEEDK * S));
@@ -828,7 +834,7 @@ static int64_t bytes_per_floe(const godar_ctx* c) {
     per += (int64_t)c->CO * (8 + 8 + 4 + 4 + 8 + 8) + (int64_t)c->CI * (8 + 4);   // oj od evf ov ssc ssb; ik ibond
     per += 2 * ((int64_t)c->HC * sizeof(HistRec) + (int64_t)c->BC * sizeof(BondRec));
     per += (int64_t)c->BC * sizeof(BondTerms) + 1 /*bgat*/;
-    per += NACC * 8 + sizeof(WallOut) + 1 /*wflag*/ + 1 /*cls*/ + 4 * 4 /*qlist*/;
+    per += NACC * 8 + sizeof(WallOut) + 1 /*wflag*/ + 1 /*cls*/ + NQUEUE * 4 /*qlist*/;
     per += (int64_t)c->SEEDK * 8 + (int64_t)c->HEAPK * 8 + (int64_t)c->TABK * sizeof(EventOut);
     per += sizeof(WallOut) /*wtab*/ + 4 /*ovw*/ + 7 * 8 /*cur*/ + 1 + 1 /*dirty, wnz*/ + 4 /*comp*/;
     per += 33 * 8 /*fout*/ + 4 + 4 /*radix keys, values*/;

@@ -20,10 +20,11 @@ constexpr int WALL_SLOT = 0xFFFF;
 constexpr uint8_t CLS_CONTACT = 1, CLS_BOND = 2;
 constexpr int FLOE_BLOCK = 256;   // upper bound of the per-floe kernels' block size
 constexpr int XCD_COUNT = 8;      // accelerator complex dies of an MI355X
-constexpr int NQUEUE = 4;         // work queues: contact, bond, wall, dense search
+constexpr int NQUEUE = 5;         // work queues: contact, bond, wall, dense search, fixup
 constexpr int QN_STRIDE = 64;     // ints between queue counters (256 B)
 constexpr int SCAN_TILE = 1024;   // positions per tile of the superset offset scan
 constexpr int QUEUE_GRID_WAVES = 4096;   // wavefronts launched for a queue kernel at most
+constexpr int FIXUP_GRID_WAVES = 2048;   // wavefronts of the fix-up (about one resident round)
 
 // Per-floe record (96 bytes) by storage slot.  Between builds of the cell
 // order the floe state lives in the ordered copies below, which the step
@@ -140,13 +141,13 @@ struct DS {
     double2* x0;               // [S] by pos (positions stand between builds)
     int32_t* rebuild;          // [3] 0: build the supersets; 1: refresh the ordered copies;
                                //     2: the copies are newer than rec
-    // Work queues (contact, bond, wall, dense search), each in XCD_COUNT
+    // Work queues (contact, bond, wall, dense search, fixup), each in XCD_COUNT
     // segments: the candidate search's blocks on XCD x cover positions
     // [x qseg, (x+1) qseg) and queue them into segment x, at
     // qlist[c S + x qseg + i], counted by their own counter (one per
     // QN_STRIDE ints), so that the counters' atomics spread over many lines.
-    int32_t* qlist;            // [4][S]
-    int32_t* qn;               // [4][XCD_COUNT] x QN_STRIDE
+    int32_t* qlist;            // [NQUEUE][S]
+    int32_t* qn;               // [NQUEUE][XCD_COUNT] x QN_STRIDE
     // ordered repair (id)
     uint64_t* seeds;           // SEEDK per floe slot, member region starts at SEEDK*first
     int32_t* nseeds;           // [members]

@@ -403,7 +403,8 @@ __global__ void k_begin(DS S, int lean) {
     }
     const bool build = !lean && S.rebuild[0] != 0, refresh = !lean && S.rebuild[1] != 0;
     const bool newer = S.rebuild[2] != 0;
-    if (i < NQUEUE * XCD_COUNT && (build || i < 3 * XCD_COUNT)) S.qn[i * QN_STRIDE] = 0;
+    // (the dense-search queue stands between builds)
+    if (i < NQUEUE * XCD_COUNT && (build || i / XCD_COUNT != 3)) S.qn[i * QN_STRIDE] = 0;
     if (build && i <= S.ncells) S.crange[i] = make_int2(0, 0);
     if (i >= S.N) return;
     if (newer && (build || refresh)) {
@@ -2709,97 +2710,113 @@ __device__ __forceinline__ void copy_event(EventOut* dst, const EventOut* src) {
 // in order, so that a floe with many candidates does not hold one lane
 // through all of them.
 template <bool MIXED>
-__global__ __launch_bounds__(FLOE_BLOCK) void k_fixup(DS S) {
+__global__ __launch_bounds__(FLOE_BLOCK, 2) void k_fixup(DS S) {
     extern __shared__ unsigned char fix_lds[];
     EventOut* ev = reinterpret_cast<EventOut*>(fix_lds + (threadIdx.x >> 6) * FIX_LDS_PER_WAVE);
-    // positions strided over the wavefronts (lane l of wavefront w takes
-    // l W + w): touched floes sit together in ridging zones, and this spreads
-    // them over many wavefronts instead of a few long ones
+    // The floes the repair touched (the fixup queue, filled by k_fixq), a few
+    // per wavefront: every wavefront of the grid (about one resident round)
+    // takes F consecutive entries, F as small as the queue allows, so that
+    // the floes' re-gathers run side by side rather than in a few long
+    // wavefronts.
     const int lane = (int)(threadIdx.x & 63);
-    const int nwave = (int)(gridDim.x * blockDim.x) >> 6;
-    const int k = lane * nwave + (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
-    // floes the repair touched (one byte per position decides)
-    bool act = k < S.N && !*S.err && (S.wflag[k] & WF_FIXUP);
-    int m = 0;
-    if (act) {
-        m = S.geo[k].member;
-        act = m >= 0;
-    }
-    if (!__ballot(act)) return;
-    if (!act) m = 0;
-    const int f = act ? S.geo[k].id : 0;
-    const int fs = act ? (int)S.cval[k] : 0;
-    const int ni = act ? S.icnt[k] : 0, no = act ? S.ocnt[k] : 0;
-    const int cnt = ni + no;
-    int incl = cnt;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const int v = __shfl_up(incl, off);
-        if (lane >= off) incl += v;
-    }
-    const int start = incl - cnt, total = __shfl(incl, 63);
-    const int64_t SS = S.S;
-    double acc[NACC];
-    acc_init(acc);
-    for (int c0 = 0; c0 < total; c0 += 64) {
-        const int idx = c0 + lane;
-        int own = 0;
-#pragma unroll
-        for (int step = 32; step > 0; step >>= 1)
-            if (__shfl(start, own + step) <= idx) own += step;
-        const int ko = __shfl(k, own), om = __shfl(m, own), of = __shfl(f, own);
-        const int ofs = __shfl(fs, own), oni = __shfl(ni, own), ostart = __shfl(start, own);
-        if (idx < total) {
-            const DevParams& P = S.params[om];
-            const int tstep = S.members[om].tstep;
-            const int first = S.members[om].first;
-            const EventOut* etab = S.etab + (int64_t)S.TABK * first;
-            const int local = idx - ostart;
-            const bool inner = local < oni;
-            int tab = -1, qp = 0, pid = 0;
-            if (inner) {
-                const int64_t e = S.ik[(int64_t)local * SS + ko];
-                pid = ent_id(e);
-                qp = ent_pos(e);
-                const int sk = slot_of(S, qp, of);
-                tab = sk >= 0 && (S.evf[(int64_t)sk * SS + qp] & EV_TABLE) ? S.ov[(int64_t)sk * SS + qp] : -1;
-            } else {
-                const int s2 = local - oni;
-                tab = (S.evf[(int64_t)s2 * SS + ko] & EV_TABLE) ? S.ov[(int64_t)s2 * SS + ko] : -1;
-                const int64_t e = S.oj[(int64_t)s2 * SS + ko];
-                pid = ent_id(e);
-                qp = ent_pos(e);
-            }
-            if (tab >= 0) {
-                copy_event(&ev[lane], &etab[tab]);
-                if (ev[lane].flags & EV_REMOVED) ev[lane].flags = 0;   // adds nothing
-            } else {
-                // inner: pair (i, f) with i's wind; outer: pair (f, j) with f's wind
-                PairState ps;
-                load_pair_state<1>(S, inner ? (int)S.cval[qp] : ofs, inner ? of : pid, ps);
-                const double2 w = wind_at(S, inner ? qp : ko);
-                Floe Fa = floe_at(S, inner ? qp : ko), Fb = floe_at(S, inner ? ko : qp);
-                EventOut o;
-                pair_event<MIXED>(P, tstep, (inner ? pid : of) - first, (inner ? of : pid) - first,
-                                  Fa, Fb, w.x, w.y, P.uw, P.vw, ps, o, inner ? 2 : 1);
-                copy_event(&ev[lane], &o);
-            }
-        }
-        wave_lds_sync();
-        // every floe adds this pass's events of its own, in order
+    const QueueView qv = queue_view(S, 4);
+    const int nw = (int)(gridDim.x * blockDim.x) >> 6;
+    const int w = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    const int F = min(64, max(1, (qv.total + nw - 1) / nw));
+    for (int t0 = w * F; t0 < qv.total; t0 += nw * F) {     // (the same for all lanes)
+        const int t = t0 + lane;
+        bool act = lane < F && t < qv.total && !*S.err;
+        const int k = act ? queue_at(S, 4, qv, t) : 0;
+        int m = 0;
         if (act) {
-            const int lo = max(start, c0), hi = min(start + cnt, c0 + 64);
-            for (int i2 = lo; i2 < hi; ++i2) {
-                if (i2 - start < ni) apply_j(acc, ev[i2 - c0]);
-                else apply_i(acc, ev[i2 - c0]);
-            }
+            m = S.geo[k].member;
+            act = m >= 0;
         }
-        wave_lds_sync();
-    }
-    if (!act) return;
+        if (!__ballot(act)) continue;
+        if (!act) m = 0;
+        const int f = act ? S.geo[k].id : 0;
+        const int fs = act ? (int)S.cval[k] : 0;
+        const int ni = act ? S.icnt[k] : 0, no = act ? S.ocnt[k] : 0;
+        const int cnt = ni + no;
+        int incl = cnt;
 #pragma unroll
-    for (int a = 0; a < NACC; ++a) S.acc[a * SS + k] = acc[a];
-    S.cls[k] = CLS_CONTACT | CLS_BOND;
+        for (int off = 1; off < 64; off <<= 1) {
+            const int v = __shfl_up(incl, off);
+            if (lane >= off) incl += v;
+        }
+        const int start = incl - cnt, total = __shfl(incl, 63);
+        const int64_t SS = S.S;
+        double acc[NACC];
+        acc_init(acc);
+        for (int c0 = 0; c0 < total; c0 += 64) {
+            const int idx = c0 + lane;
+            int own = 0;
+#pragma unroll
+            for (int step = 32; step > 0; step >>= 1)
+                if (__shfl(start, own + step) <= idx) own += step;
+            const int ko = __shfl(k, own), om = __shfl(m, own), of = __shfl(f, own);
+            const int ofs = __shfl(fs, own), oni = __shfl(ni, own), ostart = __shfl(start, own);
+            if (idx < total) {
+                const DevParams& P = S.params[om];
+                const int tstep = S.members[om].tstep;
+                const int first = S.members[om].first;
+                const EventOut* etab = S.etab + (int64_t)S.TABK * first;
+                const int local = idx - ostart;
+                const bool inner = local < oni;
+                int tab = -1, qp = 0, pid = 0;
+                if (inner) {
+                    const int64_t e = S.ik[(int64_t)local * SS + ko];
+                    pid = ent_id(e);
+                    qp = ent_pos(e);
+                    const int sk = slot_of(S, qp, of);
+                    tab = sk >= 0 && (S.evf[(int64_t)sk * SS + qp] & EV_TABLE) ? S.ov[(int64_t)sk * SS + qp] : -1;
+                } else {
+                    const int s2 = local - oni;
+                    tab = (S.evf[(int64_t)s2 * SS + ko] & EV_TABLE) ? S.ov[(int64_t)s2 * SS + ko] : -1;
+                    const int64_t e = S.oj[(int64_t)s2 * SS + ko];
+                    pid = ent_id(e);
+                    qp = ent_pos(e);
+                }
+                if (tab >= 0) {
+                    copy_event(&ev[lane], &etab[tab]);
+                    if (ev[lane].flags & EV_REMOVED) ev[lane].flags = 0;   // adds nothing
+                } else {
+                    // inner: pair (i, f) with i's wind; outer: pair (f, j) with f's wind
+                    PairState ps;
+                    load_pair_state<1>(S, inner ? (int)S.cval[qp] : ofs, inner ? of : pid, ps);
+                    const double2 w = wind_at(S, inner ? qp : ko);
+                    Floe Fa = floe_at(S, inner ? qp : ko), Fb = floe_at(S, inner ? ko : qp);
+                    EventOut o;
+                    pair_event<MIXED>(P, tstep, (inner ? pid : of) - first, (inner ? of : pid) - first,
+                                      Fa, Fb, w.x, w.y, P.uw, P.vw, ps, o, inner ? 2 : 1);
+                    copy_event(&ev[lane], &o);
+                }
+            }
+            wave_lds_sync();
+            // every floe adds this pass's events of its own, in order
+            if (act) {
+                const int lo = max(start, c0), hi = min(start + cnt, c0 + 64);
+                for (int i2 = lo; i2 < hi; ++i2) {
+                    if (i2 - start < ni) apply_j(acc, ev[i2 - c0]);
+                    else apply_i(acc, ev[i2 - c0]);
+                }
+            }
+            wave_lds_sync();
+        }
+        if (act) {
+#pragma unroll
+            for (int a = 0; a < NACC; ++a) S.acc[a * SS + k] = acc[a];
+            S.cls[k] = CLS_CONTACT | CLS_BOND;
+        }
+    }
+}
+
+// The positions the repair touched (WF_FIXUP) into the fixup queue, one
+// atomic per wavefront; the grid is laid out as the candidate search's.
+__global__ __launch_bounds__(FLOE_BLOCK) void k_fixq(DS S) {
+    const int k = xcd_block() * blockDim.x + threadIdx.x;
+    const bool want = k < S.N && !*S.err && (S.wflag[k] & WF_FIXUP);
+    enqueue(S, want, 4, k);
 }
 
 // ---------------------------------------------------------------------
