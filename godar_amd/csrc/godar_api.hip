@@ -97,6 +97,12 @@ struct Range { int64_t first, n; };
 
 struct godar_ctx {
     hipStream_t stream = nullptr;
+    // The bond and wall kernels run beside the contact kernel, on two side
+    // streams forked from and joined back to `stream` by events (a captured
+    // step graph takes the same branches); steps timed stage by stage run
+    // them one after another.
+    hipStream_t side[2] = {nullptr, nullptr};
+    hipEvent_t efork = nullptr, ejoin[2] = {nullptr, nullptr};
     int precision = GODAR_FP64;
     int64_t S = 0;
     int Mcap = 0;
@@ -525,6 +531,15 @@ int launch_step(godar_ctx* c, bool lean) {
                                grid_for(N, FLOE_BLOCK, true), dim3(FLOE_BLOCK), 0, s, d);
         }
     }
+    // contacts, bonds and walls touch disjoint state (the flag words both
+    // of the first two set are set with atomicOr), so they may run together
+    const bool fork = !c->prof;
+    hipStream_t sb = fork ? c->side[0] : s, sw = fork ? c->side[1] : s;
+    if (fork) {
+        (void)hipEventRecord(c->efork, s);
+        (void)hipStreamWaitEvent(sb, c->efork, 0);
+        (void)hipStreamWaitEvent(sw, c->efork, 0);
+    }
     {
         StageTimer t(c, 4);
         // LDS: one pass of 64 pair results per wavefront
@@ -534,12 +549,18 @@ int launch_step(godar_ctx* c, bool lean) {
     {
         StageTimer t(c, 5);
         hipLaunchKernelGGL(mixed ? k_bond<true> : k_bond<false>, grid_queue(N, B[5]),
-                           dim3(B[5]), 0, s, d);
+                           dim3(B[5]), 0, sb, d);
     }
     {
         StageTimer t(c, 7);
         hipLaunchKernelGGL(mixed ? k_walls<true> : k_walls<false>, grid_queue(N, B[7]),
-                           dim3(B[7]), 0, s, d);
+                           dim3(B[7]), 0, sw, d);
+    }
+    if (fork) {
+        (void)hipEventRecord(c->ejoin[0], sb);
+        (void)hipEventRecord(c->ejoin[1], sw);
+        (void)hipStreamWaitEvent(s, c->ejoin[0], 0);
+        (void)hipStreamWaitEvent(s, c->ejoin[1], 0);
     }
     {
         StageTimer t(c, 8);
@@ -690,6 +711,11 @@ int godar_create(int64_t max_floes, int32_t max_members, int32_t precision, goda
     DS& d = c->d;
     int64_t S = c->S;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    for (int k = 0; k < 2 && e == hipSuccess; ++k) {
+        e = hipStreamCreateWithFlags(&c->side[k], hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ejoin[k], hipEventDisableTiming);
+    }
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->efork, hipEventDisableTiming);
 #define ACHK(x)                                        \
     do {                                               \
         e = (x);                                       \
@@ -698,7 +724,7 @@ int godar_create(int64_t max_floes, int32_t max_members, int32_t precision, goda
             return GODAR_ERR_NOMEM;                    \
         }                                              \
     } while (0)
-    if (e != hipSuccess) { delete c; return GODAR_ERR_HIP; }
+    if (e != hipSuccess) { godar_destroy(c); return GODAR_ERR_HIP; }
     if (hipGetDevice(&c->device) != hipSuccess ||
         totals_scratch_of(c->device, &c->totals_scratch) != hipSuccess) {
         godar_destroy(c);
@@ -816,6 +842,11 @@ int godar_destroy(godar_ctx* c) {
     radix_free(c->rs);
     collect_events(c);
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
+    for (int k = 0; k < 2; ++k) {
+        if (c->ejoin[k]) (void)hipEventDestroy(c->ejoin[k]);
+        if (c->side[k]) (void)hipStreamDestroy(c->side[k]);
+    }
+    if (c->efork) (void)hipEventDestroy(c->efork);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return GODAR_OK;

@@ -1706,7 +1706,7 @@ __global__ __launch_bounds__(FLOE_BLOCK, 3) void k_contact(DS S) {
                         o.flags = 0;
                         o.ridged = -1;
                         contact_part<MIXED>(P, Fi, Fj, kn, ps, o);
-                        S.evf[(int64_t)s * SS + ko] |= o.flags;
+                        atomicOr(&S.evf[(int64_t)s * SS + ko], o.flags);   // (k_bond sets bits too)
                         if (S.record_forces) S.ssc[(int64_t)s * SS + ko] = o.sc;
                         r.kind = 1;
                         r.fcn_c = o.fcn_c; r.fcr_s = o.fcr_s; r.fcn_s = o.fcn_s; r.fcr_c = o.fcr_c;
@@ -1817,7 +1817,7 @@ __device__ __forceinline__ bool bond_event(const DS& S, const DevParams& P, int 
     bt.lri_fbt = o.lri_fbt; bt.lrj_fbt = o.lrj_fbt; bt.mbb_ji = o.mbb_ji; bt.mbb_ij = o.mbb_ij;
     bt.sb = o.sb; bt.fbn_sb = o.fbn_sb;
     S.bterm[(int64_t)rq * SS + k] = bt;          // (by the owner's position)
-    S.evf[s * SS + k] |= EV_BONDED | ((uint32_t)(rq + 1) << 16);
+    atomicOr(&S.evf[s * SS + k], EV_BONDED | ((uint32_t)(rq + 1) << 16));   // (as k_contact)
     // tell the partner where its terms are: its inner slot of f
     const int qi = find_entry(S.ik, SS, qj, S.icnt[qj], f);
     if (qi >= 0) S.ibond[(int64_t)qi * SS + qj] = rq + 1;
