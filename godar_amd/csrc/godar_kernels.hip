@@ -1359,7 +1359,6 @@ __global__ __launch_bounds__(64) void k_cand_dense(DS S) {
     wave_lds_sync();
     const int64_t SS = S.S;
     int64_t* oj = S.oj;
-    double* od = S.od;
     // the last min(no, DENSE_WIN) outer entries, ascending: slots [no - nw, no)
     double wd[DENSE_WIN];
     int64_t we[DENSE_WIN];
@@ -1373,6 +1372,14 @@ __global__ __launch_bounds__(64) void k_cand_dense(DS S) {
     auto less = [](double ad, int64_t ae, double bd, int64_t be) {
         return ad < bd || (ad == bd && ent_id(ae) < ent_id(be));
     };
+    // (the part of the list in HBM keeps no distances: the rare entry that
+    // belongs below the register part compares with a slot's entry by its
+    // distance recomputed from the positions, the same expression)
+    auto dist_of = [&](int64_t e) {
+        const GeoRec o = S.geo[ent_pos(e)];
+        const double dx = me.x - o.x, dy = me.y - o.y;
+        return sqrt(dx * dx + dy * dy);
+    };
     auto insert_outer = [&](double xd, int64_t xe) {
         if (no >= S.CO) { over |= ERR_OUTER_CAP; return; }
         if (nw == DENSE_WIN && less(xd, xe, wd[0], we[0])) {
@@ -1380,14 +1387,11 @@ __global__ __launch_bounds__(64) void k_cand_dense(DS S) {
             // [0, no - DENSE_WIN) move up by one from x's place on
             int p = no - DENSE_WIN;
             while (p > 0) {
-                const double dp = od[(int64_t)(p - 1) * SS + k];
                 const int64_t ep = oj[(int64_t)(p - 1) * SS + k];
-                if (!less(xd, xe, dp, ep)) break;
-                od[(int64_t)p * SS + k] = dp;
+                if (!less(xd, xe, dist_of(ep), ep)) break;
                 oj[(int64_t)p * SS + k] = ep;
                 --p;
             }
-            od[(int64_t)p * SS + k] = xd;
             oj[(int64_t)p * SS + k] = xe;
             ++no;
             return;
@@ -1395,7 +1399,6 @@ __global__ __launch_bounds__(64) void k_cand_dense(DS S) {
         if (nw == DENSE_WIN) {
             // the smallest register entry is final: to its slot
             const int sl = no - DENSE_WIN;
-            od[(int64_t)sl * SS + k] = wd[0];
             oj[(int64_t)sl * SS + k] = we[0];
 #pragma unroll
             for (int w = 0; w < DENSE_WIN - 1; ++w) { wd[w] = wd[w + 1]; we[w] = we[w + 1]; }
