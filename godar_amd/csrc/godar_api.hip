@@ -15,7 +15,7 @@
 #include <vector>
 
 namespace godar {
-__global__ void k_begin(DS, int);
+__global__ void k_begin(DS, int, int);
 __global__ void k_ranges(DS);
 template <bool M> __global__ void k_cand(DS);
 template <bool F> __global__ void k_sup(DS);
@@ -484,8 +484,11 @@ int launch_step(godar_ctx* c, bool lean) {
     // once (the launch sequence, and a captured graph, is the same).
     {
         StageTimer t(c, 0);
-        const int nt = std::max({N, c->ncells_cap + 1, NQUEUE * XCD_COUNT});
-        hipLaunchKernelGGL(k_begin, grid_for(nt, B[0], false), dim3(B[0]), 0, s, d, lean ? 1 : 0);
+        // (a lean step only resets the queue counters and the members' words)
+        const int nt = lean ? std::max(c->Mcap, NQUEUE * XCD_COUNT)
+                            : std::max({N, c->ncells_cap + 1, NQUEUE * XCD_COUNT, c->Mcap});
+        hipLaunchKernelGGL(k_begin, grid_for(nt, B[0], false), dim3(B[0]), 0, s, d, lean ? 1 : 0,
+                           (int)c->Mcap);
     }
     int bits = 1;
     while ((1u << bits) <= (uint32_t)c->ncells_cap && bits < 32) ++bits;

@@ -394,7 +394,7 @@ __global__ void k_spill(DS S, int host) {
 //    input buffers (bkey, bval).  The position is read from the ordered copy
 //    where that is newer than rec (it is the same value the spill writes),
 //    so the spill and the keys need no order between them.
-__global__ void k_begin(DS S, int lean) {
+__global__ void k_begin(DS S, int lean, int nmembers) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (*S.err) return;
     if (lean && ((S.rebuild[0] & BUILD_DUE) || S.rebuild[1])) {
@@ -406,7 +406,13 @@ __global__ void k_begin(DS S, int lean) {
     // (the dense-search queue stands between builds)
     if (i < NQUEUE * XCD_COUNT && (build || i / XCD_COUNT != 3)) S.qn[i * QN_STRIDE] = 0;
     if (build && i <= S.ncells) S.crange[i] = make_int2(0, 0);
-    if (i >= S.N) return;
+    // per-member resets (thread m for member m)
+    if (i < nmembers && member_live(S.members[i])) {
+        S.nseeds[i] = 0; S.netab[i] = 0; S.nwtab[i] = 0;
+        S.stats[4 * i + 2] = 0; S.stats[4 * i + 3] = 0;
+        S.tcur[i] = S.members[i].tstep;
+    }
+    if (lean || i >= S.N) return;   // (a lean step's grid covers the counters and members only)
     if (newer && (build || refresh)) {
         const GeoRec g = S.geo[i];
         const uint32_t fs = S.cval[i];
@@ -421,14 +427,9 @@ __global__ void k_begin(DS S, int lean) {
         }
     }
     const int f = i;
+    if (!build) return;           // the cell order of the last build stands
     const int m = S.member_of[f];
     const bool live = m >= 0 && member_live(S.members[m]);
-    if (live && f == S.members[m].first) {
-        S.nseeds[m] = 0; S.netab[m] = 0; S.nwtab[m] = 0;
-        S.stats[4 * m + 2] = 0; S.stats[4 * m + 3] = 0;
-        S.tcur[m] = S.members[m].tstep;
-    }
-    if (!build) return;           // the cell order of the last build stands
     S.bval[f] = (uint32_t)f;
     // floes of free slots and of members past their stop step sort last and
     // take no part in the step
