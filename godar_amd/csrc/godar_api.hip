@@ -1690,6 +1690,15 @@ int godar_autotune(godar_ctx* c, int32_t trial) {
     if (rc != GODAR_OK) return rc;
     for (int st = 0; st < GODAR_NSTAGES; ++st) {
         if (!tunable_stage(st)) continue;
+        // The bond and wall kernels run beside the contact kernel (side
+        // streams), where single-wavefront blocks fit in the room it leaves;
+        // timed one kernel after another, as here, larger blocks look as
+        // fast, but overlapped 256-thread bond blocks cost about 3 % of a
+        // step at the mid-run of the 512-field ensemble.
+        if (st == 5 || st == 7) {
+            c->sblock[st] = 64;
+            continue;
+        }
         int best = c->sblock[st];
         double best_ms = 1e300;
         for (int b : cand) {

@@ -210,9 +210,11 @@ class Context:
         return {self.L.godar_stage_name(k).decode(): int(t[k]) for k in range(_lib.NSTAGES)}
 
     def autotune(self, trial: int = 3) -> dict:
-        """Pick each stage's block size, then the search shape (skin, thread
-        span, dense threshold), then the launch mode (graph replay or
-        one-by-one launches), by timing real steps (members advance).
+        """Pick each stage's block size (the bond and wall stages, which run
+        beside the contact stage, take 64 threads), then the search shape
+        (skin, thread span, dense threshold), then the launch mode (graph
+        replay or one-by-one launches), by timing real steps (members
+        advance).
         Returns the stage block sizes; `search()` holds the search shape."""
         self._chk(self.L.godar_autotune(self.h, int(trial)))
         return self.stage_blocks()

@@ -254,9 +254,10 @@ int godar_get_stage_blocks(const godar_ctx *ctx, int32_t *threads);   /* GODAR_N
  * has moved 0.45 skin, at most 5,000.  Then, unless graphs are off
  * (godar_use_graphs) or steps are being profiled, time max(32, 16 trial)
  * steps in each launch mode twice, alternating, and keep the mode with the
- * faster run.  The members advance by 1 + 3 * trial * (tunable stages)
- * steps, at most 6 (1 + max(4, 4 trial)) for the search shape and
- * 4 (1 + max(32, 16 trial)) for the launch mode. */
+ * faster run.  The bond and wall stages, which run beside the contact stage,
+ * are given 64-thread blocks rather than timed.  The members advance by
+ * 1 + 3 * trial * (timed stages) steps, at most 6 (1 + max(4, 4 trial)) for
+ * the search shape and 4 (1 + max(32, 16 trial)) for the launch mode. */
 int godar_autotune(godar_ctx *ctx, int32_t trial);
 /* Step launch mode (no counterpart in the reference): on (the default), each
  * step replays a captured HIP graph of the whole kernel sequence, recaptured
