@@ -370,6 +370,38 @@ def test_graph_replay_matches_eager():
         graph.close()
 
 
+def test_side_streams_match_serial_launches():
+    """The bond and wall kernels run beside the contact kernel on two side
+    streams; steps timed stage by stage launch the three one after another.
+    Both give the same trajectory, forces and pair sets, bitwise, on a packed
+    box where pairs are bonded and in contact at once (both kernels set bits
+    of such a pair's outer flag word, with atomicOr)."""
+    fb = kick(squeeze(packed_box(64, 3)), 1)
+    p = config0_params(fb, sigmanc_crit=1e5, bond_lim=50.0, bond_frac=1.0)
+    for prec in ("fp64", "mixed"):
+        serial, forked = _ctx(fb.n, 1, prec), _ctx(fb.n, 1, prec)
+        serial.profile(True)
+        forked.use_graphs(False)
+        ms, mf = serial.add_field(p, fb), forked.add_field(p, fb)
+        both = 0
+        for rnd in range(6):
+            serial.step(10)
+            forked.step(10)
+            a, b = serial.floes(ms), forked.floes(mf)
+            for key in STATE:
+                assert np.array_equal(a[key], b[key]), (prec, rnd, key)
+            fs, ff = serial.forces(ms), forked.forces(mf)
+            for key in ("tfx", "tfy", "m", "fcx", "fbx", "mb", "fx_bc"):
+                assert np.array_equal(fs[key], ff[key]), (prec, rnd, key)
+            cs, cf = _pairs(serial, ms, "contact"), _pairs(forked, mf, "contact")
+            bs, bf = _pairs(serial, ms, "bond"), _pairs(forked, mf, "bond")
+            assert cs == cf and bs == bf, (prec, rnd)
+            both += len(set(cs) & set(bs))
+        assert both > 0, prec     # pairs bonded and in contact at once
+        serial.close()
+        forked.close()
+
+
 def test_block_sizes_and_autotune_bitwise():
     """Results do not depend on the kernels' block sizes: 64-thread blocks,
     the autotuner's choice (which itself steps the members) and the default
