@@ -1191,7 +1191,7 @@ __device__ __forceinline__ int pow2_at_least(int n) {
 // entry goes to its rank, counted over the floe's entries in LDS.  A
 // superset longer than SUP_ORDER_CAP stays in scan order (sout = -1).  Only
 // on build steps.
-constexpr int SUP_ORDER_CAP = 1024;
+constexpr int SUP_ORDER_CAP = 256;   // (16 bytes of LDS per entry: 4 KB per wavefront)
 template <bool FILL>
 __global__ __launch_bounds__(64) void k_sup_dense(DS S) {
     __shared__ double od_[FILL ? SUP_ORDER_CAP : 1];

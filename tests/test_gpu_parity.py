@@ -503,11 +503,12 @@ def test_dense_search_long_lists_and_unordered_supersets():
     """Dense floes with long candidate lists give the same lists, sheltering
     and trajectories through the dense search as through the thread search,
     bitwise, in FP64 and mixed precision: a member with a search radius of
-    8 km (many floes with more than 256 candidates on a side, so that
-    entries often go below the register part of the outer list), and one
-    with a radius of 12 km whose supersets exceed 1,024 entries (left in
-    scan order at the build and handed to the thread search)."""
-    cases = [(packed_box(600, 5, r_lo=200.0, r_hi=400.0), 8.0e3, 256, 6),
+    5 km (many floes with more than 48 candidates on a side, so that entries
+    often go below the register part of the outer list, in supersets short
+    enough for the build to order), and one with a radius of 12 km whose
+    supersets exceed 1,024 entries (left in scan order at the build and
+    handed to the thread search)."""
+    cases = [(packed_box(600, 5, r_lo=200.0, r_hi=400.0), 5.0e3, 48, 6),
              (packed_box(1500, 8, box=1.4e4, r_lo=100.0, r_hi=200.0), 1.2e4, 1024, 3)]
     for f, rtree, big, steps in cases:
         p = config0_params(f, rtree=rtree)
@@ -523,8 +524,10 @@ def test_dense_search_long_lists_and_unordered_supersets():
             (sa, ca, ha), (sb, cb, hb) = runs
             outer = np.bincount([i for i, _ in ca], minlength=f.n)
             inner = np.bincount([j for _, j in ca], minlength=f.n)
-            many = ((outer > big) | (inner > big)) if big == 256 else (outer + inner > big)
+            many = ((outer > big) | (inner > big)) if big < 1024 else (outer + inner > big)
             assert 0.02 < many.mean(), (rtree, many.mean())
+            if big < 1024:      # most supersets short enough to be ordered (256 entries)
+                assert np.mean(outer + inner < 200) > 0.9, (rtree, np.mean(outer + inner))
             for k in STATE:
                 assert np.array_equal(sa[k], sb[k]), (prec, rtree, k)
             assert ca == cb
