@@ -33,7 +33,7 @@ template <bool M> __global__ void k_walls(DS);
 template <bool M> __global__ void k_repair(DS, int);
 __global__ void k_fixq(DS);
 template <bool M> __global__ void k_fixup(DS);
-template <bool M> __global__ void k_final(DS);
+template <bool M, bool R> __global__ void k_final(DS);
 __global__ void k_totals(DS, int64_t*);
 
 __global__ void k_fill_u8(uint8_t* p, int64_t n, uint8_t v) {
@@ -587,8 +587,9 @@ int launch_step(godar_ctx* c, bool lean) {
     }
     {
         StageTimer t(c, 11);
-        hipLaunchKernelGGL(mixed ? k_final<true> : k_final<false>, grid_for(N, B[11], false),
-                           dim3(B[11]), 0, s, d);
+        auto kf = mixed ? (c->record ? k_final<true, true> : k_final<true, false>)
+                        : (c->record ? k_final<false, true> : k_final<false, false>);
+        hipLaunchKernelGGL(kf, grid_for(N, B[11], false), dim3(B[11]), 0, s, d);
     }
     HIPCHK(c, hipGetLastError());
     c->parity ^= 1;

@@ -2900,7 +2900,9 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_stress(DS S) {
 // 6. final properties, forcing + coriolis, totals, integration
 //    (stepper.f90:295-343, integration.f90:1-38)
 // ---------------------------------------------------------------------
-template <bool MIXED>
+// (REC: forces are recorded; the other instance carries no code for the
+// recorded columns and stress rows)
+template <bool MIXED, bool REC>
 __global__ __launch_bounds__(FLOE_BLOCK, 4) void k_final(DS S) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= S.N) return;
@@ -2951,7 +2953,7 @@ __global__ __launch_bounds__(FLOE_BLOCK, 4) void k_final(DS S) {
         const bool bond_row = c == A_FBX || c == A_FBY || c == A_MB || c == A_TAB || c == A_PB;
         const bool shel_row = c == A_SHA || c == A_SHW;
         const bool stress_row = c >= A_SXX;
-        const bool valid = stress_row ? S.record_forces != 0
+        const bool valid = stress_row ? REC
                          : shel_row || (bond_row ? (cl & CLS_BOND) : (cl & CLS_CONTACT));
         a[c] = valid ? S.acc[c * SS + k] : 0.0;
     }
@@ -2992,7 +2994,7 @@ __global__ __launch_bounds__(FLOE_BLOCK, 4) void k_final(DS S) {
         const int leader = __ffsll((long long)(md | ms)) - 1;
         if (bits && (int)(threadIdx.x & 63) == leader) atomicOr(S.rebuild, bits);
     }
-    if (S.record_forces) {
+    if (REC) {
         double tp = (a[A_TAC] != 0.0 ? a[A_PC] / a[A_TAC] : 0.0)
                   + (a[A_TAB] != 0.0 ? a[A_PB] / a[A_TAB] : 0.0)
                   + (W.ta_bc != 0.0 ? W.p_bc / W.ta_bc : 0.0);
@@ -3096,7 +3098,9 @@ template __global__ void k_repair<false>(DS, int);
 template __global__ void k_repair<true>(DS, int);
 template __global__ void k_fixup<false>(DS);
 template __global__ void k_fixup<true>(DS);
-template __global__ void k_final<false>(DS);
-template __global__ void k_final<true>(DS);
+template __global__ void k_final<false, false>(DS);
+template __global__ void k_final<true, false>(DS);
+template __global__ void k_final<false, true>(DS);
+template __global__ void k_final<true, true>(DS);
 
 }  // namespace godar
