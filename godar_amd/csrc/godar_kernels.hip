@@ -626,7 +626,7 @@ __device__ __forceinline__ int sup_thread(const DS& S, int k, const GeoRec& me, 
 // (floe, candidate) pairs of one wavefront in LDS: entry, distance (inner:
 // -1) and owner lane, then the per-lane pair counts
 constexpr int CAND_TCAP = 512;
-constexpr int CAND_LDS_PER_WAVE = CAND_TCAP * (8 + 8 + 1) + 64 * 4;
+constexpr int CAND_LDS_PER_WAVE = CAND_TCAP * (8 + 8 + 1) + 64 * 4 + CAND_TCAP * 2;
 
 // Neighbour search of one floe by one thread: its candidate lists are the
 // entries of its superset within the query radius, inserted into the
@@ -774,8 +774,11 @@ __device__ __forceinline__ void cand_thread(const DS& S, int k, const GeoRec& me
     int abase = 0, acount = 0, atotal = 0;
     int64_t* te = reinterpret_cast<int64_t*>(lds);                      // entry, then the water term
     double* td = reinterpret_cast<double*>(lds + 8 * CAND_TCAP);         // distance (inner: -1), then the air term
-    uint8_t* to = lds + 16 * CAND_TCAP;                                   // owner lane, then flags
+    uint8_t* to = lds + 16 * CAND_TCAP;                                   // owner lane, then flags | 0x80
     int32_t* tn = reinterpret_cast<int32_t*>(lds + 17 * CAND_TCAP);      // pairs per lane
+    // the pairs whose terms the second pass evaluates (indices into the above)
+    uint16_t* tx = reinterpret_cast<uint16_t*>(lds + 17 * CAND_TCAP + 64 * 4);
+    int nneed = 0;
     if (WAVE) {
         const int lane = (int)(threadIdx.x & 63);
         const uint64_t lt = lane ? ((~0ull) >> (64 - lane)) : 0ull;
@@ -789,6 +792,9 @@ __device__ __forceinline__ void cand_thread(const DS& S, int k, const GeoRec& me
         tn[lane] = 0;
         wave_lds_sync();
         bool ovf = false;
+        // the floe's switches for the second pass: sheltering on (bit 0), bonds
+        // may form this step (bit 1)
+        const int pflags = (shl != 0 ? 1 : 0) | (bond_step ? 2 : 0);
         // an entry's owner: the last lane whose superset starts at or before
         // it (lanes without entries share the next one's start); idx < totalL
         auto owner_of = [&](int idx) {
@@ -819,9 +825,10 @@ __device__ __forceinline__ void cand_thread(const DS& S, int k, const GeoRec& me
                 e_nxt = entry_at(idn, own_nxt);
             }
             const double ox = __shfl(mx, own), oy = __shfl(my, own), orad = __shfl(rad, own);
-            const double ort = __shfl(rtree, own);
-            const int of = __shfl(f, own);
-            bool hit = false;
+            const double ort = __shfl(rtree, own), orr = __shfl(mr, own);
+            const double ohfa = __shfl(msh.hfa, own), ohfw = __shfl(msh.hfw, own);
+            const int of = __shfl(f, own), ofl = __shfl(pflags, own);
+            bool hit = false, need = false;
             double d = 0.0;
             int j = 0, q = 0;
             if (idx < totalL) {
@@ -837,17 +844,32 @@ __device__ __forceinline__ void cand_thread(const DS& S, int k, const GeoRec& me
                     d = sqrt(dist2);
                     hit = d <= thr;
                 }
+                if (hit) {
+                    // The second pass changes nothing for a pair out of
+                    // contact, with no bond forming this step, whose two
+                    // sheltering terms are exactly 1: the test shelter_min
+                    // makes (the same overlap, the floe's own drafts, type T).
+                    const double dn = j > of ? orr + o.r - d : o.r + orr - d;
+                    const T dnt = T(dn);
+                    const T lim = sizeof(T) == 4 ? T(20) : T(40);
+                    const bool one_a = dnt < T(0) && T((double)0.18f) * -dnt >= lim * T(ohfa);
+                    const bool one_w = dnt < T(0) && T((double)0.18f) * -dnt >= lim * T(ohfw);
+                    need = dn > 0 || (ofl & 2) || ((ofl & 1) && !(one_a && one_w));
+                }
             }
             const uint64_t mh = __ballot(hit);
             if (atotal + __popcll(mh) > CAND_TCAP) { ovf = true; break; }
+            const uint64_t mn = __ballot(need);
             if (hit) {
                 const int at = atotal + __popcll(mh & lt);
                 te[at] = ent(j, q);
                 td[at] = j > of ? d : -1.0;
                 to[at] = (uint8_t)own;
                 atomicAdd(&tn[own], 1);
+                if (need) tx[nneed + __popcll(mn & lt)] = (uint16_t)at;
             }
             atotal += __popcll(mh);
+            nneed += __popcll(mn);
         }
         wave_lds_sync();
         if (!ovf) {
@@ -899,9 +921,9 @@ __device__ __forceinline__ void cand_thread(const DS& S, int k, const GeoRec& me
     if (flat) {
         const int lane = (int)(threadIdx.x & 63);
         wave_lds_sync();
-        for (int c0 = 0; c0 < atotal; c0 += 64) {
-            const int idx = c0 + lane;
-            const bool valid = idx < atotal;
+        for (int c0 = 0; c0 < nneed; c0 += 64) {
+            const bool valid = c0 + lane < nneed;
+            const int idx = valid ? (int)tx[c0 + lane] : 0;
             int64_t e = 0;
             double d = -1.0;
             int own = lane;
@@ -951,12 +973,13 @@ __device__ __forceinline__ void cand_thread(const DS& S, int k, const GeoRec& me
             if (valid) {
                 td[idx] = (double)ta;
                 te[idx] = __double_as_longlong((double)tw);
-                to[idx] = (uint8_t)fl;
+                to[idx] = (uint8_t)(fl | 0x80u);
             }
         }
         wave_lds_sync();
         if (act) {
             for (int i = abase; i < abase + acount; ++i) {
+                if (!(to[i] & 0x80u)) continue;      // not evaluated: terms 1, no flags
                 sha = dmin(sha, T(td[i]));
                 shw = dmin(shw, T(__longlong_as_double(te[i])));
                 qc = qc || (to[i] & 1u);
