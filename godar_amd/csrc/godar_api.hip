@@ -97,12 +97,12 @@ struct Range { int64_t first, n; };
 
 struct godar_ctx {
     hipStream_t stream = nullptr;
-    // The bond and wall kernels run beside the contact kernel, on two side
-    // streams forked from and joined back to `stream` by events (a captured
+    // The contact and wall kernels run beside the bond kernel, on a side
+    // stream forked from and joined back to `stream` by events (a captured
     // step graph takes the same branches); steps timed stage by stage run
     // them one after another.
-    hipStream_t side[2] = {nullptr, nullptr};
-    hipEvent_t efork = nullptr, ejoin[2] = {nullptr, nullptr};
+    hipStream_t side = nullptr;
+    hipEvent_t efork = nullptr, ejoin = nullptr;
     int precision = GODAR_FP64;
     int64_t S = 0;
     int Mcap = 0;
@@ -535,35 +535,35 @@ int launch_step(godar_ctx* c, bool lean) {
         }
     }
     // contacts, bonds and walls touch disjoint state (the flag words both
-    // of the first two set are set with atomicOr), so they may run together
+    // the contact and the bond kernel set are set with atomicOr), so they may
+    // run together: contacts and walls on the side stream, bonds (the
+    // longest) on the step's stream, whose next kernel waits for the side
+    // stream once
     const bool fork = !c->prof;
-    hipStream_t sb = fork ? c->side[0] : s, sw = fork ? c->side[1] : s;
+    hipStream_t sc = fork ? c->side : s;
     if (fork) {
         (void)hipEventRecord(c->efork, s);
-        (void)hipStreamWaitEvent(sb, c->efork, 0);
-        (void)hipStreamWaitEvent(sw, c->efork, 0);
+        (void)hipStreamWaitEvent(sc, c->efork, 0);
     }
     {
         StageTimer t(c, 4);
         // LDS: one pass of 64 pair results per wavefront
         hipLaunchKernelGGL(mixed ? k_contact<true> : k_contact<false>, grid_queue(N, B[4]),
-                           dim3(B[4]), (B[4] / 64) * CONTACT_LDS_PER_WAVE, s, d);
-    }
-    {
-        StageTimer t(c, 5);
-        hipLaunchKernelGGL(mixed ? k_bond<true> : k_bond<false>, grid_queue(N, B[5]),
-                           dim3(B[5]), 0, sb, d);
+                           dim3(B[4]), (B[4] / 64) * CONTACT_LDS_PER_WAVE, sc, d);
     }
     {
         StageTimer t(c, 7);
         hipLaunchKernelGGL(mixed ? k_walls<true> : k_walls<false>, grid_queue(N, B[7]),
-                           dim3(B[7]), 0, sw, d);
+                           dim3(B[7]), 0, sc, d);
+    }
+    {
+        StageTimer t(c, 5);
+        hipLaunchKernelGGL(mixed ? k_bond<true> : k_bond<false>, grid_queue(N, B[5]),
+                           dim3(B[5]), 0, s, d);
     }
     if (fork) {
-        (void)hipEventRecord(c->ejoin[0], sb);
-        (void)hipEventRecord(c->ejoin[1], sw);
-        (void)hipStreamWaitEvent(s, c->ejoin[0], 0);
-        (void)hipStreamWaitEvent(s, c->ejoin[1], 0);
+        (void)hipEventRecord(c->ejoin, sc);
+        (void)hipStreamWaitEvent(s, c->ejoin, 0);
     }
     {
         StageTimer t(c, 8);
@@ -715,10 +715,8 @@ int godar_create(int64_t max_floes, int32_t max_members, int32_t precision, goda
     DS& d = c->d;
     int64_t S = c->S;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
-    for (int k = 0; k < 2 && e == hipSuccess; ++k) {
-        e = hipStreamCreateWithFlags(&c->side[k], hipStreamNonBlocking);
-        if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ejoin[k], hipEventDisableTiming);
-    }
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ejoin, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->efork, hipEventDisableTiming);
 #define ACHK(x)                                        \
     do {                                               \
@@ -846,10 +844,8 @@ int godar_destroy(godar_ctx* c) {
     radix_free(c->rs);
     collect_events(c);
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
-    for (int k = 0; k < 2; ++k) {
-        if (c->ejoin[k]) (void)hipEventDestroy(c->ejoin[k]);
-        if (c->side[k]) (void)hipStreamDestroy(c->side[k]);
-    }
+    if (c->ejoin) (void)hipEventDestroy(c->ejoin);
+    if (c->side) (void)hipStreamDestroy(c->side);
     if (c->efork) (void)hipEventDestroy(c->efork);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -1691,8 +1687,9 @@ int godar_autotune(godar_ctx* c, int32_t trial) {
     if (rc != GODAR_OK) return rc;
     for (int st = 0; st < GODAR_NSTAGES; ++st) {
         if (!tunable_stage(st)) continue;
-        // The bond and wall kernels run beside the contact kernel (side
-        // streams), where single-wavefront blocks fit in the room it leaves;
+        // The bond and wall kernels run beside the contact kernel (the
+        // contacts and walls on a side stream), where single-wavefront blocks
+        // fit in the room the others leave;
         // timed one kernel after another, as here, larger blocks look as
         // fast, but overlapped 256-thread bond blocks cost about 3 % of a
         // step at the mid-run of the 512-field ensemble.

@@ -210,8 +210,8 @@ class Context:
         return {self.L.godar_stage_name(k).decode(): int(t[k]) for k in range(_lib.NSTAGES)}
 
     def autotune(self, trial: int = 3) -> dict:
-        """Pick each stage's block size (the bond and wall stages, which run
-        beside the contact stage, take 64 threads), then the search shape
+        """Pick each stage's block size (the bond and wall stages, which
+        overlap the contact stage, take 64 threads), then the search shape
         (skin, thread span, dense threshold), then the launch mode (graph
         replay or one-by-one launches), by timing real steps (members
         advance).

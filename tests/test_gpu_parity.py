@@ -371,8 +371,8 @@ def test_graph_replay_matches_eager():
 
 
 def test_side_streams_match_serial_launches():
-    """The bond and wall kernels run beside the contact kernel on two side
-    streams; steps timed stage by stage launch the three one after another.
+    """The contact and wall kernels run on a side stream beside the bond
+    kernel; steps timed stage by stage launch the three one after another.
     Both give the same trajectory, forces and pair sets, bitwise, on a packed
     box where pairs are bonded and in contact at once (both kernels set bits
     of such a pair's outer flag word, with atomicOr)."""
