@@ -703,7 +703,12 @@ int godar_create(int64_t max_floes, int32_t max_members, int32_t precision, goda
     *out = nullptr;
     godar_ctx* c = new godar_ctx();
     c->precision = precision;
+    // default block sizes: 256 threads, except the bond and wall stages,
+    // which run beside the contact stage (64), and the integration (128);
+    // measured on the 512-field ensemble and the served mix
     for (int k = 0; k < GODAR_NSTAGES; ++k) c->sblock[k] = FLOE_BLOCK;
+    c->sblock[5] = c->sblock[7] = 64;
+    c->sblock[11] = 128;
     c->S = max_floes;
     c->Mcap = max_members;
     c->members.assign(max_members, Member{0, 0, 1, 0, 0});

@@ -215,7 +215,8 @@ int godar_get_totals_noalloc(godar_ctx *ctx, int64_t *totals);
 /* Pipeline tuning (no counterpart in the reference).  Threads per block
  * (64, 128 or 256) of the per-floe kernels of every stage, or of one stage
  * (stage numbers as in godar_stage_name; the sort and repair stages have
- * fixed shapes).  Results do not depend on block sizes. */
+ * fixed shapes).  Defaults: 256, except 64 for the bond and wall stages and
+ * 128 for the integration.  Results do not depend on block sizes. */
 int godar_set_block(godar_ctx *ctx, int32_t threads);
 int godar_set_stage_block(godar_ctx *ctx, int32_t stage, int32_t threads);
 /* Members added afterwards whose neighbour search examines at least `floes`
