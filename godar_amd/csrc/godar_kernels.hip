@@ -1,8 +1,9 @@
 // Kernels of the GODAR step pipeline (one launch sequence per step):
 //
 //   k_begin -> [build steps: radix sort -> k_ranges -> k_sup ...] -> k_cand
-//    -> k_contact -> k_bond -> k_walls
-//    -> k_repair -> k_fixup -> k_final (which also gathers the bond terms)
+//    -> (k_contact -> k_walls on a side stream) beside k_bond
+//    -> k_comp -> k_repair -> k_fixq -> k_fixup
+//    -> k_final (which also gathers the bond terms)
 //
 // Sequential semantics of the reference (stepper.f90:71-280 visits floes i
 // in index order and, for each, its candidates j > i in ascending distance;
@@ -25,8 +26,9 @@
 //     per-member k_repair walks, in sequence order, the seeds and every later
 //     event of each reshaped floe, recomputes them with the current floe
 //     properties and records the results.
-//   * k_fixup re-gathers the floes touched by the repair, k_final applies
-//     the final floe properties, forcing, Coriolis, totals and integration.
+//   * k_fixq lists the floes touched by the repair, k_fixup re-gathers them,
+//     k_final applies the final floe properties, forcing, Coriolis, totals
+//     and integration.
 //
 // Layout: the per-step kernels run over floes in cell order ("pos"), reading
 // a sorted copy of the 96-byte floe records, so that a floe's neighbours sit
