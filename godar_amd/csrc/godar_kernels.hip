@@ -36,6 +36,7 @@
 // partner as (id << 32 | pos).
 #include "godar_engine.h"
 #include "godar_physics.h"
+#include <type_traits>
 
 namespace godar {
 
@@ -1168,6 +1169,16 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_cand(DS S) {
 // (floes move little between builds), so each is inserted among the last
 // DENSE_WIN entries, which stay in registers, and only the rare entry that
 // belongs below them is inserted into the part of the list in HBM.
+// The lists are slot-major (slot s of floe k at s * S + k), so a thread that
+// stored each entry as it found it would touch a line per entry, 64 lines per
+// store of a wavefront.  Instead the scan takes the outer part of the
+// supersets in one pass and the inner part in a second, so that the lanes'
+// lists grow nearly in step, and each thread stages its rows in a ring of its
+// own in LDS; the wavefront writes a row out for all its floes at once when
+// every thread still scanning has passed it, or when a thread needs the room.
+// A row a thread reaches after its wavefront wrote it out goes straight to
+// HBM; a thread that inserts below its register part first writes its staged
+// outer rows out and keeps its outer list in HBM from then on.
 // Sheltering terms are needed only for pairs close enough that
 // exp(-0.18 |dn/cos| / hf) still counts (shelter_min); the threads list those
 // pairs in LDS and the wavefront evaluates them 64 at a time, each floe
@@ -1177,7 +1188,10 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_cand(DS S) {
 constexpr int DENSE_OVER = -1;   // ocnt of a dense floe left to the thread search
 constexpr int DENSE_WIN = 4;     // outer candidates in registers
 constexpr int DENSE_NEAR = 512;  // near pairs listed per wavefront
-constexpr int DENSE_BATCH = 6;   // superset entries a thread loads at once
+constexpr int DENSE_BATCH = 6;   // superset entries a thread loads at once (mixed precision)
+constexpr int DENSE_RING = 16;   // list rows a thread stages in LDS
+static_assert((DENSE_RING & (DENSE_RING - 1)) == 0 && DENSE_RING >= DENSE_BATCH + DENSE_WIN,
+              "a ring of whole rows with room for a batch past the register part");
 
 __device__ __forceinline__ void bitonic_inner(int64_t* ke, int n, int lane) {
     for (int size = 2; size <= n; size <<= 1) {
@@ -1308,11 +1322,12 @@ __global__ __launch_bounds__(64) void k_sup_dense(DS S) {
     }
 }
 
-// A pair whose sheltering terms are needed.
+// A pair whose sheltering terms are needed (its distance is recomputed from
+// the two positions, the same expression as the scan's).
 struct DenseNear {
-    double d;
     int32_t q;                   // partner position
-    uint8_t lane, outer, pad0_, pad1_;
+    uint8_t lane, outer;
+    uint16_t pad_;
 };
 
 // Order-preserving map of doubles to unsigned keys (a < b iff key(a) < key(b)
@@ -1325,11 +1340,15 @@ __device__ __forceinline__ double ord_val(uint64_t k) {
     return __longlong_as_double((long long)((k >> 63) ? (k & ~(1ull << 63)) : ~k));
 }
 
+// (3 wavefronts per SIMD: the registers of a batch of 6 superset entries in
+// mixed precision, of 5 in FP64, and the LDS of the ring and the near-pair
+// list fit them without spilling)
 template <bool MIXED>
-__global__ __launch_bounds__(64) void k_cand_dense(DS S) {
+__global__ __launch_bounds__(64, 3) void k_cand_dense(DS S) {
     using T = typename Prec<MIXED>::T;
     __shared__ DenseNear nr[DENSE_NEAR];
     __shared__ uint64_t mina[64], minw[64];
+    __shared__ int64_t ring_[DENSE_RING * 64];   // staged rows: row r of lane l at (r % DENSE_RING) * 64 + l
     __shared__ int nnear;
     if (*S.err) return;
     const int lane = threadIdx.x;
@@ -1358,33 +1377,34 @@ __global__ __launch_bounds__(64) void k_cand_dense(DS S) {
     if (lane == 0) nnear = 0;
     const int m = act ? me.member : 0;
     const DevParams& P = S.params[m];
-    // the parameters the scan reads, held in registers (the list stores
-    // below could otherwise alias them)
-    const double rtree = P.rtree, bond_lim = P.bond_lim;
-    const uint64_t bseed = P.bond_seed;
-    const uint32_t bthr = P.bond_thr24;
+    // the parameters every entry of the scan reads, held in registers (the
+    // list stores below could otherwise alias them); the bond parameters, the
+    // wind and the floe's sheltering record are read again where needed, so
+    // that the registers go to the batch of partners
+    const double rtree = P.rtree;
     const bool shelter = P.shelter != 0;
     const double rad = me.r + rtree;
     const int f = me.id;
     const int first = S.members[m].first;
     const bool bond_step = act && S.members[m].tstep == 1 && P.restart != 1 && P.cohesion;
-    ShelRec msh;
-    msh.hfa = msh.hfw = msh.wx = msh.wy = 0.0;
+    const T lim = sizeof(T) == 4 ? T(20) : T(40);
+    T lha = T(0), lhw = T(0);   // lim times the floe's two drafts
     int fs = 0;
     bool qc = false, qb = false;
     if (act) {
-        msh = S.shel[k];
+        const ShelRec sh = S.shel[k];
+        lha = lim * T(sh.hfa);
+        lhw = lim * T(sh.hfw);
         fs = S.cval[k];
         qc = S.hcnt_old[fs] > 0;
         qb = S.bcnt_old[fs] > 0;
     }
-    if (!act) L = 0;
-    const T uwi = T(P.uw), vwi = T(P.vw);
-    const T lim = sizeof(T) == 4 ? T(20) : T(40);
+    if (!act) L = nout = 0;
     T sha = T(1), shw = T(1);
     wave_lds_sync();
     const int64_t SS = S.S;
     int64_t* oj = S.oj;
+    const int64_t* se = S.sup + soff;
     // the last min(no, DENSE_WIN) outer entries, ascending: slots [no - nw, no)
     double wd[DENSE_WIN];
     int64_t we[DENSE_WIN];
@@ -1394,7 +1414,12 @@ __global__ __launch_bounds__(64) void k_cand_dense(DS S) {
         we[w] = INT64_MAX;
     }
     int no = 0, ni = 0, nw = 0;
+    // rows below bo (outer) and bi (inner) are in HBM, a lane's rows from
+    // there on in its ring; od: this lane's outer list is all in HBM
+    int bo = 0, bi = 0;
+    bool od = false;
     uint32_t over = 0;
+    auto rslot = [&](int r) { return (r & (DENSE_RING - 1)) * 64 + lane; };
     auto less = [](double ad, int64_t ae, double bd, int64_t be) {
         return ad < bd || (ad == bd && ent_id(ae) < ent_id(be));
     };
@@ -1406,9 +1431,18 @@ __global__ __launch_bounds__(64) void k_cand_dense(DS S) {
         const double dx = me.x - o.x, dy = me.y - o.y;
         return sqrt(dx * dx + dy * dy);
     };
+    auto put_outer = [&](int s, int64_t e) {
+        if (od || s < bo) oj[(int64_t)s * SS + k] = e;
+        else ring_[rslot(s)] = e;
+    };
     auto insert_outer = [&](double xd, int64_t xe) {
         if (no >= S.CO) { over |= ERR_OUTER_CAP; return; }
         if (nw == DENSE_WIN && less(xd, xe, wd[0], we[0])) {
+            if (!od) {
+                // the staged rows [bo, no - DENSE_WIN) out first
+                for (int s = bo; s < no - DENSE_WIN; ++s) oj[(int64_t)s * SS + k] = ring_[rslot(s)];
+                od = true;
+            }
             // below the register part: into the part in HBM, whose slots
             // [0, no - DENSE_WIN) move up by one from x's place on
             int p = no - DENSE_WIN;
@@ -1424,8 +1458,7 @@ __global__ __launch_bounds__(64) void k_cand_dense(DS S) {
         }
         if (nw == DENSE_WIN) {
             // the smallest register entry is final: to its slot
-            const int sl = no - DENSE_WIN;
-            oj[(int64_t)sl * SS + k] = we[0];
+            put_outer(no - DENSE_WIN, we[0]);
 #pragma unroll
             for (int w = 0; w < DENSE_WIN - 1; ++w) { wd[w] = wd[w + 1]; we[w] = we[w + 1]; }
             wd[DENSE_WIN - 1] = __longlong_as_double(0x7ff0000000000000ll);
@@ -1447,81 +1480,128 @@ __global__ __launch_bounds__(64) void k_cand_dense(DS S) {
         ++nw;
         ++no;
     };
-    constexpr int B = DENSE_BATCH;
-    const int64_t* se = S.sup + soff;
-    for (int v0 = 0; v0 < L; v0 += B) {
-        int64_t ev[B];
-        GeoRec g[B];
+    // row bo (bi) out to HBM for every lane that has it staged (the whole
+    // wavefront takes part)
+    auto out_outer = [&]() {
+        if (!od && no - nw > bo) oj[(int64_t)bo * SS + k] = ring_[rslot(bo)];
+        ++bo;
+    };
+    auto out_inner = [&]() {
+        if (ni > bi) S.ik[(int64_t)bi * SS + k] = ring_[rslot(bi)];
+        ++bi;
+    };
+    constexpr int B = MIXED ? DENSE_BATCH : DENSE_BATCH - 1;
+    // one pass over a part of every lane's superset: outer entries [0, nout),
+    // then inner ones [nout, L), entry t of the part at the same step
+    auto scan = [&](auto outer_part) {
+        constexpr bool outer = decltype(outer_part)::value;
+        const int vb = outer ? 0 : nout;
+        const int n = outer ? nout : L - nout;
+        int nmax = n;
 #pragma unroll
-        for (int u = 0; u < B; ++u) ev[u] = se[min(v0 + u, L - 1)];   // the last entry stands in past the end
+        for (int off = 32; off > 0; off >>= 1) nmax = max(nmax, __shfl_xor(nmax, off));
+        for (int t0 = 0; t0 < nmax; t0 += B) {
+            const bool run = t0 < n;
+            // rows out: each row that every lane still scanning has passed,
+            // and those that make room for this batch (at most B rows, the
+            // register part's included)
+            for (;;) {
+                const int c = outer ? no - nw : ni;
+                const int b = outer ? bo : bi;
+                const bool mine = outer ? !od : true;
+                if (!__ballot(mine && c > b)) break;
+                if (!__ballot(run && mine && (outer ? no : ni) + B > b + DENSE_RING) &&
+                    __ballot(run && mine && c <= b))
+                    break;
+                if (outer) out_outer(); else out_inner();
+            }
+            if (!run) continue;
+            int64_t ev[B];
+            GeoRec g[B];
 #pragma unroll
-        for (int u = 0; u < B; ++u) g[u] = S.geo[ent_pos(ev[u])];
+            for (int u = 0; u < B; ++u) ev[u] = se[vb + min(t0 + u, n - 1)];   // the last entry stands in past the end
 #pragma unroll
-        for (int u = 0; u < B; ++u) {
-            if (v0 + u >= L) continue;
-            const GeoRec& o = g[u];
-            const bool outer = v0 + u < nout;
-            const double dx = me.x - o.x, dy = me.y - o.y;
-            const double dist2 = dx * dx + dy * dy;
-            const double thr = outer ? rad : o.r + rtree;
-            // far beyond the radius: no square root needed
-            if (dist2 > thr * thr * (1.0 + 1e-12)) continue;
-            const double d = sqrt(dist2);
-            if (!(d <= thr)) continue;
-            // outer: pair (f, j) with f the outer floe i of the event;
-            // inner: pair (j, f) with f the inner floe j
-            const double deltan = outer ? me.r + o.r - d : o.r + me.r - d;
-            if (deltan > 0) qc = true;
-            if (outer && bond_step && deltan >= -bond_lim &&
-                bond_keep(f - first, ent_id(ev[u]) - first, bseed, bthr))
-                qb = true;
-            if (shelter) {
-                const T dn = T(deltan);
-                const bool near = !(dn < T(0) && T((double)0.18f) * -dn >= lim * T(msh.hfa)) ||
-                                  !(dn < T(0) && T((double)0.18f) * -dn >= lim * T(msh.hfw));
-                if (near) {
-                    const int slot = atomicAdd(&nnear, 1);
-                    if (slot < DENSE_NEAR) {
-                        DenseNear r;
-                        r.d = d; r.q = ent_pos(ev[u]); r.lane = (uint8_t)lane; r.outer = outer ? 1 : 0;
-                        r.pad0_ = r.pad1_ = 0;
-                        nr[slot] = r;
-                    } else {
-                        // list full: this thread takes the terms itself
-                        const double nx = outer ? o.x - me.x : me.x - o.x;
-                        const double ny = outer ? o.y - me.y : me.y - o.y;
-                        const double cosa = nx / d, sina = ny / d;
-                        const ShelRec oh = S.shel[ent_pos(ev[u])];
-                        const T c0t = T(cosa), s0t = T(sina);
-                        const T c = outer ? -c0t : c0t, sn = outer ? -s0t : s0t;
-                        const T ux = outer ? T(msh.wx) : T(oh.wx), uy = outer ? T(msh.wy) : T(oh.wy);
-                        sha = shelter_min(sha, T(oh.hfa), T(msh.hfa), dn, c, sn, ux, uy);
-                        shw = shelter_min(shw, T(oh.hfw), T(msh.hfw), dn, c, sn, uwi, vwi);
+            for (int u = 0; u < B; ++u) g[u] = S.geo[ent_pos(ev[u])];
+#pragma unroll
+            for (int u = 0; u < B; ++u) {
+                if (t0 + u >= n) continue;
+                const GeoRec& o = g[u];
+                const double dx = me.x - o.x, dy = me.y - o.y;
+                const double dist2 = dx * dx + dy * dy;
+                const double thr = outer ? rad : o.r + rtree;
+                // far beyond the radius: no square root needed
+                if (dist2 > thr * thr * (1.0 + 1e-12)) continue;
+                const double d = sqrt(dist2);
+                if (!(d <= thr)) continue;
+                // outer: pair (f, j) with f the outer floe i of the event;
+                // inner: pair (j, f) with f the inner floe j
+                const double deltan = outer ? me.r + o.r - d : o.r + me.r - d;
+                if (deltan > 0) qc = true;
+                if (outer && bond_step && deltan >= -P.bond_lim &&
+                    bond_keep(f - first, ent_id(ev[u]) - first, P.bond_seed, P.bond_thr24))
+                    qb = true;
+                if (shelter) {
+                    const T dn = T(deltan);
+                    const bool near = !(dn < T(0) && T((double)0.18f) * -dn >= lha) ||
+                                      !(dn < T(0) && T((double)0.18f) * -dn >= lhw);
+                    if (near) {
+                        const int slot = atomicAdd(&nnear, 1);
+                        if (slot < DENSE_NEAR) {
+                            DenseNear r;
+                            r.q = ent_pos(ev[u]); r.lane = (uint8_t)lane; r.outer = outer ? 1 : 0;
+                            r.pad_ = 0;
+                            nr[slot] = r;
+                        } else {
+                            // list full: this thread takes the terms itself
+                            const double nx = outer ? o.x - me.x : me.x - o.x;
+                            const double ny = outer ? o.y - me.y : me.y - o.y;
+                            const double cosa = nx / d, sina = ny / d;
+                            const ShelRec ms = S.shel[k];
+                            const ShelRec oh = S.shel[ent_pos(ev[u])];
+                            const T c0t = T(cosa), s0t = T(sina);
+                            const T c = outer ? -c0t : c0t, sn = outer ? -s0t : s0t;
+                            const T ux = outer ? T(ms.wx) : T(oh.wx), uy = outer ? T(ms.wy) : T(oh.wy);
+                            sha = shelter_min(sha, T(oh.hfa), T(ms.hfa), dn, c, sn, ux, uy);
+                            shw = shelter_min(shw, T(oh.hfw), T(ms.hfw), dn, c, sn, T(P.uw), T(P.vw));
+                        }
                     }
                 }
+                if (outer) {
+                    insert_outer(d, ev[u]);
+                } else if (ni < S.CI) {
+                    if (ni < bi) S.ik[(int64_t)ni * SS + k] = ev[u];
+                    else ring_[rslot(ni)] = ev[u];
+                    ++ni;
+                } else {
+                    over |= ERR_INNER_CAP;
+                }
             }
-            if (outer) {
-                insert_outer(d, ev[u]);
-            } else if (ni < S.CI) {
-                S.ik[(int64_t)ni * SS + k] = ev[u];
-                ++ni;
-            } else {
-                over |= ERR_INNER_CAP;
+            if (outer && t0 + B >= n) {
+                // the lane's last outer entries: its register part is final
+#pragma unroll
+                for (int w = 0; w < DENSE_WIN; ++w)
+                    if (w < nw) put_outer(no - nw + w, we[w]);
+                nw = 0;
             }
         }
-    }
-    // the register part to its slots
-#pragma unroll
-    for (int w = 0; w < DENSE_WIN; ++w)
-        if (w < nw) oj[(int64_t)(no - nw + w) * SS + k] = we[w];
+    };
+    scan(std::true_type{});
+    // every staged outer row out (each lane's register part went to its rows
+    // with its last batch): the ring is free for the inner pass
+    while (__ballot(!od && no > bo)) out_outer();
+    scan(std::false_type{});
+    while (__ballot(ni > bi)) out_inner();
     // sheltering terms of the listed near pairs, across the lanes (every lane
     // takes part in each round; the listing thread's floe comes by shuffle)
+    ShelRec msh;
+    msh.hfa = msh.hfw = msh.wx = msh.wy = 0.0;
+    if (act) msh = S.shel[k];
     wave_lds_sync();
     const int nn = min(nnear, DENSE_NEAR);
     for (int i0 = 0; i0 < nn; i0 += 64) {
         const int i = i0 + lane;
         DenseNear r;
-        r.d = 1.0; r.q = 0; r.lane = 0; r.outer = 0; r.pad0_ = r.pad1_ = 0;
+        r.q = 0; r.lane = 0; r.outer = 0; r.pad_ = 0;
         if (i < nn) r = nr[i];
         const int ow = r.lane;
         const double ox = __shfl(me.x, ow), oy = __shfl(me.y, ow), orr = __shfl(me.r, ow);
@@ -1532,7 +1612,8 @@ __global__ __launch_bounds__(64) void k_cand_dense(DS S) {
             const GeoRec o = S.geo[r.q];
             const ShelRec oh = S.shel[r.q];
             const bool outer = r.outer != 0;
-            const double d = r.d;
+            const double ddx = ox - o.x, ddy = oy - o.y;
+            const double d = sqrt(ddx * ddx + ddy * ddy);
             // outer: f takes hsf*(j,f) with -cosa, -sina of f -> j and its own
             // wind; inner: cosa/sina from j to f and j's wind
             const double deltan = outer ? orr + o.r - d : o.r + orr - d;
