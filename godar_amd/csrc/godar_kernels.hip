@@ -3007,9 +3007,12 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_stress(DS S) {
 //    (stepper.f90:295-343, integration.f90:1-38)
 // ---------------------------------------------------------------------
 // (REC: forces are recorded; the other instance carries no code for the
-// recorded columns and stress rows)
+// recorded columns and stress rows.  The mixed-precision instance without
+// them fits 5 wavefronts per SIMD in 96 VGPRs, one value kept in scratch
+// from the first loads to the writes: its chains of dependent loads, the
+// gridded wind's among them, gain more from the extra wavefront)
 template <bool MIXED, bool REC>
-__global__ __launch_bounds__(FLOE_BLOCK, 4) void k_final(DS S) {
+__global__ __launch_bounds__(FLOE_BLOCK, (MIXED && !REC) ? 5 : 4) void k_final(DS S) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= S.N) return;
     // the position's rows are requested at once, before anything is tested
