@@ -1188,7 +1188,7 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_cand(DS S) {
 constexpr int DENSE_OVER = -1;   // ocnt of a dense floe left to the thread search
 constexpr int DENSE_WIN = 4;     // outer candidates in registers
 constexpr int DENSE_NEAR = 512;  // near pairs listed per wavefront
-constexpr int DENSE_BATCH = 6;   // superset entries a thread loads at once (mixed precision)
+constexpr int DENSE_BATCH = 5;   // superset entries a thread loads at once (mixed precision)
 constexpr int DENSE_RING = 16;   // list rows a thread stages in LDS
 static_assert((DENSE_RING & (DENSE_RING - 1)) == 0 && DENSE_RING >= DENSE_BATCH + DENSE_WIN,
               "a ring of whole rows with room for a batch past the register part");
@@ -1340,9 +1340,9 @@ __device__ __forceinline__ double ord_val(uint64_t k) {
     return __longlong_as_double((long long)((k >> 63) ? (k & ~(1ull << 63)) : ~k));
 }
 
-// (3 wavefronts per SIMD: the registers of a batch of 6 superset entries in
-// mixed precision, of 5 in FP64, and the LDS of the ring and the near-pair
-// list fit them without spilling)
+// (3 wavefronts per SIMD: the registers of a batch of partner rows and of
+// two sets of superset entries, 5 in mixed precision and 4 in FP64, and the
+// LDS of the ring and the near-pair list fit them without spilling)
 template <bool MIXED>
 __global__ __launch_bounds__(64, 3) void k_cand_dense(DS S) {
     using T = typename Prec<MIXED>::T;
@@ -1500,7 +1500,17 @@ __global__ __launch_bounds__(64, 3) void k_cand_dense(DS S) {
         int nmax = n;
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) nmax = max(nmax, __shfl_xor(nmax, off));
-        for (int t0 = 0; t0 < nmax; t0 += B) {
+        // a batch's superset entries are requested with the batch before, in
+        // the other of two register sets (a copy from one set to the other
+        // would wait for the loads); the last entry stands in past the end
+        int64_t ea[B], eb[B];
+#pragma unroll
+        for (int u = 0; u < B; ++u) ea[u] = eb[u] = 0;
+        if (n > 0) {
+#pragma unroll
+            for (int u = 0; u < B; ++u) ea[u] = se[vb + min(u, n - 1)];
+        }
+        auto batch = [&](const int t0, const int64_t (&ev)[B], int64_t (&en)[B]) {
             const bool run = t0 < n;
             // rows out: each row that every lane still scanning has passed,
             // and those that make room for this batch (at most B rows, the
@@ -1515,13 +1525,14 @@ __global__ __launch_bounds__(64, 3) void k_cand_dense(DS S) {
                     break;
                 if (outer) out_outer(); else out_inner();
             }
-            if (!run) continue;
-            int64_t ev[B];
+            if (!run) return;
             GeoRec g[B];
 #pragma unroll
-            for (int u = 0; u < B; ++u) ev[u] = se[vb + min(t0 + u, n - 1)];   // the last entry stands in past the end
-#pragma unroll
             for (int u = 0; u < B; ++u) g[u] = S.geo[ent_pos(ev[u])];
+            if (t0 + B < n) {
+#pragma unroll
+                for (int u = 0; u < B; ++u) en[u] = se[vb + min(t0 + B + u, n - 1)];
+            }
 #pragma unroll
             for (int u = 0; u < B; ++u) {
                 if (t0 + u >= n) continue;
@@ -1583,6 +1594,10 @@ __global__ __launch_bounds__(64, 3) void k_cand_dense(DS S) {
                     if (w < nw) put_outer(no - nw + w, we[w]);
                 nw = 0;
             }
+        };
+        for (int t0 = 0; t0 < nmax; t0 += 2 * B) {
+            batch(t0, ea, eb);
+            if (t0 + B < nmax) batch(t0 + B, eb, ea);
         }
     };
     scan(std::true_type{});
