@@ -169,6 +169,7 @@ struct godar_ctx {
     int full_hold = 0;                  // full steps still to launch after a missed build
     int64_t pend_launched = 0;          // steps launched since the last sync
     int device = 0;                     // the GPU the context lives on
+    int ncu = 256;                      // its compute units
     int64_t* totals_scratch = nullptr;  // device address of that GPU's `scratch`
 };
 
@@ -503,7 +504,9 @@ int launch_step(godar_ctx* c, bool lean) {
     for (int k = 0; k < c->Mcap; ++k)
         if (c->members[k].active && c->dparams[k].dense) { any_dense = true; dense_n += c->members[k].n; }
     // The dense search runs a thread per dense floe (blocks of one
-    // wavefront, each with its list of near pairs in LDS); the superset
+    // wavefront, or of two that split the supersets when the dense floes
+    // fit on the GPU at once; each block with its list of near pairs in
+    // LDS); the superset
     // builds keep 1,024 wavefronts per XCD: they return at once on most
     // steps, and dispatching many more would cost more than they save.
     const dim3 dense_grid(XCD_COUNT * (int)std::max<int64_t>(1, (dense_n + XCD_COUNT * 64 - 1) / (XCD_COUNT * 64)));
@@ -528,8 +531,13 @@ int launch_step(godar_ctx* c, bool lean) {
                            dim3(B[3]), (B[3] / 64) * CAND_LDS_PER_WAVE, s, d);
         if (any_dense) {
             // a thread per dense floe, a contiguous share of the list per XCD
-            hipLaunchKernelGGL(mixed ? k_cand_dense<true> : k_cand_dense<false>,
-                               dense_grid, dim3(64), 0, s, d);
+            // two wavefronts per 64 floes (one per half of the supersets)
+            // when they all fit on the GPU at once: a shorter chain per
+            // wavefront; with more floes, one wavefront takes both halves
+            const bool two = dense_n <= (int64_t)c->ncu * 4 * DENSE_WAVES_PER_SIMD * 32;
+            auto kd = mixed ? (two ? k_cand_dense<true, true> : k_cand_dense<true, false>)
+                            : (two ? k_cand_dense<false, true> : k_cand_dense<false, false>);
+            hipLaunchKernelGGL(kd, dense_grid, dim3(two ? 128 : 64), 0, s, d);
             hipLaunchKernelGGL(mixed ? k_cand_dense_over<true> : k_cand_dense_over<false>,
                                grid_for(N, FLOE_BLOCK, true), dim3(FLOE_BLOCK), 0, s, d);
         }
@@ -733,6 +741,7 @@ int godar_create(int64_t max_floes, int32_t max_members, int32_t precision, goda
     } while (0)
     if (e != hipSuccess) { godar_destroy(c); return GODAR_ERR_HIP; }
     if (hipGetDevice(&c->device) != hipSuccess ||
+        hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess ||
         totals_scratch_of(c->device, &c->totals_scratch) != hipSuccess) {
         godar_destroy(c);
         return GODAR_ERR_HIP;

@@ -1190,6 +1190,7 @@ constexpr int DENSE_WIN = 4;     // outer candidates in registers
 constexpr int DENSE_NEAR = 512;  // near pairs listed per wavefront
 constexpr int DENSE_BATCH = 5;   // superset entries a thread loads at once (mixed precision)
 constexpr int DENSE_RING = 16;   // list rows a thread stages in LDS
+constexpr int DENSE_WAVES_PER_SIMD = 3;   // the dense search's occupancy bound
 static_assert((DENSE_RING & (DENSE_RING - 1)) == 0 && DENSE_RING >= DENSE_BATCH + DENSE_WIN,
               "a ring of whole rows with room for a batch past the register part");
 
@@ -1340,19 +1341,39 @@ __device__ __forceinline__ double ord_val(uint64_t k) {
     return __longlong_as_double((long long)((k >> 63) ? (k & ~(1ull << 63)) : ~k));
 }
 
-// (3 wavefronts per SIMD: the registers of a batch of partner rows and of
-// two sets of superset entries, 5 in mixed precision and 4 in FP64, and the
-// LDS of the ring and the near-pair list fit them without spilling)
-template <bool MIXED>
-__global__ __launch_bounds__(64, 3) void k_cand_dense(DS S) {
+// W2: two wavefronts over the same 64 floes, one per half of the
+// supersets (for dense queues that fit on the GPU at once); else one
+// wavefront takes both halves, one after the other.  (DENSE_WAVES_PER_SIMD
+// wavefronts per SIMD: the registers of a batch of partner rows and of two
+// sets of superset entries, 5 in mixed precision and 4 in FP64, and the LDS
+// of the rings and the near-pair list fit them without spilling.)
+template <bool MIXED, bool W2>
+__global__ __launch_bounds__(W2 ? 128 : 64, DENSE_WAVES_PER_SIMD) void k_cand_dense(DS S) {
     using T = typename Prec<MIXED>::T;
     __shared__ DenseNear nr[DENSE_NEAR];
     __shared__ uint64_t mina[64], minw[64];
-    __shared__ int64_t ring_[DENSE_RING * 64];   // staged rows: row r of lane l at (r % DENSE_RING) * 64 + l
-    __shared__ int nnear;
-    if (*S.err) return;
-    const int lane = threadIdx.x;
-    const int k = queue_entry(S, 3);
+    // staged rows of each wavefront: row r of lane l at (r % DENSE_RING) * 64 + l
+    __shared__ int64_t ring2_[W2 ? 2 : 1][DENSE_RING * 64];
+    __shared__ int xni[W2 ? 64 : 1];    // the inner wavefront's counts, flags and errors
+    __shared__ uint32_t xfl[W2 ? 64 : 1], xover[W2 ? 64 : 1];
+    __shared__ int nnear, err0;
+    // (the error word is read once for the block: a block that sets it may
+    // finish between the two wavefronts' starts, and both must take the
+    // barriers below or neither)
+    if (threadIdx.x == 0) err0 = *S.err;
+    __syncthreads();
+    if (err0) return;
+    // (W2: wavefront 0 the outer lists, 1 the inner ones)
+    const int lane = threadIdx.x & 63, wv = W2 ? (int)(threadIdx.x >> 6) : 0;
+    int64_t* const ring_ = ring2_[wv];
+    int k = -1;
+    {
+        const QueueView qv = queue_view(S, 3);
+        const int per = (qv.total + XCD_COUNT - 1) / XCD_COUNT;
+        const int local = (int)(blockIdx.x / XCD_COUNT) * 64 + lane;
+        const int t = (int)(blockIdx.x % XCD_COUNT) * per + local;
+        if (local < per && t < qv.total) k = queue_at(S, 3, qv, t);
+    }
     GeoRec me;
     me.x = me.y = me.r = 0.0;
     me.id = 0;
@@ -1369,12 +1390,14 @@ __global__ __launch_bounds__(64, 3) void k_cand_dense(DS S) {
     }
     bool act = k >= 0 && me.member >= 0;
     if (act && nout < 0) {
-        S.ocnt[k] = DENSE_OVER;   // superset in scan order: the thread search
+        if (wv == 0) S.ocnt[k] = DENSE_OVER;   // superset in scan order: the thread search
         act = false;
     }
     const uint64_t one_key = ord_key(1.0);
-    mina[lane] = minw[lane] = one_key;
-    if (lane == 0) nnear = 0;
+    if (wv == 0) {
+        mina[lane] = minw[lane] = one_key;
+        if (lane == 0) nnear = 0;
+    }
     const int m = act ? me.member : 0;
     const DevParams& P = S.params[m];
     // the parameters every entry of the scan reads, held in registers (the
@@ -1401,7 +1424,7 @@ __global__ __launch_bounds__(64, 3) void k_cand_dense(DS S) {
     }
     if (!act) L = nout = 0;
     T sha = T(1), shw = T(1);
-    wave_lds_sync();
+    __syncthreads();
     const int64_t SS = S.S;
     int64_t* oj = S.oj;
     const int64_t* se = S.sup + soff;
@@ -1600,20 +1623,33 @@ __global__ __launch_bounds__(64, 3) void k_cand_dense(DS S) {
             if (t0 + B < nmax) batch(t0 + B, eb, ea);
         }
     };
-    scan(std::true_type{});
-    // every staged outer row out (each lane's register part went to its rows
-    // with its last batch): the ring is free for the inner pass
-    while (__ballot(!od && no > bo)) out_outer();
-    scan(std::false_type{});
-    while (__ballot(ni > bi)) out_inner();
+    if (wv == 0) {
+        scan(std::true_type{});
+        // every staged outer row out (each lane's register part went to its
+        // rows with its last batch): the ring is free for the inner half
+        while (__ballot(!od && no > bo)) out_outer();
+    }
+    if (!W2 || wv == 1) {
+        scan(std::false_type{});
+        while (__ballot(ni > bi)) out_inner();
+        if (W2) {
+            xni[lane] = ni;
+            xfl[lane] = qc ? 1u : 0u;
+            xover[lane] = over;
+        }
+    }
+    // minima a thread took itself (near-pair list full) join the LDS ones
+    if (sha < T(1)) atomicMin((unsigned long long*)&mina[lane], (unsigned long long)ord_key((double)sha));
+    if (shw < T(1)) atomicMin((unsigned long long*)&minw[lane], (unsigned long long)ord_key((double)shw));
     // sheltering terms of the listed near pairs, across the lanes (every lane
     // takes part in each round; the listing thread's floe comes by shuffle)
     ShelRec msh;
     msh.hfa = msh.hfw = msh.wx = msh.wy = 0.0;
     if (act) msh = S.shel[k];
-    wave_lds_sync();
+    __syncthreads();
+    // (W2: the two wavefronts take alternate rounds of 64 pairs)
     const int nn = min(nnear, DENSE_NEAR);
-    for (int i0 = 0; i0 < nn; i0 += 64) {
+    for (int i0 = wv * 64; i0 < nn; i0 += W2 ? 128 : 64) {
         const int i = i0 + lane;
         DenseNear r;
         r.q = 0; r.lane = 0; r.outer = 0; r.pad_ = 0;
@@ -1644,10 +1680,15 @@ __global__ __launch_bounds__(64, 3) void k_cand_dense(DS S) {
             if (tw < T(1)) atomicMin((unsigned long long*)&minw[ow], (unsigned long long)ord_key((double)tw));
         }
     }
-    wave_lds_sync();
-    sha = dmin(sha, T(ord_val(mina[lane])));
-    shw = dmin(shw, T(ord_val(minw[lane])));
-    if (!act) return;
+    __syncthreads();
+    if (wv != 0 || !act) return;
+    sha = T(ord_val(mina[lane]));
+    shw = T(ord_val(minw[lane]));
+    if (W2) {
+        ni = xni[lane];
+        qc = qc || (xfl[lane] & 1u);
+        over |= xover[lane];
+    }
     if (over) set_err(S, over);
     const bool qw = me.y - me.r < 0 || me.y + me.r > P.ny || me.x - me.r < 0 || me.x + me.r > P.nx;
     cand_finish<MIXED>(S, k, fs, P, no, ni, double(sha), double(shw), qc, qb, 0, 1);
@@ -3206,8 +3247,10 @@ template __global__ void k_sup<false>(DS);
 template __global__ void k_sup<true>(DS);
 template __global__ void k_sup_dense<false>(DS);
 template __global__ void k_sup_dense<true>(DS);
-template __global__ void k_cand_dense<false>(DS);
-template __global__ void k_cand_dense<true>(DS);
+template __global__ void k_cand_dense<false, false>(DS);
+template __global__ void k_cand_dense<false, true>(DS);
+template __global__ void k_cand_dense<true, false>(DS);
+template __global__ void k_cand_dense<true, true>(DS);
 template __global__ void k_cand_dense_over<false>(DS);
 template __global__ void k_cand_dense_over<true>(DS);
 template __global__ void k_cand<false>(DS);
