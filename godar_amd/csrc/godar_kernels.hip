@@ -1482,8 +1482,17 @@ __global__ __launch_bounds__(W2 ? 128 : 64, DENSE_WAVES_PER_SIMD) void k_cand_de
         if (nw == DENSE_WIN) {
             // the smallest register entry is final: to its slot
             put_outer(no - DENSE_WIN, we[0]);
+            // (most entries come above the whole register part: they take
+            // its top without the insertion below)
+            const bool top = !less(xd, xe, wd[DENSE_WIN - 1], we[DENSE_WIN - 1]);
 #pragma unroll
             for (int w = 0; w < DENSE_WIN - 1; ++w) { wd[w] = wd[w + 1]; we[w] = we[w + 1]; }
+            if (top) {
+                wd[DENSE_WIN - 1] = xd;
+                we[DENSE_WIN - 1] = xe;
+                ++no;
+                return;
+            }
             wd[DENSE_WIN - 1] = __longlong_as_double(0x7ff0000000000000ll);
             we[DENSE_WIN - 1] = INT64_MAX;
             --nw;
