@@ -1181,7 +1181,8 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_cand(DS S) {
 // outer rows out and keeps its outer list in HBM from then on.
 // Sheltering terms are needed only for pairs close enough that
 // exp(-0.18 |dn/cos| / hf) still counts (shelter_min); the threads list those
-// pairs in LDS and the wavefront evaluates them 64 at a time, each floe
+// pairs in LDS (slots taken by ballot, no atomic) and the wavefront evaluates
+// them 64 at a time, each floe
 // taking the minimum of its own (an LDS atomic: a minimum does not depend on
 // the order of its terms).  A superset left in scan order (longer than
 // SUP_ORDER_CAP) goes to the thread search (k_cand_dense_over).
@@ -1356,7 +1357,7 @@ __global__ __launch_bounds__(W2 ? 128 : 64, DENSE_WAVES_PER_SIMD) void k_cand_de
     __shared__ int64_t ring2_[W2 ? 2 : 1][DENSE_RING * 64];
     __shared__ int xni[W2 ? 64 : 1];    // the inner wavefront's counts, flags and errors
     __shared__ uint32_t xfl[W2 ? 64 : 1], xover[W2 ? 64 : 1];
-    __shared__ int nnear, err0;
+    __shared__ int ncnt_[2], err0;      // near pairs listed by each wavefront
     // (the error word is read once for the block: a block that sets it may
     // finish between the two wavefronts' starts, and both must take the
     // barriers below or neither)
@@ -1396,7 +1397,6 @@ __global__ __launch_bounds__(W2 ? 128 : 64, DENSE_WAVES_PER_SIMD) void k_cand_de
     const uint64_t one_key = ord_key(1.0);
     if (wv == 0) {
         mina[lane] = minw[lane] = one_key;
-        if (lane == 0) nnear = 0;
     }
     const int m = act ? me.member : 0;
     const DevParams& P = S.params[m];
@@ -1442,6 +1442,11 @@ __global__ __launch_bounds__(W2 ? 128 : 64, DENSE_WAVES_PER_SIMD) void k_cand_de
     int bo = 0, bi = 0;
     bool od = false;
     uint32_t over = 0;
+    // near pairs: this wavefront's part of the list (W2: half each) and the
+    // count of pairs listed so far
+    const int ncap = W2 ? DENSE_NEAR / 2 : DENSE_NEAR, nofs = wv * (DENSE_NEAR / 2);
+    int nbase = 0;
+    const uint64_t lt = lane ? ((~0ull) >> (64 - lane)) : 0ull;
     auto rslot = [&](int r) { return (r & (DENSE_RING - 1)) * 64 + lane; };
     auto less = [](double ad, int64_t ae, double bd, int64_t be) {
         return ad < bd || (ad == bd && ent_id(ae) < ent_id(be));
@@ -1557,43 +1562,53 @@ __global__ __launch_bounds__(W2 ? 128 : 64, DENSE_WAVES_PER_SIMD) void k_cand_de
                     break;
                 if (outer) out_outer(); else out_inner();
             }
-            if (!run) return;
+            // (every lane takes part in the batch, so that the near-pair
+            // count below stays the same in all of them)
             GeoRec g[B];
+            if (run) {
 #pragma unroll
-            for (int u = 0; u < B; ++u) g[u] = S.geo[ent_pos(ev[u])];
-            if (t0 + B < n) {
+                for (int u = 0; u < B; ++u) g[u] = S.geo[ent_pos(ev[u])];
+                if (t0 + B < n) {
 #pragma unroll
-                for (int u = 0; u < B; ++u) en[u] = se[vb + min(t0 + B + u, n - 1)];
+                    for (int u = 0; u < B; ++u) en[u] = se[vb + min(t0 + B + u, n - 1)];
+                }
             }
 #pragma unroll
             for (int u = 0; u < B; ++u) {
-                if (t0 + u >= n) continue;
                 const GeoRec& o = g[u];
                 const double dx = me.x - o.x, dy = me.y - o.y;
                 const double dist2 = dx * dx + dy * dy;
                 const double thr = outer ? rad : o.r + rtree;
                 // far beyond the radius: no square root needed
-                if (dist2 > thr * thr * (1.0 + 1e-12)) continue;
-                const double d = sqrt(dist2);
-                if (!(d <= thr)) continue;
+                bool inr = run && t0 + u < n && !(dist2 > thr * thr * (1.0 + 1e-12));
+                double d = 0.0;
+                if (inr) d = sqrt(dist2);
+                inr = inr && d <= thr;
                 // outer: pair (f, j) with f the outer floe i of the event;
                 // inner: pair (j, f) with f the inner floe j
                 const double deltan = outer ? me.r + o.r - d : o.r + me.r - d;
-                if (deltan > 0) qc = true;
-                if (outer && bond_step && deltan >= -P.bond_lim &&
-                    bond_keep(f - first, ent_id(ev[u]) - first, P.bond_seed, P.bond_thr24))
-                    qb = true;
-                if (shelter) {
-                    const T dn = T(deltan);
-                    const bool near = !(dn < T(0) && T((double)0.18f) * -dn >= lha) ||
-                                      !(dn < T(0) && T((double)0.18f) * -dn >= lhw);
-                    if (near) {
-                        const int slot = atomicAdd(&nnear, 1);
-                        if (slot < DENSE_NEAR) {
+                const T dn = T(deltan);
+                bool near = false;
+                if (inr) {
+                    if (deltan > 0) qc = true;
+                    if (outer && bond_step && deltan >= -P.bond_lim &&
+                        bond_keep(f - first, ent_id(ev[u]) - first, P.bond_seed, P.bond_thr24))
+                        qb = true;
+                    near = shelter && (!(dn < T(0) && T((double)0.18f) * -dn >= lha) ||
+                                       !(dn < T(0) && T((double)0.18f) * -dn >= lhw));
+                }
+                // near pairs take consecutive slots of the wavefront's part of
+                // the list, counted by ballot (the same count in every lane)
+                const uint64_t mn = __ballot(near);
+                const int slot = nbase + (int)__popcll(mn & lt);
+                nbase += (int)__popcll(mn);
+                if (near) {
+                    {
+                        if (slot < ncap) {
                             DenseNear r;
                             r.q = ent_pos(ev[u]); r.lane = (uint8_t)lane; r.outer = outer ? 1 : 0;
                             r.pad_ = 0;
-                            nr[slot] = r;
+                            nr[nofs + slot] = r;
                         } else {
                             // list full: this thread takes the terms itself
                             const double nx = outer ? o.x - me.x : me.x - o.x;
@@ -1609,6 +1624,7 @@ __global__ __launch_bounds__(W2 ? 128 : 64, DENSE_WAVES_PER_SIMD) void k_cand_de
                         }
                     }
                 }
+                if (!inr) continue;
                 if (outer) {
                     insert_outer(d, ev[u]);
                 } else if (ni < S.CI) {
@@ -1619,7 +1635,7 @@ __global__ __launch_bounds__(W2 ? 128 : 64, DENSE_WAVES_PER_SIMD) void k_cand_de
                     over |= ERR_INNER_CAP;
                 }
             }
-            if (outer && t0 + B >= n) {
+            if (outer && run && t0 + B >= n) {
                 // the lane's last outer entries: its register part is final
 #pragma unroll
                 for (int w = 0; w < DENSE_WIN; ++w)
@@ -1647,6 +1663,7 @@ __global__ __launch_bounds__(W2 ? 128 : 64, DENSE_WAVES_PER_SIMD) void k_cand_de
             xover[lane] = over;
         }
     }
+    if (lane == 0) ncnt_[wv] = min(nbase, ncap);
     // minima a thread took itself (near-pair list full) join the LDS ones
     if (sha < T(1)) atomicMin((unsigned long long*)&mina[lane], (unsigned long long)ord_key((double)sha));
     if (shw < T(1)) atomicMin((unsigned long long*)&minw[lane], (unsigned long long)ord_key((double)shw));
@@ -1657,12 +1674,12 @@ __global__ __launch_bounds__(W2 ? 128 : 64, DENSE_WAVES_PER_SIMD) void k_cand_de
     if (act) msh = S.shel[k];
     __syncthreads();
     // (W2: the two wavefronts take alternate rounds of 64 pairs)
-    const int nn = min(nnear, DENSE_NEAR);
+    const int nn0 = ncnt_[0], nn = nn0 + (W2 ? ncnt_[1] : 0);
     for (int i0 = wv * 64; i0 < nn; i0 += W2 ? 128 : 64) {
         const int i = i0 + lane;
         DenseNear r;
         r.q = 0; r.lane = 0; r.outer = 0; r.pad_ = 0;
-        if (i < nn) r = nr[i];
+        if (i < nn) r = nr[i < nn0 ? i : DENSE_NEAR / 2 + (i - nn0)];
         const int ow = r.lane;
         const double ox = __shfl(me.x, ow), oy = __shfl(me.y, ow), orr = __shfl(me.r, ow);
         const double ohfa = __shfl(msh.hfa, ow), ohfw = __shfl(msh.hfw, ow);
