@@ -1167,25 +1167,27 @@ __global__ __launch_bounds__(FLOE_BLOCK) void k_cand(DS S) {
 // ones by id).  Inner candidates therefore come in list order and are
 // appended; outer ones come in nearly the order of their current distance
 // (floes move little between builds), so each is inserted among the last
-// DENSE_WIN entries, which stay in registers, and only the rare entry that
-// belongs below them is inserted into the part of the list in HBM.
+// DENSE_WIN entries, which stay in registers (most simply take their top),
+// and only the rare entry that belongs below them is inserted into the part
+// of the list in HBM.
 // The lists are slot-major (slot s of floe k at s * S + k), so a thread that
 // stored each entry as it found it would touch a line per entry, 64 lines per
 // store of a wavefront.  Instead the scan takes the outer part of the
-// supersets in one pass and the inner part in a second, so that the lanes'
-// lists grow nearly in step, and each thread stages its rows in a ring of its
-// own in LDS; the wavefront writes a row out for all its floes at once when
-// every thread still scanning has passed it, or when a thread needs the room.
+// supersets in one pass and the inner part in another (one after the other,
+// or in two wavefronts over the same floes), so that the lanes' lists grow
+// nearly in step, and each thread stages its rows in a ring of its own in
+// LDS; the wavefront writes a row out for all its floes at once when every
+// thread still scanning has passed it, or when a thread needs the room.
 // A row a thread reaches after its wavefront wrote it out goes straight to
 // HBM; a thread that inserts below its register part first writes its staged
 // outer rows out and keeps its outer list in HBM from then on.
 // Sheltering terms are needed only for pairs close enough that
 // exp(-0.18 |dn/cos| / hf) still counts (shelter_min); the threads list those
 // pairs in LDS (slots taken by ballot, no atomic) and the wavefront evaluates
-// them 64 at a time, each floe
-// taking the minimum of its own (an LDS atomic: a minimum does not depend on
-// the order of its terms).  A superset left in scan order (longer than
-// SUP_ORDER_CAP) goes to the thread search (k_cand_dense_over).
+// them 64 at a time, each floe taking the minimum of its own (an LDS atomic:
+// a minimum does not depend on the order of its terms).  A superset left in
+// scan order (longer than SUP_ORDER_CAP) goes to the thread search
+// (k_cand_dense_over).
 constexpr int DENSE_OVER = -1;   // ocnt of a dense floe left to the thread search
 constexpr int DENSE_WIN = 4;     // outer candidates in registers
 constexpr int DENSE_NEAR = 512;  // near pairs listed per wavefront
