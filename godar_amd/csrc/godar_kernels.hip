@@ -2052,9 +2052,11 @@ constexpr int BOND_TCAP = 256;
 // records are listed in LDS and evaluated one per lane, so that an owner with
 // several bonds does not hold its lane through all of them; owners whose
 // bonds form this step (step 1) append new records in candidate order and
-// walk their own lists.
+// walk their own lists.  (3 wavefronts per SIMD, without spills: at 4 the
+// kernel spilled 40 bytes, and the step, where it shares the GPU with the
+// contact and wall kernels, ran 1.3 % slower.)
 template <bool MIXED>
-__global__ __launch_bounds__(FLOE_BLOCK, 4) void k_bond(DS S) {
+__global__ __launch_bounds__(FLOE_BLOCK, 3) void k_bond(DS S) {
     __shared__ uint32_t recs[FLOE_BLOCK / 64][BOND_TCAP];
     if (*S.err) return;
     const QueueView qv = queue_view(S, 1);
