@@ -222,8 +222,9 @@ int godar_set_stage_block(godar_ctx *ctx, int32_t stage, int32_t threads);
 /* Members added afterwards whose neighbour search examines at least `floes`
  * floes per floe (3x3 cells at their grid's mean density; default 96) take
  * the dense search (supersets ordered by distance at each build, scanned by
- * a thread per floe) instead of the thread search (0: every member, a huge
- * value: none).  Results do not depend on it. */
+ * a thread per floe, or two, one per half of the superset, when all dense
+ * floes fit on the GPU at once) instead of the thread search (0: every
+ * member, a huge value: none).  Results do not depend on it. */
 int godar_set_dense_search(godar_ctx *ctx, int32_t floes);
 /* Ridging repair: the number of pending events one member evaluates at once
  * (1-64, default 64; 1: one at a time) and the pending events kept in on-chip
