@@ -1785,8 +1785,11 @@ struct ContactOut {
 };
 constexpr int CONTACT_LDS_PER_WAVE = 64 * (int)sizeof(ContactOut);
 
+// (the mixed-precision instance gets 163 VGPRs from the looser bound, still
+// 3 wavefronts per SIMD: 0.0955 -> 0.0915 ms beside the bond kernel; FP64
+// would drop to 2 and keeps the bound of 3)
 template <bool MIXED>
-__global__ __launch_bounds__(FLOE_BLOCK, 3) void k_contact(DS S) {
+__global__ __launch_bounds__(FLOE_BLOCK, MIXED ? 2 : 3) void k_contact(DS S) {
     extern __shared__ unsigned char contact_lds[];
     ContactOut* co = reinterpret_cast<ContactOut*>(contact_lds + (threadIdx.x >> 6) * CONTACT_LDS_PER_WAVE);
     if (*S.err) return;
